@@ -1,0 +1,263 @@
+"""MI355X (gfx950) implementation of the pichromatic raw -> demosaic -> BM3D ->
+colour -> film stage chain.
+
+The Python layer mirrors the reference's host interface for this path:
+
+* :class:`Image` follows ``pichromatic::pixel::Image`` and its stage methods
+  (pichromatic/src/pixel.rs:55-171), but its pixel buffers live in device
+  memory behind a handle of the C library;
+* :func:`parse_config` / :func:`run_pixel_pipeline` follow
+  ``config::parse_config`` (pichromatic-pipeline/src/config.rs:13-34) and
+  ``run_pixel_pipeline`` (pichromatic-pipeline/src/pipeline.rs:4-13).
+
+Every stage runs as hand-written HIP kernels in ``libimops_gpu.so``; a missing
+library raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import (CS_ACESCG, CS_LINEAR_SRGB, CS_NONE, CS_SRGB, CS_XYZD65, DEMOSAIC_AMAZE, DEMOSAIC_MARKESTEIJN,
+                   ImopsError, check, fptr, lib)
+
+__all__ = [
+    "Context", "Image", "PipelineConfig", "parse_config", "run_pixel_pipeline", "ImopsError", "synth_raw_u16",
+    "synth_scene_rgb", "CS_NONE", "CS_SRGB", "CS_LINEAR_SRGB", "CS_ACESCG", "CS_XYZD65", "DEMOSAIC_AMAZE",
+    "DEMOSAIC_MARKESTEIJN", "COLOR_SPACES",
+]
+
+# ColorSpaceTag names accepted by CST (modules.rs:107-112, serde_plain names)
+COLOR_SPACES = {"Srgb": CS_SRGB, "LinearSrgb": CS_LINEAR_SRGB, "AcesCg": CS_ACESCG, "XyzD65": CS_XYZD65}
+_ALGORITHMS = {"amaze": DEMOSAIC_AMAZE, "markesteijn": DEMOSAIC_MARKESTEIJN}
+
+
+def _cs(target) -> int:
+    if isinstance(target, str):
+        if target not in COLOR_SPACES:
+            raise ImopsError(_lib.E_INVALID, f"Color space not recognized: {target}")
+        return COLOR_SPACES[target]
+    return int(target)
+
+
+class Context:
+    """One device + one HIP stream (imops_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().imops_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return lib().imops_ctx_stream(self._h) or 0
+
+    def synchronize(self) -> None:
+        check(lib().imops_ctx_synchronize(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            lib().imops_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Image:
+    """Device-resident image (raw CFA plane and/or interleaved RGB f32 + metadata)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib().imops_image_create(ctx.handle, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            lib().imops_image_free(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- ingest / readback -------------------------------------------------
+    def set_raw_u16(self, raw, cfa: str, black: float, white: float, crop=None) -> "Image":
+        """u16 sensor frame (numpy host array, or a device tensor) -> normalised CFA plane."""
+        on_dev, ptr, (h, w) = _buffer(raw, np.uint16)
+        cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
+        check(lib().imops_image_set_raw_u16(self.ctx.handle, self._h, ptr, on_dev, w, h, cx, cy, cw, ch,
+                                            float(black), float(white), cfa.encode()))
+        return self
+
+    def set_raw_f32(self, raw, cfa: str, crop_origin=(0, 0)) -> "Image":
+        on_dev, ptr, (h, w) = _buffer(raw, np.float32)
+        check(lib().imops_image_set_raw_f32(self.ctx.handle, self._h, ptr, on_dev, w, h, cfa.encode(),
+                                            crop_origin[0], crop_origin[1]))
+        return self
+
+    def set_rgb(self, rgb, color_space=CS_NONE) -> "Image":
+        on_dev, ptr, shape = _buffer(rgb, np.float32)
+        if len(shape) != 3 or shape[2] != 3:
+            raise ImopsError(_lib.E_INVALID, "rgb must have shape (H, W, 3)")
+        check(lib().imops_image_set_rgb_f32(self.ctx.handle, self._h, ptr, on_dev, shape[1], shape[0],
+                                            _cs(color_space)))
+        return self
+
+    def info(self):
+        w, h, cs, has = C.c_size_t(), C.c_size_t(), C.c_int(), C.c_int()
+        check(lib().imops_image_get_info(self._h, C.byref(w), C.byref(h), C.byref(cs), C.byref(has)))
+        return w.value, h.value, cs.value, bool(has.value)
+
+    @property
+    def width(self) -> int:
+        return self.info()[0]
+
+    @property
+    def height(self) -> int:
+        return self.info()[1]
+
+    @property
+    def color_space(self) -> int:
+        return self.info()[2]
+
+    def rgb(self) -> np.ndarray:
+        w, h, _, _ = self.info()
+        out = np.empty((h, w, 3), np.float32)
+        check(lib().imops_image_get_rgb_f32(self.ctx.handle, self._h, out.ctypes.data_as(C.c_void_p), 0, out.size))
+        return out
+
+    def rgb_device_ptr(self) -> int:
+        p = C.c_void_p()
+        check(lib().imops_image_rgb_device_ptr(self._h, C.byref(p)))
+        return p.value or 0
+
+    # -- metadata ----------------------------------------------------------
+    def set_wb_coeffs(self, wb: Sequence[float]) -> "Image":
+        a = np.asarray(wb, np.float32)
+        if a.size != 4:
+            raise ImopsError(_lib.E_INVALID, "wb_coeffs has 4 entries")
+        check(lib().imops_image_set_wb_coeffs(self._h, fptr(a)))
+        return self
+
+    def set_calibration_matrix(self, m: Sequence[float]) -> "Image":
+        a = np.ascontiguousarray(m, np.float32).ravel()
+        check(lib().imops_image_set_calibration_matrix(self._h, fptr(a), a.size))
+        return self
+
+    def set_shutter_seconds(self, s: float) -> "Image":
+        check(lib().imops_image_set_shutter_seconds(self._h, float(s)))
+        return self
+
+    # -- stages (pixel.rs:55-171) -------------------------------------------
+    def demosaic(self, algorithm="amaze") -> "Image":
+        a = _ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        check(lib().imops_demosaic(self.ctx.handle, self._h, a))
+        return self
+
+    def cfa_coeffs(self, wb: Optional[Sequence[float]] = None) -> "Image":
+        if wb is None:
+            check(lib().imops_cfa_coeffs(self.ctx.handle, self._h, None))
+        else:
+            a = np.asarray(wb, np.float32)
+            check(lib().imops_cfa_coeffs(self.ctx.handle, self._h, fptr(a)))
+        return self
+
+    def exp(self, ev: float) -> "Image":
+        check(lib().imops_exp(self.ctx.handle, self._h, float(ev)))
+        return self
+
+    def gamma(self, gamma: float) -> "Image":
+        check(lib().imops_gamma(self.ctx.handle, self._h, float(gamma)))
+        return self
+
+    def cst(self, target) -> "Image":
+        check(lib().imops_cst(self.ctx.handle, self._h, _cs(target)))
+        return self
+
+    def camera_cst(self, target, calibration_matrix: Sequence[float]) -> "Image":
+        a = np.ascontiguousarray(calibration_matrix, np.float32).ravel()
+        check(lib().imops_camera_cst(self.ctx.handle, self._h, _cs(target), fptr(a), a.size))
+        return self
+
+
+class PipelineConfig:
+    """Parsed stage list (imops_pipeline)."""
+
+    def __init__(self, text: str):
+        h = C.c_void_p()
+        data = text.encode()
+        check(lib().imops_pipeline_parse(data, len(data), C.byref(h)))
+        self._h = h
+
+    @property
+    def pipeline_modules(self) -> list[str]:
+        n = lib().imops_pipeline_num_stages(self._h)
+        return [lib().imops_pipeline_stage_name(self._h, i).decode() for i in range(n)]
+
+    def __del__(self):
+        try:
+            if self._h:
+                lib().imops_pipeline_free(self._h)
+        except Exception:
+            pass
+
+
+def parse_config(text: str) -> PipelineConfig:
+    return PipelineConfig(text)
+
+
+def run_pixel_pipeline(image: Image, config: PipelineConfig) -> Image:
+    check(lib().imops_pipeline_run(image.ctx.handle, config._h, image.handle))
+    return image
+
+
+def synth_scene_rgb(width: int, height: int, frame: int = 0) -> np.ndarray:
+    out = np.empty((height, width, 3), np.float32)
+    check(lib().imops_synth_scene_rgb(fptr(out), width, height, frame))
+    return out
+
+
+def synth_raw_u16(width: int, height: int, cfa: str = "RGGB", frame: int = 0, full_well: float = 20000.0,
+                  black: float = 600.0, white: float = 16383.0, read_sigma: float = 2.0) -> np.ndarray:
+    out = np.empty((height, width), np.uint16)
+    check(lib().imops_synth_raw_u16(out.ctypes.data_as(C.c_void_p), width, height, cfa.encode(), frame,
+                                    full_well, black, white, read_sigma))
+    return out
+
+
+def _buffer(x, dtype):
+    """(on_device, pointer, shape) for a numpy array (host) or a torch CUDA tensor (device)."""
+    if isinstance(x, np.ndarray):
+        if x.dtype != dtype or not x.flags.c_contiguous:
+            raise ImopsError(_lib.E_INVALID, f"expected a C-contiguous {np.dtype(dtype).name} array")
+        return 0, C.c_void_p(x.ctypes.data), x.shape
+    try:  # torch tensor
+        import torch
+    except ImportError as e:  # pragma: no cover
+        raise ImopsError(_lib.E_INVALID, "unsupported buffer type") from e
+    if isinstance(x, torch.Tensor):
+        # u16 data may arrive as an int16 tensor (same bits; torch has few uint16 ops)
+        want = {np.uint16: (torch.uint16, torch.int16), np.float32: (torch.float32,)}[dtype]
+        if x.dtype not in want or not x.is_contiguous():
+            raise ImopsError(_lib.E_INVALID, f"expected a contiguous {want[0]} tensor")
+        return (1 if x.is_cuda else 0), C.c_void_p(x.data_ptr()), tuple(x.shape)
+    raise ImopsError(_lib.E_INVALID, f"unsupported buffer type {type(x)!r}")

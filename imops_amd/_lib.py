@@ -1,0 +1,103 @@
+"""ctypes binding of ``libimops_gpu.so`` (the C ABI declared in include/imops_gpu.h).
+
+The library is loaded from the package directory only; a missing library is an
+error (there is no CPU fallback for any stage).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libimops_gpu.so"
+
+OK = 0
+E_INVALID = -1
+E_HIP = -2
+E_STATE = -3
+E_UNSUPPORTED = -4
+
+CS_NONE = -1
+CS_SRGB = 0
+CS_LINEAR_SRGB = 1
+CS_ACESCG = 2
+CS_XYZD65 = 3
+
+DEMOSAIC_AMAZE = 0
+DEMOSAIC_MARKESTEIJN = 1
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_i = C.c_int
+_f = C.c_float
+_fp = C.POINTER(C.c_float)
+
+# name -> (restype, argtypes); every symbol the header declares.
+SIGNATURES = {
+    "imops_last_error": (C.c_char_p, []),
+    "imops_ctx_create": (_i, [_i, C.POINTER(_vp)]),
+    "imops_ctx_destroy": (_i, [_vp]),
+    "imops_ctx_synchronize": (_i, [_vp]),
+    "imops_ctx_stream": (_vp, [_vp]),
+    "imops_image_create": (_i, [_vp, C.POINTER(_vp)]),
+    "imops_image_free": (_i, [_vp]),
+    "imops_image_set_raw_u16": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _sz, _sz, _sz, _sz, _f, _f, C.c_char_p]),
+    "imops_image_set_raw_f32": (_i, [_vp, _vp, _vp, _i, _sz, _sz, C.c_char_p, _sz, _sz]),
+    "imops_image_set_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _i]),
+    "imops_image_get_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz]),
+    "imops_image_get_info": (_i, [_vp, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_i), C.POINTER(_i)]),
+    "imops_image_rgb_device_ptr": (_i, [_vp, C.POINTER(_vp)]),
+    "imops_image_set_wb_coeffs": (_i, [_vp, _fp]),
+    "imops_image_set_calibration_matrix": (_i, [_vp, _fp, _sz]),
+    "imops_image_set_shutter_seconds": (_i, [_vp, _f]),
+    "imops_demosaic": (_i, [_vp, _vp, _i]),
+    "imops_cfa_coeffs": (_i, [_vp, _vp, _fp]),
+    "imops_exp": (_i, [_vp, _vp, _f]),
+    "imops_gamma": (_i, [_vp, _vp, _f]),
+    "imops_cst": (_i, [_vp, _vp, _i]),
+    "imops_camera_cst": (_i, [_vp, _vp, _i, _fp, _sz]),
+    "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
+    "imops_pipeline_free": (_i, [_vp]),
+    "imops_pipeline_num_stages": (_sz, [_vp]),
+    "imops_pipeline_stage_name": (C.c_char_p, [_vp, _sz]),
+    "imops_pipeline_run": (_i, [_vp, _vp, _vp]),
+    "imops_synth_scene_rgb": (_i, [_fp, _sz, _sz, C.c_uint64]),
+    "imops_synth_raw_u16": (_i, [_vp, _sz, _sz, C.c_char_p, C.c_uint64, _f, _f, _f, _f]),
+}
+
+
+class ImopsError(RuntimeError):
+    """A failed library call: status code + the library's last error message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m imops_amd.build` (no CPU fallback exists)"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = lib().imops_last_error()
+        raise ImopsError(status, msg.decode() if msg else "")
+
+
+def fptr(arr) -> C.POINTER(C.c_float):
+    return arr.ctypes.data_as(_fp)

@@ -1,0 +1,900 @@
+// AMaZE Bayer demosaic for gfx950.
+//
+// Behaviour follows pichromatic/src/demosaic.rs:358-1379 exactly: the image
+// is processed in 128x128 tiles on the grid top,left = -16 + 96k with the
+// reference's border fill, and every tile runs the same sequence of passes.
+// Three passes of the reference update a plane in place while reading
+// neighbours that the same pass has already rewritten (hcd/vcd variance
+// selection, the hvwt diagonal smoothing and the pmwt diagonal smoothing), so
+// the result depends on the tile a pixel is computed in; the kernel keeps that
+// tile structure instead of computing the image as one plane.
+//
+// Mapping: one 512-thread workgroup per tile (persistent over tiles).  The
+// CFA tile, hvwt, pmwt and the two Nyquist masks live in LDS (144 KiB); the
+// other planes of the tile live in a per-workgroup global scratch slot that
+// stays L2/MALL resident while the tile is processed.  Recurrences run as
+// register-carried chains (hcd along rows, vcd along columns) or row by row
+// in a single wave (hvwt, pmwt) while the other waves do independent passes.
+//
+// All arithmetic is IEEE f32 in the reference's operation order: the file is
+// compiled with -ffp-contract=off and correctly rounded division.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "amaze.hpp"
+
+namespace imops {
+namespace {
+
+constexpr int TS = 128, TSH = 64;
+constexpr int V1 = 128, V2 = 256, V3 = 384;
+constexpr int P1 = 127, P2 = 254, P3 = 381;
+constexpr int M1 = 129, M2 = 258, M3 = 387;
+constexpr int NT = 512;  // threads per workgroup
+
+constexpr float eps = 1e-5f, epssq = 1e-10f, arthresh = 0.75f;
+constexpr float clip_pt = 1.0f, clip_pt8 = 1.0f * 0.8f;
+
+// Offsets (in floats) of the planes inside one scratch slot.
+enum : int {
+  S_RGBGREEN = 0,
+  S_DELHVSQSUM = 1 * 16384,
+  S_DIRWTS0 = 2 * 16384,
+  S_DIRWTS1 = 3 * 16384,
+  S_VCD = 4 * 16384,
+  S_HCD = 5 * 16384,
+  S_VCDALT = 6 * 16384,
+  S_HCDALT = 7 * 16384,
+  S_CDDIFFSQ = 8 * 16384,
+  S_DGINTV = 9 * 16384,
+  S_DGINTH = 10 * 16384,
+  S_HALF = 11 * 16384,
+  S_DGRB0 = S_HALF + 0 * 8192,
+  S_DGRB1 = S_HALF + 1 * 8192,
+  S_DELP = S_HALF + 2 * 8192,
+  S_DELM = S_HALF + 3 * 8192,
+  S_RBINT = S_HALF + 4 * 8192,
+  S_DGRB2H = S_HALF + 5 * 8192,
+  S_DGRB2V = S_HALF + 6 * 8192,
+  S_DGRBSQ1M = S_HALF + 7 * 8192,
+  S_DGRBSQ1P = S_HALF + 8 * 8192,
+  S_RBM = S_HALF + 9 * 8192,
+  S_RBP = S_HALF + 10 * 8192,
+  S_SLOT_FLOATS = S_HALF + 11 * 8192,
+};
+static_assert(S_SLOT_FLOATS == kAmazeSlotFloats, "slot layout");
+
+struct Tile {
+  int top, left, rr1, cc1, rrmin, ccmin, rrmax, ccmax;
+};
+
+__device__ __forceinline__ float sqr(float x) { return x * x; }
+__device__ __forceinline__ float lim(float x, float lo, float hi) { return fmaxf(lo, fminf(x, hi)); }
+__device__ __forceinline__ float ulim(float x, float y, float z) { return y < z ? lim(x, y, z) : lim(x, z, y); }
+__device__ __forceinline__ float clampnan(float x, float m, float mx) {
+  return isnan(x) ? (m + mx) * 0.5f : (x < m ? m : (x > mx ? mx : x));
+}
+
+struct Ctx {
+  const AmazeArgs& a;
+  Tile t;
+  float* s;          // scratch slot
+  float* cfa;        // LDS 128x128
+  float* hvwt;       // LDS 128x64
+  float* pmwt;       // LDS 128x64
+  uint8_t* nyq;      // LDS 128x64
+  uint8_t* nyq2;     // LDS 128x64
+  __device__ int color_at(int r, int c) const { return a.cfa[r & 1][c & 1]; }
+  __device__ int par(int rr) const { return color_at(t.top + rr, t.left + 2) & 1; }
+  __device__ float get_in(int r, int c) const {
+    r = r < 0 ? 0 : (r > a.height - 1 ? a.height - 1 : r);
+    c = c < 0 ? 0 : (c > a.width - 1 ? a.width - 1 : c);
+    return a.in[(size_t)r * a.width + c];
+  }
+  __device__ float* P(int off) const { return s + off; }
+};
+
+// Source row of tile row rr (border fill demosaic.rs:500-607); -1 = zero.
+__device__ __forceinline__ int src_row(const Ctx& c, int rr, bool* upper, bool* lower) {
+  const Tile& t = c.t;
+  *upper = *lower = false;
+  if (rr >= t.rr1) return INT32_MIN;
+  if (rr < t.rrmin) { *upper = true; return rr; }
+  if (rr >= t.rrmax) { *lower = true; return rr - t.rrmax; }
+  return rr + t.top;
+}
+
+__device__ void pass_fill(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const int H = c.a.height, W = c.a.width;
+  float* rgbgreen = c.P(S_RGBGREEN);
+  for (int idx = tid; idx < 16384; idx += NT) {
+    const int rr = idx >> 7, cc = idx & 127;
+    float v = 0.0f;
+    if (rr < t.rr1 && cc < t.cc1) {
+      const bool up = rr < t.rrmin, lo = rr >= t.rrmax;
+      const bool le = cc < t.ccmin, ri = cc >= t.ccmax;
+      const int br = lo ? rr - t.rrmax : rr;  // index inside the 16-wide border band
+      const int bc = ri ? cc - t.ccmax : cc;
+      int row, col;
+      if ((up || lo) && (le || ri)) {  // corners
+        row = up ? 32 - br : H - br - 2;
+        col = le ? 32 - bc : W - bc - 2;
+      } else {
+        row = up ? 32 - br + t.top : (lo ? H - br - 2 : rr + t.top);
+        col = le ? 32 - bc + t.left : (ri ? W - bc - 2 : cc + t.left);
+      }
+      v = c.get_in(row, col);
+    }
+    c.cfa[idx] = v;
+    rgbgreen[idx] = v;
+  }
+}
+
+// demosaic.rs:612-622
+__device__ void pass_gradients(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  float* dirwts0 = c.P(S_DIRWTS0);
+  float* dirwts1 = c.P(S_DIRWTS1);
+  float* delhvsqsum = c.P(S_DELHVSQSUM);
+  for (int i = tid; i < 16384; i += NT) {
+    const int rr = i >> 7, cc = i & 127;
+    float d0 = 0.0f, d1 = 0.0f, dq = 0.0f;
+    if (rr >= 2 && rr < t.rr1 - 2 && cc >= 2 && cc < t.cc1 - 2) {
+      float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
+      float delv = fabsf(cfa[i + V1] - cfa[i - V1]);
+      d0 = eps + fabsf(cfa[i + V2] - cfa[i]) + fabsf(cfa[i] - cfa[i - V2]) + delv;
+      d1 = eps + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
+      dq = sqr(delh) + sqr(delv);
+    }
+    dirwts0[i] = d0;
+    dirwts1[i] = d1;
+    delhvsqsum[i] = dq;
+  }
+}
+
+// demosaic.rs:626-698
+__device__ void pass_colour_diff(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const float* dirwts0 = c.P(S_DIRWTS0);
+  const float* dirwts1 = c.P(S_DIRWTS1);
+  float* vcd = c.P(S_VCD);
+  float* hcd = c.P(S_HCD);
+  float* vcdalt = c.P(S_VCDALT);
+  float* hcdalt = c.P(S_HCDALT);
+  float* dgintv = c.P(S_DGINTV);
+  float* dginth = c.P(S_DGINTH);
+  for (int i = tid; i < 16384; i += NT) {
+    const int rr = i >> 7, cc = i & 127;
+    if (!(rr >= 4 && rr < t.rr1 - 4 && cc >= 4 && cc < t.cc1 - 4)) {
+      vcd[i] = 0.0f; hcd[i] = 0.0f; vcdalt[i] = 0.0f; hcdalt[i] = 0.0f; dgintv[i] = 0.0f; dginth[i] = 0.0f;
+      continue;
+    }
+    const bool fcswitch = (((c.color_at(t.top + rr, t.left + 4) & 1) != 0) != (((cc - 4) & 1) != 0));
+    const float ci = cfa[i];
+    const float dw0m2 = dirwts0[i - V2], dw0 = dirwts0[i], dw0p2 = dirwts0[i + V2];
+    const float dw1m2 = dirwts1[i - 2], dw1 = dirwts1[i], dw1p2 = dirwts1[i + 2];
+    float cru = cfa[i - V1] * (dw0m2 + dw0) / fmaxf(dw0m2 * (eps + ci) + dw0 * (eps + cfa[i - V2]), 1e-10f);
+    float crd = cfa[i + V1] * (dw0p2 + dw0) / fmaxf(dw0p2 * (eps + ci) + dw0 * (eps + cfa[i + V2]), 1e-10f);
+    float crl = cfa[i - 1] * (dw1m2 + dw1) / fmaxf(dw1m2 * (eps + ci) + dw1 * (eps + cfa[i - 2]), 1e-10f);
+    float crr = cfa[i + 1] * (dw1p2 + dw1) / fmaxf(dw1p2 * (eps + ci) + dw1 * (eps + cfa[i + 2]), 1e-10f);
+    float guha = cfa[i - V1] + (ci - cfa[i - V2]) * 0.5f;
+    float gdha = cfa[i + V1] + (ci - cfa[i + V2]) * 0.5f;
+    float glha = cfa[i - 1] + (ci - cfa[i - 2]) * 0.5f;
+    float grha = cfa[i + 1] + (ci - cfa[i + 2]) * 0.5f;
+    float guar = fabsf(1.0f - cru) < arthresh ? ci * cru : guha;
+    float gdar = fabsf(1.0f - crd) < arthresh ? ci * crd : gdha;
+    float glar = fabsf(1.0f - crl) < arthresh ? ci * crl : glha;
+    float grar = fabsf(1.0f - crr) < arthresh ? ci * crr : grha;
+    float hwt = dirwts1[i - 1] / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
+    float vwt = dirwts0[i - V1] / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
+    float gintvha = vwt * gdha + (1.0f - vwt) * guha;
+    float ginthha = hwt * grha + (1.0f - hwt) * glha;
+    float vcd_v, hcd_v, vcdalt_v, hcdalt_v;
+    if (fcswitch) {
+      vcd_v = ci - (vwt * gdar + (1.0f - vwt) * guar);
+      hcd_v = ci - (hwt * grar + (1.0f - hwt) * glar);
+      vcdalt_v = ci - gintvha;
+      hcdalt_v = ci - ginthha;
+    } else {
+      vcd_v = (vwt * gdar + (1.0f - vwt) * guar) - ci;
+      hcd_v = (hwt * grar + (1.0f - hwt) * glar) - ci;
+      vcdalt_v = gintvha - ci;
+      hcdalt_v = ginthha - ci;
+    }
+    float gu = guar, gd = gdar, gl = glar, gr = grar;
+    if (ci > clip_pt8 || gintvha > clip_pt8 || ginthha > clip_pt8) {
+      vcd_v = vcdalt_v;
+      hcd_v = hcdalt_v;
+      gu = guha; gd = gdha; gl = glha; gr = grha;
+    }
+    vcd[i] = vcd_v;
+    hcd[i] = hcd_v;
+    vcdalt[i] = vcdalt_v;
+    hcdalt[i] = hcdalt_v;
+    dgintv[i] = fminf(sqr(guha - gdha), sqr(gu - gd));
+    dginth[i] = fminf(sqr(glha - grha), sqr(gl - gr));
+  }
+}
+
+// One step of the variance selection + clamp of demosaic.rs:700-786 for one
+// direction.  `prev` is the already-updated value two pixels back along the
+// chain, `cur`/`next` are the values written by pass_colour_diff.
+__device__ __forceinline__ float cd_update(float prev, float cur, float next, float aprev, float acur, float anext,
+                                           bool gsite, float ci, float nm, float np) {
+  float var = 3.0f * (sqr(prev) + sqr(cur) + sqr(next)) - sqr(prev + cur + next);
+  float avar = 3.0f * (sqr(aprev) + sqr(acur) + sqr(anext)) - sqr(aprev + acur + anext);
+  float v = cur;
+  if (avar < var) v = acur;
+  if (gsite) {
+    const float gint = -v + ci;
+    if (v > 0.0f) {
+      if (3.0f * v > (gint + ci)) {
+        v = -ulim(gint, nm, np) + ci;
+      } else {
+        float w = 1.0f - 3.0f * v / (eps + gint + ci);
+        v = w * v + (1.0f - w) * (-ulim(gint, nm, np) + ci);
+      }
+    }
+    if (gint > clip_pt) v = -ulim(gint, nm, np) + ci;
+  } else {
+    const float gint = v + ci;
+    if (v < 0.0f) {
+      if (3.0f * v < -(gint + ci)) {
+        v = ulim(gint, nm, np) - ci;
+      } else {
+        float w = 1.0f + 3.0f * v / (eps + gint + ci);
+        v = w * v + (1.0f - w) * (ulim(gint, nm, np) - ci);
+      }
+    }
+    if (gint > clip_pt) v = ulim(gint, nm, np) - ci;
+  }
+  return v;
+}
+
+// hcd chains along rows (threads 0..255) and vcd chains along columns
+// (threads 256..511); each thread walks both parities of its line.
+__device__ void pass_cd_chains(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const bool horiz = tid < 256;
+  const int line = (tid & 255) >> 1, parity = tid & 1;
+  float* cd = c.P(horiz ? S_HCD : S_VCD);
+  const float* alt = c.P(horiz ? S_HCDALT : S_VCDALT);
+  const int lo = 4, hi = horiz ? t.cc1 - 4 : t.rr1 - 4;          // along the chain
+  const int lline = 4 + line, hline = horiz ? t.rr1 - 4 : t.cc1 - 4;  // the line index
+  if (lline >= hline) return;
+  const int step = horiz ? 1 : V1;   // index distance of one pixel along the chain
+  const int base = horiz ? lline * TS : lline;
+  int pos = lo + parity;
+  if (pos >= hi) return;
+  int i = base + pos * step;
+  float prev = cd[i - 2 * step];
+  float aprev = alt[i - 2 * step];
+  float cur = cd[i], acur = alt[i];
+  for (; pos < hi; pos += 2, i += 2 * step) {
+    const float next = cd[i + 2 * step], anext = alt[i + 2 * step];
+    const int rr = horiz ? lline : pos, cc = horiz ? pos : lline;
+    const bool gsite = (c.color_at(t.top + rr, t.left + cc) & 1) != 0;
+    const float v = cd_update(prev, cur, next, aprev, acur, anext, gsite, cfa[i], cfa[i - step], cfa[i + step]);
+    cd[i] = v;
+    prev = v; aprev = acur;
+    cur = next; acur = anext;
+  }
+}
+
+__device__ void pass_cddiffsq(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* vcd = c.P(S_VCD);
+  const float* hcd = c.P(S_HCD);
+  float* cddiffsq = c.P(S_CDDIFFSQ);
+  for (int i = tid; i < 16384; i += NT) {
+    const int rr = i >> 7, cc = i & 127;
+    float v = 0.0f;
+    if (rr >= 4 && rr < t.rr1 - 4 && cc >= 4 && cc < t.cc1 - 4 && (c.color_at(t.top + rr, t.left + cc) & 1) == 0)
+      v = sqr(vcd[i] - hcd[i]);
+    cddiffsq[i] = v;
+  }
+}
+
+// demosaic.rs:788-828 (hvwt) fused with the Nyquist test of 830-868
+// (nyqutest only feeds the mask and its bounding box).
+__device__ void pass_hvwt_nyq(Ctx& c, int tid, int* bbox) {
+  const Tile& t = c.t;
+  const float* vcd = c.P(S_VCD);
+  const float* hcd = c.P(S_HCD);
+  const float* dirwts0 = c.P(S_DIRWTS0);
+  const float* dirwts1 = c.P(S_DIRWTS1);
+  const float* dgintv = c.P(S_DGINTV);
+  const float* dginth = c.P(S_DGINTH);
+  const float* cddiffsq = c.P(S_CDDIFFSQ);
+  const float* delhvsqsum = c.P(S_DELHVSQSUM);
+  const float gaussodd0 = 0.14659727707323927f, gaussodd1 = 0.103592713382435f, gaussodd2 = 0.0732036125103057f,
+              gaussodd3 = 0.0365543548389495f;
+  const float nyqthresh = 0.5f;
+  const float gg0 = nyqthresh * 0.07384411893421103f, gg1 = nyqthresh * 0.06207511968171489f,
+              gg2 = nyqthresh * 0.0521818194747806f, gg3 = nyqthresh * 0.03687419286733595f,
+              gg4 = nyqthresh * 0.03099732204057846f, gg5 = nyqthresh * 0.018413194161458882f;
+  int minr = INT32_MAX, maxr = -1, minc = INT32_MAX, maxc = -1;
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    float hv = 0.0f;
+    uint8_t ny = 0;
+    if (rr >= 6 && rr < t.rr1 - 6) {
+      const int p = c.par(rr);
+      const int cc = 2 * j + p;
+      if (cc >= 6 + p && cc < t.cc1 - 6) {
+        const int i = rr * TS + cc;
+        float uave = vcd[i] + vcd[i - V1] + vcd[i - V2] + vcd[i - V3];
+        float dave = vcd[i] + vcd[i + V1] + vcd[i + V2] + vcd[i + V3];
+        float lave = hcd[i] + hcd[i - 1] + hcd[i - 2] + hcd[i - 3];
+        float rave = hcd[i] + hcd[i + 1] + hcd[i + 2] + hcd[i + 3];
+        float dgrbvvaru = sqr(vcd[i] - uave) + sqr(vcd[i - V1] - uave) + sqr(vcd[i - V2] - uave) + sqr(vcd[i - V3] - uave);
+        float dgrbvvard = sqr(vcd[i] - dave) + sqr(vcd[i + V1] - dave) + sqr(vcd[i + V2] - dave) + sqr(vcd[i + V3] - dave);
+        float dgrbhvarl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) + sqr(hcd[i - 3] - lave);
+        float dgrbhvarr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) + sqr(hcd[i + 3] - rave);
+        float hwt = dirwts1[i - 1] / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
+        float vwt = dirwts0[i - V1] / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
+        float vcdvar = epssq + vwt * dgrbvvard + (1.0f - vwt) * dgrbvvaru;
+        float hcdvar = epssq + hwt * dgrbhvarr + (1.0f - hwt) * dgrbhvarl;
+        float dgrbvvaru1 = dgintv[i] + dgintv[i - V1] + dgintv[i - V2];
+        float dgrbvvard1 = dgintv[i] + dgintv[i + V1] + dgintv[i + V2];
+        float dgrbhvarl1 = dginth[i] + dginth[i - 1] + dginth[i - 2];
+        float dgrbhvarr1 = dginth[i] + dginth[i + 1] + dginth[i + 2];
+        float vcdvar1 = epssq + vwt * dgrbvvard1 + (1.0f - vwt) * dgrbvvaru1;
+        float hcdvar1 = epssq + hwt * dgrbhvarr1 + (1.0f - hwt) * dgrbhvarl1;
+        float varwt = hcdvar / fmaxf(vcdvar + hcdvar, 1e-10f);
+        float diffwt = hcdvar1 / fmaxf(vcdvar1 + hcdvar1, 1e-10f);
+        hv = ((0.5f - varwt) * (0.5f - diffwt) > 0.0f && fabsf(0.5f - diffwt) < fabsf(0.5f - varwt)) ? varwt : diffwt;
+        float nt = (gaussodd0 * cddiffsq[i] +
+                    gaussodd1 * (cddiffsq[i - M1] + cddiffsq[i - P1] + cddiffsq[i + P1] + cddiffsq[i + M1]) +
+                    gaussodd2 * (cddiffsq[i - V2] + cddiffsq[i - 2] + cddiffsq[i + 2] + cddiffsq[i + V2]) +
+                    gaussodd3 * (cddiffsq[i - M2] + cddiffsq[i - P2] + cddiffsq[i + P2] + cddiffsq[i + M2])) -
+                   (gg0 * delhvsqsum[i] +
+                    gg1 * (delhvsqsum[i - V1] + delhvsqsum[i + 1] + delhvsqsum[i - 1] + delhvsqsum[i + V1]) +
+                    gg2 * (delhvsqsum[i - M1] + delhvsqsum[i - P1] + delhvsqsum[i + P1] + delhvsqsum[i + M1]) +
+                    gg3 * (delhvsqsum[i - V2] + delhvsqsum[i - 2] + delhvsqsum[i + 2] + delhvsqsum[i + V2]) +
+                    gg4 * (delhvsqsum[i - V2 - 1] + delhvsqsum[i - V2 + 1] + delhvsqsum[i - TS - 2] +
+                           delhvsqsum[i - TS + 2] + delhvsqsum[i + TS - 2] + delhvsqsum[i + TS + 2] +
+                           delhvsqsum[i + V2 - 1] + delhvsqsum[i + V2 + 1]) +
+                    gg5 * (delhvsqsum[i - M2] + delhvsqsum[i - P2] + delhvsqsum[i + P2] + delhvsqsum[i + M2]));
+        if (nt > 0.0f) {
+          ny = 1;
+          minr = min(minr, rr); maxr = max(maxr, rr);
+          minc = min(minc, cc); maxc = max(maxc, cc);
+        }
+      }
+    }
+    c.hvwt[h] = hv;
+    c.nyq[h] = ny;
+  }
+  if (maxr >= 0) {
+    atomicMin(&bbox[0], minr);
+    atomicMax(&bbox[1], maxr);
+    atomicMin(&bbox[2], minc);
+    atomicMax(&bbox[3], maxc);
+  }
+}
+
+// nyquist2 (demosaic.rs:888-910); bb holds the adjusted bounding box.
+__device__ void pass_nyq2(Ctx& c, int tid, bool do_nyq, const int* bb) {
+  for (int h = tid; h < 8192; h += NT) {
+    uint8_t v = 0;
+    if (do_nyq) {
+      const int rr = h >> 6, j = h & 63;
+      if (rr >= bb[0] && rr < bb[1]) {
+        const int cc = 2 * j + c.par(rr);
+        if (cc >= bb[2] + c.par(rr) && cc < bb[3]) {
+          const int i = rr * TS + cc;
+          const uint8_t* n = c.nyq;
+          unsigned s = (unsigned)n[(i - V2) >> 1] + n[(i - M1) >> 1] + n[(i - P1) >> 1] + n[(i - 2) >> 1] +
+                       n[(i + 2) >> 1] + n[(i + P1) >> 1] + n[(i + M1) >> 1] + n[(i + V2) >> 1];
+          v = s > 4 ? 1 : (s < 4 ? 0 : n[i >> 1]);
+        }
+      }
+    }
+    c.nyq2[h] = v;
+  }
+}
+
+// Area statistics on Nyquist sites (demosaic.rs:912-945).
+__device__ void pass_nyq_area(Ctx& c, int tid, const int* bb) {
+  const float* cfa = c.cfa;
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    if (rr < bb[0] || rr >= bb[1]) continue;
+    const int cc = 2 * j + c.par(rr);
+    if (cc < bb[2] + c.par(rr) || cc >= bb[3]) continue;
+    const int i = rr * TS + cc;
+    if (!c.nyq2[i >> 1]) continue;
+    float sumcfa = 0.0f, sumh = 0.0f, sumv = 0.0f, sumsqh = 0.0f, sumsqv = 0.0f, areawt = 0.0f;
+    for (int a = -6; a < 7; a += 2) {
+      int i1 = i + a * TS - 6;
+      for (int b = -6; b < 7; b += 2, i1 += 2) {
+        if (c.nyq2[i1 >> 1]) {
+          float ct = cfa[i1];
+          sumcfa += ct;
+          sumh += cfa[i1 - 1] + cfa[i1 + 1];
+          sumv += cfa[i1 - V1] + cfa[i1 + V1];
+          sumsqh += sqr(ct - cfa[i1 - 1]) + sqr(ct - cfa[i1 + 1]);
+          sumsqv += sqr(ct - cfa[i1 - V1]) + sqr(ct - cfa[i1 + V1]);
+          areawt += 1.0f;
+        }
+      }
+    }
+    sumh = sumcfa - sumh * 0.5f;
+    sumv = sumcfa - sumv * 0.5f;
+    areawt = areawt * 0.5f;
+    float hcdvar = epssq + fabsf(areawt * sumsqh - sumh * sumh);
+    float vcdvar = epssq + fabsf(areawt * sumsqv - sumv * sumv);
+    c.hvwt[i >> 1] = hcdvar / (vcdvar + hcdvar);
+  }
+}
+
+// Diagonal smoothing of a half-resolution weight plane, in place, row by row
+// in one wave (demosaic.rs:951-956 for hvwt, 1113-1117 for pmwt): a row reads
+// the already-smoothed row above and the untouched row below.
+__device__ void pass_diag_smooth(Ctx& c, int lane, float* w, int r0, int r1, int margin) {
+  for (int rr = r0; rr < r1; ++rr) {
+    const int p = c.par(rr);
+    const int cc = margin + p + 2 * lane;
+    float nv = 0.0f;
+    bool act = cc < c.t.cc1 - margin;
+    const int i = rr * TS + cc;
+    if (act) {
+      float alt = (w[(i - M1) >> 1] + w[(i - P1) >> 1] + w[(i + P1) >> 1] + w[(i + M1) >> 1]) / 4.0f;
+      float cur = w[i >> 1];
+      nv = fabsf(0.5f - cur) < fabsf(0.5f - alt) ? alt : cur;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (act) w[i >> 1] = nv;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// G at R/B sites after the hvwt smoothing (demosaic.rs:957-971).
+__device__ void pass_green(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const float* vcd = c.P(S_VCD);
+  const float* hcd = c.P(S_HCD);
+  float* rgbgreen = c.P(S_RGBGREEN);
+  float* dgrb0 = c.P(S_DGRB0);
+  float* dgrb2h = c.P(S_DGRB2H);
+  float* dgrb2v = c.P(S_DGRB2V);
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    float d0 = 0.0f, d2h = 0.0f, d2v = 0.0f;
+    if (rr >= 8 && rr < t.rr1 - 8) {
+      const int p = c.par(rr);
+      const int cc = 2 * j + p;
+      if (cc >= 8 + p && cc < t.cc1 - 8) {
+        const int i = rr * TS + cc;
+        const float hw = c.hvwt[h];
+        d0 = hw * vcd[i] + (1.0f - hw) * hcd[i];
+        const float g = cfa[i] + d0;
+        rgbgreen[i] = g;
+        if (c.nyq2[h]) {
+          d2h = sqr(g - (rgbgreen[i - 1] + rgbgreen[i + 1]) * 0.5f);
+          d2v = sqr(g - (rgbgreen[i - V1] + rgbgreen[i + V1]) * 0.5f);
+        }
+      }
+    }
+    dgrb0[h] = d0;
+    dgrb2h[h] = d2h;
+    dgrb2v[h] = d2v;
+  }
+}
+
+// Nyquist-region refinement of G (demosaic.rs:975-998).
+__device__ void pass_green_nyq(Ctx& c, int tid, const int* bb, int nthreads) {
+  const float gq0 = 0.169917f, gq1 = 0.108947f, gq2 = 0.069855f, gq3 = 0.0287182f;
+  const float* cfa = c.cfa;
+  const float* vcd = c.P(S_VCD);
+  const float* hcd = c.P(S_HCD);
+  const float* d2h = c.P(S_DGRB2H);
+  const float* d2v = c.P(S_DGRB2V);
+  float* rgbgreen = c.P(S_RGBGREEN);
+  float* dgrb0 = c.P(S_DGRB0);
+  for (int h = tid; h < 8192; h += nthreads) {
+    const int rr = h >> 6, j = h & 63;
+    if (rr < bb[0] || rr >= bb[1]) continue;
+    const int cc = 2 * j + c.par(rr);
+    if (cc < bb[2] + c.par(rr) || cc >= bb[3]) continue;
+    const int i = rr * TS + cc;
+    if (!c.nyq2[i >> 1]) continue;
+    float gvarh = epssq + (gq0 * d2h[i >> 1] +
+                           gq1 * (d2h[(i - M1) >> 1] + d2h[(i - P1) >> 1] + d2h[(i + P1) >> 1] + d2h[(i + M1) >> 1]) +
+                           gq2 * (d2h[(i - V2) >> 1] + d2h[(i - 2) >> 1] + d2h[(i + 2) >> 1] + d2h[(i + V2) >> 1]) +
+                           gq3 * (d2h[(i - M2) >> 1] + d2h[(i - P2) >> 1] + d2h[(i + P2) >> 1] + d2h[(i + M2) >> 1]));
+    float gvarv = epssq + (gq0 * d2v[i >> 1] +
+                           gq1 * (d2v[(i - M1) >> 1] + d2v[(i - P1) >> 1] + d2v[(i + P1) >> 1] + d2v[(i + M1) >> 1]) +
+                           gq2 * (d2v[(i - V2) >> 1] + d2v[(i - 2) >> 1] + d2v[(i + 2) >> 1] + d2v[(i + V2) >> 1]) +
+                           gq3 * (d2v[(i - M2) >> 1] + d2v[(i - P2) >> 1] + d2v[(i + P2) >> 1] + d2v[(i + M2) >> 1]));
+    float d0 = (hcd[i] * gvarv + vcd[i] * gvarh) / (gvarv + gvarh);
+    dgrb0[i >> 1] = d0;
+    rgbgreen[i] = cfa[i] + d0;
+  }
+}
+
+// Diagonal gradient setup (demosaic.rs:1001-1019).
+__device__ void pass_diag_setup(Ctx& c, int tid, int nthreads) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  float* delp = c.P(S_DELP);
+  float* delm = c.P(S_DELM);
+  float* sq1p = c.P(S_DGRBSQ1P);
+  float* sq1m = c.P(S_DGRBSQ1M);
+  for (int h = tid; h < 8192; h += nthreads) {
+    const int rr = h >> 6, j = h & 63;
+    const int cc = 2 * j;
+    float dp = 0.0f, dm = 0.0f, sp = 0.0f, sm = 0.0f;
+    if (rr >= 6 && rr < t.rr1 - 6 && cc >= 6 && cc < t.cc1 - 6) {
+      const int i = rr * TS + cc;
+      if (c.par(rr) == 0) {
+        dp = fabsf(cfa[i - P1] - cfa[i + P1]);
+        dm = fabsf(cfa[i + M1] - cfa[i - M1]);
+        sp = sqr(cfa[i + 1] - cfa[i + 1 - P1]) + sqr(cfa[i + 1] - cfa[i + 1 + P1]);
+        sm = sqr(cfa[i + 1] - cfa[i + 1 + M1]) + sqr(cfa[i + 1] - cfa[i + 1 - M1]);
+      } else {
+        sp = sqr(cfa[i] - cfa[i - P1]) + sqr(cfa[i] - cfa[i + P1]);
+        sm = sqr(cfa[i] - cfa[i - M1]) + sqr(cfa[i] - cfa[i + M1]);
+        dp = fabsf(cfa[i + 1 - P1] - cfa[i + 1 + P1]);
+        dm = fabsf(cfa[i + 1 - M1] - cfa[i + 1 + M1]);
+      }
+    }
+    delp[h] = dp;
+    delm[h] = dm;
+    sq1p[h] = sp;
+    sq1m[h] = sm;
+  }
+}
+
+// Diagonal R/B estimates and their weight (demosaic.rs:1023-1107).
+__device__ void pass_diag(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const float* delp = c.P(S_DELP);
+  const float* delm = c.P(S_DELM);
+  const float* sq1p = c.P(S_DGRBSQ1P);
+  const float* sq1m = c.P(S_DGRBSQ1M);
+  float* rbm = c.P(S_RBM);
+  float* rbp = c.P(S_RBP);
+  const float ge0 = 0.13719494435797422f, ge1 = 0.05640252782101291f;
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    float vm = 0.0f, vp = 0.0f, pw = 0.0f;
+    if (rr >= 8 && rr < t.rr1 - 8) {
+      const int p = c.par(rr);
+      const int cc = 2 * j + p;
+      if (cc >= 8 + p && cc < t.cc1 - 8) {
+        const int i = rr * TS + cc, i1 = i >> 1;
+        const float ci = cfa[i];
+        float crse = (cfa[i + M1] * 2.0f) / (eps + ci + cfa[i + M2]);
+        float crnw = (cfa[i - M1] * 2.0f) / (eps + ci + cfa[i - M2]);
+        float crne = (cfa[i - P1] * 2.0f) / (eps + ci + cfa[i - P2]);
+        float crsw = (cfa[i + P1] * 2.0f) / (eps + ci + cfa[i + P2]);
+        float rbse = fabsf(1.0f - crse) < arthresh ? ci * crse : cfa[i + M1] + (ci - cfa[i + M2]) * 0.5f;
+        float rbnw = fabsf(1.0f - crnw) < arthresh ? ci * crnw : cfa[i - M1] + (ci - cfa[i - M2]) * 0.5f;
+        float rbne = fabsf(1.0f - crne) < arthresh ? ci * crne : cfa[i - P1] + (ci - cfa[i - P2]) * 0.5f;
+        float rbsw = fabsf(1.0f - crsw) < arthresh ? ci * crsw : cfa[i + P1] + (ci - cfa[i + P2]) * 0.5f;
+        float wtse = eps + delm[i1] + delm[(i + M1) >> 1] + delm[(i + M2) >> 1];
+        float wtnw = eps + delm[i1] + delm[(i - M1) >> 1] + delm[(i - M2) >> 1];
+        float wtne = eps + delp[i1] + delp[(i - P1) >> 1] + delp[(i - P2) >> 1];
+        float wtsw = eps + delp[i1] + delp[(i + P1) >> 1] + delp[(i + P2) >> 1];
+        vm = (wtse * rbnw + wtnw * rbse) / (wtse + wtnw);
+        vp = (wtne * rbsw + wtsw * rbne) / (wtne + wtsw);
+        float rbvarm = epssq + ge0 * (sq1m[(i - V1) >> 1] + sq1m[(i - 1) >> 1] + sq1m[(i + 1) >> 1] + sq1m[(i + V1) >> 1]) +
+                       ge1 * (sq1m[(i - V2 - 1) >> 1] + sq1m[(i - V2 + 1) >> 1] + sq1m[(i - 2 - V1) >> 1] +
+                              sq1m[(i + 2 - V1) >> 1] + sq1m[(i - 2 + V1) >> 1] + sq1m[(i + 2 + V1) >> 1] +
+                              sq1m[(i + V2 - 1) >> 1] + sq1m[(i + V2 + 1) >> 1]);
+        float rbvarp = epssq + ge0 * (sq1p[(i - V1) >> 1] + sq1p[(i - 1) >> 1] + sq1p[(i + 1) >> 1] + sq1p[(i + V1) >> 1]) +
+                       ge1 * (sq1p[(i - V2 - 1) >> 1] + sq1p[(i - V2 + 1) >> 1] + sq1p[(i - 2 - V1) >> 1] +
+                              sq1p[(i + 2 - V1) >> 1] + sq1p[(i - 2 + V1) >> 1] + sq1p[(i + 2 + V1) >> 1] +
+                              sq1p[(i + V2 - 1) >> 1] + sq1p[(i + V2 + 1) >> 1]);
+        pw = rbvarm / (rbvarp + rbvarm);
+        if (vp < ci) {
+          if (vp * 2.0f < ci) {
+            vp = ulim(vp, cfa[i - P1], cfa[i + P1]);
+          } else {
+            float pwt = (ci - vp) * 2.0f / (eps + vp + ci);
+            vp = pwt * vp + (1.0f - pwt) * ulim(vp, cfa[i - P1], cfa[i + P1]);
+          }
+        }
+        if (vm < ci) {
+          if (vm * 2.0f < ci) {
+            vm = ulim(vm, cfa[i - M1], cfa[i + M1]);
+          } else {
+            float mwt = (ci - vm) * 2.0f / (eps + vm + ci);
+            vm = mwt * vm + (1.0f - mwt) * ulim(vm, cfa[i - M1], cfa[i + M1]);
+          }
+        }
+        if (vp > clip_pt) vp = ulim(vp, cfa[i - P1], cfa[i + P1]);
+        if (vm > clip_pt) vm = ulim(vm, cfa[i - M1], cfa[i + M1]);
+      }
+    }
+    rbm[h] = vm;
+    rbp[h] = vp;
+    c.pmwt[h] = pw;
+  }
+}
+
+// rbint after the pmwt smoothing (demosaic.rs:1118-1121).
+__device__ void pass_rbint(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const float* rbm = c.P(S_RBM);
+  const float* rbp = c.P(S_RBP);
+  float* rbint = c.P(S_RBINT);
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    float v = 0.0f;
+    if (rr >= 10 && rr < t.rr1 - 10) {
+      const int p = c.par(rr);
+      const int cc = 2 * j + p;
+      if (cc >= 10 + p && cc < t.cc1 - 10) {
+        const int i = rr * TS + cc;
+        const float pw = c.pmwt[h];
+        v = (cfa[i] + rbm[h] * (1.0f - pw) + rbp[h] * pw) * 0.5f;
+      }
+    }
+    rbint[h] = v;
+  }
+}
+
+// G refinement at R/B sites from the diagonal estimates (demosaic.rs:1125-1194).
+__device__ void pass_green_diag(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* cfa = c.cfa;
+  const float* rbint = c.P(S_RBINT);
+  const float* dirwts0 = c.P(S_DIRWTS0);
+  const float* dirwts1 = c.P(S_DIRWTS1);
+  float* rgbgreen = c.P(S_RGBGREEN);
+  float* dgrb0 = c.P(S_DGRB0);
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    if (rr < 12 || rr >= t.rr1 - 12) continue;
+    const int p = c.par(rr);
+    const int cc = 2 * j + p;
+    if (cc < 12 + p || cc >= t.cc1 - 12) continue;
+    const int i = rr * TS + cc, i1 = i >> 1;
+    if (!(fabsf(0.5f - c.pmwt[i1]) >= fabsf(0.5f - c.hvwt[i1]))) continue;
+    const float rb = rbint[i1];
+    float cru = cfa[i - V1] * 2.0f / (eps + rb + rbint[i1 - V1]);
+    float crd = cfa[i + V1] * 2.0f / (eps + rb + rbint[i1 + V1]);
+    float crl = cfa[i - 1] * 2.0f / (eps + rb + rbint[i1 - 1]);
+    float crr = cfa[i + 1] * 2.0f / (eps + rb + rbint[i1 + 1]);
+    float gu = fabsf(1.0f - cru) < arthresh ? rb * cru : cfa[i - V1] + (rb - rbint[i1 - V1]) * 0.5f;
+    float gd = fabsf(1.0f - crd) < arthresh ? rb * crd : cfa[i + V1] + (rb - rbint[i1 + V1]) * 0.5f;
+    float gl = fabsf(1.0f - crl) < arthresh ? rb * crl : cfa[i - 1] + (rb - rbint[i1 - 1]) * 0.5f;
+    float gr = fabsf(1.0f - crr) < arthresh ? rb * crr : cfa[i + 1] + (rb - rbint[i1 + 1]) * 0.5f;
+    float gintv = (dirwts0[i - V1] * gd + dirwts0[i + V1] * gu) / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
+    float ginth = (dirwts1[i - 1] * gr + dirwts1[i + 1] * gl) / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
+    if (gintv < rb) {
+      if (2.0f * gintv < rb) {
+        gintv = ulim(gintv, cfa[i - V1], cfa[i + V1]);
+      } else {
+        float vwt = 2.0f * (rb - gintv) / (eps + gintv + rb);
+        gintv = vwt * gintv + (1.0f - vwt) * ulim(gintv, cfa[i - V1], cfa[i + V1]);
+      }
+    }
+    if (ginth < rb) {
+      if (2.0f * ginth < rb) {
+        ginth = ulim(ginth, cfa[i - 1], cfa[i + 1]);
+      } else {
+        float hwt = 2.0f * (rb - ginth) / (eps + ginth + rb);
+        ginth = hwt * ginth + (1.0f - hwt) * ulim(ginth, cfa[i - 1], cfa[i + 1]);
+      }
+    }
+    if (ginth > clip_pt) ginth = ulim(ginth, cfa[i - 1], cfa[i + 1]);
+    if (gintv > clip_pt) gintv = ulim(gintv, cfa[i - V1], cfa[i + V1]);
+    const float hw = c.hvwt[i1];
+    const float g = ginth * (1.0f - hw) + gintv * hw;
+    rgbgreen[i] = g;
+    dgrb0[i1] = g - cfa[i];
+  }
+}
+
+// Split G-R / G-B differences by colour (demosaic.rs:1204-1213).
+__device__ void pass_split(Ctx& c, int tid, int ex, int ey) {
+  const Tile& t = c.t;
+  float* dgrb0 = c.P(S_DGRB0);
+  float* dgrb1 = c.P(S_DGRB1);
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6;
+    float v1 = 0.0f;
+    if (rr >= 13 - ey && rr < t.rr1 - 12 && ((rr - (13 - ey)) & 1) == 0 && h >= ((rr * TS + 13 - ex) >> 1) &&
+        h < ((rr * TS + t.cc1 - 12) >> 1)) {
+      v1 = dgrb0[h];
+      dgrb0[h] = 0.0f;
+    }
+    dgrb1[h] = v1;
+  }
+}
+
+// Chroma interpolation at R/B sites (demosaic.rs:1215-1252).
+__device__ void pass_chroma(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  for (int h = tid; h < 8192; h += NT) {
+    const int rr = h >> 6, j = h & 63;
+    if (rr < 14 || rr >= t.rr1 - 14) continue;
+    const int p = c.par(rr);
+    const int cc = 2 * j + p;
+    if (cc < 14 + p || cc >= t.cc1 - 14) continue;
+    const int i = rr * TS + cc;
+    const int ch = 1 - c.color_at(t.top + rr, t.left + cc) / 2;
+    float* d = c.P(ch == 0 ? S_DGRB0 : S_DGRB1);
+    float wtnw = 1.0f / (eps + fabsf(d[(i - M1) >> 1] - d[(i + M1) >> 1]) + fabsf(d[(i - M1) >> 1] - d[(i - M3) >> 1]) +
+                         fabsf(d[(i + M1) >> 1] - d[(i - M3) >> 1]));
+    float wtne = 1.0f / (eps + fabsf(d[(i - P1) >> 1] - d[(i + P1) >> 1]) + fabsf(d[(i - P1) >> 1] - d[(i - P3) >> 1]) +
+                         fabsf(d[(i + P1) >> 1] - d[(i - P3) >> 1]));
+    float wtsw = 1.0f / (eps + fabsf(d[(i + P1) >> 1] - d[(i - P1) >> 1]) + fabsf(d[(i + P1) >> 1] - d[(i + M3) >> 1]) +
+                         fabsf(d[(i - P1) >> 1] - d[(i + P3) >> 1]));
+    float wtse = 1.0f / (eps + fabsf(d[(i + M1) >> 1] - d[(i - M1) >> 1]) + fabsf(d[(i + M1) >> 1] - d[(i + P3) >> 1]) +
+                         fabsf(d[(i - M1) >> 1] - d[(i + M3) >> 1]));
+    float val = (wtnw * (1.325f * d[(i - M1) >> 1] - 0.175f * d[(i - M3) >> 1] - 0.075f * d[(i - M1 - 2) >> 1] -
+                         0.075f * d[(i - M1 - V2) >> 1]) +
+                 wtne * (1.325f * d[(i - P1) >> 1] - 0.175f * d[(i - P3) >> 1] - 0.075f * d[(i - P1 + 2) >> 1] -
+                         0.075f * d[(i - P1 + V2) >> 1]) +
+                 wtsw * (1.325f * d[(i + P1) >> 1] - 0.175f * d[(i + P3) >> 1] - 0.075f * d[(i + P1 - 2) >> 1] -
+                         0.075f * d[(i - M1) >> 1]) +
+                 wtse * (1.325f * d[(i + M1) >> 1] - 0.175f * d[(i + M3) >> 1] - 0.075f * d[(i + M1 + 2) >> 1] -
+                         0.075f * d[(i + M1 + V2) >> 1])) /
+                (wtnw + wtne + wtsw + wtse);
+    d[i >> 1] = val;
+  }
+}
+
+// Output of the tile's 96x96 core (demosaic.rs:1254-1366).
+__device__ void pass_output(Ctx& c, int tid) {
+  const Tile& t = c.t;
+  const float* rgbgreen = c.P(S_RGBGREEN);
+  const float* dgrb0 = c.P(S_DGRB0);
+  const float* dgrb1 = c.P(S_DGRB1);
+  const float* hvwt = c.hvwt;
+  const int ncol = t.cc1 - 32 > 0 ? t.cc1 - 32 : 0;  // output columns 16 .. cc1-16
+  const int nrow = t.rr1 - 32 > 0 ? t.rr1 - 32 : 0;
+  for (int k = tid; k < nrow * ncol; k += NT) {
+    const int rr = 16 + k / ncol, cc = 16 + k % ncol;
+    const int row = rr + t.top, col = cc + t.left;
+    if (!(col >= 0 && col < c.a.width && row >= 0 && row < c.a.height)) continue;
+    const int i = rr * TS + cc;
+    // G-first rows compute the full form at even offsets from 16, the direct
+    // form at odd ones (and vice versa); an odd trailing column takes the
+    // first form of its row.
+    const bool g_first = c.par(rr) == 1;
+    const bool even_off = ((cc - 16) & 1) == 0;
+    const bool full = g_first == even_off;
+    float r, g, b;
+    const float gg = rgbgreen[i];
+    if (full) {
+      const float hu = hvwt[(i - V1) >> 1], hr = hvwt[(i + 1) >> 1], hl = hvwt[(i - 1) >> 1], hd = hvwt[(i + V1) >> 1];
+      const float temp = 1.0f / fmaxf(hu + 2.0f - hr - hl + hd, 1e-5f);
+      r = clampnan(gg - (hu * dgrb0[(i - V1) >> 1] + (1.0f - hr) * dgrb0[(i + 1) >> 1] + (1.0f - hl) * dgrb0[(i - 1) >> 1] +
+                         hd * dgrb0[(i + V1) >> 1]) * temp,
+                   0.0f, 1.0f);
+      b = clampnan(gg - (hu * dgrb1[(i - V1) >> 1] + (1.0f - hr) * dgrb1[(i + 1) >> 1] + (1.0f - hl) * dgrb1[(i - 1) >> 1] +
+                         hd * dgrb1[(i + V1) >> 1]) * temp,
+                   0.0f, 1.0f);
+    } else {
+      r = clampnan(gg - dgrb0[i >> 1], 0.0f, 1.0f);
+      b = clampnan(gg - dgrb1[i >> 1], 0.0f, 1.0f);
+    }
+    g = clampnan(gg, 0.0f, 1.0f);
+    float* o = c.a.out + ((size_t)row * c.a.width + col) * 3;
+    o[0] = r; o[1] = g; o[2] = b;
+  }
+}
+
+__global__ void __launch_bounds__(NT) amaze_kernel(AmazeArgs a) {
+  __shared__ float s_cfa[16384];
+  __shared__ float s_hvwt[8192];
+  __shared__ float s_pmwt[8192];
+  __shared__ uint8_t s_nyq[8192];
+  __shared__ uint8_t s_nyq2[8192];
+  __shared__ int s_bbox[4];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  float* slot = a.scratch + (size_t)blockIdx.x * kAmazeSlotFloats;
+
+  // (ey, ex): location of the R site in the pattern (demosaic.rs:1196-1202)
+  int ex, ey;
+  if (a.cfa[0][0] == 1) {
+    if (a.cfa[0][1] == 0) { ex = 1; ey = 0; } else { ex = 0; ey = 1; }
+  } else {
+    if (a.cfa[0][0] == 0) { ex = 0; ey = 0; } else { ex = 1; ey = 1; }
+  }
+
+  for (int tile = blockIdx.x; tile < a.tiles_x * a.tiles_y; tile += gridDim.x) {
+    Ctx c{a, {}, slot, s_cfa, s_hvwt, s_pmwt, s_nyq, s_nyq2};
+    const int ty = tile / a.tiles_x, tx = tile % a.tiles_x;
+    Tile& t = c.t;
+    t.top = -16 + 96 * ty;
+    t.left = -16 + 96 * tx;
+    const int bottom = min(t.top + 128, a.height + 16), right = min(t.left + 128, a.width + 16);
+    t.rr1 = bottom - t.top;
+    t.cc1 = right - t.left;
+    t.rrmin = t.top < 0 ? 16 : 0;
+    t.ccmin = t.left < 0 ? 16 : 0;
+    t.rrmax = bottom > a.height ? a.height - t.top : t.rr1;
+    t.ccmax = right > a.width ? a.width - t.left : t.cc1;
+
+    if (tid == 0) { s_bbox[0] = INT32_MAX; s_bbox[1] = -1; s_bbox[2] = INT32_MAX; s_bbox[3] = -1; }
+    pass_fill(c, tid);
+    __syncthreads();
+    pass_gradients(c, tid);
+    __syncthreads();
+    pass_colour_diff(c, tid);
+    __syncthreads();
+    pass_cd_chains(c, tid);
+    __syncthreads();
+    pass_cddiffsq(c, tid);
+    __syncthreads();
+    pass_hvwt_nyq(c, tid, s_bbox);
+    __syncthreads();
+
+    // Nyquist bounding box (demosaic.rs:849-886)
+    int bb[4] = {0, 0, 0, 0};
+    bool do_nyq = false;
+    if (s_bbox[1] >= 0) {
+      int sr = s_bbox[0], er = s_bbox[1], sc = s_bbox[2], ec = s_bbox[3];
+      do_nyq = sr != er && sc != ec;
+      if (do_nyq) {
+        er += 1; ec += 1;
+        sc -= sc & 1;
+        sr = max(sr, 8); er = min(er, t.rr1 - 8);
+        sc = max(sc, 8); ec = min(ec, t.cc1 - 8);
+        bb[0] = sr; bb[1] = er; bb[2] = sc; bb[3] = ec;
+      }
+    }
+    pass_nyq2(c, tid, do_nyq, bb);
+    __syncthreads();
+    if (do_nyq) pass_nyq_area(c, tid, bb);
+    __syncthreads();
+
+    // hvwt smoothing (wave 0) overlaps the diagonal setup (waves 1..7)
+    if (wave == 0) pass_diag_smooth(c, lane, s_hvwt, 8, t.rr1 - 8, 8);
+    else pass_diag_setup(c, tid - 64, NT - 64);
+    __syncthreads();
+    pass_green(c, tid);
+    __syncthreads();
+    pass_diag(c, tid);
+    __syncthreads();
+    // pmwt smoothing (wave 0) overlaps the Nyquist G refinement (waves 1..7)
+    if (wave == 0) pass_diag_smooth(c, lane, s_pmwt, 10, t.rr1 - 10, 10);
+    else if (do_nyq) pass_green_nyq(c, tid - 64, bb, NT - 64);
+    __syncthreads();
+    pass_rbint(c, tid);
+    __syncthreads();
+    pass_green_diag(c, tid);
+    __syncthreads();
+    pass_split(c, tid, ex, ey);
+    __syncthreads();
+    pass_chroma(c, tid);
+    __syncthreads();
+    pass_output(c, tid);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t amaze_tiles(int width, int height, int* tiles_x, int* tiles_y) {
+  int tx = 0, ty = 0;
+  for (int left = -16; left < width; left += 96) ++tx;
+  for (int top = -16; top < height; top += 96) ++ty;
+  if (tiles_x) *tiles_x = tx;
+  if (tiles_y) *tiles_y = ty;
+  return (size_t)tx * ty;
+}
+
+hipError_t amaze_launch(const AmazeArgs& args, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(amaze_kernel, dim3(grid), dim3(NT), 0, stream, args);
+  return hipGetLastError();
+}
+
+}  // namespace imops

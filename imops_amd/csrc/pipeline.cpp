@@ -1,0 +1,467 @@
+// Stage-list configuration and runner: the C++ counterpart of
+// pichromatic-pipeline/src/config.rs:13-34 (parse TOML, falling back to JSON;
+// `[[pipeline_modules]]` tables; module names matched case-insensitively
+// against the registry of modules.rs:897-915) and pipeline.rs:4-13 (run the
+// stages in order).
+//
+// Runs of consecutive elementwise stages (CFACoeffs, Exp, CST, Gamma) are
+// fused into one pointwise kernel launch; each stage keeps its own f32
+// arithmetic, so the output is the same as running them one by one.
+#include <cctype>
+#include <cstdlib>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "runtime.hpp"
+#include "stages.hpp"
+
+namespace imops {
+namespace {
+
+// ------------------------------------------------------------------ values
+struct Value {
+  enum Kind { STR, NUM, INT, BOOL, ARR, OBJ } kind = STR;
+  std::string s;
+  double num = 0;
+  long long i = 0;
+  bool b = false;
+  std::vector<Value> arr;
+  std::map<std::string, Value> obj;
+};
+using Table = std::map<std::string, Value>;
+
+[[noreturn]] void bad(const std::string& m) { throw Error{IMOPS_E_INVALID, "cant decode config from string: " + m}; }
+
+struct Cursor {
+  const std::string& t;
+  size_t p = 0;
+  explicit Cursor(const std::string& s) : t(s) {}
+  bool eof() const { return p >= t.size(); }
+  char peek() const { return eof() ? '\0' : t[p]; }
+  void skip_ws(bool newlines) {
+    while (!eof()) {
+      char c = t[p];
+      if (c == ' ' || c == '\t' || c == '\r' || (newlines && c == '\n')) ++p;
+      else if (c == '#') { while (!eof() && t[p] != '\n') ++p; }
+      else break;
+    }
+  }
+  bool take(char c) {
+    if (peek() == c) { ++p; return true; }
+    return false;
+  }
+};
+
+std::string parse_string(Cursor& c, char q) {
+  std::string out;
+  while (!c.eof() && c.peek() != q) {
+    char ch = c.t[c.p++];
+    if (ch == '\\' && q == '"' && !c.eof()) {
+      char e = c.t[c.p++];
+      switch (e) {
+        case 'n': out += '\n'; break;
+        case 't': out += '\t'; break;
+        case '"': out += '"'; break;
+        case '\\': out += '\\'; break;
+        case '/': out += '/'; break;
+        default: out += e;
+      }
+    } else {
+      out += ch;
+    }
+  }
+  if (!c.take(q)) bad("unterminated string");
+  return out;
+}
+
+Value parse_number(Cursor& c) {
+  size_t s = c.p;
+  while (!c.eof() && (std::isdigit((unsigned char)c.peek()) || std::strchr("+-.eE_", c.peek()))) ++c.p;
+  std::string tok;
+  for (size_t k = s; k < c.p; ++k)
+    if (c.t[k] != '_') tok += c.t[k];
+  if (tok.empty()) bad("expected a value");
+  Value v;
+  char* end = nullptr;
+  if (tok.find_first_of(".eE") == std::string::npos) {
+    v.kind = Value::INT;
+    v.i = std::strtoll(tok.c_str(), &end, 10);
+    v.num = (double)v.i;
+  } else {
+    v.kind = Value::NUM;
+    v.num = std::strtod(tok.c_str(), &end);
+  }
+  if (!end || *end) bad("bad number '" + tok + "'");
+  return v;
+}
+
+bool parse_word(Cursor& c, const char* w) {
+  size_t n = std::strlen(w);
+  if (c.t.compare(c.p, n, w) == 0) { c.p += n; return true; }
+  return false;
+}
+
+Value parse_toml_value(Cursor& c);
+
+Value parse_toml_inline_table(Cursor& c) {
+  Value v;
+  v.kind = Value::OBJ;
+  c.skip_ws(false);
+  if (c.take('}')) return v;
+  for (;;) {
+    c.skip_ws(false);
+    std::string key;
+    if (c.take('"')) key = parse_string(c, '"');
+    else {
+      while (!c.eof() && (std::isalnum((unsigned char)c.peek()) || c.peek() == '_' || c.peek() == '-')) key += c.t[c.p++];
+    }
+    if (key.empty()) bad("bad inline-table key");
+    c.skip_ws(false);
+    if (!c.take('=')) bad("expected '=' in inline table");
+    c.skip_ws(false);
+    v.obj[key] = parse_toml_value(c);
+    c.skip_ws(false);
+    if (c.take('}')) return v;
+    if (!c.take(',')) bad("expected ',' in inline table");
+  }
+}
+
+Value parse_toml_value(Cursor& c) {
+  char ch = c.peek();
+  if (ch == '"' || ch == '\'') {
+    ++c.p;
+    Value v;
+    v.kind = Value::STR;
+    v.s = parse_string(c, ch);
+    return v;
+  }
+  if (ch == '[') {
+    ++c.p;
+    Value v;
+    v.kind = Value::ARR;
+    for (;;) {
+      c.skip_ws(true);
+      if (c.take(']')) return v;
+      v.arr.push_back(parse_toml_value(c));
+      c.skip_ws(true);
+      if (c.take(']')) return v;
+      if (!c.take(',')) bad("expected ',' in array");
+    }
+  }
+  if (ch == '{') {
+    ++c.p;
+    return parse_toml_inline_table(c);
+  }
+  if (parse_word(c, "true")) { Value v; v.kind = Value::BOOL; v.b = true; return v; }
+  if (parse_word(c, "false")) { Value v; v.kind = Value::BOOL; v.b = false; return v; }
+  return parse_number(c);
+}
+
+std::vector<Table> parse_toml(const std::string& text) {
+  std::vector<Table> mods;
+  Cursor c(text);
+  bool in_table = false;
+  for (;;) {
+    c.skip_ws(true);
+    if (c.eof()) break;
+    if (c.peek() == '[') {
+      if (c.t.compare(c.p, 20, "[[pipeline_modules]]") != 0) bad("only [[pipeline_modules]] tables are allowed");
+      c.p += 20;
+      mods.emplace_back();
+      in_table = true;
+      continue;
+    }
+    std::string key;
+    if (c.take('"')) key = parse_string(c, '"');
+    else
+      while (!c.eof() && (std::isalnum((unsigned char)c.peek()) || c.peek() == '_' || c.peek() == '-')) key += c.t[c.p++];
+    if (key.empty()) bad("expected a key");
+    if (!in_table) bad("key outside [[pipeline_modules]]");
+    c.skip_ws(false);
+    if (!c.take('=')) bad("expected '=' after key " + key);
+    c.skip_ws(false);
+    mods.back()[key] = parse_toml_value(c);
+    c.skip_ws(false);
+    if (!c.eof() && c.peek() != '\n') bad("trailing characters after value of " + key);
+  }
+  return mods;
+}
+
+Value parse_json_value(Cursor& c) {
+  c.skip_ws(true);
+  char ch = c.peek();
+  if (ch == '"') { ++c.p; Value v; v.kind = Value::STR; v.s = parse_string(c, '"'); return v; }
+  if (ch == '[') {
+    ++c.p;
+    Value v; v.kind = Value::ARR;
+    c.skip_ws(true);
+    if (c.take(']')) return v;
+    for (;;) {
+      v.arr.push_back(parse_json_value(c));
+      c.skip_ws(true);
+      if (c.take(']')) return v;
+      if (!c.take(',')) bad("expected ',' in JSON array");
+    }
+  }
+  if (ch == '{') {
+    ++c.p;
+    Value v; v.kind = Value::OBJ;
+    c.skip_ws(true);
+    if (c.take('}')) return v;
+    for (;;) {
+      c.skip_ws(true);
+      if (!c.take('"')) bad("expected JSON key");
+      std::string k = parse_string(c, '"');
+      c.skip_ws(true);
+      if (!c.take(':')) bad("expected ':'");
+      v.obj[k] = parse_json_value(c);
+      c.skip_ws(true);
+      if (c.take('}')) return v;
+      if (!c.take(',')) bad("expected ',' in JSON object");
+    }
+  }
+  if (parse_word(c, "true")) { Value v; v.kind = Value::BOOL; v.b = true; return v; }
+  if (parse_word(c, "false")) { Value v; v.kind = Value::BOOL; v.b = false; return v; }
+  if (parse_word(c, "null")) bad("null value");
+  return parse_number(c);
+}
+
+std::vector<Table> parse_json(const std::string& text) {
+  Cursor c(text);
+  Value root = parse_json_value(c);
+  c.skip_ws(true);
+  if (!c.eof()) bad("trailing JSON");
+  if (root.kind != Value::OBJ || !root.obj.count("pipeline_modules")) bad("missing pipeline_modules");
+  const Value& arr = root.obj.at("pipeline_modules");
+  if (arr.kind != Value::ARR) bad("pipeline_modules must be an array");
+  std::vector<Table> mods;
+  for (const Value& m : arr.arr) {
+    if (m.kind != Value::OBJ) bad("module must be an object");
+    mods.push_back(m.obj);
+  }
+  return mods;
+}
+
+// Parameter<T> accepts a bare value or {value = ..., description = ...}
+// (modules.rs:40-67).
+const Value* param(const Table& t, const char* key) {
+  auto it = t.find(key);
+  if (it == t.end()) return nullptr;
+  const Value* v = &it->second;
+  if (v->kind == Value::OBJ) {
+    auto jt = v->obj.find("value");
+    if (jt == v->obj.end()) throw Error{IMOPS_E_INVALID, std::string("missing field `value` in ") + key};
+    v = &jt->second;
+  }
+  return v;
+}
+
+double num_param(const Table& t, const char* key, double def, const char* mod) {
+  const Value* v = param(t, key);
+  if (!v) return def;
+  if (v->kind != Value::NUM && v->kind != Value::INT) throw Error{IMOPS_E_INVALID, std::string("Invalid ") + mod + " config"};
+  return v->num;
+}
+
+std::string str_param(const Table& t, const char* key, const std::string& def, const char* mod) {
+  const Value* v = param(t, key);
+  if (!v) return def;
+  if (v->kind != Value::STR) throw Error{IMOPS_E_INVALID, std::string("Invalid ") + mod + " config"};
+  return v->s;
+}
+
+std::string lower(std::string s) {
+  for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
+  return s;
+}
+
+std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+}  // namespace
+
+enum StageKind {
+  K_DEMOSAIC, K_CFACOEFFS, K_EXP, K_GAMMA, K_CST, K_BM3D, K_FILM, K_OTHER
+};
+
+struct Stage {
+  std::string name;  // canonical schema name
+  StageKind kind = K_OTHER;
+  int algorithm = IMOPS_DEMOSAIC_AMAZE;
+  double ev = 0.0, gamma = 2.2;
+  std::string target;
+  Bm3dStageParams bm3d;
+  FilmStageParams film;
+};
+
+}  // namespace imops
+
+struct imops_pipeline {
+  std::vector<imops::Stage> stages;
+};
+
+namespace imops {
+namespace {
+
+// Registry names in modules.rs:897-915 order.
+const char* kModuleNames[] = {"Demosaic", "ChromaDenoise", "LumaGuidedChromaDenoise", "CFACoeffs", "Vignette",
+                              "HighlightReconstruction", "BaselineExposureCompensation", "Exp", "Contrast", "CST",
+                              "Film", "LCH", "SigmoidToneMap", "Gamma", "BM3D"};
+
+Stage make_stage(const Table& t) {
+  auto it = t.find("name");
+  if (it == t.end() || it->second.kind != Value::STR) throw Error{IMOPS_E_INVALID, "Module must have a name field"};
+  const std::string want = lower(it->second.s);
+  Stage s;
+  for (const char* n : kModuleNames)
+    if (lower(n) == want) s.name = n;
+  if (s.name.empty()) throw Error{IMOPS_E_INVALID, "wrong pipeline module name " + it->second.s};
+  if (s.name == "Demosaic") {
+    s.kind = K_DEMOSAIC;
+    std::string a = str_param(t, "algorithm", "amaze", "Demosaic");  // rename_all = lowercase
+    if (a == "amaze") s.algorithm = IMOPS_DEMOSAIC_AMAZE;
+    else if (a == "markesteijn") s.algorithm = IMOPS_DEMOSAIC_MARKESTEIJN;
+    else if (a == "fast" || a == "superfast" || a == "supersuperfast") s.algorithm = -1;
+    else throw Error{IMOPS_E_INVALID, "Invalid Demosaic config"};
+  } else if (s.name == "CFACoeffs") {
+    s.kind = K_CFACOEFFS;
+  } else if (s.name == "Exp") {
+    s.kind = K_EXP;
+    s.ev = num_param(t, "ev", 0.0, "Exp");
+  } else if (s.name == "Gamma") {
+    s.kind = K_GAMMA;
+    s.gamma = num_param(t, "gamma", 2.2, "Gamma");
+  } else if (s.name == "CST") {
+    s.kind = K_CST;
+    s.target = str_param(t, "target_color_space", "", "CST");
+  } else if (s.name == "BM3D") {
+    s.kind = K_BM3D;
+    s.bm3d = bm3d_stage_params_from_table(
+        num_param(t, "intensity", 0.0, "BM3D"), param(t, "sigma") ? num_param(t, "sigma", 0, "BM3D") : -1.0,
+        param(t, "lambda") ? num_param(t, "lambda", 0, "BM3D") : -1.0,
+        (int)num_param(t, "stride", 6, "BM3D"), (int)num_param(t, "search_window", 19, "BM3D"),
+        (int)num_param(t, "max_group", 16, "BM3D"));
+  } else if (s.name == "Film") {
+    s.kind = K_FILM;
+    s.film = film_stage_params_from_table(str_param(t, "stock", "ColorNeg200", "Film"),
+                                          str_param(t, "film_format", "Film35mm", "Film"),
+                                          (uint64_t)num_param(t, "seed", 1, "Film"),
+                                          str_param(t, "output", "NegativeLinear", "Film"));
+  } else {
+    s.kind = K_OTHER;
+  }
+  return s;
+}
+
+bool is_pointwise(const Stage& s) { return s.kind == K_CFACOEFFS || s.kind == K_EXP || s.kind == K_GAMMA || s.kind == K_CST; }
+
+}  // namespace
+
+void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* img) {
+  size_t k = 0;
+  while (k < stages.size()) {
+    const Stage& s = stages[k];
+    if (is_pointwise(s)) {
+      // Fuse the run of elementwise stages, tracking the colour space as the
+      // stages would update it.
+      require_rgb(img, s.name.c_str());
+      PointwiseProgram prog;
+      int cs = img->color_space;
+      for (; k < stages.size() && is_pointwise(stages[k]); ++k) {
+        const Stage& e = stages[k];
+        if (prog.n + 3 > PointwiseProgram::kMax) {  // a stage adds at most 3 ops
+          stage_pointwise(ctx, img, prog);
+          prog = PointwiseProgram();
+        }
+        switch (e.kind) {
+          case K_CFACOEFFS:
+            if (!img->has_wb) throw Error{IMOPS_E_STATE, "wb_coeffs needs to be set to perform highlight_reconstruction"};
+            push_wb(prog, img->wb);
+            break;
+          case K_EXP: push_exp(prog, (float)e.ev); break;
+          case K_GAMMA: push_gamma(prog, (float)e.gamma); break;
+          case K_CST: {
+            const int target = parse_color_space(trim(e.target));
+            if (target < 0) throw Error{IMOPS_E_INVALID, "Color space not recognized: " + e.target};
+            if (cs == IMOPS_CS_NONE) {
+              if (img->calib.empty())
+                throw Error{IMOPS_E_STATE, "A calibration matrix must be set to perform a camera cst transform"};
+              push_camera_cst(prog, img->calib.data(), img->calib.size(), target);
+            } else {
+              push_convert(prog, cs, target);
+            }
+            cs = target;
+            break;
+          }
+          default: break;
+        }
+      }
+      stage_pointwise(ctx, img, prog);
+      img->color_space = cs;
+      continue;
+    }
+    switch (s.kind) {
+      case K_DEMOSAIC:
+        if (s.algorithm < 0) throw Error{IMOPS_E_UNSUPPORTED, "Demosaic preview modes (fast/superfast) are not on the GPU path"};
+        stage_demosaic(ctx, img, s.algorithm);
+        break;
+      case K_BM3D: stage_bm3d(ctx, img, s.bm3d); break;
+      case K_FILM: stage_film(ctx, img, s.film); break;
+      default:
+        throw Error{IMOPS_E_UNSUPPORTED, "stage " + s.name + " is not implemented by the GPU library"};
+    }
+    ++k;
+  }
+}
+
+}  // namespace imops
+
+using namespace imops;
+
+extern "C" {
+
+int imops_pipeline_parse(const char* text, size_t len, imops_pipeline** out) {
+  return guarded([&] {
+    if (!text || !out) throw Error{IMOPS_E_INVALID, "pipeline_parse: null argument"};
+    *out = nullptr;
+    const std::string s(text, len);
+    std::vector<Table> mods;
+    try {
+      mods = parse_toml(s);
+    } catch (const Error& toml_err) {
+      try {
+        mods = parse_json(s);
+      } catch (const Error&) {
+        throw toml_err;
+      }
+    }
+    auto p = std::make_unique<imops_pipeline>();
+    for (const Table& t : mods) p->stages.push_back(make_stage(t));
+    *out = p.release();
+  });
+}
+
+int imops_pipeline_free(imops_pipeline* p) {
+  delete p;
+  return IMOPS_OK;
+}
+
+size_t imops_pipeline_num_stages(const imops_pipeline* p) { return p ? p->stages.size() : 0; }
+
+const char* imops_pipeline_stage_name(const imops_pipeline* p, size_t i) {
+  if (!p || i >= p->stages.size()) return nullptr;
+  return p->stages[i].name.c_str();
+}
+
+int imops_pipeline_run(imops_ctx* ctx, imops_pipeline* p, imops_image* img) {
+  return guarded([&] {
+    if (!ctx || !p || !img) throw Error{IMOPS_E_INVALID, "pipeline_run: null argument"};
+    run_stages(ctx, p->stages, img);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Per-pixel kernel interfaces (pointwise.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace imops {
+
+// Repeating CFA pattern (period up to 12x12); colour at (r, c) for r, c >= 0.
+struct CfaPattern {
+  int w, h;
+  uint8_t base[12][12];
+  __host__ __device__ int at(int r, int c) const { return base[r % h][c % w]; }
+};
+
+enum PointwiseKind : int { PW_MAT3 = 1, PW_SCALE3 = 2, PW_OETF = 3, PW_EOTF = 4, PW_GAMMA = 5 };
+
+struct PointwiseOp {
+  int kind;
+  float m[9];
+};
+
+// A short program of elementwise ops applied in order to every pixel.
+struct PointwiseProgram {
+  static constexpr int kMax = 12;
+  int n = 0;
+  PointwiseOp ops[kMax];
+  bool push(int kind, const float* m, int nm) {
+    if (n >= kMax) return false;
+    ops[n].kind = kind;
+    for (int i = 0; i < 9; ++i) ops[n].m[i] = i < nm ? m[i] : 0.0f;
+    ++n;
+    return true;
+  }
+};
+
+hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_t cy, size_t cw, size_t ch,
+                            float black, float white, float* out, hipStream_t s);
+hipError_t markesteijn_launch(const float* in, int width, int height, const CfaPattern& cfa, float* out,
+                              hipStream_t s);
+hipError_t pointwise_launch(float* rgb, size_t npix, const PointwiseProgram& prog, hipStream_t s);
+
+}  // namespace imops

@@ -1,0 +1,533 @@
+// Host runtime behind the C ABI: contexts, device-resident images and the
+// per-stage entry points.  Errors never cross the boundary as exceptions:
+// each entry point returns a status and records the message for
+// imops_last_error() (the reference's catch_panic + LAST_ERROR,
+// pichromatic-pipeline/src/extern_pipeline.rs:18-58).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "amaze.hpp"
+#include "runtime.hpp"
+
+namespace imops {
+
+namespace {
+thread_local std::string t_last_error;
+}
+
+void set_error(int code, const std::string& msg) {
+  (void)code;
+  t_last_error = msg;
+}
+void clear_error() {}
+
+void DevBuf::reserve(size_t n) {
+  if (n <= bytes) return;
+  release();
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n);
+  if (e != hipSuccess)
+    throw Error{IMOPS_E_HIP, "hipMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};
+  ptr = p;
+  bytes = n;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+void require_rgb(const imops_image* img, const char* stage) {
+  if (!img->has_rgb) throw Error{IMOPS_E_STATE, std::string(stage) + ": image has no RGB data (demosaic first)"};
+}
+
+int parse_color_space(const std::string& name) {
+  // serde_plain names of color::ColorSpaceTag (modules.rs:578-582)
+  if (name == "Srgb") return IMOPS_CS_SRGB;
+  if (name == "LinearSrgb") return IMOPS_CS_LINEAR_SRGB;
+  if (name == "AcesCg") return IMOPS_CS_ACESCG;
+  if (name == "XyzD65") return IMOPS_CS_XYZD65;
+  return -100;
+}
+
+// ---------------------------------------------------------------- colour ops
+namespace {
+
+
+// 3x3 helpers in double for deriving colour matrices on the host.
+static void m3mul(const double a[9], const double b[9], double o[9]) {
+  double t[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) t[i * 3 + j] = a[i * 3] * b[j] + a[i * 3 + 1] * b[3 + j] + a[i * 3 + 2] * b[6 + j];
+  for (int k = 0; k < 9; ++k) o[k] = t[k];
+}
+static void m3inv(const double m[9], double o[9]) {
+  const double c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+  const double id = 1.0 / (m[0] * c00 + m[1] * c01 + m[2] * c02);
+  o[0] = c00 * id; o[1] = (m[2] * m[7] - m[1] * m[8]) * id; o[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+  o[3] = c01 * id; o[4] = (m[0] * m[8] - m[2] * m[6]) * id; o[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+  o[6] = c02 * id; o[7] = (m[1] * m[6] - m[0] * m[7]) * id; o[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+// RGB->XYZ of primaries (xy chromaticities) normalised to white (wx, wy).
+static void rgb_to_xyz(const double xy[6], double wx, double wy, double o[9]) {
+  double p[9];
+  for (int k = 0; k < 3; ++k) {
+    const double x = xy[2 * k], y = xy[2 * k + 1];
+    p[k] = x / y; p[3 + k] = 1.0; p[6 + k] = (1.0 - x - y) / y;
+  }
+  double pi[9], s[3];
+  m3inv(p, pi);
+  const double w[3] = {wx / wy, 1.0, (1.0 - wx - wy) / wy};
+  for (int i = 0; i < 3; ++i) s[i] = pi[i * 3] * w[0] + pi[i * 3 + 1] * w[1] + pi[i * 3 + 2] * w[2];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) o[i * 3 + k] = p[i * 3 + k] * s[k];
+}
+// Bradford adaptation from white (sx, sy) to white (dx, dy).
+static void bradford(double sx, double sy, double dx, double dy, double o[9]) {
+  const double b[9] = {0.8951, 0.2664, -0.1614, -0.7502, 1.7135, 0.0367, 0.0389, -0.0685, 1.0296};
+  double bi[9];
+  m3inv(b, bi);
+  const double ws[3] = {sx / sy, 1.0, (1.0 - sx - sy) / sy}, wd[3] = {dx / dy, 1.0, (1.0 - dx - dy) / dy};
+  double d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 3; ++i) {
+    const double cs = b[i * 3] * ws[0] + b[i * 3 + 1] * ws[1] + b[i * 3 + 2] * ws[2];
+    const double cd = b[i * 3] * wd[0] + b[i * 3 + 1] * wd[1] + b[i * 3 + 2] * wd[2];
+    d[i * 4] = cd / cs;
+  }
+  m3mul(bi, d, o);
+  m3mul(o, b, o);
+}
+// ACEScg (AP1 primaries, ACES white) -> linear sRGB through XYZ(D65), with the
+// Bradford transform between the two whites.
+static void acescg_to_linear_srgb(const double xyz_to_lin[9], double o[9]) {
+  const double ap1[6] = {0.713, 0.293, 0.165, 0.830, 0.128, 0.044};
+  double a2x[9], cat[9];
+  rgb_to_xyz(ap1, 0.32168, 0.33767, a2x);
+  bradford(0.32168, 0.33767, 0.3127, 0.3290, cat);
+  m3mul(cat, a2x, o);
+  m3mul(xyz_to_lin, o, o);
+}
+
+struct ColourMats {
+  float xyz_to_lin[9], lin_to_xyz[9], aces_to_lin[9], lin_to_aces[9];
+  ColourMats() {
+    // CSS Color 4 XYZ(D65) <-> linear sRGB in rational form; ACEScg from the
+    // AP1 primaries and ACES white, Bradford-adapted to D65.  Rounded to f32
+    // once, on the host.
+    const double x2s[9] = {12831.0 / 3959.0, -329.0 / 214.0, -1974.0 / 3959.0,
+                           -851781.0 / 878810.0, 1648619.0 / 878810.0, 36519.0 / 878810.0,
+                           705.0 / 12673.0, -2585.0 / 12673.0, 705.0 / 667.0};
+    const double s2x[9] = {506752.0 / 1228815.0, 87881.0 / 245763.0, 12673.0 / 70218.0,
+                           87098.0 / 409605.0, 175762.0 / 245763.0, 12673.0 / 175545.0,
+                           7918.0 / 409605.0, 87881.0 / 737289.0, 1001167.0 / 1053270.0};
+    double a2s[9], s2a[9];
+    acescg_to_linear_srgb(x2s, a2s);
+    m3inv(a2s, s2a);
+    for (int k = 0; k < 9; ++k) {
+      aces_to_lin[k] = (float)a2s[k];
+      lin_to_aces[k] = (float)s2a[k];
+      xyz_to_lin[k] = (float)x2s[k];
+      lin_to_xyz[k] = (float)s2x[k];
+    }
+  }
+};
+
+const ColourMats& mats() {
+  static ColourMats m;
+  return m;
+}
+
+// cst.rs:51-95 in f32 with the reference's loop order (no pivoting).
+void pseudo_inverse3(const float m[3][3], float out[3][3]) {
+  float work[3][6];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 6; ++j) work[i][j] = (j == i + 3) ? 1.0f : 0.0f;
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) work[i][j] += m[k][i] * m[k][j];
+  }
+  for (int i = 0; i < 3; ++i) {
+    float num = work[i][i];
+    for (int j = 0; j < 6; ++j) work[i][j] /= num;
+    for (int k = 0; k < 3; ++k) {
+      if (k == i) continue;
+      num = work[k][i];
+      for (int j = 0; j < 6; ++j) work[k][j] -= work[i][j] * num;
+    }
+  }
+  float tmp[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      tmp[i][j] = 0.0f;
+      for (int k = 0; k < 3; ++k) tmp[i][j] += work[j][k + 3] * m[i][k];
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[i][j] = tmp[j][i];
+}
+
+}  // namespace
+
+void push_convert(PointwiseProgram& prog, int src, int dst) {
+  if (src == dst) return;
+  const ColourMats& m = mats();
+  bool ok = true;
+  switch (src) {  // src -> linear sRGB
+    case IMOPS_CS_SRGB: ok = prog.push(PW_EOTF, nullptr, 0); break;
+    case IMOPS_CS_LINEAR_SRGB: break;
+    case IMOPS_CS_ACESCG: ok = prog.push(PW_MAT3, m.aces_to_lin, 9); break;
+    case IMOPS_CS_XYZD65: ok = prog.push(PW_MAT3, m.xyz_to_lin, 9); break;
+    default: throw Error{IMOPS_E_INVALID, "unsupported source colour space"};
+  }
+  switch (dst) {  // linear sRGB -> dst
+    case IMOPS_CS_SRGB: ok = ok && prog.push(PW_OETF, nullptr, 0); break;
+    case IMOPS_CS_LINEAR_SRGB: break;
+    case IMOPS_CS_ACESCG: ok = ok && prog.push(PW_MAT3, m.lin_to_aces, 9); break;
+    case IMOPS_CS_XYZD65: ok = ok && prog.push(PW_MAT3, m.lin_to_xyz, 9); break;
+    default: throw Error{IMOPS_E_INVALID, "unsupported target colour space"};
+  }
+  if (!ok) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+}
+
+// cst.rs:12-48
+void push_camera_cst(PointwiseProgram& prog, const float* cal, size_t len, int target) {
+  if (len != 9) throw Error{IMOPS_E_UNSUPPORTED, "camera_cst: calibration matrix must have 3 rows of 3"};
+  const float xyz_d65[3] = {0.95042f, 1.0f, 1.08890f};
+  float rgb_d65[3], mult[3] = {1.0f, 1.0f, 1.0f}, xyz2cam[3][3], fwd[3][3];
+  for (int i = 0; i < 3; ++i)
+    rgb_d65[i] = cal[i * 3] * xyz_d65[0] + cal[i * 3 + 1] * xyz_d65[1] + cal[i * 3 + 2] * xyz_d65[2];
+  for (int i = 0; i < 3; ++i)
+    if (rgb_d65[i] > 0.0f) mult[i] = 1.0f / rgb_d65[i];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) xyz2cam[i][j] = cal[i * 3 + j] * mult[i];
+  pseudo_inverse3(xyz2cam, fwd);
+  if (!prog.push(PW_MAT3, &fwd[0][0], 9)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+  push_convert(prog, IMOPS_CS_XYZD65, target);
+}
+
+void push_exp(PointwiseProgram& prog, float ev) {
+  const float v = powf(2.0f, ev);  // exp.rs:5, evaluated on the host like the reference
+  const float s[3] = {v, v, v};
+  if (!prog.push(PW_SCALE3, s, 3)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+}
+
+void push_gamma(PointwiseProgram& prog, float gamma) {
+  const float inv = 1.0f / std::fmax(gamma, 1e-6f);  // tone_map.rs:39
+  if (!prog.push(PW_GAMMA, &inv, 1)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+}
+
+void push_wb(PointwiseProgram& prog, const float wb[4]) {
+  if (!prog.push(PW_SCALE3, wb, 3)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+}
+
+void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& prog) {
+  IMOPS_HIP_CHECK(pointwise_launch(img->rgb.as<float>(), img->width * img->height, prog, ctx->stream));
+}
+
+void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
+  if (!img->has_raw) throw Error{IMOPS_E_STATE, "Demosaic: image has no raw CFA data"};
+  if (!img->has_cfa) throw Error{IMOPS_E_STATE, "A CFA must be set to demosaic raw images"};
+  const size_t w = img->width, h = img->height;
+  const CfaTable cfa = img->cfa.shift(img->crop_x, img->crop_y);  // demosaic.rs:93-98
+  img->rgb.reserve(w * h * 3 * sizeof(float));
+  if (algorithm == IMOPS_DEMOSAIC_AMAZE) {
+    if (!cfa.is_bayer())
+      throw Error{IMOPS_E_UNSUPPORTED, "AMaZE needs a 2x2 Bayer pattern, got " + cfa.name};
+    const size_t rw = w % 96, rh = h % 96;
+    if (w < 2 || h < 2 || (rw >= 1 && rw <= 15) || (rh >= 1 && rh <= 15))
+      throw Error{IMOPS_E_UNSUPPORTED,
+                  "AMaZE: width and height mod 96 must not be 1..15 (the reference indexes past its tile buffer)"};
+    if (w > (1u << 30) / 3 || h > (1u << 20)) throw Error{IMOPS_E_INVALID, "AMaZE: image too large"};
+    AmazeArgs a{};
+    a.in = img->raw.as<float>();
+    a.out = img->rgb.as<float>();
+    a.width = (int)w;
+    a.height = (int)h;
+    const size_t ntiles = amaze_tiles(a.width, a.height, &a.tiles_x, &a.tiles_y);
+    int grid = ctx->num_cus > 0 ? ctx->num_cus : 256;  // one 144 KiB-LDS workgroup per CU
+    if ((size_t)grid > ntiles) grid = (int)ntiles;
+    ctx->scr[SCR_AMAZE].reserve((size_t)grid * kAmazeSlotFloats * sizeof(float));
+    a.scratch = ctx->scr[SCR_AMAZE].as<float>();
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) a.cfa[r][c] = cfa.pat[r][c];
+    IMOPS_HIP_CHECK(amaze_launch(a, grid, ctx->stream));
+  } else if (algorithm == IMOPS_DEMOSAIC_MARKESTEIJN) {
+    CfaPattern p{};
+    p.w = cfa.width;
+    p.h = cfa.height;
+    for (int r = 0; r < 12; ++r)
+      for (int c = 0; c < 12; ++c) p.base[r][c] = cfa.pat[r][c];
+    IMOPS_HIP_CHECK(markesteijn_launch(img->raw.as<float>(), (int)w, (int)h, p, img->rgb.as<float>(), ctx->stream));
+  } else {
+    throw Error{IMOPS_E_UNSUPPORTED, "demosaic algorithm not on the GPU path (fast/superfast preview modes)"};
+  }
+  img->has_rgb = true;
+  img->color_space = IMOPS_CS_NONE;  // modules.rs:666
+}
+
+}  // namespace imops
+
+using namespace imops;
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* imops_last_error(void) { return t_last_error.c_str(); }
+
+int imops_ctx_create(int device, imops_ctx** out) {
+  return guarded([&] {
+    if (!out) throw Error{IMOPS_E_INVALID, "imops_ctx_create: out is null"};
+    *out = nullptr;
+    int n = 0;
+    IMOPS_HIP_CHECK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw Error{IMOPS_E_INVALID, "imops_ctx_create: no such device"};
+    IMOPS_HIP_CHECK(hipSetDevice(device));
+    auto* c = new imops_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      throw Error{IMOPS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+    }
+    *out = c;
+  });
+}
+
+int imops_ctx_destroy(imops_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : ctx->scr) b.release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int imops_ctx_synchronize(imops_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) throw Error{IMOPS_E_INVALID, "null context"};
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+void* imops_ctx_stream(imops_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int imops_image_create(imops_ctx* ctx, imops_image** out) {
+  return guarded([&] {
+    if (!ctx || !out) throw Error{IMOPS_E_INVALID, "imops_image_create: null argument"};
+    auto* img = new imops_image();
+    img->ctx = ctx;
+    *out = img;
+  });
+}
+
+int imops_image_free(imops_image* img) {
+  return guarded([&] {
+    if (!img) return;
+    (void)hipSetDevice(img->ctx->device);
+    (void)hipStreamSynchronize(img->ctx->stream);
+    img->raw.release();
+    img->rgb.release();
+    img->tmp.release();
+    delete img;
+  });
+}
+
+int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* src, int src_on_device,
+                            size_t full_w, size_t full_h, size_t cx, size_t cy, size_t cw, size_t ch, float black,
+                            float white, const char* cfa_pattern) {
+  return guarded([&] {
+    if (!ctx || !img || !src || !cfa_pattern) throw Error{IMOPS_E_INVALID, "set_raw_u16: null argument"};
+    if (cw == 0 || ch == 0 || cx + cw > full_w || cy + ch > full_h)
+      throw Error{IMOPS_E_INVALID, "set_raw_u16: crop window outside the frame"};
+    if (!(white > black)) throw Error{IMOPS_E_INVALID, "set_raw_u16: white level must exceed black level"};
+    CfaTable cfa;
+    if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
+    const uint16_t* dsrc = src;
+    if (!src_on_device) {
+      ctx->scr[SCR_STAGING].reserve(full_w * full_h * sizeof(uint16_t));
+      IMOPS_HIP_CHECK(hipMemcpyAsync(ctx->scr[SCR_STAGING].ptr, src, full_w * full_h * sizeof(uint16_t),
+                                     hipMemcpyHostToDevice, ctx->stream));
+      dsrc = ctx->scr[SCR_STAGING].as<uint16_t>();
+    }
+    img->raw.reserve(cw * ch * sizeof(float));
+    IMOPS_HIP_CHECK(normalize_launch(dsrc, full_w, cx, cy, cw, ch, black, white, img->raw.as<float>(), ctx->stream));
+    img->width = cw;
+    img->height = ch;
+    img->cfa = cfa;
+    img->has_cfa = true;
+    img->crop_x = cx;
+    img->crop_y = cy;
+    img->has_raw = true;
+    img->has_rgb = false;
+    img->color_space = IMOPS_CS_NONE;
+    if (!src_on_device) IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));  // staging buffer reuse
+  });
+}
+
+int imops_image_set_raw_f32(imops_ctx* ctx, imops_image* img, const float* src, int src_on_device, size_t w,
+                            size_t h, const char* cfa_pattern, size_t crop_x, size_t crop_y) {
+  return guarded([&] {
+    if (!ctx || !img || !src || !cfa_pattern) throw Error{IMOPS_E_INVALID, "set_raw_f32: null argument"};
+    if (w == 0 || h == 0) throw Error{IMOPS_E_INVALID, "set_raw_f32: empty image"};
+    CfaTable cfa;
+    if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
+    img->raw.reserve(w * h * sizeof(float));
+    IMOPS_HIP_CHECK(hipMemcpyAsync(img->raw.ptr, src, w * h * sizeof(float),
+                                   src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    img->width = w;
+    img->height = h;
+    img->cfa = cfa;
+    img->has_cfa = true;
+    img->crop_x = crop_x;
+    img->crop_y = crop_y;
+    img->has_raw = true;
+    img->has_rgb = false;
+    img->color_space = IMOPS_CS_NONE;
+  });
+}
+
+int imops_image_set_rgb_f32(imops_ctx* ctx, imops_image* img, const float* src, int src_on_device, size_t w,
+                            size_t h, int cs) {
+  return guarded([&] {
+    if (!ctx || !img || !src) throw Error{IMOPS_E_INVALID, "set_rgb_f32: null argument"};
+    if (w == 0 || h == 0) throw Error{IMOPS_E_INVALID, "set_rgb_f32: empty image"};
+    if (cs < IMOPS_CS_NONE || cs > IMOPS_CS_XYZD65) throw Error{IMOPS_E_INVALID, "set_rgb_f32: unknown colour space"};
+    img->rgb.reserve(w * h * 3 * sizeof(float));
+    IMOPS_HIP_CHECK(hipMemcpyAsync(img->rgb.ptr, src, w * h * 3 * sizeof(float),
+                                   src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    img->width = w;
+    img->height = h;
+    img->has_rgb = true;
+    img->has_raw = false;
+    img->color_space = cs;
+  });
+}
+
+int imops_image_get_rgb_f32(imops_ctx* ctx, const imops_image* img, float* dst, int dst_on_device, size_t count) {
+  return guarded([&] {
+    if (!ctx || !img || !dst) throw Error{IMOPS_E_INVALID, "get_rgb_f32: null argument"};
+    require_rgb(img, "get_rgb_f32");
+    if (count != img->width * img->height * 3)
+      throw Error{IMOPS_E_INVALID, "get_rgb_f32: count must be width*height*3"};
+    IMOPS_HIP_CHECK(hipMemcpyAsync(dst, img->rgb.ptr, count * sizeof(float),
+                                   dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int imops_image_get_info(const imops_image* img, size_t* w, size_t* h, int* cs, int* has_rgb) {
+  return guarded([&] {
+    if (!img) throw Error{IMOPS_E_INVALID, "get_info: null image"};
+    if (w) *w = img->width;
+    if (h) *h = img->height;
+    if (cs) *cs = img->color_space;
+    if (has_rgb) *has_rgb = img->has_rgb ? 1 : 0;
+  });
+}
+
+int imops_image_rgb_device_ptr(const imops_image* img, float** out) {
+  return guarded([&] {
+    if (!img || !out) throw Error{IMOPS_E_INVALID, "rgb_device_ptr: null argument"};
+    require_rgb(img, "rgb_device_ptr");
+    *out = img->rgb.as<float>();
+  });
+}
+
+int imops_image_set_wb_coeffs(imops_image* img, const float wb[4]) {
+  return guarded([&] {
+    if (!img || !wb) throw Error{IMOPS_E_INVALID, "set_wb_coeffs: null argument"};
+    for (int i = 0; i < 4; ++i) img->wb[i] = wb[i];
+    img->has_wb = true;
+  });
+}
+
+int imops_image_set_calibration_matrix(imops_image* img, const float* m, size_t len) {
+  return guarded([&] {
+    if (!img || !m || len == 0 || len % 3 != 0) throw Error{IMOPS_E_INVALID, "set_calibration_matrix: bad matrix"};
+    img->calib.assign(m, m + len);
+  });
+}
+
+int imops_image_set_shutter_seconds(imops_image* img, float s) {
+  return guarded([&] {
+    if (!img) throw Error{IMOPS_E_INVALID, "set_shutter_seconds: null image"};
+    img->shutter = s;
+    img->has_shutter = true;
+  });
+}
+
+int imops_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "demosaic: null argument"};
+    stage_demosaic(ctx, img, algorithm);
+  });
+}
+
+int imops_cfa_coeffs(imops_ctx* ctx, imops_image* img, const float* wb) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "cfa_coeffs: null argument"};
+    require_rgb(img, "CFACoeffs");
+    if (!wb && !img->has_wb) throw Error{IMOPS_E_STATE, "wb_coeffs needs to be set to perform highlight_reconstruction"};
+    PointwiseProgram p;
+    push_wb(p, wb ? wb : img->wb);
+    stage_pointwise(ctx, img, p);
+  });
+}
+
+int imops_exp(imops_ctx* ctx, imops_image* img, float ev) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "exp: null argument"};
+    require_rgb(img, "Exp");
+    PointwiseProgram p;
+    push_exp(p, ev);
+    stage_pointwise(ctx, img, p);
+  });
+}
+
+int imops_gamma(imops_ctx* ctx, imops_image* img, float gamma) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "gamma: null argument"};
+    require_rgb(img, "Gamma");
+    PointwiseProgram p;
+    push_gamma(p, gamma);
+    stage_pointwise(ctx, img, p);
+  });
+}
+
+int imops_cst(imops_ctx* ctx, imops_image* img, int target) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "cst: null argument"};
+    require_rgb(img, "CST");
+    if (target < IMOPS_CS_SRGB || target > IMOPS_CS_XYZD65) throw Error{IMOPS_E_INVALID, "cst: unknown target"};
+    PointwiseProgram p;
+    if (img->color_space == IMOPS_CS_NONE) {
+      if (img->calib.empty())
+        throw Error{IMOPS_E_STATE, "A calibration matrix must be set to perform a camera cst transform"};
+      push_camera_cst(p, img->calib.data(), img->calib.size(), target);
+    } else {
+      push_convert(p, img->color_space, target);
+    }
+    stage_pointwise(ctx, img, p);
+    img->color_space = target;
+  });
+}
+
+int imops_camera_cst(imops_ctx* ctx, imops_image* img, int target, const float* m, size_t len) {
+  return guarded([&] {
+    if (!ctx || !img || !m) throw Error{IMOPS_E_INVALID, "camera_cst: null argument"};
+    require_rgb(img, "camera_cst");
+    if (target < IMOPS_CS_SRGB || target > IMOPS_CS_XYZD65) throw Error{IMOPS_E_INVALID, "camera_cst: unknown target"};
+    PointwiseProgram p;
+    push_camera_cst(p, m, len, target);
+    stage_pointwise(ctx, img, p);
+    img->color_space = target;
+  });
+}
+
+}  // extern "C"

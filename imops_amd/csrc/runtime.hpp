@@ -1,0 +1,109 @@
+// Internal host-side state behind the C ABI: contexts, device images, error
+// reporting and the shared launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cfa_table.hpp"
+#include "imops_gpu.h"
+#include "pointwise.hpp"
+
+namespace imops {
+
+// Failure carried to the C boundary: status code + message.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_error(int code, const std::string& msg);
+void clear_error();
+
+#define IMOPS_HIP_CHECK(expr)                                                                  \
+  do {                                                                                         \
+    hipError_t e__ = (expr);                                                                   \
+    if (e__ != hipSuccess)                                                                     \
+      throw ::imops::Error{IMOPS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__)};   \
+  } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n);
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace imops
+
+// Scratch buffers a context owns, by use.
+enum ScratchId : int { SCR_AMAZE = 0, SCR_STAGING, SCR_BM3D_0, SCR_BM3D_1, SCR_BM3D_2, SCR_BM3D_3, SCR_FILM_0,
+                       SCR_FILM_1, SCR_FILM_2, SCR_FILM_3, SCR_COUNT };
+
+struct imops_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  imops::DevBuf scr[SCR_COUNT];
+};
+
+struct imops_image {
+  imops_ctx* ctx = nullptr;
+  size_t width = 0, height = 0;
+  bool has_raw = false, has_rgb = false;
+  imops::DevBuf raw;   // f32 CFA plane (width*height)
+  imops::DevBuf rgb;   // f32 interleaved RGB (width*height*3)
+  imops::DevBuf tmp;   // second RGB buffer for out-of-place stages
+  imops::CfaTable cfa;
+  bool has_cfa = false;
+  size_t crop_x = 0, crop_y = 0;
+  float wb[4] = {1, 1, 1, 1};
+  bool has_wb = false;
+  std::vector<float> calib;
+  int color_space = IMOPS_CS_NONE;
+  float shutter = 0.0f;
+  bool has_shutter = false;
+  void swap_rgb_tmp() { std::swap(rgb, tmp); }
+};
+
+namespace imops {
+
+// Stage implementations shared by the per-stage C calls and the pipeline runner.
+void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);
+void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& prog);
+// Append the ops of one colour conversion / stage to a pointwise program.
+void push_convert(PointwiseProgram& prog, int src, int dst);
+void push_camera_cst(PointwiseProgram& prog, const float* calib, size_t len, int target);
+void push_exp(PointwiseProgram& prog, float ev);
+void push_gamma(PointwiseProgram& prog, float gamma);
+void push_wb(PointwiseProgram& prog, const float wb[4]);
+void require_rgb(const imops_image* img, const char* stage);
+int parse_color_space(const std::string& name);
+
+// Run f, converting exceptions into a status code + last error.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    clear_error();
+    return IMOPS_OK;
+  } catch (const Error& e) {
+    set_error(e.code, e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error(IMOPS_E_HIP, "host allocation failed");
+    return IMOPS_E_HIP;
+  } catch (const std::exception& e) {
+    set_error(IMOPS_E_INVALID, e.what());
+    return IMOPS_E_INVALID;
+  } catch (...) {
+    set_error(IMOPS_E_INVALID, "unknown failure");
+    return IMOPS_E_INVALID;
+  }
+}
+
+}  // namespace imops

@@ -1,0 +1,43 @@
+// Parameters and entry points of the non-elementwise stages (BM3D, film).
+#pragma once
+#include <cstdint>
+#include <string>
+
+#include "runtime.hpp"
+
+namespace imops {
+
+// BM3D parameters.  from_intensity reproduces Bm3dParams::from_intensity
+// (pichromatic/src/bm3d.rs:22-32) and the fixed geometry of bm3d.rs:7-12;
+// the explicit fields let a stage list set the geometry and noise model
+// directly (defaults leave the reference behaviour unchanged).
+struct Bm3dStageParams {
+  float intensity = 0.0f;
+  float sigma = 0.08f;          // noise sigma in 0..255 units
+  float lambda = 2.0025f;       // hard-threshold multiplier
+  float max_dist = 3020.0f;     // step-1 match threshold (per-pixel mean SSD)
+  float chroma_scale = 1.8f;    // sigma multiplier for Cb/Cr
+  float detail = 0.0f;          // luma detail blend strength
+  int stride = 6;               // reference-patch grid step
+  int window = 19;              // search window (odd)
+  int max_group = 16;           // group size cap (power of two)
+  int max_candidates = 1024;    // accepted-candidate cap per reference patch
+  static Bm3dStageParams from_intensity(float i);
+};
+
+Bm3dStageParams bm3d_stage_params_from_table(double intensity, double sigma_norm, double lambda, int stride,
+                                             int window, int max_group);
+void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p);
+
+struct FilmStageParams {
+  int stock = 1;    // StockId order of film/stock/mod.rs (0 BwStub, 1 ColorNeg200, ...)
+  int format = 0;   // 0 Film35mm, 1 Film6x6, 2 Film4x5
+  uint64_t seed = 1;
+  int output = 0;   // 0 NegativeLinear, 1 PositiveLinear
+};
+
+FilmStageParams film_stage_params_from_table(const std::string& stock, const std::string& format, uint64_t seed,
+                                             const std::string& output);
+void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p);
+
+}  // namespace imops

@@ -1,0 +1,150 @@
+/*
+ * imops_gpu.h - C ABI of the MI355X (gfx950) image-stage library.
+ *
+ * The library replaces the CPU bodies of the pichromatic stages on the
+ * raw -> demosaic -> BM3D -> colour -> film path.  Its calls mirror the
+ * reference's stage interface:
+ *   - per-stage calls  <-> Module<T>::process (pichromatic-pipeline/src/modules.rs:175-181,
+ *                          stage impls 492-894) and the Image methods they call
+ *                          (pichromatic/src/pixel.rs:55-171);
+ *   - pipeline calls   <-> get_pixel_pipeline_c / run_pixel_pipeline_c / free_pipeline_c
+ *                          (pichromatic-pipeline/src/extern_pipeline.rs:326-359, 434-443)
+ *                          with parse_config (config.rs:13-34) semantics;
+ *   - imops_last_error <-> get_last_error_c (extern_pipeline.rs:27-38).
+ *
+ * Conventions: every call returns IMOPS_OK (0) or a negative IMOPS_E_* code
+ * and never unwinds across the boundary; the message of the last failure on
+ * the calling thread is returned by imops_last_error().  Host buffers belong
+ * to the caller.  Device buffers belong to the library and sit behind opaque
+ * handles with explicit free calls.  A context owns one HIP stream; calls on
+ * one context are serialised on that stream and are not thread-safe against
+ * each other, separate contexts are independent.
+ */
+#ifndef IMOPS_GPU_H
+#define IMOPS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMOPS_OK 0
+#define IMOPS_E_INVALID (-1)     /* bad argument (null pointer, size, unknown name) */
+#define IMOPS_E_HIP (-2)         /* HIP runtime failure */
+#define IMOPS_E_STATE (-3)       /* image not in the state the stage needs */
+#define IMOPS_E_UNSUPPORTED (-4) /* input the reference itself cannot process */
+
+/* Colour spaces (the reference's ColorSpaceTag subset: modules.rs:107-112). */
+#define IMOPS_CS_NONE (-1) /* camera RGB straight out of demosaic (color_space = None) */
+#define IMOPS_CS_SRGB 0
+#define IMOPS_CS_LINEAR_SRGB 1
+#define IMOPS_CS_ACESCG 2
+#define IMOPS_CS_XYZD65 3
+
+/* Demosaic algorithms (DemosaicAlgorithmType, modules.rs:73-100). */
+#define IMOPS_DEMOSAIC_AMAZE 0
+#define IMOPS_DEMOSAIC_MARKESTEIJN 1
+
+typedef struct imops_ctx imops_ctx;
+typedef struct imops_image imops_image;
+typedef struct imops_pipeline imops_pipeline;
+
+/* Message of the last failed call on this thread ("" if none).  Valid until the
+ * next failing call on the same thread.  Replaces get_last_error_c. */
+const char* imops_last_error(void);
+
+/* Context: device ordinal, one stream, cached tables and scratch. */
+int imops_ctx_create(int device, imops_ctx** out);
+int imops_ctx_destroy(imops_ctx* ctx);
+int imops_ctx_synchronize(imops_ctx* ctx);
+/* The hipStream_t every call of this context is enqueued on. */
+void* imops_ctx_stream(imops_ctx* ctx);
+
+/* Images: device-resident replacement of pichromatic::pixel::Image
+ * (pixel.rs:22-27, image.rs:7-43). */
+int imops_image_create(imops_ctx* ctx, imops_image** out);
+int imops_image_free(imops_image* img);
+
+/* Raw ingest: u16 sensor frame -> normalised f32 CFA plane of the crop window,
+ * value (DN - black) / (white - black).  `src` is a host pointer, or a device
+ * pointer when src_on_device != 0.  `cfa_pattern` is the sensor pattern string
+ * (cfa.rs:98-131); the crop origin shifts it as get_cfa does (demosaic.rs:93-98).
+ * Replaces the crop_and_normalize step of parse_raw_image (extern_pipeline.rs:239-324,
+ * demosaic.rs:61-91). */
+int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* src, int src_on_device,
+                            size_t full_width, size_t full_height, size_t crop_x, size_t crop_y,
+                            size_t crop_width, size_t crop_height, float black, float white,
+                            const char* cfa_pattern);
+/* An already normalised CFA plane (Image.raw_data). */
+int imops_image_set_raw_f32(imops_ctx* ctx, imops_image* img, const float* src, int src_on_device, size_t width,
+                            size_t height, const char* cfa_pattern, size_t crop_x, size_t crop_y);
+/* Interleaved RGB f32 (Image.rgb_data) in colour space `color_space`. */
+int imops_image_set_rgb_f32(imops_ctx* ctx, imops_image* img, const float* src, int src_on_device, size_t width,
+                            size_t height, int color_space);
+/* Copy the interleaved RGB f32 out (count = width*height*3 floats). */
+int imops_image_get_rgb_f32(imops_ctx* ctx, const imops_image* img, float* dst, int dst_on_device, size_t count);
+int imops_image_get_info(const imops_image* img, size_t* width, size_t* height, int* color_space, int* has_rgb);
+/* Device address of the interleaved RGB buffer (valid until the next stage call). */
+int imops_image_rgb_device_ptr(const imops_image* img, float** out);
+
+/* Metadata used by the stages (ImageMetadata fields wb_coeffs,
+ * calibration_matrix_d65, shutter_seconds). */
+int imops_image_set_wb_coeffs(imops_image* img, const float wb[4]);
+int imops_image_set_calibration_matrix(imops_image* img, const float* m, size_t len);
+int imops_image_set_shutter_seconds(imops_image* img, float seconds);
+
+/* Stages ---------------------------------------------------------------- */
+
+/* Module<Demosaic>::process (modules.rs:629-668): raw plane -> RGB, colour
+ * space reset to None.  AMaZE (demosaic.rs:358-1379) needs a Bayer pattern and
+ * rejects sizes where a dimension mod 96 is 1..15 (the reference indexes past
+ * its tile there); markesteijn (demosaic.rs:108-186) takes any pattern. */
+int imops_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);
+
+/* Module<CFACoeffs>::process (modules.rs:533-540, cfa_coeffs.rs:4-12);
+ * wb = NULL uses the image's wb_coeffs. */
+int imops_cfa_coeffs(imops_ctx* ctx, imops_image* img, const float* wb);
+
+/* Module<Exp>::process (modules.rs:308-311, exp.rs:4-11). */
+int imops_exp(imops_ctx* ctx, imops_image* img, float ev);
+
+/* Module<Gamma>::process (modules.rs:431-433, tone_map.rs:38-49). */
+int imops_gamma(imops_ctx* ctx, imops_image* img, float gamma);
+
+/* Module<CST>::process (modules.rs:576-592): camera_cst with the image's
+ * calibration matrix when the colour space is None (cst.rs:12-48), else
+ * cst (cst.rs:6-10). */
+int imops_cst(imops_ctx* ctx, imops_image* img, int target_color_space);
+
+/* Image::camera_cst with an explicit XYZ->camera matrix (3 rows, cst.rs:12-48). */
+int imops_camera_cst(imops_ctx* ctx, imops_image* img, int target_color_space, const float* calibration_matrix,
+                     size_t len);
+
+/* Pipeline --------------------------------------------------------------- */
+
+/* Parse a TOML or JSON stage list ([[pipeline_modules]] tables, names matched
+ * case-insensitively; config.rs:13-34).  Replaces get_pixel_pipeline_c. */
+int imops_pipeline_parse(const char* text, size_t len, imops_pipeline** out);
+int imops_pipeline_free(imops_pipeline* p);
+size_t imops_pipeline_num_stages(const imops_pipeline* p);
+/* Canonical stage name ("Demosaic", "CST", ...) of stage i, NULL if out of range. */
+const char* imops_pipeline_stage_name(const imops_pipeline* p, size_t i);
+/* Run the stages in order on the image (run_pixel_pipeline, pipeline.rs:4-13). */
+int imops_pipeline_run(imops_ctx* ctx, imops_pipeline* p, imops_image* img);
+
+/* Synthetic inputs --------------------------------------------------------- */
+
+/* Seeded scene in linear RGB [0,1] (gradients + 40 rectangles + sinusoids). */
+int imops_synth_scene_rgb(float* out, size_t width, size_t height, uint64_t frame);
+/* The same scene through a CFA with Poisson shot noise (full_well e-) and
+ * Gaussian read noise (read_sigma DN), quantised to u16 in [0, white]. */
+int imops_synth_raw_u16(uint16_t* out, size_t width, size_t height, const char* cfa_pattern, uint64_t frame,
+                        float full_well, float black, float white, float read_sigma);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMOPS_GPU_H */

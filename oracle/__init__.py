@@ -1,0 +1,135 @@
+"""Oracle: CPU restatement of the reference algorithms (test infrastructure).
+
+Only the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` load this package, and only as the checker / the timed CPU
+baseline.  The product library never links or calls it.
+
+Functions and the reference code they restate:
+
+* ``amaze``        pichromatic/src/demosaic.rs:358-1379 (f32 exact order; fp64 variant for error statements)
+* ``markesteijn``  pichromatic/src/demosaic.rs:108-186
+* ``normalize_raw`` demosaic.rs:61-91 + (DN-black)/(white-black) scaling (rawler apply_scaling: parity unpinned)
+* ``camera_cst``   pichromatic/src/cst.rs:12-95 (colour-crate matrices: parity unpinned)
+* ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+
+_fp = C.POINTER(C.c_float)
+_sz = C.c_size_t
+_lib = None
+
+_SIGS = {
+    "oracle_amaze": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp, C.c_int]),
+    "oracle_markesteijn": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp]),
+    "oracle_normalize_raw": (C.c_int, [C.c_void_p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_float, C.c_float, _fp]),
+    "oracle_camera_forward_matrix": (C.c_int, [_fp, _sz, _fp]),
+    "oracle_camera_cst": (C.c_int, [_fp, _sz, _fp, _sz, C.c_int]),
+    "oracle_cst": (C.c_int, [_fp, _sz, C.c_int, C.c_int]),
+    "oracle_exp": (C.c_int, [_fp, _sz, C.c_float]),
+    "oracle_gamma": (C.c_int, [_fp, _sz, C.c_float]),
+    "oracle_cfa_coeffs": (C.c_int, [_fp, _sz, _fp]),
+    "oracle_srgb_oetf": (C.c_float, [C.c_float]),
+}
+
+CS_SRGB, CS_LINEAR_SRGB, CS_ACESCG, CS_XYZD65 = 0, 1, 2, 3
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            if hasattr(h, name):
+                fn = getattr(h, name)
+                fn.restype = res
+                fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _f(a):
+    return a.ctypes.data_as(_fp)
+
+
+def _rc(code, what):
+    if code != 0:
+        raise ValueError(f"oracle {what} failed ({code})")
+
+
+def normalize_raw(raw: np.ndarray, black: float, white: float, crop=None) -> np.ndarray:
+    raw = np.ascontiguousarray(raw, np.uint16)
+    h, w = raw.shape
+    cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
+    out = np.empty((ch, cw), np.float32)
+    _rc(lib().oracle_normalize_raw(raw.ctypes.data, w, h, cx, cy, cw, ch, black, white, _f(out)), "normalize_raw")
+    return out
+
+
+def amaze(cfa_plane: np.ndarray, pattern: str = "RGGB", crop_origin=(0, 0), fp64: bool = False) -> np.ndarray:
+    x = np.ascontiguousarray(cfa_plane, np.float32)
+    h, w = x.shape
+    out = np.zeros((h, w, 3), np.float32)
+    _rc(lib().oracle_amaze(_f(x), w, h, pattern.encode(), crop_origin[0], crop_origin[1], _f(out), int(fp64)),
+        "amaze")
+    return out
+
+
+def markesteijn(cfa_plane: np.ndarray, pattern: str, crop_origin=(0, 0)) -> np.ndarray:
+    x = np.ascontiguousarray(cfa_plane, np.float32)
+    h, w = x.shape
+    out = np.zeros((h, w, 3), np.float32)
+    _rc(lib().oracle_markesteijn(_f(x), w, h, pattern.encode(), crop_origin[0], crop_origin[1], _f(out)),
+        "markesteijn")
+    return out
+
+
+def camera_forward_matrix(cal) -> np.ndarray:
+    c = np.ascontiguousarray(cal, np.float32).ravel()
+    out = np.empty(9, np.float32)
+    _rc(lib().oracle_camera_forward_matrix(_f(c), c.size, _f(out)), "camera_forward_matrix")
+    return out.reshape(3, 3)
+
+
+def camera_cst(rgb: np.ndarray, cal, target: int) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    c = np.ascontiguousarray(cal, np.float32).ravel()
+    _rc(lib().oracle_camera_cst(_f(out), out.size // 3, _f(c), c.size, target), "camera_cst")
+    return out
+
+
+def cst(rgb: np.ndarray, src: int, dst: int) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    _rc(lib().oracle_cst(_f(out), out.size // 3, src, dst), "cst")
+    return out
+
+
+def exp(rgb: np.ndarray, ev: float) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    _rc(lib().oracle_exp(_f(out), out.size // 3, ev), "exp")
+    return out
+
+
+def gamma(rgb: np.ndarray, g: float) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    _rc(lib().oracle_gamma(_f(out), out.size // 3, g), "gamma")
+    return out
+
+
+def cfa_coeffs(rgb: np.ndarray, wb) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    w = np.asarray(wb, np.float32)
+    _rc(lib().oracle_cfa_coeffs(_f(out), out.size // 3, _f(w)), "cfa_coeffs")
+    return out
+
+
+def srgb_oetf(x: float) -> float:
+    return lib().oracle_srgb_oetf(x)
