@@ -27,7 +27,7 @@ from ._lib import (CS_ACESCG, CS_LINEAR_SRGB, CS_NONE, CS_SRGB, CS_XYZD65, DEMOS
 __all__ = [
     "Context", "Image", "PipelineConfig", "parse_config", "run_pixel_pipeline", "ImopsError", "synth_raw_u16",
     "synth_scene_rgb", "CS_NONE", "CS_SRGB", "CS_LINEAR_SRGB", "CS_ACESCG", "CS_XYZD65", "DEMOSAIC_AMAZE",
-    "DEMOSAIC_MARKESTEIJN", "COLOR_SPACES",
+    "DEMOSAIC_MARKESTEIJN", "COLOR_SPACES", "bm3d_params",
 ]
 
 # ColorSpaceTag names accepted by CST (modules.rs:107-112, serde_plain names)
@@ -193,6 +193,16 @@ class Image:
         check(lib().imops_cst(self.ctx.handle, self._h, _cs(target)))
         return self
 
+    def bm3d(self, intensity: float = 0.0, **explicit) -> "Image":
+        """BM3D (bm3d.rs:34-61).  Keyword overrides: sigma (0..255 units), lambda_, max_dist,
+        chroma_scale, detail, stride, search_window, max_group, max_candidates."""
+        if not explicit:
+            check(lib().imops_bm3d(self.ctx.handle, self._h, float(intensity)))
+            return self
+        p = bm3d_params(intensity, **explicit)
+        check(lib().imops_bm3d_ex(self.ctx.handle, self._h, C.byref(p)))
+        return self
+
     def camera_cst(self, target, calibration_matrix: Sequence[float]) -> "Image":
         a = np.ascontiguousarray(calibration_matrix, np.float32).ravel()
         check(lib().imops_camera_cst(self.ctx.handle, self._h, _cs(target), fptr(a), a.size))
@@ -219,6 +229,15 @@ class PipelineConfig:
                 lib().imops_pipeline_free(self._h)
         except Exception:
             pass
+
+
+def bm3d_params(intensity: float = 0.0, **explicit) -> _lib.Bm3dParams:
+    """Bm3dParams::from_intensity (bm3d.rs:22-32) with optional explicit overrides."""
+    p = _lib.Bm3dParams()
+    check(lib().imops_bm3d_default_params(float(intensity), C.byref(p)))
+    for k, v in explicit.items():
+        setattr(p, "lambda_" if k == "lambda" else k, v)
+    return p
 
 
 def parse_config(text: str) -> PipelineConfig:

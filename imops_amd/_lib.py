@@ -31,6 +31,13 @@ _i = C.c_int
 _f = C.c_float
 _fp = C.POINTER(C.c_float)
 
+class Bm3dParams(C.Structure):
+    """imops_bm3d_params (include/imops_gpu.h)."""
+    _fields_ = [("sigma", C.c_float), ("lambda_", C.c_float), ("max_dist", C.c_float), ("chroma_scale", C.c_float),
+                ("detail", C.c_float), ("stride", C.c_int), ("search_window", C.c_int), ("max_group", C.c_int),
+                ("max_candidates", C.c_int)]
+
+
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
     "imops_last_error": (C.c_char_p, []),
@@ -55,6 +62,9 @@ SIGNATURES = {
     "imops_gamma": (_i, [_vp, _vp, _f]),
     "imops_cst": (_i, [_vp, _vp, _i]),
     "imops_camera_cst": (_i, [_vp, _vp, _i, _fp, _sz]),
+    "imops_bm3d_default_params": (_i, [_f, C.POINTER(Bm3dParams)]),
+    "imops_bm3d": (_i, [_vp, _vp, _f]),
+    "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
     "imops_pipeline_num_stages": (_sz, [_vp]),

@@ -1,0 +1,53 @@
+// BM3D kernel interfaces (bm3d.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace imops {
+
+constexpr int kBm3dMaxWindow = 41;  // search windows up to 41x41 (odd)
+constexpr int kBm3dMaxGroup = 16;
+
+struct Bm3dMatchArgs {
+  const float* guide[3];
+  int width, height;
+  int refs_x, stride, window;
+  float threshold;        // per-pixel mean SSD limit of this step
+  int max_candidates, max_group;
+  uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
+};
+
+struct Bm3dFilterArgs {
+  const float* guide[3];
+  const float* noisy[3];
+  int width, height;
+  int refs_x, refs_y, stride, window, tile;
+  float sigma, chroma_scale, lambda;
+  const uint32_t* groups;
+  unsigned long long* num;  // 3 planes of i64 (two's complement in u64)
+  unsigned long long* den;
+  const float* dct;
+  const float* idct;
+  const float* kaiser;
+};
+
+struct Bm3dBlurTaps {
+  int radius, len;
+  float k[33];
+};
+
+size_t bm3d_match_smem();
+size_t bm3d_filter_smem(int footprint_pixels);
+hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s);
+hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s);
+hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s);
+hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],
+                                float* const out[3], size_t n, hipStream_t s);
+hipError_t fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
+hipError_t bm3d_blur_launch(const float* in, float* tmp, float* out, int w, int h, const Bm3dBlurTaps& taps,
+                            hipStream_t s);
+hipError_t bm3d_merge_launch(const float* yd, const float* cb, const float* cr, const float* yorig, const float* yblur,
+                             float detail, float* rgb, size_t n, hipStream_t s);
+
+}  // namespace imops

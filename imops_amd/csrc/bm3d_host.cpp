@@ -1,0 +1,245 @@
+// Host side of the BM3D stage: parameters (Bm3dParams::from_intensity,
+// pichromatic/src/bm3d.rs:22-32), the f32 DCT/IDCT/Kaiser tables
+// (DctTables::new, bm3d.rs:417-461) and the Gaussian taps of the detail blend
+// (bm3d.rs:648-660) computed with the host libm exactly as the reference
+// computes them, and the launch sequence of bm3d() (bm3d.rs:34-61).
+#include <cmath>
+#include <string>
+
+#include "bm3d.hpp"
+#include "stages.hpp"
+
+namespace imops {
+
+Bm3dStageParams Bm3dStageParams::from_intensity(float i) {
+  Bm3dStageParams p;
+  const float val = i < 0.001f ? 0.001f : (i > 1.0f ? 1.0f : i);
+  p.intensity = i;
+  p.sigma = val * 80.0f;
+  p.lambda = 2.0f + (val * 2.5f);
+  p.max_dist = 3000.0f + (val * 20000.0f);
+  p.chroma_scale = 1.8f;
+  const float d = i * 0.5f;
+  p.detail = d < 0.0f ? 0.0f : (d > 0.5f ? 0.5f : d);
+  return p;
+}
+
+// Stage-list fields: `intensity` (the reference's only knob) plus optional
+// explicit `sigma` (normalised to 0..1), `lambda`, `stride`, `search_window`
+// and `max_group`.  With sigma set and no intensity, the intensity that maps
+// to that sigma (sigma*255/80) drives the match threshold and detail blend.
+Bm3dStageParams bm3d_stage_params_from_table(double intensity, double sigma_norm, double lambda, int stride,
+                                             int window, int max_group) {
+  float i = (float)intensity;
+  if (sigma_norm >= 0.0 && intensity == 0.0) i = (float)(sigma_norm * 255.0 / 80.0);
+  Bm3dStageParams p = Bm3dStageParams::from_intensity(i);
+  if (sigma_norm >= 0.0) p.sigma = (float)(sigma_norm * 255.0);
+  if (lambda >= 0.0) p.lambda = (float)lambda;
+  p.stride = stride;
+  p.window = window;
+  p.max_group = max_group;
+  if (stride < 1 || window < 1 || (window & 1) == 0 || window > kBm3dMaxWindow || max_group < 1 ||
+      max_group > kBm3dMaxGroup)
+    throw Error{IMOPS_E_INVALID, "Invalid BM3D config: stride >= 1, odd search_window <= 41, max_group 1..16"};
+  return p;
+}
+
+namespace {
+
+struct HostTables {
+  float dct[64], idct[64], kaiser[64];
+  HostTables() {
+    const float pi = 3.14159265358979323846f;  // std::f32::consts::PI
+    for (int k = 0; k < 8; ++k)
+      for (int n = 0; n < 8; ++n) {
+        const float c = (float)k * pi / 8.0f;
+        const float val = cosf(((float)n + 0.5f) * c);
+        dct[k * 8 + n] = val * (k == 0 ? 0.35355339f : 0.5f);
+      }
+    for (int n = 0; n < 8; ++n)
+      for (int k = 0; k < 8; ++k) {
+        const float theta = (pi / 8.0f) * ((float)n + 0.5f) * (float)k;
+        idct[n * 8 + k] = (k == 0 ? 0.35355339f : 0.5f) * cosf(theta);
+      }
+    auto i0 = [](float x) {
+      float sum = 1.0f, term = 1.0f;
+      for (int k = 1; k <= 10; ++k) {
+        const float t = x / 2.0f / (float)k;
+        term *= t * t;
+        sum += term;
+      }
+      return sum;
+    };
+    const float beta = 2.0f, i0b = i0(beta);
+    for (int y = 0; y < 8; ++y)
+      for (int x = 0; x < 8; ++x) {
+        const float ux = (2.0f * (float)x / 7.0f) - 1.0f, uy = (2.0f * (float)y / 7.0f) - 1.0f;
+        kaiser[y * 8 + x] = (i0(beta * sqrtf(1.0f - ux * ux)) / i0b) * (i0(beta * sqrtf(1.0f - uy * uy)) / i0b);
+      }
+  }
+};
+
+Bm3dBlurTaps blur_taps(float sigma) {
+  Bm3dBlurTaps t{};
+  t.radius = (int)ceilf(3.0f * sigma);
+  t.len = 2 * t.radius + 1;
+  const float two_s2 = 2.0f * sigma * sigma;
+  float sum = 0.0f;
+  for (int i = 0; i < t.len; ++i) {
+    const float k = (float)i - (float)t.radius;
+    t.k[i] = expf(-k * k / two_s2);
+  }
+  for (int i = 0; i < t.len; ++i) sum += t.k[i];
+  for (int i = 0; i < t.len; ++i) t.k[i] /= sum;
+  return t;
+}
+
+}  // namespace
+
+void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
+  require_rgb(img, "BM3D");
+  const int w = (int)img->width, h = (int)img->height;
+  const size_t n = (size_t)w * h;
+  hipStream_t s = ctx->stream;
+
+  // scratch: 3 noisy planes, original Y, 3 basic planes, 3 final planes, blur tmp/out,
+  // i64 num/den (3 planes each), group table, device tables
+  const int refs_x = w > 8 ? (w - 8 + p.stride - 1) / p.stride : 0;
+  const int refs_y = h > 8 ? (h - 8 + p.stride - 1) / p.stride : 0;
+  const size_t nrefs = (size_t)refs_x * refs_y;
+  ctx->scr[SCR_BM3D_0].reserve(12 * n * sizeof(float));
+  ctx->scr[SCR_BM3D_1].reserve(6 * n * sizeof(unsigned long long));
+  ctx->scr[SCR_BM3D_2].reserve((nrefs ? nrefs : 1) * 17 * sizeof(uint32_t));
+  ctx->scr[SCR_BM3D_3].reserve(3 * 64 * sizeof(float));
+  float* f = ctx->scr[SCR_BM3D_0].as<float>();
+  float* noisy[3] = {f, f + n, f + 2 * n};
+  float* yorig = f + 3 * n;
+  float* basic[3] = {f + 4 * n, f + 5 * n, f + 6 * n};
+  float* fin[3] = {f + 7 * n, f + 8 * n, f + 9 * n};
+  float* btmp = f + 10 * n;
+  float* bout = f + 11 * n;
+  unsigned long long* num = ctx->scr[SCR_BM3D_1].as<unsigned long long>();
+  unsigned long long* den = num + 3 * n;
+  uint32_t* groups = ctx->scr[SCR_BM3D_2].as<uint32_t>();
+  float* dtab = ctx->scr[SCR_BM3D_3].as<float>();
+
+  static const HostTables tables;
+  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab, tables.dct, 64 * sizeof(float), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab + 64, tables.idct, 64 * sizeof(float), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab + 128, tables.kaiser, 64 * sizeof(float), hipMemcpyHostToDevice, s));
+
+  IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
+
+  // workgroup tile of reference patches: largest whose LDS footprint fits 2 groups per CU
+  const int half = p.window / 2;
+  int tile = 8;
+  for (; tile > 1; tile /= 2) {
+    const int span = (tile - 1) * p.stride + 2 * half + 8;
+    if (bm3d_filter_smem(span * span) <= 80 * 1024) break;
+  }
+  const int span = (tile - 1) * p.stride + 2 * half + 8;
+  const size_t fsmem = bm3d_filter_smem(span * span);
+  const int tiles = ((refs_x + tile - 1) / tile) * ((refs_y + tile - 1) / tile);
+
+  auto run_step = [&](const float* const guide[3], bool step1, float* const out[3]) {
+    IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s));
+    if (nrefs > 0) {
+      Bm3dMatchArgs m{};
+      for (int c = 0; c < 3; ++c) m.guide[c] = guide[c];
+      m.width = w;
+      m.height = h;
+      m.refs_x = refs_x;
+      m.stride = p.stride;
+      m.window = p.window;
+      m.threshold = step1 ? p.max_dist : p.max_dist * 0.5f;
+      m.max_candidates = p.max_candidates;
+      m.max_group = p.max_group;
+      m.groups = groups;
+      IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
+      Bm3dFilterArgs fa{};
+      for (int c = 0; c < 3; ++c) {
+        fa.guide[c] = guide[c];
+        fa.noisy[c] = noisy[c];
+      }
+      fa.width = w;
+      fa.height = h;
+      fa.refs_x = refs_x;
+      fa.refs_y = refs_y;
+      fa.stride = p.stride;
+      fa.window = p.window;
+      fa.tile = tile;
+      fa.sigma = p.sigma;
+      fa.chroma_scale = p.chroma_scale;
+      fa.lambda = p.lambda;
+      fa.groups = groups;
+      fa.num = num;
+      fa.den = den;
+      fa.dct = dtab;
+      fa.idct = dtab + 64;
+      fa.kaiser = dtab + 128;
+      IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
+    }
+    const float* nz[3] = {noisy[0], noisy[1], noisy[2]};
+    IMOPS_HIP_CHECK(bm3d_finalize_launch(num, den, nz, out, n, s));
+  };
+  const float* g1[3] = {noisy[0], noisy[1], noisy[2]};
+  run_step(g1, true, basic);
+  const float* g2[3] = {basic[0], basic[1], basic[2]};
+  run_step(g2, false, fin);
+
+  const Bm3dBlurTaps taps = blur_taps(3.0f);
+  IMOPS_HIP_CHECK(bm3d_blur_launch(yorig, btmp, bout, w, h, taps, s));
+  IMOPS_HIP_CHECK(bm3d_merge_launch(fin[0], fin[1], fin[2], yorig, bout, p.detail, img->rgb.as<float>(), n, s));
+}
+
+}  // namespace imops
+
+using namespace imops;
+
+extern "C" {
+
+int imops_bm3d_default_params(float intensity, imops_bm3d_params* out) {
+  return guarded([&] {
+    if (!out) throw Error{IMOPS_E_INVALID, "bm3d_default_params: null argument"};
+    const Bm3dStageParams p = Bm3dStageParams::from_intensity(intensity);
+    out->sigma = p.sigma;
+    out->lambda = p.lambda;
+    out->max_dist = p.max_dist;
+    out->chroma_scale = p.chroma_scale;
+    out->detail = p.detail;
+    out->stride = p.stride;
+    out->search_window = p.window;
+    out->max_group = p.max_group;
+    out->max_candidates = p.max_candidates;
+  });
+}
+
+int imops_bm3d(imops_ctx* ctx, imops_image* img, float intensity) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "bm3d: null argument"};
+    stage_bm3d(ctx, img, Bm3dStageParams::from_intensity(intensity));
+  });
+}
+
+int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* q) {
+  return guarded([&] {
+    if (!ctx || !img || !q) throw Error{IMOPS_E_INVALID, "bm3d_ex: null argument"};
+    Bm3dStageParams p;
+    p.sigma = q->sigma;
+    p.lambda = q->lambda;
+    p.max_dist = q->max_dist;
+    p.chroma_scale = q->chroma_scale;
+    p.detail = q->detail;
+    p.stride = q->stride;
+    p.window = q->search_window;
+    p.max_group = q->max_group;
+    p.max_candidates = q->max_candidates;
+    if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.window > kBm3dMaxWindow || p.max_group < 1 ||
+        p.max_group > kBm3dMaxGroup || p.max_candidates < 1)
+      throw Error{IMOPS_E_INVALID, "bm3d_ex: stride >= 1, odd search_window <= 41, max_group 1..16, max_candidates >= 1"};
+    stage_bm3d(ctx, img, p);
+  });
+}
+
+}  // extern "C"

@@ -122,6 +122,27 @@ int imops_cst(imops_ctx* ctx, imops_image* img, int target_color_space);
 int imops_camera_cst(imops_ctx* ctx, imops_image* img, int target_color_space, const float* calibration_matrix,
                      size_t len);
 
+/* BM3D (pichromatic/src/bm3d.rs:34-61) in plain FP32, bit-exact with the
+ * reference's arithmetic.  imops_bm3d takes the reference's single knob
+ * (Module<BM3D>, modules.rs:506-510; Bm3dParams::from_intensity, bm3d.rs:22-32);
+ * imops_bm3d_ex takes every parameter explicitly. */
+typedef struct imops_bm3d_params {
+  float sigma;          /* noise sigma in 0..255 units (reference: 80*clamp(intensity)) */
+  float lambda;         /* hard-threshold multiplier (2 + 2.5*clamp(intensity)) */
+  float max_dist;       /* step-1 mean-SSD match limit (3000 + 20000*clamp(intensity)); step 2 uses half */
+  float chroma_scale;   /* sigma multiplier for Cb/Cr (1.8) */
+  float detail;         /* luma detail blend strength (clamp(0.5*intensity, 0, 0.5)) */
+  int stride;           /* reference-patch grid step (6) */
+  int search_window;    /* odd, <= 41 (19) */
+  int max_group;        /* 1..16, groups are the largest power of two <= min(this, matches) (16) */
+  int max_candidates;   /* accepted-candidate cap in scan order (1024) */
+} imops_bm3d_params;
+
+/* Fill `out` with the parameters intensity maps to (from_intensity). */
+int imops_bm3d_default_params(float intensity, imops_bm3d_params* out);
+int imops_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
+int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* params);
+
 /* Pipeline --------------------------------------------------------------- */
 
 /* Parse a TOML or JSON stage list ([[pipeline_modules]] tables, names matched

@@ -11,6 +11,8 @@ Functions and the reference code they restate:
 * ``normalize_raw`` demosaic.rs:61-91 + (DN-black)/(white-black) scaling (rawler apply_scaling: parity unpinned)
 * ``camera_cst``   pichromatic/src/cst.rs:12-95 (colour-crate matrices: parity unpinned)
 * ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
+* ``bm3d``         pichromatic/src/bm3d.rs:1-699 (f32 exact order; fp64 variant); ``chroma_bm3d`` bm3d.rs:705-759
+* ``bm3d_test_image`` / ``refined_sum``  the fixture and checksum of bm3d.rs:765-789
 """
 from __future__ import annotations
 
@@ -36,6 +38,12 @@ _SIGS = {
     "oracle_gamma": (C.c_int, [_fp, _sz, C.c_float]),
     "oracle_cfa_coeffs": (C.c_int, [_fp, _sz, _fp]),
     "oracle_srgb_oetf": (C.c_float, [C.c_float]),
+    "oracle_bm3d": (C.c_int, [_fp, _sz, _sz, _fp, C.c_int, C.POINTER(C.c_longlong)]),
+    "oracle_bm3d_intensity_params": (C.c_int, [C.c_float, _fp]),
+    "oracle_chroma_bm3d": (C.c_int, [_fp, _sz, _sz, C.c_float]),
+    "oracle_bm3d_tables": (C.c_int, [_fp, _fp, _fp]),
+    "oracle_bm3d_test_image": (C.c_int, [_fp, _sz, _sz]),
+    "oracle_refined_sum": (C.c_double, [_fp, _sz, C.c_int]),
 }
 
 CS_SRGB, CS_LINEAR_SRGB, CS_ACESCG, CS_XYZD65 = 0, 1, 2, 3
@@ -133,3 +141,44 @@ def cfa_coeffs(rgb: np.ndarray, wb) -> np.ndarray:
 
 def srgb_oetf(x: float) -> float:
     return lib().oracle_srgb_oetf(x)
+
+
+BM3D_FIELDS = ("intensity", "sigma", "lambda", "max_dist", "chroma_scale", "detail", "stride", "window",
+               "max_group", "max_candidates")
+
+
+def bm3d_params(intensity: float, **overrides) -> dict:
+    """Bm3dParams::from_intensity (bm3d.rs:22-32) plus detail strength, as a dict."""
+    v = np.empty(10, np.float32)
+    lib().oracle_bm3d_intensity_params(intensity, _f(v))
+    p = dict(zip(BM3D_FIELDS, [float(x) for x in v]))
+    p.update(overrides)
+    return p
+
+
+def bm3d(rgb: np.ndarray, params: dict, fp64: bool = False):
+    """BM3D of an (H, W, 3) f32 image in [0,1]; returns (output, (n_refs, n_grouped_patches))."""
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    v = np.array([params[k] for k in BM3D_FIELDS], np.float32)
+    st = (C.c_longlong * 2)()
+    _rc(lib().oracle_bm3d(_f(out), w, h, _f(v), int(fp64), st), "bm3d")
+    return out, (st[0], st[1])
+
+
+def chroma_bm3d(rgb: np.ndarray, intensity: float) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    _rc(lib().oracle_chroma_bm3d(_f(out), w, h, intensity), "chroma_bm3d")
+    return out
+
+
+def bm3d_test_image(w: int, h: int) -> np.ndarray:
+    out = np.empty((h, w, 3), np.float32)
+    _rc(lib().oracle_bm3d_test_image(_f(out), w, h), "bm3d_test_image")
+    return out
+
+
+def refined_sum(rgb: np.ndarray, channel: int) -> float:
+    a = np.ascontiguousarray(rgb, np.float32)
+    return lib().oracle_refined_sum(_f(a), a.size // 3, channel)

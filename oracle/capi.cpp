@@ -3,6 +3,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "oracle_bm3d.hpp"
 #include "oracle_colour.hpp"
 
 using namespace oracle;
@@ -62,5 +63,73 @@ int oracle_exp(float* rgb, size_t n, float ev) { exp_ev(rgb, n, ev); return 0; }
 int oracle_gamma(float* rgb, size_t n, float gamma) { gamma_encode(rgb, n, gamma); return 0; }
 int oracle_cfa_coeffs(float* rgb, size_t n, const float* wb) { cfa_coeffs(rgb, n, wb); return 0; }
 float oracle_srgb_oetf(float x) { return srgb_oetf(x); }
+
+// BM3D with explicit parameters; params[10] = {intensity, sigma, lambda,
+// max_dist, chroma_scale, detail, stride, window, max_group, max_candidates}.
+int oracle_bm3d(float* rgb, size_t w, size_t h, const float* params, int fp64, long long* stats2) {
+  Bm3dParams p;
+  p.intensity = params[0];
+  p.sigma = params[1];
+  p.lambda = params[2];
+  p.max_dist = params[3];
+  p.chroma_scale = params[4];
+  p.detail = params[5];
+  p.stride = (int)params[6];
+  p.window = (int)params[7];
+  p.max_group = (int)params[8];
+  p.max_candidates = (int)params[9];
+  if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.max_group < 1 || p.max_group > 32 ||
+      p.max_candidates < 1)
+    return -1;
+  Bm3dStats st;
+  if (fp64) bm3d<double>(rgb, w, h, p, &st);
+  else bm3d<float>(rgb, w, h, p, &st);
+  if (stats2) { stats2[0] = st.refs; stats2[1] = st.groups; }
+  return 0;
+}
+
+int oracle_bm3d_intensity_params(float intensity, float* params) {
+  Bm3dParams p = Bm3dParams::from_intensity(intensity);
+  const float v[10] = {p.intensity, p.sigma, p.lambda, p.max_dist, p.chroma_scale, p.detail,
+                       (float)p.stride, (float)p.window, (float)p.max_group, (float)p.max_candidates};
+  for (int i = 0; i < 10; ++i) params[i] = v[i];
+  return 0;
+}
+
+int oracle_chroma_bm3d(float* rgb, size_t w, size_t h, float intensity) {
+  chroma_bm3d(rgb, w, h, intensity);
+  return 0;
+}
+
+int oracle_bm3d_tables(float* dct, float* idct, float* kaiser) {
+  bm3d_tables(dct, idct, kaiser);
+  return 0;
+}
+
+// bm3d.rs:765-776 test image: channel sinusoids in [0,1].
+int oracle_bm3d_test_image(float* rgb, size_t w, size_t h) {
+  for (size_t y = 0; y < h; ++y)
+    for (size_t x = 0; x < w; ++x) {
+      float r = (sinf((float)x * 0.12f) * 0.5f + 0.5f) * 255.0f;
+      float g = (cosf((float)y * 0.17f) * 0.5f + 0.5f) * 255.0f;
+      float b = (sinf((float)(x + y) * 0.09f) * 0.5f + 0.5f) * 255.0f;
+      float* o = rgb + (y * w + x) * 3;
+      o[0] = r / 255.0f; o[1] = g / 255.0f; o[2] = b / 255.0f;
+    }
+  return 0;
+}
+
+// Kahan-compensated f64 channel sum (bm3d.rs:778-789).
+double oracle_refined_sum(const float* rgb, size_t n, int channel) {
+  double sum = 0.0, c = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double x = (double)rgb[i * 3 + channel];
+    double y = x - c;
+    double t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  }
+  return sum;
+}
 
 }  // extern "C"

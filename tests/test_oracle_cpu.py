@@ -130,3 +130,58 @@ def test_normalize_raw_crop():
     out = oracle.normalize_raw(raw, 600, 16383, crop=(2, 1, 5, 3))
     want = (raw[1:4, 2:7].astype(np.float32) - np.float32(600)) / np.float32(15783)
     assert np.array_equal(out, want)
+
+
+# ---------------------------------------------------------------- BM3D pins
+
+def test_bm3d_reference_golden_checksums():
+    """bm3d.rs:791-808.  The restatement lands within 0.03 of the recorded Kahan sums
+    (~1e-8 relative); evaluating the fixture's sinf/cosf correctly rounded instead of
+    with glibc moves the sums by 0.014, and a planted rounding-instead-of-truncation
+    in the i64 aggregation moves them by >100, so the tolerance separates libm noise
+    from restatement mistakes."""
+    img = oracle.bm3d_test_image(2048, 2048)
+    out, (refs, grouped) = oracle.bm3d(img, oracle.bm3d_params(0.1))
+    assert refs == 2 * 340 * 340
+    expected = [2100044.776915211, 2101586.5654235613, 2098591.6187395807]
+    for c in range(3):
+        assert abs(oracle.refined_sum(out, c) - expected[c]) < 0.05
+
+
+def test_chroma_bm3d_reference_golden_checksums():
+    # bm3d.rs:810-827 (DCT tables, Kaiser window, hard threshold and i64 aggregation without matching)
+    img = oracle.bm3d_test_image(2048, 2048)
+    out = oracle.chroma_bm3d(img, 0.1)
+    expected = [2099732.10061259, 2101692.5347650475, 2098392.755773447]
+    for c in range(3):
+        assert abs(oracle.refined_sum(out, c) - expected[c]) < 3e-3
+
+
+def test_bm3d_dct_tables_orthonormal():
+    import ctypes as C
+    d, i, k = (np.empty(64, np.float32) for _ in range(3))
+    oracle.lib().oracle_bm3d_tables(oracle._f(d), oracle._f(i), oracle._f(k))
+    D, I = d.reshape(8, 8).astype(np.float64), i.reshape(8, 8).astype(np.float64)
+    assert np.allclose(D @ D.T, np.eye(8), atol=1e-6)
+    assert np.allclose(I, D.T, atol=1e-6)          # idct[n][k] = dct[k][n]
+    K = k.reshape(8, 8)
+    assert np.allclose(K, K.T) and np.allclose(K, K[::-1, ::-1])
+    assert abs(K[3, 3] - K.max()) < 1e-6 and K.min() > 0
+
+
+def test_bm3d_fp32_against_fp64_oracle():
+    rng = np.random.default_rng(11)
+    from imops_amd import synth_scene_rgb
+    x = np.clip(synth_scene_rgb(96, 72, 2) + rng.normal(0, 0.03, (72, 96, 3)), 0, 1).astype(np.float32)
+    p = oracle.bm3d_params(0.2)
+    a, _ = oracle.bm3d(x, p)
+    b, _ = oracle.bm3d(x, p, fp64=True)
+    mse = float(np.mean((a.astype(np.float64) - b) ** 2))
+    assert 10 * np.log10(1.0 / max(mse, 1e-30)) > 60
+
+
+def test_bm3d_flat_image_is_fixed_point():
+    # a flat image has a flat DCT (only DC survives) and is reproduced up to fixed-point rounding
+    x = np.full((40, 40, 3), 0.5, np.float32)
+    out, _ = oracle.bm3d(x, oracle.bm3d_params(0.3))
+    assert np.abs(out - 0.5).max() < 1e-4
