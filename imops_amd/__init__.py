@@ -33,6 +33,11 @@ __all__ = [
 # ColorSpaceTag names accepted by CST (modules.rs:107-112, serde_plain names)
 COLOR_SPACES = {"Srgb": CS_SRGB, "LinearSrgb": CS_LINEAR_SRGB, "AcesCg": CS_ACESCG, "XyzD65": CS_XYZD65}
 _ALGORITHMS = {"amaze": DEMOSAIC_AMAZE, "markesteijn": DEMOSAIC_MARKESTEIJN}
+# Film stage choices (modules.rs:800-830; StockId order of film/stock/mod.rs:124-133)
+STOCKS = {"BwStub": 0, "ColorNeg200": 1, "Portra400": 2, "Ektar100": 3, "FujiPro400H": 4, "EktachromeE100": 5,
+          "TriX400": 6}
+FILM_FORMATS = {"Film35mm": 0, "Film6x6": 1, "Film4x5": 2}
+FILM_OUTPUTS = {"NegativeLinear": 0, "PositiveLinear": 1}
 
 
 def _cs(target) -> int:
@@ -201,6 +206,13 @@ class Image:
             return self
         p = bm3d_params(intensity, **explicit)
         check(lib().imops_bm3d_ex(self.ctx.handle, self._h, C.byref(p)))
+        return self
+
+    def film(self, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
+             output: str = "NegativeLinear") -> "Image":
+        """Film simulation (film/mod.rs:77-129); the image must be absolute-luminance ACEScg."""
+        check(lib().imops_film(self.ctx.handle, self._h, STOCKS[stock], FILM_FORMATS[film_format], int(seed),
+                               FILM_OUTPUTS[output]))
         return self
 
     def camera_cst(self, target, calibration_matrix: Sequence[float]) -> "Image":

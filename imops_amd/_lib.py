@@ -65,6 +65,7 @@ SIGNATURES = {
     "imops_bm3d_default_params": (_i, [_f, C.POINTER(Bm3dParams)]),
     "imops_bm3d": (_i, [_vp, _vp, _f]),
     "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
+    "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
     "imops_pipeline_num_stages": (_sz, [_vp]),

@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
     unsigned long long b = S.best[0];
     for (int w = 1; w < MT / 64; ++w) b = S.best[w] < b ? S.best[w] : b;
     const int c = (int)(b & 0xffffffffu);
-    if (tid == 0) {
+    if (tid == 0 && b != ~0ull) {
       const int cy = c / nx, cx = c - cy * nx;
       out[1 + k] = (uint32_t)(sx0 + cx) | ((uint32_t)(sy0 + cy) << 16);
       S.dist[c] = __builtin_inff();

@@ -143,6 +143,16 @@ int imops_bm3d_default_params(float intensity, imops_bm3d_params* out);
 int imops_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
 int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* params);
 
+/* Film simulation (film::process, pichromatic/src/film/mod.rs:77-129) on an
+ * absolute-luminance ACEScg image; output ACEScg.  Module<Film>
+ * (modules.rs:841-876).  stock: 0 BwStub, 1 ColorNeg200, 2 Portra400,
+ * 3 Ektar100, 4 FujiPro400H, 5 EktachromeE100, 6 TriX400 (StockId order);
+ * film_format: 0 Film35mm, 1 Film6x6, 2 Film4x5; output: 0 NegativeLinear,
+ * 1 PositiveLinear.  The shutter time is the image's shutter_seconds, else
+ * 1/box ISO.  A stock whose tables the library does not carry returns
+ * IMOPS_E_UNSUPPORTED. */
+int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_format, uint64_t seed, int output);
+
 /* Pipeline --------------------------------------------------------------- */
 
 /* Parse a TOML or JSON stage list ([[pipeline_modules]] tables, names matched

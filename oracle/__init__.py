@@ -182,3 +182,42 @@ def bm3d_test_image(w: int, h: int) -> np.ndarray:
 def refined_sum(rgb: np.ndarray, channel: int) -> float:
     a = np.ascontiguousarray(rgb, np.float32)
     return lib().oracle_refined_sum(_f(a), a.size // 3, channel)
+
+
+STOCKS = {"BwStub": 0, "ColorNeg200": 1, "Portra400": 2, "Ektar100": 3, "FujiPro400H": 4, "EktachromeE100": 5,
+          "TriX400": 6}
+FORMATS = {"Film35mm": 0, "Film6x6": 1, "Film4x5": 2}
+_dp = C.POINTER(C.c_double)
+
+
+
+def film(rgb: np.ndarray, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
+         output: str = "NegativeLinear", shutter: float = 0.0, fp64: bool = False, n_emulsions: int = 3,
+         trace: bool = False):
+    """film::process (film/mod.rs:77-129) on an (H, W, 3) absolute-luminance ACEScg image.
+
+    Returns the scanned ACEScg image, or (image, trace) with the latent, image-dye and
+    mask-dye planes (n_emulsions x H x W, f64) and the PositiveLinear mid/Dmin."""
+    L = lib()
+    L.oracle_film.restype = C.c_int
+    L.oracle_film.argtypes = [_fp, _sz, _sz, C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.c_float, C.c_int,
+                              _dp, _dp, _dp, _fp]
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    planes = [np.zeros((n_emulsions, h, w), np.float64) for _ in range(3)] if trace else [None] * 3
+    md = np.zeros(6, np.float32)
+    ptr = lambda a: a.ctypes.data_as(_dp) if a is not None else None
+    _rc(L.oracle_film(_f(out), w, h, STOCKS[stock], FORMATS[film_format], seed,
+                      1 if output == "PositiveLinear" else 0, shutter, int(fp64), ptr(planes[0]), ptr(planes[1]),
+                      ptr(planes[2]), _f(md) if trace else None), "film")
+    if not trace:
+        return out
+    return out, {"latent": planes[0], "image_dye": planes[1], "mask_dye": planes[2], "mid": md[:3].copy(),
+                 "dmin": md[3:].copy()}
+
+
+def absolute_gray(y_rel: float, box_iso: float) -> np.float32:
+    """relative_to_absolute_y under sunny-16 (film/mod.rs:153-163)."""
+    s = np.float32(max(box_iso, 1.0))
+    t = float(np.float32(1.0) / s)
+    return np.float32(12.5 * float(np.float32(y_rel)) * 64.0 / (t * float(s)))

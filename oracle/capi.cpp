@@ -1,10 +1,12 @@
 // Oracle (test infrastructure only): C entry points loaded by the tests, the
 // smoke check and the benchmark's CPU-baseline leg through ctypes.
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "oracle_bm3d.hpp"
 #include "oracle_colour.hpp"
+#include "oracle_film.hpp"
 
 using namespace oracle;
 
@@ -130,6 +132,23 @@ double oracle_refined_sum(const float* rgb, size_t n, int channel) {
     sum = t;
   }
   return sum;
+}
+
+// Film simulation; trace planes (latent, image dye, mask dye: E*w*h each as f64)
+// are written when the pointers are non-null.
+int oracle_film(float* rgb, size_t w, size_t h, int stock, int format, unsigned long long seed, int output,
+                float shutter, int fp64, double* latent, double* image_dye, double* mask_dye, float* mid_dmin6) {
+  FilmTrace tr;
+  const bool want = latent || image_dye || mask_dye || mid_dmin6;
+  int rc = fp64 ? film_process<double>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr)
+                : film_process<float>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr);
+  if (rc != 0) return rc;
+  if (latent) std::copy(tr.latent.begin(), tr.latent.end(), latent);
+  if (image_dye) std::copy(tr.image_dye.begin(), tr.image_dye.end(), image_dye);
+  if (mask_dye) std::copy(tr.mask_dye.begin(), tr.mask_dye.end(), mask_dye);
+  if (mid_dmin6)
+    for (int c = 0; c < 3; ++c) { mid_dmin6[c] = tr.mid[c]; mid_dmin6[3 + c] = tr.dmin[c]; }
+  return 0;
 }
 
 }  // extern "C"

@@ -1,0 +1,25 @@
+// Oracle (test infrastructure only): film simulation declarations.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace film_data { struct Stock; }
+
+namespace oracle {
+
+// Intermediate planes of one run (emulsion-major, width*height each).
+struct FilmTrace {
+  std::vector<double> latent, image_dye, mask_dye;
+  float mid[3] = {0, 0, 0}, dmin[3] = {0, 0, 0};
+};
+
+const film_data::Stock* film_stock(int idx);
+
+// film::process on an interleaved ACEScg buffer (absolute luminance), in place.
+// output: 0 NegativeLinear, 1 PositiveLinear.  shutter_seconds <= 0: 1/box ISO.
+template <class P>
+int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint64_t seed, int output,
+                 float shutter_seconds, FilmTrace* trace);
+
+}  // namespace oracle
