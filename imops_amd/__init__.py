@@ -68,6 +68,16 @@ class Context:
     def synchronize(self) -> None:
         check(lib().imops_ctx_synchronize(self._h))
 
+    def set_timing(self, enabled: bool) -> None:
+        """Record HIP events around the library's launches (clears earlier records)."""
+        check(lib().imops_ctx_set_timing(self._h, int(bool(enabled))))
+
+    def timing(self, region: str):
+        """(total ms, launches) recorded for a region since timing was enabled."""
+        ms, n = C.c_double(), C.c_int()
+        check(lib().imops_ctx_timing(self._h, region.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def close(self) -> None:
         if self._h:
             lib().imops_ctx_destroy(self._h)

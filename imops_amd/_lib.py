@@ -45,6 +45,8 @@ SIGNATURES = {
     "imops_ctx_destroy": (_i, [_vp]),
     "imops_ctx_synchronize": (_i, [_vp]),
     "imops_ctx_stream": (_vp, [_vp]),
+    "imops_ctx_set_timing": (_i, [_vp, _i]),
+    "imops_ctx_timing": (_i, [_vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(_i)]),
     "imops_image_create": (_i, [_vp, C.POINTER(_vp)]),
     "imops_image_free": (_i, [_vp]),
     "imops_image_set_raw_u16": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _sz, _sz, _sz, _sz, _f, _f, C.c_char_p]),

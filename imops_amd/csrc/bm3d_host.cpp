@@ -98,6 +98,7 @@ Bm3dBlurTaps blur_taps(float sigma) {
 
 void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   require_rgb(img, "BM3D");
+  TimedRegion stage_timer(ctx, "bm3d");
   const int w = (int)img->width, h = (int)img->height;
   const size_t n = (size_t)w * h;
   hipStream_t s = ctx->stream;
@@ -156,7 +157,10 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       m.max_candidates = p.max_candidates;
       m.max_group = p.max_group;
       m.groups = groups;
-      IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
+      {
+        TimedRegion tr(ctx, "bm3d_match");
+        IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
+      }
       Bm3dFilterArgs fa{};
       for (int c = 0; c < 3; ++c) {
         fa.guide[c] = guide[c];
@@ -178,6 +182,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       fa.dct = dtab;
       fa.idct = dtab + 64;
       fa.kaiser = dtab + 128;
+      TimedRegion tr(ctx, step1 ? "bm3d_filter_ht" : "bm3d_filter_wiener");
       IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
     }
     const float* nz[3] = {noisy[0], noisy[1], noisy[2]};

@@ -380,6 +380,7 @@ FilmStageParams film_stage_params_from_table(const std::string& stock, const std
 
 void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   require_rgb(img, "Film");
+  TimedRegion stage_timer(ctx, "film");
   if (img->color_space != IMOPS_CS_ACESCG)
     throw Error{IMOPS_E_STATE, "film: wrong color space: expected ACEScg"};  // film/mod.rs:78-87
   const int w = (int)img->width, h = (int)img->height;

@@ -39,6 +39,38 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+KernelTimer::Region& KernelTimer::region(const std::string& name) {
+  for (auto& r : regions)
+    if (r.name == name) return r;
+  regions.push_back(Region{name, {}});
+  return regions.back();
+}
+
+void KernelTimer::clear() {
+  for (auto& r : regions)
+    for (auto& e : r.events) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+  regions.clear();
+}
+
+TimedRegion::TimedRegion(imops_ctx* ctx, const char* name) : ctx_(ctx) {
+  if (!ctx_ || !ctx_->timer.enabled) return;
+  hipEvent_t start = nullptr;
+  if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop_) != hipSuccess) {
+    if (start) (void)hipEventDestroy(start);
+    stop_ = nullptr;
+    return;
+  }
+  (void)hipEventRecord(start, ctx_->stream);
+  ctx_->timer.region(name).events.push_back({start, stop_});
+}
+
+TimedRegion::~TimedRegion() {
+  if (stop_) (void)hipEventRecord(stop_, ctx_->stream);
+}
+
 void require_rgb(const imops_image* img, const char* stage) {
   if (!img->has_rgb) throw Error{IMOPS_E_STATE, std::string(stage) + ": image has no RGB data (demosaic first)"};
 }
@@ -221,6 +253,7 @@ void push_wb(PointwiseProgram& prog, const float wb[4]) {
 }
 
 void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& prog) {
+  TimedRegion tr(ctx, "pointwise");
   IMOPS_HIP_CHECK(pointwise_launch(img->rgb.as<float>(), img->width * img->height, prog, ctx->stream));
 }
 
@@ -250,6 +283,7 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
     a.scratch = ctx->scr[SCR_AMAZE].as<float>();
     for (int r = 0; r < 2; ++r)
       for (int c = 0; c < 2; ++c) a.cfa[r][c] = cfa.pat[r][c];
+    TimedRegion tr(ctx, "amaze");
     IMOPS_HIP_CHECK(amaze_launch(a, grid, ctx->stream));
   } else if (algorithm == IMOPS_DEMOSAIC_MARKESTEIJN) {
     CfaPattern p{};
@@ -257,6 +291,7 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
     p.h = cfa.height;
     for (int r = 0; r < 12; ++r)
       for (int c = 0; c < 12; ++c) p.base[r][c] = cfa.pat[r][c];
+    TimedRegion tr(ctx, "markesteijn");
     IMOPS_HIP_CHECK(markesteijn_launch(img->raw.as<float>(), (int)w, (int)h, p, img->rgb.as<float>(), ctx->stream));
   } else {
     throw Error{IMOPS_E_UNSUPPORTED, "demosaic algorithm not on the GPU path (fast/superfast preview modes)"};
@@ -295,11 +330,40 @@ int imops_ctx_create(int device, imops_ctx** out) {
   });
 }
 
+int imops_ctx_set_timing(imops_ctx* ctx, int enabled) {
+  return guarded([&] {
+    if (!ctx) throw Error{IMOPS_E_INVALID, "null context"};
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->timer.clear();
+    ctx->timer.enabled = enabled != 0;
+  });
+}
+
+int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* count) {
+  return guarded([&] {
+    if (!ctx || !region || !total_ms || !count) throw Error{IMOPS_E_INVALID, "ctx_timing: null argument"};
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    double total = 0.0;
+    int n = 0;
+    for (auto& r : ctx->timer.regions)
+      if (r.name == region)
+        for (auto& e : r.events) {
+          float ms = 0.0f;
+          IMOPS_HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+          total += ms;
+          ++n;
+        }
+    *total_ms = total;
+    *count = n;
+  });
+}
+
 int imops_ctx_destroy(imops_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->timer.clear();
     for (auto& b : ctx->scr) b.release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -354,6 +418,7 @@ int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* sr
       dsrc = ctx->scr[SCR_STAGING].as<uint16_t>();
     }
     img->raw.reserve(cw * ch * sizeof(float));
+    TimedRegion tr(ctx, "normalize");
     IMOPS_HIP_CHECK(normalize_launch(dsrc, full_w, cx, cy, cw, ch, black, white, img->raw.as<float>(), ctx->stream));
     img->width = cw;
     img->height = ch;

@@ -44,12 +44,43 @@ struct DevBuf {
 enum ScratchId : int { SCR_AMAZE = 0, SCR_STAGING, SCR_BM3D_0, SCR_BM3D_1, SCR_BM3D_2, SCR_BM3D_3, SCR_FILM_0,
                        SCR_FILM_1, SCR_FILM_2, SCR_FILM_3, SCR_COUNT };
 
+namespace imops {
+// Optional per-region GPU timing: HIP events recorded around launches on the
+// context stream while enabled.
+struct KernelTimer {
+  bool enabled = false;
+  struct Region {
+    std::string name;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  };
+  std::vector<Region> regions;
+  Region& region(const std::string& name);
+  void clear();
+};
+}  // namespace imops
+
 struct imops_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
   imops::DevBuf scr[SCR_COUNT];
+  imops::KernelTimer timer;
 };
+
+namespace imops {
+// Records start/stop events on the context stream around its scope when timing is on.
+class TimedRegion {
+ public:
+  TimedRegion(imops_ctx* ctx, const char* name);
+  ~TimedRegion();
+  TimedRegion(const TimedRegion&) = delete;
+  TimedRegion& operator=(const TimedRegion&) = delete;
+
+ private:
+  imops_ctx* ctx_;
+  hipEvent_t stop_ = nullptr;
+};
+}  // namespace imops
 
 struct imops_image {
   imops_ctx* ctx = nullptr;

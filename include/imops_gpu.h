@@ -61,6 +61,15 @@ int imops_ctx_destroy(imops_ctx* ctx);
 int imops_ctx_synchronize(imops_ctx* ctx);
 /* The hipStream_t every call of this context is enqueued on. */
 void* imops_ctx_stream(imops_ctx* ctx);
+/* GPU timing of named regions (HIP events on the context stream around the
+ * launches: "amaze", "markesteijn", "normalize", "pointwise", "bm3d",
+ * "bm3d_match", "bm3d_filter_ht", "bm3d_filter_wiener", "film").  Enabling or
+ * disabling clears the recorded events; imops_ctx_timing synchronises the
+ * stream and returns the summed milliseconds and the number of recorded
+ * launches of one region.  No counterpart in the reference (its CLI prints
+ * wall-clock pipeline time only, src/bin/imops-cli.rs:59-63). */
+int imops_ctx_set_timing(imops_ctx* ctx, int enabled);
+int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* count);
 
 /* Images: device-resident replacement of pichromatic::pixel::Image
  * (pixel.rs:22-27, image.rs:7-43). */
