@@ -85,50 +85,93 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
   const float thr = a.threshold * 64.0f;
   const int ncand = nx * ny;
   // Candidate SSDs in scan order; the reference position is excluded here and
-  // prepended below with distance 0.
-  for (int base = 0; base < ncand; base += MT) {
-    const int c = base + tid;
-    const bool valid = c < ncand;
-    const int cy = valid ? c / nx : 0, cx = valid ? c - cy * nx : 0;
-    const bool is_ref = valid && (sx0 + cx == rx) && (sy0 + cy == ry);
-    bool live = valid && !is_ref;
-    float d[3] = {0.0f, 0.0f, 0.0f};
-    float stop = thr;
+  // prepended below with distance 0.  A thread evaluates Q horizontally
+  // adjacent candidates so each staged window row is read once for all of them.
+  constexpr int Q = 3;
+  const int nbx = (nx + Q - 1) / Q;
+  const int nblk = nbx * ny;
+  for (int base = 0; base < nblk; base += MT) {
+    const int blk = base + tid;
+    const bool vblk = blk < nblk;
+    const int cy = vblk ? blk / nbx : 0;
+    const int cx0 = vblk ? (blk - cy * nbx) * Q : 0;
+    bool live[Q];
+    float d0[Q], d1[Q], d2[Q], stop[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int cx = cx0 + q;
+      live[q] = vblk && cx < nx && !((sx0 + cx == rx) && (sy0 + cy == ry));
+      d0[q] = d1[q] = d2[q] = 0.0f;
+      stop[q] = thr;
+    }
+#pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-      // exit when no lane of this wave still needs this channel
-      if (!__any(live)) break;
-      const float* win = &S.win[ch][cy * span_x + cx];
+      bool any_live = false;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) any_live |= live[q];
+      if (!__any(any_live)) break;  // no candidate of this wave still needs this channel
+      const float* win = &S.win[ch][cy * span_x + cx0];
       const float* rp = S.ref[ch];
-      float acc = 0.0f;
-      bool run = live;
+      float acc[Q];
+      bool run[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        acc[q] = 0.0f;
+        run[q] = live[q];
+      }
       for (int dy = 0; dy < 8; ++dy) {
-        if (!__any(run)) break;
-        if (run) {
-          const float* row = win + dy * span_x;
-          for (int dx = 0; dx < 8; ++dx) {
-            const float diff = row[dx] - rp[dy * 8 + dx];
-            acc += diff * diff;
+        bool any_run = false;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) any_run |= run[q];
+        if (!__any(any_run)) break;
+        const float* row = win + dy * span_x;
+        float wv[Q + 7], rv[8];
+#pragma unroll
+        for (int t = 0; t < Q + 7; ++t) wv[t] = row[cx0 + t < span_x ? t : 0];  // past the row only for q >= nx (dead)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) rv[t] = rp[dy * 8 + t];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          if (run[q]) {
+            float a_ = acc[q];
+#pragma unroll
+            for (int dx = 0; dx < 8; ++dx) {
+              const float diff = wv[q + dx] - rv[dx];
+              a_ += diff * diff;
+            }
+            acc[q] = a_;
+            if (a_ > stop[q]) run[q] = false;  // compute_ssd_flat returns the partial sum here
           }
-          if (acc > stop) run = false;  // compute_ssd_flat returns the partial sum here
         }
       }
-      d[ch] = acc;
-      if (live) {
-        if (ch == 0) {
-          if (d[0] > thr) live = false;
-          else stop = thr - d[0];
-        } else if (ch == 1) {
-          if (d[0] + d[1] > thr) live = false;
-          else stop = thr - (d[0] + d[1]);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (ch == 0) d0[q] = acc[q];
+        else if (ch == 1) d1[q] = acc[q];
+        else d2[q] = acc[q];
+        if (live[q]) {
+          if (ch == 0) {
+            if (d0[q] > thr) live[q] = false;
+            else stop[q] = thr - d0[q];
+          } else if (ch == 1) {
+            if (d0[q] + d1[q] > thr) live[q] = false;
+            else stop[q] = thr - (d0[q] + d1[q]);
+          }
         }
       }
     }
-    float dist = __builtin_inff();
-    if (live) {
-      const float total = d[0] + d[1] + d[2];
-      if (total < thr) dist = total / 64.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int cx = cx0 + q;
+      if (vblk && cx < nx) {
+        float dist = __builtin_inff();
+        if (live[q]) {
+          const float total = d0[q] + d1[q] + d2[q];
+          if (total < thr) dist = total / 64.0f;
+        }
+        S.dist[cy * nx + cx] = dist;
+      }
     }
-    if (valid) S.dist[c] = dist;
   }
   __syncthreads();
 
@@ -194,184 +237,187 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
 }
 
 // --------------------------------------------------------------- filtering
-constexpr int FT = 256;  // 4 waves; each wave filters one (reference patch, channel) at a time
+// A wave filters four (reference patch, channel) groups at once: a 16-lane
+// slot per group, lane k of a slot holding patch k of the group (all 64
+// samples in registers).  The 8x8 DCTs are register dot products in the
+// reference's order, the WHT across the group is a butterfly between slot
+// lanes (lane k pairs with k^h), and the Wiener weight's running sum walks
+// the stack in order by passing the partial sum from lane k to lane k+1.
+constexpr int FT = 256;
 
-// 8x8 2D DCT of the value each lane holds (lane = position r*8+c) in the
-// reference's order; `pad` is a per-wave 64-float LDS scratch.
-__device__ __forceinline__ float dct2_lane(float v, float* pad, const float* C, int lane) {
-  const int r = lane >> 3, k = lane & 7;
-  pad[lane] = v;
-  __builtin_amdgcn_wave_barrier();
-  // y[r][k] = sum_m x[r][m] * C[k][m]
-  const float* xr = pad + r * 8;
-  const float* ck = C + k * 8;
-  float y = xr[0] * ck[0] + xr[1] * ck[1] + xr[2] * ck[2] + xr[3] * ck[3] + xr[4] * ck[4] + xr[5] * ck[5] +
-            xr[6] * ck[6] + xr[7] * ck[7];
-  __builtin_amdgcn_wave_barrier();
-  pad[lane] = y;
-  __builtin_amdgcn_wave_barrier();
-  // out[j][k] = sum_n y[n][k] * C[j][n]   (lane = j*8+k)
-  const int j = r;
-  const float* cj = C + j * 8;
-  float o = pad[0 * 8 + k] * cj[0] + pad[1 * 8 + k] * cj[1] + pad[2 * 8 + k] * cj[2] + pad[3 * 8 + k] * cj[3] +
-            pad[4 * 8 + k] * cj[4] + pad[5 * 8 + k] * cj[5] + pad[6 * 8 + k] * cj[6] + pad[7 * 8 + k] * cj[7];
-  __builtin_amdgcn_wave_barrier();
-  return o;
+// out[j][k] = sum_n (sum_m x[n][m] C[k][m]) C[j][n]   (dct_2d_8x8, bm3d.rs:526-536)
+__device__ __forceinline__ void dct2_regs(float* x, const float* __restrict__ C) {
+  float y[64];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float* xr = x + r * 8;
+      const float* ck = C + k * 8;
+      y[r * 8 + k] = xr[0] * ck[0] + xr[1] * ck[1] + xr[2] * ck[2] + xr[3] * ck[3] + xr[4] * ck[4] + xr[5] * ck[5] +
+                     xr[6] * ck[6] + xr[7] * ck[7];
+    }
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float* cj = C + j * 8;
+      x[j * 8 + k] = y[0 * 8 + k] * cj[0] + y[1 * 8 + k] * cj[1] + y[2 * 8 + k] * cj[2] + y[3 * 8 + k] * cj[3] +
+                     y[4 * 8 + k] * cj[4] + y[5 * 8 + k] * cj[5] + y[6 * 8 + k] * cj[6] + y[7 * 8 + k] * cj[7];
+    }
 }
 
-// Inverse: u[k][n] = sum_j x[j][k] * D[n][j]; out[n][m] = sum_k u[k][n] * D[m][k].
-__device__ __forceinline__ float idct2_lane(float v, float* pad, const float* D, int lane) {
-  pad[lane] = v;
-  __builtin_amdgcn_wave_barrier();
-  {
-    const int k = lane >> 3, n = lane & 7;  // lane computes u[k][n]
-    const float* dn = D + n * 8;
-    float u = pad[0 * 8 + k] * dn[0] + pad[1 * 8 + k] * dn[1] + pad[2 * 8 + k] * dn[2] + pad[3 * 8 + k] * dn[3] +
-              pad[4 * 8 + k] * dn[4] + pad[5 * 8 + k] * dn[5] + pad[6 * 8 + k] * dn[6] + pad[7 * 8 + k] * dn[7];
-    __builtin_amdgcn_wave_barrier();
-    pad[lane] = u;
-    __builtin_amdgcn_wave_barrier();
+// u[k][n] = sum_j x[j][k] D[n][j]; out[n][m] = sum_k u[k][n] D[m][k]   (idct_2d_8x8, bm3d.rs:538-548)
+__device__ __forceinline__ void idct2_regs(float* x, const float* __restrict__ D) {
+  float u[64];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const float* dn = D + n * 8;
+      u[k * 8 + n] = x[0 * 8 + k] * dn[0] + x[1 * 8 + k] * dn[1] + x[2 * 8 + k] * dn[2] + x[3 * 8 + k] * dn[3] +
+                     x[4 * 8 + k] * dn[4] + x[5 * 8 + k] * dn[5] + x[6 * 8 + k] * dn[6] + x[7 * 8 + k] * dn[7];
+    }
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const float* dm = D + m * 8;
+      x[n * 8 + m] = u[0 * 8 + n] * dm[0] + u[1 * 8 + n] * dm[1] + u[2 * 8 + n] * dm[2] + u[3 * 8 + n] * dm[3] +
+                     u[4 * 8 + n] * dm[4] + u[5 * 8 + n] * dm[5] + u[6 * 8 + n] * dm[6] + u[7 * 8 + n] * dm[7];
+    }
+}
+
+// Walsh-Hadamard across the group (walsh_hadamard_1d, bm3d.rs:595-614):
+// stage h pairs lane k with k^h; the lower lane keeps x+y, the upper x-y.
+__device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
+  for (int h = 1; h < g; h *= 2) {
+    const bool upper = (k & h) != 0;
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      const float o = __shfl_xor(v[i], h, 16);
+      v[i] = upper ? o - v[i] : v[i] + o;
+    }
   }
-  const int n = lane >> 3, m = lane & 7;  // lane computes out[n][m]
-  const float* dm = D + m * 8;
-  float o = pad[0 * 8 + n] * dm[0] + pad[1 * 8 + n] * dm[1] + pad[2 * 8 + n] * dm[2] + pad[3 * 8 + n] * dm[3] +
-            pad[4 * 8 + n] * dm[4] + pad[5 * 8 + n] * dm[5] + pad[6 * 8 + n] * dm[6] + pad[7 * 8 + n] * dm[7];
-  __builtin_amdgcn_wave_barrier();
-  return o;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) v[i] *= scale;
 }
-
-// In-place WHT across the group (registers v[0..g-1]) with the reference's
-// butterfly order and 1/sqrt(g) scale.
-__device__ __forceinline__ void wht_regs(float* v, int g, float scale) {
-  for (int h = 1; h < g; h *= 2)
-    for (int i = 0; i < g; i += 2 * h)
-      for (int j = i; j < i + h; ++j) {
-        const float x = v[j], y = v[j + h];
-        v[j] = x + y;
-        v[j + h] = x - y;
-      }
-  for (int i = 0; i < g; ++i) v[i] *= scale;
-}
-
-struct FilterShared {
-  float pad[FT / 64][64];
-  float ssum[FT / 64][16 * 64];  // Wiener coefficients in stack order
-  float tab[3][64];              // dct, idct, kaiser
-};
 
 template <bool kStep1>
 __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
-  extern __shared__ unsigned char smem_raw[];
-  FilterShared& S = *reinterpret_cast<FilterShared*>(smem_raw);
-  unsigned long long* acc = reinterpret_cast<unsigned long long*>(smem_raw + sizeof(FilterShared));
+  extern __shared__ unsigned long long acc[];  // footprint num/den pairs
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // tile of reference patches owned by this workgroup
+  const int slot = lane >> 4, k = lane & 15;
   const int tiles_x = (a.refs_x + a.tile - 1) / a.tile;
   const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
   const int r0x = tx * a.tile, r0y = ty * a.tile;
   const int r1x = min(r0x + a.tile, a.refs_x), r1y = min(r0y + a.tile, a.refs_y);
   const int nrx = r1x - r0x, nref = nrx * (r1y - r0y);
-  // footprint: every patch a group of this tile can contain
   const int half = a.window / 2;
   const int fx0 = max(r0x * a.stride - half, 0), fy0 = max(r0y * a.stride - half, 0);
-  const int fx1 = min((r1x - 1) * a.stride + half, a.width - 8) + 8;   // exclusive
+  const int fx1 = min((r1x - 1) * a.stride + half, a.width - 8) + 8;  // exclusive
   const int fy1 = min((r1y - 1) * a.stride + half, a.height - 8) + 8;
   const int fw = fx1 - fx0, fh = fy1 - fy0;
 
-  if (tid < 64) {
-    S.tab[0][tid] = a.dct[tid];
-    S.tab[1][tid] = a.idct[tid];
-    S.tab[2][tid] = a.kaiser[tid];
-  }
-  float* pad = S.pad[wave];
-
   for (int ch = 0; ch < 3; ++ch) {
-    for (int k = tid; k < 2 * fw * fh; k += FT) acc[k] = 0ull;
+    for (int q = tid; q < 2 * fw * fh; q += FT) acc[q] = 0ull;
     __syncthreads();
     const float ch_sigma = ch == 0 ? a.sigma : a.sigma * a.chroma_scale;
-    const float* gplane = a.guide[ch];
-    const float* nplane = a.noisy[ch];
-    for (int r = wave; r < nref; r += FT / 64) {
-      const int ref_id = (r0y + r / nrx) * a.refs_x + r0x + r % nrx;
-      const uint32_t* grp = a.groups + (size_t)ref_id * 17;
-      const int g = (int)grp[0];
-      const int py = lane >> 3, px = lane & 7;
-      float gv[16], nv[16];
-      int gx[16], gyy[16];
-      for (int k = 0; k < 16; ++k) {
+    const float* __restrict__ gplane = a.guide[ch];
+    const float* __restrict__ nplane = a.noisy[ch];
+    // 16 groups in flight per workgroup (4 waves x 4 slots); every lane runs the
+    // same number of iterations so the slot shuffles stay converged.
+    for (int base = 0; base < nref; base += 16) {
+      const int r = base + wave * 4 + slot;
+      const bool have = r < nref;
+      int g = 0, px = 0, py = 0;
+      if (have) {
+        const int ref_id = (r0y + r / nrx) * a.refs_x + r0x + r % nrx;
+        const uint32_t* grp = a.groups + (size_t)ref_id * 17;
+        g = (int)grp[0];
         if (k < g) {
-          const uint32_t p = grp[1 + k];
-          gx[k] = (int)(p & 0xffffu);
-          gyy[k] = (int)(p >> 16);
-          const size_t off = (size_t)(gyy[k] + py) * a.width + gx[k] + px;
-          gv[k] = gplane[off];
-          nv[k] = kStep1 ? 0.0f : nplane[off];
+          const uint32_t pp = grp[1 + k];
+          px = (int)(pp & 0xffffu);
+          py = (int)(pp >> 16);
         }
       }
-      // forward 3D transform (bm3d.rs:490-506)
-      for (int k = 0; k < g; ++k) gv[k] = dct2_lane(gv[k], pad, S.tab[0], lane);
-      const float wscale = 1.0f / sqrtf((float)g);
-      wht_regs(gv, g, wscale);
-      if (!kStep1) {
-        for (int k = 0; k < g; ++k) nv[k] = dct2_lane(nv[k], pad, S.tab[0], lane);
-        wht_regs(nv, g, wscale);
-      }
+      const bool act = k < g;
+      float gv[64], nv[64];
+#pragma unroll
+      for (int dy = 0; dy < 8; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) {
+          const size_t off = (size_t)(py + dy) * a.width + px + dx;
+          gv[dy * 8 + dx] = act ? gplane[off] : 0.0f;
+          if (!kStep1) nv[dy * 8 + dx] = act ? nplane[off] : 0.0f;
+        }
+      const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
+      dct2_regs(gv, a.dct);
+      wht_lanes(gv, k, g, wscale);
       float weight;
       if (kStep1) {
-        // hard_threshold (bm3d.rs:240-255): element 0 of the stack is kept
+        // hard_threshold (bm3d.rs:240-255): stack element 0 (patch 0, coefficient 0) is kept
         const float th = a.lambda * ch_sigma;
         int nz = 0;
-        for (int k = 0; k < g; ++k) {
-          if (k == 0 && lane == 0) { ++nz; continue; }
-          if (fabsf(gv[k]) < th) gv[k] = 0.0f;
-          else ++nz;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          if (i == 0 && k == 0) { nz += act ? 1 : 0; continue; }
+          if (fabsf(gv[i]) < th) gv[i] = 0.0f;
+          else nz += act ? 1 : 0;
         }
-        for (int off = 32; off > 0; off >>= 1) nz += __shfl_xor(nz, off, 64);
+        for (int off = 8; off > 0; off >>= 1) nz += __shfl_xor(nz, off, 16);
         weight = nz > 0 ? 1.0f / (float)nz : 1.0f;
-        for (int k = 0; k < g; ++k) nv[k] = gv[k];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) nv[i] = gv[i];
       } else {
-        // wiener_filter (bm3d.rs:257-272): coefficient per element, running sum in stack order
+        dct2_regs(nv, a.dct);
+        wht_lanes(nv, k, g, wscale);
+        // wiener_filter (bm3d.rs:257-272)
         const float s2 = ch_sigma * ch_sigma;
-        float* ss = S.ssum[wave];
-        for (int k = 0; k < g; ++k) {
-          if (k == 0 && lane == 0) { ss[0] = 0.0f; continue; }
-          const float e = gv[k] * gv[k];
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+          if (i == 0 && k == 0) { gv[0] = 0.0f; continue; }  // element 0: weight term 1, sample untouched
+          const float e = gv[i] * gv[i];
           const float coef = e / (e + s2 + 1e-5f);
-          nv[k] *= coef;
-          ss[k * 64 + lane] = coef * coef;
+          nv[i] *= coef;
+          gv[i] = coef * coef;
         }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        float sum = 0.0f;
-        if (lane == 0) {
-          sum = 1.0f;
-          const int n = g * 64;
-          for (int i = 1; i < n; ++i) sum += ss[i];
+        // running sum in stack order: 1 + patch 0 (from its element 1), then patch 1, ...
+        float run = 1.0f;
+        int gmax = g;
+        for (int off = 16; off < 64; off <<= 1) gmax = max(gmax, __shfl_xor(gmax, off, 64));
+        for (int kk = 0; kk < gmax; ++kk) {
+          if (k == kk && kk < g) {
+#pragma unroll
+            for (int i = 0; i < 64; ++i)
+              if (!(kk == 0 && i == 0)) run += gv[i];
+          }
+          run = __shfl(run, (slot << 4) + kk, 64);
         }
-        sum = __shfl(sum, 0, 64);
-        weight = sum > 0.0f ? 1.0f / sum : 1.0f;
-        __builtin_amdgcn_wave_barrier();
+        weight = run > 0.0f ? 1.0f / run : 1.0f;
       }
-      // inverse (bm3d.rs:508-524)
-      wht_regs(nv, g, wscale);
-      const float kw = S.tab[2][lane] * weight;
-      const long long dfix = to_fixed(kw);
-      for (int k = 0; k < g; ++k) {
-        const float v = idct2_lane(nv[k], pad, S.tab[1], lane);
-        const int ax = gx[k] + px - fx0, ay = gyy[k] + py - fy0;
-        const int fi = ay * fw + ax;
-        atomicAdd(&acc[2 * fi], (unsigned long long)to_fixed(v * kw));
-        atomicAdd(&acc[2 * fi + 1], (unsigned long long)dfix);
+      // inverse (bm3d.rs:508-524) and aggregation into the footprint
+      wht_lanes(nv, k, g, wscale);
+      idct2_regs(nv, a.idct);
+      if (act) {
+#pragma unroll
+        for (int dy = 0; dy < 8; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < 8; ++dx) {
+            const float kw = a.kaiser[dy * 8 + dx] * weight;
+            const int fi = (py + dy - fy0) * fw + (px + dx - fx0);
+            atomicAdd(&acc[2 * fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * kw));
+            atomicAdd(&acc[2 * fi + 1], (unsigned long long)to_fixed(kw));
+          }
       }
     }
     __syncthreads();
-    // flush the footprint to the global i64 planes
     unsigned long long* gnum = a.num + (size_t)ch * a.width * a.height;
     unsigned long long* gden = a.den + (size_t)ch * a.width * a.height;
-    for (int k = tid; k < fw * fh; k += FT) {
-      const unsigned long long n = acc[2 * k], d = acc[2 * k + 1];
+    for (int q = tid; q < fw * fh; q += FT) {
+      const unsigned long long n = acc[2 * q], d = acc[2 * q + 1];
       if (n | d) {
-        const size_t gi = (size_t)(fy0 + k / fw) * a.width + fx0 + k % fw;
+        const size_t gi = (size_t)(fy0 + q / fw) * a.width + fx0 + q % fw;
         if (n) atomicAdd(&gnum[gi], n);
         if (d) atomicAdd(&gden[gi], d);
       }
@@ -445,9 +491,7 @@ int grid1d(size_t n) {
 }  // namespace
 
 size_t bm3d_match_smem() { return sizeof(MatchShared); }
-size_t bm3d_filter_smem(int footprint_pixels) {
-  return sizeof(FilterShared) + (size_t)footprint_pixels * 2 * sizeof(unsigned long long);
-}
+size_t bm3d_filter_smem(int footprint_pixels) { return (size_t)footprint_pixels * 2 * sizeof(unsigned long long); }
 
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s) {
   hipLaunchKernelGGL(rgb_to_ycbcr_kernel, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, y, cb, cr);

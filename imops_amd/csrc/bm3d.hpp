@@ -27,9 +27,7 @@ struct Bm3dFilterArgs {
   const uint32_t* groups;
   unsigned long long* num;  // 3 planes of i64 (two's complement in u64)
   unsigned long long* den;
-  const float* dct;
-  const float* idct;
-  const float* kaiser;
+  float dct[64], idct[64], kaiser[64];  // DctTables (bm3d.rs:417-461), computed on the host
 };
 
 struct Bm3dBlurTaps {

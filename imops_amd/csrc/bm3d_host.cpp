@@ -111,7 +111,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   ctx->scr[SCR_BM3D_0].reserve(12 * n * sizeof(float));
   ctx->scr[SCR_BM3D_1].reserve(6 * n * sizeof(unsigned long long));
   ctx->scr[SCR_BM3D_2].reserve((nrefs ? nrefs : 1) * 17 * sizeof(uint32_t));
-  ctx->scr[SCR_BM3D_3].reserve(3 * 64 * sizeof(float));
+
   float* f = ctx->scr[SCR_BM3D_0].as<float>();
   float* noisy[3] = {f, f + n, f + 2 * n};
   float* yorig = f + 3 * n;
@@ -122,12 +122,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   unsigned long long* num = ctx->scr[SCR_BM3D_1].as<unsigned long long>();
   unsigned long long* den = num + 3 * n;
   uint32_t* groups = ctx->scr[SCR_BM3D_2].as<uint32_t>();
-  float* dtab = ctx->scr[SCR_BM3D_3].as<float>();
-
   static const HostTables tables;
-  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab, tables.dct, 64 * sizeof(float), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab + 64, tables.idct, 64 * sizeof(float), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipMemcpyAsync(dtab + 128, tables.kaiser, 64 * sizeof(float), hipMemcpyHostToDevice, s));
 
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -137,7 +132,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   int tile = 8;
   for (; tile > 1; tile /= 2) {
     const int span = (tile - 1) * p.stride + 2 * half + 8;
-    if (bm3d_filter_smem(span * span) <= 80 * 1024) break;
+    if (bm3d_filter_smem(span * span) <= 64 * 1024) break;
   }
   const int span = (tile - 1) * p.stride + 2 * half + 8;
   const size_t fsmem = bm3d_filter_smem(span * span);
@@ -179,9 +174,11 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       fa.groups = groups;
       fa.num = num;
       fa.den = den;
-      fa.dct = dtab;
-      fa.idct = dtab + 64;
-      fa.kaiser = dtab + 128;
+      for (int i = 0; i < 64; ++i) {
+        fa.dct[i] = tables.dct[i];
+        fa.idct[i] = tables.idct[i];
+        fa.kaiser[i] = tables.kaiser[i];
+      }
       TimedRegion tr(ctx, step1 ? "bm3d_filter_ht" : "bm3d_filter_wiener");
       IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
     }
