@@ -33,6 +33,10 @@ BM3D_SIGMA = 0.02            # normalised noise sigma
 BM3D_STRIDE, BM3D_WINDOW, BM3D_LAMBDA = 3, 39, 2.7
 EV = math.log2(800.0)        # relative -> absolute luminance (sunny-16 gain) for the film stage
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per bm3d_match_kernel launch at this workload, from the FETCH_SIZE / WRITE_SIZE
+# counter runs summarised in profiles/full_chain_6016x4016_kernel_stats.md (FETCH scaled by
+# 2.000 and WRITE by 1.000, the factors a 1 GiB copy kernel reads in the same runs).
+MATCH_HBM_BYTES = (2 * 1137133.375 + 250875.6875) * 1024
 
 STAGES = f"""
 [[pipeline_modules]]
@@ -126,6 +130,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain f32 copy kernel of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -136,6 +142,13 @@ def main() -> int:
     import imops_amd as im
 
     ctx = im.Context(0)
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)  # reads and writes exactly n*4 bytes
+        torch.cuda.synchronize()
+        del src, dst
     print(f"[bench] generating {W}x{H} raw", file=sys.stderr, flush=True)
     raw = im.synth_raw_u16(W, H, "RGGB", 0)
     cal = calibration_matrix()
@@ -201,7 +214,7 @@ def main() -> int:
                    "stage_ms": regions},
         "roofline": {"bound": "mfma", "kernel": "bm3d_match_kernel (FP32 VALU, exact reference order)",
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": MATCH_HBM_BYTES,
                      "kernel_ms": round(match_ms, 3), "algorithmic_flops": flops},
         "cpu_baseline": None,
     }

@@ -221,3 +221,70 @@ def absolute_gray(y_rel: float, box_iso: float) -> np.float32:
     s = np.float32(max(box_iso, 1.0))
     t = float(np.float32(1.0) / s)
     return np.float32(12.5 * float(np.float32(y_rel)) * 64.0 / (t * float(s)))
+
+
+def _film_sigs(L):
+    L.oracle_film_blur.restype = None
+    L.oracle_film_blur.argtypes = [_fp, _sz, _sz, C.c_float]
+    L.oracle_film_grain_noise.restype = None
+    L.oracle_film_grain_noise.argtypes = [_fp, _sz, _sz, C.c_ulonglong, _sz]
+    L.oracle_film_capture_lut.restype = C.c_int
+    L.oracle_film_capture_lut.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+    L.oracle_film_lut_sample.restype = C.c_double
+    L.oracle_film_lut_sample.argtypes = [C.c_int, C.c_int, C.c_double]
+    L.oracle_film_upsample.restype = None
+    L.oracle_film_upsample.argtypes = [_fp, _dp]
+    L.oracle_film_spectrum_to_acescg.restype = None
+    L.oracle_film_spectrum_to_acescg.argtypes = [_dp, _dp]
+    L.oracle_film_normalized_dmin.restype = C.c_int
+    L.oracle_film_normalized_dmin.argtypes = [C.c_int, _fp]
+    return L
+
+
+def film_blur(plane: np.ndarray, sigma: float) -> np.ndarray:
+    """gaussian_blur_separable (film/blur.rs:10-89) on an f32 (H, W) plane."""
+    out = np.array(plane, np.float32, copy=True, order="C")
+    _film_sigs(lib()).oracle_film_blur(_f(out), out.shape[1], out.shape[0], sigma)
+    return out
+
+
+def film_grain_noise(w: int, h: int, seed: int, layer: int) -> np.ndarray:
+    """Unit grain noise field before its blur (grain.rs:20-45, 64-82)."""
+    out = np.zeros((h, w), np.float32)
+    _film_sigs(lib()).oracle_film_grain_noise(_f(out), w, h, seed, layer)
+    return out
+
+
+def film_capture_lut(stock: str, emulsion: int):
+    """(log10 phi knots, developable fractions, (mu_ln, sigma_ln, k)) of one emulsion (capture.rs:37-106)."""
+    logs, fracs, par = np.zeros(64), np.zeros(64), np.zeros(3)
+    _rc(_film_sigs(lib()).oracle_film_capture_lut(STOCKS[stock], emulsion, logs.ctypes.data_as(_dp),
+                                                   fracs.ctypes.data_as(_dp), par.ctypes.data_as(_dp)), "lut")
+    return logs, fracs, tuple(par)
+
+
+def film_lut_sample(stock: str, emulsion: int, phi: float) -> float:
+    return _film_sigs(lib()).oracle_film_lut_sample(STOCKS[stock], emulsion, phi)
+
+
+def film_upsample(rgb) -> np.ndarray:
+    """upsample_acescg (upsample.rs:145-170): 16 spectral samples 400..700 nm."""
+    a = np.asarray(rgb, np.float32)
+    out = np.zeros(16)
+    _film_sigs(lib()).oracle_film_upsample(_f(a), out.ctypes.data_as(_dp))
+    return out
+
+
+def film_spectrum_to_acescg(s) -> np.ndarray:
+    """spectrum_to_acescg_rgb (upsample.rs:179-184)."""
+    a = np.ascontiguousarray(s, np.float64)
+    out = np.zeros(3)
+    _film_sigs(lib()).oracle_film_spectrum_to_acescg(a.ctypes.data_as(_dp), out.ctypes.data_as(_dp))
+    return out
+
+
+def film_normalized_dmin(stock: str = "ColorNeg200") -> np.ndarray:
+    """normalized_dmin_acescg (densitometry.rs:76-117)."""
+    out = np.zeros(3, np.float32)
+    _rc(_film_sigs(lib()).oracle_film_normalized_dmin(STOCKS[stock], _f(out)), "dmin")
+    return out

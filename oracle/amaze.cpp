@@ -2,6 +2,9 @@
 // pichromatic/src/demosaic.rs:358-1379, one 128x128 tile at a time on the
 // tile grid top,left = -16 + 96k (demosaic.rs:397-407).  Instantiated for
 // float (exact reference op order) and double (FP64 error oracle).
+// Pins: uniform input (the reference's own demosaic tests), closed-form linear
+// ramps, native-site passthrough and the FP64 instantiation.  Absolute values
+// on textured input are otherwise parity unpinned (no reference vectors exist).
 #include <vector>
 
 #include "oracle_common.hpp"

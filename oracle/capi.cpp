@@ -151,4 +151,16 @@ int oracle_film(float* rgb, size_t w, size_t h, int stock, int format, unsigned 
   return 0;
 }
 
+void oracle_film_blur(float* buf, size_t w, size_t h, float sigma) { film_blur(buf, w, h, sigma); }
+void oracle_film_grain_noise(float* out, size_t w, size_t h, unsigned long long seed, size_t layer) {
+  film_grain_noise(out, w, h, seed, layer);
+}
+int oracle_film_capture_lut(int stock, int emulsion, double* logs, double* fracs, double* params) {
+  return film_capture_lut(stock, emulsion, logs, fracs, params);
+}
+double oracle_film_lut_sample(int stock, int emulsion, double phi) { return film_lut_sample(stock, emulsion, phi); }
+void oracle_film_upsample(const float* rgb, double* out) { film_upsample(rgb, out); }
+void oracle_film_spectrum_to_acescg(const double* s, double* out) { film_spectrum_to_acescg(s, out); }
+int oracle_film_normalized_dmin(int stock, float* out) { return film_normalized_dmin(stock, out); }
+
 }  // extern "C"

@@ -22,4 +22,12 @@ template <class P>
 int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint64_t seed, int output,
                  float shutter_seconds, FilmTrace* trace);
 
+void film_blur(float* buf, size_t w, size_t h, float sigma);
+void film_grain_noise(float* out, size_t w, size_t h, uint64_t seed, size_t layer);
+int film_capture_lut(int stock_idx, int emulsion, double* logs, double* fracs, double params[3]);
+double film_lut_sample(int stock_idx, int emulsion, double phi);
+void film_upsample(const float rgb[3], double out[16]);
+void film_spectrum_to_acescg(const double s[16], double out[3]);
+int film_normalized_dmin(int stock_idx, float out[3]);
+
 }  // namespace oracle
