@@ -59,11 +59,8 @@ struct MatchShared {
   unsigned long long best[MT / 64];
 };
 
-__global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
-  extern __shared__ unsigned char smem_raw[];
-  MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
+__device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ref_id = blockIdx.x;
   const int rx = (ref_id % a.refs_x) * a.stride, ry = (ref_id / a.refs_x) * a.stride;
   const int half = a.window / 2;
   const int wmax = a.width - 8, hmax = a.height - 8;
@@ -232,6 +229,22 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
       out[1 + k] = (uint32_t)(sx0 + cx) | ((uint32_t)(sy0 + cy) << 16);
       S.dist[c] = __builtin_inff();
     }
+    __syncthreads();
+  }
+}
+
+// Every reference patch (ref_list null: one workgroup per patch), or the
+// patches named in ref_list[0 .. *ref_count) with a grid-stride loop.
+__global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
+  if (!a.ref_list) {
+    match_one_ref(a, S, blockIdx.x);
+    return;
+  }
+  const uint32_t n = *a.ref_count;
+  for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
+    match_one_ref(a, S, (int)a.ref_list[it]);
     __syncthreads();
   }
 }
@@ -500,6 +513,11 @@ hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, fl
 
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s) {
   hipLaunchKernelGGL(bm3d_match_kernel, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
+  return hipGetLastError();
+}
+
+hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(bm3d_match_kernel, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
   return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@ struct Bm3dMatchArgs {
   float threshold;        // per-pixel mean SSD limit of this step
   int max_candidates, max_group;
   uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
+  const uint32_t* ref_list = nullptr;   // optional: match only these patches
+  const uint32_t* ref_count = nullptr;  // device count of ref_list entries
 };
 
 struct Bm3dFilterArgs {
@@ -30,6 +32,25 @@ struct Bm3dFilterArgs {
   float dct[64], idct[64], kaiser[64];  // DctTables (bm3d.rs:417-461), computed on the host
 };
 
+// Matrix-core matcher (bm3d_mfma.hip): one workgroup per 8x8 block of
+// reference patches; LDS offsets come from bm3d_mfma_layout().
+constexpr int kBm3dMfmaListCap = 48;
+struct Bm3dMfmaArgs {
+  const float* guide[3];
+  int width, height;
+  int refs_x, refs_y, stride, window;
+  float threshold;
+  int max_candidates, max_group;
+  uint32_t* groups;
+  uint32_t* overflow_list;   // patches whose candidate list overflowed (matched by the plain kernel)
+  uint32_t* overflow_count;  // [0]: patches for the plain matcher, [1]: patches selected approximately
+  int blocks_x, n_blocks;
+  int spx, spy;              // largest staged region (pixels)
+  int pitch;                 // LDS row pitch of the FP16 copies (halves)
+  int off_w16, off_w16b, off_nb, off_list, off_cnt, off_meta, off_red, smem;
+  bool max_tiles_ok;         // the geometry fits the kernel's tile bookkeeping
+};
+
 struct Bm3dBlurTaps {
   int radius, len;
   float k[33];
@@ -39,6 +60,9 @@ size_t bm3d_match_smem();
 size_t bm3d_filter_smem(int footprint_pixels);
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s);
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s);
+hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s);
+Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window);
+hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);
 hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s);
 hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],
                                 float* const out[3], size_t n, hipStream_t s);

@@ -3,6 +3,7 @@
 // (DctTables::new, bm3d.rs:417-461) and the Gaussian taps of the detail blend
 // (bm3d.rs:648-660) computed with the host libm exactly as the reference
 // computes them, and the launch sequence of bm3d() (bm3d.rs:34-61).
+#include <algorithm>
 #include <cmath>
 #include <string>
 
@@ -110,7 +111,8 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   const size_t nrefs = (size_t)refs_x * refs_y;
   ctx->scr[SCR_BM3D_0].reserve(12 * n * sizeof(float));
   ctx->scr[SCR_BM3D_1].reserve(6 * n * sizeof(unsigned long long));
-  ctx->scr[SCR_BM3D_2].reserve((nrefs ? nrefs : 1) * 17 * sizeof(uint32_t));
+  const size_t gwords = (nrefs ? nrefs : 1) * 17;
+  ctx->scr[SCR_BM3D_2].reserve((gwords + nrefs + 64) * sizeof(uint32_t));
 
   float* f = ctx->scr[SCR_BM3D_0].as<float>();
   float* noisy[3] = {f, f + n, f + 2 * n};
@@ -122,6 +124,12 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   unsigned long long* num = ctx->scr[SCR_BM3D_1].as<unsigned long long>();
   unsigned long long* den = num + 3 * n;
   uint32_t* groups = ctx->scr[SCR_BM3D_2].as<uint32_t>();
+  uint32_t* overflow_count = groups + gwords;           // 64-word aligned slot
+  uint32_t* overflow_list = overflow_count + 64;
+  // Matrix-core matching needs one 8x8 block of reference patches' search
+  // region in LDS; larger geometries use the plain matcher.
+  Bm3dMfmaArgs mf = bm3d_mfma_layout(w, h, p.stride, p.window);
+  const bool use_mfma = p.precision == IMOPS_BM3D_MIXED && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
   static const HostTables tables;
 
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
@@ -154,7 +162,38 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       m.groups = groups;
       {
         TimedRegion tr(ctx, "bm3d_match");
-        IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
+        if (use_mfma) {
+          for (int c = 0; c < 3; ++c) mf.guide[c] = guide[c];
+          mf.width = w;
+          mf.height = h;
+          mf.refs_x = refs_x;
+          mf.refs_y = refs_y;
+          mf.stride = p.stride;
+          mf.window = p.window;
+          mf.threshold = m.threshold;
+          mf.max_candidates = p.max_candidates;
+          mf.max_group = p.max_group;
+          mf.groups = groups;
+          mf.overflow_list = overflow_list;
+          mf.overflow_count = overflow_count;
+          mf.blocks_x = (refs_x + 7) / 8;
+          mf.n_blocks = mf.blocks_x * ((refs_y + 7) / 8);
+          IMOPS_HIP_CHECK(bm3d_match_mfma_launch(mf, s));
+          if (ctx->timer.enabled) {  // diagnostics: patches sent to the plain matcher / selected approximately
+            uint32_t nfb[2] = {0, 0};
+            IMOPS_HIP_CHECK(hipMemcpyAsync(nfb, overflow_count, sizeof nfb, hipMemcpyDeviceToHost, s));
+            IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+            ctx->timer.counts.emplace_back("bm3d_fallback_refs", (double)nfb[0]);
+            ctx->timer.counts.emplace_back("bm3d_approx_refs", (double)nfb[1]);
+          }
+          // patches whose contender list overflowed: the plain matcher, grid-stride over the list
+          Bm3dMatchArgs fb = m;
+          fb.ref_list = overflow_list;
+          fb.ref_count = overflow_count;
+          IMOPS_HIP_CHECK(bm3d_match_list_launch(fb, (int)std::min<size_t>(nrefs, 512), s));
+        } else {
+          IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
+        }
       }
       Bm3dFilterArgs fa{};
       for (int c = 0; c < 3; ++c) {
@@ -214,6 +253,7 @@ int imops_bm3d_default_params(float intensity, imops_bm3d_params* out) {
     out->search_window = p.window;
     out->max_group = p.max_group;
     out->max_candidates = p.max_candidates;
+    out->precision = p.precision;
   });
 }
 
@@ -237,6 +277,9 @@ int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* q) 
     p.window = q->search_window;
     p.max_group = q->max_group;
     p.max_candidates = q->max_candidates;
+    p.precision = q->precision;
+    if (p.precision != IMOPS_BM3D_FP32 && p.precision != IMOPS_BM3D_MIXED)
+      throw Error{IMOPS_E_INVALID, "bm3d_ex: precision must be IMOPS_BM3D_FP32 or IMOPS_BM3D_MIXED"};
     if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.window > kBm3dMaxWindow || p.max_group < 1 ||
         p.max_group > kBm3dMaxGroup || p.max_candidates < 1)
       throw Error{IMOPS_E_INVALID, "bm3d_ex: stride >= 1, odd search_window <= 41, max_group 1..16, max_candidates >= 1"};

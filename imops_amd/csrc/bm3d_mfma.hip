@@ -1,0 +1,695 @@
+// BM3D block matching on the gfx950 matrix cores, selecting exactly the groups
+// the plain FP32 matcher (and pichromatic/src/bm3d.rs:281-391) selects.
+//
+// Distances are evaluated as ||a||^2 + ||b||^2 - 2 a.b: the three guide planes
+// of a workgroup's search region are shifted by their mean and rounded to FP16
+// in LDS, the cross terms of 16 reference patches x 16 candidates run on
+// v_mfma_f32_16x16x32_f16 (FP32 accumulation, K = 192 = 3 channels x 8 x 8),
+// and the norms are 8x8 box sums of the same FP16 values.  Each FP16 distance d
+// comes with a rigorous interval for the FP32 sum the reference computes:
+//
+//   |sqrt(SSD) - sqrt(d_fp16_exact)| <= r = u' (||a'|| + ||b'||)  (rounding of the shifted values, u' = 2^-11 + 2^-23)
+//   |d - d_fp16_exact| <= eps = 2e-5 (na + nb) + 1e-3           (FP32 norms and MFMA accumulation)
+//   |fl32_reference_sum - SSD| <= 5e-5 SSD                       (192-term sequential FP32 sum)
+//
+// Acceptance (threshold), the 1024-candidate cap in scan order and the group
+// size are decided from those intervals, falling back to the reference's own
+// early-exit FP32 evaluation when an interval straddles the threshold.  The
+// group itself is chosen in two passes over the candidates: pass 1 tracks the
+// two smallest d per lane and reference, which bounds the 15th smallest
+// distance from above (T); pass 2 keeps every capped, accepted candidate whose
+// lower bound is <= T.  That list contains the true top 15, and its members are
+// re-evaluated in the reference's FP32 order (from the FP32 planes in global
+// memory) and ranked by (distance, scan order).  When more than 48 candidates
+// are indistinguishable at FP16 precision, the group is taken from the
+// approximate top 15 (FP16 distance <= the 15th smallest tracked one),
+// re-ranked in FP32; if even that list exceeds 32 entries (ties in flat areas),
+// the reference patch goes to the plain matcher.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "bm3d.hpp"
+
+namespace imops {
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int XW = 4;  // waves per workgroup; wave w owns a 4x4 sub-block of the 8x8 reference block
+constexpr int XT = 64 * XW;
+constexpr int kListCap = kBm3dMfmaListCap;     // exact-band contenders per reference
+constexpr int kListCapB = 32;                  // approximate top-K contenders per reference
+constexpr int kRowWords = (kBm3dMaxWindow + 1) / 2;  // 16-bit rejection counters per window row, packed in pairs
+static_assert(kRowWords * 4 <= kListCap * 2, "rejection counters alias the candidate lists");
+constexpr float kU = 4.8852e-4f;    // (2^-11 + 2^-23) * 1.001
+constexpr float kEpsRel = 2e-5f;
+constexpr float kEpsAbs = 1e-3f;
+constexpr float kSumRel = 5e-5f;
+constexpr float kInf = __builtin_inff();
+
+struct RefMeta {
+  float d15;  // K-th smallest tracked upper bound
+  float dK;   // K-th smallest tracked FP16 distance (approximate-list criterion)
+  int limit;  // scan index of the last candidate inside the cap
+  int g;      // group size
+  int pad;    // 1: the cap needs the row walk (rejections before the geometric cap)
+};
+
+__device__ __forceinline__ half8 load8(const _Float16* p) {
+  half8 v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+typedef float floatx4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load8g(const float* p, float* v) {
+  floatx4v a, b;
+  __builtin_memcpy(&a, p, 16);
+  __builtin_memcpy(&b, p + 4, 16);
+  v[0] = a[0]; v[1] = a[1]; v[2] = a[2]; v[3] = a[3];
+  v[4] = b[0]; v[5] = b[1]; v[6] = b[2]; v[7] = b[3];
+}
+
+// The plain matcher's acceptance of one candidate (early exit per row, per-channel
+// stop thresholds) on the FP32 guide planes.
+__device__ bool exact_accept(const float* const g[3], int W, int rx, int ry, int cx, int cy, float thr) {
+  float d[3] = {0.0f, 0.0f, 0.0f};
+  float stop = thr;
+  for (int ch = 0; ch < 3; ++ch) {
+    float acc = 0.0f;
+    for (int dy = 0; dy < 8; ++dy) {
+      float c[8], r[8];
+      load8g(g[ch] + (size_t)(cy + dy) * W + cx, c);
+      load8g(g[ch] + (size_t)(ry + dy) * W + rx, r);
+#pragma unroll
+      for (int dx = 0; dx < 8; ++dx) {
+        const float diff = c[dx] - r[dx];
+        acc += diff * diff;
+      }
+      if (acc > stop) break;
+    }
+    d[ch] = acc;
+    if (ch == 0) {
+      if (d[0] > thr) return false;
+      stop = thr - d[0];
+    } else if (ch == 1) {
+      if (d[0] + d[1] > thr) return false;
+      stop = thr - (d[0] + d[1]);
+    }
+  }
+  return d[0] + d[1] + d[2] < thr;
+}
+
+// Full SSD of an accepted candidate in the reference's order (no early exit can
+// have fired for it).
+__device__ float exact_total(const float* const g[3], int W, int rx, int ry, int cx, int cy) {
+  float d[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    float acc = 0.0f;
+    for (int dy = 0; dy < 8; ++dy) {
+      float c[8], r[8];
+      load8g(g[ch] + (size_t)(cy + dy) * W + cx, c);
+      load8g(g[ch] + (size_t)(ry + dy) * W + rx, r);
+#pragma unroll
+      for (int dx = 0; dx < 8; ++dx) {
+        const float diff = c[dx] - r[dx];
+        acc += diff * diff;
+      }
+    }
+    d[ch] = acc;
+  }
+  return d[0] + d[1] + d[2];
+}
+
+// Interval arithmetic below uses the hardware square root (<= 1 ulp); the
+// (1 +- 1e-6) factors absorb it.
+__device__ __forceinline__ float hsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+// Sum over the 16 lanes of this lane's group.
+__device__ __forceinline__ int group_sum(int v) {
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+  return v;
+}
+
+// B operand chunk: 8 halves starting at a 4-byte aligned LDS address.
+__device__ __forceinline__ half8 load8a(const _Float16* p) {
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(__builtin_assume_aligned(p, 4));
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 v = {q[0], q[1], q[2], q[3]};
+  return __builtin_bit_cast(half8, v);
+}
+
+__global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  // c0[ch][y][x] = fp16(v - shift); c1[ch][y][x] = c0[ch][y][x + 1], so every
+  // 8-half candidate row starts on a 4-byte boundary in one of the two copies.
+  _Float16* c0 = reinterpret_cast<_Float16*>(smem + a.off_w16);
+  _Float16* c1 = reinterpret_cast<_Float16*>(smem + a.off_w16b);
+  float* nb = reinterpret_cast<float*>(smem + a.off_nb);
+  float* hs = reinterpret_cast<float*>(smem + a.off_w16b);   // setup scratch, before c1 is staged
+  float* tot = reinterpret_cast<float*>(smem + a.off_w16b);  // exact phase, after pass 2
+  uint16_t* list = reinterpret_cast<uint16_t*>(smem + a.off_list);
+  uint32_t* rejrow = reinterpret_cast<uint32_t*>(smem + a.off_list);  // pass 1: rejections per (reference, window row)
+  int* lcnt = reinterpret_cast<int*>(smem + a.off_cnt);
+  int* lcntb = lcnt + XW * 16;
+  uint16_t* listb = reinterpret_cast<uint16_t*>(smem + a.off_list) + XW * 16 * kListCap;
+  RefMeta* meta = reinterpret_cast<RefMeta*>(smem + a.off_meta);
+  float* red = reinterpret_cast<float*>(smem + a.off_red);
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, n = lane & 15;
+
+  // Workgroups are dealt round-robin to the 8 XCDs: give each XCD a contiguous
+  // run of reference blocks so neighbouring search regions share its L2.
+  const int nwg = gridDim.x, wg = blockIdx.x;
+  const int xcd = wg & 7;
+  const int b = xcd * (nwg >> 3) + min(xcd, nwg & 7) + (wg >> 3);
+  const int bx = b % a.blocks_x, by = b / a.blocks_x;
+  const int r0x = bx * 8, r0y = by * 8;
+  const int nvx = min(8, a.refs_x - r0x), nvy = min(8, a.refs_y - r0y);
+  const int st = a.stride, half = a.window / 2;
+  const int W = a.width, wmax = a.width - 8, hmax = a.height - 8;
+  const int X0 = max(r0x * st - half, 0), X1 = min((r0x + nvx - 1) * st + half, wmax);
+  const int Y0 = max(r0y * st - half, 0), Y1 = min((r0y + nvy - 1) * st + half, hmax);
+  const int SX = X1 - X0 + 8, SY = Y1 - Y0 + 8;
+  const int NX = X1 - X0 + 1, NY = Y1 - Y0 + 1;
+  const int P = a.pitch, PN = a.spx - 7, plane = a.spy * a.pitch;
+  const float thr = a.threshold * 64.0f;
+  const int cap = a.max_candidates - 1;  // accepted candidates kept besides the reference
+
+  // ---- shift = mean of the middle reference patch (per channel); norms = 8x8
+  // box sums of the squared FP16 values; then the two FP16 copies
+  if (wave == 0) {
+    const int mx = (r0x + (nvx - 1) / 2) * st, my = (r0y + (nvy - 1) / 2) * st;
+    for (int ch = 0; ch < 3; ++ch) {
+      const float t = wave_sum(a.guide[ch][(size_t)(my + (lane >> 3)) * W + mx + (lane & 7)]);
+      if (lane == 0) red[ch] = t * (1.0f / 64.0f);
+    }
+  }
+  for (int k = tid; k < XW * 16 * kRowWords; k += XT) rejrow[k] = 0u;
+  if (tid < XW * 16) {
+    lcnt[tid] = 0;
+    lcntb[tid] = 0;
+    meta[tid].d15 = kInf;
+    meta[tid].dK = kInf;
+    meta[tid].limit = 0x7fffffff;
+    meta[tid].g = 1;
+  }
+  __syncthreads();
+  const float shift[3] = {red[0], red[1], red[2]};
+  for (int row = wave; row < SY; row += XW) {
+    for (int xx = lane; xx < NX; xx += 64) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float* src = a.guide[ch] + (size_t)(Y0 + row) * W + X0 + xx;
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) {
+          const float f = (float)(_Float16)(src[dx] - shift[ch]);
+          acc += f * f;
+        }
+      }
+      hs[row * PN + xx] = acc;
+    }
+  }
+  __syncthreads();
+  float nbm = 0.0f;
+  for (int row = wave; row < NY; row += XW) {
+    for (int xx = lane; xx < NX; xx += 64) {
+      float acc = 0.0f;
+#pragma unroll
+      for (int dy = 0; dy < 8; ++dy) acc += hs[(row + dy) * PN + xx];
+      nb[row * PN + xx] = acc;
+      nbm = fmaxf(nbm, acc);
+    }
+  }
+  nbm = wave_max(nbm);
+  if (lane == 0) red[3 * XW + wave] = nbm;
+  __syncthreads();  // hs is dead: stage both copies
+  for (int row = wave; row < 3 * SY; row += XW) {
+    const int ch = row / SY, yy = row - ch * SY;
+    const float* src = a.guide[ch] + (size_t)(Y0 + yy) * W + X0;
+    _Float16* d0 = c0 + ch * plane + yy * P;
+    _Float16* d1 = c1 + ch * plane + yy * P;
+    for (int xx = lane; xx < SX; xx += 64) {
+      const _Float16 v = (_Float16)(src[xx] - shift[ch]);
+      d0[xx] = v;
+      if (xx > 0) d1[xx - 1] = v;
+    }
+  }
+  __syncthreads();
+  float nbmax = red[3 * XW];
+  for (int w = 1; w < XW; ++w) nbmax = fmaxf(nbmax, red[3 * XW + w]);
+  const float rbmax = kU * hsqrt(nbmax) * (1.0f + 1e-6f) + 1e-5f;
+
+  // ---- this wave's 4x4 reference sub-block.  D layout of the MFMA: lane
+  // (q, n), register j holds reference m = 4q + j (sub-block row q, column j)
+  // against candidate n of the tile.
+  const int wrx0 = r0x + 4 * (wave & 1), wry0 = r0y + 4 * (wave >> 1);
+  const int wnx = min(4, a.refs_x - wrx0), wny = min(4, a.refs_y - wry0);
+  const bool wave_active = 4 * (wave & 1) < nvx && 4 * (wave >> 1) < nvy;
+  const int K = a.max_group - 1;
+
+  const int ry = (wry0 + q) * st;
+  const bool yvalid = wave_active && q < wny;
+  const int wy0 = max(ry - half, 0), wy1 = min(ry + half, hmax);
+  int rxa[4], wx0[4], ew[4], limgeo[4], npos[4];
+  bool vj[4];
+  float na[4], ra[4], asure[4], epsmax[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    rxa[j] = (wrx0 + j) * st;
+    vj[j] = yvalid && j < wnx;
+    wx0[j] = max(rxa[j] - half, 0);
+    ew[j] = min(rxa[j] + half, wmax) - wx0[j] + 1;
+    // candidates of the window besides the reference, and the scan index of
+    // the last of the first `cap` of them
+    npos[j] = vj[j] ? ew[j] * (wy1 - wy0 + 1) - 1 : 0;
+    const int self = (ry - wy0) * ew[j] + (rxa[j] - wx0[j]);
+    limgeo[j] = npos[j] <= cap ? 0x7fffffff : (cap - 1 < self ? cap - 1 : cap);
+    na[j] = vj[j] ? nb[(ry - Y0) * PN + (rxa[j] - X0)] : 0.0f;
+    ra[j] = kU * hsqrt(na[j]) * (1.0f + 1e-6f);
+    epsmax[j] = kEpsRel * (na[j] + nbmax) + kEpsAbs;
+    const float root = hsqrt(thr / (1.0f + kSumRel)) * (1.0f - 1e-6f) - ra[j] - rbmax;
+    asure[j] = root > 0.0f ? root * root * (1.0f - 1e-6f) - epsmax[j] : -kInf;
+  }
+
+  int ux0 = 0, ux1 = 0, uy0 = 0, uy1 = -1;
+  half8 A[6];
+  if (wave_active) {
+    ux0 = max(wrx0 * st - half, 0);
+    ux1 = min((wrx0 + wnx - 1) * st + half, wmax);
+    uy0 = max(wry0 * st - half, 0);
+    uy1 = min((wry0 + wny - 1) * st + half, hmax);
+    // A operand: lane (q, n) holds reference n (clamped into the valid part of
+    // the sub-block), K-group q.
+    const int aix = min(n & 3, wnx - 1), aiy = min(n >> 2, wny - 1);
+    const int arx = (wrx0 + aix) * st - X0, ary = (wry0 + aiy) * st - Y0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const int t = 4 * s + q, ch = t >> 3, dy = t & 7;
+      A[s] = load8(c0 + ch * plane + (ary + dy) * P + arx);
+    }
+  }
+  const int ntc = (ux1 - ux0) / 16 + 1;
+
+  // Precise classification of one pair from its own bounds; rb = the
+  // candidate's share of r.
+  auto classify = [&](int j, float d, float nbv, float rb, int cx, int cy) -> bool {
+    const float eps = kEpsRel * (na[j] + nbv) + kEpsAbs;
+    const float r = ra[j] + rb;
+    const float hi = hsqrt(fmaxf(d + eps, 0.0f)) * (1.0f + 1e-6f) + r;
+    if (hi * hi * (1.0f + kSumRel) * (1.0f + 1e-6f) < thr) return true;
+    const float lo = fmaxf(hsqrt(fmaxf(d - eps, 0.0f)) * (1.0f - 1e-6f) - r, 0.0f);
+    if (lo * lo * (1.0f - kSumRel) * (1.0f - 1e-6f) > thr) return false;
+    return exact_accept(a.guide, W, rxa[j], ry, cx, cy, thr);
+  };
+
+  // Candidate column of this lane in tile (cy, tc).  The lane order within a
+  // tile rotates with the row, so candidates along a vertical or diagonal
+  // structure spread over the lanes (each lane tracks its own best two).
+  auto col_of = [&](int cy, int tc) { return ux0 + 16 * tc + ((n + 7 * cy) & 15); };
+  // MFMA cross terms of the 16 references against the 16 candidates of tile
+  // (cy, tc).  Columns past the union are clamped to one of the same parity
+  // (their results are masked by the window tests).
+  // Tiles are visited row by row; the B fragments and norm of the next tile are
+  // loaded while the current one is evaluated.
+  struct TileLoad {
+    half8 B[6];
+    float nbv;
+  };
+  auto load_tile = [&](int cy, int tc, TileLoad& L) {
+    const int cx = col_of(cy, tc);
+    int cxl = cx;
+    if (cx > ux1) {
+      cxl = ux1 - ((cx - ux1) & 1);
+      if (cxl < X0) cxl += 2;  // one-column union: read past it (inside the pitch)
+    }
+    const int odd = (cxl - X0) & 1;
+    const _Float16* row = (odd ? c1 : c0) + (cy - Y0) * P + (cxl - X0 - odd);
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const int t = 4 * s + q, ch = t >> 3, dy = t & 7;
+      L.B[s] = load8a(row + ch * plane + dy * P);
+    }
+    L.nbv = nb[(cy - Y0) * PN + (min(cxl, X1) - X0)];
+  };
+  auto mfma_tile = [&](const TileLoad& L) -> floatx4 {
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 6; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[s], L.B[s], acc, 0, 0, 0);
+    return acc;
+  };
+  // visits every tile (cy, tc) of the union in scan order with its cross
+  // terms; two load buffers alternate so tile t+1 loads while t is evaluated
+  auto sweep = [&](auto&& body) {
+    const int nt = (uy1 - uy0 + 1) * ntc;
+    TileLoad L0, L1;
+    int cy = uy0, tc = 0;  // tile t
+    auto next = [&](int& y, int& c) {
+      if (++c == ntc) {
+        c = 0;
+        ++y;
+      }
+    };
+    load_tile(cy, tc, L0);
+    for (int t = 0; t < nt; t += 2) {
+      int cy1 = cy, tc1 = tc;
+      next(cy1, tc1);
+      if (t + 1 < nt) load_tile(cy1, tc1, L1);
+      {
+        const floatx4 acc = mfma_tile(L0);
+        body(cy, tc, col_of(cy, tc), acc, L0.nbv);
+      }
+      if (t + 1 >= nt) break;
+      int cy2 = cy1, tc2 = tc1;
+      next(cy2, tc2);
+      if (t + 2 < nt) load_tile(cy2, tc2, L0);
+      {
+        const floatx4 acc = mfma_tile(L1);
+        body(cy1, tc1, col_of(cy1, tc1), acc, L1.nbv);
+      }
+      cy = cy2;
+      tc = tc2;
+    }
+  };
+
+  // ---- pass 1: rejections, and per lane the two smallest d among accepted
+  // candidates inside the geometric cap.  Pairs that are surely accepted and
+  // cannot enter a lane's two smallest take no further work.
+  float b1d[4], b1n[4], b2d[4], b2n[4];
+  int rejl[4];
+  bool rejpre[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    b1d[j] = b2d[j] = kInf;
+    b1n[j] = b2n[j] = 0.0f;
+    rejl[j] = 0;
+    rejpre[j] = false;
+  }
+  // best-two update of lane-local tracking (branch-free)
+  auto track = [&](int j, bool on, float d, float nbv) {
+    const bool c1 = on && d < b1d[j], c2 = on && d < b2d[j];
+    b2n[j] = c1 ? b1n[j] : (c2 ? nbv : b2n[j]);
+    b2d[j] = c1 ? b1d[j] : (c2 ? d : b2d[j]);
+    b1n[j] = c1 ? nbv : b1n[j];
+    b1d[j] = c1 ? d : b1d[j];
+  };
+  if (wave_active) {
+    sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
+      const bool yin = yvalid && cy >= wy0 && cy <= wy1;
+      const bool selfrow = cy == ry;
+      const bool cin = yin && cx <= ux1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool inw = cin && vj[j] && cx >= wx0[j] && cx < wx0[j] + ew[j] && !(selfrow && cx == rxa[j]);
+        const float d = (na[j] + nbv) - 2.0f * acc[j];
+        const bool pre = (cy - wy0) * ew[j] + (cx - wx0[j]) <= limgeo[j];
+        const bool sure = d <= asure[j];
+        track(j, inw && sure && pre, d, nbv);
+        if (inw && !sure) {  // rare: the threshold is within reach
+          const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
+          const bool ok = classify(j, d, nbv, rb, cx, cy);
+          track(j, ok && pre, d, nbv);
+          if (!ok) {
+            ++rejl[j];
+            rejpre[j] = rejpre[j] || pre;
+            const int rr = cy - wy0;  // two 16-bit row counters per word
+            atomicAdd(&rejrow[(wave * 16 + 4 * q + j) * kRowWords + (rr >> 1)], (rr & 1) ? 65536u : 1u);
+          }
+        }
+      }
+    });
+  }
+  // per reference: accepted count and group size
+  int nacc[4];
+  bool anyrej[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    nacc[j] = npos[j] - group_sum(rejl[j]);
+    anyrej[j] = (__ballot(rejpre[j]) >> (16 * q)) & 0xffffull;
+  }
+  if (wave_active && n == 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int lim = min(a.max_group, 1 + min(nacc[j], cap));
+      int g = 1;
+      while (g * 2 <= lim) g *= 2;
+      meta[wave * 16 + 4 * q + j].g = g;
+      meta[wave * 16 + 4 * q + j].limit = nacc[j] > cap ? limgeo[j] : 0x7fffffff;
+      meta[wave * 16 + 4 * q + j].pad = (nacc[j] > cap && anyrej[j]) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  // ---- the cap when candidates before the geometric cap were rejected: walk
+  // the per-row rejection counts to the row holding the cap-th acceptance, then
+  // settle that row with the reference's own acceptance test (lane = column)
+  if (wave_active) {
+    for (int m = 0; m < 16; ++m) {
+      const int slot = wave * 16 + m;
+      if (!meta[slot].pad) continue;  // uniform: every lane reads the same flag
+      const int mrx = (wrx0 + (m & 3)) * st, mry = (wry0 + (m >> 2)) * st;
+      const int mx0 = max(mrx - half, 0), mw = min(mrx + half, wmax) - mx0 + 1;
+      const int my0 = max(mry - half, 0), mh = min(mry + half, hmax) - my0 + 1;
+      int seen = 0, row = 0;
+      for (; row < mh; ++row) {
+        const int inrow = mw - (my0 + row == mry ? 1 : 0) - (int)((rejrow[slot * kRowWords + (row >> 1)] >> (16 * (row & 1))) & 0xffffu);
+        if (seen + inrow >= cap) break;
+        seen += inrow;
+      }
+      // row < mh: the cap-th acceptance lies in this row
+      const int cy = my0 + row, cx = mx0 + lane;
+      bool ok = false;
+      if (row < mh && lane < mw && !(cy == mry && cx == mrx)) ok = exact_accept(a.guide, W, mrx, mry, cx, cy, thr);
+      const unsigned long long bal = __ballot(ok);
+      const int before = seen + __popcll(bal & ((1ull << lane) - 1ull));
+      if (ok && before == cap - 1) meta[slot].limit = row * mw + lane;
+    }
+  }
+  // ---- T: K-th smallest upper bound among the tracked candidates (ties by lane, slot)
+  if (wave_active && K > 0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float u1 = kInf, u2 = kInf;
+      if (b1d[j] != kInf) {
+        const float eps = kEpsRel * (na[j] + b1n[j]) + kEpsAbs;
+        const float f = hsqrt(fmaxf(b1d[j] + eps, 0.0f)) * (1.0f + 1e-6f) + ra[j] + kU * hsqrt(b1n[j]) * (1.0f + 1e-6f) + 1e-5f;
+        u1 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
+      }
+      if (b2d[j] != kInf) {
+        const float eps = kEpsRel * (na[j] + b2n[j]) + kEpsAbs;
+        const float f = hsqrt(fmaxf(b2d[j] + eps, 0.0f)) * (1.0f + 1e-6f) + ra[j] + kU * hsqrt(b2n[j]) * (1.0f + 1e-6f) + 1e-5f;
+        u2 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
+      }
+      int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
+      const float e1 = b1d[j], e2 = b2d[j];
+      for (int k = 0; k < 16; ++k) {
+        const float o1 = __shfl(u1, 16 * q + k, 64), o2 = __shfl(u2, 16 * q + k, 64);
+        r1 += (o1 < u1 || (o1 == u1 && k < n)) + (o2 < u1 || (o2 == u1 && k < n));
+        r2 += (o1 < u2 || (o1 == u2 && k <= n)) + (o2 < u2 || (o2 == u2 && k < n));
+        const float p1 = __shfl(e1, 16 * q + k, 64), p2 = __shfl(e2, 16 * q + k, 64);
+        s1 += (p1 < e1 || (p1 == e1 && k < n)) + (p2 < e1 || (p2 == e1 && k < n));
+        s2 += (p1 < e2 || (p1 == e2 && k <= n)) + (p2 < e2 || (p2 == e2 && k < n));
+      }
+      if (r1 == K - 1) meta[wave * 16 + 4 * q + j].d15 = u1;
+      if (r2 == K - 1) meta[wave * 16 + 4 * q + j].d15 = u2;
+      if (s1 == K - 1) meta[wave * 16 + 4 * q + j].dK = e1;
+      if (s2 == K - 1) meta[wave * 16 + 4 * q + j].dK = e2;
+    }
+  }
+  __syncthreads();
+
+  // ---- pass 2: collect every capped, accepted candidate whose lower bound reaches T.
+  // A candidate's lower bound is <= T exactly when d <= (sT + r)^2 + eps with
+  // sT = sqrt(T / ((1 - kSumRel)(1 - 1e-6))).
+  const int UW = ux1 - ux0 + 1;
+  if (wave_active) {
+    float sT[4], dK[4];
+    int limit[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const RefMeta& mt = meta[wave * 16 + 4 * q + j];
+      limit[j] = mt.limit;
+      if (K <= 0) sT[j] = -1.0f;                                        // group of one: nothing to rank
+      else if (min(nacc[j], cap) <= K || mt.d15 == kInf) sT[j] = kInf;  // every accepted candidate
+      else sT[j] = hsqrt(mt.d15 / ((1.0f - kSumRel) * (1.0f - 1e-6f))) * (1.0f + 1e-6f);
+      dK[j] = K <= 0 ? -kInf : (sT[j] == kInf ? kInf : mt.dK);
+    }
+    sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
+      const bool yin = yvalid && cy >= wy0 && cy <= wy1;
+      const bool selfrow = cy == ry;
+      const bool cin = yin && cx <= ux1;
+      const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool inw = cin && vj[j] && cx >= wx0[j] && cx < wx0[j] + ew[j] && !(selfrow && cx == rxa[j]);
+        const float d = (na[j] + nbv) - 2.0f * acc[j];
+        const float f = sT[j] + ra[j] + rb;
+        const float eps = kEpsRel * (na[j] + nbv) + kEpsAbs;
+        const bool inA = d <= f * f * (1.0f + 1e-5f) + eps;  // lower bound reaches T
+        const bool inB = d <= dK[j];                          // approximate top K
+        if (inw && (inA || inB) && (cy - wy0) * ew[j] + (cx - wx0[j]) <= limit[j] &&
+            (d <= asure[j] || classify(j, d, nbv, rb, cx, cy))) {
+          const int slot = wave * 16 + 4 * q + j;
+          const uint16_t li = (uint16_t)((cy - uy0) * UW + (cx - ux0));
+          if (inA) {
+            const int pos = atomicAdd(&lcnt[slot], 1);
+            if (pos < kListCap) list[slot * kListCap + pos] = li;
+          }
+          if (inB) {
+            const int pos = atomicAdd(&lcntb[slot], 1);
+            if (pos < kListCapB) listb[slot * kListCapB + pos] = li;
+          }
+        }
+      }
+    });
+  }
+  __syncthreads();  // c1 is dead: its space holds the exact distances
+
+  // ---- exact re-rank of the lists; entries of the wave's 16 references are
+  // spread over the 64 lanes.  A reference uses its exact-band list when that
+  // fit; otherwise the approximate top-K list (mixed-precision selection);
+  // otherwise it goes to the plain matcher.
+  int nms = 0, offs = 0, total = 0, usea = 1;
+  if (wave_active) {
+    const int m = lane & 15;
+    const bool valid = (m & 3) < wnx && (m >> 2) < wny;
+    const int ca = valid ? lcnt[wave * 16 + m] : 0, cb = valid ? lcntb[wave * 16 + m] : 0;
+    usea = ca <= kListCap;
+    nms = usea ? ca : (cb <= kListCapB ? cb : 0);
+    for (int k = 0; k < 16; ++k) {
+      const int v = __shfl(nms, k, 64);
+      offs += k < m ? v : 0;
+      total += v;
+    }
+  }
+  // entry e of the flattened lists -> (reference rm, index idx, list position)
+  auto entry = [&](int e, int& rm, int& idx, int& nm) -> int {
+    rm = 0;
+    for (int k = 0; k < 16; ++k) {
+      const int ok = __shfl(offs, k, 64), nk = __shfl(nms, k, 64);
+      if (e >= ok && e < ok + nk) rm = k;
+    }
+    idx = e - __shfl(offs, rm, 64);
+    nm = __shfl(nms, rm, 64);
+    return __shfl(usea, rm, 64);
+  };
+  if (wave_active) {
+    for (int base = 0; base < total; base += 64) {
+      const int e = base + lane;
+      int rm, idx, nm;
+      const int ua = entry(e, rm, idx, nm);
+      if (e < total) {
+        const int slot = wave * 16 + rm;
+        const int li = ua ? list[slot * kListCap + idx] : listb[slot * kListCapB + idx];
+        const int py = uy0 + li / UW, px = ux0 + li % UW;
+        const int rrx = (wrx0 + (rm & 3)) * st, rry = (wry0 + (rm >> 2)) * st;
+        tot[slot * kListCap + idx] = exact_total(a.guide, W, rrx, rry, px, py);
+      }
+    }
+  }
+  __syncthreads();
+  if (wave_active) {
+    for (int base = 0; base < total; base += 64) {
+      const int e = base + lane;
+      int rm, idx, nm;
+      const int ua = entry(e, rm, idx, nm);
+      if (e < total) {
+        const int slot = wave * 16 + rm;
+        const uint16_t* lst = ua ? list + slot * kListCap : listb + slot * kListCapB;
+        const int li = lst[idx];
+        const int py = uy0 + li / UW, px = ux0 + li % UW;
+        const float t = tot[slot * kListCap + idx];
+        const int rrx = (wrx0 + (rm & 3)) * st, rry = (wry0 + (rm >> 2)) * st;
+        const int ex0 = max(rrx - half, 0), ex1 = min(rrx + half, wmax), ey0 = max(rry - half, 0);
+        const int eww = ex1 - ex0 + 1;
+        const int myscan = (py - ey0) * eww + (px - ex0);
+        int rank = 0;
+        for (int k = 0; k < nm; ++k) {
+          const int lk = lst[k];
+          const float tk = tot[slot * kListCap + k];
+          const int sk = (uy0 + lk / UW - ey0) * eww + (ux0 + lk % UW - ex0);
+          rank += (tk < t || (tk == t && sk < myscan)) ? 1 : 0;
+        }
+        const int g = meta[slot].g;
+        const int ref_id = (wry0 + (rm >> 2)) * a.refs_x + wrx0 + (rm & 3);
+        if (rank < g - 1) a.groups[(size_t)ref_id * 17 + 2 + rank] = (uint32_t)px | ((uint32_t)py << 16);
+      }
+    }
+    if (lane < 16) {
+      const int m = lane;
+      if ((m & 3) < wnx && (m >> 2) < wny) {
+        const int slot = wave * 16 + m;
+        const int rrx = (wrx0 + (m & 3)) * st, rry = (wry0 + (m >> 2)) * st;
+        const int ref_id = (wry0 + (m >> 2)) * a.refs_x + wrx0 + (m & 3);
+        if (lcnt[slot] > kListCap && lcntb[slot] > kListCapB) {  // both lists overflowed: plain matcher
+          const uint32_t at = atomicAdd(a.overflow_count, 1u);
+          a.overflow_list[at] = (uint32_t)ref_id;
+        } else {
+          if (lcnt[slot] > kListCap) atomicAdd(a.overflow_count + 1, 1u);  // approximate selection
+          a.groups[(size_t)ref_id * 17] = (uint32_t)meta[slot].g;
+          a.groups[(size_t)ref_id * 17 + 1] = (uint32_t)rrx | ((uint32_t)rry << 16);
+        }
+      }
+    }
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace
+
+Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window) {
+  Bm3dMfmaArgs a{};
+  const int half = window / 2;
+  a.spx = std::min(7 * stride + 2 * half + 8, width);
+  a.spy = std::min(7 * stride + 2 * half + 8, height);
+  a.pitch = (a.spx + 1 + 1) & ~1;  // even, with room for reads one column past the staged span
+  auto up16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
+  const size_t copy = (size_t)3 * a.spy * a.pitch * 2;
+  const size_t hs = (size_t)a.spy * (a.spx - 7) * 4;
+  const size_t tot = (size_t)XW * 16 * kListCap * 4;
+  size_t off = 0;
+  a.off_w16 = 0;
+  off = up16(copy);
+  a.off_w16b = (int)off;  // second copy; also the setup scratch and the exact distances
+  off = up16(off + std::max(copy, std::max(hs, tot)));
+  a.off_nb = (int)off;
+  off = up16(off + (size_t)(a.spy - 7) * (a.spx - 7) * 4);
+  a.off_list = (int)off;
+  off = up16(off + (size_t)XW * 16 * (kListCap + kListCapB) * 2);
+  a.off_cnt = (int)off;
+  off = up16(off + XW * 16 * 4 * 2);
+  a.off_meta = (int)off;
+  off = up16(off + XW * 16 * sizeof(RefMeta));
+  a.off_red = (int)off;
+  off = up16(off + 4 * XW * 4);
+  a.smem = (int)off;
+  // tile columns are tracked in a 32-bit mask (8 tiles of 16 candidates);
+  // list entries index the wave's candidate union in 16 bits
+  const int uw = 3 * stride + 2 * half + 1;
+  a.max_tiles_ok = uw <= 128 && (size_t)uw * uw < 65536;
+  return a;
+}
+
+hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.overflow_count, (size_t)2, 0u);
+  hipLaunchKernelGGL(bm3d_match_mfma_kernel, dim3(a.n_blocks), dim3(XT), (size_t)a.smem, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace imops

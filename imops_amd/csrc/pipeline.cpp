@@ -345,6 +345,10 @@ Stage make_stage(const Table& t) {
         param(t, "lambda") ? num_param(t, "lambda", 0, "BM3D") : -1.0,
         (int)num_param(t, "stride", 6, "BM3D"), (int)num_param(t, "search_window", 19, "BM3D"),
         (int)num_param(t, "max_group", 16, "BM3D"));
+    const std::string prec = str_param(t, "precision", "fp32", "BM3D");
+    if (prec == "fp32") s.bm3d.precision = IMOPS_BM3D_FP32;
+    else if (prec == "mixed") s.bm3d.precision = IMOPS_BM3D_MIXED;
+    else throw Error{IMOPS_E_INVALID, "Invalid BM3D config"};
   } else if (s.name == "Film") {
     s.kind = K_FILM;
     s.film = film_stage_params_from_table(str_param(t, "stock", "ColorNeg200", "Film"),

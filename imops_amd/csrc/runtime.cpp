@@ -53,6 +53,7 @@ void KernelTimer::clear() {
       (void)hipEventDestroy(e.second);
     }
   regions.clear();
+  counts.clear();
 }
 
 TimedRegion::TimedRegion(imops_ctx* ctx, const char* name) : ctx_(ctx) {
@@ -353,6 +354,11 @@ int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* 
           total += ms;
           ++n;
         }
+    for (auto& c : ctx->timer.counts)
+      if (c.first == region) {
+        total += c.second;
+        ++n;
+      }
     *total_ms = total;
     *count = n;
   });

@@ -54,6 +54,9 @@ struct KernelTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
   };
   std::vector<Region> regions;
+  // Named event counts recorded while timing is on (e.g. patches a stage
+  // handed to a slower path); reported through imops_ctx_timing as the sum.
+  std::vector<std::pair<std::string, double>> counts;
   Region& region(const std::string& name);
   void clear();
 };

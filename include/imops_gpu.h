@@ -43,6 +43,10 @@ extern "C" {
 #define IMOPS_CS_ACESCG 2
 #define IMOPS_CS_XYZD65 3
 
+/* BM3D arithmetic (imops_bm3d_params.precision). */
+#define IMOPS_BM3D_FP32 0
+#define IMOPS_BM3D_MIXED 1
+
 /* Demosaic algorithms (DemosaicAlgorithmType, modules.rs:73-100). */
 #define IMOPS_DEMOSAIC_AMAZE 0
 #define IMOPS_DEMOSAIC_MARKESTEIJN 1
@@ -66,8 +70,11 @@ void* imops_ctx_stream(imops_ctx* ctx);
  * "bm3d_match", "bm3d_filter_ht", "bm3d_filter_wiener", "film").  Enabling or
  * disabling clears the recorded events; imops_ctx_timing synchronises the
  * stream and returns the summed milliseconds and the number of recorded
- * launches of one region.  No counterpart in the reference (its CLI prints
- * wall-clock pipeline time only, src/bin/imops-cli.rs:59-63). */
+ * launches of one region.  The name "bm3d_fallback_refs" instead sums the
+ * reference patches the mixed-precision matcher handed to the FP32 matcher
+ * (recording it synchronises the stream once per BM3D step while timing is on).
+ * No counterpart in the reference (its CLI prints wall-clock pipeline time
+ * only, src/bin/imops-cli.rs:59-63). */
 int imops_ctx_set_timing(imops_ctx* ctx, int enabled);
 int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* count);
 
@@ -145,6 +152,9 @@ typedef struct imops_bm3d_params {
   int search_window;    /* odd, <= 41 (19) */
   int max_group;        /* 1..16, groups are the largest power of two <= min(this, matches) (16) */
   int max_candidates;   /* accepted-candidate cap in scan order (1024) */
+  int precision;        /* IMOPS_BM3D_FP32 (0): plain FP32 throughout;
+                           IMOPS_BM3D_MIXED (1): FP16 matrix-core block matching with
+                           FP32 re-ranking (same groups as FP32) */
 } imops_bm3d_params;
 
 /* Fill `out` with the parameters intensity maps to (from_intensity). */
