@@ -7,8 +7,11 @@ scene generator, resident in HBM as u16.  A step is one pass of
   sigma 0.02} -> CST{AcesCg} -> Exp{ev = log2 800} -> Film{ColorNeg200} ->
   CST{Srgb}
 
-in plain FP32 (the reference's arithmetic), from the u16 frame to the f32
-interleaved sRGB output in HBM.  Prints one JSON line.
+from the u16 frame to the f32 interleaved sRGB output in HBM.  `value` is the
+plain FP32 form (the reference's arithmetic); the same stage list with
+BM3D{precision = "mixed"} (FP16 matrix-core block matching) is timed beside it
+and reported under "mixed" with its PSNR against the FP32 form (full frame)
+and against the FP64 oracle (on the CPU-baseline crop).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
@@ -33,23 +36,26 @@ BM3D_SIGMA = 0.02            # normalised noise sigma
 BM3D_STRIDE, BM3D_WINDOW, BM3D_LAMBDA = 3, 39, 2.7
 EV = math.log2(800.0)        # relative -> absolute luminance (sunny-16 gain) for the film stage
 FP32_PEAK_TFLOPS = 157.3
+FP16_PEAK_TFLOPS = 2500.0    # dense FP16 MFMA
+CROP_W, CROP_H = 768, 512
 # HBM bytes per bm3d_match_kernel launch at this workload, from the FETCH_SIZE / WRITE_SIZE
 # counter runs summarised in profiles/full_chain_6016x4016_kernel_stats.md (FETCH scaled by
 # 2.000 and WRITE by 1.000, the factors a 1 GiB copy kernel reads in the same runs).
 MATCH_HBM_BYTES = (2 * 1137133.375 + 250875.6875) * 1024
 
-STAGES = f"""
+STAGES_TMPL = """
 [[pipeline_modules]]
 name = "Demosaic"
 algorithm = "amaze"
 
 [[pipeline_modules]]
 name = "BM3D"
-sigma = {BM3D_SIGMA}
-lambda = {BM3D_LAMBDA}
-stride = {BM3D_STRIDE}
-search_window = {BM3D_WINDOW}
+sigma = {sigma}
+lambda = {lam}
+stride = {stride}
+search_window = {window}
 max_group = 16
+precision = "{precision}"
 
 [[pipeline_modules]]
 name = "CST"
@@ -57,7 +63,7 @@ target_color_space = "AcesCg"
 
 [[pipeline_modules]]
 name = "Exp"
-ev = {EV!r}
+ev = {ev}
 
 [[pipeline_modules]]
 name = "Film"
@@ -70,6 +76,13 @@ output = "NegativeLinear"
 name = "CST"
 target_color_space = "Srgb"
 """
+
+
+def stages(precision: str) -> str:
+    return STAGES_TMPL.format(sigma=BM3D_SIGMA, lam=BM3D_LAMBDA, stride=BM3D_STRIDE, window=BM3D_WINDOW,
+                              precision=precision, ev=repr(EV))
+
+
 STAGE_NAMES = ["Demosaic{amaze}", f"BM3D{{stride {BM3D_STRIDE}, window {BM3D_WINDOW}, lambda {BM3D_LAMBDA}, "
                f"sigma {BM3D_SIGMA}}}", "CST{AcesCg}", "Exp{log2 800}", "Film{ColorNeg200}", "CST{Srgb}"]
 
@@ -99,29 +112,40 @@ def match_flops(w: int, h: int, stride: int, window: int) -> float:
     return cands * 192 * 2
 
 
-def cpu_baseline(raw: np.ndarray, cal: np.ndarray, budget_s: float = 15.0) -> dict:
-    """The oracle (C++ restatement, all host cores via OpenMP) on a crop of the same frame."""
+def psnr(a: np.ndarray, b: np.ndarray) -> float:
+    mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10.0 * math.log10(1.0 / mse)
+
+
+def oracle_chain(raw_crop: np.ndarray, cal: np.ndarray, fp64: bool) -> np.ndarray:
     import oracle
-    cw, ch = 768, 512
     p = oracle.bm3d_params(BM3D_SIGMA * 255 / 80, sigma=BM3D_SIGMA * 255, stride=BM3D_STRIDE, window=BM3D_WINDOW)
     p["lambda"] = BM3D_LAMBDA
+    plane = oracle.normalize_raw(raw_crop, BLACK, WHITE)
+    x = oracle.amaze(plane, "RGGB", fp64=fp64)
+    x, _ = oracle.bm3d(x, p, fp64=fp64)
+    x = oracle.camera_cst(x, cal.ravel(), oracle.CS_ACESCG)
+    x = oracle.exp(x, np.float32(EV))
+    x = oracle.film(x, seed=1, fp64=fp64)
+    return oracle.cst(x, oracle.CS_ACESCG, oracle.CS_SRGB)
+
+
+def cpu_baseline(raw: np.ndarray, cal: np.ndarray, budget_s: float = 15.0):
+    """The oracle (C++ restatement, all host cores via OpenMP) on a crop of the same frame;
+    also returns the FP64 oracle's output for that crop."""
+    crop = np.ascontiguousarray(raw[:CROP_H, :CROP_W])
     times = []
     t_end = time.perf_counter() + budget_s
     while not times or (time.perf_counter() < t_end and len(times) < 5):
         t0 = time.perf_counter()
-        plane = oracle.normalize_raw(raw, BLACK, WHITE, crop=(0, 0, cw, ch))
-        x = oracle.amaze(plane, "RGGB")
-        x, _ = oracle.bm3d(x, p)
-        x = oracle.camera_cst(x, cal.ravel(), oracle.CS_ACESCG)
-        x = oracle.exp(x, np.float32(EV))
-        x = oracle.film(x, seed=1)
-        oracle.cst(x, oracle.CS_ACESCG, oracle.CS_SRGB)
+        oracle_chain(crop, cal, fp64=False)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(cw * ch / 1e6 / t, 4), "unit": "MP/s", "cores": cores, "kind": "port",
-            "sample": f"{cw}x{ch} crop of the same frame through the same stage list, {len(times)} run(s), "
-                      f"median {t:.2f} s"}
+    base = {"value": round(CROP_W * CROP_H / 1e6 / t, 4), "unit": "MP/s", "cores": cores, "kind": "port",
+            "sample": f"{CROP_W}x{CROP_H} crop of the same frame through the same stage list (FP32 oracle), "
+                      f"{len(times)} run(s), median {t:.2f} s"}
+    return base, oracle_chain(crop, cal, fp64=True)
 
 
 def main() -> int:
@@ -156,45 +180,57 @@ def main() -> int:
     torch.cuda.synchronize()
 
     img = im.Image(ctx).set_calibration_matrix(cal.ravel())
-    cfg = im.parse_config(STAGES)
     stream = torch.cuda.ExternalStream(ctx.stream)
+    region_names = ("normalize", "amaze", "bm3d", "bm3d_match", "bm3d_filter_ht", "bm3d_filter_wiener", "pointwise",
+                    "film")
 
-    def step():
-        img.set_raw_u16(d_raw, "RGGB", BLACK, WHITE)
-        im.run_pixel_pipeline(img, cfg)
+    def run_form(precision: str):
+        cfg = im.parse_config(stages(precision))
 
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    torch.cuda.synchronize()
+        def step():
+            img.set_raw_u16(d_raw, "RGGB", BLACK, WHITE)
+            im.run_pixel_pipeline(img, cfg)
 
-    ctx.set_timing(True)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for k in range(args.steps):
-        step()
-        print(f"[bench] step {k + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
-    ev1.record(stream)
-    ev1.synchronize()
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
-    regions = {}
-    for r in ("normalize", "amaze", "bm3d", "bm3d_match", "bm3d_filter_ht", "bm3d_filter_wiener", "pointwise",
-              "film"):
-        tot, n = ctx.timing(r)
-        regions[r] = {"ms_per_step": round(tot / args.steps, 3), "launches": n}
-    ctx.set_timing(False)
-    print(f"[bench] {args.steps} steps, {ms:.2f} ms/step (wall {wall / args.steps * 1e3:.2f} ms); "
-          f"regions {json.dumps(regions)}", file=sys.stderr, flush=True)
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for k in range(args.steps):
+            step()
+            print(f"[bench] {precision} step {k + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
+        ev1.record(stream)
+        ev1.synchronize()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = ev0.elapsed_time(ev1) / args.steps
+        regions = {}
+        for r in region_names:
+            tot, n = ctx.timing(r)
+            regions[r] = {"ms_per_step": round(tot / args.steps, 3), "launches": n}
+        extra = {r: ctx.timing(r)[0] / args.steps for r in ("bm3d_fallback_refs", "bm3d_approx_refs")}
+        ctx.set_timing(False)
+        print(f"[bench] {precision}: {args.steps} steps, {ms:.2f} ms/step (wall {wall / args.steps * 1e3:.2f} ms); "
+              f"regions {json.dumps(regions)}", file=sys.stderr, flush=True)
+        return ms, regions, extra, img.rgb()
+
+    ms, regions, _, out_fp32 = run_form("fp32")
+    ms_mx, regions_mx, extra_mx, out_mixed = run_form("mixed")
+    psnr_full = psnr(out_mixed, out_fp32)
+    del out_fp32, out_mixed
 
     # Dominant kernel: block matching (two launches per step, one per BM3D pass).
     n = regions["bm3d_match"]["launches"]
     match_ms = regions["bm3d_match"]["ms_per_step"] * args.steps / max(n, 1)
     flops = match_flops(W, H, BM3D_STRIDE, BM3D_WINDOW)
     achieved = flops / (match_ms * 1e-3) / 1e12
+    n_mx = regions_mx["bm3d_match"]["launches"]
+    match_ms_mx = regions_mx["bm3d_match"]["ms_per_step"] * args.steps / max(n_mx, 1)
+    achieved_mx = flops / (match_ms_mx * 1e-3) / 1e12
 
     out = {
         "metric": "24MP raw->sRGB MP/s (plain FP32 full pipeline)",
@@ -216,10 +252,28 @@ def main() -> int:
                      "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": MATCH_HBM_BYTES,
                      "kernel_ms": round(match_ms, 3), "algorithmic_flops": flops},
+        "mixed": {"value": round(W * H / 1e6 / (ms_mx * 1e-3), 3), "unit": "MP/s", "ms_per_step": round(ms_mx, 3),
+                  "speedup_vs_fp32": round(ms / ms_mx, 3),
+                  "form": "BM3D block matching on FP16 MFMA (FP32 accumulate) with FP32 re-rank; other stages FP32",
+                  "psnr_db_vs_fp32_form_full_frame": None if math.isinf(psnr_full) else round(psnr_full, 2),
+                  "bit_identical_to_fp32_form": math.isinf(psnr_full),
+                  "bm3d_patches_fallback_per_step": extra_mx["bm3d_fallback_refs"],
+                  "bm3d_patches_approximate_per_step": extra_mx["bm3d_approx_refs"],
+                  "stage_ms": regions_mx,
+                  "roofline": {"bound": "mfma", "kernel": "bm3d_match_mfma_kernel + fallback (FP16 MFMA)",
+                               "achieved": round(achieved_mx, 3), "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": round(achieved_mx / FP16_PEAK_TFLOPS, 5), "kernel_ms": round(match_ms_mx, 3),
+                               "algorithmic_flops": flops}},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(raw, cal)
+        out["cpu_baseline"], crop_fp64 = cpu_baseline(raw, cal)
+        crop = np.ascontiguousarray(raw[:CROP_H, :CROP_W])
+        cimg = im.Image(ctx).set_calibration_matrix(cal.ravel())
+        cimg.set_raw_u16(crop, "RGGB", BLACK, WHITE)
+        im.run_pixel_pipeline(cimg, im.parse_config(stages("mixed")))
+        out["mixed"]["psnr_db_vs_fp64_oracle_crop"] = round(psnr(cimg.rgb(), crop_fp64), 2)
+        out["mixed"]["psnr_crop"] = f"{CROP_W}x{CROP_H} crop; oracle runs AMaZE, BM3D and film in FP64"
     print(json.dumps(out), flush=True)
     return 0
 
