@@ -148,6 +148,39 @@ def cpu_baseline(raw: np.ndarray, cal: np.ndarray, budget_s: float = 15.0):
     return base, oracle_chain(crop, cal, fp64=True)
 
 
+def film_adjoint_bench(ctx, n=8, size=512, steps=200) -> dict:
+    """Film-model adjoint: 8 ACEScg 512x512 crops of seeded scenes, target rendered with
+    hidden perturbed parameters, Adam from the nominal stock on the L2 loss.  Reports the
+    forward + adjoint time per iteration (HIP events) and the loss reduction."""
+    import torch
+    import imops_amd as im
+    from imops_amd.film_fit import FilmModel, fit
+    g = 12.5 * 64.0 / ((1.0 / 200.0) * 200.0)  # sunny-16 absolute luminance of relative 1.0 at ISO 200
+    x = torch.from_numpy(np.stack([im.synth_scene_rgb(size, size, 100 + i) * np.float32(g) for i in range(n)])).cuda()
+    model = FilmModel(ctx)
+    nom = model.nominal()
+    rng = np.random.default_rng(7)
+    hidden = {"sens": nom["sens"] * torch.tensor(1 + 0.15 * rng.uniform(-1, 1, nom["sens"].shape), dtype=torch.float32),
+              "d_max": nom["d_max"] + 0.2 * torch.tensor(rng.uniform(-1, 1, nom["d_max"].shape), dtype=torch.float32),
+              "gamma": nom["gamma"] * torch.tensor(1 + 0.1 * rng.uniform(-1, 1, nom["gamma"].shape), dtype=torch.float32),
+              "hal_strength": torch.tensor(1.3), "hal_radius": torch.tensor(0.95)}
+    with torch.no_grad():
+        target = model(x, **hidden)
+    fit(model, x, target, nom, steps=3)  # warm-up
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    res = fit(model, x, target, nom, steps=steps)
+    ev1.record(stream)
+    ev1.synchronize()
+    return {"metric": "film-model adjoint ms/iter (forward + hand-written adjoint + Adam step)",
+            "ms_per_iter": round(ev0.elapsed_time(ev1) / steps, 3),
+            "ms_per_iter_fwd_bwd_host_median": round(res["ms_per_iter"], 3),
+            "batch": f"{n}x{size}x{size} ACEScg crops", "steps": steps,
+            "loss_first": res["loss"][0], "loss_last": res["loss"][-1],
+            "gradient_check": "tests/test_gpu_film_fit.py: vector relative error vs FP64 central differences"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +299,7 @@ def main() -> int:
                                "algorithmic_flops": flops}},
         "cpu_baseline": None,
     }
+    out["film_adjoint"] = film_adjoint_bench(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"], crop_fp64 = cpu_baseline(raw, cal)
         crop = np.ascontiguousarray(raw[:CROP_H, :CROP_W])

@@ -40,6 +40,12 @@ class Bm3dParams(C.Structure):
                 ("max_candidates", C.c_int), ("precision", C.c_int)]
 
 
+class FilmFitParams(C.Structure):
+    """imops_film_fit_params (include/imops_gpu.h)."""
+    _fields_ = [("n_emulsions", C.c_int), ("sens", (C.c_float * 16) * 8), ("d_max", C.c_float * 8),
+                ("gamma", C.c_float * 8), ("hal_strength", C.c_float), ("hal_radius", C.c_float)]
+
+
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
     "imops_last_error": (C.c_char_p, []),
@@ -70,6 +76,10 @@ SIGNATURES = {
     "imops_bm3d": (_i, [_vp, _vp, _f]),
     "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
+    "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
+    "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz]),
+    "imops_film_fit_backward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz,
+                                     C.POINTER(FilmFitParams)]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
     "imops_pipeline_num_stages": (_sz, [_vp]),

@@ -322,6 +322,15 @@ hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out,
   return hipGetLastError();
 }
 
+hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
+                      int radius, hipStream_t s) {
+  const size_t n = (size_t)w * h;
+  hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps_h, radius);
+  hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, nullptr, out, w, h, taps_v, radius,
+                     (int)BLUR_STORE, 0.0f);
+  return hipGetLastError();
+}
+
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
                                float* image, float* mask, hipStream_t s) {
   hipLaunchKernelGGL(capture_reduce_kernel, dim3(grid1d(n)), dim3(256), 0, s, absorbed, bounce, n, T, image, mask);

@@ -40,9 +40,29 @@ struct FilmDeviceTables {
   float inv_dmin[3], inv_shoulder[3], inv_gain[3];
 };
 
+// Differentiable film model (film_fit.hip).
+struct FitExposeConsts {
+  double minv[3][3];                 // ACEScg -> basis weights
+  double c[3][kFilmMaxEmulsions];    // band-absorbed fluence of emulsion e per unit basis weight k
+  int n_emulsions;
+};
+
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s);
+hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s);
+// out = c0 * orig + c1 * (V o H)^T in   (the transpose of film_blur's reflect-101 Gaussian; radius <= w-1, h-1)
+hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, float* out, int w, int h,
+                             const float* taps, int radius, float c0, float c1, hipStream_t s);
+hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, const float* gout, size_t n,
+                              float* sbar, float* bbar, double* acc, hipStream_t s);
+hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k, double* acc,
+                                hipStream_t s);
+hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* acc, hipStream_t s);
+hipError_t film_fit_zero(double* p, int n, hipStream_t s);
 hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps,
                      int radius, int mode, float a, hipStream_t s);
+// separable blur with different horizontal and vertical taps (same radius), plain store
+hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
+                      int radius, hipStream_t s);
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
                                float* image, float* mask, hipStream_t s);
 hipError_t film_dir_apply(float* image, const float* inh, size_t n, const FilmDeviceTables* T, hipStream_t s);

@@ -116,6 +116,9 @@ struct HostStock {
   bool wide_reflect = false;                 // any backing reflectance > 0 at the four bands
   bool any_reversal = false;
   double dmin_raw[3] = {0, 0, 0};
+  double alpha[kFilmMaxLayers] = {};  // emulsion layers: optical density per unit sensitivity (2 rho t)
+  int layer_of[kFilmMaxLayers] = {};   // stock layer of each absorbing layer
+  double reflect[NB] = {};            // antihalation backing reflectance samples
 };
 
 HostStock build_stock(const Stock* st) {
@@ -153,7 +156,7 @@ HostStock build_stock(const Stock* st) {
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) T.xyz2ap1[r][c] = kXyz2Ap1[r][c];
   sample_curve(st->scanner_light, T.scanner);
-  double reflect[NB];
+  double* reflect = h.reflect;
   sample_curve(st->ah_reflectance, reflect);
   const double bands[4] = {450.0, 550.0, 650.0, 700.0};
   float max_r = 0.0f;
@@ -168,6 +171,7 @@ HostStock build_stock(const Stock* st) {
     double curve[NB];
     sample_curve(L.sensitivity, curve);
     const int a = T.n_absorbing++;
+    h.layer_of[a] = l;
     const double thickness = (double)L.thickness_um;
     for (int i = 0; i < NB; ++i) {
       const double od = L.kind == L_EMULSION ? (curve[i] * 2.0) * (double)L.silver_halide_fraction * thickness
@@ -176,6 +180,7 @@ HostStock build_stock(const Stock* st) {
     }
     T.is_emulsion[a] = L.kind == L_EMULSION;
     if (L.kind != L_EMULSION) continue;
+    h.alpha[a] = 2.0 * (double)L.silver_halide_fraction * thickness;
     FilmEmulsionTables& E = T.em[e];
     const double mu = std::log(L.crystal_mean_um), sg = L.crystal_sigma_ln;
     build_lut(mu, sg, L.capture_k, E.lut_log, E.lut_frac);
@@ -460,6 +465,225 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   img->color_space = IMOPS_CS_ACESCG;
 }
 
+// ---------------------------------------------------------------- film fit
+// The differentiable film model: the film stage's expose -> local scatter ->
+// halation -> capture -> reduction -> scan (NegativeLinear, shutter 1/box ISO)
+// with grain, DIR inhibition and adjacency off, the emulsion sensitivities,
+// D_max and gamma as parameters, the halation gain and PSF width scaled by two
+// more, and the scan normalised by the nominal stock's Dmin.
+namespace {
+
+struct FitSetup {
+  FilmDeviceTables T;                 // stock tables with the parameter-dependent fields
+  FitExposeConsts k;
+  double W[3][NB];                    // trapezoid weight * basis_k / e_rel / 300
+  double tab[kFilmMaxEmulsions][NB];  // transmittance of every absorbing layer above emulsion e
+  double te[kFilmMaxEmulsions][NB];   // transmittance of emulsion e
+  double alpha[kFilmMaxEmulsions];
+  int above[kFilmMaxEmulsions][kFilmMaxEmulsions];  // [e'][e]: emulsion e' lies above emulsion e
+  float rb[kFilmMaxEmulsions];        // r_e * bleed_e
+  double bleed[kFilmMaxEmulsions], rnum[kFilmMaxEmulsions], rden[kFilmMaxEmulsions];
+  float sigma_l, sigma_h, sigma_h_unit;
+  bool do_local, do_wide;
+};
+
+double trap_w(int i) { return (i == 0 || i == NB - 1) ? 10.0 : 20.0; }
+
+FitSetup fit_setup(const HostStock& hs, const imops_film_fit_params& p, float pitch) {
+  FitSetup f{};
+  const Stock& st = *hs.s;
+  const int E = hs.t.n_emulsions;
+  if (p.n_emulsions != E) throw Error{IMOPS_E_INVALID, "film_fit: n_emulsions does not match the stock"};
+  f.T = hs.t;
+  f.T.invert = 0;
+  // sensitivities -> per-layer transmittances, C matrix (exposure/mod.rs:68-81 collapsed over the basis)
+  double trans[kFilmMaxLayers][NB];
+  int emul_of_layer[kFilmMaxLayers];
+  int e = 0;
+  for (int a = 0; a < hs.t.n_absorbing; ++a) {
+    emul_of_layer[a] = -1;
+    if (!hs.t.is_emulsion[a]) {
+      for (int i = 0; i < NB; ++i) trans[a][i] = hs.t.trans[a][i];
+      continue;
+    }
+    const Layer& L = st.layers[hs.layer_of[a]];
+    for (int i = 0; i < NB; ++i)
+      trans[a][i] = std::exp(-((double)p.sens[e][i] * 2.0) * (double)L.silver_halide_fraction * (double)L.thickness_um);
+    f.alpha[e] = hs.alpha[a];
+    emul_of_layer[a] = e++;
+  }
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < NB; ++i) f.W[k][i] = trap_w(i) * hs.t.basis[k][i] / hs.t.e_rel[i] / 300.0;
+  double above_t[NB];
+  for (int i = 0; i < NB; ++i) above_t[i] = 1.0;
+  for (int a = 0; a < hs.t.n_absorbing; ++a) {
+    const int em = emul_of_layer[a];
+    if (em >= 0) {
+      for (int i = 0; i < NB; ++i) {
+        f.tab[em][i] = above_t[i];
+        f.te[em][i] = trans[a][i];
+      }
+      for (int e2 = 0; e2 < em; ++e2) f.above[e2][em] = 1;
+    }
+    for (int i = 0; i < NB; ++i) above_t[i] *= trans[a][i];
+  }
+  for (int m = 0; m < 3; ++m)
+    for (int c = 0; c < 3; ++c) f.k.minv[m][c] = hs.t.minv[m][c];
+  f.k.n_emulsions = E;
+  for (int k = 0; k < 3; ++k)
+    for (int em = 0; em < E; ++em) {
+      double acc = 0.0;
+      for (int i = 0; i < NB; ++i) acc += f.W[k][i] * f.tab[em][i] * (1.0 - f.te[em][i]);
+      f.k.c[k][em] = acc;
+    }
+  // halation (halation.rs:24-132): bleed from the sensitivity peak, reflectance weighted by the sensitivity
+  f.sigma_l = std::max(st.psf_local_um / std::max(pitch, 1e-6f), 0.0f);
+  f.sigma_h_unit = std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
+  f.sigma_h = (float)((double)f.sigma_h_unit * (double)p.hal_radius);
+  f.do_local = f.sigma_l >= 1e-3f;
+  f.do_wide = f.sigma_h >= 1e-3f && hs.wide_reflect && E > 0;
+  for (int em = 0; em < E; ++em) {
+    double curve[NB];
+    for (int i = 0; i < NB; ++i) curve[i] = p.sens[em][i];
+    double peak = 550.0, mx = -INFINITY;
+    for (int i = 0; i < NB; ++i)
+      if (curve[i] > mx) { mx = curve[i]; peak = lam(i); }
+    const float wb = 0.20f, wg = 0.45f, wr = 1.0f;
+    float bl;
+    if (peak <= 450.0) bl = wb;
+    else if (peak <= 550.0) bl = wb + (wg - wb) * (float)((peak - 450.0) / 100.0);
+    else if (peak <= 650.0) bl = wg + (wr - wg) * (float)((peak - 550.0) / 100.0);
+    else bl = wr;
+    const double norm = trapz(curve);
+    float r;
+    double prod[NB];
+    for (int i = 0; i < NB; ++i) prod[i] = hs.reflect[i] * curve[i];
+    f.rnum[em] = trapz(prod);
+    f.rden[em] = norm;
+    r = norm <= 1e-9 ? (float)eval_curve(hs.reflect, 650.0) : (float)(f.rnum[em] / norm);
+    f.bleed[em] = bl;
+    f.rb[em] = r * bl;
+    FilmEmulsionTables& Em = f.T.em[em];
+    Em.gain = f.do_wide ? (float)((double)f.rb[em] * (double)p.hal_strength) : 0.0f;
+    Em.eta = 1.0f;  // shutter 1/box ISO <= 1 s: no reciprocity failure
+    Em.d_max = p.d_max[em];
+    Em.inv_gamma = 1.0f / std::max(p.gamma[em], 1e-6f);
+    Em.mask_scale = p.d_max[em] * 0.4f;
+  }
+  return f;
+}
+
+// d/dsigma of the normalised Gaussian taps of gauss_taps (fixed radius)
+std::vector<float> gauss_taps_dsigma(float sigma, int radius) {
+  std::vector<double> g(2 * radius + 1), gd(2 * radius + 1);
+  double sg = 0.0, sgd = 0.0;
+  const double s = sigma;
+  for (int i = 0; i <= 2 * radius; ++i) {
+    const double x = (double)i - (double)radius;
+    g[i] = std::exp(-x * x / (2.0 * s * s));
+    gd[i] = g[i] * x * x / (s * s * s);
+    sg += g[i];
+    sgd += gd[i];
+  }
+  std::vector<float> out(2 * radius + 1);
+  for (int i = 0; i <= 2 * radius; ++i) out[i] = (float)((gd[i] * sg - g[i] * sgd) / (sg * sg));
+  return out;
+}
+
+struct FitRun {
+  imops_ctx* ctx;
+  const HostStock* hs;
+  FitSetup f;
+  int w, h, E;
+  size_t n;
+  float* S;      // E planes
+  float* Sbar;   // E planes
+  float* Bn;
+  float* Bbar;
+  float* tmp;
+  float* u1;
+  float* u2;
+  float* img;    // E planes image dye
+  float* msk;    // E planes mask dye
+  float* taps;   // local, halation, halation d/dsigma
+  int rl, rh, off_l, off_h, off_hd;
+  FilmDeviceTables* T;
+  double* acc;   // 3E (d_max, gamma, gain) at [0, 8, 16], C at [24 + k*8 + e], sigma at [48]
+};
+
+FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_params& p, size_t w, size_t h) {
+  if (!ctx) throw Error{IMOPS_E_INVALID, "film_fit: null context"};
+  if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit: unknown stock"};
+  if (w < 2 || h < 2 || !(pitch > 0.0f)) throw Error{IMOPS_E_INVALID, "film_fit: invalid dimensions or pitch"};
+  FitRun R{};
+  R.ctx = ctx;
+  R.hs = &host_stock(stock);
+  if (R.hs->any_reversal) throw Error{IMOPS_E_UNSUPPORTED, "film_fit: reversal stocks are not supported"};
+  R.f = fit_setup(*R.hs, p, pitch);
+  R.w = (int)w;
+  R.h = (int)h;
+  R.E = R.hs->t.n_emulsions;
+  R.n = w * h;
+  std::vector<float> all;
+  R.off_l = R.off_h = R.off_hd = 0;
+  R.rl = R.rh = 0;
+  if (R.f.do_local) {
+    auto t = gauss_taps(R.f.sigma_l, &R.rl);
+    R.off_l = (int)all.size();
+    all.insert(all.end(), t.begin(), t.end());
+  }
+  if (R.f.do_wide) {
+    auto t = gauss_taps(R.f.sigma_h, &R.rh);
+    R.off_h = (int)all.size();
+    all.insert(all.end(), t.begin(), t.end());
+    auto d = gauss_taps_dsigma(R.f.sigma_h, R.rh);
+    R.off_hd = (int)all.size();
+    all.insert(all.end(), d.begin(), d.end());
+  }
+  const int rmax = std::max(R.rl, R.rh);
+  if (rmax > (int)std::min(w, h) - 1)
+    throw Error{IMOPS_E_UNSUPPORTED, "film_fit: image smaller than the blur support"};
+  const size_t np = (size_t)(4 * R.E + 6);
+  ctx->scr[SCR_FILM_0].reserve(np * R.n * sizeof(float));
+  ctx->scr[SCR_FILM_2].reserve(sizeof(FilmDeviceTables) + 64 * sizeof(double));
+  ctx->scr[SCR_FILM_3].reserve(std::max<size_t>(all.size(), 1) * sizeof(float));
+  float* b = ctx->scr[SCR_FILM_0].as<float>();
+  R.S = b;
+  R.Sbar = b + (size_t)R.E * R.n;
+  R.img = b + (size_t)2 * R.E * R.n;
+  R.msk = b + (size_t)3 * R.E * R.n;
+  R.Bn = b + (size_t)4 * R.E * R.n;
+  R.Bbar = R.Bn + R.n;
+  R.tmp = R.Bbar + R.n;
+  R.u1 = R.tmp + R.n;
+  R.u2 = R.u1 + R.n;
+  R.T = ctx->scr[SCR_FILM_2].as<FilmDeviceTables>();
+  R.acc = reinterpret_cast<double*>(reinterpret_cast<char*>(ctx->scr[SCR_FILM_2].ptr) + sizeof(FilmDeviceTables));
+  R.taps = ctx->scr[SCR_FILM_3].as<float>();
+  hipStream_t s = ctx->stream;
+  IMOPS_HIP_CHECK(hipMemcpyAsync(R.taps, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(R.T, &R.f.T, sizeof(FilmDeviceTables), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipStreamSynchronize(s));  // the host sources are locals
+  return R;
+}
+
+// expose + local scatter + halation bounce of one image into R.S / R.Bn
+void fit_front(FitRun& R, const float* rgb) {
+  hipStream_t s = R.ctx->stream;
+  IMOPS_HIP_CHECK(film_fit_expose(rgb, R.n, R.f.k, R.S, s));
+  if (R.f.do_local)
+    for (int e = 0; e < R.E; ++e) {
+      float* p = R.S + (size_t)e * R.n;
+      IMOPS_HIP_CHECK(film_blur(p, R.tmp, p, p, R.w, R.h, R.taps + R.off_l, R.rl, BLUR_MIX, 0.18f, s));
+    }
+  if (R.f.do_wide) {
+    const float* deep = R.S + (size_t)(R.E - 1) * R.n;
+    IMOPS_HIP_CHECK(film_blur(deep, R.tmp, deep, R.Bn, R.w, R.h, R.taps + R.off_h, R.rh, BLUR_STORE, 0.0f, s));
+  }
+}
+
+}  // namespace
+
 }  // namespace imops
 
 using namespace imops;
@@ -475,5 +699,115 @@ extern "C" int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_
     if (stock < 0 || stock >= STOCK_COUNT || film_format < 0 || film_format > 2 || output < 0 || output > 1)
       throw Error{IMOPS_E_INVALID, "film: stock / format / output out of range"};
     stage_film(ctx, img, p);
+  });
+}
+
+extern "C" int imops_film_fit_nominal(int stock, imops_film_fit_params* out) {
+  return guarded([&] {
+    if (!out) throw Error{IMOPS_E_INVALID, "film_fit_nominal: null argument"};
+    if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit_nominal: unknown stock"};
+    const HostStock& hs = host_stock(stock);
+    *out = imops_film_fit_params{};
+    out->n_emulsions = hs.t.n_emulsions;
+    for (int a = 0, e = 0; a < hs.t.n_absorbing; ++a) {
+      if (!hs.t.is_emulsion[a]) continue;
+      const Layer& L = hs.s->layers[hs.layer_of[a]];
+      double curve[NB];
+      sample_curve(L.sensitivity, curve);
+      for (int i = 0; i < NB; ++i) out->sens[e][i] = (float)curve[i];
+      out->d_max[e] = L.d_max;
+      out->gamma[e] = L.gamma_contrast;
+      ++e;
+    }
+    out->hal_strength = 1.0f;
+    out->hal_radius = 1.0f;
+  });
+}
+
+extern "C" int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+                                      const float* rgb_in, float* rgb_out, size_t n_images, size_t width,
+                                      size_t height) {
+  return guarded([&] {
+    if (!p || !rgb_in || !rgb_out) throw Error{IMOPS_E_INVALID, "film_fit_forward: null argument"};
+    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height);
+    hipStream_t s = ctx->stream;
+    TimedRegion tr(ctx, "film_fit_forward");
+    for (size_t i = 0; i < n_images; ++i) {
+      const float* in = rgb_in + i * R.n * 3;
+      fit_front(R, in);
+      IMOPS_HIP_CHECK(film_capture_reduce(R.S, R.f.do_wide ? R.Bn : nullptr, R.n, R.T, R.img, R.msk, s));
+      IMOPS_HIP_CHECK(film_scan(R.img, R.msk, R.n, R.T, rgb_out + i * R.n * 3, s));
+    }
+  });
+}
+
+extern "C" int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+                                       const float* rgb_in, const float* grad_out, size_t n_images, size_t width,
+                                       size_t height, imops_film_fit_params* grad) {
+  return guarded([&] {
+    if (!p || !rgb_in || !grad_out || !grad) throw Error{IMOPS_E_INVALID, "film_fit_backward: null argument"};
+    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height);
+    hipStream_t s = ctx->stream;
+    const int E = R.E;
+    {
+      TimedRegion tr(ctx, "film_fit_backward");
+      IMOPS_HIP_CHECK(film_fit_zero(R.acc, 64, s));
+      for (size_t i = 0; i < n_images; ++i) {
+        const float* in = rgb_in + i * R.n * 3;
+        fit_front(R, in);
+        IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, grad_out + i * R.n * 3, R.n, R.Sbar,
+                                           R.f.do_wide ? R.Bbar : nullptr, R.acc, s));
+        if (R.f.do_wide) {
+          float* deep = R.S + (size_t)(E - 1) * R.n;
+          float* deep_bar = R.Sbar + (size_t)(E - 1) * R.n;
+          const float* k = R.taps + R.off_h;
+          const float* kd = R.taps + R.off_hd;
+          // d/dsigma of V(H(x)) = V'(H x) + V(H' x)
+          IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u1, R.w, R.h, k, kd, R.rh, s));
+          IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u2, R.w, R.h, kd, k, R.rh, s));
+          IMOPS_HIP_CHECK(film_fit_dot(R.Bbar, R.u1, R.u2, R.n, R.acc + 48, s));
+          IMOPS_HIP_CHECK(film_blur_adjoint(R.Bbar, R.tmp, deep_bar, deep_bar, R.w, R.h, k, R.rh, 1.0f, 1.0f, s));
+        }
+        if (R.f.do_local)
+          for (int e = 0; e < E; ++e) {
+            float* q = R.Sbar + (size_t)e * R.n;
+            IMOPS_HIP_CHECK(film_blur_adjoint(q, R.tmp, q, q, R.w, R.h, R.taps + R.off_l, R.rl, 1.0f - 0.18f, 0.18f, s));
+          }
+        IMOPS_HIP_CHECK(film_fit_back_expose(in, R.Sbar, R.n, R.f.k, R.acc + 24, s));
+      }
+    }
+    double acc[64];
+    IMOPS_HIP_CHECK(hipMemcpyAsync(acc, R.acc, sizeof acc, hipMemcpyDeviceToHost, s));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+    const FitSetup& f = R.f;
+    *grad = imops_film_fit_params{};
+    grad->n_emulsions = E;
+    double g_hs = 0.0;
+    for (int e = 0; e < E; ++e) {
+      grad->d_max[e] = (float)acc[e];
+      grad->gamma[e] = (float)acc[8 + e];
+      g_hs += acc[16 + e] * (double)f.rb[e];
+    }
+    grad->hal_strength = f.do_wide ? (float)g_hs : 0.0f;
+    grad->hal_radius = f.do_wide ? (float)(acc[48] * (double)f.sigma_h_unit) : 0.0f;
+    // sensitivities: through the exposure contraction C and the halation reflectance r_e
+    for (int e2 = 0; e2 < E; ++e2)
+      for (int i = 0; i < NB; ++i) {
+        double g = 0.0;
+        for (int k = 0; k < 3; ++k)
+          for (int e = 0; e < E; ++e) {
+            const double cbar = acc[24 + k * 8 + e];
+            double d = 0.0;
+            if (e2 == e) d += f.tab[e][i] * f.alpha[e] * f.te[e][i];
+            if (f.above[e2][e]) d -= f.alpha[e2] * f.tab[e][i] * (1.0 - f.te[e][i]);
+            g += cbar * f.W[k][i] * d;
+          }
+        if (f.do_wide && f.rden[e2] > 1e-9) {
+          const double tw = trap_w(i);
+          const double dr = tw * (R.hs->reflect[i] * f.rden[e2] - f.rnum[e2]) / (f.rden[e2] * f.rden[e2]);
+          g += acc[16 + e2] * (double)p->hal_strength * f.bleed[e2] * dr;
+        }
+        grad->sens[e2][i] = (float)g;
+      }
   });
 }

@@ -172,6 +172,33 @@ int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* par
  * IMOPS_E_UNSUPPORTED. */
 int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_format, uint64_t seed, int output);
 
+/* Differentiable film model --------------------------------------------------
+ * No counterpart in the reference (its film simulation has no gradients); it is
+ * film::process (film/mod.rs:77-129) in the NegativeLinear form with shutter
+ * 1/box ISO and a given pixel pitch, grain, DIR inhibition and adjacency off
+ * (development/mod.rs:18-48), parameterised by the emulsion sensitivities on the
+ * 16-sample 400..700 nm grid (stock/*.rs layer curves), the characteristic
+ * curve D = d_max f^(1/gamma) (reduction.rs:16-68), a halation gain multiplier
+ * and a halation PSF width multiplier (halation.rs:24-132); the scan is
+ * normalised by the nominal stock's Dmin.  Images are N x H x W x 3 ACEScg
+ * (absolute luminance) f32 on the device.  imops_film_fit_backward returns
+ * dL/dparams for dL/dout = grad_out, summed over the images. */
+typedef struct imops_film_fit_params {
+  int n_emulsions;       /* emulsion layers top to bottom (the stock's count) */
+  float sens[8][16];     /* sensitivity samples 400, 420, ..., 700 nm */
+  float d_max[8];
+  float gamma[8];
+  float hal_strength;    /* 1 = the stock's halation gain */
+  float hal_radius;      /* 1 = the stock's halation PSF width */
+} imops_film_fit_params;
+
+int imops_film_fit_nominal(int stock, imops_film_fit_params* out);
+int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+                           const float* rgb_in, float* rgb_out, size_t n_images, size_t width, size_t height);
+int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+                            const float* rgb_in, const float* grad_out, size_t n_images, size_t width, size_t height,
+                            imops_film_fit_params* grad);
+
 /* Pipeline --------------------------------------------------------------- */
 
 /* Parse a TOML or JSON stage list ([[pipeline_modules]] tables, names matched

@@ -288,3 +288,30 @@ def film_normalized_dmin(stock: str = "ColorNeg200") -> np.ndarray:
     out = np.zeros(3, np.float32)
     _rc(_film_sigs(lib()).oracle_film_normalized_dmin(STOCKS[stock], _f(out)), "dmin")
     return out
+
+
+FIT_NPARAMS = 147  # n_emulsions, sens 8x16, d_max 8, gamma 8, hal_strength, hal_radius
+
+
+def film_fit_nominal(stock: str = "ColorNeg200") -> np.ndarray:
+    """Nominal parameters of the differentiable film model (flat f64 vector)."""
+    L = lib()
+    L.oracle_film_fit_nominal.restype = C.c_int
+    L.oracle_film_fit_nominal.argtypes = [C.c_int, _dp]
+    p = np.zeros(FIT_NPARAMS)
+    _rc(L.oracle_film_fit_nominal(STOCKS[stock], p.ctypes.data_as(_dp)), "film_fit_nominal")
+    return p
+
+
+def film_fit(rgb: np.ndarray, params: np.ndarray, stock: str = "ColorNeg200", pitch_um: float = 36000.0 / 6016,
+             fp64: bool = True) -> np.ndarray:
+    """The differentiable film model (film::process with grain, DIR and adjacency off,
+    parameters overridden, scan normalised by the nominal Dmin) on an (H, W, 3) image."""
+    L = lib()
+    L.oracle_film_fit.restype = C.c_int
+    L.oracle_film_fit.argtypes = [_fp, _sz, _sz, C.c_int, C.c_float, _dp, C.c_int]
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    p = np.ascontiguousarray(params, np.float64)
+    _rc(L.oracle_film_fit(_f(out), w, h, STOCKS[stock], pitch_um, p.ctypes.data_as(_dp), int(fp64)), "film_fit")
+    return out

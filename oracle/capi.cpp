@@ -163,4 +163,40 @@ void oracle_film_upsample(const float* rgb, double* out) { film_upsample(rgb, ou
 void oracle_film_spectrum_to_acescg(const double* s, double* out) { film_spectrum_to_acescg(s, out); }
 int oracle_film_normalized_dmin(int stock, float* out) { return film_normalized_dmin(stock, out); }
 
+// Differentiable film model: params = [n_emulsions, sens (8x16), d_max (8), gamma (8),
+// hal_strength, hal_radius] as doubles (147 values).
+static FilmFitParams unpack_fit(const double* p) {
+  FilmFitParams f{};
+  f.n_emulsions = (int)p[0];
+  for (int e = 0; e < 8; ++e)
+    for (int i = 0; i < 16; ++i) f.sens[e][i] = p[1 + e * 16 + i];
+  for (int e = 0; e < 8; ++e) {
+    f.d_max[e] = p[129 + e];
+    f.gamma[e] = p[137 + e];
+  }
+  f.hal_strength = p[145];
+  f.hal_radius = p[146];
+  return f;
+}
+int oracle_film_fit(float* rgb, size_t w, size_t h, int stock, float pitch_um, const double* params, int fp64) {
+  const FilmFitParams f = unpack_fit(params);
+  return fp64 ? film_fit_process<double>(rgb, w, h, stock, pitch_um, f)
+              : film_fit_process<float>(rgb, w, h, stock, pitch_um, f);
+}
+int oracle_film_fit_nominal(int stock, double* params) {
+  FilmFitParams f;
+  const int rc = film_fit_nominal(stock, &f);
+  if (rc) return rc;
+  params[0] = f.n_emulsions;
+  for (int e = 0; e < 8; ++e)
+    for (int i = 0; i < 16; ++i) params[1 + e * 16 + i] = f.sens[e][i];
+  for (int e = 0; e < 8; ++e) {
+    params[129 + e] = f.d_max[e];
+    params[137 + e] = f.gamma[e];
+  }
+  params[145] = f.hal_strength;
+  params[146] = f.hal_radius;
+  return 0;
+}
+
 }  // extern "C"

@@ -204,14 +204,20 @@ struct Derived {
   bool absorbs[12], is_emulsion[12];
   double reflect[NB], scanner[NB];
   bool any_reversal = false;
-  explicit Derived(const Stock* st) : s(st) {
+  // differentiable-model switches (FilmFitParams); defaults reproduce film::process
+  double hal_strength = 1.0, hal_radius = 1.0, scan_scale = 0.0;  // scan_scale <= 0: from this stock's Dmin
+  bool grain = true, dir = true, adjacency = true;
+  explicit Derived(const Stock* st, const FilmFitParams* fit = nullptr) : s(st) {
     curve_samples(st->ah_reflectance, reflect);
     curve_samples(st->scanner_light, scanner);
+    int ei = 0;
     for (int l = 0; l < st->n_layers; ++l) {
       const Layer& L = st->layers[l];
       if (L.is_reversal) any_reversal = true;
       double curve[NB];
       curve_samples(L.sensitivity, curve);
+      if (fit && L.kind == L_EMULSION && L.sensitivity.kind != CURVE_NONE)
+        for (int i = 0; i < NB; ++i) curve[i] = fit->sens[ei][i];
       is_emulsion[l] = L.kind == L_EMULSION;
       absorbs[l] = L.kind != L_SUPPORT && L.sensitivity.kind != CURVE_NONE;
       const double thickness = (double)L.thickness_um;
@@ -233,8 +239,9 @@ struct Derived {
       e.has_mask = L.mask_epsilon.kind != CURVE_NONE;
       curve_samples(L.mask_epsilon, e.mask);
       e.reversal = L.is_reversal;
-      e.d_max = L.d_max;
-      e.gamma = L.gamma_contrast;
+      e.d_max = fit ? (float)fit->d_max[ei] : L.d_max;
+      e.gamma = fit ? (float)fit->gamma[ei] : L.gamma_contrast;
+      ++ei;
       e.recip_p = L.reciprocity_p;
       const double mu = std::log(L.crystal_mean_um), sg = L.crystal_sigma_ln;
       e.lut.build(mu, sg, L.capture_k);
@@ -341,7 +348,7 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
       for (size_t i = 0; i < n; ++i) plane[i] = keep * plane[i] + f * sc[i];
     }
   }
-  const float sigma_wide = std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
+  const float sigma_wide = (float)((double)std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f) * d.hal_radius);
   bool do_wide = sigma_wide >= 1e-3f;
   if (do_wide) {
     float max_r = 0.0f;
@@ -374,7 +381,7 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
         for (int i = 0; i < NB; ++i) prod[i] = d.reflect[i] * em.sens[i];
         r_e = (float)(integrate(prod) / norm);
       }
-      const float gain = r_e * bleed;
+      const float gain = (float)((double)(r_e * bleed) * d.hal_strength);
       if (gain <= 0.0f) continue;
       for (size_t i = 0; i < n; ++i) abs[e][i] += (P)gain * bounce[i];
     }
@@ -446,7 +453,7 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
   }
   // DIR inhibition (diffusion.rs:39-84)
   const float sigma_dir = st.dir_diffusion_um / std::max(pitch, 1e-6f);
-  if (E > 0 && st.n_dir > 0 && sigma_dir >= 1e-3f) {
+  if (d.dir && E > 0 && st.n_dir > 0 && sigma_dir >= 1e-3f) {
     Planes<P> inh = dy.image;
     for (auto& p : inh) blur(p, w, h, sigma_dir);
     for (size_t j = 0; j < E; ++j) {
@@ -467,7 +474,7 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
   }
   // adjacency (diffusion.rs:11-28)
   const float sigma_adj = st.developer_diffusion_um / std::max(pitch, 1e-6f);
-  if (!(std::fabs(st.adjacency_beta) < 1e-8f || sigma_adj < 1e-3f)) {
+  if (d.adjacency && !(std::fabs(st.adjacency_beta) < 1e-8f || sigma_adj < 1e-3f)) {
     for (auto& plane : dy.image) {
       std::vector<P> b = plane;
       blur(b, w, h, sigma_adj);
@@ -476,7 +483,7 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
   }
   // grain (grain.rs:53-100)
   const float sigma_g = 3.0f / std::max(pitch, 1e-6f);
-  for (size_t e = 0; e < E; ++e) {
+  for (size_t e = 0; e < (d.grain ? E : 0); ++e) {
     const P d_max = (P)d.em[e].d_max;
     const P kappa = (P)((d.em[e].kappa_ref / std::max(pitch, 1e-6f)) * 0.15f);
     if (kappa <= (P)0 || d_max <= (P)0) continue;
@@ -524,7 +531,7 @@ void scan(const Derived& d, const Dyes<P>& dy, size_t n, float* rgb) {
   double dmin[3];
   dmin_reference(d, dmin);
   const double peak = std::max(std::max(std::max(dmin[0], dmin[1]), dmin[2]), 1e-12);
-  const double scale = 1.0 / peak;
+  const double scale = d.scan_scale > 0.0 ? d.scan_scale : 1.0 / peak;
   const size_t E = d.em.size();
 #pragma omp parallel for schedule(static)
   for (long long p = 0; p < (long long)n; ++p) {
@@ -640,6 +647,51 @@ int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint
 
 template int film_process<float>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*);
 template int film_process<double>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*);
+
+// The differentiable film model: film::process (NegativeLinear, shutter 1/box
+// ISO, a given pixel pitch) with grain, DIR inhibition and adjacency off, the emulsion sensitivities,
+// D_max and gamma taken from `fit`, the halation gain and PSF width scaled by
+// fit->hal_strength / fit->hal_radius, and the scan normalised by the nominal
+// stock's Dmin (so the normalisation is not a function of the parameters).
+template <class P>
+int film_fit_process(float* rgb, size_t w, size_t h, int stock_idx, float pitch_um, const FilmFitParams& fit) {
+  const Stock* st = stock_by_index(stock_idx);
+  if (!st || w == 0 || h == 0) return -1;
+  double nominal_scale;
+  {
+    const Derived nom(st);
+    double dmin[3];
+    dmin_reference(nom, dmin);
+    nominal_scale = 1.0 / std::max(std::max(std::max(dmin[0], dmin[1]), dmin[2]), 1e-12);
+  }
+  Derived d(st, &fit);
+  d.hal_strength = fit.hal_strength;
+  d.hal_radius = fit.hal_radius;
+  d.scan_scale = nominal_scale;
+  d.grain = d.dir = d.adjacency = false;
+  std::vector<float> out(w * h * 3);
+  run_negative<P>(d, rgb, w, h, pitch_um, 1.0f / st->box_iso, 0, out.data(), nullptr);
+  std::copy(out.begin(), out.end(), rgb);
+  return 0;
+}
+template int film_fit_process<float>(float*, size_t, size_t, int, float, const FilmFitParams&);
+template int film_fit_process<double>(float*, size_t, size_t, int, float, const FilmFitParams&);
+
+int film_fit_nominal(int stock_idx, FilmFitParams* out) {
+  const Stock* st = stock_by_index(stock_idx);
+  if (!st) return -1;
+  const Derived d(st);
+  *out = FilmFitParams{};
+  out->n_emulsions = (int)d.em.size();
+  for (size_t e = 0; e < d.em.size(); ++e) {
+    for (int i = 0; i < NB; ++i) out->sens[e][i] = d.em[e].sens[i];
+    out->d_max[e] = d.em[e].d_max;
+    out->gamma[e] = d.em[e].gamma;
+  }
+  out->hal_strength = 1.0;
+  out->hal_radius = 1.0;
+  return 0;
+}
 
 // Pieces of the restatement exposed on their own so tests can check them
 // against independent computations.

@@ -16,6 +16,19 @@ struct FilmTrace {
 
 const film_data::Stock* film_stock(int idx);
 
+// Parameters of the differentiable film model (emulsions top to bottom).
+struct FilmFitParams {
+  int n_emulsions;
+  double sens[8][16];          // sensitivity samples 400..700 nm
+  double d_max[8], gamma[8];   // characteristic curve D = d_max f^(1/gamma)
+  double hal_strength;         // halation gain multiplier
+  double hal_radius;           // halation PSF width multiplier
+};
+
+template <class P>
+int film_fit_process(float* rgb, size_t w, size_t h, int stock_idx, float pitch_um, const FilmFitParams& fit);
+int film_fit_nominal(int stock_idx, FilmFitParams* out);
+
 // film::process on an interleaved ACEScg buffer (absolute luminance), in place.
 // output: 0 NegativeLinear, 1 PositiveLinear.  shutter_seconds <= 0: 1/box ISO.
 template <class P>

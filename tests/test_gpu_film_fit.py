@@ -1,0 +1,125 @@
+"""Differentiable film model: GPU forward against the FP64 oracle, and the
+hand-written adjoint against FP64 central finite differences of the oracle.
+
+The model is film::process with grain, DIR inhibition and adjacency off and the
+stock's sensitivities, characteristic curves and halation as parameters
+(oracle: `film_fit`, the oracle's own film pipeline with those switches).
+Bars: forward max abs error <= 2e-5 against the FP64 oracle; gradients within
+1e-3 relative of FP64 central differences (h = 1e-3) of the oracle's L2 loss,
+per entry (against max(|fd|, 1% of the largest entry)) and as a vector.
+"""
+import numpy as np
+import pytest
+import torch
+
+import imops_amd as im
+import oracle
+from imops_amd.film_fit import FilmModel, fit
+
+pytestmark = pytest.mark.gpu
+
+PITCH = 36000.0 / 6016  # 35 mm frame at 6016 px: halation sigma 13.4 px, radius 41
+
+
+def _images(n, w, h, seed):
+    g = oracle.absolute_gray(1.0, 200)
+    return np.stack([(im.synth_scene_rgb(w, h, seed + i) * g).astype(np.float32) for i in range(n)])
+
+
+def _flat(params, E):
+    v = np.zeros(oracle.FIT_NPARAMS)
+    v[0] = E
+    s = np.zeros((8, 16))
+    s[:E] = params["sens"].double().numpy()
+    v[1:129] = s.ravel()
+    v[129:129 + E] = params["d_max"].double().numpy()
+    v[137:137 + E] = params["gamma"].double().numpy()
+    v[137 + E:145] = 1.0
+    v[145] = float(params["hal_strength"])
+    v[146] = float(params["hal_radius"])
+    return v
+
+
+def _oracle_forward(imgs, params, E):
+    v = _flat(params, E)
+    return np.stack([oracle.film_fit(x, v, pitch_um=PITCH) for x in imgs])
+
+
+def _perturbed(nom, seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = {k: v.clone() for k, v in nom.items()}
+    p["sens"] = (p["sens"] * torch.tensor(1 + 0.15 * scale * rng.uniform(-1, 1, p["sens"].shape), dtype=torch.float32))
+    p["d_max"] = p["d_max"] + 0.2 * scale * torch.tensor(rng.uniform(-1, 1, p["d_max"].shape), dtype=torch.float32)
+    p["gamma"] = p["gamma"] * torch.tensor(1 + 0.1 * scale * rng.uniform(-1, 1, p["gamma"].shape), dtype=torch.float32)
+    p["hal_strength"] = torch.tensor(1.0 + 0.3 * scale, dtype=torch.float32)
+    p["hal_radius"] = torch.tensor(1.0 - 0.05 * scale, dtype=torch.float32)
+    return p
+
+
+@pytest.fixture(scope="module")
+def model():
+    return FilmModel(im.Context(0), pitch_um=PITCH)
+
+
+@pytest.mark.parametrize("scale", [0.0, 1.0])
+def test_forward_matches_fp64_oracle(model, scale):
+    imgs = _images(2, 96, 88, 3)
+    p = _perturbed(model.nominal(), 11, scale)
+    got = model(torch.from_numpy(imgs).cuda(), **p).cpu().numpy()
+    want = _oracle_forward(imgs, p, model.n_emulsions)
+    err = np.abs(got - want).max()
+    print(f"forward max abs err {err:.3g}", flush=True)
+    assert err <= 2e-5
+
+
+def test_gradients_match_fp64_central_differences(model):
+    E = model.n_emulsions
+    imgs = _images(2, 96, 88, 5)
+    hidden = _perturbed(model.nominal(), 21, 1.0)
+    target = _oracle_forward(imgs, hidden, E)
+    theta = _perturbed(model.nominal(), 33, 0.4)
+
+    x = torch.from_numpy(imgs).cuda()
+    t = torch.from_numpy(target).cuda()
+    params = {k: v.clone().requires_grad_(True) for k, v in theta.items()}
+    loss = ((model(x, **params) - t) ** 2).sum()
+    loss.backward()
+
+    def oracle_loss(p):
+        out = _oracle_forward(imgs, p, E)
+        return float(np.sum((out.astype(np.float64) - target.astype(np.float64)) ** 2))
+
+    h = 1e-3
+    checks = [("d_max", (e,)) for e in range(E)] + [("gamma", (e,)) for e in range(E)] + \
+             [("hal_strength", ()), ("hal_radius", ())] + \
+             [("sens", (e, i)) for e in range(E) for i in (2, 5, 8, 12)]
+    adj, fds = [], []
+    for name, idx in checks:
+        plus = {k: v.clone() for k, v in theta.items()}
+        minus = {k: v.clone() for k, v in theta.items()}
+        plus[name][idx] += h
+        minus[name][idx] -= h
+        fds.append((oracle_loss(plus) - oracle_loss(minus)) / (2 * h))
+        adj.append(float(params[name].grad[idx]))
+    adj, fds = np.array(adj), np.array(fds)
+    # The oracle rounds a few per-stock scalars to f32 as the reference does (e.g. the
+    # halation reflectance), which puts ~1e-3 absolute noise on a difference quotient; entries
+    # are therefore compared relative to max(|fd|, 1% of the largest gradient).
+    floor = 1e-2 * np.abs(fds).max()
+    rel = np.abs(adj - fds) / np.maximum(np.abs(fds), floor)
+    for (name, idx), g, fd, r in zip(checks, adj, fds, rel):
+        print(f"{name}{list(idx)}: adjoint {g:.6g} fd {fd:.6g} rel {r:.2e}", flush=True)
+    vec = np.linalg.norm(adj - fds) / np.linalg.norm(fds)
+    print(f"worst entry {rel.max():.2e}, vector relative error {vec:.2e}", flush=True)
+    assert rel.max() <= 1e-3 and vec <= 1e-3
+
+
+def test_fit_reduces_loss(model):
+    imgs = _images(2, 96, 88, 7)
+    hidden = _perturbed(model.nominal(), 41, 1.0)
+    x = torch.from_numpy(imgs).cuda()
+    with torch.no_grad():
+        target = model(x, **hidden)
+    res = fit(model, x, target, model.nominal(), steps=30, lr=0.01)
+    print(f"loss {res['loss'][0]:.4g} -> {res['loss'][-1]:.4g}, {res['ms_per_iter']:.2f} ms/iter", flush=True)
+    assert res["loss"][-1] < 0.5 * res["loss"][0]
