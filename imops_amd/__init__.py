@@ -211,7 +211,8 @@ class Image:
     def bm3d(self, intensity: float = 0.0, **explicit) -> "Image":
         """BM3D (bm3d.rs:34-61).  Keyword overrides: sigma (0..255 units), lambda_, max_dist,
         chroma_scale, detail, stride, search_window, max_group, max_candidates, and
-        precision ("fp32" or "mixed": FP16 matrix-core matching, same groups)."""
+        precision ("fp32"; "mixed-match": FP16 matrix-core matching, same groups, FP32 filter;
+        "mixed": that matcher plus the filter's transforms on FP16 matrix cores)."""
         if not explicit:
             check(lib().imops_bm3d(self.ctx.handle, self._h, float(intensity)))
             return self
@@ -260,7 +261,8 @@ def bm3d_params(intensity: float = 0.0, **explicit) -> _lib.Bm3dParams:
     check(lib().imops_bm3d_default_params(float(intensity), C.byref(p)))
     for k, v in explicit.items():
         if k == "precision":
-            v = {"fp32": _lib.BM3D_FP32, "mixed": _lib.BM3D_MIXED}[v] if isinstance(v, str) else int(v)
+            v = {"fp32": _lib.BM3D_FP32, "mixed": _lib.BM3D_MIXED,
+                     "mixed-match": _lib.BM3D_MIXED_MATCH}[v] if isinstance(v, str) else int(v)
         setattr(p, "lambda_" if k == "lambda" else k, v)
     return p
 

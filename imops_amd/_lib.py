@@ -26,6 +26,7 @@ DEMOSAIC_AMAZE = 0
 DEMOSAIC_MARKESTEIJN = 1
 BM3D_FP32 = 0
 BM3D_MIXED = 1
+BM3D_MIXED_MATCH = 2
 
 _vp = C.c_void_p
 _sz = C.c_size_t

@@ -63,6 +63,12 @@ hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s);
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s);
 Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window);
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);
+// Mixed-precision collaborative filter (bm3d_mfma.hip): DCT / Hadamard products on FP16 MFMA.
+constexpr size_t kBm3dMfmaFragBytes = 2 * 4 * 2 * 64 * 16;
+void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out);
+size_t bm3d_filter_mfma_smem(int footprint_pixels);
+hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
+                                   hipStream_t s);
 hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s);
 hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],
                                 float* const out[3], size_t n, hipStream_t s);

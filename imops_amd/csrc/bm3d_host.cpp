@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "bm3d.hpp"
 #include "stages.hpp"
@@ -112,7 +113,8 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   ctx->scr[SCR_BM3D_0].reserve(12 * n * sizeof(float));
   ctx->scr[SCR_BM3D_1].reserve(6 * n * sizeof(unsigned long long));
   const size_t gwords = (nrefs ? nrefs : 1) * 17;
-  ctx->scr[SCR_BM3D_2].reserve((gwords + nrefs + 64) * sizeof(uint32_t));
+  const size_t frag_word = (gwords + nrefs + 64 + 3) & ~(size_t)3;  // FP16 DCT fragments of the mixed filter
+  ctx->scr[SCR_BM3D_2].reserve(frag_word * sizeof(uint32_t) + kBm3dMfmaFragBytes);
 
   float* f = ctx->scr[SCR_BM3D_0].as<float>();
   float* noisy[3] = {f, f + n, f + 2 * n};
@@ -126,11 +128,21 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   uint32_t* groups = ctx->scr[SCR_BM3D_2].as<uint32_t>();
   uint32_t* overflow_count = groups + gwords;           // 64-word aligned slot
   uint32_t* overflow_list = overflow_count + 64;
+  void* filter_frags = groups + frag_word;
   // Matrix-core matching needs one 8x8 block of reference patches' search
   // region in LDS; larger geometries use the plain matcher.
   Bm3dMfmaArgs mf = bm3d_mfma_layout(w, h, p.stride, p.window);
-  const bool use_mfma = p.precision == IMOPS_BM3D_MIXED && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
+  const bool use_mfma = p.precision != IMOPS_BM3D_FP32 && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
   static const HostTables tables;
+  const bool mixed_filter = p.precision == IMOPS_BM3D_MIXED;
+  if (mixed_filter) {
+    static const std::vector<uint16_t> frags = [] {
+      std::vector<uint16_t> v(kBm3dMfmaFragBytes / 2);
+      bm3d_filter_mfma_fragments(tables.dct, v.data());
+      return v;
+    }();
+    IMOPS_HIP_CHECK(hipMemcpyAsync(filter_frags, frags.data(), kBm3dMfmaFragBytes, hipMemcpyHostToDevice, s));
+  }
 
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -143,7 +155,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
     if (bm3d_filter_smem(span * span) <= 64 * 1024) break;
   }
   const int span = (tile - 1) * p.stride + 2 * half + 8;
-  const size_t fsmem = bm3d_filter_smem(span * span);
+  const size_t fsmem = mixed_filter ? bm3d_filter_mfma_smem(span * span) : bm3d_filter_smem(span * span);
   const int tiles = ((refs_x + tile - 1) / tile) * ((refs_y + tile - 1) / tile);
 
   auto run_step = [&](const float* const guide[3], bool step1, float* const out[3]) {
@@ -219,7 +231,8 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
         fa.kaiser[i] = tables.kaiser[i];
       }
       TimedRegion tr(ctx, step1 ? "bm3d_filter_ht" : "bm3d_filter_wiener");
-      IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
+      if (mixed_filter) IMOPS_HIP_CHECK(bm3d_filter_mfma_launch(fa, step1, tiles, fsmem, filter_frags, s));
+      else IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
     }
     const float* nz[3] = {noisy[0], noisy[1], noisy[2]};
     IMOPS_HIP_CHECK(bm3d_finalize_launch(num, den, nz, out, n, s));
@@ -278,8 +291,8 @@ int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* q) 
     p.max_group = q->max_group;
     p.max_candidates = q->max_candidates;
     p.precision = q->precision;
-    if (p.precision != IMOPS_BM3D_FP32 && p.precision != IMOPS_BM3D_MIXED)
-      throw Error{IMOPS_E_INVALID, "bm3d_ex: precision must be IMOPS_BM3D_FP32 or IMOPS_BM3D_MIXED"};
+    if (p.precision != IMOPS_BM3D_FP32 && p.precision != IMOPS_BM3D_MIXED && p.precision != IMOPS_BM3D_MIXED_MATCH)
+      throw Error{IMOPS_E_INVALID, "bm3d_ex: precision must be IMOPS_BM3D_FP32, _MIXED or _MIXED_MATCH"};
     if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.window > kBm3dMaxWindow || p.max_group < 1 ||
         p.max_group > kBm3dMaxGroup || p.max_candidates < 1)
       throw Error{IMOPS_E_INVALID, "bm3d_ex: stride >= 1, odd search_window <= 41, max_group 1..16, max_candidates >= 1"};

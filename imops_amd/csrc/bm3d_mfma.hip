@@ -647,6 +647,230 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   }
 }
 
+// ------------------------------------------------------------ collaborative filter
+// Mixed-precision filter: one wave per (group, channel).  With the group's
+// patches as rows, the 2-D DCT is one product with Kd = D (x) D (64 x 64, FP16),
+// the Walsh-Hadamard transform across the group a product with the g x g
+// Sylvester matrix / sqrt(g) (16 x 16, rows past g zero), then the hard
+// threshold or Wiener shrinkage, the inverse Hadamard product, a transpose
+// through LDS and the product with Kd^T; FP32 accumulation throughout.
+// Aggregation is the FP32 form's fixed-point footprint (bm3d.rs:463-488).
+//
+// Before the FP16 conversion every patch of a group has the same level c (the
+// mean of the reference patch's four centre pixels) subtracted.  A constant
+// moves only stack element 0 (its DC coefficient, through the first Hadamard
+// row), which the hard threshold keeps and the Wiener filter passes untouched,
+// so c is added back after the inverse transforms; meanwhile the FP16 operands
+// hold residuals of the local contrast instead of absolute levels.
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ long long to_fixed_m(float v) {
+  const float s = v * 100000.0f;
+  if (isnan(s)) return 0;
+  if (s >= 9.2233720368547758e18f) return INT64_MAX;
+  if (s <= -9.2233720368547758e18f) return INT64_MIN;
+  return (long long)s;
+}
+
+__device__ __forceinline__ half4 to_h4(const floatx4& v) {
+  half4 h = {(_Float16)v[0], (_Float16)v[1], (_Float16)v[2], (_Float16)v[3]};
+  return h;
+}
+
+// Group level: mean of the reference patch's centre 2x2 pixels (wave-uniform).
+__device__ __forceinline__ float group_level(const float* __restrict__ plane, int W, uint32_t p0) {
+  const float* c = plane + (size_t)((p0 >> 16) + 3) * W + (p0 & 0xffffu) + 3;
+  const float v = 0.25f * ((c[0] + c[1]) + (c[W] + c[W + 1]));
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// A fragments of the group's patches: lane (patch n, q) holds rows q and 4 + q, minus the level c.
+__device__ __forceinline__ void load_patch_rows(const float* __restrict__ plane, int W, bool act, int px, int py, int q,
+                                                float c, half8& a0, half8& a1) {
+  float v0[8], v1[8];
+  if (act) {
+    load8g(plane + (size_t)(py + q) * W + px, v0);
+    load8g(plane + (size_t)(py + 4 + q) * W + px, v1);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v0[e] = v1[e] = 0.0f;
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    a0[e] = (_Float16)(act ? v0[e] - c : 0.0f);
+    a1[e] = (_Float16)(act ? v1[e] - c : 0.0f);
+  }
+}
+
+template <bool kStep1>
+__global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, const half8* __restrict__ frags) {
+  extern __shared__ unsigned long long facc[];  // footprint num/den pairs, then 4 x 16 x 64 FP16 transposes
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int q = lane >> 4, n = lane & 15;
+  const int tiles_x = (a.refs_x + a.tile - 1) / a.tile;
+  const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const int r0x = tx * a.tile, r0y = ty * a.tile;
+  const int r1x = min(r0x + a.tile, a.refs_x), r1y = min(r0y + a.tile, a.refs_y);
+  const int nrx = r1x - r0x, nref = nrx * (r1y - r0y);
+  const int half_w = a.window / 2;
+  const int fx0 = max(r0x * a.stride - half_w, 0), fy0 = max(r0y * a.stride - half_w, 0);
+  const int fx1 = min((r1x - 1) * a.stride + half_w, a.width - 8) + 8;  // exclusive
+  const int fy1 = min((r1y - 1) * a.stride + half_w, a.height - 8) + 8;
+  const int fw = fx1 - fx0, fh = fy1 - fy0;
+  _Float16* tbuf = reinterpret_cast<_Float16*>(facc + 2 * fw * fh) + wave * 16 * 64;
+
+  // constant B fragments: forward (Kd^T) and inverse (Kd), 4 N-tiles x 2 K-steps each
+  half8 KF[4][2], KI[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      KF[t][s] = frags[((0 * 4 + t) * 2 + s) * 64 + lane];
+      KI[t][s] = frags[((1 * 4 + t) * 2 + s) * 64 + lane];
+    }
+
+  for (int ch = 0; ch < 3; ++ch) {
+    for (int k = tid; k < 2 * fw * fh; k += 256) facc[k] = 0ull;
+    __syncthreads();
+    const float ch_sigma = ch == 0 ? a.sigma : a.sigma * a.chroma_scale;
+    const float* __restrict__ gplane = a.guide[ch];
+    const float* __restrict__ nplane = a.noisy[ch];
+    for (int base = 0; base < nref; base += 4) {
+      const int r = base + wave;
+      if (r >= nref) continue;  // wave-uniform
+      const int ref_id = (r0y + r / nrx) * a.refs_x + r0x + r % nrx;
+      const uint32_t* grp = a.groups + (size_t)ref_id * 17;
+      const int g = (int)grp[0];
+      if (g == 0) continue;  // wave-uniform; nothing to aggregate
+      // patch n (A rows) and patches 4q + j (C rows) positions
+      const bool actn = n < g;
+      const uint32_t pn = actn ? grp[1 + n] : 0u;
+      const int pxn = (int)(pn & 0xffffu), pyn = (int)(pn >> 16);
+      int cpx[4], cpy[4];
+      bool cact[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        cact[j] = 4 * q + j < g;
+        const uint32_t pp = cact[j] ? grp[1 + 4 * q + j] : 0u;
+        cpx[j] = (int)(pp & 0xffffu);
+        cpy[j] = (int)(pp >> 16);
+      }
+      // Hadamard fragment (A of the 16x16x16 product): lane (row n, q) holds H[n][4q .. 4q+3]
+      const float hs = 1.0f / sqrtf((float)g);
+      half4 Hf;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int kk = 4 * q + e;
+        const float v = (n < g && kk < g) ? ((__popc(n & kk) & 1) ? -hs : hs) : 0.0f;
+        Hf[e] = (_Float16)v;
+      }
+      // forward: 2-D DCT (rows = patches) then Hadamard across the group
+      const uint32_t p0 = grp[1];
+      const float cg = group_level(gplane, a.width, p0);
+      half8 A0, A1;
+      load_patch_rows(gplane, a.width, actn, pxn, pyn, q, cg, A0, A1);
+      floatx4 Z[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, KF[t][0], y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, KF[t][1], y, 0, 0, 0);
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        Z[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(y), zero, 0, 0, 0);
+      }
+      float weight, level;
+      if (kStep1) {
+        level = cg;
+        const float th = a.lambda * ch_sigma;
+        int nz = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const bool dc = t == 0 && n == 0 && q == 0 && j == 0;  // stack element 0 is kept
+            const bool keep = dc || !(fabsf(Z[t][j]) < th);
+            if (!keep) Z[t][j] = 0.0f;
+            nz += (keep && (4 * q + j < g)) ? 1 : 0;
+          }
+        for (int off = 32; off > 0; off >>= 1) nz += __shfl_xor(nz, off, 64);
+        weight = nz > 0 ? 1.0f / (float)nz : 1.0f;
+      } else {
+        level = group_level(nplane, a.width, p0);
+        half8 N0, N1;
+        load_patch_rows(nplane, a.width, actn, pxn, pyn, q, level, N0, N1);
+        const float s2 = ch_sigma * ch_sigma;
+        float wsum = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N0, KF[t][0], y, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N1, KF[t][1], y, 0, 0, 0);
+          const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+          floatx4 zn = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(y), zero, 0, 0, 0);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const bool dc = t == 0 && n == 0 && q == 0 && j == 0;
+            const float e = Z[t][j] * Z[t][j];
+            const float coef = e / (e + s2 + 1e-5f);
+            if (!dc) {
+              zn[j] *= coef;
+              wsum += coef * coef;
+            }
+          }
+          Z[t] = zn;
+        }
+        for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
+        const float run = 1.0f + wsum;
+        weight = run > 0.0f ? 1.0f / run : 1.0f;
+      }
+      // inverse Hadamard, transpose to patch rows through LDS, inverse DCT
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4 y = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(Z[t]), zero, 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tbuf[(4 * q + j) * 64 + 16 * t + n] = (_Float16)y[j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      const half8 B0 = load8(tbuf + n * 64 + 8 * q), B1 = load8(tbuf + n * 64 + 32 + 8 * q);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        floatx4 pv = {0.0f, 0.0f, 0.0f, 0.0f};
+        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B0, KI[t][0], pv, 0, 0, 0);
+        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B1, KI[t][1], pv, 0, 0, 0);
+        // lane (pixel 16t + n, q), register j -> patch 4q + j
+        const int pix = 16 * t + n, dy = pix >> 3, dx = pix & 7;
+        const float kw = a.kaiser[pix] * weight;
+        const long long kwf = to_fixed_m(kw);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (!cact[j]) continue;
+          const int fi = (cpy[j] + dy - fy0) * fw + (cpx[j] + dx - fx0);
+          atomicAdd(&facc[2 * fi], (unsigned long long)to_fixed_m((pv[j] + level) * kw));
+          atomicAdd(&facc[2 * fi + 1], (unsigned long long)kwf);
+        }
+      }
+    }
+    __syncthreads();
+    unsigned long long* gnum = a.num + (size_t)ch * a.width * a.height;
+    unsigned long long* gden = a.den + (size_t)ch * a.width * a.height;
+    for (int k = tid; k < fw * fh; k += 256) {
+      const unsigned long long nv = facc[2 * k], dv = facc[2 * k + 1];
+      if (nv | dv) {
+        const size_t gi = (size_t)(fy0 + k / fw) * a.width + fx0 + k % fw;
+        if (nv) atomicAdd(&gnum[gi], nv);
+        if (dv) atomicAdd(&gden[gi], dv);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -684,6 +908,35 @@ Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window) {
   const int uw = 3 * stride + 2 * half + 1;
   a.max_tiles_ok = uw <= 128 && (size_t)uw * uw < 65536;
   return a;
+}
+
+// B-operand fragments of the DCT products in lane order: [fwd|inv][N-tile t][K-step s][lane][8 halves],
+// fwd: Kd^T (pixel x coefficient), inv: Kd (coefficient x pixel), Kd[(j,k)][(n,m)] = dct[j][n] dct[k][m].
+void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out) {
+  for (int which = 0; which < 2; ++which)
+    for (int t = 0; t < 4; ++t)
+      for (int s = 0; s < 2; ++s)
+        for (int l = 0; l < 64; ++l)
+          for (int e = 0; e < 8; ++e) {
+            const int nn = l & 15, qq = l >> 4;
+            const int kidx = 32 * s + 8 * qq + e, col = 16 * t + nn;
+            const int coef = which == 0 ? col : kidx, pix = which == 0 ? kidx : col;
+            const double v = (double)dct[(coef >> 3) * 8 + (pix >> 3)] * (double)dct[(coef & 7) * 8 + (pix & 7)];
+            const _Float16 h = (_Float16)(float)v;
+            out[((((which * 4 + t) * 2 + s) * 64 + l) * 8) + e] = __builtin_bit_cast(uint16_t, h);
+          }
+}
+
+size_t bm3d_filter_mfma_smem(int footprint_pixels) {
+  return (size_t)footprint_pixels * 2 * sizeof(unsigned long long) + (size_t)4 * 16 * 64 * sizeof(_Float16);
+}
+
+hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
+                                   hipStream_t s) {
+  const half8* f = reinterpret_cast<const half8*>(frags);
+  if (step1) hipLaunchKernelGGL(bm3d_filter_mfma_kernel<true>, dim3(nblocks), dim3(256), smem, s, a, f);
+  else hipLaunchKernelGGL(bm3d_filter_mfma_kernel<false>, dim3(nblocks), dim3(256), smem, s, a, f);
+  return hipGetLastError();
 }
 
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s) {

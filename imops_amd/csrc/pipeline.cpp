@@ -348,6 +348,7 @@ Stage make_stage(const Table& t) {
     const std::string prec = str_param(t, "precision", "fp32", "BM3D");
     if (prec == "fp32") s.bm3d.precision = IMOPS_BM3D_FP32;
     else if (prec == "mixed") s.bm3d.precision = IMOPS_BM3D_MIXED;
+    else if (prec == "mixed-match") s.bm3d.precision = IMOPS_BM3D_MIXED_MATCH;
     else throw Error{IMOPS_E_INVALID, "Invalid BM3D config"};
   } else if (s.name == "Film") {
     s.kind = K_FILM;

@@ -22,7 +22,7 @@ struct Bm3dStageParams {
   int window = 19;              // search window (odd)
   int max_group = 16;           // group size cap (power of two)
   int max_candidates = 1024;    // accepted-candidate cap per reference patch
-  int precision = IMOPS_BM3D_FP32;  // IMOPS_BM3D_FP32 or IMOPS_BM3D_MIXED
+  int precision = IMOPS_BM3D_FP32;  // IMOPS_BM3D_FP32, _MIXED or _MIXED_MATCH
   static Bm3dStageParams from_intensity(float i);
 };
 

@@ -46,6 +46,7 @@ extern "C" {
 /* BM3D arithmetic (imops_bm3d_params.precision). */
 #define IMOPS_BM3D_FP32 0
 #define IMOPS_BM3D_MIXED 1
+#define IMOPS_BM3D_MIXED_MATCH 2
 
 /* Demosaic algorithms (DemosaicAlgorithmType, modules.rs:73-100). */
 #define IMOPS_DEMOSAIC_AMAZE 0
@@ -153,8 +154,10 @@ typedef struct imops_bm3d_params {
   int max_group;        /* 1..16, groups are the largest power of two <= min(this, matches) (16) */
   int max_candidates;   /* accepted-candidate cap in scan order (1024) */
   int precision;        /* IMOPS_BM3D_FP32 (0): plain FP32 throughout;
-                           IMOPS_BM3D_MIXED (1): FP16 matrix-core block matching with
-                           FP32 re-ranking (same groups as FP32) */
+                           IMOPS_BM3D_MIXED_MATCH (2): FP16 matrix-core block matching with
+                           FP32 re-ranking (same groups as FP32), FP32 filter;
+                           IMOPS_BM3D_MIXED (1): that matcher plus the collaborative filter's
+                           DCT / Hadamard transforms on FP16 matrix cores (FP32 accumulation) */
 } imops_bm3d_params;
 
 /* Fill `out` with the parameters intensity maps to (from_intensity). */
