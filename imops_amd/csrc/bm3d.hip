@@ -55,9 +55,11 @@ struct MatchShared {
   float win[3][kMaxSpan * kMaxSpan];
   float ref[3][64];
   float dist[kMaxWin * kMaxWin];      // +inf when rejected
-  int scan[MT + 1];
-  unsigned long long best[MT / 64];
+  int chunk_count[(kMaxWin * kMaxWin + 63) / 64];  // accepted candidates per 64 in scan order
+  unsigned long long top[MT / 64][16];  // each wave's smallest (dist bits, scan index) keys
 };
+constexpr int kMatchChunks = (kMaxWin * kMaxWin + 63) / 64;
+constexpr int kChunksPerWave = (kMatchChunks + MT / 64 - 1) / (MT / 64);
 
 __device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -172,64 +174,84 @@ __device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int 
   }
   __syncthreads();
 
-  // Cap: only the first max_candidates-1 accepted candidates (scan order)
-  // join the reference patch.  Block-wide exclusive scan of accept counts over
-  // contiguous chunks of the scan order.
-  const int chunk = (ncand + MT - 1) / MT;
-  const int c0 = tid * chunk, c1 = min(c0 + chunk, ncand);
-  int cnt = 0;
-  for (int c = c0; c < c1; ++c) cnt += S.dist[c] != __builtin_inff();
-  S.scan[tid + 1] = cnt;
-  __syncthreads();
-  if (tid == 0) {
-    S.scan[0] = 0;
-    for (int t = 1; t <= MT; ++t) S.scan[t] += S.scan[t - 1];
+  // Cap: only the first max_candidates-1 accepted candidates (scan order) join
+  // the reference patch.  Accepted counts per 64-candidate chunk, then each
+  // candidate's rank among the accepted ones from a prefix over the chunks.
+  const int nchunks = (ncand + 63) >> 6;
+  for (int k = wave; k < nchunks; k += MT / 64) {
+    const int c = (k << 6) + lane;
+    const unsigned long long m = __ballot(c < ncand && S.dist[c] != __builtin_inff());
+    if (lane == 0) S.chunk_count[k] = __popcll(m);
   }
   __syncthreads();
-  const int total_acc = S.scan[MT];
-  int rank = S.scan[tid];
-  for (int c = c0; c < c1; ++c)
-    if (S.dist[c] != __builtin_inff()) {
-      if (rank >= a.max_candidates - 1) S.dist[c] = __builtin_inff();
-      ++rank;
-    }
-  __syncthreads();
-  const int n_list = 1 + min(total_acc, a.max_candidates - 1);
+  const int cnt_l = lane < nchunks ? S.chunk_count[lane] : 0;
+  int incl = cnt_l;  // inclusive prefix over chunks, chunk k at lane k
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += o;
+  }
+  const int total_acc = __shfl(incl, 63, 64);
+  const int cap = a.max_candidates - 1;
+  const int n_list = 1 + min(total_acc, cap);
   const int limit = min(a.max_group, n_list);
   int g = 1;
   while (g * 2 <= limit) g *= 2;
+  const int need = g - 1;
 
-  // Selection: the reference patch first, then g-1 rounds of block argmin on
-  // (dist bits, scan index); dist >= 0 so its bit pattern orders like the value.
-  uint32_t* out = a.groups + (size_t)ref_id * 17;
-  if (tid == 0) {
-    out[0] = (uint32_t)g;
-    out[1] = (uint32_t)rx | ((uint32_t)ry << 16);
+  // Selection: the reference patch first, then the need smallest keys
+  // (dist bits, scan index) among the capped accepted candidates; dist >= 0 so
+  // its bit pattern orders like the value.  Each wave finds its own smallest
+  // need keys (wave w holds chunks w, w + 4, ...; lane l candidate 64k + l),
+  // then one wave ranks the merged lists.
+  unsigned long long key[kChunksPerWave];
+#pragma unroll
+  for (int i = 0; i < kChunksPerWave; ++i) {
+    const int k = wave + i * (MT / 64);
+    const int c = (k << 6) + lane;
+    const bool accepted = k < nchunks && c < ncand && S.dist[c] != __builtin_inff();
+    const unsigned long long m = __ballot(accepted);
+    const int excl = __shfl(incl - cnt_l, k < nchunks ? k : 0, 64);
+    const int rank = excl + __popcll(m & ((1ull << lane) - 1ull));
+    key[i] = (accepted && rank < cap) ? (((unsigned long long)__float_as_uint(S.dist[c]) << 32) | (unsigned)c) : ~0ull;
   }
-  for (int k = 1; k < g; ++k) {
-    unsigned long long best = ~0ull;
-    for (int c = tid; c < ncand; c += MT) {
-      const float dv = S.dist[c];
-      if (dv != __builtin_inff()) {
-        const unsigned long long key = ((unsigned long long)__float_as_uint(dv) << 32) | (unsigned)c;
-        best = key < best ? key : best;
+  unsigned long long lmin = key[0];
+#pragma unroll
+  for (int i = 1; i < kChunksPerWave; ++i) lmin = key[i] < lmin ? key[i] : lmin;
+  for (int r = 0; r < 16; ++r) {
+    unsigned long long wmin = lmin;
+    if (r < need) {
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(wmin, off, 64);
+        wmin = o < wmin ? o : wmin;
+      }
+    } else {
+      wmin = ~0ull;
+    }
+    if (lane == 0) S.top[wave][r] = wmin;
+    if (r < need && wmin != ~0ull) {
+      lmin = ~0ull;
+#pragma unroll
+      for (int i = 0; i < kChunksPerWave; ++i) {
+        if (key[i] == wmin) key[i] = ~0ull;
+        lmin = key[i] < lmin ? key[i] : lmin;
       }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const unsigned long long o = __shfl_xor(best, off, 64);
-      best = o < best ? o : best;
-    }
-    if (lane == 0) S.best[wave] = best;
-    __syncthreads();
-    unsigned long long b = S.best[0];
-    for (int w = 1; w < MT / 64; ++w) b = S.best[w] < b ? S.best[w] : b;
-    const int c = (int)(b & 0xffffffffu);
-    if (tid == 0 && b != ~0ull) {
+  }
+  __syncthreads();
+  uint32_t* out = a.groups + (size_t)ref_id * 17;
+  if (wave == 0) {
+    const unsigned long long mine = S.top[lane >> 4][lane & 15];
+    int rank = 0;
+    for (int j = 0; j < MT / 64 * 16; ++j) rank += S.top[j >> 4][j & 15] < mine ? 1 : 0;
+    if (mine != ~0ull && rank < need) {
+      const int c = (int)(mine & 0xffffffffu);
       const int cy = c / nx, cx = c - cy * nx;
-      out[1 + k] = (uint32_t)(sx0 + cx) | ((uint32_t)(sy0 + cy) << 16);
-      S.dist[c] = __builtin_inff();
+      out[2 + rank] = (uint32_t)(sx0 + cx) | ((uint32_t)(sy0 + cy) << 16);
     }
-    __syncthreads();
+    if (lane == 0) {
+      out[0] = (uint32_t)g;
+      out[1] = (uint32_t)rx | ((uint32_t)ry << 16);
+    }
   }
 }
 
@@ -316,9 +338,12 @@ __device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
   for (int i = 0; i < 64; ++i) v[i] *= scale;
 }
 
+// LDS: the footprint's i64 numerators, then its u32 denominators.  A
+// workgroup's denominator sum at one pixel is at most 64 references x 16
+// patches x 1e5 < 2^32, so the 32-bit partial sums are exact.
 template <bool kStep1>
 __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
-  extern __shared__ unsigned long long acc[];  // footprint num/den pairs
+  extern __shared__ unsigned long long acc[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int slot = lane >> 4, k = lane & 15;
   const int tiles_x = (a.refs_x + a.tile - 1) / a.tile;
@@ -330,10 +355,14 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
   const int fx0 = max(r0x * a.stride - half, 0), fy0 = max(r0y * a.stride - half, 0);
   const int fx1 = min((r1x - 1) * a.stride + half, a.width - 8) + 8;  // exclusive
   const int fy1 = min((r1y - 1) * a.stride + half, a.height - 8) + 8;
-  const int fw = fx1 - fx0, fh = fy1 - fy0;
+  const int fw = fx1 - fx0, fh = fy1 - fy0, fp = fw * fh;
+  uint32_t* den = reinterpret_cast<uint32_t*>(acc + fp);
 
   for (int ch = 0; ch < 3; ++ch) {
-    for (int q = tid; q < 2 * fw * fh; q += FT) acc[q] = 0ull;
+    for (int q = tid; q < fp; q += FT) {
+      acc[q] = 0ull;
+      den[q] = 0u;
+    }
     __syncthreads();
     const float ch_sigma = ch == 0 ? a.sigma : a.sigma * a.chroma_scale;
     const float* __restrict__ gplane = a.guide[ch];
@@ -419,16 +448,16 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
           for (int dx = 0; dx < 8; ++dx) {
             const float kw = a.kaiser[dy * 8 + dx] * weight;
             const int fi = (py + dy - fy0) * fw + (px + dx - fx0);
-            atomicAdd(&acc[2 * fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * kw));
-            atomicAdd(&acc[2 * fi + 1], (unsigned long long)to_fixed(kw));
+            atomicAdd(&acc[fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * kw));
+            atomicAdd(&den[fi], (uint32_t)to_fixed(kw));
           }
       }
     }
     __syncthreads();
     unsigned long long* gnum = a.num + (size_t)ch * a.width * a.height;
     unsigned long long* gden = a.den + (size_t)ch * a.width * a.height;
-    for (int q = tid; q < fw * fh; q += FT) {
-      const unsigned long long n = acc[2 * q], d = acc[2 * q + 1];
+    for (int q = tid; q < fp; q += FT) {
+      const unsigned long long n = acc[q], d = den[q];
       if (n | d) {
         const size_t gi = (size_t)(fy0 + q / fw) * a.width + fx0 + q % fw;
         if (n) atomicAdd(&gnum[gi], n);
@@ -504,7 +533,9 @@ int grid1d(size_t n) {
 }  // namespace
 
 size_t bm3d_match_smem() { return sizeof(MatchShared); }
-size_t bm3d_filter_smem(int footprint_pixels) { return (size_t)footprint_pixels * 2 * sizeof(unsigned long long); }
+size_t bm3d_filter_smem(int footprint_pixels) {
+  return (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
+}
 
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s) {
   hipLaunchKernelGGL(rgb_to_ycbcr_kernel, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, y, cb, cr);

@@ -147,15 +147,18 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
 
-  // workgroup tile of reference patches: largest whose LDS footprint fits 2 groups per CU
+  // workgroup tile of reference patches: the largest whose LDS footprint lets
+  // four plain-filter workgroups (16 waves) or two 8-wave mixed-filter
+  // workgroups share a CU
   const int half = p.window / 2;
+  auto filter_lds = [&](int t) {
+    const int sp = (t - 1) * p.stride + 2 * half + 8;
+    return mixed_filter ? bm3d_filter_mfma_smem(sp * sp) : bm3d_filter_smem(sp * sp);
+  };
+  const size_t lds_budget = mixed_filter ? 80 * 1024 : 40 * 1024;
   int tile = 8;
-  for (; tile > 1; tile /= 2) {
-    const int span = (tile - 1) * p.stride + 2 * half + 8;
-    if (bm3d_filter_smem(span * span) <= 64 * 1024) break;
-  }
-  const int span = (tile - 1) * p.stride + 2 * half + 8;
-  const size_t fsmem = mixed_filter ? bm3d_filter_mfma_smem(span * span) : bm3d_filter_smem(span * span);
+  while (tile > 1 && filter_lds(tile) > lds_budget) tile /= 2;
+  const size_t fsmem = filter_lds(tile);
   const int tiles = ((refs_x + tile - 1) / tile) * ((refs_y + tile - 1) / tile);
 
   auto run_step = [&](const float* const guide[3], bool step1, float* const out[3]) {

@@ -702,10 +702,18 @@ __device__ __forceinline__ void load_patch_rows(const float* __restrict__ plane,
   }
 }
 
+// LDS: DCT fragments (16 KB), one 16 x 64 FP16 transpose per wave (8 x 2 KB), then
+// the footprint's i64 numerators and u32 denominators.  A workgroup's denominator
+// sum at one pixel is at most 64 references x 16 patches x 1e5 < 2^32, so the
+// 32-bit partial sums are exact.
+constexpr int FM = 512;
+constexpr int kFilterFixedLds = 16384 + (FM / 64) * 2048;
+
 template <bool kStep1>
-__global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, const half8* __restrict__ frags) {
-  extern __shared__ unsigned long long facc[];  // footprint num/den pairs, then 4 x 16 x 64 FP16 transposes
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+__global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, const half8* __restrict__ frags) {
+  extern __shared__ __align__(16) unsigned char fsh[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int q = lane >> 4, n = lane & 15;
   const int tiles_x = (a.refs_x + a.tile - 1) / a.tile;
   const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
@@ -716,32 +724,30 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
   const int fx0 = max(r0x * a.stride - half_w, 0), fy0 = max(r0y * a.stride - half_w, 0);
   const int fx1 = min((r1x - 1) * a.stride + half_w, a.width - 8) + 8;  // exclusive
   const int fy1 = min((r1y - 1) * a.stride + half_w, a.height - 8) + 8;
-  const int fw = fx1 - fx0, fh = fy1 - fy0;
-  _Float16* tbuf = reinterpret_cast<_Float16*>(facc + 2 * fw * fh) + wave * 16 * 64;
-
-  // constant B fragments: forward (Kd^T) and inverse (Kd), 4 N-tiles x 2 K-steps each
-  half8 KF[4][2], KI[4][2];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      KF[t][s] = frags[((0 * 4 + t) * 2 + s) * 64 + lane];
-      KI[t][s] = frags[((1 * 4 + t) * 2 + s) * 64 + lane];
-    }
+  const int fw = fx1 - fx0, fh = fy1 - fy0, fp = fw * fh;
+  half8* lfrag = reinterpret_cast<half8*>(fsh);
+  _Float16* tbuf = reinterpret_cast<_Float16*>(fsh + 16384) + wave * 16 * 64;
+  unsigned long long* fnum = reinterpret_cast<unsigned long long*>(fsh + kFilterFixedLds);
+  uint32_t* fden = reinterpret_cast<uint32_t*>(fnum + fp);
+  // constant B fragments: forward (Kd^T) at [t][s], inverse (Kd) at [4 + t][s]
+  for (int i = tid; i < 1024; i += FM) lfrag[i] = frags[i];
+#define KF(t, s) lfrag[((t) * 2 + (s)) * 64 + lane]
+#define KI(t, s) lfrag[((4 + (t)) * 2 + (s)) * 64 + lane]
 
   for (int ch = 0; ch < 3; ++ch) {
-    for (int k = tid; k < 2 * fw * fh; k += 256) facc[k] = 0ull;
+    for (int k = tid; k < fp; k += FM) {
+      fnum[k] = 0ull;
+      fden[k] = 0u;
+    }
     __syncthreads();
     const float ch_sigma = ch == 0 ? a.sigma : a.sigma * a.chroma_scale;
     const float* __restrict__ gplane = a.guide[ch];
     const float* __restrict__ nplane = a.noisy[ch];
-    for (int base = 0; base < nref; base += 4) {
-      const int r = base + wave;
-      if (r >= nref) continue;  // wave-uniform
+    for (int r = wave; r < nref; r += FM / 64) {  // wave-uniform
       const int ref_id = (r0y + r / nrx) * a.refs_x + r0x + r % nrx;
       const uint32_t* grp = a.groups + (size_t)ref_id * 17;
       const int g = (int)grp[0];
-      if (g == 0) continue;  // wave-uniform; nothing to aggregate
+      if (g == 0) continue;  // nothing to aggregate
       // patch n (A rows) and patches 4q + j (C rows) positions
       const bool actn = n < g;
       const uint32_t pn = actn ? grp[1 + n] : 0u;
@@ -773,8 +779,8 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
-        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, KF[t][0], y, 0, 0, 0);
-        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, KF[t][1], y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A0, KF(t, 0), y, 0, 0, 0);
+        y = __builtin_amdgcn_mfma_f32_16x16x32_f16(A1, KF(t, 1), y, 0, 0, 0);
         const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
         Z[t] = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(y), zero, 0, 0, 0);
       }
@@ -803,8 +809,8 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           floatx4 y = {0.0f, 0.0f, 0.0f, 0.0f};
-          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N0, KF[t][0], y, 0, 0, 0);
-          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N1, KF[t][1], y, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N0, KF(t, 0), y, 0, 0, 0);
+          y = __builtin_amdgcn_mfma_f32_16x16x32_f16(N1, KF(t, 1), y, 0, 0, 0);
           const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
           floatx4 zn = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(y), zero, 0, 0, 0);
 #pragma unroll
@@ -841,8 +847,8 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         floatx4 pv = {0.0f, 0.0f, 0.0f, 0.0f};
-        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B0, KI[t][0], pv, 0, 0, 0);
-        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B1, KI[t][1], pv, 0, 0, 0);
+        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B0, KI(t, 0), pv, 0, 0, 0);
+        pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B1, KI(t, 1), pv, 0, 0, 0);
         // lane (pixel 16t + n, q), register j -> patch 4q + j
         const int pix = 16 * t + n, dy = pix >> 3, dx = pix & 7;
         const float kw = a.kaiser[pix] * weight;
@@ -851,16 +857,16 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
         for (int j = 0; j < 4; ++j) {
           if (!cact[j]) continue;
           const int fi = (cpy[j] + dy - fy0) * fw + (cpx[j] + dx - fx0);
-          atomicAdd(&facc[2 * fi], (unsigned long long)to_fixed_m((pv[j] + level) * kw));
-          atomicAdd(&facc[2 * fi + 1], (unsigned long long)kwf);
+          atomicAdd(&fnum[fi], (unsigned long long)to_fixed_m((pv[j] + level) * kw));
+          atomicAdd(&fden[fi], (uint32_t)kwf);
         }
       }
     }
     __syncthreads();
     unsigned long long* gnum = a.num + (size_t)ch * a.width * a.height;
     unsigned long long* gden = a.den + (size_t)ch * a.width * a.height;
-    for (int k = tid; k < fw * fh; k += 256) {
-      const unsigned long long nv = facc[2 * k], dv = facc[2 * k + 1];
+    for (int k = tid; k < fp; k += FM) {
+      const unsigned long long nv = fnum[k], dv = fden[k];
       if (nv | dv) {
         const size_t gi = (size_t)(fy0 + k / fw) * a.width + fx0 + k % fw;
         if (nv) atomicAdd(&gnum[gi], nv);
@@ -869,6 +875,8 @@ __global__ void __launch_bounds__(256) bm3d_filter_mfma_kernel(Bm3dFilterArgs a,
     }
     __syncthreads();
   }
+#undef KF
+#undef KI
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
@@ -928,14 +936,14 @@ void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out) {
 }
 
 size_t bm3d_filter_mfma_smem(int footprint_pixels) {
-  return (size_t)footprint_pixels * 2 * sizeof(unsigned long long) + (size_t)4 * 16 * 64 * sizeof(_Float16);
+  return kFilterFixedLds + (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
 }
 
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
                                    hipStream_t s) {
   const half8* f = reinterpret_cast<const half8*>(frags);
-  if (step1) hipLaunchKernelGGL(bm3d_filter_mfma_kernel<true>, dim3(nblocks), dim3(256), smem, s, a, f);
-  else hipLaunchKernelGGL(bm3d_filter_mfma_kernel<false>, dim3(nblocks), dim3(256), smem, s, a, f);
+  if (step1) hipLaunchKernelGGL(bm3d_filter_mfma_kernel<true>, dim3(nblocks), dim3(FM), smem, s, a, f);
+  else hipLaunchKernelGGL(bm3d_filter_mfma_kernel<false>, dim3(nblocks), dim3(FM), smem, s, a, f);
   return hipGetLastError();
 }
 
