@@ -27,11 +27,16 @@ namespace {
 
 __device__ __forceinline__ long long to_fixed(float v) {
   const float s = v * 100000.0f;
+  if (fabsf(s) < 2147483648.0f) return (long long)(int)s;  // truncation; the common case
   if (isnan(s)) return 0;
   if (s >= 9.2233720368547758e18f) return INT64_MAX;
   if (s <= -9.2233720368547758e18f) return INT64_MIN;
   return (long long)s;  // truncation toward zero
 }
+
+// to_fixed for a weight in [0, 1] (Kaiser sample x group weight, both <= 1):
+// the product is below 2^31, so the truncation is the 32-bit conversion.
+__device__ __forceinline__ long long to_fixed_unit(float w) { return (long long)(int)(w * 100000.0f); }
 
 // ------------------------------------------------------------------ colour
 __global__ void rgb_to_ycbcr_kernel(const float* __restrict__ rgb, size_t n, float* __restrict__ y,
@@ -126,7 +131,9 @@ __device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int 
         const float* row = win + dy * span_x;
         float wv[Q + 7], rv[8];
 #pragma unroll
-        for (int t = 0; t < Q + 7; ++t) wv[t] = row[cx0 + t < span_x ? t : 0];  // past the row only for q >= nx (dead)
+        // values past the row end (at most Q - 1, read from the next row or the
+        // following LDS array) only reach candidates with cx >= nx, which are dead
+        for (int t = 0; t < Q + 7; ++t) wv[t] = row[t];
 #pragma unroll
         for (int t = 0; t < 8; ++t) rv[t] = rp[dy * 8 + t];
 #pragma unroll
@@ -272,6 +279,19 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
 }
 
 // --------------------------------------------------------------- filtering
+// DCT, IDCT and Kaiser tables (DctTables, bm3d.rs:417-461) in constant memory,
+// copied in by bm3d_upload_tables before each filter launch.  The kernels read
+// them through a pointer the compiler cannot prove loop-invariant, so each
+// table is fetched into scalar registers next to its use instead of 192 values
+// being held (and spilled) for the whole kernel.
+__constant__ float g_bm3d_tables[3 * 64];
+typedef const __attribute__((address_space(4))) float* ctab_ptr;
+__device__ __forceinline__ ctab_ptr bm3d_table(int which) {
+  ctab_ptr p = (ctab_ptr)g_bm3d_tables + which * 64;
+  asm volatile("" : "+s"(p));
+  return p;
+}
+
 // A wave filters four (reference patch, channel) groups at once: a 16-lane
 // slot per group, lane k of a slot holding patch k of the group (all 64
 // samples in registers).  The 8x8 DCTs are register dot products in the
@@ -281,14 +301,14 @@ __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
 constexpr int FT = 256;
 
 // out[j][k] = sum_n (sum_m x[n][m] C[k][m]) C[j][n]   (dct_2d_8x8, bm3d.rs:526-536)
-__device__ __forceinline__ void dct2_regs(float* x, const float* __restrict__ C) {
+__device__ __forceinline__ void dct2_regs(float* x, ctab_ptr C) {
   float y[64];
 #pragma unroll
   for (int r = 0; r < 8; ++r)
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const float* xr = x + r * 8;
-      const float* ck = C + k * 8;
+      const auto ck = C + k * 8;
       y[r * 8 + k] = xr[0] * ck[0] + xr[1] * ck[1] + xr[2] * ck[2] + xr[3] * ck[3] + xr[4] * ck[4] + xr[5] * ck[5] +
                      xr[6] * ck[6] + xr[7] * ck[7];
     }
@@ -296,20 +316,20 @@ __device__ __forceinline__ void dct2_regs(float* x, const float* __restrict__ C)
   for (int j = 0; j < 8; ++j)
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const float* cj = C + j * 8;
+      const auto cj = C + j * 8;
       x[j * 8 + k] = y[0 * 8 + k] * cj[0] + y[1 * 8 + k] * cj[1] + y[2 * 8 + k] * cj[2] + y[3 * 8 + k] * cj[3] +
                      y[4 * 8 + k] * cj[4] + y[5 * 8 + k] * cj[5] + y[6 * 8 + k] * cj[6] + y[7 * 8 + k] * cj[7];
     }
 }
 
 // u[k][n] = sum_j x[j][k] D[n][j]; out[n][m] = sum_k u[k][n] D[m][k]   (idct_2d_8x8, bm3d.rs:538-548)
-__device__ __forceinline__ void idct2_regs(float* x, const float* __restrict__ D) {
+__device__ __forceinline__ void idct2_regs(float* x, ctab_ptr D) {
   float u[64];
 #pragma unroll
   for (int k = 0; k < 8; ++k)
 #pragma unroll
     for (int n = 0; n < 8; ++n) {
-      const float* dn = D + n * 8;
+      const auto dn = D + n * 8;
       u[k * 8 + n] = x[0 * 8 + k] * dn[0] + x[1 * 8 + k] * dn[1] + x[2 * 8 + k] * dn[2] + x[3 * 8 + k] * dn[3] +
                      x[4 * 8 + k] * dn[4] + x[5 * 8 + k] * dn[5] + x[6 * 8 + k] * dn[6] + x[7 * 8 + k] * dn[7];
     }
@@ -317,23 +337,43 @@ __device__ __forceinline__ void idct2_regs(float* x, const float* __restrict__ D
   for (int n = 0; n < 8; ++n)
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const float* dm = D + m * 8;
+      const auto dm = D + m * 8;
       x[n * 8 + m] = u[0 * 8 + n] * dm[0] + u[1 * 8 + n] * dm[1] + u[2 * 8 + n] * dm[2] + u[3 * 8 + n] * dm[3] +
                      u[4 * 8 + n] * dm[4] + u[5 * 8 + n] * dm[5] + u[6 * 8 + n] * dm[6] + u[7 * 8 + n] * dm[7];
     }
 }
 
 // Walsh-Hadamard across the group (walsh_hadamard_1d, bm3d.rs:595-614):
-// stage h pairs lane k with k^h; the lower lane keeps x+y, the upper x-y.
-__device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
-  for (int h = 1; h < g; h *= 2) {
-    const bool upper = (k & h) != 0;
-#pragma unroll
-    for (int i = 0; i < 64; ++i) {
-      const float o = __shfl_xor(v[i], h, 16);
-      v[i] = upper ? o - v[i] : v[i] + o;
-    }
+// stage h pairs lane k with k^h; the lower lane keeps x+y, the upper x-y
+// (computed as y + (-x), the same IEEE result).  The exchange stays inside the
+// slot's 16-lane DPP row: h = 1, 2 are quad permutes, h = 4 a half-row mirror
+// followed by the quad reversal (7 - (k^3) = k^4), h = 8 a row rotation by 8.
+template <int H>
+__device__ __forceinline__ float dpp_xor(float v) {
+  int b = __float_as_int(v);
+  if (H == 1) b = __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, true);       // quad_perm [1,0,3,2]
+  else if (H == 2) b = __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+  else if (H == 4) {
+    b = __builtin_amdgcn_update_dpp(0, b, 0x141, 0xF, 0xF, true);                 // row_half_mirror
+    b = __builtin_amdgcn_update_dpp(0, b, 0x1B, 0xF, 0xF, true);                  // quad_perm [3,2,1,0]
+  } else {
+    b = __builtin_amdgcn_update_dpp(0, b, 0x128, 0xF, 0xF, true);                 // row_ror:8
   }
+  return __int_as_float(b);
+}
+
+template <int H>
+__device__ __forceinline__ void wht_stage(float* v, int k) {
+  const unsigned sign = (k & H) ? 0x80000000u : 0u;
+#pragma unroll
+  for (int i = 0; i < 64; ++i) v[i] = dpp_xor<H>(v[i]) + __uint_as_float(__float_as_uint(v[i]) ^ sign);
+}
+
+__device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
+  if (g > 1) wht_stage<1>(v, k);
+  if (g > 2) wht_stage<2>(v, k);
+  if (g > 4) wht_stage<4>(v, k);
+  if (g > 8) wht_stage<8>(v, k);
 #pragma unroll
   for (int i = 0; i < 64; ++i) v[i] *= scale;
 }
@@ -394,7 +434,7 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
           if (!kStep1) nv[dy * 8 + dx] = act ? nplane[off] : 0.0f;
         }
       const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
-      dct2_regs(gv, a.dct);
+      dct2_regs(gv, bm3d_table(0));
       wht_lanes(gv, k, g, wscale);
       float weight;
       if (kStep1) {
@@ -412,7 +452,7 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] = gv[i];
       } else {
-        dct2_regs(nv, a.dct);
+        dct2_regs(nv, bm3d_table(0));
         wht_lanes(nv, k, g, wscale);
         // wiener_filter (bm3d.rs:257-272)
         const float s2 = ch_sigma * ch_sigma;
@@ -440,16 +480,17 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
       }
       // inverse (bm3d.rs:508-524) and aggregation into the footprint
       wht_lanes(nv, k, g, wscale);
-      idct2_regs(nv, a.idct);
+      idct2_regs(nv, bm3d_table(1));
+      const ctab_ptr kaiser = bm3d_table(2);
       if (act) {
 #pragma unroll
         for (int dy = 0; dy < 8; ++dy)
 #pragma unroll
           for (int dx = 0; dx < 8; ++dx) {
-            const float kw = a.kaiser[dy * 8 + dx] * weight;
+            const float kw = kaiser[dy * 8 + dx] * weight;
             const int fi = (py + dy - fy0) * fw + (px + dx - fx0);
             atomicAdd(&acc[fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * kw));
-            atomicAdd(&den[fi], (uint32_t)to_fixed(kw));
+            atomicAdd(&den[fi], (uint32_t)to_fixed_unit(kw));
           }
       }
     }
@@ -533,6 +574,17 @@ int grid1d(size_t n) {
 }  // namespace
 
 size_t bm3d_match_smem() { return sizeof(MatchShared); }
+
+hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* kaiser, hipStream_t s) {
+  hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_bm3d_tables), dct, 64 * sizeof(float), 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_bm3d_tables), idct, 64 * sizeof(float), 64 * sizeof(float),
+                               hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_bm3d_tables), kaiser, 64 * sizeof(float), 128 * sizeof(float),
+                               hipMemcpyHostToDevice, s);
+  return e;
+}
 size_t bm3d_filter_smem(int footprint_pixels) {
   return (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
 }

@@ -70,6 +70,8 @@ size_t bm3d_filter_mfma_smem(int footprint_pixels);
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
                                    hipStream_t s);
 hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s);
+// copies the DCT / IDCT / Kaiser tables the plain filter reads (64 floats each)
+hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* kaiser, hipStream_t s);
 hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],
                                 float* const out[3], size_t n, hipStream_t s);
 hipError_t fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);

@@ -77,6 +77,8 @@ struct HostTables {
       for (int x = 0; x < 8; ++x) {
         const float ux = (2.0f * (float)x / 7.0f) - 1.0f, uy = (2.0f * (float)y / 7.0f) - 1.0f;
         kaiser[y * 8 + x] = (i0(beta * sqrtf(1.0f - ux * ux)) / i0b) * (i0(beta * sqrtf(1.0f - uy * uy)) / i0b);
+        // the filters' 32-bit fixed-point path for the weights relies on this
+        if (!(kaiser[y * 8 + x] >= 0.0f && kaiser[y * 8 + x] <= 1.0f)) throw Error{IMOPS_E_STATE, "internal: Kaiser table outside [0, 1]"};
       }
   }
 };
@@ -135,6 +137,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   const bool use_mfma = p.precision != IMOPS_BM3D_FP32 && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
   static const HostTables tables;
   const bool mixed_filter = p.precision == IMOPS_BM3D_MIXED;
+  if (!mixed_filter) IMOPS_HIP_CHECK(bm3d_upload_tables(tables.dct, tables.idct, tables.kaiser, s));
   if (mixed_filter) {
     static const std::vector<uint16_t> frags = [] {
       std::vector<uint16_t> v(kBm3dMfmaFragBytes / 2);

@@ -666,6 +666,7 @@ typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ long long to_fixed_m(float v) {
   const float s = v * 100000.0f;
+  if (fabsf(s) < 2147483648.0f) return (long long)(int)s;  // truncation; the common case
   if (isnan(s)) return 0;
   if (s >= 9.2233720368547758e18f) return INT64_MAX;
   if (s <= -9.2233720368547758e18f) return INT64_MIN;
@@ -852,7 +853,7 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
         // lane (pixel 16t + n, q), register j -> patch 4q + j
         const int pix = 16 * t + n, dy = pix >> 3, dx = pix & 7;
         const float kw = a.kaiser[pix] * weight;
-        const long long kwf = to_fixed_m(kw);
+        const long long kwf = (long long)(int)(kw * 100000.0f);  // kw in [0, 1]: exact 32-bit truncation
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if (!cact[j]) continue;
