@@ -66,106 +66,119 @@ struct MatchShared {
 constexpr int kMatchChunks = (kMaxWin * kMaxWin + 63) / 64;
 constexpr int kChunksPerWave = (kMatchChunks + MT / 64 - 1) / (MT / 64);
 
-__device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rx = (ref_id % a.refs_x) * a.stride, ry = (ref_id / a.refs_x) * a.stride;
-  const int half = a.window / 2;
-  const int wmax = a.width - 8, hmax = a.height - 8;
-  const int sx0 = rx >= half ? rx - half : 0, sx1 = min(rx + half, wmax);
-  const int sy0 = ry >= half ? ry - half : 0, sy1 = min(ry + half, hmax);
-  const int nx = sx1 - sx0 + 1, ny = sy1 - sy0 + 1;
-  const int span_x = nx + 7, span_y = ny + 7;
-
-  for (int ch = 0; ch < 3; ++ch) {
-    const float* plane = a.guide[ch];
-    for (int k = tid; k < span_x * span_y; k += MT) {
-      const int yy = k / span_x, xx = k - yy * span_x;
-      S.win[ch][yy * span_x + xx] = plane[(size_t)(sy0 + yy) * a.width + sx0 + xx];
-    }
-    if (tid < 64) S.ref[ch][tid] = plane[(size_t)(ry + (tid >> 3)) * a.width + rx + (tid & 7)];
+// compute_ssd_flat (bm3d.rs:367-391) for Q horizontally adjacent candidates of
+// one channel: the running sum over rows in the reference's order, returning
+// the sum at the first row end where it exceeds stop, else the full sum.  Rows
+// are summed unconditionally; the sum never decreases (non-negative terms), so
+// the early-exit value is the smallest row-end sum above stop.  For sums and
+// stop >= 0 the float bit patterns order like the values, so that value is
+// tracked with one unsigned difference and min per row:
+// bits(acc) - (bits(stop) + 1) < 2^31 exactly when acc > stop.  (A NaN sum
+// never meets the threshold tests, as in the reference, whichever branch it
+// takes here.)
+template <int Q>
+__device__ __forceinline__ void ssd_channel(const float* win, const float* ref, int span_x, const uint32_t* stop1,
+                                            float* out) {
+  float acc[Q];
+  uint32_t mind[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    acc[q] = 0.0f;
+    mind[q] = 0xffffffffu;
   }
-  __syncthreads();
+#pragma unroll 1
+  for (int dy = 0; dy < 8; ++dy) {
+    const float* row = win + dy * span_x;
+    float wv[Q + 7], rv[8];
+    // values past the row end (at most Q - 2, read from the next row or the
+    // following LDS array) only reach candidates with cx >= nx, which are dead
+#pragma unroll
+    for (int t = 0; t < Q + 7; ++t) wv[t] = row[t];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) rv[t] = ref[dy * 8 + t];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      float a_ = acc[q];
+#pragma unroll
+      for (int dx = 0; dx < 8; ++dx) {
+        const float diff = wv[q + dx] - rv[dx];
+        a_ += diff * diff;
+      }
+      acc[q] = a_;
+      const uint32_t d = __float_as_uint(a_) - stop1[q];
+      mind[q] = d < mind[q] ? d : mind[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) out[q] = mind[q] < 0x80000000u ? __uint_as_float(mind[q] + stop1[q]) : acc[q];
+}
 
-  const float thr = a.threshold * 64.0f;
-  const int ncand = nx * ny;
-  // Candidate SSDs in scan order; the reference position is excluded here and
-  // prepended below with distance 0.  A thread evaluates Q horizontally
-  // adjacent candidates so each staged window row is read once for all of them.
-  constexpr int Q = 3;
+// Widest candidate block per thread that keeps the window in one or two passes
+// of the workgroup (cost ~ passes x per-row work).
+__device__ __forceinline__ int ssd_block_width(int nx, int ny) {
+  int best = 2, best_cost = 1 << 30;
+  const int widths[3] = {2, 4, 7};
+  for (int i = 0; i < 3; ++i) {
+    const int q = widths[i];
+    const int passes = ((nx + q - 1) / q * ny + MT - 1) / MT;
+    const int cost = passes * (26 * q + (q + 7) / 2 + 4);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = q;
+    }
+  }
+  return best;
+}
+
+// Joint 3-channel distances of every candidate of the staged window, in scan
+// order, into S.dist (+inf when rejected) -- block_matching_joint's per-candidate
+// logic (bm3d.rs:318-350): channel 1 and 2 stop at thr minus the distance so
+// far, a candidate whose partial sum exceeds thr is dropped, the total must be
+// below thr.  The reference position itself is excluded (prepended later).
+template <int Q>
+__device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, int span_x, int ref_cx, int ref_cy, float thr) {
+  const int tid = threadIdx.x;
   const int nbx = (nx + Q - 1) / Q;
   const int nblk = nbx * ny;
+  const uint32_t thr1 = __float_as_uint(thr) + 1u;
   for (int base = 0; base < nblk; base += MT) {
     const int blk = base + tid;
     const bool vblk = blk < nblk;
     const int cy = vblk ? blk / nbx : 0;
     const int cx0 = vblk ? (blk - cy * nbx) * Q : 0;
     bool live[Q];
-    float d0[Q], d1[Q], d2[Q], stop[Q];
+    bool any = false;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int cx = cx0 + q;
-      live[q] = vblk && cx < nx && !((sx0 + cx == rx) && (sy0 + cy == ry));
+      live[q] = vblk && cx < nx && !(cx == ref_cx && cy == ref_cy);
+      any |= live[q];
+    }
+    float d0[Q], d1[Q], d2[Q];
+    uint32_t stop1[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      stop1[q] = thr1;
       d0[q] = d1[q] = d2[q] = 0.0f;
-      stop[q] = thr;
     }
+    const int off = cy * span_x + cx0;
+    if (__any(any)) ssd_channel<Q>(S.win[0] + off, S.ref[0], span_x, stop1, d0);
+    any = false;
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-      bool any_live = false;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) any_live |= live[q];
-      if (!__any(any_live)) break;  // no candidate of this wave still needs this channel
-      const float* win = &S.win[ch][cy * span_x + cx0];
-      const float* rp = S.ref[ch];
-      float acc[Q];
-      bool run[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        acc[q] = 0.0f;
-        run[q] = live[q];
-      }
-      for (int dy = 0; dy < 8; ++dy) {
-        bool any_run = false;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) any_run |= run[q];
-        if (!__any(any_run)) break;
-        const float* row = win + dy * span_x;
-        float wv[Q + 7], rv[8];
-#pragma unroll
-        // values past the row end (at most Q - 1, read from the next row or the
-        // following LDS array) only reach candidates with cx >= nx, which are dead
-        for (int t = 0; t < Q + 7; ++t) wv[t] = row[t];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) rv[t] = rp[dy * 8 + t];
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          if (run[q]) {
-            float a_ = acc[q];
-#pragma unroll
-            for (int dx = 0; dx < 8; ++dx) {
-              const float diff = wv[q + dx] - rv[dx];
-              a_ += diff * diff;
-            }
-            acc[q] = a_;
-            if (a_ > stop[q]) run[q] = false;  // compute_ssd_flat returns the partial sum here
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        if (ch == 0) d0[q] = acc[q];
-        else if (ch == 1) d1[q] = acc[q];
-        else d2[q] = acc[q];
-        if (live[q]) {
-          if (ch == 0) {
-            if (d0[q] > thr) live[q] = false;
-            else stop[q] = thr - d0[q];
-          } else if (ch == 1) {
-            if (d0[q] + d1[q] > thr) live[q] = false;
-            else stop[q] = thr - (d0[q] + d1[q]);
-          }
-        }
-      }
+    for (int q = 0; q < Q; ++q) {
+      if (d0[q] > thr) live[q] = false;
+      stop1[q] = __float_as_uint(thr - d0[q]) + 1u;
+      any |= live[q];
     }
+    if (__any(any)) ssd_channel<Q>(S.win[1] + off, S.ref[1], span_x, stop1, d1);
+    any = false;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if (d0[q] + d1[q] > thr) live[q] = false;
+      stop1[q] = __float_as_uint(thr - (d0[q] + d1[q])) + 1u;
+      any |= live[q];
+    }
+    if (__any(any)) ssd_channel<Q>(S.win[2] + off, S.ref[2], span_x, stop1, d2);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int cx = cx0 + q;
@@ -177,6 +190,40 @@ __device__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int 
         }
         S.dist[cy * nx + cx] = dist;
       }
+    }
+  }
+}
+
+__device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rx = (ref_id % a.refs_x) * a.stride, ry = (ref_id / a.refs_x) * a.stride;
+  const int half = a.window / 2;
+  const int wmax = a.width - 8, hmax = a.height - 8;
+  const int sx0 = rx >= half ? rx - half : 0, sx1 = min(rx + half, wmax);
+  const int sy0 = ry >= half ? ry - half : 0, sy1 = min(ry + half, hmax);
+  const int nx = sx1 - sx0 + 1, ny = sy1 - sy0 + 1;
+  const int span_x = nx + 7, span_y = ny + 7;
+
+  // one window row per wave and step, one lane per column (span_x <= 48)
+  for (int ch = 0; ch < 3; ++ch) {
+    const float* plane = a.guide[ch];
+    for (int yy = wave; yy < span_y; yy += MT / 64)
+      if (lane < span_x) S.win[ch][yy * span_x + lane] = plane[(size_t)(sy0 + yy) * a.width + sx0 + lane];
+    if (tid < 64) S.ref[ch][tid] = plane[(size_t)(ry + (tid >> 3)) * a.width + rx + (tid & 7)];
+  }
+  __syncthreads();
+
+  const float thr = a.threshold * 64.0f;
+  const int ncand = nx * ny;
+  const int ref_cx = rx - sx0, ref_cy = ry - sy0;
+  if (!(thr >= 0.0f)) {
+    // every candidate's first row already exceeds a negative threshold (or thr is NaN)
+    for (int c = tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
+  } else {
+    switch (ssd_block_width(nx, ny)) {
+      case 7: ssd_candidates<7>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
+      case 4: ssd_candidates<4>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
+      default: ssd_candidates<2>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
     }
   }
   __syncthreads();
