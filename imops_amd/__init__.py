@@ -276,6 +276,17 @@ def run_pixel_pipeline(image: Image, config: PipelineConfig) -> Image:
     return image
 
 
+def run_burst(frames: list, config: PipelineConfig, graphs: bool = True) -> list:
+    """The stage list over every frame (imops_pipeline_run_burst); frames of one
+    shape after the first replay a HIP graph captured on a staging image."""
+    if not frames:
+        return frames
+    ctx = frames[0].ctx
+    arr = (C.c_void_p * len(frames))(*[f.handle for f in frames])
+    check(lib().imops_pipeline_run_burst(ctx.handle, config._h, arr, len(frames), 1 if graphs else 0))
+    return frames
+
+
 def synth_scene_rgb(width: int, height: int, frame: int = 0) -> np.ndarray:
     out = np.empty((height, width, 3), np.float32)
     check(lib().imops_synth_scene_rgb(fptr(out), width, height, frame))

@@ -115,8 +115,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   ctx->scr[SCR_BM3D_0].reserve(12 * n * sizeof(float));
   ctx->scr[SCR_BM3D_1].reserve(6 * n * sizeof(unsigned long long));
   const size_t gwords = (nrefs ? nrefs : 1) * 17;
-  const size_t frag_word = (gwords + nrefs + 64 + 3) & ~(size_t)3;  // FP16 DCT fragments of the mixed filter
-  ctx->scr[SCR_BM3D_2].reserve(frag_word * sizeof(uint32_t) + kBm3dMfmaFragBytes);
+  ctx->scr[SCR_BM3D_2].reserve((gwords + nrefs + 64) * sizeof(uint32_t));
 
   float* f = ctx->scr[SCR_BM3D_0].as<float>();
   float* noisy[3] = {f, f + n, f + 2 * n};
@@ -130,22 +129,27 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   uint32_t* groups = ctx->scr[SCR_BM3D_2].as<uint32_t>();
   uint32_t* overflow_count = groups + gwords;           // 64-word aligned slot
   uint32_t* overflow_list = overflow_count + 64;
-  void* filter_frags = groups + frag_word;
   // Matrix-core matching needs one 8x8 block of reference patches' search
   // region in LDS; larger geometries use the plain matcher.
   Bm3dMfmaArgs mf = bm3d_mfma_layout(w, h, p.stride, p.window);
   const bool use_mfma = p.precision != IMOPS_BM3D_FP32 && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
   static const HostTables tables;
   const bool mixed_filter = p.precision == IMOPS_BM3D_MIXED;
-  if (!mixed_filter) IMOPS_HIP_CHECK(bm3d_upload_tables(tables.dct, tables.idct, tables.kaiser, s));
-  if (mixed_filter) {
+  // constant tables, uploaded once per context (the host sources are static)
+  if (!ctx->bm3d_tables_ready) {
+    if (ctx->capturing) throw Error{IMOPS_E_STATE, "BM3D: tables not uploaded before stream capture"};
     static const std::vector<uint16_t> frags = [] {
       std::vector<uint16_t> v(kBm3dMfmaFragBytes / 2);
       bm3d_filter_mfma_fragments(tables.dct, v.data());
       return v;
     }();
-    IMOPS_HIP_CHECK(hipMemcpyAsync(filter_frags, frags.data(), kBm3dMfmaFragBytes, hipMemcpyHostToDevice, s));
+    ctx->bm3d_frags.reserve(kBm3dMfmaFragBytes);
+    IMOPS_HIP_CHECK(bm3d_upload_tables(tables.dct, tables.idct, tables.kaiser, s));
+    IMOPS_HIP_CHECK(hipMemcpyAsync(ctx->bm3d_frags.ptr, frags.data(), kBm3dMfmaFragBytes, hipMemcpyHostToDevice, s));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+    ctx->bm3d_tables_ready = true;
   }
+  const void* filter_frags = ctx->bm3d_frags.ptr;
 
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -197,7 +201,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
           mf.blocks_x = (refs_x + 7) / 8;
           mf.n_blocks = mf.blocks_x * ((refs_y + 7) / 8);
           IMOPS_HIP_CHECK(bm3d_match_mfma_launch(mf, s));
-          if (ctx->timer.enabled) {  // diagnostics: patches sent to the plain matcher / selected approximately
+          if (ctx->timer.enabled && !ctx->capturing) {  // diagnostics: patches sent to the plain matcher / selected approximately
             uint32_t nfb[2] = {0, 0};
             IMOPS_HIP_CHECK(hipMemcpyAsync(nfb, overflow_count, sizeof nfb, hipMemcpyDeviceToHost, s));
             IMOPS_HIP_CHECK(hipStreamSynchronize(s));

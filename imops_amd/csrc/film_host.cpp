@@ -6,6 +6,7 @@
 // reference evaluates it, so the device kernels only stream pixels.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -280,86 +281,122 @@ struct FilmBuffers {
   float* inh;        // E planes (DIR inhibitors)
   float* tmp;        // blur scratch
   float* bounce;     // halation bounce / grain noise
-  float* taps;       // blur taps (up to 5 kernels)
   double* partial;   // grain variance partials + norm
-  FilmDeviceTables* tables;
 };
 
-// Expose -> develop -> scan (NegativeLinear) or with invert when T.invert.
-void run_film(hipStream_t s, const HostStock& hs, FilmDeviceTables& T, const float* rgb_in, float* rgb_out, int w,
-              int h, float pitch, float shutter, uint64_t seed, FilmBuffers& B, imops::DevBuf& tapbuf) {
-  const size_t n = (size_t)w * h;
+unsigned bits_of(float f) {
+  unsigned u;
+  std::memcpy(&u, &f, sizeof u);
+  return u;
+}
+
+// Per-shape constants of a film run: blur taps and the device tables with the
+// per-call fields (reciprocity, bounce gains, PositiveLinear inversion) filled in.
+struct FilmPrepared {
+  imops::DevBuf taps, tables;
+  int radius[5] = {}, off[5] = {};
+  float sigmas[5] = {};
+  bool do_wide = false;
+  FilmDeviceTables T{};  // host copy of what `tables` holds
+  ~FilmPrepared() {
+    taps.release();
+    tables.release();
+  }
+};
+
+// Taps and per-call table fields at this pitch and shutter (radiance.rs:43-49,
+// halation.rs:24-93, diffusion.rs, grain.rs:53-100); uploads both and waits
+// (the host sources are locals).
+void prepare_film(hipStream_t s, const HostStock& hs, const FilmDeviceTables& base, float pitch, float shutter,
+                  FilmPrepared& P) {
   const Stock& st = *hs.s;
-  const int E = T.n_emulsions;
-  // per-call constants: reciprocity (radiance.rs:43-49), bounce gains
+  P.T = base;
+  const int E = P.T.n_emulsions;
   const float sigma_local = std::max(st.psf_local_um / std::max(pitch, 1e-6f), 0.0f);
   const float sigma_wide = std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
-  const bool do_wide = sigma_wide >= 1e-3f && hs.wide_reflect && E > 0;
+  P.do_wide = sigma_wide >= 1e-3f && hs.wide_reflect && E > 0;
   for (int e = 0; e < E; ++e) {
     const double p = (double)hs.recip_p[e];
-    T.em[e].eta = ((double)shutter <= 1.0 || p >= 0.9999) ? 1.0f
-                                                          : (float)std::pow(std::max((double)shutter, 1.0), p - 1.0);
-    T.em[e].gain = do_wide ? hs.bleed_gain[e] : 0.0f;
+    P.T.em[e].eta = ((double)shutter <= 1.0 || p >= 0.9999) ? 1.0f
+                                                            : (float)std::pow(std::max((double)shutter, 1.0), p - 1.0);
+    P.T.em[e].gain = P.do_wide ? hs.bleed_gain[e] : 0.0f;
   }
   // blur tap sets: local scatter, halation, DIR, adjacency, grain
-  const float sig_dir = st.dir_diffusion_um / std::max(pitch, 1e-6f);
-  const float sig_adj = st.developer_diffusion_um / std::max(pitch, 1e-6f);
-  const float sig_grain = std::max(3.0f / std::max(pitch, 1e-6f), 1.0f);
-  const float sigmas[5] = {sigma_local, sigma_wide, sig_dir, sig_adj, sig_grain};
-  int radius[5], off[5];
+  P.sigmas[0] = sigma_local;
+  P.sigmas[1] = sigma_wide;
+  P.sigmas[2] = st.dir_diffusion_um / std::max(pitch, 1e-6f);
+  P.sigmas[3] = st.developer_diffusion_um / std::max(pitch, 1e-6f);
+  P.sigmas[4] = std::max(3.0f / std::max(pitch, 1e-6f), 1.0f);
   std::vector<float> all;
   for (int k = 0; k < 5; ++k) {
-    off[k] = (int)all.size();
-    if (sigmas[k] >= 1e-3f) {
-      auto t = gauss_taps(sigmas[k], &radius[k]);
+    P.off[k] = (int)all.size();
+    if (P.sigmas[k] >= 1e-3f) {
+      auto t = gauss_taps(P.sigmas[k], &P.radius[k]);
       all.insert(all.end(), t.begin(), t.end());
     } else {
-      radius[k] = 0;
+      P.radius[k] = 0;
     }
   }
-  tapbuf.reserve(std::max<size_t>(all.size(), 1) * sizeof(float));
-  IMOPS_HIP_CHECK(hipMemcpyAsync(tapbuf.ptr, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipMemcpyAsync(B.tables, &T, sizeof(T), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipStreamSynchronize(s));  // the host sources of both uploads are locals
-  const float* taps = tapbuf.as<float>();
+  P.taps.reserve(std::max<size_t>(all.size(), 1) * sizeof(float));
+  P.tables.reserve(sizeof(FilmDeviceTables));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(P.taps.ptr, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(P.tables.ptr, &P.T, sizeof(P.T), hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipStreamSynchronize(s));  // `all` is a local
+}
 
-  IMOPS_HIP_CHECK(film_expose(rgb_in, n, B.tables, B.absorbed, s));
+// Expose -> develop -> scan (NegativeLinear), or with the inversion when the
+// prepared tables carry it.  Kernels only: nothing is uploaded or awaited.
+void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const float* rgb_in, float* rgb_out, int w,
+              int h, float pitch, uint64_t seed, FilmBuffers& B) {
+  const size_t n = (size_t)w * h;
+  const Stock& st = *hs.s;
+  const int E = P.T.n_emulsions;
+  const float* taps = P.taps.as<float>();
+  const FilmDeviceTables* T = P.tables.as<FilmDeviceTables>();
+  const float sigma_local = P.sigmas[0], sig_dir = P.sigmas[2], sig_adj = P.sigmas[3];
+
+  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s));
   if (sigma_local >= 1e-3f)  // local gelatin scatter, mix 0.18 (halation.rs:38-44)
     for (int e = 0; e < E; ++e) {
       float* p = B.absorbed + (size_t)e * n;
-      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, p, w, h, taps + off[0], radius[0], BLUR_MIX, 0.18f, s));
+      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, p, w, h, taps + P.off[0], P.radius[0], BLUR_MIX, 0.18f, s));
     }
-  if (do_wide) {
+  if (P.do_wide) {
     const float* deep = B.absorbed + (size_t)(E - 1) * n;
-    IMOPS_HIP_CHECK(film_blur(deep, B.tmp, deep, B.bounce, w, h, taps + off[1], radius[1], BLUR_STORE, 0.0f, s));
+    IMOPS_HIP_CHECK(film_blur(deep, B.tmp, deep, B.bounce, w, h, taps + P.off[1], P.radius[1], BLUR_STORE, 0.0f, s));
   }
-  IMOPS_HIP_CHECK(film_capture_reduce(B.absorbed, do_wide ? B.bounce : nullptr, n, B.tables, B.image, B.mask, s));
+  IMOPS_HIP_CHECK(film_capture_reduce(B.absorbed, P.do_wide ? B.bounce : nullptr, n, T, B.image, B.mask, s));
   // DIR inhibition (diffusion.rs:39-84)
   if (E > 0 && st.n_dir > 0 && sig_dir >= 1e-3f) {
     for (int e = 0; e < E; ++e) {
       const float* p = B.image + (size_t)e * n;
       float* q = B.inh + (size_t)e * n;
-      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, q, w, h, taps + off[2], radius[2], BLUR_STORE, 0.0f, s));
+      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, q, w, h, taps + P.off[2], P.radius[2], BLUR_STORE, 0.0f, s));
     }
-    IMOPS_HIP_CHECK(film_dir_apply(B.image, B.inh, n, B.tables, s));
+    IMOPS_HIP_CHECK(film_dir_apply(B.image, B.inh, n, T, s));
   }
   // adjacency (diffusion.rs:11-28)
   if (!(std::fabs(st.adjacency_beta) < 1e-8f || sig_adj < 1e-3f))
     for (int e = 0; e < E; ++e) {
       float* p = B.image + (size_t)e * n;
-      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, p, w, h, taps + off[3], radius[3], BLUR_ADJ, st.adjacency_beta, s));
+      IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, p, w, h, taps + P.off[3], P.radius[3], BLUR_ADJ, st.adjacency_beta, s));
     }
   // grain (grain.rs:53-100)
   for (int e = 0; e < E; ++e) {
-    const float d_max = T.em[e].d_max;
+    const float d_max = P.T.em[e].d_max;
     const float kappa = (hs.kappa_ref[e] / std::max(pitch, 1e-6f)) * 0.15f;
     if (kappa <= 0.0f || d_max <= 0.0f) continue;
     const uint64_t key0 = seed * 0xD1B54A32D192ED03ull + (uint64_t)e * 0x9E3779B97F4A7C15ull;
-    IMOPS_HIP_CHECK(film_grain(B.image + (size_t)e * n, B.bounce, B.tmp, w, h, key0, taps + off[4], radius[4],
+    IMOPS_HIP_CHECK(film_grain(B.image + (size_t)e * n, B.bounce, B.tmp, w, h, key0, taps + P.off[4], P.radius[4],
                                B.partial, d_max, kappa, s));
   }
-  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, B.tables, rgb_out, s));
+  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, T, rgb_out, s));
 }
+
+// Prepared runs of a context, by (stock, format, width, shutter, output).
+struct FilmCache : imops::StageCache {
+  std::vector<std::pair<std::string, std::unique_ptr<FilmPrepared>>> runs;
+};
 
 }  // namespace
 
@@ -404,7 +441,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   const size_t planes = (size_t)(4 * E + 2);
   ctx->scr[SCR_FILM_0].reserve(planes * n * sizeof(float));
   ctx->scr[SCR_FILM_1].reserve(planes * mid_n * sizeof(float) + 2 * mid_n * 3 * sizeof(float));
-  ctx->scr[SCR_FILM_2].reserve(sizeof(FilmDeviceTables) + (kFilmReduceBlocks + 1) * sizeof(double));
+  ctx->scr[SCR_FILM_2].reserve((kFilmReduceBlocks + 1) * sizeof(double));
   auto layout = [&](float* base, size_t np, FilmBuffers& B) {
     B.absorbed = base;
     B.image = base + (size_t)E * np;
@@ -412,56 +449,73 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
     B.inh = base + (size_t)3 * E * np;
     B.tmp = base + (size_t)4 * E * np;
     B.bounce = base + (size_t)(4 * E + 1) * np;
-    B.tables = ctx->scr[SCR_FILM_2].as<FilmDeviceTables>();
-    B.partial = reinterpret_cast<double*>(reinterpret_cast<char*>(ctx->scr[SCR_FILM_2].ptr) + sizeof(FilmDeviceTables));
+    B.partial = reinterpret_cast<double*>(ctx->scr[SCR_FILM_2].ptr);
   };
-  FilmDeviceTables T = hs.t;
-  T.invert = 0;
-
-  if (p.output == 1 && !hs.any_reversal) {
-    // PositiveLinear: Dmin of the film base and the mean of a 32x32 flat
-    // mid-gray run with seed 0 at this image's pitch and shutter (film/mod.rs:110-121, 136-151).
-    FilmBuffers M;
-    float* mb = ctx->scr[SCR_FILM_1].as<float>();
-    layout(mb, mid_n, M);
-    float* gray = mb + planes * mid_n;
-    float* gout = gray + mid_n * 3;
-    const float iso = std::max(hs.s->box_iso, 1.0f);
-    const float gv = (float)(12.5 * (double)0.185f * (8.0 * 8.0) / ((double)(1.0f / iso) * (double)iso));
-    std::vector<float> hgray(mid_n * 3, gv), hout(mid_n * 3);
-    IMOPS_HIP_CHECK(hipMemcpyAsync(gray, hgray.data(), hgray.size() * sizeof(float), hipMemcpyHostToDevice, s));
-    run_film(s, hs, T, gray, gout, 32, 32, pitch, shutter, 0, M, ctx->scr[SCR_FILM_3]);
-    IMOPS_HIP_CHECK(hipMemcpyAsync(hout.data(), gout, hout.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-    IMOPS_HIP_CHECK(hipStreamSynchronize(s));
-    double acc[3] = {0, 0, 0};
-    for (size_t q = 0; q < mid_n; ++q)
-      for (int c = 0; c < 3; ++c) acc[c] += (double)hout[q * 3 + c];
-    float mid[3], dmin[3];
-    const double pk = std::max(std::max(std::max(hs.dmin_raw[0], hs.dmin_raw[1]), hs.dmin_raw[2]), 1e-12);
-    for (int c = 0; c < 3; ++c) {
-      mid[c] = (float)(acc[c] / (double)mid_n);
-      dmin[c] = (float)(hs.dmin_raw[c] / pk);
-    }
-    // invert constants (invert.rs:41-77)
-    const float eps = 1e-6f;
-    for (int c = 0; c < 3; ++c) {
-      const float dm = std::fmax(dmin[c], eps);
-      float mt = mid[c] / dm;
-      mt = mt < eps ? eps : (mt > 1.0f - eps ? 1.0f - eps : mt);
-      const float inv_mid = std::fmax(1.0f / mt - 1.0f, 0.0f);
-      const float sh = std::fmax(inv_mid * 8.0f, eps);
-      const float iv = std::fmax(inv_mid, 0.0f);
-      const float soft = std::fmax(iv / (1.0f + iv / std::fmax(sh, 1e-6f)), eps);
-      T.inv_dmin[c] = dm;
-      T.inv_shoulder[c] = sh;
-      T.inv_gain[c] = 0.185f / soft;
-    }
-    T.invert = 1;
-  }
   FilmBuffers B;
   layout(ctx->scr[SCR_FILM_0].as<float>(), n, B);
+
+  // prepared constants of this shape (computed once per context)
+  char keybuf[160];
+  snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%08x/%d", p.stock, p.format, w, bits_of(shutter), p.output);
+  const std::string key(keybuf);
+  if (!ctx->film_cache) ctx->film_cache.reset(new FilmCache());
+  FilmCache& cache = static_cast<FilmCache&>(*ctx->film_cache);
+  FilmPrepared* P = nullptr;
+  for (auto& r : cache.runs)
+    if (r.first == key) P = r.second.get();
+  if (!P) {
+    if (ctx->capturing) throw Error{IMOPS_E_STATE, "film: shape not prepared before stream capture"};
+    if (cache.runs.size() >= 64) cache.runs.clear();
+    cache.runs.emplace_back(key, std::unique_ptr<FilmPrepared>(new FilmPrepared()));
+    P = cache.runs.back().second.get();
+    FilmDeviceTables T = hs.t;
+    T.invert = 0;
+    prepare_film(s, hs, T, pitch, shutter, *P);
+    if (p.output == 1 && !hs.any_reversal) {
+      // PositiveLinear: Dmin of the film base and the mean of a 32x32 flat
+      // mid-gray run with seed 0 at this image's pitch and shutter (film/mod.rs:110-121, 136-151).
+      FilmBuffers M;
+      float* mb = ctx->scr[SCR_FILM_1].as<float>();
+      layout(mb, mid_n, M);
+      float* gray = mb + planes * mid_n;
+      float* gout = gray + mid_n * 3;
+      const float iso = std::max(hs.s->box_iso, 1.0f);
+      const float gv = (float)(12.5 * (double)0.185f * (8.0 * 8.0) / ((double)(1.0f / iso) * (double)iso));
+      std::vector<float> hgray(mid_n * 3, gv), hout(mid_n * 3);
+      IMOPS_HIP_CHECK(hipMemcpyAsync(gray, hgray.data(), hgray.size() * sizeof(float), hipMemcpyHostToDevice, s));
+      run_film(s, hs, *P, gray, gout, 32, 32, pitch, 0, M);
+      IMOPS_HIP_CHECK(hipMemcpyAsync(hout.data(), gout, hout.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+      IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+      double acc[3] = {0, 0, 0};
+      for (size_t q = 0; q < mid_n; ++q)
+        for (int c = 0; c < 3; ++c) acc[c] += (double)hout[q * 3 + c];
+      float mid[3], dmin[3];
+      const double pk = std::max(std::max(std::max(hs.dmin_raw[0], hs.dmin_raw[1]), hs.dmin_raw[2]), 1e-12);
+      for (int c = 0; c < 3; ++c) {
+        mid[c] = (float)(acc[c] / (double)mid_n);
+        dmin[c] = (float)(hs.dmin_raw[c] / pk);
+      }
+      // invert constants (invert.rs:41-77)
+      const float eps = 1e-6f;
+      for (int c = 0; c < 3; ++c) {
+        const float dm = std::fmax(dmin[c], eps);
+        float mt = mid[c] / dm;
+        mt = mt < eps ? eps : (mt > 1.0f - eps ? 1.0f - eps : mt);
+        const float inv_mid = std::fmax(1.0f / mt - 1.0f, 0.0f);
+        const float sh = std::fmax(inv_mid * 8.0f, eps);
+        const float iv = std::fmax(inv_mid, 0.0f);
+        const float soft = std::fmax(iv / (1.0f + iv / std::fmax(sh, 1e-6f)), eps);
+        P->T.inv_dmin[c] = dm;
+        P->T.inv_shoulder[c] = sh;
+        P->T.inv_gain[c] = 0.185f / soft;
+      }
+      P->T.invert = 1;
+      IMOPS_HIP_CHECK(hipMemcpyAsync(P->tables.ptr, &P->T, sizeof(P->T), hipMemcpyHostToDevice, s));
+      IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+    }
+  }
   // the scan writes straight into the image: every pixel's input was consumed by expose
-  run_film(s, hs, T, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, shutter, p.seed, B, ctx->scr[SCR_FILM_3]);
+  run_film(s, hs, *P, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, p.seed, B);
   img->color_space = IMOPS_CS_ACESCG;
 }
 

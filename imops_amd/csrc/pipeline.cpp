@@ -11,6 +11,8 @@
 #include <cstdlib>
 #include <map>
 #include <memory>
+#include <algorithm>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -423,6 +425,158 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
   }
 }
 
+// ------------------------------------------------------------------ bursts
+// A burst of frames of uneven sizes and CFA patterns through one stage list.
+// Frames are grouped by everything the stage list reads besides pixel data
+// (size, CFA and crop, white balance, calibration matrix, shutter, colour
+// space).  The first frame of every group runs eagerly, largest group first, so
+// every scratch buffer and per-shape table reaches its final size and content.
+// Each group with more frames is then captured once into a HIP graph on a
+// staging image of its shape and replayed per frame: the frame's input planes
+// are copied into the staging image, the graph runs, the output is copied back.
+namespace {
+
+struct FrameState {
+  bool has_raw, has_rgb;
+  int color_space;
+};
+
+std::string shape_key(const imops_image* f) {
+  std::string k = std::to_string(f->width) + "x" + std::to_string(f->height) + "|" + (f->has_raw ? "r" : "-") +
+                  (f->has_rgb ? "c" : "-") + "|" + std::to_string(f->color_space) + "|";
+  if (f->has_cfa) k += f->cfa.name + "@" + std::to_string(f->crop_x) + "," + std::to_string(f->crop_y);
+  auto add_floats = [&](const float* v, size_t m) {
+    k += "|";
+    for (size_t i = 0; i < m; ++i) {
+      uint32_t u;
+      std::memcpy(&u, &v[i], sizeof u);
+      k += std::to_string(u) + ",";
+    }
+  };
+  if (f->has_wb) add_floats(f->wb, 4);
+  add_floats(f->calib.data(), f->calib.size());
+  if (f->has_shutter) add_floats(&f->shutter, 1);
+  return k;
+}
+
+struct BurstShape {
+  std::vector<size_t> frames;
+  imops_image* staging = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* const* frames, size_t n) {
+  std::vector<FrameState> st(n);
+  std::vector<BurstShape> shapes;
+  std::vector<std::string> keys;
+  std::vector<size_t> shape_of(n);
+  for (size_t i = 0; i < n; ++i) {
+    st[i] = FrameState{frames[i]->has_raw, frames[i]->has_rgb, frames[i]->color_space};
+    const std::string k = shape_key(frames[i]);
+    size_t j = 0;
+    while (j < keys.size() && keys[j] != k) ++j;
+    if (j == keys.size()) {
+      keys.push_back(k);
+      shapes.emplace_back();
+    }
+    shapes[j].frames.push_back(i);
+    shape_of[i] = j;
+  }
+  std::vector<size_t> order(shapes.size());
+  for (size_t j = 0; j < order.size(); ++j) order[j] = j;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    const imops_image* fa = frames[shapes[a].frames[0]];
+    const imops_image* fb = frames[shapes[b].frames[0]];
+    return fa->width * fa->height > fb->width * fb->height;
+  });
+  for (size_t j : order) run_stages(ctx, stages, frames[shapes[j].frames[0]]);
+
+  const bool timing = ctx->timer.enabled;
+  ctx->timer.enabled = false;
+  auto cleanup = [&] {
+    ctx->timer.enabled = timing;
+    ctx->capturing = false;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (BurstShape& sh : shapes) {
+      if (sh.exec) (void)hipGraphExecDestroy(sh.exec);
+      if (sh.graph) (void)hipGraphDestroy(sh.graph);
+      if (sh.staging) {
+        sh.staging->raw.release();
+        sh.staging->rgb.release();
+        sh.staging->tmp.release();
+        delete sh.staging;
+      }
+    }
+  };
+  try {
+    for (size_t j : order) {
+      BurstShape& sh = shapes[j];
+      if (sh.frames.size() < 2) continue;
+      const imops_image* rep = frames[sh.frames[0]];
+      const FrameState& rs = st[sh.frames[0]];
+      imops_image* S = new imops_image();
+      sh.staging = S;
+      S->ctx = ctx;
+      S->width = rep->width;
+      S->height = rep->height;
+      S->has_raw = rs.has_raw;
+      S->has_rgb = rs.has_rgb;
+      S->color_space = rs.color_space;
+      S->cfa = rep->cfa;
+      S->has_cfa = rep->has_cfa;
+      S->crop_x = rep->crop_x;
+      S->crop_y = rep->crop_y;
+      std::memcpy(S->wb, rep->wb, sizeof S->wb);
+      S->has_wb = rep->has_wb;
+      S->calib = rep->calib;
+      S->shutter = rep->shutter;
+      S->has_shutter = rep->has_shutter;
+      const size_t px = S->width * S->height;
+      if (S->has_raw) S->raw.reserve(px * sizeof(float));
+      S->rgb.reserve(px * 3 * sizeof(float));
+      void* before[SCR_COUNT];
+      for (int b = 0; b < SCR_COUNT; ++b) before[b] = ctx->scr[b].ptr;
+      ctx->capturing = true;
+      IMOPS_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeGlobal));
+      try {
+        run_stages(ctx, stages, S);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      IMOPS_HIP_CHECK(hipStreamEndCapture(ctx->stream, &sh.graph));
+      ctx->capturing = false;
+      for (int b = 0; b < SCR_COUNT; ++b)
+        if (ctx->scr[b].ptr != before[b]) throw Error{IMOPS_E_STATE, "burst: scratch moved during graph capture"};
+      IMOPS_HIP_CHECK(hipGraphInstantiate(&sh.exec, sh.graph, nullptr, nullptr, 0));
+    }
+    for (size_t i = 0; i < n; ++i) {
+      BurstShape& sh = shapes[shape_of[i]];
+      if (i == sh.frames[0]) continue;
+      imops_image* F = frames[i];
+      imops_image* S = sh.staging;
+      const size_t px = F->width * F->height;
+      if (st[i].has_raw) IMOPS_HIP_CHECK(hipMemcpyAsync(S->raw.ptr, F->raw.ptr, px * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      if (st[i].has_rgb)
+        IMOPS_HIP_CHECK(hipMemcpyAsync(S->rgb.ptr, F->rgb.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      IMOPS_HIP_CHECK(hipGraphLaunch(sh.exec, ctx->stream));
+      F->rgb.reserve(px * 3 * sizeof(float));
+      IMOPS_HIP_CHECK(hipMemcpyAsync(F->rgb.ptr, S->rgb.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      F->has_rgb = S->has_rgb;
+      F->color_space = S->color_space;
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
+}  // namespace
+
 }  // namespace imops
 
 using namespace imops;
@@ -466,6 +620,20 @@ int imops_pipeline_run(imops_ctx* ctx, imops_pipeline* p, imops_image* img) {
   return guarded([&] {
     if (!ctx || !p || !img) throw Error{IMOPS_E_INVALID, "pipeline_run: null argument"};
     run_stages(ctx, p->stages, img);
+  });
+}
+
+int imops_pipeline_run_burst(imops_ctx* ctx, imops_pipeline* p, imops_image* const* frames, size_t n, int use_graphs) {
+  return guarded([&] {
+    if (!ctx || !p || (!frames && n)) throw Error{IMOPS_E_INVALID, "pipeline_run_burst: null argument"};
+    for (size_t i = 0; i < n; ++i)
+      if (!frames[i] || frames[i]->ctx != ctx)
+        throw Error{IMOPS_E_INVALID, "pipeline_run_burst: frame " + std::to_string(i) + " is null or of another context"};
+    if (!use_graphs) {
+      for (size_t i = 0; i < n; ++i) run_stages(ctx, p->stages, frames[i]);
+      return;
+    }
+    run_burst_graphs(ctx, p->stages, frames, n);
   });
 }
 

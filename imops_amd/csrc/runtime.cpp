@@ -371,6 +371,8 @@ int imops_ctx_destroy(imops_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     ctx->timer.clear();
     for (auto& b : ctx->scr) b.release();
+    ctx->bm3d_frags.release();
+    ctx->film_cache.reset();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });

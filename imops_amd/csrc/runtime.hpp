@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,11 @@ void clear_error();
     if (e__ != hipSuccess)                                                                     \
       throw ::imops::Error{IMOPS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e__)};   \
   } while (0)
+
+// State a stage keeps in its context across calls (prepared tables etc.).
+struct StageCache {
+  virtual ~StageCache() = default;
+};
 
 // Device buffer that only grows.
 struct DevBuf {
@@ -68,6 +74,12 @@ struct imops_ctx {
   int num_cus = 0;
   imops::DevBuf scr[SCR_COUNT];
   imops::KernelTimer timer;
+  std::unique_ptr<imops::StageCache> film_cache;  // prepared film runs by shape
+  imops::DevBuf bm3d_frags;                       // FP16 DCT fragments of the mixed BM3D filter
+  bool bm3d_tables_ready = false;                 // constant tables of the FP32 BM3D filter uploaded
+  // Set while the stage code runs under stream capture: stages then only
+  // enqueue work (no host uploads, synchronisation or allocation).
+  bool capturing = false;
 };
 
 namespace imops {

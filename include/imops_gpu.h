@@ -213,6 +213,16 @@ size_t imops_pipeline_num_stages(const imops_pipeline* p);
 const char* imops_pipeline_stage_name(const imops_pipeline* p, size_t i);
 /* Run the stages in order on the image (run_pixel_pipeline, pipeline.rs:4-13). */
 int imops_pipeline_run(imops_ctx* ctx, imops_pipeline* p, imops_image* img);
+/* Run the stages on each of n frames (any mix of sizes and CFA patterns the stage
+ * list accepts); no counterpart in the reference, which processes one image per
+ * call.  use_graphs = 0: imops_pipeline_run per frame.  use_graphs != 0: frames
+ * are grouped by shape (size, CFA and crop, white balance, calibration, shutter,
+ * colour space); the first frame of each group runs directly, largest group
+ * first, then each group with more frames is captured once into a HIP graph on
+ * a staging image of its shape and replayed per frame (input copied in, output
+ * copied out).  Results are identical either way.  Timing regions are not
+ * recorded for replayed frames. */
+int imops_pipeline_run_burst(imops_ctx* ctx, imops_pipeline* p, imops_image* const* frames, size_t n, int use_graphs);
 
 /* Synthetic inputs --------------------------------------------------------- */
 
