@@ -9,7 +9,8 @@ scene generator, resident in HBM as u16.  A step is one pass of
 
 from the u16 frame to the f32 interleaved sRGB output in HBM.  `value` is the
 plain FP32 form (the reference's arithmetic); the same stage list with
-BM3D{precision = "mixed"} (FP16 matrix-core block matching) is timed beside it
+BM3D{precision = "mixed"} (FP16 matrix-core block matching and DCT/Hadamard
+transforms) is timed beside it
 and reported under "mixed" with its PSNR against the FP32 form (full frame)
 and against the FP64 oracle (on the CPU-baseline crop).  Prints one JSON line.
 
@@ -181,12 +182,103 @@ def film_adjoint_bench(ctx, n=8, size=512, steps=200) -> dict:
             "gradient_check": "tests/test_gpu_film_fit.py: vector relative error vs FP64 central differences"}
 
 
+XTRANS = "GGRGGB" "GGBGGR" "BRGRBG" "GGBGGR" "GGRGGB" "RBGBRG"
+XT_W, XT_H = 6240, 4160
+BURST = [(4000, 3000, "RGGB", 12), (6016, 4016, "RGGB", 12), (8256, 5504, "RGGB", 4), (XT_W, XT_H, XTRANS, 4)]
+
+
+def stages_demosaic(precision: str, demosaic: str) -> str:
+    return stages(precision).replace('algorithm = "amaze"', f'algorithm = "{demosaic}"')
+
+
+def xtrans_bench(ctx, cal, steps: int = 3, warmup: int = 1) -> dict:
+    """6240x4160 X-Trans raw through Demosaic{markesteijn} and the same stage list,
+    plain FP32 and mixed forms; MP/s of each and the mixed form's PSNR against FP32."""
+    import torch
+    import imops_amd as im
+    print(f"[bench] generating {XT_W}x{XT_H} X-Trans raw", file=sys.stderr, flush=True)
+    raw = im.synth_raw_u16(XT_W, XT_H, XTRANS, 3)
+    d_raw = torch.from_numpy(raw.view(np.int16)).to("cuda")
+    torch.cuda.synchronize()
+    img = im.Image(ctx).set_calibration_matrix(cal.ravel())
+    res, outs = {}, {}
+    for precision in ("fp32", "mixed"):
+        cfg = im.parse_config(stages_demosaic(precision, "markesteijn"))
+        for _ in range(warmup):
+            img.set_raw_u16(d_raw, XTRANS, BLACK, WHITE)
+            im.run_pixel_pipeline(img, cfg)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            img.set_raw_u16(d_raw, XTRANS, BLACK, WHITE)
+            im.run_pixel_pipeline(img, cfg)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        res[precision] = {"value": round(XT_W * XT_H / 1e6 / (ms * 1e-3), 3), "ms_per_step": round(ms, 3)}
+        outs[precision] = img.rgb()
+        print(f"[bench] X-Trans {precision}: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+    p = psnr(outs["mixed"], outs["fp32"])
+    return {"workload": f"{XT_W}x{XT_H} X-Trans raw u16 -> Demosaic{{markesteijn}} -> " + " -> ".join(STAGE_NAMES[1:]),
+            "unit": "MP/s", "steps": steps, "fp32": res["fp32"], "mixed": res["mixed"],
+            "speedup": round(res["fp32"]["ms_per_step"] / res["mixed"]["ms_per_step"], 3),
+            "psnr_db_mixed_vs_fp32": None if math.isinf(p) else round(p, 2)}
+
+
+def burst_bench(cal) -> dict:
+    """32 raws of uneven sizes (12x 4000x3000, 12x 6016x4016, 4x 8256x5504 RGGB and
+    4x 6240x4160 X-Trans, read noise 1-8 DN per frame) through the mixed form as two
+    bursts (Bayer: Demosaic{amaze}; X-Trans: Demosaic{markesteijn}) with per-shape HIP
+    graphs and the launch-shape autotuner; wall-clock MP/s of the whole burst."""
+    import torch
+    import imops_amd as im
+    ctx = im.Context(0)
+    ctx.set_autotune(True)
+    rng = np.random.default_rng(17)
+    frames = []
+    print("[bench] generating the 32-frame burst", file=sys.stderr, flush=True)
+    for w, h, cfa, count in BURST:
+        for k in range(count):
+            raw = im.synth_raw_u16(w, h, cfa, 100 + len(frames), read_sigma=float(rng.uniform(1.0, 8.0)))
+            frames.append((cfa, torch.from_numpy(raw.view(np.int16)).to("cuda")))
+    torch.cuda.synchronize()
+    total_mp = sum(w * h * c for w, h, _, c in BURST) / 1e6
+    imgs = [im.Image(ctx).set_calibration_matrix(cal.ravel()) for _ in frames]
+    cfgs = {"RGGB": im.parse_config(stages_demosaic("mixed", "amaze")),
+            XTRANS: im.parse_config(stages_demosaic("mixed", "markesteijn"))}
+
+    def run(graphs: bool) -> float:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for (cfa, d), img in zip(frames, imgs):
+            img.set_raw_u16(d, cfa, BLACK, WHITE)
+        for cfa, cfg in cfgs.items():
+            im.run_burst([img for (c, _), img in zip(frames, imgs) if c == cfa], cfg, graphs=graphs)
+        ctx.synchronize()
+        return time.perf_counter() - t0
+
+    run(True)  # warm-up: scratch, per-shape tables, autotuner trials
+    t_graph = run(True)
+    t_eager = run(False)
+    print(f"[bench] burst: {total_mp:.1f} MP, graphs {t_graph:.3f} s, per-frame {t_eager:.3f} s",
+          file=sys.stderr, flush=True)
+    out = {"workload": "32 raws: " + ", ".join(f"{c}x {w}x{h} {'X-Trans' if cfa == XTRANS else cfa}"
+                                               for w, h, cfa, c in BURST) + "; mixed form, read noise U[1,8] DN",
+           "unit": "MP/s", "megapixels": round(total_mp, 2),
+           "value": round(total_mp / t_graph, 3), "seconds": round(t_graph, 3),
+           "value_without_graphs": round(total_mp / t_eager, 3),
+           "autotuned": ctx.tuning()}
+    del imgs
+    ctx.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the X-Trans and burst workloads")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain f32 copy kernel of this size first (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -199,6 +291,7 @@ def main() -> int:
     import imops_amd as im
 
     ctx = im.Context(0)
+    ctx.set_autotune(True)  # launch shapes timed during the warm-up steps; results do not depend on them
     if args.copy_probe_mib:
         n = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -287,7 +380,8 @@ def main() -> int:
                      "kernel_ms": round(match_ms, 3), "algorithmic_flops": flops},
         "mixed": {"value": round(W * H / 1e6 / (ms_mx * 1e-3), 3), "unit": "MP/s", "ms_per_step": round(ms_mx, 3),
                   "speedup_vs_fp32": round(ms / ms_mx, 3),
-                  "form": "BM3D block matching on FP16 MFMA (FP32 accumulate) with FP32 re-rank; other stages FP32",
+                  "form": "BM3D block matching (FP32 re-rank) and 8x8 DCT / Walsh-Hadamard group transforms on FP16 MFMA "
+                          "with FP32 accumulation; other stages FP32",
                   "psnr_db_vs_fp32_form_full_frame": None if math.isinf(psnr_full) else round(psnr_full, 2),
                   "bit_identical_to_fp32_form": math.isinf(psnr_full),
                   "bm3d_patches_fallback_per_step": extra_mx["bm3d_fallback_refs"],
@@ -298,8 +392,12 @@ def main() -> int:
                                "frac": round(achieved_mx / FP16_PEAK_TFLOPS, 5), "kernel_ms": round(match_ms_mx, 3),
                                "algorithmic_flops": flops}},
         "cpu_baseline": None,
+        "autotuned": ctx.tuning(),
     }
     out["film_adjoint"] = film_adjoint_bench(ctx)
+    if not args.no_extra:
+        out["xtrans"] = xtrans_bench(ctx, cal)
+        out["burst"] = burst_bench(cal)
     if not args.no_cpu_baseline:
         out["cpu_baseline"], crop_fp64 = cpu_baseline(raw, cal)
         crop = np.ascontiguousarray(raw[:CROP_H, :CROP_W])

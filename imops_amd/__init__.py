@@ -78,6 +78,23 @@ class Context:
         check(lib().imops_ctx_timing(self._h, region.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def set_autotune(self, enabled: bool) -> None:
+        """Time every launch-shape candidate of a tunable kernel on its first call at a
+        new shape and keep the fastest (results are identical for every candidate)."""
+        check(lib().imops_ctx_set_autotune(self._h, int(bool(enabled))))
+
+    def tuning(self) -> dict:
+        """Launch-shape choices made so far: {key: value}."""
+        need = C.c_size_t()
+        check(lib().imops_ctx_tuning(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value)
+        check(lib().imops_ctx_tuning(self._h, buf, need.value, None))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            k, _, v = line.partition("=")
+            out[k] = int(v)
+        return out
+
     def close(self) -> None:
         if self._h:
             lib().imops_ctx_destroy(self._h)

@@ -87,6 +87,8 @@ SIGNATURES = {
     "imops_pipeline_stage_name": (C.c_char_p, [_vp, _sz]),
     "imops_pipeline_run": (_i, [_vp, _vp, _vp]),
     "imops_pipeline_run_burst": (_i, [_vp, _vp, C.POINTER(_vp), _sz, _i]),
+    "imops_ctx_set_autotune": (_i, [_vp, _i]),
+    "imops_ctx_tuning": (_i, [_vp, C.c_char_p, _sz, C.POINTER(_sz)]),
     "imops_synth_scene_rgb": (_i, [_fp, _sz, _sz, C.c_uint64]),
     "imops_synth_raw_u16": (_i, [_vp, _sz, _sz, C.c_char_p, C.c_uint64, _f, _f, _f, _f]),
 }

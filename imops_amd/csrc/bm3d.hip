@@ -220,7 +220,7 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
     // every candidate's first row already exceeds a negative threshold (or thr is NaN)
     for (int c = tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
   } else {
-    switch (ssd_block_width(nx, ny)) {
+    switch (a.block_width ? a.block_width : ssd_block_width(nx, ny)) {
       case 7: ssd_candidates<7>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
       case 4: ssd_candidates<4>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
       default: ssd_candidates<2>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;

@@ -18,6 +18,7 @@ struct Bm3dMatchArgs {
   uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
   const uint32_t* ref_list = nullptr;   // optional: match only these patches
   const uint32_t* ref_count = nullptr;  // device count of ref_list entries
+  int block_width = 0;                  // candidates per thread (2, 4, 7); 0: chosen per window
 };
 
 struct Bm3dFilterArgs {

@@ -163,10 +163,14 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
     return mixed_filter ? bm3d_filter_mfma_smem(sp * sp) : bm3d_filter_smem(sp * sp);
   };
   const size_t lds_budget = mixed_filter ? 80 * 1024 : 40 * 1024;
-  int tile = 8;
-  while (tile > 1 && filter_lds(tile) > lds_budget) tile /= 2;
-  const size_t fsmem = filter_lds(tile);
-  const int tiles = ((refs_x + tile - 1) / tile) * ((refs_y + tile - 1) / tile);
+  int default_tile = 8;
+  while (default_tile > 1 && filter_lds(default_tile) > lds_budget) default_tile /= 2;
+  // candidates for the autotuner: every tile whose footprint fits one CU's LDS
+  std::vector<int> tile_cands;
+  for (int t = 2; t <= 8; t *= 2)
+    if (filter_lds(t) <= 160 * 1024) tile_cands.push_back(t);
+  const std::string shape_key = std::to_string(w) + "x" + std::to_string(h) + "/s" + std::to_string(p.stride) + "/w" +
+                                std::to_string(p.window) + "/g" + std::to_string(p.max_group);
 
   auto run_step = [&](const float* const guide[3], bool step1, float* const out[3]) {
     IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s));
@@ -214,6 +218,15 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
           fb.ref_count = overflow_count;
           IMOPS_HIP_CHECK(bm3d_match_list_launch(fb, (int)std::min<size_t>(nrefs, 512), s));
         } else {
+          // candidate block width per thread: every choice selects the same groups
+          m.block_width = tuned_choice(
+              ctx, "bm3d_match/" + shape_key + (step1 ? "/1" : "/2"), {0, 2, 4, 7}, 0,
+              [&](int q) {
+                Bm3dMatchArgs t = m;
+                t.block_width = q;
+                IMOPS_HIP_CHECK(bm3d_match_launch(t, (int)nrefs, s));
+              },
+              [] {});
           IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
         }
       }
@@ -228,7 +241,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       fa.refs_y = refs_y;
       fa.stride = p.stride;
       fa.window = p.window;
-      fa.tile = tile;
+      fa.tile = default_tile;
       fa.sigma = p.sigma;
       fa.chroma_scale = p.chroma_scale;
       fa.lambda = p.lambda;
@@ -240,9 +253,22 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
         fa.idct[i] = tables.idct[i];
         fa.kaiser[i] = tables.kaiser[i];
       }
+      // reference tile per workgroup: the aggregation is exact integer
+      // arithmetic, so every tile gives the same sums
+      auto launch_filter = [&](int t) {
+        Bm3dFilterArgs ft = fa;
+        ft.tile = t;
+        const int nblk = ((refs_x + t - 1) / t) * ((refs_y + t - 1) / t);
+        if (mixed_filter) IMOPS_HIP_CHECK(bm3d_filter_mfma_launch(ft, step1, nblk, filter_lds(t), filter_frags, s));
+        else IMOPS_HIP_CHECK(bm3d_filter_launch(ft, step1, nblk, filter_lds(t), s));
+      };
+      const size_t tuned_before = ctx->tuned.size();
+      const int tile = tuned_choice(
+          ctx, std::string(mixed_filter ? "bm3d_filter_mfma/" : "bm3d_filter/") + shape_key + (step1 ? "/ht" : "/wiener"),
+          tile_cands, default_tile, launch_filter, [&] { IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s)); });
+      if (ctx->tuned.size() != tuned_before) IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s));  // trials ran
       TimedRegion tr(ctx, step1 ? "bm3d_filter_ht" : "bm3d_filter_wiener");
-      if (mixed_filter) IMOPS_HIP_CHECK(bm3d_filter_mfma_launch(fa, step1, tiles, fsmem, filter_frags, s));
-      else IMOPS_HIP_CHECK(bm3d_filter_launch(fa, step1, tiles, fsmem, s));
+      launch_filter(tile);
     }
     const float* nz[3] = {noisy[0], noisy[1], noisy[2]};
     IMOPS_HIP_CHECK(bm3d_finalize_launch(num, den, nz, out, n, s));

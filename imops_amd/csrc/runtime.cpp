@@ -3,6 +3,7 @@
 // each entry point returns a status and records the message for
 // imops_last_error() (the reference's catch_panic + LAST_ERROR,
 // pichromatic-pipeline/src/extern_pipeline.rs:18-58).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -37,6 +38,45 @@ void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+}
+
+int tuned_choice(imops_ctx* ctx, const std::string& key, const std::vector<int>& candidates, int fallback,
+                 const std::function<void(int)>& run, const std::function<void()>& reset) {
+  const auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) return it->second;
+  if (!ctx->autotune || ctx->capturing || candidates.size() < 2) return fallback;
+  hipEvent_t e0, e1;
+  IMOPS_HIP_CHECK(hipEventCreate(&e0));
+  IMOPS_HIP_CHECK(hipEventCreate(&e1));
+  int best = fallback;
+  float best_ms = 0.0f;
+  try {
+    for (int c : candidates) {
+      float ms = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {
+        reset();
+        IMOPS_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+        run(c);
+        IMOPS_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+        IMOPS_HIP_CHECK(hipEventSynchronize(e1));
+        float t = 0.0f;
+        IMOPS_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+        ms = rep == 0 ? t : std::min(ms, t);
+      }
+      if (best_ms == 0.0f || ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->tuned[key] = best;
+  return best;
 }
 
 KernelTimer::Region& KernelTimer::region(const std::string& name) {
@@ -328,6 +368,27 @@ int imops_ctx_create(int device, imops_ctx** out) {
       throw Error{IMOPS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
     }
     *out = c;
+  });
+}
+
+int imops_ctx_set_autotune(imops_ctx* ctx, int enabled) {
+  return guarded([&] {
+    if (!ctx) throw Error{IMOPS_E_INVALID, "null context"};
+    ctx->autotune = enabled != 0;
+  });
+}
+
+int imops_ctx_tuning(imops_ctx* ctx, char* buf, size_t len, size_t* needed) {
+  return guarded([&] {
+    if (!ctx) throw Error{IMOPS_E_INVALID, "null context"};
+    std::string out;
+    for (const auto& kv : ctx->tuned) out += kv.first + "=" + std::to_string(kv.second) + "\n";
+    if (needed) *needed = out.size() + 1;
+    if (buf && len) {
+      const size_t m = std::min(len - 1, out.size());
+      std::memcpy(buf, out.data(), m);
+      buf[m] = '\0';
+    }
   });
 }
 

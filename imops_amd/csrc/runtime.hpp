@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -80,6 +82,10 @@ struct imops_ctx {
   // Set while the stage code runs under stream capture: stages then only
   // enqueue work (no host uploads, synchronisation or allocation).
   bool capturing = false;
+  // Launch-shape autotuning: when on, the first call of a tunable kernel at a
+  // new shape times every candidate and keeps the fastest (by key).
+  bool autotune = false;
+  std::map<std::string, int> tuned;
 };
 
 namespace imops {
@@ -117,6 +123,13 @@ struct imops_image {
 };
 
 namespace imops {
+
+// Candidate `fallback` unless the context autotunes: then, once per key, each
+// candidate is run (after `reset`) twice on the context stream and timed with
+// HIP events, and the fastest is remembered.  Every candidate must compute the
+// same result; the caller runs the chosen one afterwards.
+int tuned_choice(imops_ctx* ctx, const std::string& key, const std::vector<int>& candidates, int fallback,
+                 const std::function<void(int)>& run, const std::function<void()>& reset);
 
 // Stage implementations shared by the per-stage C calls and the pipeline runner.
 void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);

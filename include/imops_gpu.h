@@ -77,6 +77,15 @@ void* imops_ctx_stream(imops_ctx* ctx);
  * No counterpart in the reference (its CLI prints wall-clock pipeline time
  * only, src/bin/imops-cli.rs:59-63). */
 int imops_ctx_set_timing(imops_ctx* ctx, int enabled);
+/* Launch-shape autotuning (no counterpart in the reference).  When enabled, the
+ * first call of a tunable kernel at a new shape (BM3D filter: references per
+ * workgroup tile; FP32 BM3D matcher: candidates per thread) times every
+ * candidate on the call's own data and keeps the fastest for the context.
+ * Every candidate computes identical results.  imops_ctx_tuning writes the
+ * choices as "key=value" lines (NUL-terminated, truncated to len) and the size
+ * the whole report needs to *needed. */
+int imops_ctx_set_autotune(imops_ctx* ctx, int enabled);
+int imops_ctx_tuning(imops_ctx* ctx, char* buf, size_t len, size_t* needed);
 int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* count);
 
 /* Images: device-resident replacement of pichromatic::pixel::Image
