@@ -309,15 +309,20 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
   }
 }
 
-// Every reference patch (ref_list null: one workgroup per patch), or the
-// patches named in ref_list[0 .. *ref_count) with a grid-stride loop.
+// One workgroup per reference patch.  kTrial only separates autotuner trial
+// launches from real ones in kernel traces.
+template <bool kTrial>
 __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
   extern __shared__ unsigned char smem_raw[];
   MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
-  if (!a.ref_list) {
-    match_one_ref(a, S, blockIdx.x);
-    return;
-  }
+  match_one_ref(a, S, blockIdx.x);
+}
+
+// The patches named in ref_list[0 .. *ref_count), grid-stride (the patches the
+// matrix-core matcher hands back).
+__global__ void __launch_bounds__(MT) bm3d_match_list_kernel(Bm3dMatchArgs a) {
+  extern __shared__ unsigned char smem_raw[];
+  MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
   const uint32_t n = *a.ref_count;
   for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
     match_one_ref(a, S, (int)a.ref_list[it]);
@@ -428,7 +433,7 @@ __device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
 // LDS: the footprint's i64 numerators, then its u32 denominators.  A
 // workgroup's denominator sum at one pixel is at most 64 references x 16
 // patches x 1e5 < 2^32, so the 32-bit partial sums are exact.
-template <bool kStep1>
+template <bool kStep1, bool kTrial>
 __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
   extern __shared__ unsigned long long acc[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -641,19 +646,25 @@ hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, fl
   return hipGetLastError();
 }
 
-hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s) {
-  hipLaunchKernelGGL(bm3d_match_kernel, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
+hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial) {
+  if (trial) hipLaunchKernelGGL(bm3d_match_kernel<true>, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
+  else hipLaunchKernelGGL(bm3d_match_kernel<false>, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
   return hipGetLastError();
 }
 
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(bm3d_match_kernel, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
+  hipLaunchKernelGGL(bm3d_match_list_kernel, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
   return hipGetLastError();
 }
 
-hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s) {
-  if (step1) hipLaunchKernelGGL(bm3d_filter_kernel<true>, dim3(nblocks), dim3(FT), smem, s, a);
-  else hipLaunchKernelGGL(bm3d_filter_kernel<false>, dim3(nblocks), dim3(FT), smem, s, a);
+hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s, bool trial) {
+  if (trial) {
+    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, true>), dim3(nblocks), dim3(FT), smem, s, a);
+    else hipLaunchKernelGGL((bm3d_filter_kernel<false, true>), dim3(nblocks), dim3(FT), smem, s, a);
+  } else {
+    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, false>), dim3(nblocks), dim3(FT), smem, s, a);
+    else hipLaunchKernelGGL((bm3d_filter_kernel<false, false>), dim3(nblocks), dim3(FT), smem, s, a);
+  }
   return hipGetLastError();
 }
 

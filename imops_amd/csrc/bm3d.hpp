@@ -16,7 +16,7 @@ struct Bm3dMatchArgs {
   float threshold;        // per-pixel mean SSD limit of this step
   int max_candidates, max_group;
   uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
-  const uint32_t* ref_list = nullptr;   // optional: match only these patches
+  const uint32_t* ref_list = nullptr;   // list launch: the patches to match
   const uint32_t* ref_count = nullptr;  // device count of ref_list entries
   int block_width = 0;                  // candidates per thread (2, 4, 7); 0: chosen per window
 };
@@ -60,7 +60,7 @@ struct Bm3dBlurTaps {
 size_t bm3d_match_smem();
 size_t bm3d_filter_smem(int footprint_pixels);
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s);
-hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s);
+hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial = false);
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s);
 Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window);
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);
@@ -69,8 +69,10 @@ constexpr size_t kBm3dMfmaFragBytes = 2 * 4 * 2 * 64 * 16;
 void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out);
 size_t bm3d_filter_mfma_smem(int footprint_pixels);
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
-                                   hipStream_t s);
-hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s);
+                                   hipStream_t s, bool trial = false);
+// trial: an autotuner trial launch (a separate kernel instantiation, same code)
+hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s,
+                              bool trial = false);
 // copies the DCT / IDCT / Kaiser tables the plain filter reads (64 floats each)
 hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* kaiser, hipStream_t s);
 hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],

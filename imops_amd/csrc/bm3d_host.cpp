@@ -224,7 +224,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
               [&](int q) {
                 Bm3dMatchArgs t = m;
                 t.block_width = q;
-                IMOPS_HIP_CHECK(bm3d_match_launch(t, (int)nrefs, s));
+                IMOPS_HIP_CHECK(bm3d_match_launch(t, (int)nrefs, s, true));
               },
               [] {});
           IMOPS_HIP_CHECK(bm3d_match_launch(m, (int)nrefs, s));
@@ -255,20 +255,22 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       }
       // reference tile per workgroup: the aggregation is exact integer
       // arithmetic, so every tile gives the same sums
-      auto launch_filter = [&](int t) {
+      auto launch_filter = [&](int t, bool trial) {
         Bm3dFilterArgs ft = fa;
         ft.tile = t;
         const int nblk = ((refs_x + t - 1) / t) * ((refs_y + t - 1) / t);
-        if (mixed_filter) IMOPS_HIP_CHECK(bm3d_filter_mfma_launch(ft, step1, nblk, filter_lds(t), filter_frags, s));
-        else IMOPS_HIP_CHECK(bm3d_filter_launch(ft, step1, nblk, filter_lds(t), s));
+        if (mixed_filter)
+          IMOPS_HIP_CHECK(bm3d_filter_mfma_launch(ft, step1, nblk, filter_lds(t), filter_frags, s, trial));
+        else IMOPS_HIP_CHECK(bm3d_filter_launch(ft, step1, nblk, filter_lds(t), s, trial));
       };
       const size_t tuned_before = ctx->tuned.size();
       const int tile = tuned_choice(
           ctx, std::string(mixed_filter ? "bm3d_filter_mfma/" : "bm3d_filter/") + shape_key + (step1 ? "/ht" : "/wiener"),
-          tile_cands, default_tile, launch_filter, [&] { IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s)); });
+          tile_cands, default_tile, [&](int t) { launch_filter(t, true); },
+          [&] { IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s)); });
       if (ctx->tuned.size() != tuned_before) IMOPS_HIP_CHECK(fill_zero_u64(num, 6 * n, s));  // trials ran
       TimedRegion tr(ctx, step1 ? "bm3d_filter_ht" : "bm3d_filter_wiener");
-      launch_filter(tile);
+      launch_filter(tile, false);
     }
     const float* nz[3] = {noisy[0], noisy[1], noisy[2]};
     IMOPS_HIP_CHECK(bm3d_finalize_launch(num, den, nz, out, n, s));
