@@ -34,9 +34,12 @@ void sample_curve(const Curve& c, double* out) {
     } else if (c.kind == CURVE_GAUSS) {  // gaussian_curve (color_neg_200.rs:38-50)
       const double d = (l - c.b) / c.c;
       out[i] = c.a * std::exp(-0.5 * d * d);
-    } else if (c.kind == CURVE_ORANGE) {  // orange_mask_epsilon (color_neg_200.rs:52-66)
+    } else if (c.kind == CURVE_ORANGE) {  // orange_mask_epsilon (color_neg_200.rs:51-65, portra_400.rs:28-40)
       const double u = (l - 450.0) / 50.0, v = (l - 520.0) / 60.0;
-      out[i] = 0.35 * std::exp(-0.5 * (u * u)) + 0.25 * (c.a * std::exp(-0.5 * (v * v))) + 0.05;
+      out[i] = c.a * std::exp(-0.5 * (u * u)) + c.b * (c.c * std::exp(-0.5 * (v * v))) + c.d;
+    } else if (c.kind == CURVE_PANCHRO) {  // panchromatic_sensitivity (bw_stub.rs:19-32, trix_400.rs:16-28)
+      const double t = (l - 400.0) / 300.0;
+      out[i] = c.a + c.b * std::fmax(std::sin(3.14159265358979323846 * t), 0.0);
     } else {
       out[i] = 0.0;
     }

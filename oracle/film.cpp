@@ -56,11 +56,17 @@ void curve_samples(const Curve& c, double* out) {
         out[i] = c.a * std::exp(-0.5 * d * d);
         break;
       }
-      case CURVE_ORANGE: {
+      case CURVE_ORANGE: {  // orange_mask_epsilon (color_neg_200.rs:51-65 and the other colour negatives)
         const double u = (l - 450.0) / 50.0, v = (l - 520.0) / 60.0;
         const double blue = std::exp(-0.5 * (u * u));
-        const double green = c.a * std::exp(-0.5 * (v * v));
-        out[i] = 0.35 * blue + 0.25 * green + 0.05;
+        const double green = c.c * std::exp(-0.5 * (v * v));
+        out[i] = c.a * blue + c.b * green + c.d;
+        break;
+      }
+      case CURVE_PANCHRO: {  // panchromatic_sensitivity (bw_stub.rs:19-32, trix_400.rs:16-28)
+        const double t = (l - 400.0) / 300.0;
+        const double envelope = std::fmax(std::sin(M_PI * t), 0.0);
+        out[i] = c.a + c.b * envelope;
         break;
       }
       default: out[i] = 0.0;

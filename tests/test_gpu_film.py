@@ -41,6 +41,21 @@ def test_film_negative_colorneg200(gpu_ctx, w, h, seed):
     assert err <= TOL, f"max abs err {err}"
 
 
+STOCKS_ALL = ["BwStub", "ColorNeg200", "Portra400", "Ektar100", "FujiPro400H", "EktachromeE100", "TriX400"]
+
+
+@pytest.mark.parametrize("stock", STOCKS_ALL)
+@pytest.mark.parametrize("output", ["NegativeLinear", "PositiveLinear"])
+def test_film_every_stock(gpu_ctx, stock, output):
+    """Every stock of film/stock/*.rs (1 to 7 emulsions, masks, reversal, B&W) against the oracle."""
+    print(f"film {stock} {output}", flush=True)
+    rgb = _scene(96, 72, 11)
+    got = _gpu_film(gpu_ctx, rgb, stock=stock, output=output, seed=2)
+    want = oracle.film(rgb, stock=stock, output=output, seed=2)
+    err = np.abs(got - want).max()
+    assert err <= TOL, f"max abs err {err}"
+
+
 def test_film_positive_linear(gpu_ctx):
     rgb = _scene(160, 120, 2)
     got = _gpu_film(gpu_ctx, rgb, output="PositiveLinear", seed=5)

@@ -188,3 +188,71 @@ def test_film_deterministic_and_finite():
     x = _flat(8, 8, [0.185, 0.1, 0.05])
     a, b = oracle.film(x), oracle.film(x)
     assert np.array_equal(a, b) and np.isfinite(a).all()
+
+
+# ---------------------------------------------------------------- all stocks
+STOCKS_ALL = ["BwStub", "ColorNeg200", "Portra400", "Ektar100", "FujiPro400H", "EktachromeE100", "TriX400"]
+AP1_Y = np.array([0.2722287168, 0.6740817658, 0.0536895174])
+
+
+def test_all_stocks_run_and_stay_finite():
+    # film/mod.rs tests all_stocks_load_and_validate and reversal_ektachrome_positive_slide
+    for st in STOCKS_ALL:
+        for output in ("NegativeLinear", "PositiveLinear"):
+            out = oracle.film(_flat(16, 16, [0.185] * 3, iso=100.0), stock=st, output=output)
+            assert np.isfinite(out).all(), (st, output)
+
+
+def test_emulsion_counts_and_gammas():
+    # layer lists of stock/*.rs; gamma claims of ektar_100.rs test ektar_loads_and_is_steeper_than_portra
+    want_e = {"BwStub": 1, "ColorNeg200": 3, "Portra400": 6, "Ektar100": 6, "FujiPro400H": 7,
+              "EktachromeE100": 3, "TriX400": 2}
+    gam = {}
+    for st, e in want_e.items():
+        v = oracle.film_fit_nominal(st)
+        assert int(v[0]) == e, st
+        gam[st] = v[137:137 + e]
+    assert (gam["Ektar100"] < 0.36).any() and (gam["Ektar100"] > 0.55).any()
+    assert ((gam["Portra400"] >= 0.5) & (gam["Portra400"] < 0.6)).all()
+
+
+def test_bw_hd_curve_monotonic():
+    # film/mod.rs test hd_curve_monotonic_patches (BwStub, NegativeLinear, box ISO 100)
+    ys = []
+    for y in [0.02, 0.05, 0.1, 0.185, 0.3, 0.5, 0.8, 1.2]:
+        out = oracle.film(_flat(8, 8, [y] * 3, iso=100.0), stock="BwStub")
+        ys.append(float((out.reshape(-1, 3) @ AP1_Y).mean()))
+    assert all(b <= a + 1e-4 for a, b in zip(ys, ys[1:])), ys
+
+
+def test_bw_unexposed_and_midgray_gates():
+    # film/mod.rs tests unexposed_is_dmin and clipping_free_midgray (BwStub)
+    out = oracle.film(np.zeros((8, 8, 3), np.float32), stock="BwStub")
+    assert abs(float(out.max(axis=2).mean()) - 1.0) < 0.05
+    out = oracle.film(_flat(8, 8, [0.185] * 3, iso=100.0), stock="BwStub")
+    assert np.isfinite(out).all() and (out > 0).all() and (out < 1.5).all()
+
+
+@pytest.mark.parametrize("stock", ["BwStub", "TriX400"])
+def test_bw_scan_is_the_base_tint_at_every_exposure(stock):
+    # A spectrally flat silver density gives a flat transmittance at every pixel, so every
+    # scanned pixel is a scalar multiple of the film base's colour (the normalised Dmin).
+    rng = np.random.default_rng(3)
+    x = _flat(16, 16, [0.185] * 3, iso=400.0) * rng.uniform(0.05, 3.0, (16, 16, 1)).astype(np.float32)
+    out = oracle.film(x, stock=stock).reshape(-1, 3)
+    d = oracle.film_normalized_dmin(stock)
+    np.testing.assert_allclose(out / out[:, :1], np.broadcast_to(d / d[0], out.shape), rtol=1e-5)
+
+
+@pytest.mark.parametrize("stock", ["Portra400", "Ektar100", "FujiPro400H"])
+def test_colour_negative_base_is_orange(stock):
+    # the coloured couplers' mask absorbs blue most: film base R > G > B, peak normalised to 1
+    d = oracle.film_normalized_dmin(stock)
+    assert abs(float(d.max()) - 1.0) < 1e-5 and d[0] > d[1] > d[2]
+
+
+def test_reversal_is_a_positive():
+    # E-6 reversal (reduction.rs:36-41, f_eff = 1 - f): more exposure, more light through the slide
+    ys = [float((oracle.film(_flat(8, 8, [r] * 3, iso=100.0), stock="EktachromeE100").reshape(-1, 3) @ AP1_Y).mean())
+          for r in (0.05, 0.185, 0.5, 1.2)]
+    assert all(b > a for a, b in zip(ys, ys[1:])), ys
