@@ -307,6 +307,101 @@ int grid1d(size_t n) {
   return (int)(g > 65536 ? 65536 : (g ? g : 1));
 }
 
+// LDS-tiled separable passes (gaussian_blur_separable, blur.rs:10-37), with the
+// same k-ordered sums as blur_h_kernel / blur_v_kernel.  Rows: one workgroup
+// per segment of kRowSeg outputs; the segment and its reflected halo are staged
+// with coalesced loads.  Columns: one workgroup per 64-column x kColH-row tile,
+// one lane per column, each tile row a 256-byte load.
+constexpr int BT = 256;
+constexpr int kRowSeg = 2048;
+constexpr int kColH = 128;
+
+size_t blur_row_smem(int radius) { return (size_t)(2 * radius + 1 + kRowSeg + 2 * radius) * sizeof(float); }
+size_t blur_col_smem(int radius) { return (size_t)(2 * radius + 1 + (kColH + 2 * radius) * 64) * sizeof(float); }
+
+__global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
+                                                      int h, const float* __restrict__ taps, int radius) {
+  extern __shared__ float sh[];
+  const int tid = threadIdx.x, nt = 2 * radius + 1;
+  const int segs = (w + kRowSeg - 1) / kRowSeg;
+  const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * kRowSeg;
+  const int nx = min(kRowSeg, w - x0);
+  float* tp = sh;
+  float* buf = sh + nt;
+  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  const float* row = in + (size_t)y * w;
+  for (int j = tid; j < nx + 2 * radius; j += BT) buf[j] = row[reflect_index((long long)x0 - radius + j, w)];
+  __syncthreads();
+  float* orow = out + (size_t)y * w + x0;
+  for (int t = tid; t < nx; t += BT) {
+    float acc = 0.0f;
+    for (int k = 0; k < nt; ++k) acc += buf[t + k] * tp[k];
+    orow[t] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(BT) blur_col_kernel(const float* __restrict__ tmp, const float* orig, float* out,
+                                                      int w, int h, const float* __restrict__ taps, int radius,
+                                                      int mode, float a) {
+  extern __shared__ float sh[];
+  const int tid = threadIdx.x, nt = 2 * radius + 1;
+  const int tiles_x = (w + 63) / 64;
+  const int xs = (blockIdx.x % tiles_x) * 64, ys = (blockIdx.x / tiles_x) * kColH;
+  const int nyo = min(kColH, h - ys);
+  const int c = tid & 63, g = tid >> 6, x = xs + c;
+  float* tp = sh;
+  float* buf = sh + nt;
+  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  const int rows = nyo + 2 * radius;
+  if (x < w)
+    for (int j = g; j < rows; j += BT / 64) buf[j * 64 + c] = tmp[(size_t)reflect_index((long long)ys - radius + j, h) * w + x];
+  __syncthreads();
+  if (x >= w) return;
+  for (int t = g; t < nyo; t += BT / 64) {
+    float acc = 0.0f;
+    for (int k = 0; k < nt; ++k) acc += buf[(t + k) * 64 + c] * tp[k];
+    const size_t i = (size_t)(ys + t) * w + x;
+    float v;
+    switch (mode) {
+      case BLUR_MIX: {  // (1-f) p + f blur(p)      halation.rs:164-181
+        const float keep = 1.0f - a;
+        v = keep * orig[i] + a * acc;
+        break;
+      }
+      case BLUR_ADJ: {  // d + beta (d - blur(d))   diffusion.rs:23-26
+        const float d = orig[i];
+        v = d + a * (d - acc);
+        break;
+      }
+      default: v = acc;
+    }
+    out[i] = v;
+  }
+}
+
+constexpr size_t kBlurSmemMax = 120 * 1024;
+
+// Horizontal then vertical pass (tiled kernels while their LDS fits, else the
+// one-pixel-per-thread ones).
+void blur_passes(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps_h,
+                 const float* taps_v, int radius, int mode, float a, hipStream_t s) {
+  const size_t n = (size_t)w * h;
+  if (blur_row_smem(radius) <= kBlurSmemMax) {
+    const int segs = (w + kRowSeg - 1) / kRowSeg;
+    hipLaunchKernelGGL(blur_row_kernel, dim3(h * segs), dim3(BT), blur_row_smem(radius), s, in, tmp, w, h, taps_h,
+                       radius);
+  } else {
+    hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps_h, radius);
+  }
+  if (blur_col_smem(radius) <= kBlurSmemMax) {
+    const int tiles = ((w + 63) / 64) * ((h + kColH - 1) / kColH);
+    hipLaunchKernelGGL(blur_col_kernel, dim3(tiles), dim3(BT), blur_col_smem(radius), s, tmp, orig, out, w, h, taps_v,
+                       radius, mode, a);
+  } else {
+    hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, orig, out, w, h, taps_v, radius, mode, a);
+  }
+}
+
 }  // namespace
 
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s) {
@@ -316,18 +411,13 @@ hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, fl
 
 hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps,
                      int radius, int mode, float a, hipStream_t s) {
-  const size_t n = (size_t)w * h;
-  hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps, radius);
-  hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, orig, out, w, h, taps, radius, mode, a);
+  blur_passes(in, tmp, orig, out, w, h, taps, taps, radius, mode, a, s);
   return hipGetLastError();
 }
 
 hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
                       int radius, hipStream_t s) {
-  const size_t n = (size_t)w * h;
-  hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps_h, radius);
-  hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, nullptr, out, w, h, taps_v, radius,
-                     (int)BLUR_STORE, 0.0f);
+  blur_passes(in, tmp, nullptr, out, w, h, taps_h, taps_v, radius, (int)BLUR_STORE, 0.0f, s);
   return hipGetLastError();
 }
 
@@ -346,9 +436,7 @@ hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64
                       double* partial, float d_max, float kappa, hipStream_t s) {
   const size_t n = (size_t)w * h;
   hipLaunchKernelGGL(grain_noise_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, w, h, key0);
-  hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, tmp, w, h, taps, radius);
-  hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, noise, noise, w, h, taps, radius,
-                     (int)BLUR_STORE, 0.0f);
+  blur_passes(noise, tmp, noise, noise, w, h, taps, taps, radius, (int)BLUR_STORE, 0.0f, s);
   const int parts = kFilmReduceBlocks;
   hipLaunchKernelGGL(sumsq_partial_kernel, dim3(parts), dim3(256), 0, s, noise, n, partial);
   float* norm = reinterpret_cast<float*>(partial + parts);
