@@ -66,6 +66,21 @@ struct MatchShared {
 constexpr int kMatchChunks = (kMaxWin * kMaxWin + 63) / 64;
 constexpr int kChunksPerWave = (kMatchChunks + MT / 64 - 1) / (MT / 64);
 
+// Wave-wide minimum of a u32: butterflies inside each 16-lane DPP row (xor 1
+// and 2 as quad permutes, xor 4 as a half-row mirror then a quad reversal,
+// xor 8 as a row rotation), then the four row results through scalar registers.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true));
+  int t = __builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);
+  t = __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, true);
+  v = min(v, (uint32_t)t);
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true));
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+  return min(min(r0, r1), min(r2, r3));
+}
+
 // compute_ssd_flat (bm3d.rs:367-391) for Q horizontally adjacent candidates of
 // one channel: the running sum over rows in the reference's order, returning
 // the sum at the first row end where it exceeds stop, else the full sum.  Rows
@@ -255,9 +270,13 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
   // Selection: the reference patch first, then the need smallest keys
   // (dist bits, scan index) among the capped accepted candidates; dist >= 0 so
   // its bit pattern orders like the value.  Each wave finds its own smallest
-  // need keys (wave w holds chunks w, w + 4, ...; lane l candidate 64k + l),
-  // then one wave ranks the merged lists.
-  unsigned long long key[kChunksPerWave];
+  // need keys (wave w holds chunks w, w + 4, ...; lane l candidate 64k + l) by
+  // repeated wave minima over the lanes' current heads, then one wave ranks
+  // the merged lists.
+  // Keys of one lane: the distance bits of its candidates in chunk order (the
+  // scan index grows with the chunk), +inf bits when not eligible.
+  constexpr uint32_t kInfBits = 0x7f800000u;
+  uint32_t kb[kChunksPerWave];
 #pragma unroll
   for (int i = 0; i < kChunksPerWave; ++i) {
     const int k = wave + i * (MT / 64);
@@ -266,30 +285,44 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
     const unsigned long long m = __ballot(accepted);
     const int excl = __shfl(incl - cnt_l, k < nchunks ? k : 0, 64);
     const int rank = excl + __popcll(m & ((1ull << lane) - 1ull));
-    key[i] = (accepted && rank < cap) ? (((unsigned long long)__float_as_uint(S.dist[c]) << 32) | (unsigned)c) : ~0ull;
+    kb[i] = (accepted && rank < cap) ? __float_as_uint(S.dist[c]) : kInfBits;
   }
-  unsigned long long lmin = key[0];
+  // lane head: its first smallest key
+  uint32_t hb = kb[0];
+  int hi = 0;
 #pragma unroll
-  for (int i = 1; i < kChunksPerWave; ++i) lmin = key[i] < lmin ? key[i] : lmin;
+  for (int i = 1; i < kChunksPerWave; ++i)
+    if (kb[i] < hb) {
+      hb = kb[i];
+      hi = i;
+    }
   for (int r = 0; r < 16; ++r) {
-    unsigned long long wmin = lmin;
-    if (r < need) {
-      for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(wmin, off, 64);
-        wmin = o < wmin ? o : wmin;
-      }
-    } else {
-      wmin = ~0ull;
-    }
-    if (lane == 0) S.top[wave][r] = wmin;
-    if (r < need && wmin != ~0ull) {
-      lmin = ~0ull;
+    unsigned long long key = ~0ull;
+    if (r < need) {  // workgroup-uniform
+      const uint32_t wb = wave_min_u32(hb);
+      if (wb != kInfBits) {
+        const uint32_t hc = (uint32_t)(((wave + hi * (MT / 64)) << 6) + lane);
+        const unsigned long long m = __ballot(hb == wb);
+        // equal distances: the smallest scan index wins (one lane holds it)
+        const uint32_t wc = __popcll(m) == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)hc, __ffsll((long long)m) - 1)
+                                             : wave_min_u32(hb == wb ? hc : 0xffffffffu);
+        key = ((unsigned long long)wb << 32) | wc;
+        if (hb == wb && hc == wc) {  // the winning lane drops the key and finds its next head
 #pragma unroll
-      for (int i = 0; i < kChunksPerWave; ++i) {
-        if (key[i] == wmin) key[i] = ~0ull;
-        lmin = key[i] < lmin ? key[i] : lmin;
+          for (int i = 0; i < kChunksPerWave; ++i)
+            if (i == hi) kb[i] = kInfBits;
+          hb = kb[0];
+          hi = 0;
+#pragma unroll
+          for (int i = 1; i < kChunksPerWave; ++i)
+            if (kb[i] < hb) {
+              hb = kb[i];
+              hi = i;
+            }
+        }
       }
     }
+    if (lane == 0) S.top[wave][r] = key;
   }
   __syncthreads();
   uint32_t* out = a.groups + (size_t)ref_id * 17;
