@@ -55,9 +55,10 @@ __global__ void rgb_to_ycbcr_kernel(const float* __restrict__ rgb, size_t n, flo
 constexpr int MT = 256;               // matching threads
 constexpr int kMaxWin = kBm3dMaxWindow; // largest supported search window
 constexpr int kMaxSpan = kMaxWin + 7;  // staged rows/cols
+constexpr int kWinCap = kMaxSpan * (kMaxSpan + 4);  // staged floats per channel (one window, or a pair's union)
 
 struct MatchShared {
-  float win[3][kMaxSpan * kMaxSpan];
+  float win[3][kWinCap];
   float ref[3][64];
   float dist[kMaxWin * kMaxWin];      // +inf when rejected
   int chunk_count[(kMaxWin * kMaxWin + 63) / 64];  // accepted candidates per 64 in scan order
@@ -151,7 +152,8 @@ __device__ __forceinline__ int ssd_block_width(int nx, int ny) {
 // far, a candidate whose partial sum exceeds thr is dropped, the total must be
 // below thr.  The reference position itself is excluded (prepended later).
 template <int Q>
-__device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, int span_x, int ref_cx, int ref_cy, float thr) {
+__device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, int span_x, int col0, int ref_cx,
+                                               int ref_cy, float thr) {
   const int tid = threadIdx.x;
   const int nbx = (nx + Q - 1) / Q;
   const int nblk = nbx * ny;
@@ -176,7 +178,7 @@ __device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, i
       stop1[q] = thr1;
       d0[q] = d1[q] = d2[q] = 0.0f;
     }
-    const int off = cy * span_x + cx0;
+    const int off = cy * span_x + col0 + cx0;
     if (__any(any)) ssd_channel<Q>(S.win[0] + off, S.ref[0], span_x, stop1, d0);
     any = false;
 #pragma unroll
@@ -209,22 +211,47 @@ __device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, i
   }
 }
 
-__device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rx = (ref_id % a.refs_x) * a.stride, ry = (ref_id / a.refs_x) * a.stride;
+// Search window of one reference patch (clamped to the image, bm3d.rs:296-305).
+struct RefGeom {
+  int rx, ry, sx0, sy0, nx, ny;
+};
+__device__ __forceinline__ RefGeom ref_geom(const Bm3dMatchArgs& a, int ref_id) {
+  RefGeom G;
+  G.rx = (ref_id % a.refs_x) * a.stride;
+  G.ry = (ref_id / a.refs_x) * a.stride;
   const int half = a.window / 2;
   const int wmax = a.width - 8, hmax = a.height - 8;
-  const int sx0 = rx >= half ? rx - half : 0, sx1 = min(rx + half, wmax);
-  const int sy0 = ry >= half ? ry - half : 0, sy1 = min(ry + half, hmax);
-  const int nx = sx1 - sx0 + 1, ny = sy1 - sy0 + 1;
-  const int span_x = nx + 7, span_y = ny + 7;
+  G.sx0 = G.rx >= half ? G.rx - half : 0;
+  G.sy0 = G.ry >= half ? G.ry - half : 0;
+  G.nx = min(G.rx + half, wmax) - G.sx0 + 1;
+  G.ny = min(G.ry + half, hmax) - G.sy0 + 1;
+  return G;
+}
 
-  // one window row per wave and step, one lane per column (span_x <= 48)
+// Stage rows sy0 .. sy0 + rows - 1 and columns sx0 .. sx0 + cols - 1 (cols <= 64)
+// of the three guide planes, row pitch cols: one row per wave and step, one lane
+// per column.
+__device__ __forceinline__ void stage_window(const Bm3dMatchArgs& a, MatchShared& S, int sx0, int sy0, int cols,
+                                             int rows) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int ch = 0; ch < 3; ++ch) {
     const float* plane = a.guide[ch];
-    for (int yy = wave; yy < span_y; yy += MT / 64)
-      if (lane < span_x) S.win[ch][yy * span_x + lane] = plane[(size_t)(sy0 + yy) * a.width + sx0 + lane];
-    if (tid < 64) S.ref[ch][tid] = plane[(size_t)(ry + (tid >> 3)) * a.width + rx + (tid & 7)];
+    for (int yy = wave; yy < rows; yy += MT / 64)
+      if (lane < cols) S.win[ch][yy * cols + lane] = plane[(size_t)(sy0 + yy) * a.width + sx0 + lane];
+  }
+}
+
+// Match one reference patch against the staged window, whose columns for this
+// patch start at col0 (row pitch `pitch`); writes the patch's group.
+__device__ __forceinline__ void match_staged_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id,
+                                                 const RefGeom& G, int col0, int pitch) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int rx = G.rx, ry = G.ry, sx0 = G.sx0, sy0 = G.sy0, nx = G.nx, ny = G.ny;
+  const int span_x = pitch;
+  // the reference patch lies inside its own window
+  if (tid < 192) {
+    const int ch = tid >> 6, e = tid & 63;
+    S.ref[ch][e] = S.win[ch][(ry - sy0 + (e >> 3)) * pitch + col0 + (rx - sx0) + (e & 7)];
   }
   __syncthreads();
 
@@ -236,9 +263,9 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
     for (int c = tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
   } else {
     switch (a.block_width ? a.block_width : ssd_block_width(nx, ny)) {
-      case 7: ssd_candidates<7>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
-      case 4: ssd_candidates<4>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
-      default: ssd_candidates<2>(S, nx, ny, span_x, ref_cx, ref_cy, thr); break;
+      case 7: ssd_candidates<7>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
+      case 4: ssd_candidates<4>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
+      default: ssd_candidates<2>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
     }
   }
   __syncthreads();
@@ -342,13 +369,34 @@ __device__ __forceinline__ void match_one_ref(const Bm3dMatchArgs& a, MatchShare
   }
 }
 
-// One workgroup per reference patch.  kTrial only separates autotuner trial
-// launches from real ones in kernel traces.
+// One workgroup per reference patch, or (a.pairs) per two horizontally adjacent
+// patches of a row matched against one staged union of their windows.  kTrial
+// only separates autotuner trial launches from real ones in kernel traces.
 template <bool kTrial>
 __global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
   extern __shared__ unsigned char smem_raw[];
   MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
-  match_one_ref(a, S, blockIdx.x);
+  if (!a.pairs) {
+    const RefGeom G = ref_geom(a, blockIdx.x);
+    stage_window(a, S, G.sx0, G.sy0, G.nx + 7, G.ny + 7);
+    __syncthreads();
+    match_staged_ref(a, S, blockIdx.x, G, 0, G.nx + 7);
+    return;
+  }
+  const int pairs_x = (a.refs_x + 1) / 2;
+  const int row = blockIdx.x / pairs_x, px = 2 * (blockIdx.x % pairs_x);
+  const int r0 = row * a.refs_x + px;
+  const bool has1 = px + 1 < a.refs_x;
+  const RefGeom G0 = ref_geom(a, r0);
+  const RefGeom G1 = has1 ? ref_geom(a, r0 + 1) : G0;
+  const int pitch = G1.sx0 + G1.nx - G0.sx0 + 7;  // union columns (same rows)
+  stage_window(a, S, G0.sx0, G0.sy0, pitch, G0.ny + 7);
+  __syncthreads();
+  match_staged_ref(a, S, r0, G0, 0, pitch);
+  if (has1) {
+    __syncthreads();
+    match_staged_ref(a, S, r0 + 1, G1, G1.sx0 - G0.sx0, pitch);
+  }
 }
 
 // The patches named in ref_list[0 .. *ref_count), grid-stride (the patches the
@@ -358,7 +406,11 @@ __global__ void __launch_bounds__(MT) bm3d_match_list_kernel(Bm3dMatchArgs a) {
   MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
   const uint32_t n = *a.ref_count;
   for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
-    match_one_ref(a, S, (int)a.ref_list[it]);
+    const int ref_id = (int)a.ref_list[it];
+    const RefGeom G = ref_geom(a, ref_id);
+    stage_window(a, S, G.sx0, G.sy0, G.nx + 7, G.ny + 7);
+    __syncthreads();
+    match_staged_ref(a, S, ref_id, G, 0, G.nx + 7);
     __syncthreads();
   }
 }
@@ -679,9 +731,15 @@ hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, fl
   return hipGetLastError();
 }
 
+bool bm3d_match_pairs_fit(int window, int stride) {
+  const int span = window + 7, pitch = window + stride + 7;  // interior references have the widest union
+  return pitch <= 64 && span * pitch <= kWinCap;
+}
+
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial) {
-  if (trial) hipLaunchKernelGGL(bm3d_match_kernel<true>, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
-  else hipLaunchKernelGGL(bm3d_match_kernel<false>, dim3(nrefs), dim3(MT), bm3d_match_smem(), s, a);
+  const int grid = a.pairs ? (nrefs / a.refs_x) * ((a.refs_x + 1) / 2) : nrefs;
+  if (trial) hipLaunchKernelGGL(bm3d_match_kernel<true>, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
+  else hipLaunchKernelGGL(bm3d_match_kernel<false>, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct Bm3dMatchArgs {
   const uint32_t* ref_list = nullptr;   // list launch: the patches to match
   const uint32_t* ref_count = nullptr;  // device count of ref_list entries
   int block_width = 0;                  // candidates per thread (2, 4, 7); 0: chosen per window
+  bool pairs = false;                   // one workgroup per two adjacent patches (bm3d_match_pairs_fit)
 };
 
 struct Bm3dFilterArgs {
@@ -61,6 +62,8 @@ size_t bm3d_match_smem();
 size_t bm3d_filter_smem(int footprint_pixels);
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s);
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial = false);
+// whether two adjacent patches' union window fits the matcher's staging area
+bool bm3d_match_pairs_fit(int window, int stride);
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s);
 Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window);
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);
