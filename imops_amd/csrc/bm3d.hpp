@@ -70,7 +70,7 @@ hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);
 // Mixed-precision collaborative filter (bm3d_mfma.hip): DCT / Hadamard products on FP16 MFMA.
 constexpr size_t kBm3dMfmaFragBytes = 2 * 4 * 2 * 64 * 16;
 void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out);
-size_t bm3d_filter_mfma_smem(int footprint_pixels);
+size_t bm3d_filter_mfma_smem(int footprint_pixels, int tile);
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
                                    hipStream_t s, bool trial = false);
 // trial: an autotuner trial launch (a separate kernel instantiation, same code)

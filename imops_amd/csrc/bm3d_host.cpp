@@ -155,14 +155,14 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
 
   // workgroup tile of reference patches: the largest whose LDS footprint lets
-  // four plain-filter workgroups (16 waves) or two 8-wave mixed-filter
-  // workgroups share a CU
+  // four plain-filter workgroups (16 waves) share a CU, or one mixed-filter
+  // workgroup (16 waves for an 8x8 tile) fit
   const int half = p.window / 2;
   auto filter_lds = [&](int t) {
     const int sp = (t - 1) * p.stride + 2 * half + 8;
-    return mixed_filter ? bm3d_filter_mfma_smem(sp * sp) : bm3d_filter_smem(sp * sp);
+    return mixed_filter ? bm3d_filter_mfma_smem(sp * sp, t) : bm3d_filter_smem(sp * sp);
   };
-  const size_t lds_budget = mixed_filter ? 80 * 1024 : 40 * 1024;
+  const size_t lds_budget = mixed_filter ? 128 * 1024 : 40 * 1024;
   int default_tile = 8;
   while (default_tile > 1 && filter_lds(default_tile) > lds_budget) default_tile /= 2;
   // candidates for the autotuner: every tile whose footprint fits one CU's LDS

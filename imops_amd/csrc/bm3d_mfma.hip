@@ -707,12 +707,15 @@ __device__ __forceinline__ void load_patch_rows(const float* __restrict__ plane,
 // the footprint's i64 numerators and u32 denominators.  A workgroup's denominator
 // sum at one pixel is at most 64 references x 16 patches x 1e5 < 2^32, so the
 // 32-bit partial sums are exact.
-constexpr int FM = 512;
-constexpr int kFilterFixedLds = 16384 + (FM / 64) * 2048;
+// Workgroups of 8 waves for tiles of up to 4x4 reference patches, 16 waves for
+// 8x8 tiles (one such workgroup per CU still holds 16 waves).
+constexpr int filter_threads(int tile) { return tile >= 8 ? 1024 : 512; }
+constexpr int filter_fixed_lds(int threads) { return 16384 + (threads / 64) * 2048; }
 
 // kTrial only separates autotuner trial launches from real ones in kernel traces.
-template <bool kStep1, bool kTrial>
+template <bool kStep1, bool kTrial, int FM>
 __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, const half8* __restrict__ frags) {
+  constexpr int kFilterFixedLds = filter_fixed_lds(FM);
   extern __shared__ __align__(16) unsigned char fsh[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -937,20 +940,34 @@ void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out) {
           }
 }
 
-size_t bm3d_filter_mfma_smem(int footprint_pixels) {
-  return kFilterFixedLds + (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
+size_t bm3d_filter_mfma_smem(int footprint_pixels, int tile) {
+  return filter_fixed_lds(filter_threads(tile)) + (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
 }
 
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
                                    hipStream_t s, bool trial) {
   const half8* f = reinterpret_cast<const half8*>(frags);
-  if (trial) {
-    if (step1) hipLaunchKernelGGL((bm3d_filter_mfma_kernel<true, true>), dim3(nblocks), dim3(FM), smem, s, a, f);
-    else hipLaunchKernelGGL((bm3d_filter_mfma_kernel<false, true>), dim3(nblocks), dim3(FM), smem, s, a, f);
+  const int threads = filter_threads(a.tile);
+#define LAUNCH_FILTER(S1, TR, TH) \
+  hipLaunchKernelGGL((bm3d_filter_mfma_kernel<S1, TR, TH>), dim3(nblocks), dim3(TH), smem, s, a, f)
+  if (threads == 1024) {
+    if (trial) {
+      if (step1) LAUNCH_FILTER(true, true, 1024);
+      else LAUNCH_FILTER(false, true, 1024);
+    } else {
+      if (step1) LAUNCH_FILTER(true, false, 1024);
+      else LAUNCH_FILTER(false, false, 1024);
+    }
   } else {
-    if (step1) hipLaunchKernelGGL((bm3d_filter_mfma_kernel<true, false>), dim3(nblocks), dim3(FM), smem, s, a, f);
-    else hipLaunchKernelGGL((bm3d_filter_mfma_kernel<false, false>), dim3(nblocks), dim3(FM), smem, s, a, f);
+    if (trial) {
+      if (step1) LAUNCH_FILTER(true, true, 512);
+      else LAUNCH_FILTER(false, true, 512);
+    } else {
+      if (step1) LAUNCH_FILTER(true, false, 512);
+      else LAUNCH_FILTER(false, false, 512);
+    }
   }
+#undef LAUNCH_FILTER
   return hipGetLastError();
 }
 
