@@ -703,8 +703,9 @@ __device__ __forceinline__ void load_patch_rows(const float* __restrict__ plane,
   }
 }
 
-// LDS: DCT fragments (16 KB), one 16 x 64 FP16 transpose per wave (8 x 2 KB), then
-// the footprint's i64 numerators and u32 denominators.  A workgroup's denominator
+// LDS: DCT fragments (16 KB), one 16 x 64 FP16 transpose per wave (2 KB each), the
+// footprint's i64 numerators and u32 denominators, then the tile's group tables
+// (read once, used by the three channels).  A workgroup's denominator
 // sum at one pixel is at most 64 references x 16 patches x 1e5 < 2^32, so the
 // 32-bit partial sums are exact.
 // Workgroups of 8 waves for tiles of up to 4x4 reference patches, 16 waves for
@@ -734,8 +735,13 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
   _Float16* tbuf = reinterpret_cast<_Float16*>(fsh + 16384) + wave * 16 * 64;
   unsigned long long* fnum = reinterpret_cast<unsigned long long*>(fsh + kFilterFixedLds);
   uint32_t* fden = reinterpret_cast<uint32_t*>(fnum + fp);
+  uint32_t* lgrp = fden + fp;  // the tile's group tables, shared by the three channels
   // constant B fragments: forward (Kd^T) at [t][s], inverse (Kd) at [4 + t][s]
   for (int i = tid; i < 1024; i += FM) lfrag[i] = frags[i];
+  for (int i = tid; i < nref * 17; i += FM) {
+    const int r = i / 17;
+    lgrp[i] = a.groups[(size_t)((r0y + r / nrx) * a.refs_x + r0x + r % nrx) * 17 + (i - r * 17)];
+  }
 #define KF(t, s) lfrag[((t) * 2 + (s)) * 64 + lane]
 #define KI(t, s) lfrag[((4 + (t)) * 2 + (s)) * 64 + lane]
 
@@ -749,8 +755,7 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
     const float* __restrict__ gplane = a.guide[ch];
     const float* __restrict__ nplane = a.noisy[ch];
     for (int r = wave; r < nref; r += FM / 64) {  // wave-uniform
-      const int ref_id = (r0y + r / nrx) * a.refs_x + r0x + r % nrx;
-      const uint32_t* grp = a.groups + (size_t)ref_id * 17;
+      const uint32_t* grp = lgrp + r * 17;
       const int g = (int)grp[0];
       if (g == 0) continue;  // nothing to aggregate
       // patch n (A rows) and patches 4q + j (C rows) positions
@@ -941,7 +946,9 @@ void bm3d_filter_mfma_fragments(const float* dct, uint16_t* out) {
 }
 
 size_t bm3d_filter_mfma_smem(int footprint_pixels, int tile) {
-  return filter_fixed_lds(filter_threads(tile)) + (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
+  return filter_fixed_lds(filter_threads(tile)) +
+         (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t)) +
+         (size_t)tile * tile * 17 * sizeof(uint32_t);
 }
 
 hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, const void* frags,
