@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   const int ry = (wry0 + q) * st;
   const bool yvalid = wave_active && q < wny;
   const int wy0 = max(ry - half, 0), wy1 = min(ry + half, hmax);
-  int rxa[4], wx0[4], ew[4], limgeo[4], npos[4];
+  int rxa[4], wx0[4], ew[4], ewv[4], limgeo[4], npos[4];
   bool vj[4];
   float na[4], ra[4], asure[4], epsmax[4];
 #pragma unroll
@@ -274,6 +274,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     vj[j] = yvalid && j < wnx;
     wx0[j] = max(rxa[j] - half, 0);
     ew[j] = min(rxa[j] + half, wmax) - wx0[j] + 1;
+    ewv[j] = vj[j] ? ew[j] : 0;  // window width for the unsigned column test (0: no such reference)
     // candidates of the window besides the reference, and the scan index of
     // the last of the first `cap` of them
     npos[j] = vj[j] ? ew[j] * (wy1 - wy0 + 1) - 1 : 0;
@@ -389,40 +390,38 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   // ---- pass 1: rejections, and per lane the two smallest d among accepted
   // candidates inside the geometric cap.  Pairs that are surely accepted and
   // cannot enter a lane's two smallest take no further work.
-  float b1d[4], b1n[4], b2d[4], b2n[4];
+  float b1d[4], b2d[4];
   int rejl[4];
   bool rejpre[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     b1d[j] = b2d[j] = kInf;
-    b1n[j] = b2n[j] = 0.0f;
     rejl[j] = 0;
     rejpre[j] = false;
   }
-  // best-two update of lane-local tracking (branch-free)
-  auto track = [&](int j, bool on, float d, float nbv) {
-    const bool c1 = on && d < b1d[j], c2 = on && d < b2d[j];
-    b2n[j] = c1 ? b1n[j] : (c2 ? nbv : b2n[j]);
-    b2d[j] = c1 ? b1d[j] : (c2 ? d : b2d[j]);
-    b1n[j] = c1 ? nbv : b1n[j];
-    b1d[j] = c1 ? d : b1d[j];
+  // the lane's two smallest distances (min / max keep them in order)
+  auto track = [&](int j, bool on, float d) {
+    const float dv = on ? d : kInf;
+    b2d[j] = fminf(b2d[j], fmaxf(b1d[j], dv));
+    b1d[j] = fminf(b1d[j], dv);
   };
   if (wave_active) {
     sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
+      // the union's columns past ux1 fail every reference's column test
       const bool yin = yvalid && cy >= wy0 && cy <= wy1;
       const bool selfrow = cy == ry;
-      const bool cin = yin && cx <= ux1;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const bool inw = cin && vj[j] && cx >= wx0[j] && cx < wx0[j] + ew[j] && !(selfrow && cx == rxa[j]);
+        const int ox = cx - wx0[j];
+        const bool inw = yin && (unsigned)ox < (unsigned)ewv[j] && !(selfrow && cx == rxa[j]);
         const float d = (na[j] + nbv) - 2.0f * acc[j];
-        const bool pre = (cy - wy0) * ew[j] + (cx - wx0[j]) <= limgeo[j];
+        const bool pre = (int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limgeo[j];
         const bool sure = d <= asure[j];
-        track(j, inw && sure && pre, d, nbv);
+        track(j, inw && sure && pre, d);
         if (inw && !sure) {  // rare: the threshold is within reach
           const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
           const bool ok = classify(j, d, nbv, rb, cx, cy);
-          track(j, ok && pre, d, nbv);
+          track(j, ok && pre, d);
           if (!ok) {
             ++rejl[j];
             rejpre[j] = rejpre[j] || pre;
@@ -483,14 +482,13 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       float u1 = kInf, u2 = kInf;
+      // upper bounds with the block's largest candidate norm (each term grows with the norm)
       if (b1d[j] != kInf) {
-        const float eps = kEpsRel * (na[j] + b1n[j]) + kEpsAbs;
-        const float f = hsqrt(fmaxf(b1d[j] + eps, 0.0f)) * (1.0f + 1e-6f) + ra[j] + kU * hsqrt(b1n[j]) * (1.0f + 1e-6f) + 1e-5f;
+        const float f = hsqrt(fmaxf(b1d[j] + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
         u1 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
       }
       if (b2d[j] != kInf) {
-        const float eps = kEpsRel * (na[j] + b2n[j]) + kEpsAbs;
-        const float f = hsqrt(fmaxf(b2d[j] + eps, 0.0f)) * (1.0f + 1e-6f) + ra[j] + kU * hsqrt(b2n[j]) * (1.0f + 1e-6f) + 1e-5f;
+        const float f = hsqrt(fmaxf(b2d[j] + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
         u2 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
       }
       int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
@@ -512,11 +510,12 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   __syncthreads();
 
   // ---- pass 2: collect every capped, accepted candidate whose lower bound reaches T.
-  // A candidate's lower bound is <= T exactly when d <= (sT + r)^2 + eps with
-  // sT = sqrt(T / ((1 - kSumRel)(1 - 1e-6))).
+  // A candidate's lower bound is <= T only if d <= (sT + r)^2 + eps with
+  // sT = sqrt(T / ((1 - kSumRel)(1 - 1e-6))); r and eps are taken at the block's
+  // largest candidate norm, so the test keeps a superset of those candidates.
   const int UW = ux1 - ux0 + 1;
   if (wave_active) {
-    float sT[4], dK[4];
+    float sT[4], dK[4], dA[4];
     int limit[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -526,22 +525,27 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       else if (min(nacc[j], cap) <= K || mt.d15 == kInf) sT[j] = kInf;  // every accepted candidate
       else sT[j] = hsqrt(mt.d15 / ((1.0f - kSumRel) * (1.0f - 1e-6f))) * (1.0f + 1e-6f);
       dK[j] = K <= 0 ? -kInf : (sT[j] == kInf ? kInf : mt.dK);
+      // d bound of the exact band with the block's largest candidate norm (a superset
+      // of every candidate's own band)
+      if (K <= 0) {
+        dA[j] = -kInf;
+      } else {
+        const float f = sT[j] + ra[j] + rbmax;
+        dA[j] = f * f * (1.0f + 1e-5f) + epsmax[j];
+      }
     }
     sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
       const bool yin = yvalid && cy >= wy0 && cy <= wy1;
       const bool selfrow = cy == ry;
-      const bool cin = yin && cx <= ux1;
-      const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const bool inw = cin && vj[j] && cx >= wx0[j] && cx < wx0[j] + ew[j] && !(selfrow && cx == rxa[j]);
+        const int ox = cx - wx0[j];
+        const bool inw = yin && (unsigned)ox < (unsigned)ewv[j] && !(selfrow && cx == rxa[j]);
         const float d = (na[j] + nbv) - 2.0f * acc[j];
-        const float f = sT[j] + ra[j] + rb;
-        const float eps = kEpsRel * (na[j] + nbv) + kEpsAbs;
-        const bool inA = d <= f * f * (1.0f + 1e-5f) + eps;  // lower bound reaches T
-        const bool inB = d <= dK[j];                          // approximate top K
-        if (inw && (inA || inB) && (cy - wy0) * ew[j] + (cx - wx0[j]) <= limit[j] &&
-            (d <= asure[j] || classify(j, d, nbv, rb, cx, cy))) {
+        const bool inA = d <= dA[j];  // lower bound may reach T
+        const bool inB = d <= dK[j];  // approximate top K
+        if (inw && (inA || inB) && (int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limit[j] &&
+            (d <= asure[j] || classify(j, d, nbv, kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f, cx, cy))) {
           const int slot = wave * 16 + 4 * q + j;
           const uint16_t li = (uint16_t)((cy - uy0) * UW + (cx - ux0));
           if (inA) {
