@@ -468,10 +468,13 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         if (seen + inrow >= cap) break;
         seen += inrow;
       }
-      // row < mh: the cap-th acceptance lies in this row
+      // row < mh: the cap-th acceptance lies in this row; a row without rejections
+      // accepts every candidate, otherwise the reference's own test settles it
       const int cy = my0 + row, cx = mx0 + lane;
+      const bool row_rej = row < mh && ((rejrow[slot * kRowWords + (row >> 1)] >> (16 * (row & 1))) & 0xffffu) != 0;
       bool ok = false;
-      if (row < mh && lane < mw && !(cy == mry && cx == mrx)) ok = exact_accept(a.guide, W, mrx, mry, cx, cy, thr);
+      if (row < mh && lane < mw && !(cy == mry && cx == mrx))
+        ok = row_rej ? exact_accept(a.guide, W, mrx, mry, cx, cy, thr) : true;
       const unsigned long long bal = __ballot(ok);
       const int before = seen + __popcll(bal & ((1ull << lane) - 1ull));
       if (ok && before == cap - 1) meta[slot].limit = row * mw + lane;
