@@ -78,8 +78,9 @@ SIGNATURES = {
     "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
     "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
-    "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz]),
-    "imops_film_fit_backward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz,
+    "imops_film_fit_state_floats": (_i, [_i, _sz, _sz, _sz, C.POINTER(_sz)]),
+    "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp]),
+    "imops_film_fit_backward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp,
                                      C.POINTER(FilmFitParams)]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
@@ -113,6 +114,14 @@ def lib() -> C.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -m imops_amd.build` (no CPU fallback exists)"
             )
+        # PyTorch-ROCm wheels carry their own HIP runtime under the same soname as the
+        # system one this library links.  Whichever loads first serves the process; if it
+        # were the system one, torch's later GPU initialisation finds no devices.  Loading
+        # torch first keeps one runtime, torch's, whatever order the caller imports in.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

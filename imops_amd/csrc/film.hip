@@ -70,8 +70,10 @@ __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const Fil
 }
 
 __global__ void blur_h_kernel(const float* __restrict__ in, float* __restrict__ out, int w, int h,
-                              const float* __restrict__ taps, int radius) {
+                              const float* __restrict__ taps, int radius, BlurPlanes pl) {
   const size_t n = (size_t)w * h;
+  in += blockIdx.y * pl.in;
+  out += blockIdx.y * n;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
     const float* row = in + (size_t)y * w;
@@ -87,8 +89,11 @@ __global__ void blur_h_kernel(const float* __restrict__ in, float* __restrict__ 
 
 // Vertical pass + epilogue on the unblurred plane `orig` (may alias `out`).
 __global__ void blur_v_kernel(const float* __restrict__ tmp, const float* orig, float* out, int w, int h,
-                              const float* __restrict__ taps, int radius, int mode, float a) {
+                              const float* __restrict__ taps, int radius, int mode, float a, BlurPlanes pl) {
   const size_t n = (size_t)w * h;
+  tmp += blockIdx.y * n;
+  if (orig) orig += blockIdx.y * pl.orig;
+  out += blockIdx.y * pl.out;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
     float acc = 0.0f;
@@ -119,6 +124,11 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, const float* orig, 
 __global__ void capture_reduce_kernel(const float* __restrict__ absorbed, const float* __restrict__ bounce, size_t n,
                                       const FilmDeviceTables* __restrict__ T, float* __restrict__ image,
                                       float* __restrict__ mask) {
+  const size_t b = blockIdx.y, planes = (size_t)T->n_emulsions * n;  // image b of a batch
+  absorbed += b * planes;
+  if (bounce) bounce += b * n;
+  image += b * planes;
+  mask += b * planes;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     for (int e = 0; e < T->n_emulsions; ++e) {
       const FilmEmulsionTables& E = T->em[e];
@@ -250,6 +260,10 @@ __global__ void grain_apply_kernel(float* __restrict__ dye, const float* __restr
 __global__ void scan_kernel(const float* __restrict__ image, const float* __restrict__ mask, size_t n,
                             const FilmDeviceTables* __restrict__ T, float* __restrict__ rgb) {
   const int E = T->n_emulsions;
+  const size_t b = blockIdx.y;  // image b of a batch
+  image += b * E * n;
+  mask += b * E * n;
+  rgb += b * 3 * n;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     double dens[16];
     for (int i = 0; i < 16; ++i) dens[i] = 0.0;
@@ -263,7 +277,7 @@ __global__ void scan_kernel(const float* __restrict__ image, const float* __rest
       }
     }
     double t[16];
-    for (int i = 0; i < 16; ++i) t[i] = pow(10.0, -dens[i]) * T->scanner[i];
+    for (int i = 0; i < 16; ++i) t[i] = exp(-dens[i] * 2.302585092994045684) * T->scanner[i];  // 10^-d
     double xyz[3];
     for (int c = 0; c < 3; ++c) {
       double acc = 0.0;
@@ -310,18 +324,33 @@ int grid1d(size_t n) {
 // LDS-tiled separable passes (gaussian_blur_separable, blur.rs:10-37), with the
 // same k-ordered sums as blur_h_kernel / blur_v_kernel.  Rows: one workgroup
 // per segment of kRowSeg outputs; the segment and its reflected halo are staged
-// with coalesced loads.  Columns: one workgroup per 64-column x kColH-row tile,
-// one lane per column, each tile row a 256-byte load.
+// with coalesced loads.  Columns: one workgroup per 64-column x th-row tile,
+// one lane per column, each tile row a 256-byte load; th is kColH on large
+// images and halves (down to kColHMin) until the grid has kColTiles tiles, so
+// small images (the adjoint's crops) still fill the chip.
 constexpr int BT = 256;
 constexpr int kRowSeg = 2048;
 constexpr int kColH = 128;
+constexpr int kColHMin = 16;
+constexpr int kColTiles = 1024;
 
 size_t blur_row_smem(int radius) { return (size_t)(2 * radius + 1 + kRowSeg + 2 * radius) * sizeof(float); }
-size_t blur_col_smem(int radius) { return (size_t)(2 * radius + 1 + (kColH + 2 * radius) * 64) * sizeof(float); }
+size_t blur_col_smem(int radius, int th) { return (size_t)(2 * radius + 1 + (th + 2 * radius) * 64) * sizeof(float); }
+
+int blur_col_tiles(int w, int h, int th) { return ((w + 63) / 64) * ((h + th - 1) / th); }
+
+int blur_col_height(int w, int h) {
+  int th = kColH;
+  while (th > kColHMin && blur_col_tiles(w, h, th) < kColTiles) th /= 2;
+  return th;
+}
 
 __global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
-                                                      int h, const float* __restrict__ taps, int radius) {
+                                                      int h, const float* __restrict__ taps, int radius,
+                                                      BlurPlanes pl) {
   extern __shared__ float sh[];
+  in += blockIdx.y * pl.in;
+  out += blockIdx.y * (size_t)w * h;
   const int tid = threadIdx.x, nt = 2 * radius + 1;
   const int segs = (w + kRowSeg - 1) / kRowSeg;
   const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * kRowSeg;
@@ -342,12 +371,15 @@ __global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ 
 
 __global__ void __launch_bounds__(BT) blur_col_kernel(const float* __restrict__ tmp, const float* orig, float* out,
                                                       int w, int h, const float* __restrict__ taps, int radius,
-                                                      int mode, float a) {
+                                                      int mode, float a, int th, BlurPlanes pl) {
   extern __shared__ float sh[];
+  tmp += blockIdx.y * (size_t)w * h;
+  if (orig) orig += blockIdx.y * pl.orig;
+  out += blockIdx.y * pl.out;
   const int tid = threadIdx.x, nt = 2 * radius + 1;
   const int tiles_x = (w + 63) / 64;
-  const int xs = (blockIdx.x % tiles_x) * 64, ys = (blockIdx.x / tiles_x) * kColH;
-  const int nyo = min(kColH, h - ys);
+  const int xs = (blockIdx.x % tiles_x) * 64, ys = (blockIdx.x / tiles_x) * th;
+  const int nyo = min(th, h - ys);
   const int c = tid & 63, g = tid >> 6, x = xs + c;
   float* tp = sh;
   float* buf = sh + nt;
@@ -381,24 +413,26 @@ __global__ void __launch_bounds__(BT) blur_col_kernel(const float* __restrict__ 
 
 constexpr size_t kBlurSmemMax = 120 * 1024;
 
-// Horizontal then vertical pass (tiled kernels while their LDS fits, else the
-// one-pixel-per-thread ones).
+// Horizontal then vertical pass over pl.count planes (tiled kernels while their
+// LDS fits, else the one-pixel-per-thread ones).  tmp holds pl.count planes.
 void blur_passes(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps_h,
-                 const float* taps_v, int radius, int mode, float a, hipStream_t s) {
+                 const float* taps_v, int radius, int mode, float a, const BlurPlanes& pl, hipStream_t s) {
   const size_t n = (size_t)w * h;
   if (blur_row_smem(radius) <= kBlurSmemMax) {
     const int segs = (w + kRowSeg - 1) / kRowSeg;
-    hipLaunchKernelGGL(blur_row_kernel, dim3(h * segs), dim3(BT), blur_row_smem(radius), s, in, tmp, w, h, taps_h,
-                       radius);
+    hipLaunchKernelGGL(blur_row_kernel, dim3(h * segs, pl.count), dim3(BT), blur_row_smem(radius), s, in, tmp, w, h,
+                       taps_h, radius, pl);
   } else {
-    hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps_h, radius);
+    hipLaunchKernelGGL(blur_h_kernel, dim3(grid1d(n), pl.count), dim3(256), 0, s, in, tmp, w, h, taps_h, radius, pl);
   }
-  if (blur_col_smem(radius) <= kBlurSmemMax) {
-    const int tiles = ((w + 63) / 64) * ((h + kColH - 1) / kColH);
-    hipLaunchKernelGGL(blur_col_kernel, dim3(tiles), dim3(BT), blur_col_smem(radius), s, tmp, orig, out, w, h, taps_v,
-                       radius, mode, a);
+  // the column tile height counts every plane's tiles towards filling the chip
+  const int th = blur_col_height(w, h * pl.count);
+  if (blur_col_smem(radius, th) <= kBlurSmemMax) {
+    hipLaunchKernelGGL(blur_col_kernel, dim3(blur_col_tiles(w, h, th), pl.count), dim3(BT), blur_col_smem(radius, th),
+                       s, tmp, orig, out, w, h, taps_v, radius, mode, a, th, pl);
   } else {
-    hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n)), dim3(256), 0, s, tmp, orig, out, w, h, taps_v, radius, mode, a);
+    hipLaunchKernelGGL(blur_v_kernel, dim3(grid1d(n), pl.count), dim3(256), 0, s, tmp, orig, out, w, h, taps_v, radius,
+                       mode, a, pl);
   }
 }
 
@@ -410,20 +444,21 @@ hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, fl
 }
 
 hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps,
-                     int radius, int mode, float a, hipStream_t s) {
-  blur_passes(in, tmp, orig, out, w, h, taps, taps, radius, mode, a, s);
+                     int radius, int mode, float a, hipStream_t s, const BlurPlanes& pl) {
+  blur_passes(in, tmp, orig, out, w, h, taps, taps, radius, mode, a, pl, s);
   return hipGetLastError();
 }
 
 hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
-                      int radius, hipStream_t s) {
-  blur_passes(in, tmp, nullptr, out, w, h, taps_h, taps_v, radius, (int)BLUR_STORE, 0.0f, s);
+                      int radius, hipStream_t s, const BlurPlanes& pl) {
+  blur_passes(in, tmp, nullptr, out, w, h, taps_h, taps_v, radius, (int)BLUR_STORE, 0.0f, pl, s);
   return hipGetLastError();
 }
 
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
-                               float* image, float* mask, hipStream_t s) {
-  hipLaunchKernelGGL(capture_reduce_kernel, dim3(grid1d(n)), dim3(256), 0, s, absorbed, bounce, n, T, image, mask);
+                               float* image, float* mask, hipStream_t s, int images) {
+  hipLaunchKernelGGL(capture_reduce_kernel, dim3(grid1d(n), images), dim3(256), 0, s, absorbed, bounce, n, T, image,
+                     mask);
   return hipGetLastError();
 }
 
@@ -436,7 +471,7 @@ hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64
                       double* partial, float d_max, float kappa, hipStream_t s) {
   const size_t n = (size_t)w * h;
   hipLaunchKernelGGL(grain_noise_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, w, h, key0);
-  blur_passes(noise, tmp, noise, noise, w, h, taps, taps, radius, (int)BLUR_STORE, 0.0f, s);
+  blur_passes(noise, tmp, noise, noise, w, h, taps, taps, radius, (int)BLUR_STORE, 0.0f, BlurPlanes{}, s);
   const int parts = kFilmReduceBlocks;
   hipLaunchKernelGGL(sumsq_partial_kernel, dim3(parts), dim3(256), 0, s, noise, n, partial);
   float* norm = reinterpret_cast<float*>(partial + parts);
@@ -446,8 +481,8 @@ hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64
 }
 
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(scan_kernel, dim3(grid1d(n)), dim3(256), 0, s, image, mask, n, T, rgb);
+                     hipStream_t s, int images) {
+  hipLaunchKernelGGL(scan_kernel, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
   return hipGetLastError();
 }
 

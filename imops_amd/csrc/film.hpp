@@ -12,6 +12,14 @@ constexpr int kFilmReduceBlocks = 1024;  // f64 partial sums of the grain varian
 
 enum FilmBlurMode : int { BLUR_STORE = 0, BLUR_MIX = 1, BLUR_ADJ = 2 };
 
+// A batch of W x H planes blurred by one launch: plane i reads in + i * in,
+// its epilogue source orig + i * orig and writes out + i * out (element
+// strides); the row pass keeps plane i at tmp + i * W * H.
+struct BlurPlanes {
+  int count = 1;
+  size_t in = 0, orig = 0, out = 0;
+};
+
 struct FilmEmulsionTables {
   double lut_log[64], lut_frac[64];  // developable-fraction LUT (log10 fluence -> fraction)
   double eps[16], mask_eps[16];      // dye / coloured-mask absorptivity
@@ -40,7 +48,14 @@ struct FilmDeviceTables {
   float inv_dmin[3], inv_shoulder[3], inv_gain[3];
 };
 
-// Differentiable film model (film_fit.hip).
+// Differentiable film model (film_fit.hip).  Parameter gradients are f64 sums:
+// each reducing kernel stores one row of kFitSlots partial sums per block
+// (d_max, gamma, gain per emulsion at kFitSlotDmax + {0, 8, 16}; the exposure
+// contraction C[k][e] at kFitSlotC + 8 k + e; the halation width at
+// kFitSlotSigma), and film_fit_reduce adds the rows in order into acc.
+constexpr int kFitSlots = 64;
+constexpr int kFitSlotDmax = 0, kFitSlotC = 24, kFitSlotSigma = 48;
+constexpr int kFitReduceRows = 1024;  // blocks of a reducing launch (whole batch), when the pixels allow
 struct FitExposeConsts {
   double minv[3][3];                 // ACEScg -> basis weights
   double c[3][kFilmMaxEmulsions];    // band-absorbed fluence of emulsion e per unit basis weight k
@@ -48,27 +63,38 @@ struct FitExposeConsts {
 };
 
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s);
-hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s);
-// out = c0 * orig + c1 * (V o H)^T in   (the transpose of film_blur's reflect-101 Gaussian; radius <= w-1, h-1)
+// Batched over `images` images: rgb image b at rgb + 3 b n, its E planes at planes + b E n.
+hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s,
+                           int images = 1);
+// out = c0 * orig + c1 * (V o H)^T in   (the transpose of film_blur's reflect-101 Gaussian; radius <= w-1, h-1);
+// orig, when used, is out (same strides)
 hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, float* out, int w, int h,
-                             const float* taps, int radius, float c0, float c1, hipStream_t s);
+                             const float* taps, int radius, float c0, float c1, hipStream_t s,
+                             const BlurPlanes& pl = BlurPlanes{});
+// Rows of partial sums a reducing launch over `images` images of n pixels writes.
+int fit_partial_rows(size_t n, int images);
+// S, sbar: E planes per image; B, bbar: one plane per image; gout: 3n per image
 hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, const float* gout, size_t n,
-                              float* sbar, float* bbar, double* acc, hipStream_t s);
-hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k, double* acc,
-                                hipStream_t s);
-hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* acc, hipStream_t s);
-hipError_t film_fit_zero(double* p, int n, hipStream_t s);
+                              float* sbar, float* bbar, double* part_rows, hipStream_t s, int images = 1);
+hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k,
+                                double* part_rows, hipStream_t s, int images = 1);
+// sum_p a (b + c) over one plane per image (planes of consecutive images n apart)
+hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* part_rows, hipStream_t s,
+                        int images = 1);
+hipError_t film_fit_zero(double* p, size_t n, hipStream_t s);
+hipError_t film_fit_reduce(const double* part_rows, int rows, double* acc, hipStream_t s);
 hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps,
-                     int radius, int mode, float a, hipStream_t s);
+                     int radius, int mode, float a, hipStream_t s, const BlurPlanes& pl = BlurPlanes{});
 // separable blur with different horizontal and vertical taps (same radius), plain store
 hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
-                      int radius, hipStream_t s);
+                      int radius, hipStream_t s, const BlurPlanes& pl = BlurPlanes{});
+// Batched over `images`: absorbed, image, mask E planes per image, bounce one plane per image.
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
-                               float* image, float* mask, hipStream_t s);
+                               float* image, float* mask, hipStream_t s, int images = 1);
 hipError_t film_dir_apply(float* image, const float* inh, size_t n, const FilmDeviceTables* T, hipStream_t s);
 hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64_t key0, const float* taps, int radius,
                       double* partial, float d_max, float kappa, hipStream_t s);
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s);
+                     hipStream_t s, int images = 1);
 
 }  // namespace imops

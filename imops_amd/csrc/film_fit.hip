@@ -11,9 +11,13 @@
 //
 // (pichromatic/src/film: exposure/mod.rs:45-137, halation.rs:24-181,
 // capture.rs:37-106, reduction.rs:16-68, densitometry.rs:13-73) with grain,
-// DIR inhibition and adjacency off.  Parameter gradients are reduced in f64.
+// DIR inhibition and adjacency off.  The reverse pass is f32 arithmetic (the
+// capture LUT lookup aside); parameter gradients are summed in f64 in a fixed
+// order, so they are deterministic.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <algorithm>
 
 #include "film.hpp"
 
@@ -26,6 +30,8 @@ int grid_of(size_t n) {
 }
 
 __global__ void fit_expose_kernel(const float* __restrict__ rgb, size_t n, FitExposeConsts k, float* __restrict__ planes) {
+  rgb += blockIdx.y * 3 * n;
+  planes += blockIdx.y * k.n_emulsions * n;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     const double r = rgb[p * 3], g = rgb[p * 3 + 1], b = rgb[p * 3 + 2];
     double w[3];
@@ -63,8 +69,10 @@ __device__ __forceinline__ float adj_sum(const float* __restrict__ v, size_t str
 }
 
 __global__ void blur_vT_kernel(const float* __restrict__ in, float* __restrict__ out, int w, int h,
-                               const float* __restrict__ taps, int r) {
+                               const float* __restrict__ taps, int r, BlurPlanes pl) {
   const size_t n = (size_t)w * h;
+  in += blockIdx.y * pl.in;
+  out += blockIdx.y * n;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
     out[i] = adj_sum(in + x, (size_t)w, h, y, taps, r);
@@ -73,8 +81,11 @@ __global__ void blur_vT_kernel(const float* __restrict__ in, float* __restrict__
 
 // out = c0 * orig + c1 * H^T(in)   (orig may alias out)
 __global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, float* out, int w, int h,
-                               const float* __restrict__ taps, int r, float c0, float c1) {
+                               const float* __restrict__ taps, int r, float c0, float c1, BlurPlanes pl) {
   const size_t n = (size_t)w * h;
+  in += blockIdx.y * n;
+  if (orig) orig += blockIdx.y * pl.orig;
+  out += blockIdx.y * pl.out;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
     const float acc = adj_sum(in + (size_t)y * w, 1, w, x, taps, r);
@@ -82,9 +93,12 @@ __global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, 
   }
 }
 
-// Block-wide f64 sums of `cnt` per-thread values, added atomically to out[0..cnt).
+// Block-wide f64 sums of `cnt` per-thread values, stored to this block's row of
+// the partial-sum table (kFitSlots doubles per block, block (x, y) at row
+// y * gridDim.x + x); fit_reduce_kernel adds the rows in a fixed order, so the
+// gradients do not depend on the order in which blocks finish.
 template <int kMax>
-__device__ void block_accumulate(double (&v)[kMax], int cnt, double* out) {
+__device__ void block_partials(double (&v)[kMax], int cnt, double* part_rows, int slot0) {
   __shared__ double part[4][kMax];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int k = 0; k < cnt; ++k) {
@@ -96,33 +110,58 @@ __device__ void block_accumulate(double (&v)[kMax], int cnt, double* out) {
   if (threadIdx.x < cnt) {
     double s = 0.0;
     for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) s += part[wv][threadIdx.x];
-    atomicAdd(out + threadIdx.x, s);
+    part_rows[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kFitSlots + slot0 + threadIdx.x] = s;
   }
 }
 
-// Reverse of capture + reduction + scan for one image.  In: S planes, bounce B,
-// dL/dout.  Out: dL/dS planes (into sbar), dL/dB, and f64 sums of dL/dd_max,
-// dL/dgamma, dL/dgain (3 x E).
+// One batch image's pixels [0, n) split over gridDim.x blocks of 256 threads.
+#define FIT_PIXELS(p, n) \
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < (n); p += (size_t)gridDim.x * blockDim.x)
+
+// Reverse of capture + reduction + scan for one image of a batch.  In: S planes,
+// bounce B, dL/dout.  Out: dL/dS planes (into sbar), dL/dB, and f64 sums of
+// dL/dd_max, dL/dgamma, dL/dgain (3 x E).  The capture LUT lookup is the
+// forward's (f64, so the curve segment and fraction are the forward's); the
+// rest of the reverse pass is f32 arithmetic on f32 copies of the spectral
+// tables, with the per-block parameter sums in f64.
 __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restrict__ S, const float* __restrict__ B,
                                                             const FilmDeviceTables* __restrict__ T,
                                                             const float* __restrict__ gout, size_t n,
                                                             float* __restrict__ sbar, float* __restrict__ bbar,
-                                                            double* __restrict__ acc) {
+                                                            double* __restrict__ part_rows) {
+  __shared__ float eps_s[kFilmMaxEmulsions][16], meps_s[kFilmMaxEmulsions][16], cmf_s[3][16], scan_s[16], x2a_s[3][3];
   const int E = T->n_emulsions;
+  for (int i = threadIdx.x; i < kFilmMaxEmulsions * 16; i += blockDim.x) {
+    const int e = i / 16, l = i % 16;
+    eps_s[e][l] = e < E ? (float)T->em[e].eps[l] : 0.0f;
+    meps_s[e][l] = e < E && T->em[e].has_mask ? (float)T->em[e].mask_eps[l] : 0.0f;
+  }
+  for (int i = threadIdx.x; i < 48; i += blockDim.x) {
+    const float tw = (i % 16 == 0 || i % 16 == 15) ? 10.0f : 20.0f;  // trapezoid weights of the scan integral
+    cmf_s[i / 16][i % 16] = tw * (float)T->cmf[i / 16][i % 16];
+  }
+  if (threadIdx.x < 16) scan_s[threadIdx.x] = (float)T->scanner[threadIdx.x];
+  if (threadIdx.x < 9) x2a_s[threadIdx.x / 3][threadIdx.x % 3] = (float)(T->xyz2ap1[threadIdx.x / 3][threadIdx.x % 3] * T->dmin_scale);
+  __syncthreads();
+  const size_t b = blockIdx.y;  // image b of a batch
+  S += b * E * n;
+  sbar += b * E * n;
+  if (B) B += b * n;
+  if (bbar) bbar += b * n;
+  gout += b * 3 * n;
   double part[3 * kFilmMaxEmulsions];
   for (int k = 0; k < 3 * kFilmMaxEmulsions; ++k) part[k] = 0.0;
   const double ln10 = 2.302585092994045684;
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    float D[kFilmMaxEmulsions], M[kFilmMaxEmulsions], dDdf[kFilmMaxEmulsions], P[kFilmMaxEmulsions];
-    float eff[kFilmMaxEmulsions];
-    double slope[kFilmMaxEmulsions];
+  const float ln10f = 2.30258512f, log2_10f = 3.32192802f;
+  FIT_PIXELS(p, n) {
+    float D[kFilmMaxEmulsions], M[kFilmMaxEmulsions], dDdf[kFilmMaxEmulsions], eff[kFilmMaxEmulsions];
+    float slope[kFilmMaxEmulsions];
     const float bv = B ? B[p] : 0.0f;
     for (int e = 0; e < E; ++e) {
       const FilmEmulsionTables& Em = T->em[e];
       const float phi = S[(size_t)e * n + p] + (B && Em.gain > 0.0f ? Em.gain * bv : 0.0f);
-      P[e] = phi;
       const double x = (double)(phi * Em.eta);
-      double f, s = 0.0;
+      double f, sl = 0.0;
       if (!(x > 0.0)) {
         f = Em.lut_frac[0];
       } else {
@@ -141,17 +180,17 @@ __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restr
           const double dl = Em.lut_log[hi] - Em.lut_log[lo];
           const double t = (lp - Em.lut_log[lo]) / dl;
           f = Em.lut_frac[lo] * (1.0 - t) + Em.lut_frac[hi] * t;
-          s = (Em.lut_frac[hi] - Em.lut_frac[lo]) / dl / (x * ln10) * (double)Em.eta;  // df/dP
+          sl = (Em.lut_frac[hi] - Em.lut_frac[lo]) / dl / (x * ln10) * (double)Em.eta;  // df/dP
         }
       }
-      slope[e] = s;
+      slope[e] = (float)sl;
       const float fr = (float)f;
       const bool inside = fr > 0.0f && fr < 1.0f;
       const float c = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
       const float ef = Em.reversal ? 1.0f - c : c;
       eff[e] = ef;
       D[e] = Em.d_max * powf(ef, Em.inv_gamma);
-      const float deff = ef > 0.0f ? Em.d_max * Em.inv_gamma * powf(ef, Em.inv_gamma - 1.0f) : 0.0f;
+      const float deff = ef > 0.0f ? Em.inv_gamma * D[e] / ef : 0.0f;  // d_max g^-1 ef^(1/g - 1)
       dDdf[e] = inside ? (Em.reversal ? -deff : deff) : 0.0f;
       float m = 0.0f;
       if (Em.has_mask) {
@@ -160,70 +199,64 @@ __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restr
       }
       M[e] = m;
     }
-    // forward scan to the transmittances
-    double dens[16], t[16];
-    for (int i = 0; i < 16; ++i) dens[i] = 0.0;
-    for (int e = 0; e < E; ++e)
-      for (int i = 0; i < 16; ++i) {
-        dens[i] += (double)D[e] * T->em[e].eps[i];
-        if (T->em[e].has_mask) dens[i] += (double)M[e] * T->em[e].mask_eps[i];
-      }
-    for (int i = 0; i < 16; ++i) t[i] = pow(10.0, -dens[i]) * T->scanner[i];
-    // reverse: out = scale * X xyz,  xyz_c = sum_i trap_i t_i cmf_c(i)
-    double gx[3];
-    for (int c = 0; c < 3; ++c) {
-      gx[c] = 0.0;
-      for (int r = 0; r < 3; ++r) gx[c] += T->xyz2ap1[r][c] * (double)gout[p * 3 + r];
-      gx[c] *= T->dmin_scale;
-    }
-    double gdens[16];
+    // forward scan to the transmittances t = scanner 10^-dens
+    float t[16];
     for (int i = 0; i < 16; ++i) {
-      const double tw = (i == 0 || i == 15) ? 10.0 : 20.0;
-      const double gt = tw * (gx[0] * T->cmf[0][i] + gx[1] * T->cmf[1][i] + gx[2] * T->cmf[2][i]);
-      gdens[i] = -ln10 * t[i] * gt;
+      float d = 0.0f;
+      for (int e = 0; e < E; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
+      t[i] = exp2f(-d * log2_10f) * scan_s[i];
     }
+    // reverse: out_r = sum_c X[r][c] xyz_c,  xyz_c = sum_i trap_i t_i cmf_c(i)
+    float gx[3];
+    for (int c = 0; c < 3; ++c)
+      gx[c] = x2a_s[0][c] * gout[p * 3] + x2a_s[1][c] * gout[p * 3 + 1] + x2a_s[2][c] * gout[p * 3 + 2];
+    float gdens[16];
+    for (int i = 0; i < 16; ++i)
+      gdens[i] = -ln10f * t[i] * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
     float gB = 0.0f;
     for (int e = 0; e < E; ++e) {
       const FilmEmulsionTables& Em = T->em[e];
-      double gD = 0.0, gM = 0.0;
+      float gD = 0.0f, gM = 0.0f;
       for (int i = 0; i < 16; ++i) {
-        gD += gdens[i] * Em.eps[i];
-        if (Em.has_mask) gM += gdens[i] * Em.mask_eps[i];
+        gD += gdens[i] * eps_s[e][i];
+        gM += gdens[i] * meps_s[e][i];
       }
-      double gdmax = 0.0;
+      float gdmax = 0.0f;
       if (Em.has_mask) {
         const float q = D[e] / Em.d_max;
         if (q > 0.0f && q < 1.0f) {
-          gD += -0.4 * gM;  // dM/dD
-          gdmax += 0.4 * gM;
+          gD += -0.4f * gM;  // dM/dD
+          gdmax += 0.4f * gM;
         } else if (q <= 0.0f) {
-          gdmax += 0.4 * gM;
+          gdmax += 0.4f * gM;
         }
       }
-      const double ef = eff[e];
-      gdmax += gD * (ef > 0.0 ? pow(ef, (double)Em.inv_gamma) : 0.0);
-      const double ggamma = ef > 0.0 ? gD * (-(double)D[e] * log(ef) * (double)Em.inv_gamma * (double)Em.inv_gamma) : 0.0;
-      const double gP = gD * (double)dDdf[e] * slope[e];
-      sbar[(size_t)e * n + p] = (float)gP;
-      part[e] += gdmax;
-      part[kFilmMaxEmulsions + e] += ggamma;
+      const float ef = eff[e];
+      gdmax += gD * (D[e] / Em.d_max);  // dD/dd_max = ef^(1/g)
+      const float ggamma = ef > 0.0f ? -gD * D[e] * logf(ef) * Em.inv_gamma * Em.inv_gamma : 0.0f;
+      const float gP = gD * dDdf[e] * slope[e];
+      sbar[(size_t)e * n + p] = gP;
+      part[e] += (double)gdmax;
+      part[kFilmMaxEmulsions + e] += (double)ggamma;
       if (B && Em.gain > 0.0f) {
-        part[2 * kFilmMaxEmulsions + e] += gP * (double)bv;
-        gB += (float)(gP * (double)Em.gain);
+        part[2 * kFilmMaxEmulsions + e] += (double)gP * (double)bv;
+        gB += gP * Em.gain;
       }
     }
     if (bbar) bbar[p] = gB;
   }
-  block_accumulate<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, acc);
+  block_partials<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, part_rows, kFitSlotDmax);
 }
 
 // dL/dC[k][e] = sum_p w_k(p) Abar_e(p)
 __global__ void __launch_bounds__(256) fit_back_expose_kernel(const float* __restrict__ rgb,
                                                               const float* __restrict__ abar, size_t n,
-                                                              FitExposeConsts k, double* __restrict__ acc) {
+                                                              FitExposeConsts k, double* __restrict__ part_rows) {
+  rgb += blockIdx.y * 3 * n;
+  abar += blockIdx.y * k.n_emulsions * n;
   double part[3 * kFilmMaxEmulsions];
   for (int i = 0; i < 3 * kFilmMaxEmulsions; ++i) part[i] = 0.0;
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+  FIT_PIXELS(p, n) {
     const double r = rgb[p * 3], g = rgb[p * 3 + 1], b = rgb[p * 3 + 2];
     double w[3];
     for (int i = 0; i < 3; ++i) {
@@ -235,57 +268,94 @@ __global__ void __launch_bounds__(256) fit_back_expose_kernel(const float* __res
       for (int i = 0; i < 3; ++i) part[i * kFilmMaxEmulsions + e] += w[i] * a;
     }
   }
-  block_accumulate<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, acc);
+  block_partials<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, part_rows, kFitSlotC);
 }
 
-// acc[0] += sum_p a(p) (b(p) + c(p))
+// sum_p a(p) (b(p) + c(p)) over one plane per image
 __global__ void __launch_bounds__(256) fit_dot_kernel(const float* __restrict__ a, const float* __restrict__ b,
-                                                      const float* __restrict__ c, size_t n, double* __restrict__ acc) {
+                                                      const float* __restrict__ c, size_t n,
+                                                      double* __restrict__ part_rows) {
+  const size_t off = blockIdx.y * n;
   double part[1] = {0.0};
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
-    part[0] += (double)a[p] * ((double)b[p] + (double)c[p]);
-  block_accumulate<1>(part, 1, acc);
+  FIT_PIXELS(p, n) part[0] += (double)a[off + p] * ((double)b[off + p] + (double)c[off + p]);
+  block_partials<1>(part, 1, part_rows, kFitSlotSigma);
 }
 
-__global__ void fill_f64_kernel(double* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0.0;
+__global__ void fill_f64_kernel(double* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// acc[slot] = sum over the partial rows: kReduceGroups groups of kFitSlots
+// threads each add every kReduceGroups-th row, then group sums are added in
+// group order (a fixed order, whatever the timing).
+constexpr int kReduceGroups = 16;
+__global__ void __launch_bounds__(kFitSlots * kReduceGroups) fit_reduce_kernel(const double* __restrict__ part_rows,
+                                                                              int rows, double* __restrict__ acc) {
+  __shared__ double grp[kReduceGroups][kFitSlots];
+  const int slot = threadIdx.x % kFitSlots, g = threadIdx.x / kFitSlots;
+  double s = 0.0;
+  for (int r = g; r < rows; r += kReduceGroups) s += part_rows[(size_t)r * kFitSlots + slot];
+  grp[g][slot] = s;
+  __syncthreads();
+  if (g == 0) {
+    double t = 0.0;
+    for (int k = 0; k < kReduceGroups; ++k) t += grp[k][slot];
+    acc[slot] = t;
+  }
 }
 
 }  // namespace
 
-hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s) {
-  hipLaunchKernelGGL(fit_expose_kernel, dim3(grid_of(n)), dim3(256), 0, s, rgb, n, k, planes);
+hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s,
+                           int images) {
+  hipLaunchKernelGGL(fit_expose_kernel, dim3(grid_of(n), images), dim3(256), 0, s, rgb, n, k, planes);
   return hipGetLastError();
 }
 
 hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, float* out, int w, int h,
-                             const float* taps, int radius, float c0, float c1, hipStream_t s) {
+                             const float* taps, int radius, float c0, float c1, hipStream_t s, const BlurPlanes& pl) {
   const size_t n = (size_t)w * h;
-  hipLaunchKernelGGL(blur_vT_kernel, dim3(grid_of(n)), dim3(256), 0, s, in, tmp, w, h, taps, radius);
-  hipLaunchKernelGGL(blur_hT_kernel, dim3(grid_of(n)), dim3(256), 0, s, tmp, orig, out, w, h, taps, radius, c0, c1);
+  hipLaunchKernelGGL(blur_vT_kernel, dim3(grid_of(n), pl.count), dim3(256), 0, s, in, tmp, w, h, taps, radius, pl);
+  hipLaunchKernelGGL(blur_hT_kernel, dim3(grid_of(n), pl.count), dim3(256), 0, s, tmp, orig, out, w, h, taps, radius,
+                     c0, c1, pl);
   return hipGetLastError();
 }
+
+int fit_blocks_per_image(size_t n, int images) {
+  const int want = std::max(1, kFitReduceRows / std::max(images, 1));
+  return std::min(want, grid_of(n));
+}
+
+int fit_partial_rows(size_t n, int images) { return fit_blocks_per_image(n, images) * images; }
 
 hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, const float* gout, size_t n,
-                              float* sbar, float* bbar, double* acc, hipStream_t s) {
-  hipLaunchKernelGGL(fit_back_scan_kernel, dim3(grid_of(n)), dim3(256), 0, s, S, B, T, gout, n, sbar, bbar, acc);
+                              float* sbar, float* bbar, double* part_rows, hipStream_t s, int images) {
+  hipLaunchKernelGGL(fit_back_scan_kernel, dim3(fit_blocks_per_image(n, images), images), dim3(256), 0, s, S, B, T,
+                     gout, n, sbar, bbar, part_rows);
   return hipGetLastError();
 }
 
-hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k, double* acc,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(fit_back_expose_kernel, dim3(grid_of(n)), dim3(256), 0, s, rgb, abar, n, k, acc);
+hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k,
+                                double* part_rows, hipStream_t s, int images) {
+  hipLaunchKernelGGL(fit_back_expose_kernel, dim3(fit_blocks_per_image(n, images), images), dim3(256), 0, s, rgb, abar,
+                     n, k, part_rows);
   return hipGetLastError();
 }
 
-hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* acc, hipStream_t s) {
-  hipLaunchKernelGGL(fit_dot_kernel, dim3(grid_of(n)), dim3(256), 0, s, a, b, c, n, acc);
+hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* part_rows, hipStream_t s,
+                        int images) {
+  hipLaunchKernelGGL(fit_dot_kernel, dim3(fit_blocks_per_image(n, images), images), dim3(256), 0, s, a, b, c, n,
+                     part_rows);
   return hipGetLastError();
 }
 
-hipError_t film_fit_zero(double* p, int n, hipStream_t s) {
-  hipLaunchKernelGGL(fill_f64_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n);
+hipError_t film_fit_zero(double* p, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+hipError_t film_fit_reduce(const double* part_rows, int rows, double* acc, hipStream_t s) {
+  hipLaunchKernelGGL(fit_reduce_kernel, dim3(1), dim3(kFitSlots * kReduceGroups), 0, s, part_rows, rows, acc);
   return hipGetLastError();
 }
 

@@ -647,31 +647,40 @@ std::vector<float> gauss_taps_dsigma(float sigma, int radius) {
   return out;
 }
 
+// Device planes of one batched forward/backward over N images (image-major:
+// image b's E planes of a quantity at base + b E n, its single planes at base + b n).
 struct FitRun {
   imops_ctx* ctx;
   const HostStock* hs;
   FitSetup f;
-  int w, h, E;
+  int w, h, E, N;
   size_t n;
-  float* S;      // E planes
-  float* Sbar;   // E planes
-  float* Bn;
+  float* S;      // E planes per image (scratch, or the caller's saved state)
+  float* Sbar;   // E planes per image
+  float* Bn;     // one plane per image (follows S in the saved state)
   float* Bbar;
-  float* tmp;
+  float* tmp;    // E planes per image
   float* u1;
   float* u2;
-  float* img;    // E planes image dye
-  float* msk;    // E planes mask dye
+  float* img;    // E planes per image: image dye
+  float* msk;    // E planes per image: mask dye
   float* taps;   // local, halation, halation d/dsigma
   int rl, rh, off_l, off_h, off_hd;
   FilmDeviceTables* T;
-  double* acc;   // 3E (d_max, gamma, gain) at [0, 8, 16], C at [24 + k*8 + e], sigma at [48]
+  double* acc;   // kFitSlots sums: 3E (d_max, gamma, gain) at [0, 8, 16], C at [24 + k*8 + e], sigma at [48]
+  double* part;  // rows x kFitSlots partial sums of the reducing kernels
+  int rows;
 };
 
-FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_params& p, size_t w, size_t h) {
+size_t fit_state_floats(int emulsions, size_t images, size_t n) { return images * n * (size_t)(emulsions + 1); }
+
+FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_params& p, size_t w, size_t h,
+                   size_t images, const float* state) {
   if (!ctx) throw Error{IMOPS_E_INVALID, "film_fit: null context"};
   if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit: unknown stock"};
   if (w < 2 || h < 2 || !(pitch > 0.0f)) throw Error{IMOPS_E_INVALID, "film_fit: invalid dimensions or pitch"};
+  if (images < 1 || images > 65535 || w > (size_t)INT32_MAX / 2 || h > (size_t)INT32_MAX / 2)
+    throw Error{IMOPS_E_INVALID, "film_fit: image count must be 1..65535"};
   FitRun R{};
   R.ctx = ctx;
   R.hs = &host_stock(stock);
@@ -681,6 +690,9 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   R.h = (int)h;
   R.E = R.hs->t.n_emulsions;
   R.n = w * h;
+  R.N = (int)images;
+  if (images * (size_t)std::max(R.E, 1) > 65535)
+    throw Error{IMOPS_E_INVALID, "film_fit: at most 65535 emulsion planes (images x emulsions) per call"};
   std::vector<float> all;
   R.off_l = R.off_h = R.off_hd = 0;
   R.rl = R.rh = 0;
@@ -700,22 +712,27 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   const int rmax = std::max(R.rl, R.rh);
   if (rmax > (int)std::min(w, h) - 1)
     throw Error{IMOPS_E_UNSUPPORTED, "film_fit: image smaller than the blur support"};
-  const size_t np = (size_t)(4 * R.E + 6);
-  ctx->scr[SCR_FILM_0].reserve(np * R.n * sizeof(float));
-  ctx->scr[SCR_FILM_2].reserve(sizeof(FilmDeviceTables) + 64 * sizeof(double));
+  // planes per image: Sbar, img, msk, tmp (E each), Bbar, u1, u2, plus S and Bn (E + 1) without a state
+  const size_t plane = R.n * images;  // one plane for every image
+  const size_t np = (size_t)(4 * R.E + 3) + (state ? 0 : (size_t)(R.E + 1));
+  ctx->scr[SCR_FILM_0].reserve(np * plane * sizeof(float));
+  R.rows = fit_partial_rows(R.n, R.N);
+  ctx->scr[SCR_FILM_2].reserve(sizeof(FilmDeviceTables) + (size_t)(1 + R.rows) * kFitSlots * sizeof(double));
   ctx->scr[SCR_FILM_3].reserve(std::max<size_t>(all.size(), 1) * sizeof(float));
   float* b = ctx->scr[SCR_FILM_0].as<float>();
-  R.S = b;
-  R.Sbar = b + (size_t)R.E * R.n;
-  R.img = b + (size_t)2 * R.E * R.n;
-  R.msk = b + (size_t)3 * R.E * R.n;
-  R.Bn = b + (size_t)4 * R.E * R.n;
-  R.Bbar = R.Bn + R.n;
-  R.tmp = R.Bbar + R.n;
-  R.u1 = R.tmp + R.n;
-  R.u2 = R.u1 + R.n;
+  R.Sbar = b;
+  R.img = b + (size_t)R.E * plane;
+  R.msk = b + (size_t)2 * R.E * plane;
+  R.tmp = b + (size_t)3 * R.E * plane;
+  R.Bbar = b + (size_t)4 * R.E * plane;
+  R.u1 = R.Bbar + plane;
+  R.u2 = R.u1 + plane;
+  float* sv = state ? const_cast<float*>(state) : R.u2 + plane;  // S then Bn (fit_state_floats)
+  R.S = sv;
+  R.Bn = sv + (size_t)R.E * plane;
   R.T = ctx->scr[SCR_FILM_2].as<FilmDeviceTables>();
   R.acc = reinterpret_cast<double*>(reinterpret_cast<char*>(ctx->scr[SCR_FILM_2].ptr) + sizeof(FilmDeviceTables));
+  R.part = R.acc + kFitSlots;
   R.taps = ctx->scr[SCR_FILM_3].as<float>();
   hipStream_t s = ctx->stream;
   IMOPS_HIP_CHECK(hipMemcpyAsync(R.taps, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, s));
@@ -724,18 +741,18 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   return R;
 }
 
-// expose + local scatter + halation bounce of one image into R.S / R.Bn
+// expose + local scatter + halation bounce of all N images into R.S / R.Bn
 void fit_front(FitRun& R, const float* rgb) {
   hipStream_t s = R.ctx->stream;
-  IMOPS_HIP_CHECK(film_fit_expose(rgb, R.n, R.f.k, R.S, s));
-  if (R.f.do_local)
-    for (int e = 0; e < R.E; ++e) {
-      float* p = R.S + (size_t)e * R.n;
-      IMOPS_HIP_CHECK(film_blur(p, R.tmp, p, p, R.w, R.h, R.taps + R.off_l, R.rl, BLUR_MIX, 0.18f, s));
-    }
+  const size_t n = R.n, En = (size_t)R.E * n;
+  IMOPS_HIP_CHECK(film_fit_expose(rgb, n, R.f.k, R.S, s, R.N));
+  if (R.f.do_local)  // every emulsion plane of every image in one launch pair
+    IMOPS_HIP_CHECK(film_blur(R.S, R.tmp, R.S, R.S, R.w, R.h, R.taps + R.off_l, R.rl, BLUR_MIX, 0.18f, s,
+                              BlurPlanes{R.N * R.E, n, n, n}));
   if (R.f.do_wide) {
-    const float* deep = R.S + (size_t)(R.E - 1) * R.n;
-    IMOPS_HIP_CHECK(film_blur(deep, R.tmp, deep, R.Bn, R.w, R.h, R.taps + R.off_h, R.rh, BLUR_STORE, 0.0f, s));
+    const float* deep = R.S + (size_t)(R.E - 1) * n;
+    IMOPS_HIP_CHECK(film_blur(deep, R.tmp, deep, R.Bn, R.w, R.h, R.taps + R.off_h, R.rh, BLUR_STORE, 0.0f, s,
+                              BlurPlanes{R.N, En, En, n}));
   }
 }
 
@@ -781,59 +798,62 @@ extern "C" int imops_film_fit_nominal(int stock, imops_film_fit_params* out) {
   });
 }
 
+extern "C" int imops_film_fit_state_floats(int stock, size_t n_images, size_t width, size_t height, size_t* floats) {
+  return guarded([&] {
+    if (!floats) throw Error{IMOPS_E_INVALID, "film_fit_state_floats: null argument"};
+    if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit_state_floats: unknown stock"};
+    *floats = fit_state_floats(host_stock(stock).t.n_emulsions, n_images, width * height);
+  });
+}
+
 extern "C" int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
                                       const float* rgb_in, float* rgb_out, size_t n_images, size_t width,
-                                      size_t height) {
+                                      size_t height, float* state) {
   return guarded([&] {
     if (!p || !rgb_in || !rgb_out) throw Error{IMOPS_E_INVALID, "film_fit_forward: null argument"};
-    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height);
+    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, state);
     hipStream_t s = ctx->stream;
     TimedRegion tr(ctx, "film_fit_forward");
-    for (size_t i = 0; i < n_images; ++i) {
-      const float* in = rgb_in + i * R.n * 3;
-      fit_front(R, in);
-      IMOPS_HIP_CHECK(film_capture_reduce(R.S, R.f.do_wide ? R.Bn : nullptr, R.n, R.T, R.img, R.msk, s));
-      IMOPS_HIP_CHECK(film_scan(R.img, R.msk, R.n, R.T, rgb_out + i * R.n * 3, s));
-    }
+    fit_front(R, rgb_in);
+    IMOPS_HIP_CHECK(film_capture_reduce(R.S, R.f.do_wide ? R.Bn : nullptr, R.n, R.T, R.img, R.msk, s, R.N));
+    IMOPS_HIP_CHECK(film_scan(R.img, R.msk, R.n, R.T, rgb_out, s, R.N));
   });
 }
 
 extern "C" int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
                                        const float* rgb_in, const float* grad_out, size_t n_images, size_t width,
-                                       size_t height, imops_film_fit_params* grad) {
+                                       size_t height, const float* state, imops_film_fit_params* grad) {
   return guarded([&] {
     if (!p || !rgb_in || !grad_out || !grad) throw Error{IMOPS_E_INVALID, "film_fit_backward: null argument"};
-    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height);
+    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, state);
     hipStream_t s = ctx->stream;
     const int E = R.E;
+    const size_t n = R.n, En = (size_t)E * n;
     {
       TimedRegion tr(ctx, "film_fit_backward");
-      IMOPS_HIP_CHECK(film_fit_zero(R.acc, 64, s));
-      for (size_t i = 0; i < n_images; ++i) {
-        const float* in = rgb_in + i * R.n * 3;
-        fit_front(R, in);
-        IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, grad_out + i * R.n * 3, R.n, R.Sbar,
-                                           R.f.do_wide ? R.Bbar : nullptr, R.acc, s));
-        if (R.f.do_wide) {
-          float* deep = R.S + (size_t)(E - 1) * R.n;
-          float* deep_bar = R.Sbar + (size_t)(E - 1) * R.n;
-          const float* k = R.taps + R.off_h;
-          const float* kd = R.taps + R.off_hd;
-          // d/dsigma of V(H(x)) = V'(H x) + V(H' x)
-          IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u1, R.w, R.h, k, kd, R.rh, s));
-          IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u2, R.w, R.h, kd, k, R.rh, s));
-          IMOPS_HIP_CHECK(film_fit_dot(R.Bbar, R.u1, R.u2, R.n, R.acc + 48, s));
-          IMOPS_HIP_CHECK(film_blur_adjoint(R.Bbar, R.tmp, deep_bar, deep_bar, R.w, R.h, k, R.rh, 1.0f, 1.0f, s));
-        }
-        if (R.f.do_local)
-          for (int e = 0; e < E; ++e) {
-            float* q = R.Sbar + (size_t)e * R.n;
-            IMOPS_HIP_CHECK(film_blur_adjoint(q, R.tmp, q, q, R.w, R.h, R.taps + R.off_l, R.rl, 1.0f - 0.18f, 0.18f, s));
-          }
-        IMOPS_HIP_CHECK(film_fit_back_expose(in, R.Sbar, R.n, R.f.k, R.acc + 24, s));
+      IMOPS_HIP_CHECK(film_fit_zero(R.part, (size_t)R.rows * kFitSlots, s));
+      if (!state) fit_front(R, rgb_in);
+      IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, grad_out, n, R.Sbar,
+                                         R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N));
+      if (R.f.do_wide) {
+        const float* deep = R.S + (size_t)(E - 1) * n;
+        float* deep_bar = R.Sbar + (size_t)(E - 1) * n;
+        const float* k = R.taps + R.off_h;
+        const float* kd = R.taps + R.off_hd;
+        // d/dsigma of V(H(x)) = V'(H x) + V(H' x)
+        IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u1, R.w, R.h, k, kd, R.rh, s, BlurPlanes{R.N, En, 0, n}));
+        IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u2, R.w, R.h, kd, k, R.rh, s, BlurPlanes{R.N, En, 0, n}));
+        IMOPS_HIP_CHECK(film_fit_dot(R.Bbar, R.u1, R.u2, n, R.part, s, R.N));
+        IMOPS_HIP_CHECK(film_blur_adjoint(R.Bbar, R.tmp, deep_bar, deep_bar, R.w, R.h, k, R.rh, 1.0f, 1.0f, s,
+                                          BlurPlanes{R.N, n, En, En}));
       }
+      if (R.f.do_local)
+        IMOPS_HIP_CHECK(film_blur_adjoint(R.Sbar, R.tmp, R.Sbar, R.Sbar, R.w, R.h, R.taps + R.off_l, R.rl,
+                                          1.0f - 0.18f, 0.18f, s, BlurPlanes{R.N * E, n, n, n}));
+      IMOPS_HIP_CHECK(film_fit_back_expose(rgb_in, R.Sbar, n, R.f.k, R.part, s, R.N));
+      IMOPS_HIP_CHECK(film_fit_reduce(R.part, R.rows, R.acc, s));
     }
-    double acc[64];
+    double acc[kFitSlots];
     IMOPS_HIP_CHECK(hipMemcpyAsync(acc, R.acc, sizeof acc, hipMemcpyDeviceToHost, s));
     IMOPS_HIP_CHECK(hipStreamSynchronize(s));
     const FitSetup& f = R.f;

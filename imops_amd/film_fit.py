@@ -62,24 +62,35 @@ class FilmModel:
         p.hal_radius = float(hal_radius)
         return p
 
-    def forward_raw(self, rgb: torch.Tensor, p: FilmFitParams) -> torch.Tensor:
+    def state_like(self, rgb: torch.Tensor) -> torch.Tensor:
+        """Device buffer for the forward state that a backward pass reuses."""
+        n, h, w, _ = rgb.shape
+        floats = C.c_size_t()
+        check(lib().imops_film_fit_state_floats(self.stock, n, w, h, C.byref(floats)))
+        return torch.empty(floats.value, dtype=torch.float32, device=rgb.device)
+
+    def forward_raw(self, rgb: torch.Tensor, p: FilmFitParams, state: torch.Tensor | None = None) -> torch.Tensor:
         _check_images(rgb)
         n, h, w, _ = rgb.shape
         out = torch.empty_like(rgb)
         torch.cuda.current_stream(rgb.device).synchronize()
         check(lib().imops_film_fit_forward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
-                                           out.data_ptr(), n, w, h))
+                                           out.data_ptr(), n, w, h, None if state is None else state.data_ptr()))
         self.ctx.synchronize()
         return out
 
-    def backward_raw(self, rgb: torch.Tensor, grad_out: torch.Tensor, p: FilmFitParams) -> FilmFitParams:
+    def backward_raw(self, rgb: torch.Tensor, grad_out: torch.Tensor, p: FilmFitParams,
+                     state: torch.Tensor | None = None) -> FilmFitParams:
+        """dL/dparams; `state` (from forward_raw on the same images and parameters) skips the
+        recomputation of the exposure and blurs."""
         _check_images(rgb)
         grad_out = grad_out.contiguous()
         n, h, w, _ = rgb.shape
         g = FilmFitParams()
         torch.cuda.current_stream(rgb.device).synchronize()
         check(lib().imops_film_fit_backward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
-                                            grad_out.data_ptr(), n, w, h, C.byref(g)))
+                                            grad_out.data_ptr(), n, w, h, None if state is None else state.data_ptr(),
+                                            C.byref(g)))
         return g
 
     def __call__(self, rgb, sens, d_max, gamma, hal_strength, hal_radius) -> torch.Tensor:
@@ -99,13 +110,16 @@ class FilmFit(torch.autograd.Function):
     def forward(fctx, rgb, sens, d_max, gamma, hal_strength, hal_radius, model):
         p = model.pack(sens, d_max, gamma, hal_strength, hal_radius)
         fctx.model, fctx.packed = model, p
-        fctx.save_for_backward(rgb, sens, d_max, gamma, hal_strength, hal_radius)
-        return model.forward_raw(rgb, p)
+        # keep the forward's exposure planes for the backward when gradients are wanted
+        state = model.state_like(rgb) if any(fctx.needs_input_grad[1:6]) else None
+        out = model.forward_raw(rgb, p, state)
+        fctx.save_for_backward(rgb, sens, d_max, gamma, hal_strength, hal_radius, state)
+        return out
 
     @staticmethod
     def backward(fctx, grad_out):
-        rgb, sens, d_max, gamma, hs, hr = fctx.saved_tensors
-        g = fctx.model.backward_raw(rgb, grad_out, fctx.packed)
+        rgb, sens, d_max, gamma, hs, hr, state = fctx.saved_tensors
+        g = fctx.model.backward_raw(rgb, grad_out, fctx.packed, state)
         E = fctx.model.n_emulsions
         as_t = lambda a, like: torch.tensor(np.asarray(a, np.float32), dtype=like.dtype, device=like.device)
         return (None, as_t(np.ctypeslib.as_array(g.sens)[:E], sens), as_t(np.ctypeslib.as_array(g.d_max)[:E], d_max),

@@ -194,7 +194,13 @@ int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_format, uin
  * and a halation PSF width multiplier (halation.rs:24-132); the scan is
  * normalised by the nominal stock's Dmin.  Images are N x H x W x 3 ACEScg
  * (absolute luminance) f32 on the device.  imops_film_fit_backward returns
- * dL/dparams for dL/dout = grad_out, summed over the images. */
+ * dL/dparams for dL/dout = grad_out, summed over the images.  Every stage runs
+ * once over all N images (one launch per stage, not per image).
+ *
+ * `state` (optional, device, imops_film_fit_state_floats floats): the forward
+ * pass saves the blurred exposure planes and the halation bounce there, and a
+ * backward pass given the same state, parameters and images reads them instead
+ * of running the exposure and blurs again.  NULL: nothing saved / recomputed. */
 typedef struct imops_film_fit_params {
   int n_emulsions;       /* emulsion layers top to bottom (the stock's count) */
   float sens[8][16];     /* sensitivity samples 400, 420, ..., 700 nm */
@@ -205,11 +211,13 @@ typedef struct imops_film_fit_params {
 } imops_film_fit_params;
 
 int imops_film_fit_nominal(int stock, imops_film_fit_params* out);
+int imops_film_fit_state_floats(int stock, size_t n_images, size_t width, size_t height, size_t* floats);
 int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
-                           const float* rgb_in, float* rgb_out, size_t n_images, size_t width, size_t height);
+                           const float* rgb_in, float* rgb_out, size_t n_images, size_t width, size_t height,
+                           float* state);
 int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
                             const float* rgb_in, const float* grad_out, size_t n_images, size_t width, size_t height,
-                            imops_film_fit_params* grad);
+                            const float* state, imops_film_fit_params* grad);
 
 /* Pipeline --------------------------------------------------------------- */
 

@@ -3,8 +3,8 @@
 The spectral exposure walk, blurs, grain noise (SplitMix64), DIR/adjacency
 algebra and the scan matrices run in the reference's operation order; the
 only per-pixel transcendentals evaluated on the device are powf (H&D
-reduction), expf (DIR), log10 (capture LUT, f64) and pow(10, -d) (scan, f64),
-whose device and glibc results may differ by an ulp.  Bar: max abs error
+reduction), expf (DIR), log10 (capture LUT, f64) and 10^-d as exp(-d ln 10) (scan, f64),
+whose device and glibc results may differ by a few f64 ulps.  Bar: max abs error
 <= 1e-5 on the scanned ACEScg (values ~0..1), and the grain noise is checked
 bit-exact through a flat-field run.
 """

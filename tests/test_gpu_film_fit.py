@@ -123,3 +123,36 @@ def test_fit_reduces_loss(model):
     res = fit(model, x, target, model.nominal(), steps=30, lr=0.01)
     print(f"loss {res['loss'][0]:.4g} -> {res['loss'][-1]:.4g}, {res['ms_per_iter']:.2f} ms/iter", flush=True)
     assert res["loss"][-1] < 0.5 * res["loss"][0]
+
+
+def _grad_vector(g, E):
+    return np.concatenate([np.ctypeslib.as_array(g.sens)[:E].ravel(), np.ctypeslib.as_array(g.d_max)[:E],
+                           np.ctypeslib.as_array(g.gamma)[:E], [g.hal_strength, g.hal_radius]]).astype(np.float64)
+
+
+def test_batch_and_saved_state_match_single_images(model):
+    """One call over N images (every stage one launch for the whole batch) gives each image
+    the bits of a call on that image alone; a backward pass that reads the forward's saved
+    state gives the gradients of one that recomputes it, and both equal the sum of the
+    single-image gradients (f64 partial sums, so to a few f32 ulps)."""
+    E = model.n_emulsions
+    imgs = _images(3, 96, 88, 9)
+    p = model.pack(**_perturbed(model.nominal(), 5, 0.5))
+    x = torch.from_numpy(imgs).cuda()
+    whole = model.forward_raw(x, p).cpu().numpy()
+    for i in range(3):
+        single = model.forward_raw(x[i:i + 1].contiguous(), p).cpu().numpy()
+        assert np.array_equal(whole[i].view(np.uint32), single[0].view(np.uint32))
+    state = model.state_like(x)
+    saved = model.forward_raw(x, p, state).cpu().numpy()
+    assert np.array_equal(saved.view(np.uint32), whole.view(np.uint32))
+    gout = torch.from_numpy(np.random.default_rng(3).normal(size=imgs.shape).astype(np.float32)).cuda()
+    g_re = _grad_vector(model.backward_raw(x, gout, p), E)
+    g_st = _grad_vector(model.backward_raw(x, gout, p, state), E)
+    g_sum = sum(_grad_vector(model.backward_raw(x[i:i + 1].contiguous(), gout[i:i + 1].contiguous(), p), E)
+                for i in range(3))
+    scale = np.abs(g_re).max()
+    print(f"recompute vs state {np.abs(g_re - g_st).max() / scale:.2e}, "
+          f"batch vs single sum {np.abs(g_re - g_sum).max() / scale:.2e}", flush=True)
+    assert np.allclose(g_st, g_re, rtol=1e-6, atol=1e-7 * scale)
+    assert np.allclose(g_sum, g_re, rtol=1e-5, atol=1e-6 * scale)
