@@ -327,15 +327,24 @@ int grid1d(size_t n) {
 // with coalesced loads.  Columns: one workgroup per 64-column x th-row tile,
 // one lane per column, each tile row a 256-byte load; th is kColH on large
 // images and halves (down to kColHMin) until the grid has kColTiles tiles, so
-// small images (the adjoint's crops) still fill the chip.
+// small images (the adjoint's crops) still fill the chip.  Each thread makes 4
+// neighbouring outputs at once and slides a register window along the staged
+// data, so one LDS read feeds 4 (columns) or 16 (rows: 16-byte quads) products;
+// every output still adds its taps one at a time in tap order.
 constexpr int BT = 256;
 constexpr int kRowSeg = 2048;
 constexpr int kColH = 128;
 constexpr int kColHMin = 16;
 constexpr int kColTiles = 1024;
 
-size_t blur_row_smem(int radius) { return (size_t)(2 * radius + 1 + kRowSeg + 2 * radius) * sizeof(float); }
-size_t blur_col_smem(int radius, int th) { return (size_t)(2 * radius + 1 + (th + 2 * radius) * 64) * sizeof(float); }
+// Row LDS: taps (padded to a quad), then the segment + halo at a 16-byte aligned
+// offset, with 8 slack floats for the last quads.  Column LDS: taps, then
+// th + 2r rows of 64 with 4 slack rows for the last window.
+__host__ __device__ inline int row_buf_off(int nt) { return (nt + 3) & ~3; }
+size_t blur_row_smem(int radius) {
+  return (size_t)(row_buf_off(2 * radius + 1) + kRowSeg + 2 * radius + 8) * sizeof(float);
+}
+size_t blur_col_smem(int radius, int th) { return (size_t)(2 * radius + 1 + (th + 2 * radius + 4) * 64) * sizeof(float); }
 
 int blur_col_tiles(int w, int h, int th) { return ((w + 63) / 64) * ((h + th - 1) / th); }
 
@@ -356,16 +365,56 @@ __global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ 
   const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * kRowSeg;
   const int nx = min(kRowSeg, w - x0);
   float* tp = sh;
-  float* buf = sh + nt;
-  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  float* buf = sh + row_buf_off(nt);
+  for (int k = tid; k < row_buf_off(nt); k += BT) tp[k] = k < nt ? taps[k] : 0.0f;
   const float* row = in + (size_t)y * w;
   for (int j = tid; j < nx + 2 * radius; j += BT) buf[j] = row[reflect_index((long long)x0 - radius + j, w)];
   __syncthreads();
   float* orow = out + (size_t)y * w + x0;
-  for (int t = tid; t < nx; t += BT) {
-    float acc = 0.0f;
-    for (int k = 0; k < nt; ++k) acc += buf[t + k] * tp[k];
-    orow[t] = acc;
+  // outputs 4u..4u+3 read buf[4u + j + k]; q holds buf[4u + k .. +3], nq the next quad
+  for (int u = tid; 4 * u < nx; u += BT) {
+    const float* bp = buf + 4 * u;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    float4 q = *reinterpret_cast<const float4*>(bp);
+    int k = 0;
+    for (; k + 4 <= nt; k += 4) {
+      const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
+      const float4 tk = *reinterpret_cast<const float4*>(tp + k);
+      a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
+      a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
+      a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
+      a0 += q.w * tk.w; a1 += nq.x * tk.w; a2 += nq.y * tk.w; a3 += nq.z * tk.w;
+      q = nq;
+    }
+    if (k < nt) {  // 1 or 3 taps left (nt is odd)
+      const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
+      const float4 tk = *reinterpret_cast<const float4*>(tp + k);
+      a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
+      if (k + 1 < nt) {
+        a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
+        a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
+      }
+    }
+    const int o = 4 * u;
+    orow[o] = a0;
+    if (o + 1 < nx) orow[o + 1] = a1;
+    if (o + 2 < nx) orow[o + 2] = a2;
+    if (o + 3 < nx) orow[o + 3] = a3;
+  }
+}
+
+// Column epilogue: plain store, scatter mix or adjacency on the unblurred plane.
+__device__ __forceinline__ float blur_epilogue(int mode, float a, const float* orig, size_t i, float acc) {
+  switch (mode) {
+    case BLUR_MIX: {  // (1-f) p + f blur(p)      halation.rs:164-181
+      const float keep = 1.0f - a;
+      return keep * orig[i] + a * acc;
+    }
+    case BLUR_ADJ: {  // d + beta (d - blur(d))   diffusion.rs:23-26
+      const float d = orig[i];
+      return d + a * (d - acc);
+    }
+    default: return acc;
   }
 }
 
@@ -389,25 +438,23 @@ __global__ void __launch_bounds__(BT) blur_col_kernel(const float* __restrict__ 
     for (int j = g; j < rows; j += BT / 64) buf[j * 64 + c] = tmp[(size_t)reflect_index((long long)ys - radius + j, h) * w + x];
   __syncthreads();
   if (x >= w) return;
-  for (int t = g; t < nyo; t += BT / 64) {
-    float acc = 0.0f;
-    for (int k = 0; k < nt; ++k) acc += buf[(t + k) * 64 + c] * tp[k];
-    const size_t i = (size_t)(ys + t) * w + x;
-    float v;
-    switch (mode) {
-      case BLUR_MIX: {  // (1-f) p + f blur(p)      halation.rs:164-181
-        const float keep = 1.0f - a;
-        v = keep * orig[i] + a * acc;
-        break;
-      }
-      case BLUR_ADJ: {  // d + beta (d - blur(d))   diffusion.rs:23-26
-        const float d = orig[i];
-        v = d + a * (d - acc);
-        break;
-      }
-      default: v = acc;
+  // rows t0..t0+3 read buf rows t0 + j + k; the window w0..w3 slides down one row per tap
+  for (int t0 = 4 * g; t0 < nyo; t0 += 4 * (BT / 64)) {
+    const float* bp = buf + t0 * 64 + c;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    float w0 = bp[0], w1 = bp[64], w2 = bp[128];
+#pragma unroll 4
+    for (int k = 0; k < nt; ++k) {
+      const float w3 = bp[(k + 3) * 64];
+      const float tk = tp[k];
+      a0 += w0 * tk; a1 += w1 * tk; a2 += w2 * tk; a3 += w3 * tk;
+      w0 = w1; w1 = w2; w2 = w3;
     }
-    out[i] = v;
+    const size_t i = (size_t)(ys + t0) * w + x;
+    out[i] = blur_epilogue(mode, a, orig, i, a0);
+    if (t0 + 1 < nyo) out[i + w] = blur_epilogue(mode, a, orig, i + w, a1);
+    if (t0 + 2 < nyo) out[i + 2 * (size_t)w] = blur_epilogue(mode, a, orig, i + 2 * (size_t)w, a2);
+    if (t0 + 3 < nyo) out[i + 3 * (size_t)w] = blur_epilogue(mode, a, orig, i + 3 * (size_t)w, a3);
   }
 }
 
@@ -436,6 +483,142 @@ void blur_passes(const float* in, float* tmp, const float* orig, float* out, int
   }
 }
 
+// Transposed blur (the adjoint of blur_passes, for the film-model gradients).
+// Along one axis of length n the forward reads source reflect(p + k - r) with
+// tap k; its transpose at q sums tap j times v[q - j + r] over the p inside
+// [0, n), plus the sources that reflect onto q: -q (q <= r) and 2n - 2 - q
+// (q >= n - 1 - r); one reflection at most, as radius <= n - 1.
+__device__ __forceinline__ float adj_edges(const float* __restrict__ v, size_t stride, int n, int q,
+                                           const float* __restrict__ taps, int r, float acc) {
+  if (q >= 1 && q <= r) {  // sources -q reflect to q
+    for (int j = 0; j <= 2 * r; ++j) {
+      const int p = -q - j + r;
+      if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
+    }
+  }
+  if (q <= n - 2 && q >= n - 1 - r) {  // sources 2n-2-q reflect to q
+    for (int j = 0; j <= 2 * r; ++j) {
+      const int p = 2 * n - 2 - q - j + r;
+      if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float adj_sum(const float* __restrict__ v, size_t stride, int n, int q,
+                                         const float* __restrict__ taps, int r) {
+  float acc = 0.0f;
+  for (int j = 0; j <= 2 * r; ++j) {
+    const int p = q - j + r;
+    if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
+  }
+  return adj_edges(v, stride, n, q, taps, r, acc);
+}
+
+// Column transpose, LDS-tiled like blur_col_kernel: the tile's rows and a halo
+// of r above and below, zero outside the plane (those sources do not exist).
+__global__ void __launch_bounds__(BT) blur_colT_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
+                                                       int h, const float* __restrict__ taps, int radius, int th,
+                                                       BlurPlanes pl) {
+  extern __shared__ float sh[];
+  in += blockIdx.y * pl.in;
+  out += blockIdx.y * (size_t)w * h;
+  const int tid = threadIdx.x, nt = 2 * radius + 1;
+  const int tiles_x = (w + 63) / 64;
+  const int xs = (blockIdx.x % tiles_x) * 64, ys = (blockIdx.x / tiles_x) * th;
+  const int nyo = min(th, h - ys);
+  const int c = tid & 63, g = tid >> 6, x = xs + c;
+  float* tp = sh;
+  float* buf = sh + nt;
+  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  const int rows = nyo + 2 * radius;
+  if (x < w)
+    for (int j = g; j < rows; j += BT / 64) {
+      const int yy = ys - radius + j;
+      buf[j * 64 + c] = (yy >= 0 && yy < h) ? in[(size_t)yy * w + x] : 0.0f;
+    }
+  __syncthreads();
+  if (x >= w) return;
+  // rows t0..t0+3 read buf rows t0 + j + 2r - k; the window slides up one row per tap
+  for (int t0 = 4 * g; t0 < nyo; t0 += 4 * (BT / 64)) {
+    const float* bp = buf + (t0 + 2 * radius) * 64 + c;
+    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float w1 = bp[64], w2 = bp[128], w3 = bp[192];
+#pragma unroll 4
+    for (int k = 0; k < nt; ++k) {
+      const float w0 = bp[-k * 64];
+      const float tk = tp[k];
+      a[0] += tk * w0; a[1] += tk * w1; a[2] += tk * w2; a[3] += tk * w3;
+      w3 = w2; w2 = w1; w1 = w0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int q = ys + t0 + j;
+      if (t0 + j >= nyo) break;
+      float acc = a[j];
+      if (q <= radius || q >= h - 1 - radius) acc = adj_edges(in + x, (size_t)w, h, q, tp, radius, acc);
+      out[(size_t)q * w + x] = acc;
+    }
+  }
+}
+
+// Row transpose, LDS-tiled like blur_row_kernel, then out = c0 orig + c1 acc
+// (orig may alias out).
+__global__ void __launch_bounds__(BT) blur_rowT_kernel(const float* __restrict__ in, const float* orig, float* out,
+                                                       int w, int h, const float* __restrict__ taps, int radius,
+                                                       float c0, float c1, BlurPlanes pl) {
+  extern __shared__ float sh[];
+  in += blockIdx.y * (size_t)w * h;
+  if (orig) orig += blockIdx.y * pl.orig;
+  out += blockIdx.y * pl.out;
+  const int tid = threadIdx.x, nt = 2 * radius + 1;
+  const int segs = (w + kRowSeg - 1) / kRowSeg;
+  const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * kRowSeg;
+  const int nx = min(kRowSeg, w - x0);
+  float* tp = sh;
+  float* buf = sh + nt;
+  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  const float* row = in + (size_t)y * w;
+  for (int j = tid; j < nx + 2 * radius; j += BT) {
+    const int xx = x0 - radius + j;
+    buf[j] = (xx >= 0 && xx < w) ? row[xx] : 0.0f;
+  }
+  __syncthreads();
+  for (int t = tid; t < nx; t += BT) {
+    float acc = 0.0f;
+    for (int k = 0; k < nt; ++k) acc += tp[k] * buf[t + 2 * radius - k];
+    const int q = x0 + t;
+    if (q <= radius || q >= w - 1 - radius) acc = adj_edges(row, 1, w, q, tp, radius, acc);
+    const size_t i = (size_t)y * w + q;
+    out[i] = (c0 != 0.0f ? c0 * orig[i] : 0.0f) + c1 * acc;
+  }
+}
+
+// One-pixel-per-thread transposes, for supports whose tiles exceed the LDS budget.
+__global__ void blur_vT_kernel(const float* __restrict__ in, float* __restrict__ out, int w, int h,
+                               const float* __restrict__ taps, int r, BlurPlanes pl) {
+  const size_t n = (size_t)w * h;
+  in += blockIdx.y * pl.in;
+  out += blockIdx.y * n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
+    out[i] = adj_sum(in + x, (size_t)w, h, y, taps, r);
+  }
+}
+
+__global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, float* out, int w, int h,
+                               const float* __restrict__ taps, int r, float c0, float c1, BlurPlanes pl) {
+  const size_t n = (size_t)w * h;
+  in += blockIdx.y * n;
+  if (orig) orig += blockIdx.y * pl.orig;
+  out += blockIdx.y * pl.out;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
+    const float acc = adj_sum(in + (size_t)y * w, 1, w, x, taps, r);
+    out[i] = (c0 != 0.0f ? c0 * orig[i] : 0.0f) + c1 * acc;
+  }
+}
+
 }  // namespace
 
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s) {
@@ -452,6 +635,27 @@ hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out,
 hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, const float* taps_h, const float* taps_v,
                       int radius, hipStream_t s, const BlurPlanes& pl) {
   blur_passes(in, tmp, nullptr, out, w, h, taps_h, taps_v, radius, (int)BLUR_STORE, 0.0f, pl, s);
+  return hipGetLastError();
+}
+
+hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, float* out, int w, int h,
+                             const float* taps, int radius, float c0, float c1, hipStream_t s, const BlurPlanes& pl) {
+  const size_t n = (size_t)w * h;
+  const int th = blur_col_height(w, h * pl.count);
+  if (blur_col_smem(radius, th) <= kBlurSmemMax) {
+    hipLaunchKernelGGL(blur_colT_kernel, dim3(blur_col_tiles(w, h, th), pl.count), dim3(BT), blur_col_smem(radius, th),
+                       s, in, tmp, w, h, taps, radius, th, pl);
+  } else {
+    hipLaunchKernelGGL(blur_vT_kernel, dim3(grid1d(n), pl.count), dim3(256), 0, s, in, tmp, w, h, taps, radius, pl);
+  }
+  if (blur_row_smem(radius) <= kBlurSmemMax) {
+    const int segs = (w + kRowSeg - 1) / kRowSeg;
+    hipLaunchKernelGGL(blur_rowT_kernel, dim3(h * segs, pl.count), dim3(BT), blur_row_smem(radius), s, tmp, orig, out,
+                       w, h, taps, radius, c0, c1, pl);
+  } else {
+    hipLaunchKernelGGL(blur_hT_kernel, dim3(grid1d(n), pl.count), dim3(256), 0, s, tmp, orig, out, w, h, taps, radius,
+                       c0, c1, pl);
+  }
   return hipGetLastError();
 }
 

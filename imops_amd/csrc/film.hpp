@@ -74,8 +74,10 @@ hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, flo
 // Rows of partial sums a reducing launch over `images` images of n pixels writes.
 int fit_partial_rows(size_t n, int images);
 // S, sbar: E planes per image; B, bbar: one plane per image; gout: 3n per image
-hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, const float* gout, size_t n,
-                              float* sbar, float* bbar, double* part_rows, hipStream_t s, int images = 1);
+// (emulsions = T->n_emulsions, 1..kFilmMaxEmulsions)
+hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions,
+                              const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, hipStream_t s,
+                              int images = 1);
 hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k,
                                 double* part_rows, hipStream_t s, int images = 1);
 // sum_p a (b + c) over one plane per image (planes of consecutive images n apart)

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "film.hpp"
 
@@ -44,73 +45,29 @@ __global__ void fit_expose_kernel(const float* __restrict__ rgb, size_t n, FitEx
   }
 }
 
-// sum over the taps whose reflect-101 source index is q (one reflection at most:
-// radius <= n - 1), i.e. the transpose of the forward blur along one axis
-__device__ __forceinline__ float adj_sum(const float* __restrict__ v, size_t stride, int n, int q,
-                                         const float* __restrict__ taps, int r) {
-  float acc = 0.0f;
-  for (int j = 0; j <= 2 * r; ++j) {
-    const int p = q - j + r;
-    if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
-  }
-  if (q >= 1 && q <= r) {  // sources -q reflect to q
-    for (int j = 0; j <= 2 * r; ++j) {
-      const int p = -q - j + r;
-      if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
-    }
-  }
-  if (q <= n - 2 && q >= n - 1 - r) {  // sources 2n-2-q reflect to q
-    for (int j = 0; j <= 2 * r; ++j) {
-      const int p = 2 * n - 2 - q - j + r;
-      if (p >= 0 && p < n) acc += taps[j] * v[(size_t)p * stride];
-    }
-  }
-  return acc;
-}
-
-__global__ void blur_vT_kernel(const float* __restrict__ in, float* __restrict__ out, int w, int h,
-                               const float* __restrict__ taps, int r, BlurPlanes pl) {
-  const size_t n = (size_t)w * h;
-  in += blockIdx.y * pl.in;
-  out += blockIdx.y * n;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
-    out[i] = adj_sum(in + x, (size_t)w, h, y, taps, r);
-  }
-}
-
-// out = c0 * orig + c1 * H^T(in)   (orig may alias out)
-__global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, float* out, int w, int h,
-                               const float* __restrict__ taps, int r, float c0, float c1, BlurPlanes pl) {
-  const size_t n = (size_t)w * h;
-  in += blockIdx.y * n;
-  if (orig) orig += blockIdx.y * pl.orig;
-  out += blockIdx.y * pl.out;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const int y = (int)(i / w), x = (int)(i - (size_t)y * w);
-    const float acc = adj_sum(in + (size_t)y * w, 1, w, x, taps, r);
-    out[i] = (c0 != 0.0f ? c0 * orig[i] : 0.0f) + c1 * acc;
-  }
-}
-
-// Block-wide f64 sums of `cnt` per-thread values, stored to this block's row of
-// the partial-sum table (kFitSlots doubles per block, block (x, y) at row
-// y * gridDim.x + x); fit_reduce_kernel adds the rows in a fixed order, so the
-// gradients do not depend on the order in which blocks finish.
-template <int kMax>
-__device__ void block_partials(double (&v)[kMax], int cnt, double* part_rows, int slot0) {
-  __shared__ double part[4][kMax];
+// Block-wide f64 sums of kGroups x kGroup per-thread values, stored to this
+// block's row of the partial-sum table (kFitSlots doubles per block, block
+// (x, y) at row y * gridDim.x + x; group g's entry e at slot0 + 8 g + e);
+// fit_reduce_kernel adds the rows in a fixed order, so the gradients do not
+// depend on the order in which blocks finish.  Sizes are compile-time so the
+// per-thread sums stay in registers.
+template <int kGroup, int kGroups>
+__device__ void block_partials(double (&v)[kGroup * kGroups], double* part_rows, int slot0) {
+  constexpr int kCnt = kGroup * kGroups;
+  __shared__ double part[4][kCnt];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < cnt; ++k) {
+#pragma unroll
+  for (int k = 0; k < kCnt; ++k) {
     double x = v[k];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) part[wave][k] = x;
   }
   __syncthreads();
-  if (threadIdx.x < cnt) {
+  if (threadIdx.x < kCnt) {
     double s = 0.0;
     for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) s += part[wv][threadIdx.x];
-    part_rows[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kFitSlots + slot0 + threadIdx.x] = s;
+    const int slot = slot0 + (threadIdx.x / kGroup) * kFilmMaxEmulsions + threadIdx.x % kGroup;
+    part_rows[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kFitSlots + slot] = s;
   }
 }
 
@@ -120,17 +77,22 @@ __device__ void block_partials(double (&v)[kMax], int cnt, double* part_rows, in
 
 // Reverse of capture + reduction + scan for one image of a batch.  In: S planes,
 // bounce B, dL/dout.  Out: dL/dS planes (into sbar), dL/dB, and f64 sums of
-// dL/dd_max, dL/dgamma, dL/dgain (3 x E).  The capture LUT lookup is the
-// forward's (f64, so the curve segment and fraction are the forward's); the
-// rest of the reverse pass is f32 arithmetic on f32 copies of the spectral
-// tables, with the per-block parameter sums in f64.
-__global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restrict__ S, const float* __restrict__ B,
+// dL/dd_max, dL/dgamma, dL/dgain (3 x E).  f32 arithmetic on f32 copies of
+// the capture LUTs and spectral tables (staged in LDS), with the per-block
+// parameter sums in f64.
+template <int kE>
+__global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __restrict__ S, const float* __restrict__ B,
                                                             const FilmDeviceTables* __restrict__ T,
                                                             const float* __restrict__ gout, size_t n,
                                                             float* __restrict__ sbar, float* __restrict__ bbar,
                                                             double* __restrict__ part_rows) {
   __shared__ float eps_s[kFilmMaxEmulsions][16], meps_s[kFilmMaxEmulsions][16], cmf_s[3][16], scan_s[16], x2a_s[3][3];
-  const int E = T->n_emulsions;
+  __shared__ float llog_s[kFilmMaxEmulsions][64], lfrac_s[kFilmMaxEmulsions][64];
+  constexpr int E = kE;  // == T->n_emulsions
+  for (int i = threadIdx.x; i < E * 64; i += blockDim.x) {
+    llog_s[i / 64][i % 64] = (float)T->em[i / 64].lut_log[i % 64];
+    lfrac_s[i / 64][i % 64] = (float)T->em[i / 64].lut_frac[i % 64];
+  }
   for (int i = threadIdx.x; i < kFilmMaxEmulsions * 16; i += blockDim.x) {
     const int e = i / 16, l = i % 16;
     eps_s[e][l] = e < E ? (float)T->em[e].eps[l] : 0.0f;
@@ -149,42 +111,41 @@ __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restr
   if (B) B += b * n;
   if (bbar) bbar += b * n;
   gout += b * 3 * n;
-  double part[3 * kFilmMaxEmulsions];
-  for (int k = 0; k < 3 * kFilmMaxEmulsions; ++k) part[k] = 0.0;
-  const double ln10 = 2.302585092994045684;
+  double part[3 * kE];
+  for (int k = 0; k < 3 * kE; ++k) part[k] = 0.0;
   const float ln10f = 2.30258512f, log2_10f = 3.32192802f;
   FIT_PIXELS(p, n) {
-    float D[kFilmMaxEmulsions], M[kFilmMaxEmulsions], dDdf[kFilmMaxEmulsions], eff[kFilmMaxEmulsions];
-    float slope[kFilmMaxEmulsions];
+    float D[kE], M[kE], dDdf[kE], eff[kE], slope[kE];
     const float bv = B ? B[p] : 0.0f;
     for (int e = 0; e < E; ++e) {
       const FilmEmulsionTables& Em = T->em[e];
       const float phi = S[(size_t)e * n + p] + (B && Em.gain > 0.0f ? Em.gain * bv : 0.0f);
-      const double x = (double)(phi * Em.eta);
-      double f, sl = 0.0;
-      if (!(x > 0.0)) {
-        f = Em.lut_frac[0];
+      const float x = phi * Em.eta;
+      const float* ll = llog_s[e];
+      const float* lf = lfrac_s[e];
+      float fr, sl = 0.0f;
+      if (!(x > 0.0f)) {
+        fr = lf[0];
       } else {
-        const double lp = log10(x);
-        if (lp <= Em.lut_log[0]) {
-          f = Em.lut_frac[0];
-        } else if (lp >= Em.lut_log[63]) {
-          f = Em.lut_frac[63];
+        const float lp = log10f(x);
+        if (lp <= ll[0]) {
+          fr = lf[0];
+        } else if (lp >= ll[63]) {
+          fr = lf[63];
         } else {
           int lo = 0, hi = 63;
           while (hi - lo > 1) {
             const int mid = (lo + hi) / 2;
-            if (Em.lut_log[mid] <= lp) lo = mid;
+            if (ll[mid] <= lp) lo = mid;
             else hi = mid;
           }
-          const double dl = Em.lut_log[hi] - Em.lut_log[lo];
-          const double t = (lp - Em.lut_log[lo]) / dl;
-          f = Em.lut_frac[lo] * (1.0 - t) + Em.lut_frac[hi] * t;
-          sl = (Em.lut_frac[hi] - Em.lut_frac[lo]) / dl / (x * ln10) * (double)Em.eta;  // df/dP
+          const float dl = ll[hi] - ll[lo];
+          const float t = (lp - ll[lo]) / dl;
+          fr = lf[lo] * (1.0f - t) + lf[hi] * t;
+          sl = (lf[hi] - lf[lo]) / dl / (x * ln10f) * Em.eta;  // df/dP
         }
       }
-      slope[e] = (float)sl;
-      const float fr = (float)f;
+      slope[e] = sl;
       const bool inside = fr > 0.0f && fr < 1.0f;
       const float c = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
       const float ef = Em.reversal ? 1.0f - c : c;
@@ -199,28 +160,30 @@ __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restr
       }
       M[e] = m;
     }
-    // forward scan to the transmittances t = scanner 10^-dens
-    float t[16];
-    for (int i = 0; i < 16; ++i) {
-      float d = 0.0f;
-      for (int e = 0; e < E; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
-      t[i] = exp2f(-d * log2_10f) * scan_s[i];
-    }
     // reverse: out_r = sum_c X[r][c] xyz_c,  xyz_c = sum_i trap_i t_i cmf_c(i)
     float gx[3];
     for (int c = 0; c < 3; ++c)
       gx[c] = x2a_s[0][c] * gout[p * 3] + x2a_s[1][c] * gout[p * 3 + 1] + x2a_s[2][c] * gout[p * 3 + 2];
-    float gdens[16];
-    for (int i = 0; i < 16; ++i)
-      gdens[i] = -ln10f * t[i] * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
+    // per wavelength: the forward transmittance t = scanner 10^-dens, then
+    // dL/ddens = -ln10 t (gx . cmf) into the per-emulsion dye and mask sums
+    float gDs[kE], gMs[kE];
+    for (int e = 0; e < kE; ++e) gDs[e] = gMs[e] = 0.0f;
+#pragma unroll 2
+    for (int i = 0; i < 16; ++i) {
+      float d = 0.0f;
+      for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
+      const float t = exp2f(-d * log2_10f) * scan_s[i];
+      const float gd = -ln10f * t * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
+      for (int e = 0; e < kE; ++e) {
+        gDs[e] += gd * eps_s[e][i];
+        gMs[e] += gd * meps_s[e][i];
+      }
+    }
     float gB = 0.0f;
     for (int e = 0; e < E; ++e) {
       const FilmEmulsionTables& Em = T->em[e];
-      float gD = 0.0f, gM = 0.0f;
-      for (int i = 0; i < 16; ++i) {
-        gD += gdens[i] * eps_s[e][i];
-        gM += gdens[i] * meps_s[e][i];
-      }
+      float gD = gDs[e];
+      const float gM = gMs[e];
       float gdmax = 0.0f;
       if (Em.has_mask) {
         const float q = D[e] / Em.d_max;
@@ -237,25 +200,26 @@ __global__ void __launch_bounds__(256) fit_back_scan_kernel(const float* __restr
       const float gP = gD * dDdf[e] * slope[e];
       sbar[(size_t)e * n + p] = gP;
       part[e] += (double)gdmax;
-      part[kFilmMaxEmulsions + e] += (double)ggamma;
+      part[kE + e] += (double)ggamma;
       if (B && Em.gain > 0.0f) {
-        part[2 * kFilmMaxEmulsions + e] += (double)gP * (double)bv;
+        part[2 * kE + e] += (double)gP * (double)bv;
         gB += gP * Em.gain;
       }
     }
     if (bbar) bbar[p] = gB;
   }
-  block_partials<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, part_rows, kFitSlotDmax);
+  block_partials<kE, 3>(part, part_rows, kFitSlotDmax);
 }
 
 // dL/dC[k][e] = sum_p w_k(p) Abar_e(p)
+template <int kE>
 __global__ void __launch_bounds__(256) fit_back_expose_kernel(const float* __restrict__ rgb,
                                                               const float* __restrict__ abar, size_t n,
                                                               FitExposeConsts k, double* __restrict__ part_rows) {
   rgb += blockIdx.y * 3 * n;
-  abar += blockIdx.y * k.n_emulsions * n;
-  double part[3 * kFilmMaxEmulsions];
-  for (int i = 0; i < 3 * kFilmMaxEmulsions; ++i) part[i] = 0.0;
+  abar += blockIdx.y * kE * n;
+  double part[3 * kE];
+  for (int i = 0; i < 3 * kE; ++i) part[i] = 0.0;
   FIT_PIXELS(p, n) {
     const double r = rgb[p * 3], g = rgb[p * 3 + 1], b = rgb[p * 3 + 2];
     double w[3];
@@ -263,12 +227,12 @@ __global__ void __launch_bounds__(256) fit_back_expose_kernel(const float* __res
       const double v = k.minv[i][0] * r + k.minv[i][1] * g + k.minv[i][2] * b;
       w[i] = v < 0.0 ? 0.0 : v;
     }
-    for (int e = 0; e < k.n_emulsions; ++e) {
+    for (int e = 0; e < kE; ++e) {
       const double a = abar[(size_t)e * n + p];
-      for (int i = 0; i < 3; ++i) part[i * kFilmMaxEmulsions + e] += w[i] * a;
+      for (int i = 0; i < 3; ++i) part[i * kE + e] += w[i] * a;
     }
   }
-  block_partials<3 * kFilmMaxEmulsions>(part, 3 * kFilmMaxEmulsions, part_rows, kFitSlotC);
+  block_partials<kE, 3>(part, part_rows, kFitSlotC);
 }
 
 // sum_p a(p) (b(p) + c(p)) over one plane per image
@@ -278,7 +242,7 @@ __global__ void __launch_bounds__(256) fit_dot_kernel(const float* __restrict__ 
   const size_t off = blockIdx.y * n;
   double part[1] = {0.0};
   FIT_PIXELS(p, n) part[0] += (double)a[off + p] * ((double)b[off + p] + (double)c[off + p]);
-  block_partials<1>(part, 1, part_rows, kFitSlotSigma);
+  block_partials<1, 1>(part, part_rows, kFitSlotSigma);
 }
 
 __global__ void fill_f64_kernel(double* p, size_t n) {
@@ -312,15 +276,6 @@ hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k,
   return hipGetLastError();
 }
 
-hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, float* out, int w, int h,
-                             const float* taps, int radius, float c0, float c1, hipStream_t s, const BlurPlanes& pl) {
-  const size_t n = (size_t)w * h;
-  hipLaunchKernelGGL(blur_vT_kernel, dim3(grid_of(n), pl.count), dim3(256), 0, s, in, tmp, w, h, taps, radius, pl);
-  hipLaunchKernelGGL(blur_hT_kernel, dim3(grid_of(n), pl.count), dim3(256), 0, s, tmp, orig, out, w, h, taps, radius,
-                     c0, c1, pl);
-  return hipGetLastError();
-}
-
 int fit_blocks_per_image(size_t n, int images) {
   const int want = std::max(1, kFitReduceRows / std::max(images, 1));
   return std::min(want, grid_of(n));
@@ -328,18 +283,39 @@ int fit_blocks_per_image(size_t n, int images) {
 
 int fit_partial_rows(size_t n, int images) { return fit_blocks_per_image(n, images) * images; }
 
-hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, const float* gout, size_t n,
-                              float* sbar, float* bbar, double* part_rows, hipStream_t s, int images) {
-  hipLaunchKernelGGL(fit_back_scan_kernel, dim3(fit_blocks_per_image(n, images), images), dim3(256), 0, s, S, B, T,
-                     gout, n, sbar, bbar, part_rows);
+// Calls f(std::integral_constant<int, E>{}) for the emulsion count E in 1..kFilmMaxEmulsions.
+template <class F>
+hipError_t with_emulsions(int emulsions, F&& f) {
+  switch (emulsions) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    case 6: f(std::integral_constant<int, 6>{}); break;
+    case 7: f(std::integral_constant<int, 7>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions,
+                              const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, hipStream_t s,
+                              int images) {
+  const dim3 grid(fit_blocks_per_image(n, images), images);
+  return with_emulsions(emulsions, [&](auto e) {
+    hipLaunchKernelGGL(fit_back_scan_kernel<decltype(e)::value>, grid, dim3(256), 0, s, S, B, T, gout, n, sbar, bbar,
+                       part_rows);
+  });
 }
 
 hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k,
                                 double* part_rows, hipStream_t s, int images) {
-  hipLaunchKernelGGL(fit_back_expose_kernel, dim3(fit_blocks_per_image(n, images), images), dim3(256), 0, s, rgb, abar,
-                     n, k, part_rows);
-  return hipGetLastError();
+  const dim3 grid(fit_blocks_per_image(n, images), images);
+  return with_emulsions(k.n_emulsions, [&](auto e) {
+    hipLaunchKernelGGL(fit_back_expose_kernel<decltype(e)::value>, grid, dim3(256), 0, s, rgb, abar, n, k, part_rows);
+  });
 }
 
 hipError_t film_fit_dot(const float* a, const float* b, const float* c, size_t n, double* part_rows, hipStream_t s,

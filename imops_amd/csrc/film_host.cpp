@@ -674,6 +674,32 @@ struct FitRun {
 
 size_t fit_state_floats(int emulsions, size_t images, size_t n) { return images * n * (size_t)(emulsions + 1); }
 
+// The film model's per-call tables on the device (its own buffers, no other
+// stage writes them), uploaded from pinned staging so the upload does not wait
+// for the stream; an upload identical to the last one is skipped (a backward
+// pass after its forward).  `done` marks the end of the last copy out of the
+// staging, which must pass before the host rewrites it.
+struct FitUpload : StageCache {
+  DevBuf tables;  // FilmDeviceTables, then kFitSlots sums, then the partial-sum rows
+  DevBuf taps;
+  void* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  hipEvent_t done = nullptr;
+  std::vector<unsigned char> key;  // what the device tables were built from
+  ~FitUpload() override {
+    if (done) (void)hipEventDestroy(done);
+    if (pinned) (void)hipHostFree(pinned);
+    tables.release();
+    taps.release();
+  }
+};
+
+template <class T>
+void append_bytes(std::vector<unsigned char>& v, const T& x) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&x);
+  v.insert(v.end(), p, p + sizeof(T));
+}
+
 FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_params& p, size_t w, size_t h,
                    size_t images, const float* state) {
   if (!ctx) throw Error{IMOPS_E_INVALID, "film_fit: null context"};
@@ -717,8 +743,19 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   const size_t np = (size_t)(4 * R.E + 3) + (state ? 0 : (size_t)(R.E + 1));
   ctx->scr[SCR_FILM_0].reserve(np * plane * sizeof(float));
   R.rows = fit_partial_rows(R.n, R.N);
-  ctx->scr[SCR_FILM_2].reserve(sizeof(FilmDeviceTables) + (size_t)(1 + R.rows) * kFitSlots * sizeof(double));
-  ctx->scr[SCR_FILM_3].reserve(std::max<size_t>(all.size(), 1) * sizeof(float));
+  if (!ctx->fit_cache) ctx->fit_cache.reset(new FitUpload());
+  FitUpload& U = static_cast<FitUpload&>(*ctx->fit_cache);
+  const size_t tab_bytes = sizeof(FilmDeviceTables) + (size_t)(1 + R.rows) * kFitSlots * sizeof(double);
+  const size_t taps_bytes = std::max<size_t>(all.size(), 1) * sizeof(float);
+  std::vector<unsigned char> key;
+  append_bytes(key, stock);
+  append_bytes(key, pitch);
+  append_bytes(key, p);
+  const void* old_tables = U.tables.ptr;
+  const void* old_taps = U.taps.ptr;
+  U.tables.reserve(tab_bytes);
+  U.taps.reserve(taps_bytes);
+  const bool fresh = key == U.key && U.tables.ptr == old_tables && U.taps.ptr == old_taps;
   float* b = ctx->scr[SCR_FILM_0].as<float>();
   R.Sbar = b;
   R.img = b + (size_t)R.E * plane;
@@ -730,14 +767,32 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   float* sv = state ? const_cast<float*>(state) : R.u2 + plane;  // S then Bn (fit_state_floats)
   R.S = sv;
   R.Bn = sv + (size_t)R.E * plane;
-  R.T = ctx->scr[SCR_FILM_2].as<FilmDeviceTables>();
-  R.acc = reinterpret_cast<double*>(reinterpret_cast<char*>(ctx->scr[SCR_FILM_2].ptr) + sizeof(FilmDeviceTables));
+  R.T = U.tables.as<FilmDeviceTables>();
+  R.acc = reinterpret_cast<double*>(reinterpret_cast<char*>(U.tables.ptr) + sizeof(FilmDeviceTables));
   R.part = R.acc + kFitSlots;
-  R.taps = ctx->scr[SCR_FILM_3].as<float>();
-  hipStream_t s = ctx->stream;
-  IMOPS_HIP_CHECK(hipMemcpyAsync(R.taps, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipMemcpyAsync(R.T, &R.f.T, sizeof(FilmDeviceTables), hipMemcpyHostToDevice, s));
-  IMOPS_HIP_CHECK(hipStreamSynchronize(s));  // the host sources are locals
+  R.taps = U.taps.as<float>();
+  if (!fresh) {
+    hipStream_t s = ctx->stream;
+    const size_t need = sizeof(FilmDeviceTables) + all.size() * sizeof(float);
+    if (U.done) IMOPS_HIP_CHECK(hipEventSynchronize(U.done));  // the staging is free again
+    if (need > U.pinned_bytes) {
+      if (U.pinned) IMOPS_HIP_CHECK(hipHostFree(U.pinned));
+      U.pinned = nullptr;
+      U.pinned_bytes = 0;
+      IMOPS_HIP_CHECK(hipHostMalloc(&U.pinned, need, hipHostMallocDefault));
+      U.pinned_bytes = need;
+    }
+    if (!U.done) IMOPS_HIP_CHECK(hipEventCreateWithFlags(&U.done, hipEventDisableTiming));
+    char* st = static_cast<char*>(U.pinned);
+    std::memcpy(st, &R.f.T, sizeof(FilmDeviceTables));
+    if (!all.empty()) std::memcpy(st + sizeof(FilmDeviceTables), all.data(), all.size() * sizeof(float));
+    IMOPS_HIP_CHECK(hipMemcpyAsync(R.T, st, sizeof(FilmDeviceTables), hipMemcpyHostToDevice, s));
+    if (!all.empty())
+      IMOPS_HIP_CHECK(hipMemcpyAsync(R.taps, st + sizeof(FilmDeviceTables), all.size() * sizeof(float),
+                                     hipMemcpyHostToDevice, s));
+    IMOPS_HIP_CHECK(hipEventRecord(U.done, s));
+    U.key = std::move(key);
+  }
   return R;
 }
 
@@ -833,7 +888,7 @@ extern "C" int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um
       TimedRegion tr(ctx, "film_fit_backward");
       IMOPS_HIP_CHECK(film_fit_zero(R.part, (size_t)R.rows * kFitSlots, s));
       if (!state) fit_front(R, rgb_in);
-      IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, grad_out, n, R.Sbar,
+      IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, E, grad_out, n, R.Sbar,
                                          R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N));
       if (R.f.do_wide) {
         const float* deep = R.S + (size_t)(E - 1) * n;

@@ -434,6 +434,7 @@ int imops_ctx_destroy(imops_ctx* ctx) {
     for (auto& b : ctx->scr) b.release();
     ctx->bm3d_frags.release();
     ctx->film_cache.reset();
+    ctx->fit_cache.reset();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
   });

@@ -77,6 +77,7 @@ struct imops_ctx {
   imops::DevBuf scr[SCR_COUNT];
   imops::KernelTimer timer;
   std::unique_ptr<imops::StageCache> film_cache;  // prepared film runs by shape
+  std::unique_ptr<imops::StageCache> fit_cache;   // film-model tables on the device and their pinned staging
   imops::DevBuf bm3d_frags;                       // FP16 DCT fragments of the mixed BM3D filter
   bool bm3d_tables_ready = false;                 // constant tables of the FP32 BM3D filter uploaded
   // Set while the stage code runs under stream capture: stages then only

@@ -34,6 +34,22 @@ class FilmModel:
         p = FilmFitParams()
         check(lib().imops_film_fit_nominal(self.stock, C.byref(p)))
         self.n_emulsions = p.n_emulsions
+        self._lib_stream = None
+
+    def _after_torch(self, device) -> None:
+        """Order the library's stream after the work queued on torch's current stream
+        (an event wait on the device, no host synchronisation)."""
+        if self._lib_stream is None:
+            self._lib_stream = torch.cuda.ExternalStream(self.ctx.stream, device=device)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(device))
+        self._lib_stream.wait_event(ev)
+
+    def _before_torch(self, device) -> None:
+        """Order torch's current stream after the library's queued work."""
+        ev = torch.cuda.Event()
+        ev.record(self._lib_stream)
+        torch.cuda.current_stream(device).wait_event(ev)
 
     def nominal(self) -> dict:
         p = FilmFitParams()
@@ -73,10 +89,10 @@ class FilmModel:
         _check_images(rgb)
         n, h, w, _ = rgb.shape
         out = torch.empty_like(rgb)
-        torch.cuda.current_stream(rgb.device).synchronize()
+        self._after_torch(rgb.device)
         check(lib().imops_film_fit_forward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
                                            out.data_ptr(), n, w, h, None if state is None else state.data_ptr()))
-        self.ctx.synchronize()
+        self._before_torch(rgb.device)
         return out
 
     def backward_raw(self, rgb: torch.Tensor, grad_out: torch.Tensor, p: FilmFitParams,
@@ -87,10 +103,10 @@ class FilmModel:
         grad_out = grad_out.contiguous()
         n, h, w, _ = rgb.shape
         g = FilmFitParams()
-        torch.cuda.current_stream(rgb.device).synchronize()
+        self._after_torch(rgb.device)
         check(lib().imops_film_fit_backward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
                                             grad_out.data_ptr(), n, w, h, None if state is None else state.data_ptr(),
-                                            C.byref(g)))
+                                            C.byref(g)))  # returns after the device work (it reads the sums back)
         return g
 
     def __call__(self, rgb, sens, d_max, gamma, hal_strength, hal_radius) -> torch.Tensor:
@@ -148,8 +164,9 @@ def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, s
             params["d_max"].clamp_(min=0.05)
             params["hal_strength"].clamp_(min=0.0)
             params["hal_radius"].clamp_(min=0.05)
-        history.append(loss.item())
+        history.append(loss.detach())  # read back once at the end, not every step
         if log_every and it % log_every == 0:
-            print(f"[fit] step {it} loss {history[-1]:.6g}", flush=True)
+            print(f"[fit] step {it} loss {float(history[-1]):.6g}", flush=True)
+    history = torch.stack(history).cpu().tolist() if history else []
     return {"params": {k: v.detach() for k, v in params.items()}, "loss": history,
             "ms_per_iter": 1e3 * float(np.median(times))}
