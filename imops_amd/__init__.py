@@ -237,6 +237,11 @@ class Image:
         check(lib().imops_bm3d_ex(self.ctx.handle, self._h, C.byref(p)))
         return self
 
+    def chroma_bm3d(self, intensity: float = 0.0) -> "Image":
+        """Chroma-only denoise (Image::chroma_bm3d, pixel.rs:122-126; bm3d.rs:705-759)."""
+        check(lib().imops_chroma_bm3d(self.ctx.handle, self._h, float(intensity)))
+        return self
+
     def film(self, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
              output: str = "NegativeLinear") -> "Image":
         """Film simulation (film/mod.rs:77-129); the image must be absolute-luminance ACEScg."""

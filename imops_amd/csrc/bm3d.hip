@@ -703,6 +703,86 @@ __global__ void bm3d_merge_kernel(const float* __restrict__ yd, const float* __r
   }
 }
 
+// ----------------------------------------------------- chroma-only denoise
+// dct_chroma_denoise (bm3d.rs:719-759): every 8x8 patch at a 6-pixel step
+// (x < W-8, y < H-8) is DCT'd, hard-thresholded (all but coefficient 0),
+// inverse-DCT'd and aggregated with the Kaiser window in i64 fixed point.
+// Gather form, no atomics: a workgroup owns a kChromaTile-pixel square and
+// its 256 threads each filter one of the 16 x 16 patches touching it (those
+// of the first row and column also touch the previous tiles, which filter
+// them again); the Kaiser-weighted samples wait in LDS, then every pixel of
+// the tile adds the fixed-point terms of the one to four patches over it.
+// Integer sums do not depend on order, so the totals are the reference's.
+constexpr int kChromaSteps = 15;                   // patch steps per tile side
+constexpr int kChromaTile = kChromaSteps * 6;      // 90 pixels
+constexpr int kChromaPos = kChromaSteps + 1;       // patch positions per side (one before the tile)
+
+__global__ void __launch_bounds__(kChromaPos * kChromaPos) chroma_dct_kernel(
+    const float* __restrict__ in_cb, const float* __restrict__ in_cr, float* __restrict__ out_cb,
+    float* __restrict__ out_cr, int w, int h, float th) {
+  __shared__ float prod[kChromaPos * kChromaPos][65];  // odd pitch: lane t's row starts in bank t
+  __shared__ int valid[kChromaPos * kChromaPos];
+  const float* __restrict__ in = blockIdx.y == 0 ? in_cb : in_cr;
+  float* __restrict__ out = blockIdx.y == 0 ? out_cb : out_cr;
+  const int tiles_x = (w + kChromaTile - 1) / kChromaTile;
+  const int X0 = (blockIdx.x % tiles_x) * kChromaTile, Y0 = (blockIdx.x / tiles_x) * kChromaTile;
+  const int t = threadIdx.x;
+  {
+    const int px = X0 - 6 + 6 * (t % kChromaPos), py = Y0 - 6 + 6 * (t / kChromaPos);
+    const bool ok = px >= 0 && py >= 0 && px < w - 8 && py < h - 8;
+    valid[t] = ok;
+    if (ok) {
+      float v[64];
+#pragma unroll
+      for (int dy = 0; dy < 8; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) v[dy * 8 + dx] = in[(size_t)(py + dy) * w + px + dx];
+      dct2_regs(v, bm3d_table(0));
+#pragma unroll
+      for (int i = 1; i < 64; ++i)  // hard_threshold (bm3d.rs:240-255) keeps coefficient 0
+        if (fabsf(v[i]) < th) v[i] = 0.0f;
+      idct2_regs(v, bm3d_table(1));
+      const ctab_ptr kaiser = bm3d_table(2);
+#pragma unroll
+      for (int i = 0; i < 64; ++i) prod[t][i] = v[i] * kaiser[i];
+    }
+  }
+  __syncthreads();
+  const ctab_ptr kaiser = bm3d_table(2);
+  for (int q = t; q < kChromaTile * kChromaTile; q += kChromaPos * kChromaPos) {
+    const int lx = q % kChromaTile, ly = q / kChromaTile;
+    const int x = X0 + lx, y = Y0 + ly;
+    if (x >= w || y >= h) continue;
+    long long num = 0, den = 0;
+    // patch positions i with X0 - 6 + 6 i <= x <= X0 + 1 + 6 i (likewise j for rows)
+    for (int j = (ly + 4) / 6; j <= (ly + 6) / 6; ++j)
+      for (int i = (lx + 4) / 6; i <= (lx + 6) / 6; ++i) {
+        const int pid = j * kChromaPos + i;
+        if (!valid[pid]) continue;
+        const int e = (ly + 6 - 6 * j) * 8 + (lx + 6 - 6 * i);
+        num += to_fixed(prod[pid][e]);
+        den += to_fixed_unit(kaiser[e]);
+      }
+    const size_t o = (size_t)y * w + x;
+    const float nn = (float)num / 100000.0f, dd = (float)den / 100000.0f;
+    out[o] = dd > 1e-6f ? nn / dd : in[o];
+  }
+}
+
+// ycbcr_to_rgb_inplace + merge_channels_inplace (bm3d.rs:83-96, 629-635)
+__global__ void ycbcr_merge_kernel(const float* __restrict__ y, const float* __restrict__ cb,
+                                   const float* __restrict__ cr, float* __restrict__ rgb, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float yv = y[i], cbv = cb[i] - 128.0f, crv = cr[i] - 128.0f;
+    const float r = yv + 1.402f * crv;
+    const float g = yv - 0.344136f * cbv - 0.714136f * crv;
+    const float b = yv + 1.772f * cbv;
+    rgb[i * 3] = fmaxf(r, 0.0f) / 255.0f;
+    rgb[i * 3 + 1] = fmaxf(g, 0.0f) / 255.0f;
+    rgb[i * 3 + 2] = fmaxf(b, 0.0f) / 255.0f;
+  }
+}
+
 int grid1d(size_t n) {
   size_t g = (n + 255) / 256;
   return (int)(g > 65536 ? 65536 : (g ? g : 1));
@@ -776,6 +856,19 @@ hipError_t bm3d_blur_launch(const float* in, float* tmp, float* out, int w, int 
   const size_t n = (size_t)w * h;
   hipLaunchKernelGGL(blur1_kernel<false>, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps);
   hipLaunchKernelGGL(blur1_kernel<true>, dim3(grid1d(n)), dim3(256), 0, s, tmp, out, w, h, taps);
+  return hipGetLastError();
+}
+
+hipError_t chroma_dct_launch(const float* cb, const float* cr, float* out_cb, float* out_cr, int w, int h, float th,
+                            hipStream_t s) {
+  const int tiles = ((w + kChromaTile - 1) / kChromaTile) * ((h + kChromaTile - 1) / kChromaTile);
+  hipLaunchKernelGGL(chroma_dct_kernel, dim3(tiles, 2), dim3(kChromaPos * kChromaPos), 0, s, cb, cr, out_cb, out_cr, w,
+                     h, th);
+  return hipGetLastError();
+}
+
+hipError_t ycbcr_merge_launch(const float* y, const float* cb, const float* cr, float* rgb, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(ycbcr_merge_kernel, dim3(grid1d(n)), dim3(256), 0, s, y, cb, cr, rgb, n);
   return hipGetLastError();
 }
 

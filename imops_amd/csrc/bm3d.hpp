@@ -83,6 +83,11 @@ hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned lo
 hipError_t fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t bm3d_blur_launch(const float* in, float* tmp, float* out, int w, int h, const Bm3dBlurTaps& taps,
                             hipStream_t s);
+// chroma-only denoise of the Cb and Cr planes (dct_chroma_denoise, bm3d.rs:719-759); th is the hard threshold
+hipError_t chroma_dct_launch(const float* cb, const float* cr, float* out_cb, float* out_cr, int w, int h, float th,
+                            hipStream_t s);
+// YCbCr (0..255, +128 offsets) -> RGB / 255 clamped at 0 (bm3d.rs:83-96, 629-635)
+hipError_t ycbcr_merge_launch(const float* y, const float* cb, const float* cr, float* rgb, size_t n, hipStream_t s);
 hipError_t bm3d_merge_launch(const float* yd, const float* cb, const float* cr, const float* yorig, const float* yblur,
                              float detail, float* rgb, size_t n, hipStream_t s);
 

@@ -100,6 +100,53 @@ Bm3dBlurTaps blur_taps(float sigma) {
 
 }  // namespace
 
+namespace {
+
+const HostTables& host_tables() {
+  static const HostTables t;
+  return t;
+}
+
+// The DCT/IDCT/Kaiser constant tables and the FP16 DCT fragments, uploaded once
+// per context (the host sources are static).
+void ensure_bm3d_tables(imops_ctx* ctx) {
+  if (ctx->bm3d_tables_ready) return;
+  if (ctx->capturing) throw Error{IMOPS_E_STATE, "BM3D: tables not uploaded before stream capture"};
+  const HostTables& tables = host_tables();
+  static const std::vector<uint16_t> frags = [&] {
+    std::vector<uint16_t> v(kBm3dMfmaFragBytes / 2);
+    bm3d_filter_mfma_fragments(tables.dct, v.data());
+    return v;
+  }();
+  hipStream_t s = ctx->stream;
+  ctx->bm3d_frags.reserve(kBm3dMfmaFragBytes);
+  IMOPS_HIP_CHECK(bm3d_upload_tables(tables.dct, tables.idct, tables.kaiser, s));
+  IMOPS_HIP_CHECK(hipMemcpyAsync(ctx->bm3d_frags.ptr, frags.data(), kBm3dMfmaFragBytes, hipMemcpyHostToDevice, s));
+  IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+  ctx->bm3d_tables_ready = true;
+}
+
+}  // namespace
+
+// chroma_bm3d (bm3d.rs:705-717): Y is kept, Cb and Cr are denoised patch-wise.
+void stage_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity) {
+  require_rgb(img, "ChromaDenoise");
+  TimedRegion stage_timer(ctx, "chroma_bm3d");
+  const int w = (int)img->width, h = (int)img->height;
+  const size_t n = (size_t)w * h;
+  hipStream_t s = ctx->stream;
+  const float ic = std::min(std::max(intensity, 0.001f), 1.0f);
+  const float sigma = ic * 80.0f;
+  const float threshold = (2.0f + (ic * 2.5f)) * sigma * 1.8f;
+  ensure_bm3d_tables(ctx);
+  ctx->scr[SCR_BM3D_0].reserve(5 * n * sizeof(float));
+  float* f = ctx->scr[SCR_BM3D_0].as<float>();
+  float *y = f, *cb = f + n, *cr = f + 2 * n, *cb2 = f + 3 * n, *cr2 = f + 4 * n;
+  IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, y, cb, cr, s));
+  IMOPS_HIP_CHECK(chroma_dct_launch(cb, cr, cb2, cr2, w, h, threshold, s));
+  IMOPS_HIP_CHECK(ycbcr_merge_launch(y, cb2, cr2, img->rgb.as<float>(), n, s));
+}
+
 void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   require_rgb(img, "BM3D");
   TimedRegion stage_timer(ctx, "bm3d");
@@ -133,22 +180,9 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   // region in LDS; larger geometries use the plain matcher.
   Bm3dMfmaArgs mf = bm3d_mfma_layout(w, h, p.stride, p.window);
   const bool use_mfma = p.precision != IMOPS_BM3D_FP32 && nrefs > 0 && mf.smem <= 160 * 1024 && mf.max_tiles_ok;
-  static const HostTables tables;
+  static const HostTables& tables = host_tables();
   const bool mixed_filter = p.precision == IMOPS_BM3D_MIXED;
-  // constant tables, uploaded once per context (the host sources are static)
-  if (!ctx->bm3d_tables_ready) {
-    if (ctx->capturing) throw Error{IMOPS_E_STATE, "BM3D: tables not uploaded before stream capture"};
-    static const std::vector<uint16_t> frags = [] {
-      std::vector<uint16_t> v(kBm3dMfmaFragBytes / 2);
-      bm3d_filter_mfma_fragments(tables.dct, v.data());
-      return v;
-    }();
-    ctx->bm3d_frags.reserve(kBm3dMfmaFragBytes);
-    IMOPS_HIP_CHECK(bm3d_upload_tables(tables.dct, tables.idct, tables.kaiser, s));
-    IMOPS_HIP_CHECK(hipMemcpyAsync(ctx->bm3d_frags.ptr, frags.data(), kBm3dMfmaFragBytes, hipMemcpyHostToDevice, s));
-    IMOPS_HIP_CHECK(hipStreamSynchronize(s));
-    ctx->bm3d_tables_ready = true;
-  }
+  ensure_bm3d_tables(ctx);
   const void* filter_frags = ctx->bm3d_frags.ptr;
 
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
@@ -313,6 +347,13 @@ int imops_bm3d(imops_ctx* ctx, imops_image* img, float intensity) {
   return guarded([&] {
     if (!ctx || !img) throw Error{IMOPS_E_INVALID, "bm3d: null argument"};
     stage_bm3d(ctx, img, Bm3dStageParams::from_intensity(intensity));
+  });
+}
+
+int imops_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "chroma_bm3d: null argument"};
+    stage_chroma_bm3d(ctx, img, intensity);
   });
 }
 

@@ -29,6 +29,8 @@ struct Bm3dStageParams {
 Bm3dStageParams bm3d_stage_params_from_table(double intensity, double sigma_norm, double lambda, int stride,
                                              int window, int max_group);
 void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p);
+// chroma_bm3d (pichromatic/src/bm3d.rs:705-759)
+void stage_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
 
 struct FilmStageParams {
   int stock = 1;    // StockId order of film/stock/mod.rs (0 BwStub, 1 ColorNeg200, ...)

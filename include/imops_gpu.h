@@ -174,6 +174,14 @@ int imops_bm3d_default_params(float intensity, imops_bm3d_params* out);
 int imops_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
 int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* params);
 
+/* Chroma-only denoise (Image::chroma_bm3d, pichromatic/src/pixel.rs:122-126;
+ * chroma_bm3d + dct_chroma_denoise, bm3d.rs:705-759; stage ChromaDenoise,
+ * modules.rs:688-722): Y kept, every 8x8 Cb/Cr patch at a 6-pixel step DCT'd,
+ * hard-thresholded at (2 + 2.5 i) 80 i 1.8 (i = intensity clamped to
+ * [0.001, 1]), inverse-DCT'd and aggregated with the Kaiser window in i64
+ * fixed point; bit-exact with the reference. */
+int imops_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
+
 /* Film simulation (film::process, pichromatic/src/film/mod.rs:77-129) on an
  * absolute-luminance ACEScg image; output ACEScg.  Module<Film>
  * (modules.rs:841-876).  stock: 0 BwStub, 1 ColorNeg200, 2 Portra400,
