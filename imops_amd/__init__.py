@@ -242,6 +242,11 @@ class Image:
         check(lib().imops_chroma_bm3d(self.ctx.handle, self._h, float(intensity)))
         return self
 
+    def chroma_denoise(self, radius: int = 4, epsilon: float = 0.01) -> "Image":
+        """Luma-guided chroma denoise (Image::chroma_denoise, pixel.rs:128-137; chroma_denoise.rs:14-152)."""
+        check(lib().imops_luma_chroma_denoise(self.ctx.handle, self._h, int(radius), float(epsilon)))
+        return self
+
     def film(self, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
              output: str = "NegativeLinear") -> "Image":
         """Film simulation (film/mod.rs:77-129); the image must be absolute-luminance ACEScg."""

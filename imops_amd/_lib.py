@@ -77,6 +77,7 @@ SIGNATURES = {
     "imops_bm3d": (_i, [_vp, _vp, _f]),
     "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
     "imops_chroma_bm3d": (_i, [_vp, _vp, _f]),
+    "imops_luma_chroma_denoise": (_i, [_vp, _vp, _i, _f]),
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
     "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
     "imops_film_fit_state_floats": (_i, [_i, _sz, _sz, _sz, C.POINTER(_sz)]),

@@ -287,14 +287,15 @@ std::string trim(const std::string& s) {
 }  // namespace
 
 enum StageKind {
-  K_DEMOSAIC, K_CFACOEFFS, K_EXP, K_GAMMA, K_CST, K_BM3D, K_CHROMA_BM3D, K_FILM, K_OTHER
+  K_DEMOSAIC, K_CFACOEFFS, K_EXP, K_GAMMA, K_CST, K_BM3D, K_CHROMA_BM3D, K_LUMA_CHROMA, K_FILM, K_OTHER
 };
 
 struct Stage {
   std::string name;  // canonical schema name
   StageKind kind = K_OTHER;
   int algorithm = IMOPS_DEMOSAIC_AMAZE;
-  double ev = 0.0, gamma = 2.2, intensity = 0.0;
+  double ev = 0.0, gamma = 2.2, intensity = 0.0, epsilon = 0.01;
+  int radius = 4;
   std::string target;
   Bm3dStageParams bm3d;
   FilmStageParams film;
@@ -355,6 +356,13 @@ Stage make_stage(const Table& t) {
   } else if (s.name == "ChromaDenoise") {  // modules.rs:688-722
     s.kind = K_CHROMA_BM3D;
     s.intensity = num_param(t, "intensity", 0.0, "ChromaDenoise");
+  } else if (s.name == "LumaGuidedChromaDenoise") {  // modules.rs:724-760
+    s.kind = K_LUMA_CHROMA;
+    const double r = num_param(t, "radius", 4, "LumaGuidedChromaDenoise");
+    if (!(r >= 0 && r <= 1e9) || r != (double)(long long)r)
+      throw Error{IMOPS_E_INVALID, "Invalid LumaGuidedChromaDenoise config"};
+    s.radius = (int)r;
+    s.epsilon = num_param(t, "epsilon", 0.01, "LumaGuidedChromaDenoise");
   } else if (s.name == "Film") {
     s.kind = K_FILM;
     s.film = film_stage_params_from_table(str_param(t, "stock", "ColorNeg200", "Film"),
@@ -421,6 +429,7 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
         break;
       case K_BM3D: stage_bm3d(ctx, img, s.bm3d); break;
       case K_CHROMA_BM3D: stage_chroma_bm3d(ctx, img, (float)s.intensity); break;
+      case K_LUMA_CHROMA: stage_luma_chroma_denoise(ctx, img, s.radius, (float)s.epsilon); break;
       case K_FILM: stage_film(ctx, img, s.film); break;
       default:
         throw Error{IMOPS_E_UNSUPPORTED, "stage " + s.name + " is not implemented by the GPU library"};

@@ -31,6 +31,8 @@ Bm3dStageParams bm3d_stage_params_from_table(double intensity, double sigma_norm
 void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p);
 // chroma_bm3d (pichromatic/src/bm3d.rs:705-759)
 void stage_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
+// chroma_denoise (pichromatic/src/chroma_denoise.rs:14-152)
+void stage_luma_chroma_denoise(imops_ctx* ctx, imops_image* img, int radius, float epsilon);
 
 struct FilmStageParams {
   int stock = 1;    // StockId order of film/stock/mod.rs (0 BwStub, 1 ColorNeg200, ...)

@@ -182,6 +182,13 @@ int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* par
  * fixed point; bit-exact with the reference. */
 int imops_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
 
+/* Luma-guided chroma denoise (Image::chroma_denoise, pixel.rs:128-137;
+ * chroma_denoise.rs:14-152; stage LumaGuidedChromaDenoise, modules.rs:724-760):
+ * guided filter (radius, epsilon) of C_R = R - Y and C_B = B - Y with the luma
+ * as guide, box means as the reference's running sums; bit-exact with the
+ * reference's f32 arithmetic. */
+int imops_luma_chroma_denoise(imops_ctx* ctx, imops_image* img, int radius, float epsilon);
+
 /* Film simulation (film::process, pichromatic/src/film/mod.rs:77-129) on an
  * absolute-luminance ACEScg image; output ACEScg.  Module<Film>
  * (modules.rs:841-876).  stock: 0 BwStub, 1 ColorNeg200, 2 Portra400,

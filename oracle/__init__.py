@@ -13,6 +13,7 @@ Functions and the reference code they restate:
 * ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
 * ``bm3d``         pichromatic/src/bm3d.rs:1-699 (f32 exact order; fp64 variant); ``chroma_bm3d`` bm3d.rs:705-759
 * ``bm3d_test_image`` / ``refined_sum``  the fixture and checksum of bm3d.rs:765-789
+* ``box_filter``, ``luma_chroma_denoise``  pichromatic/src/chroma_denoise.rs:14-152 (f32 exact order)
 """
 from __future__ import annotations
 
@@ -44,6 +45,8 @@ _SIGS = {
     "oracle_bm3d_tables": (C.c_int, [_fp, _fp, _fp]),
     "oracle_bm3d_test_image": (C.c_int, [_fp, _sz, _sz]),
     "oracle_refined_sum": (C.c_double, [_fp, _sz, C.c_int]),
+    "oracle_box_filter": (C.c_int, [_fp, _fp, _sz, _sz, _sz]),
+    "oracle_luma_chroma_denoise": (C.c_int, [_fp, _sz, _sz, _sz, C.c_float]),
 }
 
 CS_SRGB, CS_LINEAR_SRGB, CS_ACESCG, CS_XYZD65 = 0, 1, 2, 3
@@ -170,6 +173,21 @@ def chroma_bm3d(rgb: np.ndarray, intensity: float) -> np.ndarray:
     out = np.array(rgb, np.float32, copy=True, order="C")
     h, w, _ = out.shape
     _rc(lib().oracle_chroma_bm3d(_f(out), w, h, intensity), "chroma_bm3d")
+    return out
+
+
+def box_filter(plane: np.ndarray, radius: int) -> np.ndarray:
+    a = np.ascontiguousarray(plane, np.float32)
+    out = np.empty_like(a)
+    h, w = a.shape
+    _rc(lib().oracle_box_filter(_f(a), _f(out), w, h, radius), "box_filter")
+    return out
+
+
+def luma_chroma_denoise(rgb: np.ndarray, radius: int, epsilon: float) -> np.ndarray:
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    _rc(lib().oracle_luma_chroma_denoise(_f(out), w, h, radius, epsilon), "luma_chroma_denoise")
     return out
 
 

@@ -1,4 +1,4 @@
-"""GPU parity of the chroma-only denoise stages against the oracle, through the C ABI.
+"""GPU parity of the chroma denoise stages against the oracle, through the C ABI.
 
 ChromaDenoise (chroma_bm3d, bm3d.rs:705-759): the DCTs follow the reference's
 f32 operation order and the Kaiser aggregation is exact i64 arithmetic, so the
@@ -51,3 +51,25 @@ def test_chroma_denoise_stage_in_pipeline(gpu_ctx):
     img = im.Image(gpu_ctx).set_rgb(rgb, im.CS_NONE)
     im.run_pixel_pipeline(img, cfg)
     _check(img.rgb(), oracle.chroma_bm3d(rgb, 0.4))
+
+
+# LumaGuidedChromaDenoise (chroma_denoise.rs:14-152): the box filter's running
+# sums are walked in the reference's order, one lane per line, so the bar is
+# bit-exact.  Sizes cross the 64-line / 64-position chunks; radii include 0,
+# radii past a chunk and past the frame (clamped windows), and 1-pixel frames.
+@pytest.mark.parametrize("w,h,radius,eps", [(200, 150, 4, 0.01), (131, 67, 0, 0.01), (64, 300, 8, 0.05),
+                                            (97, 65, 70, 0.01), (1, 50, 3, 0.01), (70, 1, 2, 0.01),
+                                            (300, 200, 33, 0.1), (129, 129, 64, 0.01)])
+def test_luma_chroma_denoise_matches_oracle(gpu_ctx, w, h, radius, eps):
+    print(f"luma_chroma_denoise {w}x{h} r={radius} eps={eps}", flush=True)
+    rgb = _noisy_scene(w, h, seed=w + 3 * h)
+    got = im.Image(gpu_ctx).set_rgb(rgb, im.CS_NONE).chroma_denoise(radius, eps).rgb()
+    _check(got, oracle.luma_chroma_denoise(rgb, radius, eps))
+
+
+def test_luma_guided_stage_in_pipeline(gpu_ctx):
+    rgb = _noisy_scene(150, 100, 9)
+    cfg = im.parse_config('[[pipeline_modules]]\nname = "LumaGuidedChromaDenoise"\nradius = 5\nepsilon = 0.02\n')
+    img = im.Image(gpu_ctx).set_rgb(rgb, im.CS_NONE)
+    im.run_pixel_pipeline(img, cfg)
+    _check(img.rgb(), oracle.luma_chroma_denoise(rgb, 5, np.float32(0.02)))
