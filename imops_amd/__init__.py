@@ -199,6 +199,15 @@ class Image:
         check(lib().imops_image_set_shutter_seconds(self._h, float(s)))
         return self
 
+    def set_exposure_metadata(self, baseline_ev: float, f_number: float, iso: float) -> "Image":
+        check(lib().imops_image_set_exposure_metadata(self._h, float(baseline_ev), float(f_number), float(iso)))
+        return self
+
+    def set_opcode_list3(self, data: bytes) -> "Image":
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().imops_image_set_opcode_list3(self._h, C.cast(buf, C.c_void_p), len(data)))
+        return self
+
     # -- stages (pixel.rs:55-171) -------------------------------------------
     def demosaic(self, algorithm="amaze") -> "Image":
         a = _ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
@@ -245,6 +254,36 @@ class Image:
     def chroma_denoise(self, radius: int = 4, epsilon: float = 0.01) -> "Image":
         """Luma-guided chroma denoise (Image::chroma_denoise, pixel.rs:128-137; chroma_denoise.rs:14-152)."""
         check(lib().imops_luma_chroma_denoise(self.ctx.handle, self._h, int(radius), float(epsilon)))
+        return self
+
+    def contrast(self, c: float) -> "Image":
+        """Image::contrast (pixel.rs:107-110)."""
+        check(lib().imops_contrast(self.ctx.handle, self._h, float(c)))
+        return self
+
+    def highlight_reconstruction(self) -> "Image":
+        """Image::highlight_reconstruction (pixel.rs:111-114) with the image's white balance."""
+        check(lib().imops_highlight_reconstruction(self.ctx.handle, self._h))
+        return self
+
+    def lch(self, lc: float = 1.0, cc: float = 1.0, hc: float = 1.0) -> "Image":
+        """Image::lch (pixel.rs:60-68)."""
+        check(lib().imops_lch(self.ctx.handle, self._h, float(lc), float(cc), float(hc)))
+        return self
+
+    def sigmoid_tone_map(self) -> "Image":
+        """Image::sigmoid_tone_map (pixel.rs:91-94)."""
+        check(lib().imops_sigmoid_tone_map(self.ctx.handle, self._h))
+        return self
+
+    def baseline_exposure(self) -> "Image":
+        """BaselineExposureCompensation (modules.rs:332-371) from the image's exposure metadata."""
+        check(lib().imops_baseline_exposure(self.ctx.handle, self._h))
+        return self
+
+    def vignette(self, strength: float = 1.0) -> "Image":
+        """Image::vignette (pixel.rs:149-160) from the image's OpcodeList3."""
+        check(lib().imops_vignette(self.ctx.handle, self._h, float(strength)))
         return self
 
     def film(self, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,

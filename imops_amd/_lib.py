@@ -67,6 +67,8 @@ SIGNATURES = {
     "imops_image_set_wb_coeffs": (_i, [_vp, _fp]),
     "imops_image_set_calibration_matrix": (_i, [_vp, _fp, _sz]),
     "imops_image_set_shutter_seconds": (_i, [_vp, _f]),
+    "imops_image_set_exposure_metadata": (_i, [_vp, _f, _f, _f]),
+    "imops_image_set_opcode_list3": (_i, [_vp, _vp, _sz]),
     "imops_demosaic": (_i, [_vp, _vp, _i]),
     "imops_cfa_coeffs": (_i, [_vp, _vp, _fp]),
     "imops_exp": (_i, [_vp, _vp, _f]),
@@ -78,6 +80,12 @@ SIGNATURES = {
     "imops_bm3d_ex": (_i, [_vp, _vp, C.POINTER(Bm3dParams)]),
     "imops_chroma_bm3d": (_i, [_vp, _vp, _f]),
     "imops_luma_chroma_denoise": (_i, [_vp, _vp, _i, _f]),
+    "imops_contrast": (_i, [_vp, _vp, _f]),
+    "imops_highlight_reconstruction": (_i, [_vp, _vp]),
+    "imops_lch": (_i, [_vp, _vp, _f, _f, _f]),
+    "imops_sigmoid_tone_map": (_i, [_vp, _vp]),
+    "imops_baseline_exposure": (_i, [_vp, _vp]),
+    "imops_vignette": (_i, [_vp, _vp, _f]),
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
     "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
     "imops_film_fit_state_floats": (_i, [_i, _sz, _sz, _sz, C.POINTER(_sz)]),

@@ -287,14 +287,16 @@ std::string trim(const std::string& s) {
 }  // namespace
 
 enum StageKind {
-  K_DEMOSAIC, K_CFACOEFFS, K_EXP, K_GAMMA, K_CST, K_BM3D, K_CHROMA_BM3D, K_LUMA_CHROMA, K_FILM, K_OTHER
+  K_DEMOSAIC, K_CFACOEFFS, K_EXP, K_GAMMA, K_CST, K_CONTRAST, K_HIGHLIGHT, K_LCH, K_SIGMOID, K_BASELINE,
+  K_BM3D, K_CHROMA_BM3D, K_LUMA_CHROMA, K_VIGNETTE, K_FILM, K_OTHER
 };
 
 struct Stage {
   std::string name;  // canonical schema name
   StageKind kind = K_OTHER;
   int algorithm = IMOPS_DEMOSAIC_AMAZE;
-  double ev = 0.0, gamma = 2.2, intensity = 0.0, epsilon = 0.01;
+  double ev = 0.0, gamma = 2.2, intensity = 0.0, epsilon = 0.01, contrast = 1.0, strength = 1.0;
+  double lch[3] = {1.0, 1.0, 1.0};
   int radius = 4;
   std::string target;
   Bm3dStageParams bm3d;
@@ -356,6 +358,23 @@ Stage make_stage(const Table& t) {
   } else if (s.name == "ChromaDenoise") {  // modules.rs:688-722
     s.kind = K_CHROMA_BM3D;
     s.intensity = num_param(t, "intensity", 0.0, "ChromaDenoise");
+  } else if (s.name == "Contrast") {  // modules.rs:454-490
+    s.kind = K_CONTRAST;
+    s.contrast = num_param(t, "c", 1.0, "Contrast");
+  } else if (s.name == "HighlightReconstruction") {  // modules.rs:265-296
+    s.kind = K_HIGHLIGHT;
+  } else if (s.name == "LCH") {  // modules.rs:221-262
+    s.kind = K_LCH;
+    s.lch[0] = num_param(t, "lc", 1.0, "LCH");
+    s.lch[1] = num_param(t, "cc", 1.0, "LCH");
+    s.lch[2] = num_param(t, "hc", 1.0, "LCH");
+  } else if (s.name == "SigmoidToneMap") {  // modules.rs:392-416
+    s.kind = K_SIGMOID;
+  } else if (s.name == "BaselineExposureCompensation") {  // modules.rs:331-384
+    s.kind = K_BASELINE;
+  } else if (s.name == "Vignette") {  // modules.rs:762-798
+    s.kind = K_VIGNETTE;
+    s.strength = num_param(t, "strength", 1.0, "Vignette");
   } else if (s.name == "LumaGuidedChromaDenoise") {  // modules.rs:724-760
     s.kind = K_LUMA_CHROMA;
     const double r = num_param(t, "radius", 4, "LumaGuidedChromaDenoise");
@@ -375,7 +394,10 @@ Stage make_stage(const Table& t) {
   return s;
 }
 
-bool is_pointwise(const Stage& s) { return s.kind == K_CFACOEFFS || s.kind == K_EXP || s.kind == K_GAMMA || s.kind == K_CST; }
+bool is_pointwise(const Stage& s) {
+  return s.kind == K_CFACOEFFS || s.kind == K_EXP || s.kind == K_GAMMA || s.kind == K_CST || s.kind == K_CONTRAST ||
+         s.kind == K_HIGHLIGHT || s.kind == K_LCH || s.kind == K_SIGMOID || s.kind == K_BASELINE;
+}
 
 }  // namespace
 
@@ -391,7 +413,7 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
       int cs = img->color_space;
       for (; k < stages.size() && is_pointwise(stages[k]); ++k) {
         const Stage& e = stages[k];
-        if (prog.n + 3 > PointwiseProgram::kMax) {  // a stage adds at most 3 ops
+        if (prog.n + kMaxOpsPerStage > PointwiseProgram::kMax) {
           stage_pointwise(ctx, img, prog);
           prog = PointwiseProgram();
         }
@@ -401,6 +423,14 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
             push_wb(prog, img->wb);
             break;
           case K_EXP: push_exp(prog, (float)e.ev); break;
+          case K_CONTRAST: push_contrast(prog, (float)e.contrast); break;
+          case K_HIGHLIGHT:
+            if (!img->has_wb) throw Error{IMOPS_E_STATE, "wb_coeffs needs to be set to perform highlight_reconstruction"};
+            push_highlight(prog, img->wb);
+            break;
+          case K_LCH: push_lch(prog, cs, (float)e.lch[0], (float)e.lch[1], (float)e.lch[2]); break;
+          case K_SIGMOID: push_sigmoid(prog); break;
+          case K_BASELINE: push_baseline_exposure(prog, img); break;
           case K_GAMMA: push_gamma(prog, (float)e.gamma); break;
           case K_CST: {
             const int target = parse_color_space(trim(e.target));
@@ -430,6 +460,7 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
       case K_BM3D: stage_bm3d(ctx, img, s.bm3d); break;
       case K_CHROMA_BM3D: stage_chroma_bm3d(ctx, img, (float)s.intensity); break;
       case K_LUMA_CHROMA: stage_luma_chroma_denoise(ctx, img, s.radius, (float)s.epsilon); break;
+      case K_VIGNETTE: stage_vignette(ctx, img, (float)s.strength); break;
       case K_FILM: stage_film(ctx, img, s.film); break;
       default:
         throw Error{IMOPS_E_UNSUPPORTED, "stage " + s.name + " is not implemented by the GPU library"};
@@ -469,6 +500,14 @@ std::string shape_key(const imops_image* f) {
   if (f->has_wb) add_floats(f->wb, 4);
   add_floats(f->calib.data(), f->calib.size());
   if (f->has_shutter) add_floats(&f->shutter, 1);
+  if (f->has_baseline) {
+    const float e[3] = {f->baseline_ev, f->f_number, f->iso};
+    add_floats(e, 3);
+  }
+  if (f->has_opcodes) {
+    k += "|o";
+    for (uint8_t b : f->opcode_list3) k += std::to_string(b) + ",";
+  }
   return k;
 }
 

@@ -71,6 +71,86 @@ __device__ __forceinline__ float eotf(float x) {
   return x / 12.92f;
 }
 
+// Oklab with Ottosson's published matrices (linear sRGB -> LMS, cube root,
+// LMS' -> Lab) in the polar form the colour crate calls Oklch: C = |(a, b)|,
+// hue = atan2(b, a) in degrees, wrapped to [0, 360).
+__device__ __forceinline__ void lin_to_oklch(float r, float g, float b, float& L, float& C, float& H) {
+  const float l = 0.4122214708f * r + 0.5363325363f * g + 0.0514459929f * b;
+  const float m = 0.2119034982f * r + 0.6806995451f * g + 0.1073969566f * b;
+  const float s = 0.0883024619f * r + 0.2817188376f * g + 0.6299787005f * b;
+  const float l_ = cbrtf(l), m_ = cbrtf(m), s_ = cbrtf(s);
+  L = 0.2104542553f * l_ + 0.7936177850f * m_ - 0.0040720468f * s_;
+  const float A = 1.9779984951f * l_ - 2.4285922050f * m_ + 0.4505937099f * s_;
+  const float B = 0.0259040371f * l_ + 0.7827717662f * m_ - 0.8086757660f * s_;
+  C = sqrtf(A * A + B * B);
+  float h = atan2f(B, A) * (180.0f / 3.14159265358979f);
+  if (h < 0.0f) h += 360.0f;
+  H = h;
+}
+
+__device__ __forceinline__ void oklch_to_lin(float L, float C, float H, float& r, float& g, float& b) {
+  const float hr = H * (3.14159265358979f / 180.0f);
+  const float A = C * cosf(hr), B = C * sinf(hr);
+  const float l_ = L + 0.3963377774f * A + 0.2158037573f * B;
+  const float m_ = L - 0.1055613458f * A - 0.0638541728f * B;
+  const float s_ = L - 0.0894841775f * A - 1.2914855480f * B;
+  const float l = l_ * l_ * l_, m = m_ * m_ * m_, s = s_ * s_ * s_;
+  r = 4.0767416621f * l - 3.3077115913f * m + 0.2309699292f * s;
+  g = -1.2684380046f * l + 2.6097574011f * m - 0.3413193965f * s;
+  b = -0.0041960863f * l - 0.7034186147f * m + 1.7076147010f * s;
+}
+
+// ACES reference gamut compression, forward (tone_map.rs:75-170, RgcParams::default)
+__device__ __forceinline__ float rgc_compress(float dist, float lim, float thr, float power) {
+  if (dist < thr) return dist;
+  const float base_inner = (1.0f - thr) / (lim - thr);
+  if (base_inner <= 0.0f || fabsf(lim - thr) < 1e-6f) return dist;
+  const float denom_inner = powf(base_inner, -power) - 1.0f;
+  if (denom_inner <= 0.0f) return dist;
+  const float s = (lim - thr) / powf(denom_inner, 1.0f / power);
+  const float dn = (dist - thr) / s;
+  const float den = powf(1.0f + powf(dn, power), 1.0f / power);
+  return thr + s * (dn / den);
+}
+
+__device__ __forceinline__ float square_sigmoid(float x) {  // tone_map.rs:7-15
+  const float c = 1.219512f;
+  if (x <= 0.0f) return 0.0f;
+  return 1.0f / (1.0f + (1.0f / (c * x)));
+}
+
+__device__ __forceinline__ void mat3(const float* m, float& v0, float& v1, float& v2) {
+  const float a = m[0] * v0 + m[1] * v1 + m[2] * v2;
+  const float b = m[3] * v0 + m[4] * v1 + m[5] * v2;
+  const float c = m[6] * v0 + m[7] * v1 + m[8] * v2;
+  v0 = a; v1 = b; v2 = c;
+}
+
+// sigmoid (tone_map.rs:16-31) on one ACEScg pixel; a2l / l2a convert to and from linear sRGB
+__device__ void sigmoid_tone_map(const float* a2l, const float* l2a, float& v0, float& v1, float& v2) {
+  float g0, g1, g2;
+  const float ach = fmaxf(fmaxf(v0, v1), v2);
+  if (ach == 0.0f) {
+    g0 = g1 = g2 = 0.0f;
+  } else {
+    const float aa = fabsf(ach);
+    g0 = ach - rgc_compress((ach - v0) / aa, 1.147f, 0.815f, 1.2f) * aa;
+    g1 = ach - rgc_compress((ach - v1) / aa, 1.264f, 0.803f, 1.2f) * aa;
+    g2 = ach - rgc_compress((ach - v2) / aa, 1.312f, 0.88f, 1.2f) * aa;
+  }
+  float x0 = g0, x1 = g1, x2 = g2, L, C, H;
+  mat3(a2l, x0, x1, x2);
+  lin_to_oklch(x0, x1, x2, L, C, H);
+  const float p0 = square_sigmoid(g0), p1 = square_sigmoid(g1), p2 = square_sigmoid(g2);
+  const float sl = 0.2722287168f * p0 + 0.6740817658f * p1 + 0.0536895174f * p2;  // luminance (pixel.rs:43-47)
+  const float mult = 1.0f - powf(sl, 5.0f);
+  float y0 = p0, y1 = p1, y2 = p2, L2, C2, H2;
+  mat3(a2l, y0, y1, y2);
+  lin_to_oklch(y0, y1, y2, L2, C2, H2);
+  oklch_to_lin(L2, C2 * mult, H, v0, v1, v2);
+  mat3(l2a, v0, v1, v2);
+}
+
 __global__ void pointwise_kernel(float* __restrict__ rgb, size_t npix, PointwiseProgram prog) {
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += (size_t)gridDim.x * blockDim.x) {
     float v0 = rgb[p * 3], v1 = rgb[p * 3 + 1], v2 = rgb[p * 3 + 2];
@@ -101,12 +181,56 @@ __global__ void pointwise_kernel(float* __restrict__ rgb, size_t npix, Pointwise
           v2 = v2 <= 0.0f ? 0.0f : powf(v2, inv);
           break;
         }
+        case PW_CONTRAST: {  // contrast.rs:4-8, MIDDLE_GRAY 0.185 (pixel.rs:36)
+          const float c = op.m[0];
+          v0 = 0.185f * powf(v0 / 0.185f, c);
+          v1 = 0.185f * powf(v1 / 0.185f, c);
+          v2 = 0.185f * powf(v2 / 0.185f, c);
+          break;
+        }
+        case PW_HIGHLIGHT: {  // highlight_reconstruction.rs:5-14
+          const float f = v1 / op.m[0];
+          v1 = ((1.0f - f) * v1) + (f * (v0 + v2) * 0.5f);
+          break;
+        }
+        case PW_LIN_TO_OKLCH: {
+          float L, C, H;
+          lin_to_oklch(v0, v1, v2, L, C, H);
+          v0 = L; v1 = C; v2 = H;
+          break;
+        }
+        case PW_OKLCH_TO_LIN: {
+          float r, g, b;
+          oklch_to_lin(v0, v1, v2, r, g, b);
+          v0 = r; v1 = g; v2 = b;
+          break;
+        }
+        case PW_SIGMOID: sigmoid_tone_map(op.m, op.m + 9, v0, v1, v2); break;
         default: break;
       }
     }
     rgb[p * 3] = v0;
     rgb[p * 3 + 1] = v1;
     rgb[p * 3 + 2] = v2;
+  }
+}
+
+// apply_vignette_radial_correction (vignette.rs:71-96) for one FixVignetteRadial opcode
+__global__ void vignette_kernel(float* __restrict__ rgb, int width, int height, VignetteParams v) {
+  const size_t n = (size_t)width * height;
+  const float wf = (float)(width - 1), hf = (float)(height - 1);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float py = (float)(i / width), px = (float)(i % width);
+    const float u = px / wf, vv = py / hf;
+    const float du = u - v.cx, dv = vv - v.cy;
+    const float d = sqrtf(du * du + dv * dv);
+    const float r = d / v.d_max;
+    const float r2 = r * r, r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4, r10 = r8 * r2;
+    const float corr = v.k[0] * r2 + v.k[1] * r4 + v.k[2] * r6 + v.k[3] * r8 + v.k[4] * r10;
+    const float gain = fmaxf(1.0f + v.strength * corr, 0.0f);
+    rgb[i * 3] *= gain;
+    rgb[i * 3 + 1] *= gain;
+    rgb[i * 3 + 2] *= gain;
   }
 }
 
@@ -130,6 +254,12 @@ hipError_t markesteijn_launch(const float* in, int width, int height, const CfaP
                               hipStream_t s) {
   hipLaunchKernelGGL(markesteijn_kernel, dim3(grid_for((size_t)width * height, 256)), dim3(256), 0, s, in, width,
                      height, cfa, out);
+  return hipGetLastError();
+}
+
+hipError_t vignette_launch(float* rgb, int width, int height, const VignetteParams& v, hipStream_t s) {
+  hipLaunchKernelGGL(vignette_kernel, dim3(grid_for((size_t)width * height, 256)), dim3(256), 0, s, rgb, width, height,
+                     v);
   return hipGetLastError();
 }
 

@@ -13,11 +13,18 @@ struct CfaPattern {
   __host__ __device__ int at(int r, int c) const { return base[r % h][c % w]; }
 };
 
-enum PointwiseKind : int { PW_MAT3 = 1, PW_SCALE3 = 2, PW_OETF = 3, PW_EOTF = 4, PW_GAMMA = 5 };
+enum PointwiseKind : int {
+  PW_MAT3 = 1, PW_SCALE3 = 2, PW_OETF = 3, PW_EOTF = 4, PW_GAMMA = 5,
+  PW_CONTRAST = 6,      // m[0] = contrast exponent
+  PW_HIGHLIGHT = 7,     // m[0] = green clip (white-balance coefficient 1)
+  PW_LIN_TO_OKLCH = 8,  // linear sRGB -> Oklch (L, C, hue in degrees)
+  PW_OKLCH_TO_LIN = 9,  // Oklch -> linear sRGB
+  PW_SIGMOID = 10,      // SigmoidToneMap on ACEScg; m[0..8] ACEScg -> linear sRGB, m[9..17] back
+};
 
 struct PointwiseOp {
   int kind;
-  float m[9];
+  float m[18];
 };
 
 // A short program of elementwise ops applied in order to every pixel.
@@ -28,7 +35,7 @@ struct PointwiseProgram {
   bool push(int kind, const float* m, int nm) {
     if (n >= kMax) return false;
     ops[n].kind = kind;
-    for (int i = 0; i < 9; ++i) ops[n].m[i] = i < nm ? m[i] : 0.0f;
+    for (int i = 0; i < 18; ++i) ops[n].m[i] = i < nm ? m[i] : 0.0f;
     ++n;
     return true;
   }
@@ -39,5 +46,11 @@ hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_
 hipError_t markesteijn_launch(const float* in, int width, int height, const CfaPattern& cfa, float* out,
                               hipStream_t s);
 hipError_t pointwise_launch(float* rgb, size_t npix, const PointwiseProgram& prog, hipStream_t s);
+
+// FixVignetteRadial (vignette.rs:56-96): k[0..4] polynomial, (cx, cy) centre, d_max corner distance.
+struct VignetteParams {
+  float k[5], cx, cy, d_max, strength;
+};
+hipError_t vignette_launch(float* rgb, int width, int height, const VignetteParams& v, hipStream_t s);
 
 }  // namespace imops

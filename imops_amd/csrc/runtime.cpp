@@ -293,6 +293,99 @@ void push_wb(PointwiseProgram& prog, const float wb[4]) {
   if (!prog.push(PW_SCALE3, wb, 3)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
 }
 
+namespace {
+void push_or_throw(PointwiseProgram& prog, int kind, const float* m, int nm) {
+  if (!prog.push(kind, m, nm)) throw Error{IMOPS_E_INVALID, "too many fused pointwise ops"};
+}
+}  // namespace
+
+void push_contrast(PointwiseProgram& prog, float c) { push_or_throw(prog, PW_CONTRAST, &c, 1); }
+
+// highlight_reconstruction.rs:5-14: clip_g is wb_coeffs[1]
+void push_highlight(PointwiseProgram& prog, const float wb[4]) { push_or_throw(prog, PW_HIGHLIGHT, &wb[1], 1); }
+
+// lch.rs:5-21: source space -> Oklch (through linear sRGB, as the colour crate
+// converts), scale L, C and hue, and back
+void push_lch(PointwiseProgram& prog, int color_space, float lc, float cc, float hc) {
+  if (color_space == IMOPS_CS_NONE) throw Error{IMOPS_E_STATE, "LCH needs the image's colour space"};
+  push_convert(prog, color_space, IMOPS_CS_LINEAR_SRGB);
+  push_or_throw(prog, PW_LIN_TO_OKLCH, nullptr, 0);
+  const float s[3] = {lc, cc, hc};
+  push_or_throw(prog, PW_SCALE3, s, 3);
+  push_or_throw(prog, PW_OKLCH_TO_LIN, nullptr, 0);
+  push_convert(prog, IMOPS_CS_LINEAR_SRGB, color_space);
+}
+
+// tone_map.rs:16-31: the buffer is taken as ACEScg whatever its tag
+void push_sigmoid(PointwiseProgram& prog) {
+  const ColourMats& m = mats();
+  float both[18];
+  for (int k = 0; k < 9; ++k) {
+    both[k] = m.aces_to_lin[k];
+    both[9 + k] = m.lin_to_aces[k];
+  }
+  push_or_throw(prog, PW_SIGMOID, both, 18);
+}
+
+// modules.rs:335-371: 2^ev when |ev| > 1e-6, then K N^2 / (t S) (radiance.rs:19-31,
+// K = 12.5) when shutter, f-number and ISO are all known and positive
+void push_baseline_exposure(PointwiseProgram& prog, const imops_image* img) {
+  const float ev = img->has_baseline ? img->baseline_ev : 0.0f;
+  if (std::fabs(ev) > 1e-6f) push_exp(prog, ev);
+  if (img->has_shutter && img->has_f_number && img->has_iso && img->shutter > 0.0f && img->f_number > 0.0f &&
+      img->iso > 0.0f) {
+    const double t = img->shutter, n = img->f_number, iso = img->iso;
+    const float g = (float)(12.5 * 1.0 * (n * n) / (t * iso));
+    const float s[3] = {g, g, g};
+    push_or_throw(prog, PW_SCALE3, s, 3);
+  }
+}
+
+// apply_vignette_radial_correction (vignette.rs:3-99): big-endian opcode list,
+// one pass per FixVignetteRadial (id 3) opcode with >= 56 parameter bytes
+void stage_vignette(imops_ctx* ctx, imops_image* img, float strength) {
+  require_rgb(img, "Vignette");
+  if (!img->has_opcodes) return;
+  const std::vector<uint8_t>& op = img->opcode_list3;
+  if (op.size() < 4) return;
+  auto u32 = [&](size_t o) {
+    return ((uint32_t)op[o] << 24) | ((uint32_t)op[o + 1] << 16) | ((uint32_t)op[o + 2] << 8) | (uint32_t)op[o + 3];
+  };
+  auto f64 = [&](size_t o) {
+    uint64_t b = 0;
+    for (int i = 0; i < 8; ++i) b = (b << 8) | op[o + i];
+    double d;
+    std::memcpy(&d, &b, sizeof d);
+    return d;
+  };
+  const size_t count = u32(0);
+  size_t off = 4;
+  TimedRegion tr(ctx, "vignette");
+  for (size_t c = 0; c < count; ++c) {
+    if (off + 16 > op.size()) break;
+    const uint32_t id = u32(off);
+    const size_t psize = u32(off + 12);
+    off += 16;
+    if (off + psize > op.size()) break;
+    const size_t p = off;
+    off += psize;
+    if (id != 3 || psize < 56) continue;
+    VignetteParams v;
+    for (int k = 0; k < 5; ++k) v.k[k] = (float)f64(p + 8 * k);
+    v.cx = (float)f64(p + 40);
+    v.cy = (float)f64(p + 48);
+    const float cx = v.cx, cy = v.cy;
+    const float d00 = std::sqrt(cx * cx + cy * cy);
+    const float d10 = std::sqrt((1.0f - cx) * (1.0f - cx) + cy * cy);
+    const float d01 = std::sqrt(cx * cx + (1.0f - cy) * (1.0f - cy));
+    const float d11 = std::sqrt((1.0f - cx) * (1.0f - cx) + (1.0f - cy) * (1.0f - cy));
+    v.d_max = std::fmax(std::fmax(std::fmax(d00, d10), d01), d11);
+    v.strength = strength;
+    if (v.d_max > 0.0f)
+      IMOPS_HIP_CHECK(vignette_launch(img->rgb.as<float>(), (int)img->width, (int)img->height, v, ctx->stream));
+  }
+}
+
 void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& prog) {
   TimedRegion tr(ctx, "pointwise");
   IMOPS_HIP_CHECK(pointwise_launch(img->rgb.as<float>(), img->width * img->height, prog, ctx->stream));
@@ -346,6 +439,19 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
 using namespace imops;
 
 // ---------------------------------------------------------------- C ABI
+namespace imops {
+template <class F>
+int run_program(imops_ctx* ctx, imops_image* img, const char* name, F&& build) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, std::string(name) + ": null argument"};
+    require_rgb(img, name);
+    PointwiseProgram p;
+    build(p);
+    stage_pointwise(ctx, img, p);
+  });
+}
+}  // namespace imops
+
 extern "C" {
 
 const char* imops_last_error(void) { return t_last_error.c_str(); }
@@ -430,6 +536,14 @@ int imops_ctx_destroy(imops_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (imops_image* img : ctx->images) {
+      img->raw.release();
+      img->rgb.release();
+      img->tmp.release();
+      img->has_raw = img->has_rgb = false;
+      img->ctx = nullptr;
+    }
+    ctx->images.clear();
     ctx->timer.clear();
     for (auto& b : ctx->scr) b.release();
     ctx->bm3d_frags.release();
@@ -454,6 +568,7 @@ int imops_image_create(imops_ctx* ctx, imops_image** out) {
     if (!ctx || !out) throw Error{IMOPS_E_INVALID, "imops_image_create: null argument"};
     auto* img = new imops_image();
     img->ctx = ctx;
+    ctx->images.insert(img);
     *out = img;
   });
 }
@@ -461,11 +576,14 @@ int imops_image_create(imops_ctx* ctx, imops_image** out) {
 int imops_image_free(imops_image* img) {
   return guarded([&] {
     if (!img) return;
-    (void)hipSetDevice(img->ctx->device);
-    (void)hipStreamSynchronize(img->ctx->stream);
-    img->raw.release();
-    img->rgb.release();
-    img->tmp.release();
+    if (img->ctx) {  // else its context was destroyed first and released the buffers
+      (void)hipSetDevice(img->ctx->device);
+      (void)hipStreamSynchronize(img->ctx->stream);
+      img->raw.release();
+      img->rgb.release();
+      img->tmp.release();
+      img->ctx->images.erase(img);
+    }
     delete img;
   });
 }
@@ -475,6 +593,7 @@ int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* sr
                             float white, const char* cfa_pattern) {
   return guarded([&] {
     if (!ctx || !img || !src || !cfa_pattern) throw Error{IMOPS_E_INVALID, "set_raw_u16: null argument"};
+    if (!img->ctx) throw Error{IMOPS_E_STATE, "set_raw_u16: the image's context was destroyed"};
     if (cw == 0 || ch == 0 || cx + cw > full_w || cy + ch > full_h)
       throw Error{IMOPS_E_INVALID, "set_raw_u16: crop window outside the frame"};
     if (!(white > black)) throw Error{IMOPS_E_INVALID, "set_raw_u16: white level must exceed black level"};
@@ -507,6 +626,7 @@ int imops_image_set_raw_f32(imops_ctx* ctx, imops_image* img, const float* src, 
                             size_t h, const char* cfa_pattern, size_t crop_x, size_t crop_y) {
   return guarded([&] {
     if (!ctx || !img || !src || !cfa_pattern) throw Error{IMOPS_E_INVALID, "set_raw_f32: null argument"};
+    if (!img->ctx) throw Error{IMOPS_E_STATE, "set_raw_f32: the image's context was destroyed"};
     if (w == 0 || h == 0) throw Error{IMOPS_E_INVALID, "set_raw_f32: empty image"};
     CfaTable cfa;
     if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
@@ -530,6 +650,7 @@ int imops_image_set_rgb_f32(imops_ctx* ctx, imops_image* img, const float* src, 
                             size_t h, int cs) {
   return guarded([&] {
     if (!ctx || !img || !src) throw Error{IMOPS_E_INVALID, "set_rgb_f32: null argument"};
+    if (!img->ctx) throw Error{IMOPS_E_STATE, "set_rgb_f32: the image's context was destroyed"};
     if (w == 0 || h == 0) throw Error{IMOPS_E_INVALID, "set_rgb_f32: empty image"};
     if (cs < IMOPS_CS_NONE || cs > IMOPS_CS_XYZD65) throw Error{IMOPS_E_INVALID, "set_rgb_f32: unknown colour space"};
     img->rgb.reserve(w * h * 3 * sizeof(float));
@@ -594,6 +715,55 @@ int imops_image_set_shutter_seconds(imops_image* img, float s) {
     if (!img) throw Error{IMOPS_E_INVALID, "set_shutter_seconds: null image"};
     img->shutter = s;
     img->has_shutter = true;
+  });
+}
+
+int imops_image_set_exposure_metadata(imops_image* img, float baseline_ev, float f_number, float iso) {
+  return guarded([&] {
+    if (!img) throw Error{IMOPS_E_INVALID, "set_exposure_metadata: null image"};
+    img->baseline_ev = baseline_ev;
+    img->f_number = f_number;
+    img->iso = iso;
+    img->has_baseline = img->has_f_number = img->has_iso = true;
+  });
+}
+
+int imops_image_set_opcode_list3(imops_image* img, const uint8_t* bytes, size_t len) {
+  return guarded([&] {
+    if (!img || (!bytes && len)) throw Error{IMOPS_E_INVALID, "set_opcode_list3: null argument"};
+    img->opcode_list3.assign(bytes, bytes + len);
+    img->has_opcodes = true;
+  });
+}
+
+int imops_contrast(imops_ctx* ctx, imops_image* img, float c) {
+  return run_program(ctx, img, "Contrast", [&](PointwiseProgram& p) { push_contrast(p, c); });
+}
+
+int imops_highlight_reconstruction(imops_ctx* ctx, imops_image* img) {
+  return run_program(ctx, img, "HighlightReconstruction", [&](PointwiseProgram& p) {
+    if (!img->has_wb) throw Error{IMOPS_E_STATE, "wb_coeffs needs to be set to perform highlight_reconstruction"};
+    push_highlight(p, img->wb);
+  });
+}
+
+int imops_lch(imops_ctx* ctx, imops_image* img, float lc, float cc, float hc) {
+  return run_program(ctx, img, "LCH", [&](PointwiseProgram& p) { push_lch(p, img->color_space, lc, cc, hc); });
+}
+
+int imops_sigmoid_tone_map(imops_ctx* ctx, imops_image* img) {
+  return run_program(ctx, img, "SigmoidToneMap", [&](PointwiseProgram& p) { push_sigmoid(p); });
+}
+
+int imops_baseline_exposure(imops_ctx* ctx, imops_image* img) {
+  return run_program(ctx, img, "BaselineExposureCompensation",
+                     [&](PointwiseProgram& p) { push_baseline_exposure(p, img); });
+}
+
+int imops_vignette(imops_ctx* ctx, imops_image* img, float strength) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "Vignette: null argument"};
+    stage_vignette(ctx, img, strength);
   });
 }
 

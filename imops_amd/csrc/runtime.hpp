@@ -7,6 +7,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -87,6 +88,10 @@ struct imops_ctx {
   // new shape times every candidate and keeps the fastest (by key).
   bool autotune = false;
   std::map<std::string, int> tuned;
+  // Images created on this context and not yet freed: destroying the context
+  // releases their device buffers and detaches them, so a later
+  // imops_image_free only deletes the handle.
+  std::set<imops_image*> images;
 };
 
 namespace imops {
@@ -120,6 +125,11 @@ struct imops_image {
   int color_space = IMOPS_CS_NONE;
   float shutter = 0.0f;
   bool has_shutter = false;
+  // exposure metadata of BaselineExposureCompensation (ImageMetadata baseline_exposure, f_number, iso)
+  float baseline_ev = 0.0f, f_number = 0.0f, iso = 0.0f;
+  bool has_baseline = false, has_f_number = false, has_iso = false;
+  std::vector<uint8_t> opcode_list3;  // DNG OpcodeList3 bytes (Vignette)
+  bool has_opcodes = false;
   void swap_rgb_tmp() { std::swap(rgb, tmp); }
 };
 
@@ -135,6 +145,16 @@ int tuned_choice(imops_ctx* ctx, const std::string& key, const std::vector<int>&
 // Stage implementations shared by the per-stage C calls and the pipeline runner.
 void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);
 void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& prog);
+// Display-tail program builders (each adds at most kMaxOpsPerStage ops).
+constexpr int kMaxOpsPerStage = 6;
+void push_contrast(PointwiseProgram& prog, float c);
+void push_highlight(PointwiseProgram& prog, const float wb[4]);
+void push_lch(PointwiseProgram& prog, int color_space, float lc, float cc, float hc);
+void push_sigmoid(PointwiseProgram& prog);
+// BaselineExposureCompensation: ev scale then the absolute-luminance gain, from the image's metadata.
+void push_baseline_exposure(PointwiseProgram& prog, const imops_image* img);
+// Vignette (vignette.rs:3-99) from the image's OpcodeList3; no opcodes: nothing to do.
+void stage_vignette(imops_ctx* ctx, imops_image* img, float strength);
 // Append the ops of one colour conversion / stage to a pointwise program.
 void push_convert(PointwiseProgram& prog, int src, int dst);
 void push_camera_cst(PointwiseProgram& prog, const float* calib, size_t len, int target);

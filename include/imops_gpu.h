@@ -120,6 +120,10 @@ int imops_image_rgb_device_ptr(const imops_image* img, float** out);
 int imops_image_set_wb_coeffs(imops_image* img, const float wb[4]);
 int imops_image_set_calibration_matrix(imops_image* img, const float* m, size_t len);
 int imops_image_set_shutter_seconds(imops_image* img, float seconds);
+/* ImageMetadata baseline_exposure, f_number and iso (BaselineExposureCompensation). */
+int imops_image_set_exposure_metadata(imops_image* img, float baseline_ev, float f_number, float iso);
+/* ImageMetadata opcode_list3: the DNG OpcodeList3 bytes, big-endian as stored (Vignette). */
+int imops_image_set_opcode_list3(imops_image* img, const uint8_t* bytes, size_t len);
 
 /* Stages ---------------------------------------------------------------- */
 
@@ -188,6 +192,26 @@ int imops_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
  * as guide, box means as the reference's running sums; bit-exact with the
  * reference's f32 arithmetic. */
 int imops_luma_chroma_denoise(imops_ctx* ctx, imops_image* img, int radius, float epsilon);
+
+/* Display tail: elementwise stages, fused with neighbouring elementwise stages
+ * by the stage runner.  The Oklch conversions restate Oklab from Ottosson's
+ * published matrices through linear sRGB, as the colour crate converts. */
+/* Image::contrast (pixel.rs:107-110; contrast.rs:4-8): 0.185 (x / 0.185)^c. */
+int imops_contrast(imops_ctx* ctx, imops_image* img, float c);
+/* Image::highlight_reconstruction (pixel.rs:111-114; highlight_reconstruction.rs:5-14);
+ * needs the image's white balance (green clip = wb[1]). */
+int imops_highlight_reconstruction(imops_ctx* ctx, imops_image* img);
+/* Image::lch (pixel.rs:60-68; lch.rs:5-21): Oklch L, C, hue scaled in the image's colour space. */
+int imops_lch(imops_ctx* ctx, imops_image* img, float lc, float cc, float hc);
+/* Image::sigmoid_tone_map (pixel.rs:91-94; tone_map.rs:7-170): ACES reference gamut
+ * compression, square sigmoid, chroma roll-off by luminance^5, on ACEScg. */
+int imops_sigmoid_tone_map(imops_ctx* ctx, imops_image* img);
+/* BaselineExposureCompensation (modules.rs:332-371): 2^baseline_ev, then the
+ * absolute-luminance gain 12.5 N^2 / (t S) when shutter, f-number and ISO are set. */
+int imops_baseline_exposure(imops_ctx* ctx, imops_image* img);
+/* Image::vignette (pixel.rs:149-160; vignette.rs:3-99): FixVignetteRadial opcodes of
+ * the image's OpcodeList3 (none set: no change). */
+int imops_vignette(imops_ctx* ctx, imops_image* img, float strength);
 
 /* Film simulation (film::process, pichromatic/src/film/mod.rs:77-129) on an
  * absolute-luminance ACEScg image; output ACEScg.  Module<Film>

@@ -14,6 +14,9 @@ Functions and the reference code they restate:
 * ``bm3d``         pichromatic/src/bm3d.rs:1-699 (f32 exact order; fp64 variant); ``chroma_bm3d`` bm3d.rs:705-759
 * ``bm3d_test_image`` / ``refined_sum``  the fixture and checksum of bm3d.rs:765-789
 * ``box_filter``, ``luma_chroma_denoise``  pichromatic/src/chroma_denoise.rs:14-152 (f32 exact order)
+* ``contrast``, ``highlight``, ``lch``, ``sigmoid``, ``baseline``, ``vignette``  contrast.rs, highlight_reconstruction.rs,
+  lch.rs, tone_map.rs, modules.rs:335-371, vignette.rs (Oklab/Oklch of the colour crate: parity unpinned beyond
+  the published Oklab values)
 """
 from __future__ import annotations
 
@@ -46,6 +49,15 @@ _SIGS = {
     "oracle_bm3d_test_image": (C.c_int, [_fp, _sz, _sz]),
     "oracle_refined_sum": (C.c_double, [_fp, _sz, C.c_int]),
     "oracle_box_filter": (C.c_int, [_fp, _fp, _sz, _sz, _sz]),
+    "oracle_contrast": (C.c_int, [_fp, _sz, C.c_float]),
+    "oracle_highlight": (C.c_int, [_fp, _sz, C.c_float]),
+    "oracle_lch": (C.c_int, [_fp, _sz, C.c_int, C.c_float, C.c_float, C.c_float]),
+    "oracle_sigmoid": (C.c_int, [_fp, _sz]),
+    "oracle_baseline": (C.c_int, [_fp, _sz, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "oracle_vignette": (C.c_int, [_fp, _sz, _sz, C.c_void_p, _sz, C.c_float]),
+    "oracle_oklch": (C.c_int, [_fp, _fp]),
+    "oracle_rgc": (C.c_int, [_fp, _fp]),
+    "oracle_rgc_curve": (C.c_float, [C.c_float, C.c_int]),
     "oracle_luma_chroma_denoise": (C.c_int, [_fp, _sz, _sz, _sz, C.c_float]),
 }
 
@@ -188,6 +200,54 @@ def luma_chroma_denoise(rgb: np.ndarray, radius: int, epsilon: float) -> np.ndar
     out = np.array(rgb, np.float32, copy=True, order="C")
     h, w, _ = out.shape
     _rc(lib().oracle_luma_chroma_denoise(_f(out), w, h, radius, epsilon), "luma_chroma_denoise")
+    return out
+
+
+def _rgb_op(rgb, fn, *args):
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    _rc(fn(_f(out), out.size // 3, *args), "display stage")
+    return out
+
+
+def contrast(rgb, c):
+    return _rgb_op(rgb, lib().oracle_contrast, c)
+
+
+def highlight(rgb, clip_g):
+    return _rgb_op(rgb, lib().oracle_highlight, clip_g)
+
+
+def lch(rgb, cs, lc, cc, hc):
+    return _rgb_op(rgb, lib().oracle_lch, cs, lc, cc, hc)
+
+
+def sigmoid(rgb):
+    return _rgb_op(rgb, lib().oracle_sigmoid)
+
+
+def baseline(rgb, ev, shutter, f_number, iso):
+    return _rgb_op(rgb, lib().oracle_baseline, ev, shutter, f_number, iso)
+
+
+def vignette(rgb, opcodes: bytes, strength):
+    out = np.array(rgb, np.float32, copy=True, order="C")
+    h, w, _ = out.shape
+    buf = C.create_string_buffer(bytes(opcodes), len(opcodes))
+    _rc(lib().oracle_vignette(_f(out), w, h, C.cast(buf, C.c_void_p), len(opcodes), strength), "vignette")
+    return out
+
+
+def oklch(lin):
+    a = np.ascontiguousarray(lin, np.float32)
+    out = np.empty(3, np.float32)
+    lib().oracle_oklch(_f(a), _f(out))
+    return out
+
+
+def rgc(rgb):
+    a = np.ascontiguousarray(rgb, np.float32)
+    out = np.empty(3, np.float32)
+    lib().oracle_rgc(_f(a), _f(out))
     return out
 
 

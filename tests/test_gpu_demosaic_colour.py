@@ -147,5 +147,18 @@ def test_stage_errors(gpu_ctx):
     with pytest.raises(im.ImopsError, match="calibration matrix"):
         img.cst("Srgb")
     cfg = im.parse_config('[[pipeline_modules]]\nname = "LCH"\n')
-    with pytest.raises(im.ImopsError, match="not implemented"):
+    with pytest.raises(im.ImopsError, match="colour space"):
         im.run_pixel_pipeline(img, cfg)
+    cfg = im.parse_config('[[pipeline_modules]]\nname = "Demosaic"\nalgorithm = "fast"\n')
+    with pytest.raises(im.ImopsError, match="not on the GPU path"):
+        im.run_pixel_pipeline(img, cfg)
+
+
+def test_image_outliving_its_context():
+    # freeing an image after its context is destroyed only deletes the handle
+    ctx = im.Context(0)
+    img = im.Image(ctx).set_rgb(np.ones((8, 8, 3), np.float32), im.CS_NONE)
+    ctx.close()
+    with pytest.raises(im.ImopsError):
+        img.set_rgb(np.ones((8, 8, 3), np.float32), im.CS_NONE)
+    img.close()
