@@ -177,6 +177,22 @@ class Image:
         check(lib().imops_image_get_rgb_f32(self.ctx.handle, self._h, out.ctypes.data_as(C.c_void_p), 0, out.size))
         return out
 
+    def _encoded(self, fn, dtype, orientation: int) -> np.ndarray:
+        w, h, _, _ = self.info()
+        shape = (w, h, 3) if orientation in (1, 3) else (h, w, 3)
+        out = np.empty(shape, dtype)
+        check(fn(self.ctx.handle, self._h, out.ctypes.data_as(C.c_void_p), 0, int(orientation), out.size))
+        return out
+
+    def rgb_u8(self, orientation: int = 0) -> np.ndarray:
+        """8-bit RGB as get_image_rgb_data_c encodes it (extern_pipeline.rs:362-395), oriented
+        (0 none, 1 Rotate90, 2 Rotate180, 3 Rotate270; output.rs:31-70)."""
+        return self._encoded(lib().imops_image_get_rgb_u8, np.uint8, orientation)
+
+    def rgb_f16(self, orientation: int = 0) -> np.ndarray:
+        """Half-float RGB as save_exr writes it (output.rs:201-209), oriented as rgb_u8."""
+        return self._encoded(lib().imops_image_get_rgb_f16, np.uint16, orientation).view(np.float16)
+
     def rgb_device_ptr(self) -> int:
         p = C.c_void_p()
         check(lib().imops_image_rgb_device_ptr(self._h, C.byref(p)))

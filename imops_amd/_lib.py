@@ -63,6 +63,8 @@ SIGNATURES = {
     "imops_image_set_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _i]),
     "imops_image_get_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz]),
     "imops_image_get_info": (_i, [_vp, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_i), C.POINTER(_i)]),
+    "imops_image_get_rgb_u8": (_i, [_vp, _vp, _vp, _i, _i, _sz]),
+    "imops_image_get_rgb_f16": (_i, [_vp, _vp, _vp, _i, _i, _sz]),
     "imops_image_rgb_device_ptr": (_i, [_vp, C.POINTER(_vp)]),
     "imops_image_set_wb_coeffs": (_i, [_vp, _fp]),
     "imops_image_set_calibration_matrix": (_i, [_vp, _fp, _sz]),

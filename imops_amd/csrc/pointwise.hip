@@ -234,6 +234,43 @@ __global__ void vignette_kernel(float* __restrict__ rgb, int width, int height, 
   }
 }
 
+// output pixel of input (x, y) under the orientation; the output is w x h, or h x w when rotated by 90/270
+__device__ __forceinline__ size_t oriented_index(int x, int y, int w, int h, int orientation) {
+  switch (orientation) {
+    case 1: return (size_t)x * h + (h - 1 - y);              // Rotate90: (x, y) -> (h-1-y, x)
+    case 2: return (size_t)(h - 1 - y) * w + (w - 1 - x);    // Rotate180
+    case 3: return (size_t)(w - 1 - x) * h + y;              // Rotate270: (x, y) -> (y, w-1-x)
+    default: return (size_t)y * w + x;
+  }
+}
+
+__global__ void encode_u8_kernel(const float* __restrict__ rgb, int w, int h, int orientation,
+                                 uint8_t* __restrict__ out) {
+  const size_t n = (size_t)w * h;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t o = oriented_index((int)(i % w), (int)(i / w), w, h, orientation);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const float v = rgb[i * 3 + c];
+      const float q = (v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v)) * 255.0f;  // NaN stays NaN
+      out[o * 3 + c] = q == q ? (uint8_t)(int)q : (uint8_t)0;              // `as u8`: NaN -> 0, truncation
+    }
+  }
+}
+
+__global__ void encode_f16_kernel(const float* __restrict__ rgb, int w, int h, int orientation,
+                                  uint16_t* __restrict__ out) {
+  const size_t n = (size_t)w * h;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t o = oriented_index((int)(i % w), (int)(i / w), w, h, orientation);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const _Float16 hv = (_Float16)rgb[i * 3 + c];  // v_cvt_f16_f32: round to nearest even
+      out[o * 3 + c] = __builtin_bit_cast(uint16_t, hv);
+    }
+  }
+}
+
 int grid_for(size_t n, int block) {
   size_t g = (n + block - 1) / block;
   if (g > 65536) g = 65536;
@@ -260,6 +297,18 @@ hipError_t markesteijn_launch(const float* in, int width, int height, const CfaP
 hipError_t vignette_launch(float* rgb, int width, int height, const VignetteParams& v, hipStream_t s) {
   hipLaunchKernelGGL(vignette_kernel, dim3(grid_for((size_t)width * height, 256)), dim3(256), 0, s, rgb, width, height,
                      v);
+  return hipGetLastError();
+}
+
+hipError_t encode_u8_launch(const float* rgb, int width, int height, int orientation, uint8_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(encode_u8_kernel, dim3(grid_for((size_t)width * height, 256)), dim3(256), 0, s, rgb, width, height,
+                     orientation, out);
+  return hipGetLastError();
+}
+
+hipError_t encode_f16_launch(const float* rgb, int width, int height, int orientation, uint16_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(encode_f16_kernel, dim3(grid_for((size_t)width * height, 256)), dim3(256), 0, s, rgb, width, height,
+                     orientation, out);
   return hipGetLastError();
 }
 

@@ -53,4 +53,11 @@ struct VignetteParams {
 };
 hipError_t vignette_launch(float* rgb, int width, int height, const VignetteParams& v, hipStream_t s);
 
+// Output encodings of interleaved RGB, with the EXIF/DNG orientation applied
+// (0 none, 1 Rotate90, 2 Rotate180, 3 Rotate270; apply_orientation, output.rs:31-70):
+// u8 = (clamp(c, 0, 1) * 255) truncated (get_image_rgb_data_c, extern_pipeline.rs:362-395),
+// f16 = IEEE binary16 round-to-nearest-even (save_exr's f16::from_f32, output.rs:201-209).
+hipError_t encode_u8_launch(const float* rgb, int width, int height, int orientation, uint8_t* out, hipStream_t s);
+hipError_t encode_f16_launch(const float* rgb, int width, int height, int orientation, uint16_t* out, hipStream_t s);
+
 }  // namespace imops

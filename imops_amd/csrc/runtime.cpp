@@ -452,6 +452,30 @@ int run_program(imops_ctx* ctx, imops_image* img, const char* name, F&& build) {
 }
 }  // namespace imops
 
+namespace imops {
+// Encoded readback: the kernel writes dst directly when it is device memory, else a
+// device staging buffer that is copied out (the encoded bytes only cross the link).
+template <class T, class Launch>
+void get_encoded(imops_ctx* ctx, const imops_image* img, T* dst, int dst_on_device, int orientation, size_t count,
+                 const char* name, Launch launch) {
+  if (!ctx || !img || !dst) throw Error{IMOPS_E_INVALID, std::string(name) + ": null argument"};
+  require_rgb(img, name);
+  if (orientation < 0 || orientation > 3)
+    throw Error{IMOPS_E_INVALID, std::string(name) + ": orientation must be 0 (none), 1 (90), 2 (180) or 3 (270)"};
+  if (count != img->width * img->height * 3)
+    throw Error{IMOPS_E_INVALID, std::string(name) + ": count must be width*height*3"};
+  T* out = dst;
+  if (!dst_on_device) {
+    ctx->scr[SCR_STAGING].reserve(count * sizeof(T));
+    out = ctx->scr[SCR_STAGING].as<T>();
+  }
+  IMOPS_HIP_CHECK(launch(img->rgb.as<float>(), (int)img->width, (int)img->height, orientation, out, ctx->stream));
+  if (!dst_on_device)
+    IMOPS_HIP_CHECK(hipMemcpyAsync(dst, out, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+}  // namespace imops
+
 extern "C" {
 
 const char* imops_last_error(void) { return t_last_error.c_str(); }
@@ -675,6 +699,19 @@ int imops_image_get_rgb_f32(imops_ctx* ctx, const imops_image* img, float* dst, 
                                    dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, ctx->stream));
     IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
+}
+
+
+int imops_image_get_rgb_u8(imops_ctx* ctx, const imops_image* img, uint8_t* dst, int dst_on_device, int orientation,
+                           size_t count) {
+  return guarded(
+      [&] { get_encoded(ctx, img, dst, dst_on_device, orientation, count, "get_rgb_u8", encode_u8_launch); });
+}
+
+int imops_image_get_rgb_f16(imops_ctx* ctx, const imops_image* img, uint16_t* dst, int dst_on_device,
+                            int orientation, size_t count) {
+  return guarded(
+      [&] { get_encoded(ctx, img, dst, dst_on_device, orientation, count, "get_rgb_f16", encode_f16_launch); });
 }
 
 int imops_image_get_info(const imops_image* img, size_t* w, size_t* h, int* cs, int* has_rgb) {

@@ -112,6 +112,16 @@ int imops_image_set_rgb_f32(imops_ctx* ctx, imops_image* img, const float* src, 
 /* Copy the interleaved RGB f32 out (count = width*height*3 floats). */
 int imops_image_get_rgb_f32(imops_ctx* ctx, const imops_image* img, float* dst, int dst_on_device, size_t count);
 int imops_image_get_info(const imops_image* img, size_t* width, size_t* height, int* color_space, int* has_rgb);
+/* Encoded readback (count = width*height*3 values), encoded on the device so only the
+ * encoded bytes cross the link; orientation 0 none, 1 Rotate90, 2 Rotate180,
+ * 3 Rotate270 (apply_orientation, src/output.rs:31-70: rotated frames are height x width).
+ * u8: (clamp(c, 0, 1) * 255) truncated, NaN -> 0 (get_image_rgb_data_c,
+ * pichromatic-pipeline/src/extern_pipeline.rs:362-395; save_jpeg, src/output.rs:221-233).
+ * f16: IEEE binary16 bits, round to nearest even (save_exr's f16::from_f32, src/output.rs:201-209). */
+int imops_image_get_rgb_u8(imops_ctx* ctx, const imops_image* img, uint8_t* dst, int dst_on_device, int orientation,
+                           size_t count);
+int imops_image_get_rgb_f16(imops_ctx* ctx, const imops_image* img, uint16_t* dst, int dst_on_device,
+                            int orientation, size_t count);
 /* Device address of the interleaved RGB buffer (valid until the next stage call). */
 int imops_image_rgb_device_ptr(const imops_image* img, float** out);
 

@@ -13,6 +13,8 @@ Functions and the reference code they restate:
 * ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
 * ``bm3d``         pichromatic/src/bm3d.rs:1-699 (f32 exact order; fp64 variant); ``chroma_bm3d`` bm3d.rs:705-759
 * ``bm3d_test_image`` / ``refined_sum``  the fixture and checksum of bm3d.rs:765-789
+* ``encode_u8``, ``encode_f16``, ``orient``  extern_pipeline.rs:362-395 (u8), output.rs:201-209 (f16),
+  output.rs:31-70 (apply_orientation); numpy, integer/IEEE conversions
 * ``box_filter``, ``luma_chroma_denoise``  pichromatic/src/chroma_denoise.rs:14-152 (f32 exact order)
 * ``contrast``, ``highlight``, ``lch``, ``sigmoid``, ``baseline``, ``vignette``  contrast.rs, highlight_reconstruction.rs,
   lch.rs, tone_map.rs, modules.rs:335-371, vignette.rs (Oklab/Oklch of the colour crate: parity unpinned beyond
@@ -201,6 +203,37 @@ def luma_chroma_denoise(rgb: np.ndarray, radius: int, epsilon: float) -> np.ndar
     h, w, _ = out.shape
     _rc(lib().oracle_luma_chroma_denoise(_f(out), w, h, radius, epsilon), "luma_chroma_denoise")
     return out
+
+
+def orient(rgb: np.ndarray, orientation: int) -> np.ndarray:
+    """apply_orientation (output.rs:31-70) by its index maps: Rotate90 sends (x, y) to
+    (h-1-y, x) of an h-wide frame, Rotate180 to (w-1-x, h-1-y), Rotate270 to (y, w-1-x)."""
+    h, w, _ = rgb.shape
+    if orientation == 0:
+        return rgb.copy()
+    y, x = np.mgrid[0:h, 0:w]
+    if orientation == 2:
+        out = np.empty_like(rgb)
+        out[h - 1 - y, w - 1 - x] = rgb[y, x]
+        return out
+    out = np.empty((w, h, 3), rgb.dtype)  # rotated: h wide, w tall
+    if orientation == 1:
+        out[x, h - 1 - y] = rgb[y, x]
+    else:
+        out[w - 1 - x, y] = rgb[y, x]
+    return out
+
+
+def encode_u8(rgb: np.ndarray) -> np.ndarray:
+    """(clamp(c, 0, 1) * 255) as u8: f32 multiply, truncation, NaN -> 0 (extern_pipeline.rs:381-383)."""
+    a = np.asarray(rgb, np.float32)
+    q = np.clip(a, np.float32(0), np.float32(1)) * np.float32(255)
+    return np.where(np.isnan(q), 0, np.trunc(np.nan_to_num(q))).astype(np.uint8)
+
+
+def encode_f16(rgb: np.ndarray) -> np.ndarray:
+    """f16::from_f32: IEEE binary16, round to nearest even, overflow to infinity."""
+    return np.asarray(rgb, np.float32).astype(np.float16)
 
 
 def _rgb_op(rgb, fn, *args):
