@@ -24,6 +24,9 @@ CS_XYZD65 = 3
 
 DEMOSAIC_AMAZE = 0
 DEMOSAIC_MARKESTEIJN = 1
+DEMOSAIC_FAST = 2
+DEMOSAIC_SUPERFAST = 3
+DEMOSAIC_SUPERSUPERFAST = 4
 BM3D_FP32 = 0
 BM3D_MIXED = 1
 BM3D_MIXED_MATCH = 2

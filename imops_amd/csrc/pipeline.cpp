@@ -330,7 +330,9 @@ Stage make_stage(const Table& t) {
     std::string a = str_param(t, "algorithm", "amaze", "Demosaic");  // rename_all = lowercase
     if (a == "amaze") s.algorithm = IMOPS_DEMOSAIC_AMAZE;
     else if (a == "markesteijn") s.algorithm = IMOPS_DEMOSAIC_MARKESTEIJN;
-    else if (a == "fast" || a == "superfast" || a == "supersuperfast") s.algorithm = -1;
+    else if (a == "fast") s.algorithm = IMOPS_DEMOSAIC_FAST;
+    else if (a == "superfast") s.algorithm = IMOPS_DEMOSAIC_SUPERFAST;
+    else if (a == "supersuperfast") s.algorithm = IMOPS_DEMOSAIC_SUPERSUPERFAST;
     else throw Error{IMOPS_E_INVALID, "Invalid Demosaic config"};
   } else if (s.name == "CFACoeffs") {
     s.kind = K_CFACOEFFS;
@@ -453,10 +455,7 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
       continue;
     }
     switch (s.kind) {
-      case K_DEMOSAIC:
-        if (s.algorithm < 0) throw Error{IMOPS_E_UNSUPPORTED, "Demosaic preview modes (fast/superfast) are not on the GPU path"};
-        stage_demosaic(ctx, img, s.algorithm);
-        break;
+      case K_DEMOSAIC: stage_demosaic(ctx, img, s.algorithm); break;
       case K_BM3D: stage_bm3d(ctx, img, s.bm3d); break;
       case K_CHROMA_BM3D: stage_chroma_bm3d(ctx, img, (float)s.intensity); break;
       case K_LUMA_CHROMA: stage_luma_chroma_denoise(ctx, img, s.radius, (float)s.epsilon); break;

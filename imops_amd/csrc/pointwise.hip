@@ -53,6 +53,26 @@ __global__ void markesteijn_kernel(const float* __restrict__ in, int width, int 
   }
 }
 
+__global__ void preview_demosaic_kernel(const float* __restrict__ in, int width, int height, int f, int c00, int c01,
+                                        int c10, int c11, float* __restrict__ out) {
+  const int nw = width / f, nh = height / f;
+  const size_t n = (size_t)nw * nh;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = (k / nw) * f, c = (k % nw) * f;
+    const size_t tl = r * width + c;
+    const float s00 = in[tl], s01 = in[tl + 1], s10 = in[tl + width], s11 = in[tl + width + 1];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {  // the sites are written in order: the last one of a colour wins
+      float v = 0.0f;
+      if (c00 == ch) v = s00;
+      if (c01 == ch) v = s01;
+      if (c10 == ch) v = s10;
+      if (c11 == ch) v = s11;
+      out[k * 3 + ch] = v;
+    }
+  }
+}
+
 __device__ __forceinline__ float oetf(float x) {
   const float ax = fabsf(x);
   if (ax > 0.0031308f) {
@@ -284,6 +304,14 @@ hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_
   const float range = white - black;
   hipLaunchKernelGGL(normalize_kernel, dim3(grid_for(cw * ch, 256)), dim3(256), 0, s, raw, full_w, cx, cy, cw, ch,
                      black, range, out);
+  return hipGetLastError();
+}
+
+hipError_t preview_demosaic_launch(const float* in, int width, int height, int factor, const int cell[4], float* out,
+                                   hipStream_t s) {
+  const size_t n = (size_t)(width / factor) * (height / factor);
+  hipLaunchKernelGGL(preview_demosaic_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, in, width, height, factor,
+                     cell[0], cell[1], cell[2], cell[3], out);
   return hipGetLastError();
 }
 

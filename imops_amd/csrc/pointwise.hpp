@@ -43,6 +43,10 @@ struct PointwiseProgram {
 
 hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_t cy, size_t cw, size_t ch,
                             float black, float white, float* out, hipStream_t s);
+// Fast / SuperFast / SuperSuperFast (demosaic.rs:189-354): output (w/f) x (h/f); cell[k] is the
+// colour of CFA site (k/2, k%2), written in that order (a repeated colour keeps the later site).
+hipError_t preview_demosaic_launch(const float* in, int width, int height, int factor, const int cell[4], float* out,
+                                   hipStream_t s);
 hipError_t markesteijn_launch(const float* in, int width, int height, const CfaPattern& cfa, float* out,
                               hipStream_t s);
 hipError_t pointwise_launch(float* rgb, size_t npix, const PointwiseProgram& prog, hipStream_t s);

@@ -427,8 +427,23 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
       for (int c = 0; c < 12; ++c) p.base[r][c] = cfa.pat[r][c];
     TimedRegion tr(ctx, "markesteijn");
     IMOPS_HIP_CHECK(markesteijn_launch(img->raw.as<float>(), (int)w, (int)h, p, img->rgb.as<float>(), ctx->stream));
+  } else if (algorithm >= IMOPS_DEMOSAIC_FAST && algorithm <= IMOPS_DEMOSAIC_SUPERSUPERFAST) {
+    const int f = algorithm == IMOPS_DEMOSAIC_FAST ? 2 : (algorithm == IMOPS_DEMOSAIC_SUPERFAST ? 4 : 8);
+    const size_t nw = w / f, nh = h / f;
+    if (nw == 0 || nh == 0) throw Error{IMOPS_E_UNSUPPORTED, "Demosaic preview: frame smaller than its block"};
+    int cell[4];
+    for (int k = 0; k < 4; ++k) {
+      cell[k] = cfa.pat[k / 2][k % 2];
+      if (cell[k] > 2) throw Error{IMOPS_E_UNSUPPORTED, "Demosaic preview needs an RGB pattern, got " + cfa.name};
+    }
+    TimedRegion tr(ctx, "preview_demosaic");
+    IMOPS_HIP_CHECK(preview_demosaic_launch(img->raw.as<float>(), (int)w, (int)h, f, cell, img->rgb.as<float>(),
+                                            ctx->stream));
+    img->width = nw;
+    img->height = nh;
+    img->has_raw = false;  // the raw plane keeps the old size
   } else {
-    throw Error{IMOPS_E_UNSUPPORTED, "demosaic algorithm not on the GPU path (fast/superfast preview modes)"};
+    throw Error{IMOPS_E_INVALID, "unknown demosaic algorithm"};
   }
   img->has_rgb = true;
   img->color_space = IMOPS_CS_NONE;  // modules.rs:666

@@ -51,6 +51,12 @@ extern "C" {
 /* Demosaic algorithms (DemosaicAlgorithmType, modules.rs:73-100). */
 #define IMOPS_DEMOSAIC_AMAZE 0
 #define IMOPS_DEMOSAIC_MARKESTEIJN 1
+/* Preview modes (demosaic.rs:189-354): the top-left 2x2 CFA cell of every 2x2,
+ * 4x4 or 8x8 block gives one RGB pixel, so the frame shrinks by 2, 4 or 8 per
+ * side; the raw plane no longer matches the new size and is dropped. */
+#define IMOPS_DEMOSAIC_FAST 2
+#define IMOPS_DEMOSAIC_SUPERFAST 3
+#define IMOPS_DEMOSAIC_SUPERSUPERFAST 4
 
 typedef struct imops_ctx imops_ctx;
 typedef struct imops_image imops_image;

@@ -7,7 +7,7 @@ baseline.  The product library never links or calls it.
 Functions and the reference code they restate:
 
 * ``amaze``        pichromatic/src/demosaic.rs:358-1379 (f32 exact order; fp64 variant for error statements)
-* ``markesteijn``  pichromatic/src/demosaic.rs:108-186
+* ``markesteijn``  pichromatic/src/demosaic.rs:108-186; ``preview_demosaic`` demosaic.rs:189-354 (Fast/SuperFast/SuperSuperFast)
 * ``normalize_raw`` demosaic.rs:61-91 + (DN-black)/(white-black) scaling (rawler apply_scaling: parity unpinned)
 * ``camera_cst``   pichromatic/src/cst.rs:12-95 (colour-crate matrices: parity unpinned)
 * ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
@@ -36,6 +36,7 @@ _lib = None
 _SIGS = {
     "oracle_amaze": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp, C.c_int]),
     "oracle_markesteijn": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp]),
+    "oracle_preview_demosaic": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, C.c_int, _fp]),
     "oracle_normalize_raw": (C.c_int, [C.c_void_p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_float, C.c_float, _fp]),
     "oracle_camera_forward_matrix": (C.c_int, [_fp, _sz, _fp]),
     "oracle_camera_cst": (C.c_int, [_fp, _sz, _fp, _sz, C.c_int]),
@@ -281,6 +282,14 @@ def rgc(rgb):
     a = np.ascontiguousarray(rgb, np.float32)
     out = np.empty(3, np.float32)
     lib().oracle_rgc(_f(a), _f(out))
+    return out
+
+
+def preview_demosaic(plane: np.ndarray, cfa: str, factor: int, crop=(0, 0)) -> np.ndarray:
+    a = np.ascontiguousarray(plane, np.float32)
+    h, w = a.shape
+    out = np.empty((h // factor, w // factor, 3), np.float32)
+    _rc(lib().oracle_preview_demosaic(_f(a), w, h, cfa.encode(), crop[0], crop[1], factor, _f(out)), "preview_demosaic")
     return out
 
 

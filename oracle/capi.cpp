@@ -32,6 +32,27 @@ int oracle_markesteijn(const float* in, size_t w, size_t h, const char* cfa_patt
   return 0;
 }
 
+// Fast / SuperFast / SuperSuperFast (demosaic.rs:189-354): factor 2, 4 or 8; out (w/f) x (h/f) x 3.
+int oracle_preview_demosaic(const float* in, size_t w, size_t h, const char* cfa_pattern, size_t crop_x,
+                            size_t crop_y, int factor, float* out) {
+  Cfa base;
+  if (!base.parse(cfa_pattern)) return -1;
+  Cfa cfa = base.shift(crop_x, crop_y);
+  const int c00 = cfa.color_at(0, 0), c01 = cfa.color_at(0, 1), c10 = cfa.color_at(1, 0), c11 = cfa.color_at(1, 1);
+  if (c00 > 2 || c01 > 2 || c10 > 2 || c11 > 2) return -1;  // the reference indexes a 3-channel pixel
+  const size_t nw = w / factor, nh = h / factor;
+  for (size_t k = 0; k < nw * nh; ++k) {
+    const size_t r = (k / nw) * factor, c = (k % nw) * factor, tl = r * w + c;
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    v[c00] = in[tl];
+    v[c01] = in[tl + 1];
+    v[c10] = in[tl + w];
+    v[c11] = in[tl + w + 1];
+    for (int ch = 0; ch < 3; ++ch) out[k * 3 + ch] = v[ch];
+  }
+  return 0;
+}
+
 // u16 raw -> normalised f32 CFA plane of the crop window ((DN-black)/(white-black)).
 int oracle_normalize_raw(const uint16_t* raw, size_t full_w, size_t full_h, size_t cx, size_t cy, size_t cw,
                          size_t ch, float black, float white, float* out) {

@@ -86,6 +86,24 @@ def test_markesteijn_bit_exact(gpu_ctx, cfa, w, h):
     assert _mismatch(img.rgb(), want) == 0
 
 
+@pytest.mark.parametrize("algorithm,factor", [("fast", 2), ("superfast", 4), ("supersuperfast", 8)])
+@pytest.mark.parametrize("cfa,w,h", [("RGGB", 257, 131), ("BGGR", 64, 48), (XTRANS, 99, 50)])
+def test_preview_demosaic_bit_exact(gpu_ctx, algorithm, factor, cfa, w, h):
+    _, img, plane = _ingest(gpu_ctx, w, h, cfa, frame=7)
+    img.demosaic(algorithm)
+    want = oracle.preview_demosaic(plane, cfa, factor)
+    assert img.info()[:2] == (w // factor, h // factor)
+    assert _mismatch(img.rgb(), want) == 0
+
+
+def test_preview_demosaic_crop_and_pipeline(gpu_ctx):
+    raw = im.synth_raw_u16(120, 90, "RGGB", 11)
+    img = im.Image(gpu_ctx).set_raw_u16(raw, "RGGB", BLACK, WHITE, crop=(3, 2, 100, 80))
+    im.run_pixel_pipeline(img, im.parse_config('[[pipeline_modules]]\nname = "Demosaic"\nalgorithm = "fast"\n'))
+    plane = oracle.normalize_raw(raw, BLACK, WHITE, crop=(3, 2, 100, 80))
+    assert _mismatch(img.rgb(), oracle.preview_demosaic(plane, "RGGB", 2, crop=(3, 2))) == 0
+
+
 def test_ingest_device_buffer_and_crop(gpu_ctx):
     import ctypes
     raw = im.synth_raw_u16(200, 100, "RGGB", 9)
@@ -150,7 +168,7 @@ def test_stage_errors(gpu_ctx):
     with pytest.raises(im.ImopsError, match="colour space"):
         im.run_pixel_pipeline(img, cfg)
     cfg = im.parse_config('[[pipeline_modules]]\nname = "Demosaic"\nalgorithm = "fast"\n')
-    with pytest.raises(im.ImopsError, match="not on the GPU path"):
+    with pytest.raises(im.ImopsError, match="no raw CFA data"):
         im.run_pixel_pipeline(img, cfg)
 
 

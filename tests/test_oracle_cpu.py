@@ -185,3 +185,20 @@ def test_bm3d_flat_image_is_fixed_point():
     x = np.full((40, 40, 3), 0.5, np.float32)
     out, _ = oracle.bm3d(x, oracle.bm3d_params(0.3))
     assert np.abs(out - 0.5).max() < 1e-4
+
+
+def test_preview_demosaic_reference_uniform_and_sites():
+    # demosaic.rs:1389-1407: a uniform 16x16 RGGB frame halves to 8x8 of that value
+    out = oracle.preview_demosaic(np.full((16, 16), 0.5, np.float32), "RGGB", 2)
+    assert out.shape == (8, 8, 3) and np.all(out == np.float32(0.5))
+    # the top-left 2x2 cell of each block: RGGB -> R = (0,0), G = the later green site (1,0), B = (1,1)
+    a = np.arange(64, dtype=np.float32).reshape(8, 8)
+    for f in (2, 4, 8):
+        o = oracle.preview_demosaic(a, "RGGB", f)
+        for by in range(8 // f):
+            for bx in range(8 // f):
+                r, c = by * f, bx * f
+                assert o[by, bx].tolist() == [a[r, c], a[r + 1, c], a[r + 1, c + 1]]
+    # a crop shifted by one column reads the GRBG cell: R = (0,1), G = (1,1), B = (1,0)
+    o = oracle.preview_demosaic(a, "RGGB", 2, crop=(1, 0))
+    assert o[0, 0].tolist() == [a[0, 1], a[1, 1], a[1, 0]]
