@@ -174,8 +174,16 @@ def film_adjoint_bench(ctx, n=8, size=512, steps=200) -> dict:
     res = fit(model, x, target, nom, steps=steps)
     ev1.record(stream)
     ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    # compulsory HBM bytes of one step: forward reads the images and writes the output and the saved
+    # exposure state (E + 1 planes); backward reads the images, dL/dout and the state
+    E = model.n_emulsions
+    px = n * size * size
+    step_bytes = px * (12 + 12 + 4 * (E + 1)) + px * (12 + 12 + 4 * (E + 1))
     return {"metric": "film-model adjoint ms/iter (forward + hand-written adjoint + Adam step)",
-            "ms_per_iter": round(ev0.elapsed_time(ev1) / steps, 3),
+            "ms_per_iter": round(ms, 3),
+            "compulsory_bytes_per_iter": step_bytes,
+            "achieved_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
             "ms_per_iter_fwd_bwd_host_median": round(res["ms_per_iter"], 3),
             "batch": f"{n}x{size}x{size} ACEScg crops", "steps": steps,
             "loss_first": res["loss"][0], "loss_last": res["loss"][-1],
@@ -281,6 +289,8 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the X-Trans and burst workloads")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain f32 copy kernel of this size first (calibrates HBM byte counters)")
+    ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
+                    help="run only the film-model adjoint workload for STEPS Adam steps and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -291,6 +301,9 @@ def main() -> int:
     import imops_amd as im
 
     ctx = im.Context(0)
+    if args.adjoint_only:
+        print(json.dumps(film_adjoint_bench(ctx, steps=args.adjoint_only)), flush=True)
+        return 0
     ctx.set_autotune(True)  # launch shapes timed during the warm-up steps; results do not depend on them
     if args.copy_probe_mib:
         n = args.copy_probe_mib * (1 << 20) // 4
