@@ -769,14 +769,13 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
       const bool actn = n < g;
       const uint32_t pn = actn ? grp[1 + n] : 0u;
       const int pxn = (int)(pn & 0xffffu), pyn = (int)(pn >> 16);
-      int cpx[4], cpy[4];
+      int cbase[4];  // footprint index of patch 4q + j's top-left pixel
       bool cact[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         cact[j] = 4 * q + j < g;
         const uint32_t pp = cact[j] ? grp[1 + 4 * q + j] : 0u;
-        cpx[j] = (int)(pp & 0xffffu);
-        cpy[j] = (int)(pp >> 16);
+        cbase[j] = ((int)(pp >> 16) - fy0) * fw + ((int)(pp & 0xffffu) - fx0);
       }
       // Hadamard fragment (A of the 16x16x16 product): lane (row n, q) holds H[n][4q .. 4q+3]
       const float hs = 1.0f / sqrtf((float)g);
@@ -834,7 +833,8 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
           for (int j = 0; j < 4; ++j) {
             const bool dc = t == 0 && n == 0 && q == 0 && j == 0;
             const float e = Z[t][j] * Z[t][j];
-            const float coef = e / (e + s2 + 1e-5f);
+            // hardware reciprocal (1 ulp): far below the FP16 operands' rounding
+            const float coef = e * __builtin_amdgcn_rcpf(e + s2 + 1e-5f);
             if (!dc) {
               zn[j] *= coef;
               wsum += coef * coef;
@@ -846,18 +846,23 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
         const float run = 1.0f + wsum;
         weight = run > 0.0f ? 1.0f / run : 1.0f;
       }
-      // inverse Hadamard, transpose to patch rows through LDS, inverse DCT
+      // inverse Hadamard, transpose to patch rows through LDS, inverse DCT.  The
+      // 16 x 64 transpose buffer stores each row's 8-half blocks XOR-swizzled by
+      // 2 (row / 4), so the four rows 4q + j one store writes fall in different
+      // banks; the reads undo the swizzle (a block stays 8 contiguous halves).
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
         const floatx4 y = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(Z[t]), zero, 0, 0, 0);
+        const int blk = ((2 * t + (n >> 3)) ^ (2 * q)) * 8 + (n & 7);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) tbuf[(4 * q + j) * 64 + 16 * t + n] = (_Float16)y[j];
+        for (int j = 0; j < 4; ++j) tbuf[(4 * q + j) * 64 + blk] = (_Float16)y[j];
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      const half8 B0 = load8(tbuf + n * 64 + 8 * q), B1 = load8(tbuf + n * 64 + 32 + 8 * q);
+      const int rsw = 2 * (n >> 2);  // row n's swizzle
+      const half8 B0 = load8(tbuf + n * 64 + 8 * (q ^ rsw)), B1 = load8(tbuf + n * 64 + 8 * ((4 + q) ^ rsw));
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -867,13 +872,13 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
         pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B0, KI(t, 0), pv, 0, 0, 0);
         pv = __builtin_amdgcn_mfma_f32_16x16x32_f16(B1, KI(t, 1), pv, 0, 0, 0);
         // lane (pixel 16t + n, q), register j -> patch 4q + j
-        const int pix = 16 * t + n, dy = pix >> 3, dx = pix & 7;
+        const int pix = 16 * t + n, poff = (pix >> 3) * fw + (pix & 7);
         const float kw = a.kaiser[pix] * weight;
         const long long kwf = (long long)(int)(kw * 100000.0f);  // kw in [0, 1]: exact 32-bit truncation
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           if (!cact[j]) continue;
-          const int fi = (cpy[j] + dy - fy0) * fw + (cpx[j] + dx - fx0);
+          const int fi = cbase[j] + poff;
           atomicAdd(&fnum[fi], (unsigned long long)to_fixed_m((pv[j] + level) * kw));
           atomicAdd(&fden[fi], (uint32_t)kwf);
         }
