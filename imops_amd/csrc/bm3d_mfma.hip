@@ -206,7 +206,10 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     meta[tid].dK = kInf;
     meta[tid].limit = 0x7fffffff;
     meta[tid].g = 1;
+    meta[tid].pad = 0;
   }
+  int* cap_next = reinterpret_cast<int*>(red + 4);  // next reference slot of the cap walk
+  if (tid == 0) *cap_next = 0;
   __syncthreads();
   const float shift[3] = {red[0], red[1], red[2]};
   for (int row = wave; row < SY; row += XW) {
@@ -454,12 +457,18 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   __syncthreads();
   // ---- the cap when candidates before the geometric cap were rejected: walk
   // the per-row rejection counts to the row holding the cap-th acceptance, then
-  // settle that row with the reference's own acceptance test (lane = column)
-  if (wave_active) {
-    for (int m = 0; m < 16; ++m) {
-      const int slot = wave * 16 + m;
-      if (!meta[slot].pad) continue;  // uniform: every lane reads the same flag
-      const int mrx = (wrx0 + (m & 3)) * st, mry = (wry0 + (m >> 2)) * st;
+  // settle that row with the reference's own acceptance test (lane = column).
+  // The workgroup's waves take the 64 reference slots from a shared counter, so
+  // a sub-block with many such references does not hold the others at the barrier.
+  for (;;) {
+    int slot = 0;
+    if (lane == 0) slot = atomicAdd(cap_next, 1);
+    slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, 64));
+    if (slot >= XW * 16) break;
+    if (!meta[slot].pad) continue;  // uniform: every lane reads the same flag
+    {
+      const int m = slot & 15, ow = slot >> 4;  // reference m of wave ow's sub-block
+      const int mrx = (r0x + 4 * (ow & 1) + (m & 3)) * st, mry = (r0y + 4 * (ow >> 1) + (m >> 2)) * st;
       const int mx0 = max(mrx - half, 0), mw = min(mrx + half, wmax) - mx0 + 1;
       const int my0 = max(mry - half, 0), mh = min(mry + half, hmax) - my0 + 1;
       int seen = 0, row = 0;
