@@ -320,8 +320,8 @@ def main() -> int:
 
     img = im.Image(ctx).set_calibration_matrix(cal.ravel())
     stream = torch.cuda.ExternalStream(ctx.stream)
-    region_names = ("normalize", "amaze", "bm3d", "bm3d_match", "bm3d_filter_ht", "bm3d_filter_wiener", "pointwise",
-                    "film")
+    region_names = ("normalize", "amaze", "bm3d", "bm3d_match", "bm3d_row_dct", "bm3d_filter_ht", "bm3d_filter_wiener",
+                    "pointwise", "film")
 
     def run_form(precision: str):
         cfg = im.parse_config(stages(precision))

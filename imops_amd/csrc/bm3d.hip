@@ -437,18 +437,16 @@ __device__ __forceinline__ ctab_ptr bm3d_table(int which) {
 // the stack in order by passing the partial sum from lane k to lane k+1.
 constexpr int FT = 256;
 
-// out[j][k] = sum_n (sum_m x[n][m] C[k][m]) C[j][n]   (dct_2d_8x8, bm3d.rs:526-536)
-__device__ __forceinline__ void dct2_regs(float* x, ctab_ptr C) {
-  float y[64];
-#pragma unroll
-  for (int r = 0; r < 8; ++r)
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const float* xr = x + r * 8;
-      const auto ck = C + k * 8;
-      y[r * 8 + k] = xr[0] * ck[0] + xr[1] * ck[1] + xr[2] * ck[2] + xr[3] * ck[3] + xr[4] * ck[4] + xr[5] * ck[5] +
-                     xr[6] * ck[6] + xr[7] * ck[7];
-    }
+// coefficient k of the row pass of one 8-sample row: sum_m x[m] C[k][m]
+// (dct_1d_8, bm3d.rs:552-566)
+__device__ __forceinline__ float dct_row_coef(const float* xr, ctab_ptr C, int k) {
+  const auto ck = C + k * 8;
+  return xr[0] * ck[0] + xr[1] * ck[1] + xr[2] * ck[2] + xr[3] * ck[3] + xr[4] * ck[4] + xr[5] * ck[5] +
+         xr[6] * ck[6] + xr[7] * ck[7];
+}
+
+// column pass of dct_2d_8x8 on the row-pass output y: x[j][k] = sum_n y[n][k] C[j][n]
+__device__ __forceinline__ void dct2_cols(const float* y, float* x, ctab_ptr C) {
 #pragma unroll
   for (int j = 0; j < 8; ++j)
 #pragma unroll
@@ -457,6 +455,43 @@ __device__ __forceinline__ void dct2_regs(float* x, ctab_ptr C) {
       x[j * 8 + k] = y[0 * 8 + k] * cj[0] + y[1 * 8 + k] * cj[1] + y[2 * 8 + k] * cj[2] + y[3 * 8 + k] * cj[3] +
                      y[4 * 8 + k] * cj[4] + y[5 * 8 + k] * cj[5] + y[6 * 8 + k] * cj[6] + y[7 * 8 + k] * cj[7];
     }
+}
+
+// out[j][k] = sum_n (sum_m x[n][m] C[k][m]) C[j][n]   (dct_2d_8x8, bm3d.rs:526-536)
+__device__ __forceinline__ void dct2_regs(float* x, ctab_ptr C) {
+  float y[64];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) y[r * 8 + k] = dct_row_coef(x + r * 8, C, k);
+  dct2_cols(y, x, C);
+}
+
+// The row pass of a patch's forward DCT depends only on the 8-pixel row
+// segment, which every patch and group containing it shares: it is computed
+// once per segment here, and the filter runs only the column pass.
+struct RowDctPlanes {
+  const float* in[3];
+  float* rows[3];
+};
+__global__ void row_dct_kernel(RowDctPlanes pl, int w, int h) {
+  const float* __restrict__ in = pl.in[blockIdx.y];
+  float* __restrict__ rows = pl.rows[blockIdx.y];
+  const int rw = w - 7;
+  const size_t n = (size_t)rw * h;
+  const ctab_ptr C = bm3d_table(0);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const int y = (int)(i / rw), x = (int)(i - (size_t)y * rw);
+    const float* src = in + (size_t)y * w + x;
+    float xr[8], o[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) xr[m] = src[m];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = dct_row_coef(xr, C, k);
+    float4* dst = reinterpret_cast<float4*>(rows + i * 8);
+    dst[0] = make_float4(o[0], o[1], o[2], o[3]);
+    dst[1] = make_float4(o[4], o[5], o[6], o[7]);
+  }
 }
 
 // u[k][n] = sum_j x[j][k] D[n][j]; out[n][m] = sum_k u[k][n] D[m][k]   (idct_2d_8x8, bm3d.rs:538-548)
@@ -542,8 +577,9 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
     }
     __syncthreads();
     const float ch_sigma = ch == 0 ? a.sigma : a.sigma * a.chroma_scale;
-    const float* __restrict__ gplane = a.guide[ch];
-    const float* __restrict__ nplane = a.noisy[ch];
+    const int rw = a.width - 7;
+    const float* __restrict__ grows = a.guide_rows[ch];
+    const float* __restrict__ nrows = a.noisy_rows[ch];
     // 16 groups in flight per workgroup (4 waves x 4 slots); every lane runs the
     // same number of iterations so the slot shuffles stay converged.
     for (int base = 0; base < nref; base += 16) {
@@ -561,17 +597,27 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
         }
       }
       const bool act = k < g;
-      float gv[64], nv[64];
+      // the patch's row segments, already through the DCT's row pass.  Lanes
+      // without a patch hold position (0, 0): they load in bounds and keep zeros.
+      float gv[64], nv[64], yr[64];
 #pragma unroll
-      for (int dy = 0; dy < 8; ++dy)
+      for (int dy = 0; dy < 8; ++dy) {
+        const size_t off = ((size_t)(py + dy) * rw + px) * 8;
+        const float4 g0 = *reinterpret_cast<const float4*>(grows + off);
+        const float4 g1 = *reinterpret_cast<const float4*>(grows + off + 4);
+        const float gr[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
 #pragma unroll
-        for (int dx = 0; dx < 8; ++dx) {
-          const size_t off = (size_t)(py + dy) * a.width + px + dx;
-          gv[dy * 8 + dx] = act ? gplane[off] : 0.0f;
-          if (!kStep1) nv[dy * 8 + dx] = act ? nplane[off] : 0.0f;
+        for (int i = 0; i < 8; ++i) yr[dy * 8 + i] = act ? gr[i] : 0.0f;
+        if (!kStep1) {
+          const float4 n0 = *reinterpret_cast<const float4*>(nrows + off);
+          const float4 n1 = *reinterpret_cast<const float4*>(nrows + off + 4);
+          const float nr[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
+#pragma unroll
+          for (int i = 0; i < 8; ++i) nv[dy * 8 + i] = act ? nr[i] : 0.0f;
         }
+      }
       const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
-      dct2_regs(gv, bm3d_table(0));
+      dct2_cols(yr, gv, bm3d_table(0));
       wht_lanes(gv, k, g, wscale);
       float weight;
       if (kStep1) {
@@ -589,7 +635,9 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] = gv[i];
       } else {
-        dct2_regs(nv, bm3d_table(0));
+        dct2_cols(nv, yr, bm3d_table(0));
+#pragma unroll
+        for (int i = 0; i < 64; ++i) nv[i] = yr[i];
         wht_lanes(nv, k, g, wscale);
         // wiener_filter (bm3d.rs:257-272)
         const float s2 = ch_sigma * ch_sigma;
@@ -804,6 +852,13 @@ hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* 
 }
 size_t bm3d_filter_smem(int footprint_pixels) {
   return (size_t)footprint_pixels * (sizeof(unsigned long long) + sizeof(uint32_t));
+}
+
+hipError_t bm3d_row_dct_launch(const float* const in[3], int w, int h, float* const rows[3], hipStream_t s) {
+  if (w < 8 || h <= 0) return hipSuccess;
+  const RowDctPlanes pl{{in[0], in[1], in[2]}, {rows[0], rows[1], rows[2]}};
+  hipLaunchKernelGGL(row_dct_kernel, dim3(grid1d((size_t)(w - 7) * h), 3), dim3(256), 0, s, pl, w, h);
+  return hipGetLastError();
 }
 
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s) {

@@ -25,6 +25,10 @@ struct Bm3dMatchArgs {
 struct Bm3dFilterArgs {
   const float* guide[3];
   const float* noisy[3];
+  // plain filter: the row pass of the forward DCT of every 8-pixel row segment
+  // of guide and noisy (bm3d_row_dct_launch); noisy rows are read in step 2 only
+  const float* guide_rows[3];
+  const float* noisy_rows[3];
   int width, height;
   int refs_x, refs_y, stride, window, tile;
   float sigma, chroma_scale, lambda;
@@ -76,6 +80,11 @@ hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblo
 // trial: an autotuner trial launch (a separate kernel instantiation, same code)
 hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s,
                               bool trial = false);
+// Row pass of dct_2d_8x8 (bm3d.rs:526-530) for every 8-pixel row segment of
+// three w x h planes (w >= 8): rows[c][(y * (w - 7) + x) * 8 + k] in the
+// reference's order.  Needs the tables of bm3d_upload_tables.
+hipError_t bm3d_row_dct_launch(const float* const in[3], int w, int h, float* const rows[3], hipStream_t s);
+inline size_t bm3d_row_dct_floats(int w, int h) { return w >= 8 ? (size_t)(w - 7) * h * 8 : 0; }
 // copies the DCT / IDCT / Kaiser tables the plain filter reads (64 floats each)
 hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* kaiser, hipStream_t s);
 hipError_t bm3d_finalize_launch(const unsigned long long* num, const unsigned long long* den, const float* const noisy[3],

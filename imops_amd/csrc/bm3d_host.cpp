@@ -188,6 +188,20 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   IMOPS_HIP_CHECK(bm3d_rgb_to_ycbcr(img->rgb.as<float>(), n, noisy[0], noisy[1], noisy[2], s));
   IMOPS_HIP_CHECK(hipMemcpyAsync(yorig, noisy[0], n * sizeof(float), hipMemcpyDeviceToDevice, s));
 
+  // plain filter: row-pass planes of the noisy image (step 1's guide and step
+  // 2's noisy stack) and of the basic estimate (step 2's guide)
+  const size_t rfl = mixed_filter || nrefs == 0 ? 0 : bm3d_row_dct_floats(w, h);
+  float* noisy_rows[3] = {nullptr, nullptr, nullptr};
+  float* basic_rows[3] = {nullptr, nullptr, nullptr};
+  if (rfl) {
+    ctx->scr[SCR_BM3D_3].reserve(6 * rfl * sizeof(float));
+    float* r = ctx->scr[SCR_BM3D_3].as<float>();
+    for (int c = 0; c < 3; ++c) {
+      noisy_rows[c] = r + c * rfl;
+      basic_rows[c] = r + (3 + c) * rfl;
+    }
+  }
+
   // workgroup tile of reference patches: the largest whose LDS footprint lets
   // four plain-filter workgroups (16 waves) share a CU, or one mixed-filter
   // workgroup (16 waves for an 8x8 tile) fit
@@ -269,6 +283,13 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       for (int c = 0; c < 3; ++c) {
         fa.guide[c] = guide[c];
         fa.noisy[c] = noisy[c];
+        fa.guide_rows[c] = step1 ? noisy_rows[c] : basic_rows[c];
+        fa.noisy_rows[c] = noisy_rows[c];
+      }
+      if (rfl) {
+        TimedRegion tr(ctx, "bm3d_row_dct");
+        if (step1) IMOPS_HIP_CHECK(bm3d_row_dct_launch(noisy, w, h, noisy_rows, s));
+        else IMOPS_HIP_CHECK(bm3d_row_dct_launch(guide, w, h, basic_rows, s));
       }
       fa.width = w;
       fa.height = h;
