@@ -516,38 +516,85 @@ __device__ __forceinline__ void idct2_regs(float* x, ctab_ptr D) {
 }
 
 // Walsh-Hadamard across the group (walsh_hadamard_1d, bm3d.rs:595-614):
-// stage h pairs lane k with k^h; the lower lane keeps x+y, the upper x-y
-// (computed as y + (-x), the same IEEE result).  The exchange stays inside the
-// slot's 16-lane DPP row: h = 1, 2 are quad permutes, h = 4 a half-row mirror
-// followed by the quad reversal (7 - (k^3) = k^4), h = 8 a row rotation by 8.
-template <int H>
-__device__ __forceinline__ float dpp_xor(float v) {
-  int b = __float_as_int(v);
-  if (H == 1) b = __builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, true);       // quad_perm [1,0,3,2]
-  else if (H == 2) b = __builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
-  else if (H == 4) {
-    b = __builtin_amdgcn_update_dpp(0, b, 0x141, 0xF, 0xF, true);                 // row_half_mirror
-    b = __builtin_amdgcn_update_dpp(0, b, 0x1B, 0xF, 0xF, true);                  // quad_perm [3,2,1,0]
-  } else {
-    b = __builtin_amdgcn_update_dpp(0, b, 0x128, 0xF, 0xF, true);                 // row_ror:8
-  }
-  return __int_as_float(b);
-}
+// stage h pairs lane k with k^h, the lower lane keeping x + y and the upper
+// x - y.  Each stage is one fused multiply-add per value with the partner read
+// through DPP inside the slot's 16-lane row (h = 1, 2: quad permutes; h = 4: a
+// half-row mirror, then the quad reversal, 7 - (k^3) = k^4; h = 8: a row
+// rotation by 8): w_k <- w_k + m_k w_(k^h), m_k = +1 on the lower lane and -1
+// on the upper one.  The multiply by +-1 is exact and the add rounds once, so
+// the lower lane gets x + y and the upper y - x = -(x - y) (round to nearest is
+// symmetric): the upper lane holds the reference's value negated.  Partners
+// have been negated alike (they differ only in bits not yet processed), so the
+// per-lane sign is (-1)^popcount(k & (g - 1)) after the stages, and it is folded
+// into the final scale (exact).  Only zeros can come out with the other sign,
+// which no later step distinguishes (thresholds compare |x|, the fixed-point
+// aggregation truncates both to 0).
+//
+// The DPP operand must not read a VGPR written by the previous two VALU
+// instructions, nor follow an EXEC write by fewer than five wait states; each
+// block starts with s_nop 4 (the stages run under per-slot EXEC masks).
+#define IMOPS_FMAC_DPP(i, ctrl) "v_fmac_f32_dpp %" #i ", %" #i ", %16 " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+#define IMOPS_FMAC_DPP16(ctrl)                                                                                   \
+  "s_nop 4\n" IMOPS_FMAC_DPP(0, ctrl) IMOPS_FMAC_DPP(1, ctrl) IMOPS_FMAC_DPP(2, ctrl) IMOPS_FMAC_DPP(3, ctrl)   \
+      IMOPS_FMAC_DPP(4, ctrl) IMOPS_FMAC_DPP(5, ctrl) IMOPS_FMAC_DPP(6, ctrl) IMOPS_FMAC_DPP(7, ctrl)           \
+          IMOPS_FMAC_DPP(8, ctrl) IMOPS_FMAC_DPP(9, ctrl) IMOPS_FMAC_DPP(10, ctrl) IMOPS_FMAC_DPP(11, ctrl)     \
+              IMOPS_FMAC_DPP(12, ctrl) IMOPS_FMAC_DPP(13, ctrl) IMOPS_FMAC_DPP(14, ctrl) IMOPS_FMAC_DPP(15, ctrl)
+#define IMOPS_V16(b)                                                                                          \
+  "+v"(v[b + 0]), "+v"(v[b + 1]), "+v"(v[b + 2]), "+v"(v[b + 3]), "+v"(v[b + 4]), "+v"(v[b + 5]),          \
+      "+v"(v[b + 6]), "+v"(v[b + 7]), "+v"(v[b + 8]), "+v"(v[b + 9]), "+v"(v[b + 10]), "+v"(v[b + 11]),    \
+      "+v"(v[b + 12]), "+v"(v[b + 13]), "+v"(v[b + 14]), "+v"(v[b + 15])
+// h = 4: the mirror into temporaries, then the quad reversal fused into the FMA
 
 template <int H>
-__device__ __forceinline__ void wht_stage(float* v, int k) {
-  const unsigned sign = (k & H) ? 0x80000000u : 0u;
+__device__ __forceinline__ void wht_stage(float* v, float m) {
+  if (H == 1) {
 #pragma unroll
-  for (int i = 0; i < 64; ++i) v[i] = dpp_xor<H>(v[i]) + __uint_as_float(__float_as_uint(v[i]) ^ sign);
+    for (int b = 0; b < 64; b += 16) asm volatile(IMOPS_FMAC_DPP16("quad_perm:[1,0,3,2]") : IMOPS_V16(b) : "v"(m));
+  } else if (H == 2) {
+#pragma unroll
+    for (int b = 0; b < 64; b += 16) asm volatile(IMOPS_FMAC_DPP16("quad_perm:[2,3,0,1]") : IMOPS_V16(b) : "v"(m));
+  } else if (H == 8) {
+#pragma unroll
+    for (int b = 0; b < 64; b += 16) asm volatile(IMOPS_FMAC_DPP16("row_ror:8") : IMOPS_V16(b) : "v"(m));
+  } else {
+#pragma unroll
+    for (int b = 0; b < 64; b += 8) {
+      float t0, t1, t2, t3, t4, t5, t6, t7;
+      asm volatile(
+          "s_nop 4\n"
+          "v_mov_b32_dpp %8, %0 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %9, %1 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %10, %2 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %11, %3 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %12, %4 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %13, %5 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %14, %6 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "v_mov_b32_dpp %15, %7 row_half_mirror row_mask:0xf bank_mask:0xf\n"
+          "s_nop 1\n"
+          "v_fmac_f32_dpp %0, %8, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %1, %9, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %2, %10, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %3, %11, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %4, %12, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %5, %13, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %6, %14, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          "v_fmac_f32_dpp %7, %15, %16 quad_perm:[3,2,1,0] row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+          : "+v"(v[b + 0]), "+v"(v[b + 1]), "+v"(v[b + 2]), "+v"(v[b + 3]), "+v"(v[b + 4]), "+v"(v[b + 5]),
+            "+v"(v[b + 6]), "+v"(v[b + 7]), "=&v"(t0), "=&v"(t1), "=&v"(t2), "=&v"(t3), "=&v"(t4), "=&v"(t5),
+            "=&v"(t6), "=&v"(t7)
+          : "v"(m));
+    }
+  }
 }
 
 __device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
-  if (g > 1) wht_stage<1>(v, k);
-  if (g > 2) wht_stage<2>(v, k);
-  if (g > 4) wht_stage<4>(v, k);
-  if (g > 8) wht_stage<8>(v, k);
+  if (g > 1) wht_stage<1>(v, (k & 1) ? -1.0f : 1.0f);
+  if (g > 2) wht_stage<2>(v, (k & 2) ? -1.0f : 1.0f);
+  if (g > 4) wht_stage<4>(v, (k & 4) ? -1.0f : 1.0f);
+  if (g > 8) wht_stage<8>(v, (k & 8) ? -1.0f : 1.0f);
+  const float s = (__popc(k & (g - 1)) & 1) ? -scale : scale;
 #pragma unroll
-  for (int i = 0; i < 64; ++i) v[i] *= scale;
+  for (int i = 0; i < 64; ++i) v[i] *= s;
 }
 
 // LDS: the footprint's i64 numerators, then its u32 denominators.  A
