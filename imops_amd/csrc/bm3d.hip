@@ -647,22 +647,18 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
       // the patch's row segments, already through the DCT's row pass.  Lanes
       // without a patch hold position (0, 0): they load in bounds and keep zeros.
       float gv[64], nv[64], yr[64];
+      auto load_rows = [&](const float* __restrict__ rows, float* y) {
 #pragma unroll
-      for (int dy = 0; dy < 8; ++dy) {
-        const size_t off = ((size_t)(py + dy) * rw + px) * 8;
-        const float4 g0 = *reinterpret_cast<const float4*>(grows + off);
-        const float4 g1 = *reinterpret_cast<const float4*>(grows + off + 4);
-        const float gr[8] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w};
+        for (int dy = 0; dy < 8; ++dy) {
+          const size_t off = ((size_t)(py + dy) * rw + px) * 8;
+          const float4 r0 = *reinterpret_cast<const float4*>(rows + off);
+          const float4 r1 = *reinterpret_cast<const float4*>(rows + off + 4);
+          const float rr[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
 #pragma unroll
-        for (int i = 0; i < 8; ++i) yr[dy * 8 + i] = act ? gr[i] : 0.0f;
-        if (!kStep1) {
-          const float4 n0 = *reinterpret_cast<const float4*>(nrows + off);
-          const float4 n1 = *reinterpret_cast<const float4*>(nrows + off + 4);
-          const float nr[8] = {n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, n1.z, n1.w};
-#pragma unroll
-          for (int i = 0; i < 8; ++i) nv[dy * 8 + i] = act ? nr[i] : 0.0f;
+          for (int i = 0; i < 8; ++i) y[dy * 8 + i] = act ? rr[i] : 0.0f;
         }
-      }
+      };
+      load_rows(grows, yr);
       const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
       dct2_cols(yr, gv, bm3d_table(0));
       wht_lanes(gv, k, g, wscale);
@@ -682,9 +678,8 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] = gv[i];
       } else {
-        dct2_cols(nv, yr, bm3d_table(0));
-#pragma unroll
-        for (int i = 0; i < 64; ++i) nv[i] = yr[i];
+        load_rows(nrows, yr);
+        dct2_cols(yr, nv, bm3d_table(0));
         wht_lanes(nv, k, g, wscale);
         // wiener_filter (bm3d.rs:257-272)
         const float s2 = ch_sigma * ch_sigma;
