@@ -489,7 +489,10 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       if (ok && before == cap - 1) meta[slot].limit = row * mw + lane;
     }
   }
-  // ---- T: K-th smallest upper bound among the tracked candidates (ties by lane, slot)
+  // ---- T: K-th smallest upper bound among the tracked candidates.  The bound
+  // grows with the distance (same reference, block-wide norm terms), so it is
+  // the bound of the K-th smallest tracked distance (ranked with ties by lane,
+  // slot); that K-th distance is kept as well.
   if (wave_active && K > 0) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -503,20 +506,21 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         const float f = hsqrt(fmaxf(b2d[j] + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
         u2 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
       }
-      int r1 = 0, r2 = 0, s1 = 0, s2 = 0;
+      int s1 = 0, s2 = 0;
       const float e1 = b1d[j], e2 = b2d[j];
       for (int k = 0; k < 16; ++k) {
-        const float o1 = __shfl(u1, 16 * q + k, 64), o2 = __shfl(u2, 16 * q + k, 64);
-        r1 += (o1 < u1 || (o1 == u1 && k < n)) + (o2 < u1 || (o2 == u1 && k < n));
-        r2 += (o1 < u2 || (o1 == u2 && k <= n)) + (o2 < u2 || (o2 == u2 && k < n));
         const float p1 = __shfl(e1, 16 * q + k, 64), p2 = __shfl(e2, 16 * q + k, 64);
         s1 += (p1 < e1 || (p1 == e1 && k < n)) + (p2 < e1 || (p2 == e1 && k < n));
         s2 += (p1 < e2 || (p1 == e2 && k <= n)) + (p2 < e2 || (p2 == e2 && k < n));
       }
-      if (r1 == K - 1) meta[wave * 16 + 4 * q + j].d15 = u1;
-      if (r2 == K - 1) meta[wave * 16 + 4 * q + j].d15 = u2;
-      if (s1 == K - 1) meta[wave * 16 + 4 * q + j].dK = e1;
-      if (s2 == K - 1) meta[wave * 16 + 4 * q + j].dK = e2;
+      if (s1 == K - 1) {
+        meta[wave * 16 + 4 * q + j].dK = e1;
+        meta[wave * 16 + 4 * q + j].d15 = u1;
+      }
+      if (s2 == K - 1) {
+        meta[wave * 16 + 4 * q + j].dK = e2;
+        meta[wave * 16 + 4 * q + j].d15 = u2;
+      }
     }
   }
   __syncthreads();
