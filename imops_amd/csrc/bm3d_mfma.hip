@@ -590,21 +590,28 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     usea = ca <= kListCap;
     nms = usea ? ca : (cb <= kListCapB ? cb : 0);
     for (int k = 0; k < 16; ++k) {
-      const int v = __shfl(nms, k, 64);
+      const int v = __builtin_amdgcn_readlane(nms, k);
       offs += k < m ? v : 0;
       total += v;
     }
   }
-  // entry e of the flattened lists -> (reference rm, index idx, list position)
+  // entry e of the flattened lists -> (reference rm, index idx, list length nm,
+  // exact-band list used); lane k < 16 holds reference k's offset and length
   auto entry = [&](int e, int& rm, int& idx, int& nm) -> int {
     rm = 0;
+    int o = 0, ua = 1;
+    nm = 0;
     for (int k = 0; k < 16; ++k) {
-      const int ok = __shfl(offs, k, 64), nk = __shfl(nms, k, 64);
-      if (e >= ok && e < ok + nk) rm = k;
+      const int ok = __builtin_amdgcn_readlane(offs, k), nk = __builtin_amdgcn_readlane(nms, k);
+      if (e >= ok && e < ok + nk) {
+        rm = k;
+        o = ok;
+        nm = nk;
+        ua = __builtin_amdgcn_readlane(usea, k);
+      }
     }
-    idx = e - __shfl(offs, rm, 64);
-    nm = __shfl(nms, rm, 64);
-    return __shfl(usea, rm, 64);
+    idx = e - o;
+    return ua;
   };
   if (wave_active) {
     for (int base = 0; base < total; base += 64) {
