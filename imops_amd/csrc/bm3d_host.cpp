@@ -255,11 +255,8 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
           mf.n_blocks = mf.blocks_x * ((refs_y + 7) / 8);
           IMOPS_HIP_CHECK(bm3d_match_mfma_launch(mf, s));
           if (ctx->timer.enabled && !ctx->capturing) {  // diagnostics: patches sent to the plain matcher / selected approximately
-            uint32_t nfb[2] = {0, 0};
-            IMOPS_HIP_CHECK(hipMemcpyAsync(nfb, overflow_count, sizeof nfb, hipMemcpyDeviceToHost, s));
-            IMOPS_HIP_CHECK(hipStreamSynchronize(s));
-            ctx->timer.counts.emplace_back("bm3d_fallback_refs", (double)nfb[0]);
-            ctx->timer.counts.emplace_back("bm3d_approx_refs", (double)nfb[1]);
+            static const char* const names[2] = {"bm3d_fallback_refs", "bm3d_approx_refs"};
+            ctx->timer.count_async(names, overflow_count, 2, s);
           }
           // patches whose contender list overflowed: the plain matcher, grid-stride over the list
           Bm3dMatchArgs fb = m;

@@ -94,6 +94,31 @@ void KernelTimer::clear() {
     }
   regions.clear();
   counts.clear();
+  pinned_used = 0;
+}
+
+void KernelTimer::count_async(const char* const* names, const uint32_t* dev, int n, hipStream_t s) {
+  if (n <= 0 || (size_t)n > kPinnedWords) return;
+  size_t blk = pinned_used / kPinnedWords, off = pinned_used % kPinnedWords;
+  if (off + n > kPinnedWords) {  // the words stay in one block
+    ++blk;
+    off = 0;
+  }
+  if (blk == pinned.size()) {
+    void* b = nullptr;
+    IMOPS_HIP_CHECK(hipHostMalloc(&b, kPinnedWords * sizeof(uint32_t), hipHostMallocDefault));
+    pinned.push_back(static_cast<uint32_t*>(b));
+  }
+  uint32_t* h = pinned[blk] + off;
+  pinned_used = blk * kPinnedWords + off + n;
+  IMOPS_HIP_CHECK(hipMemcpyAsync(h, dev, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  for (int i = 0; i < n; ++i) counts.push_back({names[i], h + i});
+}
+
+void KernelTimer::release() {
+  clear();
+  for (uint32_t* b : pinned) (void)hipHostFree(b);
+  pinned.clear();
 }
 
 TimedRegion::TimedRegion(imops_ctx* ctx, const char* name) : ctx_(ctx) {
@@ -561,8 +586,8 @@ int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* 
           ++n;
         }
     for (auto& c : ctx->timer.counts)
-      if (c.first == region) {
-        total += c.second;
+      if (c.name == region) {
+        total += (double)*c.host;
         ++n;
       }
     *total_ms = total;
@@ -583,7 +608,7 @@ int imops_ctx_destroy(imops_ctx* ctx) {
       img->ctx = nullptr;
     }
     ctx->images.clear();
-    ctx->timer.clear();
+    ctx->timer.release();
     for (auto& b : ctx->scr) b.release();
     ctx->bm3d_frags.release();
     ctx->film_cache.reset();

@@ -65,9 +65,21 @@ struct KernelTimer {
   std::vector<Region> regions;
   // Named event counts recorded while timing is on (e.g. patches a stage
   // handed to a slower path); reported through imops_ctx_timing as the sum.
-  std::vector<std::pair<std::string, double>> counts;
+  // Each is a device word copied asynchronously into a pinned host slot, read
+  // once the stream has drained, so counting never stalls the stream.
+  struct Count {
+    std::string name;
+    const uint32_t* host;
+  };
+  std::vector<Count> counts;
+  std::vector<uint32_t*> pinned;  // blocks of kPinnedWords slots, kept until release()
+  size_t pinned_used = 0;
+  static constexpr size_t kPinnedWords = 1024;
   Region& region(const std::string& name);
+  // queues the copy of n consecutive device words, counted under names[0..n)
+  void count_async(const char* const* names, const uint32_t* dev, int n, hipStream_t s);
   void clear();
+  void release();  // clear() and free the pinned blocks (stream drained)
 };
 }  // namespace imops
 
