@@ -9,7 +9,8 @@
 // the result depends on the tile a pixel is computed in; the kernel keeps that
 // tile structure instead of computing the image as one plane.
 //
-// Mapping: one 512-thread workgroup per tile (persistent over tiles).  The
+// Mapping: one 1024-thread workgroup per tile (persistent over tiles; 16 waves
+// per CU hide the scratch-slot and barrier latency better than 8).  The
 // CFA tile, hvwt, pmwt and the two Nyquist masks live in LDS (144 KiB); the
 // other planes of the tile live in a per-workgroup global scratch slot that
 // stays L2/MALL resident while the tile is processed.  Recurrences run as
@@ -30,7 +31,7 @@ constexpr int TS = 128, TSH = 64;
 constexpr int V1 = 128, V2 = 256, V3 = 384;
 constexpr int P1 = 127, P2 = 254, P3 = 381;
 constexpr int M1 = 129, M2 = 258, M3 = 387;
-constexpr int NT = 512;  // threads per workgroup
+constexpr int NT = 1024;  // threads per workgroup
 
 constexpr float eps = 1e-5f, epssq = 1e-10f, arthresh = 0.75f;
 constexpr float clip_pt = 1.0f, clip_pt8 = 1.0f * 0.8f;
@@ -259,6 +260,7 @@ __device__ __forceinline__ float cd_update(float prev, float cur, float next, fl
 __device__ void pass_cd_chains(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
+  if (tid >= 512) return;  // 128 lines x 2 parities per direction
   const bool horiz = tid < 256;
   const int line = (tid & 255) >> 1, parity = tid & 1;
   float* cd = c.P(horiz ? S_HCD : S_VCD);
