@@ -49,7 +49,7 @@ __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const Fil
     double phi[16];
     for (int i = 0; i < 16; ++i) {
       const double s = w[0] * T->basis[0][i] + w[1] * T->basis[1][i] + w[2] * T->basis[2][i];
-      phi[i] = s / T->e_rel[i];
+      phi[i] = s * T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp of f64
     }
     int e = 0;
     for (int l = 0; l < T->n_absorbing; ++l) {
@@ -62,7 +62,7 @@ __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const Fil
       if (T->is_emulsion[l]) {
         double acc = 0.0;
         for (int i = 0; i < 15; ++i) acc += 0.5 * (absorbed[i] + absorbed[i + 1]) * 20.0;
-        planes[(size_t)e * n + p] = (float)(acc / 300.0);
+        planes[(size_t)e * n + p] = (float)(acc * (1.0 / 300.0));
         ++e;
       }
     }
@@ -175,14 +175,22 @@ __global__ void dir_apply_kernel(float* __restrict__ image, const float* __restr
                                  const FilmDeviceTables* __restrict__ T) {
   const int E = T->n_emulsions;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    for (int j = 0; j < E; ++j) {
+    // the pixel's blurred dyes, loaded together (fixed trip counts)
+    float hv[kFilmMaxEmulsions];
+#pragma unroll
+    for (int i = 0; i < kFilmMaxEmulsions; ++i) hv[i] = i < E ? inh[(size_t)i * n + p] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < kFilmMaxEmulsions; ++j) {
+      if (j >= E) break;
       float total = 0.0f;
       bool active = false;
-      for (int i = 0; i < E; ++i) {
+#pragma unroll
+      for (int i = 0; i < kFilmMaxEmulsions; ++i) {
+        if (i >= E) break;
         const float wgt = T->dir[i][j];
         if (T->dir_valid[i][j] && fabsf(wgt) > 1e-6f) {
           active = true;
-          total += wgt * inh[(size_t)i * n + p];
+          total += wgt * hv[i];
         }
       }
       if (active) image[(size_t)j * n + p] *= expf(-total);

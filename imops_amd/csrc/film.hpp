@@ -33,6 +33,7 @@ struct FilmDeviceTables {
   double minv[3][3];       // ACEScg -> basis weights
   double basis[3][16];     // Gaussian basis spectra
   double e_rel[16];        // 550 / lambda
+  double inv_e_rel[16];    // 1 / e_rel: the per-pixel division becomes a product (<= 1 ulp in f64)
   int n_absorbing;         // absorbing layers, top to bottom
   int is_emulsion[kFilmMaxLayers];
   double trans[kFilmMaxLayers][16];

@@ -154,7 +154,10 @@ HostStock build_stock(const Stock* st) {
   T.minv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
   T.minv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
   T.minv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
-  for (int i = 0; i < NB; ++i) T.e_rel[i] = 550.0 / lam(i);
+  for (int i = 0; i < NB; ++i) {
+    T.e_rel[i] = 550.0 / lam(i);
+    T.inv_e_rel[i] = 1.0 / T.e_rel[i];
+  }
   for (int c = 0; c < 3; ++c)
     for (int i = 0; i < NB; ++i) T.cmf[c][i] = kCmf[c][i];
   for (int r = 0; r < 3; ++r)
