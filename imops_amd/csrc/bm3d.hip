@@ -775,6 +775,44 @@ __global__ void blur1_kernel(const float* __restrict__ in, float* __restrict__ o
   }
 }
 
+// The same blur with the radius fixed at compile time: each thread makes Q
+// neighbouring outputs of a line from one register window of Q + 2R samples
+// (every tap still added in order, out-of-bounds taps skipped as above).
+template <bool kVertical, int R>
+__global__ void blur1_win_kernel(const float* __restrict__ in, float* __restrict__ out, int w, int h,
+                                 Bm3dBlurTaps taps) {
+  constexpr int Q = 4, L = 2 * R + 1;
+  const int len = kVertical ? h : w, nseg = (len + Q - 1) / Q;
+  const size_t total = (size_t)(kVertical ? w : h) * nseg;
+  auto at = [&](int line, int q) -> size_t { return kVertical ? (size_t)q * w + line : (size_t)line * w + q; };
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    // consecutive threads: neighbouring columns (vertical) or segments of one row
+    const int line = kVertical ? (int)(i % w) : (int)(i / nseg);
+    const int p0 = (kVertical ? (int)(i / w) : (int)(i % nseg)) * Q;
+    float win[Q + 2 * R];
+#pragma unroll
+    for (int t = 0; t < Q + 2 * R; ++t) {
+      const int q = p0 - R + t;
+      win[t] = (q >= 0 && q < len) ? in[at(line, q)] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      const int p = p0 + j;
+      if (p >= len) break;
+      float val = 0.0f, ws = 0.0f;
+#pragma unroll
+      for (int k = 0; k < L; ++k) {
+        const int q = p - R + k;
+        if (q >= 0 && q < len) {
+          val += win[j + k] * taps.k[k];
+          ws += taps.k[k];
+        }
+      }
+      out[at(line, p)] = val / ws;
+    }
+  }
+}
+
 // Detail blend of the denoised luma + YCbCr -> RGB /255 (bm3d.rs:51-60, 79-92, 629-635).
 __global__ void bm3d_merge_kernel(const float* __restrict__ yd, const float* __restrict__ cb,
                                   const float* __restrict__ cr, const float* __restrict__ yorig,
@@ -951,6 +989,12 @@ hipError_t fill_zero_u64(unsigned long long* p, size_t n, hipStream_t s) {
 hipError_t bm3d_blur_launch(const float* in, float* tmp, float* out, int w, int h, const Bm3dBlurTaps& taps,
                             hipStream_t s) {
   const size_t n = (size_t)w * h;
+  if (taps.radius == 9 && taps.len == 19) {  // sigma 3, the detail blend's blur
+    const size_t segs = (size_t)h * ((w + 3) / 4), segs_v = (size_t)w * ((h + 3) / 4);
+    hipLaunchKernelGGL((blur1_win_kernel<false, 9>), dim3(grid1d(segs)), dim3(256), 0, s, in, tmp, w, h, taps);
+    hipLaunchKernelGGL((blur1_win_kernel<true, 9>), dim3(grid1d(segs_v)), dim3(256), 0, s, tmp, out, w, h, taps);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(blur1_kernel<false>, dim3(grid1d(n)), dim3(256), 0, s, in, tmp, w, h, taps);
   hipLaunchKernelGGL(blur1_kernel<true>, dim3(grid1d(n)), dim3(256), 0, s, tmp, out, w, h, taps);
   return hipGetLastError();
