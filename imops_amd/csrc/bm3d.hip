@@ -445,16 +445,21 @@ __device__ __forceinline__ float dct_row_coef(const float* xr, ctab_ptr C, int k
          xr[6] * ck[6] + xr[7] * ck[7];
 }
 
-// column pass of dct_2d_8x8 on the row-pass output y: x[j][k] = sum_n y[n][k] C[j][n]
+// column pass of dct_2d_8x8 on the row-pass output y: x[j][k] = sum_n y[n][k] C[j][n].
+// Column by column, so each column's inputs die as its outputs appear (x may be y).
 __device__ __forceinline__ void dct2_cols(const float* y, float* x, ctab_ptr C) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j)
+  for (int k = 0; k < 8; ++k) {
+    float o[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
+    for (int j = 0; j < 8; ++j) {
       const auto cj = C + j * 8;
-      x[j * 8 + k] = y[0 * 8 + k] * cj[0] + y[1 * 8 + k] * cj[1] + y[2 * 8 + k] * cj[2] + y[3 * 8 + k] * cj[3] +
-                     y[4 * 8 + k] * cj[4] + y[5 * 8 + k] * cj[5] + y[6 * 8 + k] * cj[6] + y[7 * 8 + k] * cj[7];
+      o[j] = y[0 * 8 + k] * cj[0] + y[1 * 8 + k] * cj[1] + y[2 * 8 + k] * cj[2] + y[3 * 8 + k] * cj[3] +
+             y[4 * 8 + k] * cj[4] + y[5 * 8 + k] * cj[5] + y[6 * 8 + k] * cj[6] + y[7 * 8 + k] * cj[7];
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j * 8 + k] = o[j];
+  }
 }
 
 // out[j][k] = sum_n (sum_m x[n][m] C[k][m]) C[j][n]   (dct_2d_8x8, bm3d.rs:526-536)
@@ -601,7 +606,7 @@ __device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
 // workgroup's denominator sum at one pixel is at most 64 references x 16
 // patches x 1e5 < 2^32, so the 32-bit partial sums are exact.
 template <bool kStep1, bool kTrial>
-__global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
+__global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
   extern __shared__ unsigned long long acc[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int slot = lane >> 4, k = lane & 15;
@@ -678,20 +683,17 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] = gv[i];
       } else {
-        load_rows(nrows, yr);
-        dct2_cols(yr, nv, bm3d_table(0));
-        wht_lanes(nv, k, g, wscale);
-        // wiener_filter (bm3d.rs:257-272)
+        // wiener_filter (bm3d.rs:257-272): the coefficients depend on the guide
+        // stack only, so they (and the weight) are formed before the noisy stack
+        // is transformed, which keeps one 64-value stack fewer live
         const float s2 = ch_sigma * ch_sigma;
 #pragma unroll
         for (int i = 0; i < 64; ++i) {
-          if (i == 0 && k == 0) { gv[0] = 0.0f; continue; }  // element 0: weight term 1, sample untouched
+          if (i == 0 && k == 0) { gv[0] = 1.0f; continue; }  // element 0: weight term 1, sample untouched
           const float e = gv[i] * gv[i];
-          const float coef = e / (e + s2 + 1e-5f);
-          nv[i] *= coef;
-          gv[i] = coef * coef;
+          gv[i] = e / (e + s2 + 1e-5f);
         }
-        // running sum in stack order: 1 + patch 0 (from its element 1), then patch 1, ...
+        // running sum of coef * coef in stack order: 1 + patch 0 (from its element 1), then patch 1, ...
         float run = 1.0f;
         int gmax = g;
         for (int off = 16; off < 64; off <<= 1) gmax = max(gmax, __shfl_xor(gmax, off, 64));
@@ -699,11 +701,16 @@ __global__ void __launch_bounds__(FT) bm3d_filter_kernel(Bm3dFilterArgs a) {
           if (k == kk && kk < g) {
 #pragma unroll
             for (int i = 0; i < 64; ++i)
-              if (!(kk == 0 && i == 0)) run += gv[i];
+              if (!(kk == 0 && i == 0)) run += gv[i] * gv[i];
           }
           run = __shfl(run, (slot << 4) + kk, 64);
         }
         weight = run > 0.0f ? 1.0f / run : 1.0f;
+        load_rows(nrows, yr);
+        dct2_cols(yr, nv, bm3d_table(0));
+        wht_lanes(nv, k, g, wscale);
+#pragma unroll
+        for (int i = 0; i < 64; ++i) nv[i] *= gv[i];  // element 0 of patch 0: coefficient 1 (exact)
       }
       // inverse (bm3d.rs:508-524) and aggregation into the footprint
       wht_lanes(nv, k, g, wscale);
