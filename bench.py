@@ -72,6 +72,7 @@ stock = "ColorNeg200"
 film_format = "Film35mm"
 seed = 1
 output = "NegativeLinear"
+precision = "{film_precision}"
 
 [[pipeline_modules]]
 name = "CST"
@@ -80,8 +81,10 @@ target_color_space = "Srgb"
 
 
 def stages(precision: str) -> str:
+    # the mixed form also runs the film's per-pixel spectral math in FP32
     return STAGES_TMPL.format(sigma=BM3D_SIGMA, lam=BM3D_LAMBDA, stride=BM3D_STRIDE, window=BM3D_WINDOW,
-                              precision=precision, ev=repr(EV))
+                              precision=precision, ev=repr(EV),
+                              film_precision="mixed" if precision == "mixed" else "fp64")
 
 
 STAGE_NAMES = ["Demosaic{amaze}", f"BM3D{{stride {BM3D_STRIDE}, window {BM3D_WINDOW}, lambda {BM3D_LAMBDA}, "

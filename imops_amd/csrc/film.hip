@@ -37,32 +37,35 @@ __device__ __forceinline__ int reflect_index(long long i, long long n) {
   }
 }
 
+// Real = double is the reference's arithmetic; Real = float is the mixed
+// form's (its tables are rounded to float once per pixel loop, hoisted).
+template <typename Real>
 __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const FilmDeviceTables* __restrict__ T,
                               float* __restrict__ planes) {
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    const double r = (double)rgb[p * 3], g = (double)rgb[p * 3 + 1], b = (double)rgb[p * 3 + 2];
-    double w[3];
+    const Real r = (Real)rgb[p * 3], g = (Real)rgb[p * 3 + 1], b = (Real)rgb[p * 3 + 2];
+    Real w[3];
     for (int k = 0; k < 3; ++k) {
-      const double v = T->minv[k][0] * r + T->minv[k][1] * g + T->minv[k][2] * b;
-      w[k] = v < 0.0 ? 0.0 : v;
+      const Real v = (Real)T->minv[k][0] * r + (Real)T->minv[k][1] * g + (Real)T->minv[k][2] * b;
+      w[k] = v < (Real)0 ? (Real)0 : v;
     }
-    double phi[16];
+    Real phi[16];
     for (int i = 0; i < 16; ++i) {
-      const double s = w[0] * T->basis[0][i] + w[1] * T->basis[1][i] + w[2] * T->basis[2][i];
-      phi[i] = s * T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp of f64
+      const Real s = w[0] * (Real)T->basis[0][i] + w[1] * (Real)T->basis[1][i] + w[2] * (Real)T->basis[2][i];
+      phi[i] = s * (Real)T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp
     }
     int e = 0;
     for (int l = 0; l < T->n_absorbing; ++l) {
-      double absorbed[16];
+      Real absorbed[16];
       for (int i = 0; i < 16; ++i) {
-        const double pt = phi[i] * T->trans[l][i];
+        const Real pt = phi[i] * (Real)T->trans[l][i];
         absorbed[i] = phi[i] - pt;
         phi[i] = pt;
       }
       if (T->is_emulsion[l]) {
-        double acc = 0.0;
-        for (int i = 0; i < 15; ++i) acc += 0.5 * (absorbed[i] + absorbed[i + 1]) * 20.0;
-        planes[(size_t)e * n + p] = (float)(acc * (1.0 / 300.0));
+        Real acc = 0;
+        for (int i = 0; i < 15; ++i) acc += (Real)0.5 * (absorbed[i] + absorbed[i + 1]) * (Real)20.0;
+        planes[(size_t)e * n + p] = (float)(acc * (Real)(1.0 / 300.0));
         ++e;
       }
     }
@@ -121,6 +124,7 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, const float* orig, 
 }
 
 // Halation bounce + reciprocity + capture LUT + reduction (image and mask dye).
+template <typename Real>
 __global__ void capture_reduce_kernel(const float* __restrict__ absorbed, const float* __restrict__ bounce, size_t n,
                                       const FilmDeviceTables* __restrict__ T, float* __restrict__ image,
                                       float* __restrict__ mask) {
@@ -134,25 +138,25 @@ __global__ void capture_reduce_kernel(const float* __restrict__ absorbed, const 
       const FilmEmulsionTables& E = T->em[e];
       float phi = absorbed[(size_t)e * n + p];
       if (bounce && E.gain > 0.0f) phi += E.gain * bounce[p];
-      const double x = (double)(phi * E.eta);
-      double f;
-      if (!(x > 0.0)) {
-        f = E.lut_frac[0];
+      const Real x = (Real)(phi * E.eta);
+      Real f;
+      if (!(x > (Real)0)) {
+        f = (Real)E.lut_frac[0];
       } else {
-        const double lp = log10(x);
-        if (lp <= E.lut_log[0]) {
-          f = E.lut_frac[0];
-        } else if (lp >= E.lut_log[63]) {
-          f = E.lut_frac[63];
+        const Real lp = log10(x);
+        if (lp <= (Real)E.lut_log[0]) {
+          f = (Real)E.lut_frac[0];
+        } else if (lp >= (Real)E.lut_log[63]) {
+          f = (Real)E.lut_frac[63];
         } else {
           int lo = 0, hi = 63;
           while (hi - lo > 1) {
             const int mid = (lo + hi) / 2;
-            if (E.lut_log[mid] <= lp) lo = mid;
+            if ((Real)E.lut_log[mid] <= lp) lo = mid;
             else hi = mid;
           }
-          const double t = (lp - E.lut_log[lo]) / (E.lut_log[hi] - E.lut_log[lo]);
-          f = E.lut_frac[lo] * (1.0 - t) + E.lut_frac[hi] * t;
+          const Real t = (lp - (Real)E.lut_log[lo]) / ((Real)E.lut_log[hi] - (Real)E.lut_log[lo]);
+          f = (Real)E.lut_frac[lo] * ((Real)1 - t) + (Real)E.lut_frac[hi] * t;
         }
       }
       const float fr = (float)f;
@@ -265,6 +269,7 @@ __global__ void grain_apply_kernel(float* __restrict__ dye, const float* __restr
 }
 
 // Densitometric scan (densitometry.rs:47-73) and the positive invert (invert.rs:36-108).
+template <typename Real>
 __global__ void scan_kernel(const float* __restrict__ image, const float* __restrict__ mask, size_t n,
                             const FilmDeviceTables* __restrict__ T, float* __restrict__ rgb) {
   const int E = T->n_emulsions;
@@ -273,32 +278,33 @@ __global__ void scan_kernel(const float* __restrict__ image, const float* __rest
   mask += b * E * n;
   rgb += b * 3 * n;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    double dens[16];
-    for (int i = 0; i < 16; ++i) dens[i] = 0.0;
+    Real dens[16];
+    for (int i = 0; i < 16; ++i) dens[i] = 0;
     for (int e = 0; e < E; ++e) {
-      const double di = (double)image[(size_t)e * n + p];
-      const double dm = (double)mask[(size_t)e * n + p];
+      const Real di = (Real)image[(size_t)e * n + p];
+      const Real dm = (Real)mask[(size_t)e * n + p];
       const FilmEmulsionTables& M = T->em[e];
       for (int l = 0; l < 16; ++l) {
-        dens[l] += di * M.eps[l];
-        if (M.has_mask) dens[l] += dm * M.mask_eps[l];
+        dens[l] += di * (Real)M.eps[l];
+        if (M.has_mask) dens[l] += dm * (Real)M.mask_eps[l];
       }
     }
-    double t[16];
-    for (int i = 0; i < 16; ++i) t[i] = exp(-dens[i] * 2.302585092994045684) * T->scanner[i];  // 10^-d
-    double xyz[3];
+    Real t[16];
+    for (int i = 0; i < 16; ++i)
+      t[i] = exp(-dens[i] * (Real)2.302585092994045684) * (Real)T->scanner[i];  // 10^-d
+    Real xyz[3];
     for (int c = 0; c < 3; ++c) {
-      double acc = 0.0;
+      Real acc = 0;
       for (int i = 0; i < 15; ++i) {
-        const double a = t[i] * T->cmf[c][i], b = t[i + 1] * T->cmf[c][i + 1];
-        acc += 0.5 * (a + b) * 20.0;
+        const Real a = t[i] * (Real)T->cmf[c][i], b = t[i + 1] * (Real)T->cmf[c][i + 1];
+        acc += (Real)0.5 * (a + b) * (Real)20.0;
       }
       xyz[c] = acc;
     }
     float out[3];
     for (int c = 0; c < 3; ++c) {
-      const double v = T->xyz2ap1[c][0] * xyz[0] + T->xyz2ap1[c][1] * xyz[1] + T->xyz2ap1[c][2] * xyz[2];
-      out[c] = (float)((double)(float)v * T->dmin_scale);
+      const Real v = (Real)T->xyz2ap1[c][0] * xyz[0] + (Real)T->xyz2ap1[c][1] * xyz[1] + (Real)T->xyz2ap1[c][2] * xyz[2];
+      out[c] = (float)((Real)(float)v * (Real)T->dmin_scale);
     }
     if (T->invert) {
       const float eps = 1e-6f;
@@ -629,8 +635,10 @@ __global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, 
 
 }  // namespace
 
-hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s) {
-  hipLaunchKernelGGL(expose_kernel, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, T, planes);
+hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
+                       bool fp32) {
+  if (fp32) hipLaunchKernelGGL(expose_kernel<float>, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, T, planes);
+  else hipLaunchKernelGGL(expose_kernel<double>, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, T, planes);
   return hipGetLastError();
 }
 
@@ -668,9 +676,13 @@ hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, flo
 }
 
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
-                               float* image, float* mask, hipStream_t s, int images) {
-  hipLaunchKernelGGL(capture_reduce_kernel, dim3(grid1d(n), images), dim3(256), 0, s, absorbed, bounce, n, T, image,
-                     mask);
+                               float* image, float* mask, hipStream_t s, int images, bool fp32) {
+  if (fp32)
+    hipLaunchKernelGGL(capture_reduce_kernel<float>, dim3(grid1d(n), images), dim3(256), 0, s, absorbed, bounce, n, T,
+                       image, mask);
+  else
+    hipLaunchKernelGGL(capture_reduce_kernel<double>, dim3(grid1d(n), images), dim3(256), 0, s, absorbed, bounce, n, T,
+                       image, mask);
   return hipGetLastError();
 }
 
@@ -693,8 +705,9 @@ hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64
 }
 
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s, int images) {
-  hipLaunchKernelGGL(scan_kernel, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
+                     hipStream_t s, int images, bool fp32) {
+  if (fp32) hipLaunchKernelGGL(scan_kernel<float>, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
+  else hipLaunchKernelGGL(scan_kernel<double>, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
   return hipGetLastError();
 }
 

@@ -63,7 +63,9 @@ struct FitExposeConsts {
   int n_emulsions;
 };
 
-hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s);
+// fp32: the mixed form's FP32 per-pixel arithmetic instead of the reference's FP64 (expose, capture LUT, scan)
+hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
+                       bool fp32 = false);
 // Batched over `images` images: rgb image b at rgb + 3 b n, its E planes at planes + b E n.
 hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s,
                            int images = 1);
@@ -93,11 +95,11 @@ hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, con
                       int radius, hipStream_t s, const BlurPlanes& pl = BlurPlanes{});
 // Batched over `images`: absorbed, image, mask E planes per image, bounce one plane per image.
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
-                               float* image, float* mask, hipStream_t s, int images = 1);
+                               float* image, float* mask, hipStream_t s, int images = 1, bool fp32 = false);
 hipError_t film_dir_apply(float* image, const float* inh, size_t n, const FilmDeviceTables* T, hipStream_t s);
 hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64_t key0, const float* taps, int radius,
                       double* partial, float d_max, float kappa, hipStream_t s);
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s, int images = 1);
+                     hipStream_t s, int images = 1, bool fp32 = false);
 
 }  // namespace imops

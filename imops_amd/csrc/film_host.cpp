@@ -352,8 +352,9 @@ void prepare_film(hipStream_t s, const HostStock& hs, const FilmDeviceTables& ba
 
 // Expose -> develop -> scan (NegativeLinear), or with the inversion when the
 // prepared tables carry it.  Kernels only: nothing is uploaded or awaited.
+// fp32: the mixed form's FP32 per-pixel spectral arithmetic (expose, capture LUT, scan)
 void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const float* rgb_in, float* rgb_out, int w,
-              int h, float pitch, uint64_t seed, FilmBuffers& B) {
+              int h, float pitch, uint64_t seed, FilmBuffers& B, bool fp32 = false) {
   const size_t n = (size_t)w * h;
   const Stock& st = *hs.s;
   const int E = P.T.n_emulsions;
@@ -361,7 +362,7 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
   const FilmDeviceTables* T = P.tables.as<FilmDeviceTables>();
   const float sigma_local = P.sigmas[0], sig_dir = P.sigmas[2], sig_adj = P.sigmas[3];
 
-  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s));
+  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s, fp32));
   if (sigma_local >= 1e-3f)  // local gelatin scatter, mix 0.18 (halation.rs:38-44)
     for (int e = 0; e < E; ++e) {
       float* p = B.absorbed + (size_t)e * n;
@@ -371,7 +372,7 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
     const float* deep = B.absorbed + (size_t)(E - 1) * n;
     IMOPS_HIP_CHECK(film_blur(deep, B.tmp, deep, B.bounce, w, h, taps + P.off[1], P.radius[1], BLUR_STORE, 0.0f, s));
   }
-  IMOPS_HIP_CHECK(film_capture_reduce(B.absorbed, P.do_wide ? B.bounce : nullptr, n, T, B.image, B.mask, s));
+  IMOPS_HIP_CHECK(film_capture_reduce(B.absorbed, P.do_wide ? B.bounce : nullptr, n, T, B.image, B.mask, s, 1, fp32));
   // DIR inhibition (diffusion.rs:39-84)
   if (E > 0 && st.n_dir > 0 && sig_dir >= 1e-3f) {
     for (int e = 0; e < E; ++e) {
@@ -396,7 +397,7 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
     IMOPS_HIP_CHECK(film_grain(B.image + (size_t)e * n, B.bounce, B.tmp, w, h, key0, taps + P.off[4], P.radius[4],
                                B.partial, d_max, kappa, s));
   }
-  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, T, rgb_out, s));
+  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, T, rgb_out, s, 1, fp32));
 }
 
 // Prepared runs of a context, by (stock, format, width, shutter, output).
@@ -521,7 +522,8 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
     }
   }
   // the scan writes straight into the image: every pixel's input was consumed by expose
-  run_film(s, hs, *P, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, p.seed, B);
+  // the 32x32 mid-gray calibration above stays FP64 in both forms
+  run_film(s, hs, *P, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, p.seed, B, p.fp32);
   img->color_space = IMOPS_CS_ACESCG;
 }
 

@@ -390,6 +390,9 @@ Stage make_stage(const Table& t) {
                                           str_param(t, "film_format", "Film35mm", "Film"),
                                           (uint64_t)num_param(t, "seed", 1, "Film"),
                                           str_param(t, "output", "NegativeLinear", "Film"));
+    const std::string prec = str_param(t, "precision", "fp64", "Film");
+    if (prec == "mixed") s.film.fp32 = true;
+    else if (prec != "fp64") throw Error{IMOPS_E_INVALID, "Invalid Film config"};
   } else {
     s.kind = K_OTHER;
   }

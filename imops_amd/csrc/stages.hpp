@@ -39,6 +39,7 @@ struct FilmStageParams {
   int format = 0;   // 0 Film35mm, 1 Film6x6, 2 Film4x5
   uint64_t seed = 1;
   int output = 0;   // 0 NegativeLinear, 1 PositiveLinear
+  bool fp32 = false;  // precision = "mixed": FP32 per-pixel spectral arithmetic (the reference's is FP64)
 };
 
 FilmStageParams film_stage_params_from_table(const std::string& stock, const std::string& format, uint64_t seed,

@@ -130,3 +130,37 @@ target_color_space = "Srgb"
     x = oracle.film(x, seed=3, output="PositiveLinear")
     x = oracle.cst(x, oracle.CS_ACESCG, oracle.CS_SRGB)
     assert np.abs(got - x).max() <= 2e-5
+
+
+# The mixed form's film (precision = "mixed"): FP32 per-pixel expose, capture LUT
+# and scan instead of the reference's FP64.  Bar: max abs error against the
+# oracle below 1e-4 of the output range (single-precision rounding of the 16-bin
+# spectral sums, log10 and 10^-d, amplified where the positive inversion takes
+# 1/T of a dense negative) and PSNR >= 80 dB, well above the form's 50 dB bar.
+@pytest.mark.parametrize("stock,output", [("ColorNeg200", "NegativeLinear"), ("Portra400", "PositiveLinear"),
+                                          ("TriX400", "NegativeLinear")])
+def test_film_mixed_precision_matches_fp64_oracle(gpu_ctx, stock, output):
+    print(f"film mixed {stock} {output}", flush=True)
+    rgb = _scene(160, 120, 5)
+    cfg = im.parse_config(f"""
+[[pipeline_modules]]
+name = "Film"
+stock = "{stock}"
+seed = 2
+output = "{output}"
+precision = "mixed"
+""")
+    img = im.Image(gpu_ctx).set_rgb(rgb, im.CS_ACESCG)
+    im.run_pixel_pipeline(img, cfg)
+    got = img.rgb()
+    want = oracle.film(rgb, stock=stock, seed=2, output=output)
+    scale = float(np.abs(want).max())
+    err = float(np.abs(got - want).max())
+    assert err <= 1e-4 * max(scale, 1.0), f"max abs err {err} (output range {scale})"
+    psnr = 10 * np.log10(scale ** 2 / max(float(np.mean((got - want) ** 2)), 1e-30))
+    assert psnr >= 80.0, psnr
+
+
+def test_film_precision_rejects_unknown(gpu_ctx):
+    with pytest.raises(Exception):
+        im.parse_config('[[pipeline_modules]]\nname = "Film"\nprecision = "fp16"\n')
