@@ -411,16 +411,18 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   if (wave_active) {
     sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
       // the union's columns past ux1 fail every reference's column test
-      const bool yin = yvalid && cy >= wy0 && cy <= wy1;
+      const bool yin = yvalid & (cy >= wy0) & (cy <= wy1);
       const bool selfrow = cy == ry;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        // the cheap tests combine as masks (no branches); only the rare
+        // classification below branches
         const int ox = cx - wx0[j];
-        const bool inw = yin && (unsigned)ox < (unsigned)ewv[j] && !(selfrow && cx == rxa[j]);
+        const bool inw = yin & ((unsigned)ox < (unsigned)ewv[j]) & !(selfrow & (cx == rxa[j]));
         const float d = (na[j] + nbv) - 2.0f * acc[j];
         const bool pre = (int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limgeo[j];
         const bool sure = d <= asure[j];
-        track(j, inw && sure && pre, d);
+        track(j, inw & sure & pre, d);
         if (inw && !sure) {  // rare: the threshold is within reach
           const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
           const bool ok = classify(j, d, nbv, rb, cx, cy);
@@ -551,17 +553,17 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       }
     }
     sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
-      const bool yin = yvalid && cy >= wy0 && cy <= wy1;
+      const bool yin = yvalid & (cy >= wy0) & (cy <= wy1);
       const bool selfrow = cy == ry;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int ox = cx - wx0[j];
-        const bool inw = yin && (unsigned)ox < (unsigned)ewv[j] && !(selfrow && cx == rxa[j]);
+        const bool inw = yin & ((unsigned)ox < (unsigned)ewv[j]) & !(selfrow & (cx == rxa[j]));
         const float d = (na[j] + nbv) - 2.0f * acc[j];
         const bool inA = d <= dA[j];  // lower bound may reach T
         const bool inB = d <= dK[j];  // approximate top K
-        if (inw && (inA || inB) && (int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limit[j] &&
-            (d <= asure[j] || classify(j, d, nbv, kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f, cx, cy))) {
+        const bool cand = inw & (inA | inB) & ((int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limit[j]);
+        if (cand && (d <= asure[j] || classify(j, d, nbv, kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f, cx, cy))) {
           const int slot = wave * 16 + 4 * q + j;
           const uint16_t li = (uint16_t)((cy - uy0) * UW + (cx - ux0));
           if (inA) {
