@@ -397,7 +397,8 @@ def main() -> int:
         "mixed": {"value": round(W * H / 1e6 / (ms_mx * 1e-3), 3), "unit": "MP/s", "ms_per_step": round(ms_mx, 3),
                   "speedup_vs_fp32": round(ms / ms_mx, 3),
                   "form": "BM3D block matching (FP32 re-rank) and 8x8 DCT / Walsh-Hadamard group transforms on FP16 MFMA "
-                          "with FP32 accumulation; other stages FP32",
+                          "with FP32 accumulation; film per-pixel spectral math in FP32 (FP64 in the plain form); "
+                          "other stages as the plain form",
                   "psnr_db_vs_fp32_form_full_frame": None if math.isinf(psnr_full) else round(psnr_full, 2),
                   "bit_identical_to_fp32_form": math.isinf(psnr_full),
                   "bm3d_patches_fallback_per_step": extra_mx["bm3d_fallback_refs"],
