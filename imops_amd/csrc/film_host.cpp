@@ -472,7 +472,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
     if (r.first == key) P = r.second.get();
   if (!P) {
     if (ctx->capturing) throw Error{IMOPS_E_STATE, "film: shape not prepared before stream capture"};
-    if (cache.runs.size() >= 64) cache.runs.clear();
+    if (cache.runs.size() >= 64 && !ctx->in_burst) cache.runs.clear();  // a burst keeps every shape it prepared
     cache.runs.emplace_back(key, std::unique_ptr<FilmPrepared>(new FilmPrepared()));
     P = cache.runs.back().second.get();
     FilmDeviceTables T = hs.t;

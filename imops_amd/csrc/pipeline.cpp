@@ -8,6 +8,8 @@
 // fused into one pointwise kernel launch; each stage keeps its own f32
 // arithmetic, so the output is the same as running them one by one.
 #include <cctype>
+#include <cerrno>
+#include <climits>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -28,6 +30,8 @@ struct Value {
   std::string s;
   double num = 0;
   long long i = 0;
+  unsigned long long u = 0;  // INT: the value when it is non-negative (covers 2^63 .. 2^64-1)
+  bool neg = false;          // INT: written with a minus sign
   bool b = false;
   std::vector<Value> arr;
   std::map<std::string, Value> obj;
@@ -89,8 +93,18 @@ Value parse_number(Cursor& c) {
   char* end = nullptr;
   if (tok.find_first_of(".eE") == std::string::npos) {
     v.kind = Value::INT;
-    v.i = std::strtoll(tok.c_str(), &end, 10);
-    v.num = (double)v.i;
+    v.neg = tok[0] == '-';
+    errno = 0;
+    if (v.neg) {
+      v.i = std::strtoll(tok.c_str(), &end, 10);
+      v.num = (double)v.i;
+    } else {
+      v.u = std::strtoull(tok.c_str(), &end, 10);
+      v.i = v.u > (unsigned long long)LLONG_MAX ? LLONG_MAX : (long long)v.u;
+      v.num = (double)v.u;
+    }
+    if (errno == ERANGE) bad("integer out of range '" + tok + "'");
+    if (v.neg && v.i == 0) v.neg = false;  // -0
   } else {
     v.kind = Value::NUM;
     v.num = std::strtod(tok.c_str(), &end);
@@ -267,6 +281,14 @@ double num_param(const Table& t, const char* key, double def, const char* mod) {
   return v->num;
 }
 
+// A u64 field (serde rejects floats, negative and out-of-range values for it).
+uint64_t u64_param(const Table& t, const char* key, uint64_t def, const char* mod) {
+  const Value* v = param(t, key);
+  if (!v) return def;
+  if (v->kind != Value::INT || v->neg) throw Error{IMOPS_E_INVALID, std::string("Invalid ") + mod + " config"};
+  return (uint64_t)v->u;
+}
+
 std::string str_param(const Table& t, const char* key, const std::string& def, const char* mod) {
   const Value* v = param(t, key);
   if (!v) return def;
@@ -388,7 +410,7 @@ Stage make_stage(const Table& t) {
     s.kind = K_FILM;
     s.film = film_stage_params_from_table(str_param(t, "stock", "ColorNeg200", "Film"),
                                           str_param(t, "film_format", "Film35mm", "Film"),
-                                          (uint64_t)num_param(t, "seed", 1, "Film"),
+                                          u64_param(t, "seed", 1, "Film"),
                                           str_param(t, "output", "NegativeLinear", "Film"));
     const std::string prec = str_param(t, "precision", "fp64", "Film");
     if (prec == "mixed") s.film.fp32 = true;
@@ -482,11 +504,6 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
 // are copied into the staging image, the graph runs, the output is copied back.
 namespace {
 
-struct FrameState {
-  bool has_raw, has_rgb;
-  int color_space;
-};
-
 std::string shape_key(const imops_image* f) {
   std::string k = std::to_string(f->width) + "x" + std::to_string(f->height) + "|" + (f->has_raw ? "r" : "-") +
                   (f->has_rgb ? "c" : "-") + "|" + std::to_string(f->color_space) + "|";
@@ -502,10 +519,9 @@ std::string shape_key(const imops_image* f) {
   if (f->has_wb) add_floats(f->wb, 4);
   add_floats(f->calib.data(), f->calib.size());
   if (f->has_shutter) add_floats(&f->shutter, 1);
-  if (f->has_baseline) {
-    const float e[3] = {f->baseline_ev, f->f_number, f->iso};
-    add_floats(e, 3);
-  }
+  k += std::string("|e") + (f->has_baseline ? "b" : "-") + (f->has_f_number ? "f" : "-") + (f->has_iso ? "i" : "-");
+  const float e[3] = {f->baseline_ev, f->f_number, f->iso};
+  add_floats(e, 3);
   if (f->has_opcodes) {
     k += "|o";
     for (uint8_t b : f->opcode_list3) k += std::to_string(b) + ",";
@@ -521,12 +537,13 @@ struct BurstShape {
 };
 
 void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* const* frames, size_t n) {
-  std::vector<FrameState> st(n);
+  // every frame's state before the stage list runs (the eager first frames change theirs)
+  std::vector<imops_image> pre(n);
   std::vector<BurstShape> shapes;
   std::vector<std::string> keys;
   std::vector<size_t> shape_of(n);
   for (size_t i = 0; i < n; ++i) {
-    st[i] = FrameState{frames[i]->has_raw, frames[i]->has_rgb, frames[i]->color_space};
+    pre[i].copy_state_from(*frames[i]);
     const std::string k = shape_key(frames[i]);
     size_t j = 0;
     while (j < keys.size() && keys[j] != k) ++j;
@@ -548,9 +565,11 @@ void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_im
 
   const bool timing = ctx->timer.enabled;
   ctx->timer.enabled = false;
+  ctx->in_burst = true;
   auto cleanup = [&] {
     ctx->timer.enabled = timing;
     ctx->capturing = false;
+    ctx->in_burst = false;
     (void)hipStreamSynchronize(ctx->stream);
     for (BurstShape& sh : shapes) {
       if (sh.exec) (void)hipGraphExecDestroy(sh.exec);
@@ -567,25 +586,10 @@ void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_im
     for (size_t j : order) {
       BurstShape& sh = shapes[j];
       if (sh.frames.size() < 2) continue;
-      const imops_image* rep = frames[sh.frames[0]];
-      const FrameState& rs = st[sh.frames[0]];
       imops_image* S = new imops_image();
       sh.staging = S;
+      S->copy_state_from(pre[sh.frames[0]]);  // all metadata the stages read, as the frames have it
       S->ctx = ctx;
-      S->width = rep->width;
-      S->height = rep->height;
-      S->has_raw = rs.has_raw;
-      S->has_rgb = rs.has_rgb;
-      S->color_space = rs.color_space;
-      S->cfa = rep->cfa;
-      S->has_cfa = rep->has_cfa;
-      S->crop_x = rep->crop_x;
-      S->crop_y = rep->crop_y;
-      std::memcpy(S->wb, rep->wb, sizeof S->wb);
-      S->has_wb = rep->has_wb;
-      S->calib = rep->calib;
-      S->shutter = rep->shutter;
-      S->has_shutter = rep->has_shutter;
       const size_t px = S->width * S->height;
       if (S->has_raw) S->raw.reserve(px * sizeof(float));
       S->rgb.reserve(px * 3 * sizeof(float));
@@ -611,16 +615,20 @@ void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_im
       BurstShape& sh = shapes[shape_of[i]];
       if (i == sh.frames[0]) continue;
       imops_image* F = frames[i];
-      imops_image* S = sh.staging;
-      const size_t px = F->width * F->height;
-      if (st[i].has_raw) IMOPS_HIP_CHECK(hipMemcpyAsync(S->raw.ptr, F->raw.ptr, px * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-      if (st[i].has_rgb)
+      imops_image* S = sh.staging;  // holds the state the captured stage list left (shape, flags, colour space)
+      const size_t px = pre[i].width * pre[i].height;
+      if (pre[i].has_raw)
+        IMOPS_HIP_CHECK(hipMemcpyAsync(S->raw.ptr, F->raw.ptr, px * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+      if (pre[i].has_rgb)
         IMOPS_HIP_CHECK(hipMemcpyAsync(S->rgb.ptr, F->rgb.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
       IMOPS_HIP_CHECK(hipGraphLaunch(sh.exec, ctx->stream));
-      F->rgb.reserve(px * 3 * sizeof(float));
-      IMOPS_HIP_CHECK(hipMemcpyAsync(F->rgb.ptr, S->rgb.ptr, px * 3 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-      F->has_rgb = S->has_rgb;
-      F->color_space = S->color_space;
+      const size_t px_out = S->width * S->height;  // a preview demosaic bins the frame
+      if (S->has_rgb) {
+        F->rgb.reserve(px_out * 3 * sizeof(float));
+        IMOPS_HIP_CHECK(hipMemcpyAsync(F->rgb.ptr, S->rgb.ptr, px_out * 3 * sizeof(float), hipMemcpyDeviceToDevice,
+                                       ctx->stream));
+      }
+      F->copy_state_from(*S);
     }
   } catch (...) {
     cleanup();

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -96,6 +97,9 @@ struct imops_ctx {
   // Set while the stage code runs under stream capture: stages then only
   // enqueue work (no host uploads, synchronisation or allocation).
   bool capturing = false;
+  // Set while a burst runs: per-shape prepared state (film runs) is then kept
+  // rather than evicted, since the burst's captures rely on it.
+  bool in_burst = false;
   // Launch-shape autotuning: when on, the first call of a tunable kernel at a
   // new shape times every candidate and keeps the fastest (by key).
   bool autotune = false;
@@ -143,6 +147,33 @@ struct imops_image {
   std::vector<uint8_t> opcode_list3;  // DNG OpcodeList3 bytes (Vignette)
   bool has_opcodes = false;
   void swap_rgb_tmp() { std::swap(rgb, tmp); }
+  // Every field but the device buffers and the context: the per-frame state a
+  // stage list reads or updates (shape, plane flags, CFA and crop, colour
+  // space and all metadata).
+  void copy_state_from(const imops_image& o) {
+    width = o.width;
+    height = o.height;
+    has_raw = o.has_raw;
+    has_rgb = o.has_rgb;
+    cfa = o.cfa;
+    has_cfa = o.has_cfa;
+    crop_x = o.crop_x;
+    crop_y = o.crop_y;
+    std::copy(o.wb, o.wb + 4, wb);
+    has_wb = o.has_wb;
+    calib = o.calib;
+    color_space = o.color_space;
+    shutter = o.shutter;
+    has_shutter = o.has_shutter;
+    baseline_ev = o.baseline_ev;
+    f_number = o.f_number;
+    iso = o.iso;
+    has_baseline = o.has_baseline;
+    has_f_number = o.has_f_number;
+    has_iso = o.has_iso;
+    opcode_list3 = o.opcode_list3;
+    has_opcodes = o.has_opcodes;
+  }
 };
 
 namespace imops {

@@ -164,3 +164,19 @@ precision = "mixed"
 def test_film_precision_rejects_unknown(gpu_ctx):
     with pytest.raises(Exception):
         im.parse_config('[[pipeline_modules]]\nname = "Film"\nprecision = "fp16"\n')
+
+
+def test_film_seed_beyond_2_pow_53_through_the_stage_list(gpu_ctx):
+    """The Film seed is a u64 (film/mod.rs:58): seeds 2^53 and 2^53 + 1 (equal as doubles)
+    must give different grain, each the oracle's, when they come through the stage list."""
+    rgb = _scene(96, 64, 5)
+    outs = {}
+    for seed in (2 ** 53, 2 ** 53 + 1, 2 ** 64 - 1):
+        cfg = im.parse_config('{"pipeline_modules": [{"name": "Film", "stock": "ColorNeg200", '
+                              f'"seed": {seed}}}]}}')
+        img = im.Image(gpu_ctx).set_rgb(rgb, im.CS_ACESCG)
+        im.run_pixel_pipeline(img, cfg)
+        outs[seed] = img.rgb()
+        err = np.abs(outs[seed] - oracle.film(rgb, seed=seed)).max()
+        assert err <= TOL, f"seed {seed}: max abs err {err}"
+    assert not np.array_equal(outs[2 ** 53], outs[2 ** 53 + 1])

@@ -108,10 +108,22 @@ target_color_space = "Srgb"
     ('[[pipeline_modules]]\nname = "Demosaic"\nalgorithm = "Amaze"\n', "Invalid Demosaic config"),
     ('[[pipeline_modules]]\nname = "Exp"\nev = "one"\n', "Invalid Exp config"),
     ("not a config", "cant decode"),
+    # Film seed is a u64 (film/mod.rs:58): floats and negatives are rejected, not rounded or wrapped
+    ('[[pipeline_modules]]\nname = "Film"\nseed = -3\n', "Invalid Film config"),
+    ('[[pipeline_modules]]\nname = "Film"\nseed = 1.5\n', "Invalid Film config"),
+    ('[[pipeline_modules]]\nname = "Film"\nseed = 1e3\n', "Invalid Film config"),
+    ('{"pipeline_modules": [{"name": "Film", "seed": 18446744073709551616}]}', "cant decode"),
 ])
 def test_parse_errors(bad, msg):
     with pytest.raises(im.ImopsError, match=msg):
         im.parse_config(bad)
+
+
+def test_film_seed_accepts_the_whole_u64_range():
+    for text in ('[[pipeline_modules]]\nname = "Film"\nseed = 9007199254740993\n',
+                 '{"pipeline_modules": [{"name": "Film", "seed": 18446744073709551615}]}',
+                 '[[pipeline_modules]]\nname = "Film"\nseed = 0\n'):
+        assert im.parse_config(text).pipeline_modules == ["Film"]
 
 
 def test_synth_deterministic_and_in_range():
