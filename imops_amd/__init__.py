@@ -359,6 +359,15 @@ def run_pixel_pipeline(image: Image, config: PipelineConfig) -> Image:
     return image
 
 
+def demosaic_amaze_noalloc(ctx: Context, cfa_plane_ptr: int, rgb_out_ptr: int, width: int, height: int,
+                           cfa: str = "RGGB", crop_origin=(0, 0)) -> None:
+    """AMaZE from a caller-owned device CFA plane into a caller-owned device RGB buffer
+    (raw device addresses, e.g. torch's data_ptr()), allocating nothing; enqueued on the
+    context's stream.  See imops_demosaic_amaze_noalloc in include/imops_gpu.h."""
+    check(lib().imops_demosaic_amaze_noalloc(ctx.handle, cfa_plane_ptr, rgb_out_ptr, width, height, cfa.encode(),
+                                             crop_origin[0], crop_origin[1]))
+
+
 def run_burst(frames: list, config: PipelineConfig, graphs: bool = True) -> list:
     """The stage list over every frame (imops_pipeline_run_burst); frames of one
     shape after the first replay a HIP graph captured on a staging image."""

@@ -75,6 +75,7 @@ SIGNATURES = {
     "imops_image_set_exposure_metadata": (_i, [_vp, _f, _f, _f]),
     "imops_image_set_opcode_list3": (_i, [_vp, _vp, _sz]),
     "imops_demosaic": (_i, [_vp, _vp, _i]),
+    "imops_demosaic_amaze_noalloc": (_i, [_vp, _vp, _vp, _sz, _sz, C.c_char_p, _sz, _sz]),
     "imops_cfa_coeffs": (_i, [_vp, _vp, _fp]),
     "imops_exp": (_i, [_vp, _vp, _f]),
     "imops_gamma": (_i, [_vp, _vp, _f]),

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "amaze.hpp"
 
 namespace imops {
@@ -883,7 +885,31 @@ __global__ void __launch_bounds__(NT) amaze_kernel(AmazeArgs a) {
   }
 }
 
+// Scratch of the allocation-free entry point: kAmazeNoallocSlots tile slots.
+// Every pass writes each element of the plane it defines before any pass
+// reads it (zeros outside the pass's valid region included), so nothing in a
+// slot has to start at zero, and a slot's contents from an earlier tile or
+// call are never read.  Host code must use the address from
+// amaze_noalloc_scratch, never this name.
+__device__ __align__(256) float scratch[(size_t)kAmazeNoallocSlots * kAmazeSlotFloats];
+
 }  // namespace
+
+hipError_t amaze_noalloc_scratch(int device, float** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static float* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices || !out) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = static_cast<float*>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
 
 size_t amaze_tiles(int width, int height, int* tiles_x, int* tiles_y) {
   int tx = 0, ty = 0;

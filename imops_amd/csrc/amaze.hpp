@@ -19,7 +19,15 @@ struct AmazeArgs {
   uint8_t cfa[2][2]; // colour of the (crop-shifted) Bayer pattern at (r&1, c&1)
 };
 
+// Tile slots of the allocation-free entry point, whose scratch is a static
+// device array of kAmazeNoallocSlots * kAmazeSlotFloats floats (69 MB) per GPU.
+constexpr int kAmazeNoallocSlots = 64;
+
 size_t amaze_tiles(int width, int height, int* tiles_x, int* tiles_y);
+// Device address of the static scratch array on `device`, which must be the
+// current device.  Looked up once per device (later calls return the cached
+// address); safe to call from several threads.
+hipError_t amaze_noalloc_scratch(int device, float** out);
 hipError_t amaze_launch(const AmazeArgs& args, int grid, hipStream_t stream);
 
 }  // namespace imops

@@ -416,6 +416,26 @@ void stage_pointwise(imops_ctx* ctx, imops_image* img, const PointwiseProgram& p
   IMOPS_HIP_CHECK(pointwise_launch(img->rgb.as<float>(), img->width * img->height, prog, ctx->stream));
 }
 
+// AMaZE launch arguments for a w x h CFA plane with the crop-shifted pattern `cfa`;
+// rejects the shapes the reference cannot process.
+AmazeArgs amaze_args(const CfaTable& cfa, size_t w, size_t h, const float* in, float* out) {
+  if (!cfa.is_bayer()) throw Error{IMOPS_E_UNSUPPORTED, "AMaZE needs a 2x2 Bayer pattern, got " + cfa.name};
+  const size_t rw = w % 96, rh = h % 96;
+  if (w < 2 || h < 2 || (rw >= 1 && rw <= 15) || (rh >= 1 && rh <= 15))
+    throw Error{IMOPS_E_UNSUPPORTED,
+                "AMaZE: width and height mod 96 must not be 1..15 (the reference indexes past its tile buffer)"};
+  if (w > (1u << 30) / 3 || h > (1u << 20)) throw Error{IMOPS_E_INVALID, "AMaZE: image too large"};
+  AmazeArgs a{};
+  a.in = in;
+  a.out = out;
+  a.width = (int)w;
+  a.height = (int)h;
+  amaze_tiles(a.width, a.height, &a.tiles_x, &a.tiles_y);
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) a.cfa[r][c] = cfa.pat[r][c];
+  return a;
+}
+
 void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
   if (!img->has_raw) throw Error{IMOPS_E_STATE, "Demosaic: image has no raw CFA data"};
   if (!img->has_cfa) throw Error{IMOPS_E_STATE, "A CFA must be set to demosaic raw images"};
@@ -423,25 +443,12 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
   const CfaTable cfa = img->cfa.shift(img->crop_x, img->crop_y);  // demosaic.rs:93-98
   img->rgb.reserve(w * h * 3 * sizeof(float));
   if (algorithm == IMOPS_DEMOSAIC_AMAZE) {
-    if (!cfa.is_bayer())
-      throw Error{IMOPS_E_UNSUPPORTED, "AMaZE needs a 2x2 Bayer pattern, got " + cfa.name};
-    const size_t rw = w % 96, rh = h % 96;
-    if (w < 2 || h < 2 || (rw >= 1 && rw <= 15) || (rh >= 1 && rh <= 15))
-      throw Error{IMOPS_E_UNSUPPORTED,
-                  "AMaZE: width and height mod 96 must not be 1..15 (the reference indexes past its tile buffer)"};
-    if (w > (1u << 30) / 3 || h > (1u << 20)) throw Error{IMOPS_E_INVALID, "AMaZE: image too large"};
-    AmazeArgs a{};
-    a.in = img->raw.as<float>();
-    a.out = img->rgb.as<float>();
-    a.width = (int)w;
-    a.height = (int)h;
-    const size_t ntiles = amaze_tiles(a.width, a.height, &a.tiles_x, &a.tiles_y);
+    AmazeArgs a = amaze_args(cfa, w, h, img->raw.as<float>(), img->rgb.as<float>());
+    const size_t ntiles = (size_t)a.tiles_x * a.tiles_y;
     int grid = ctx->num_cus > 0 ? ctx->num_cus : 256;  // one 144 KiB-LDS workgroup per CU
     if ((size_t)grid > ntiles) grid = (int)ntiles;
     ctx->scr[SCR_AMAZE].reserve((size_t)grid * kAmazeSlotFloats * sizeof(float));
     a.scratch = ctx->scr[SCR_AMAZE].as<float>();
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) a.cfa[r][c] = cfa.pat[r][c];
     TimedRegion tr(ctx, "amaze");
     IMOPS_HIP_CHECK(amaze_launch(a, grid, ctx->stream));
   } else if (algorithm == IMOPS_DEMOSAIC_MARKESTEIJN) {
@@ -532,7 +539,13 @@ int imops_ctx_create(int device, imops_ctx** out) {
     c->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // the static scratch of the allocation-free entry point, looked up once per device
+    hipError_t e = amaze_noalloc_scratch(device, &c->amaze_noalloc);
+    if (e != hipSuccess) {
+      delete c;
+      throw Error{IMOPS_E_HIP, std::string("hipGetSymbolAddress(AMaZE scratch): ") + hipGetErrorString(e)};
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       delete c;
       throw Error{IMOPS_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
@@ -848,6 +861,28 @@ int imops_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
   return guarded([&] {
     if (!ctx || !img) throw Error{IMOPS_E_INVALID, "demosaic: null argument"};
     stage_demosaic(ctx, img, algorithm);
+  });
+}
+
+int imops_demosaic_amaze_noalloc(imops_ctx* ctx, const float* cfa_plane, float* rgb_out, size_t width,
+                                 size_t height, const char* cfa_pattern, size_t crop_x, size_t crop_y) {
+  return guarded([&] {
+    if (!ctx || !cfa_plane || !rgb_out || !cfa_pattern)
+      throw Error{IMOPS_E_INVALID, "demosaic_amaze_noalloc: null argument"};
+    if (!ctx->amaze_noalloc) throw Error{IMOPS_E_STATE, "demosaic_amaze_noalloc: no static scratch on this device"};
+    CfaTable cfa;
+    if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
+    AmazeArgs a = amaze_args(cfa.shift(crop_x, crop_y), width, height, cfa_plane, rgb_out);
+    // the scratch holds kAmazeNoallocSlots tiles in flight whatever the image size, so no
+    // image is too large for it; the grid never exceeds the slots it has
+    const size_t ntiles = (size_t)a.tiles_x * a.tiles_y;
+    int grid = kAmazeNoallocSlots;
+    if (ctx->num_cus > 0 && ctx->num_cus < grid) grid = ctx->num_cus;
+    if ((size_t)grid > ntiles) grid = (int)ntiles;
+    if ((size_t)grid * kAmazeSlotFloats > (size_t)kAmazeNoallocSlots * kAmazeSlotFloats)
+      throw Error{IMOPS_E_INVALID, "demosaic_amaze_noalloc: grid exceeds the static scratch"};
+    a.scratch = ctx->amaze_noalloc;
+    IMOPS_HIP_CHECK(amaze_launch(a, grid, ctx->stream));
   });
 }
 

@@ -88,6 +88,7 @@ struct imops_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  float* amaze_noalloc = nullptr;  // this device's static AMaZE scratch (allocation-free entry point)
   imops::DevBuf scr[SCR_COUNT];
   imops::KernelTimer timer;
   std::unique_ptr<imops::StageCache> film_cache;  // prepared film runs by shape

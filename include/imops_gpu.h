@@ -149,6 +149,24 @@ int imops_image_set_opcode_list3(imops_image* img, const uint8_t* bytes, size_t 
  * its tile there); markesteijn (demosaic.rs:108-186) takes any pattern. */
 int imops_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);
 
+/* AMaZE demosaic that allocates nothing.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Reads the
+ * caller's device f32 CFA plane (width*height, normalised, of the crop window
+ * whose origin is crop_x, crop_y in the pattern's frame) and writes the
+ * caller's device RGB buffer (width*height*3 f32), both on ctx's stream; the
+ * call only enqueues (synchronise the context, or order with events, before
+ * reading the output).  Same results and the same size rules as
+ * imops_demosaic(IMOPS_DEMOSAIC_AMAZE).  Its tile scratch is a static device
+ * array of 64 tile slots (69 MB) that each GPU holds until the process ends;
+ * the slot count bounds the tiles in flight, not the image size, and every
+ * pass rewrites the slot planes it reads, so nothing has to be cleared between
+ * calls.  All calls on one GPU share that array: calls that overlap on one GPU
+ * (two contexts' streams, or two threads) race and must be serialised by the
+ * caller. */
+int imops_demosaic_amaze_noalloc(imops_ctx* ctx, const float* cfa_plane, float* rgb_out, size_t width,
+                                 size_t height, const char* cfa_pattern, size_t crop_x, size_t crop_y);
+
 /* Module<CFACoeffs>::process (modules.rs:533-540, cfa_coeffs.rs:4-12);
  * wb = NULL uses the image's wb_coeffs. */
 int imops_cfa_coeffs(imops_ctx* ctx, imops_image* img, const float* wb);
