@@ -121,12 +121,17 @@ def psnr(a: np.ndarray, b: np.ndarray) -> float:
     return float("inf") if mse == 0 else 10.0 * math.log10(1.0 / mse)
 
 
-def oracle_chain(raw_crop: np.ndarray, cal: np.ndarray, fp64: bool) -> np.ndarray:
+def oracle_chain(raw_crop: np.ndarray, cal: np.ndarray, fp64: bool, cfa: str = "RGGB",
+                 demosaic: str = "amaze") -> np.ndarray:
+    """The oracle (FP32 exact-order or FP64) through the same stage list as stages()."""
     import oracle
     p = oracle.bm3d_params(BM3D_SIGMA * 255 / 80, sigma=BM3D_SIGMA * 255, stride=BM3D_STRIDE, window=BM3D_WINDOW)
     p["lambda"] = BM3D_LAMBDA
     plane = oracle.normalize_raw(raw_crop, BLACK, WHITE)
-    x = oracle.amaze(plane, "RGGB", fp64=fp64)
+    if demosaic == "amaze":
+        x = oracle.amaze(plane, cfa, fp64=fp64)
+    else:
+        x = oracle.markesteijn(plane, cfa)
     x, _ = oracle.bm3d(x, p, fp64=fp64)
     x = oracle.camera_cst(x, cal.ravel(), oracle.CS_ACESCG)
     x = oracle.exp(x, np.float32(EV))

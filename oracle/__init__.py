@@ -35,6 +35,7 @@ _lib = None
 
 _SIGS = {
     "oracle_amaze": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp, C.c_int]),
+    "oracle_amaze_hooks": (None, [C.c_int, C.c_void_p, C.c_int, C.c_int]),
     "oracle_markesteijn": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, _fp]),
     "oracle_preview_demosaic": (C.c_int, [_fp, _sz, _sz, C.c_char_p, _sz, _sz, C.c_int, _fp]),
     "oracle_normalize_raw": (C.c_int, [C.c_void_p, _sz, _sz, _sz, _sz, _sz, _sz, C.c_float, C.c_float, _fp]),
@@ -107,6 +108,24 @@ def amaze(cfa_plane: np.ndarray, pattern: str = "RGGB", crop_origin=(0, 0), fp64
     _rc(lib().oracle_amaze(_f(x), w, h, pattern.encode(), crop_origin[0], crop_origin[1], _f(out), int(fp64)),
         "amaze")
     return out
+
+
+AMAZE_TRACE = ["dirwts0", "dirwts1", "delhvsqsum", "vcd_pre", "hcd_pre", "vcdalt", "hcdalt", "dgintv", "dginth",
+               "vcd", "hcd", "cddiffsq", "hvwt", "nyqutest", "nyquist", "do_nyquist"]
+
+
+def amaze_traced(cfa_plane: np.ndarray, pattern: str = "RGGB", tile=(-16, -16), plant: int = 0):
+    """AMaZE (FP32 order) returning (output, {plane name: 128 x 128 tile plane}) for the
+    tile whose top-left corner is `tile` = (top, left); `plant` selects a deliberate
+    restatement mistake (test hook, 0 = none)."""
+    tr = np.full((len(AMAZE_TRACE), 128, 128), np.nan, np.float32)
+    L = lib()
+    L.oracle_amaze_hooks(plant, tr.ctypes.data, tile[0], tile[1])
+    try:
+        out = amaze(cfa_plane, pattern)
+    finally:
+        L.oracle_amaze_hooks(0, None, 0, 0)
+    return out, dict(zip(AMAZE_TRACE, tr))
 
 
 def markesteijn(cfa_plane: np.ndarray, pattern: str, crop_origin=(0, 0)) -> np.ndarray:

@@ -3,13 +3,24 @@
 // tile grid top,left = -16 + 96k (demosaic.rs:397-407).  Instantiated for
 // float (exact reference op order) and double (FP64 error oracle).
 // Pins: uniform input (the reference's own demosaic tests), closed-form linear
-// ramps, native-site passthrough and the FP64 instantiation.  Absolute values
-// on textured input are otherwise parity unpinned (no reference vectors exist).
+// ramps, native-site passthrough, and on textured input the exact transpose
+// equivariance of the passes demosaic.rs:612-786, the rotation equivariance of
+// the stencil passes and the closed-form Nyquist boundary
+// (tests/test_oracle_amaze_cpu.py, which also plants mistakes through the
+// hooks below to show each pin catches one).
 #include <vector>
 
-#include "oracle_common.hpp"
+#include "oracle_colour.hpp"
 
 namespace oracle {
+
+// Test hooks, used only by tests/test_oracle_amaze_cpu.py: `plant` selects a
+// deliberate restatement mistake (0: none) so the tests can show that their
+// checks catch it; `trace`, when set, receives per-pass planes of the tile
+// whose top-left corner is (trace_top, trace_left), each as a 128 x 128 float
+// plane in tile coordinates (half-resolution planes expanded to the sites
+// they belong to, NaN elsewhere).
+AmazeHooks g_amaze_hooks;
 
 namespace {
 
@@ -38,12 +49,32 @@ struct AmazeTile {
   static T clampnan(T x, T m, T mx) { return std::isnan(x) ? (m + mx) * (T)0.5 : rclamp(x, m, mx); }
   static T interpolatef(T w, T a, T b) { return w * a + ((T)1 - w) * b; }
 
+  float* trace_plane(int top, int left, int k) const {
+    const AmazeHooks& hk = g_amaze_hooks;
+    if (!hk.trace || top != hk.trace_top || left != hk.trace_left) return nullptr;
+    return hk.trace + (size_t)k * 16384;
+  }
+  void trace_full(int top, int left, int k, const std::vector<T>& v) const {
+    if (float* d = trace_plane(top, left, k))
+      for (int i = 0; i < 16384; ++i) d[i] = (float)v[i];
+  }
+  template <class V>
+  void trace_half(int top, int left, int k, const std::vector<V>& v, const Cfa& cf) const {
+    float* d = trace_plane(top, left, k);
+    if (!d) return;
+    for (int rr = 0; rr < 128; ++rr)
+      for (int cc = 0; cc < 128; ++cc) {
+        const int par = cf.color_at((size_t)((top + rr) % 48 + 48) % 48, (size_t)((left + 2) % 48 + 48) % 48) & 1;
+        d[rr * 128 + cc] = ((cc & 1) == par) ? (float)v[(rr * 128 + cc) >> 1] : NAN;
+      }
+  }
+
   void run(const float* input, size_t width, size_t height, const Cfa& cf, int top, int left, float* out,
            T clip_pt) {
     const T eps = (T)1e-5f, epssq = (T)1e-10f, arthresh = (T)0.75f;
     const T gaussodd[4] = {(T)0.14659727707323927f, (T)0.103592713382435f, (T)0.0732036125103057f,
                            (T)0.0365543548389495f};
-    const T nyqthresh = (T)0.5f;
+    const T nyqthresh = g_amaze_hooks.plant == 5 ? (T)5.0f : (g_amaze_hooks.plant == 6 ? (T)0.05f : (T)0.5f);
     const T gaussgrad[6] = {nyqthresh * (T)0.07384411893421103f, nyqthresh * (T)0.06207511968171489f,
                             nyqthresh * (T)0.0521818194747806f,  nyqthresh * (T)0.03687419286733595f,
                             nyqthresh * (T)0.03099732204057846f, nyqthresh * (T)0.018413194161458882f};
@@ -100,9 +131,13 @@ struct AmazeTile {
         T delh = std::fabs(cfa[i + 1] - cfa[i - 1]);
         T delv = std::fabs(cfa[i + V1] - cfa[i - V1]);
         dirwts0[i] = eps + std::fabs(cfa[i + V2] - cfa[i]) + std::fabs(cfa[i] - cfa[i - V2]) + delv;
-        dirwts1[i] = eps + std::fabs(cfa[i + 2] - cfa[i]) + std::fabs(cfa[i] - cfa[i - 2]) + delh;
+        dirwts1[i] = eps + std::fabs(cfa[i + 2] - cfa[i]) +
+                     std::fabs(cfa[i] - cfa[g_amaze_hooks.plant == 1 ? i + 2 : i - 2]) + delh;
         delhvsqsum[i] = sqr(delh) + sqr(delv);
       }
+    trace_full(top, left, 0, dirwts0);
+    trace_full(top, left, 1, dirwts1);
+    trace_full(top, left, 2, delhvsqsum);
 
     const T clip_pt8 = clip_pt * (T)0.8f;
 
@@ -116,7 +151,9 @@ struct AmazeTile {
         T crd = cfa[i + V1] * (dirwts0[i + V2] + dirwts0[i]) /
                 rmax(dirwts0[i + V2] * (eps + cfa[i]) + dirwts0[i] * (eps + cfa[i + V2]), (T)1e-10f);
         T crl = cfa[i - 1] * (dirwts1[i - 2] + dirwts1[i]) /
-                rmax(dirwts1[i - 2] * (eps + cfa[i]) + dirwts1[i] * (eps + cfa[i - 2]), (T)1e-10f);
+                rmax(dirwts1[g_amaze_hooks.plant == 2 ? i + 2 : i - 2] * (eps + cfa[i]) +
+                         dirwts1[i] * (eps + cfa[i - 2]),
+                     (T)1e-10f);
         T crr = cfa[i + 1] * (dirwts1[i + 2] + dirwts1[i]) /
                 rmax(dirwts1[i + 2] * (eps + cfa[i]) + dirwts1[i] * (eps + cfa[i + 2]), (T)1e-10f);
         T guha = cfa[i - V1] + (cfa[i] - cfa[i - V2]) * (T)0.5f;
@@ -158,6 +195,12 @@ struct AmazeTile {
         dginth[i] = rmin(sqr(glha - grha), sqr(gl - gr));
       }
     }
+    trace_full(top, left, 3, vcd);
+    trace_full(top, left, 4, hcd);
+    trace_full(top, left, 5, vcdalt);
+    trace_full(top, left, 6, hcdalt);
+    trace_full(top, left, 7, dgintv);
+    trace_full(top, left, 8, dginth);
 
     // variance-based choice and clamp, in place (demosaic.rs:700-786)
     for (int rr = 4; rr < rr1 - 4; ++rr)
@@ -198,7 +241,7 @@ struct AmazeTile {
           T gintv = vcd[i] + cfa[i];
           if (hcd[i] < 0) {
             if ((T)3 * hcd[i] < -(ginth + cfa[i])) {
-              hcd[i] = ulim(ginth, cfa[i - 1], cfa[i + 1]) - cfa[i];
+              hcd[i] = ulim(ginth, cfa[g_amaze_hooks.plant == 3 ? i + 1 : i - 1], cfa[i + 1]) - cfa[i];
             } else {
               T hwt = (T)1 + (T)3 * hcd[i] / (eps + ginth + cfa[i]);
               hcd[i] = hwt * hcd[i] + ((T)1 - hwt) * (ulim(ginth, cfa[i - 1], cfa[i + 1]) - cfa[i]);
@@ -217,6 +260,10 @@ struct AmazeTile {
           cddiffsq[i] = sqr(vcd[i] - hcd[i]);
         }
       }
+
+    trace_full(top, left, 9, vcd);
+    trace_full(top, left, 10, hcd);
+    trace_full(top, left, 11, cddiffsq);
 
     // horizontal/vertical weights (demosaic.rs:788-828)
     for (int rr = 6; rr < rr1 - 6; ++rr)
@@ -242,11 +289,13 @@ struct AmazeTile {
         T hcdvar1 = epssq + hwt * dgrbhvarr1 + ((T)1 - hwt) * dgrbhvarl1;
         T varwt = hcdvar / rmax(vcdvar + hcdvar, (T)1e-10f);
         T diffwt = hcdvar1 / rmax(vcdvar1 + hcdvar1, (T)1e-10f);
+        if (g_amaze_hooks.plant == 4) diffwt = hcdvar1 / rmax(vcdvar1 + hcdvar1 + dginth[i + 1] * (T)1e-3f, (T)1e-10f);
         if (((T)0.5f - varwt) * ((T)0.5f - diffwt) > 0 && std::fabs((T)0.5f - diffwt) < std::fabs((T)0.5f - varwt))
           hvwt[i >> 1] = varwt;
         else
           hvwt[i >> 1] = diffwt;
       }
+    trace_half(top, left, 12, hvwt, cf);
 
     // Nyquist texture test (demosaic.rs:830-847)
     for (int rr = 6; rr < rr1 - 6; ++rr)
@@ -267,6 +316,7 @@ struct AmazeTile {
              gaussgrad[5] * (delhvsqsum[i - M2] + delhvsqsum[i - P2] + delhvsqsum[i + P2] + delhvsqsum[i + M2]));
       }
 
+    trace_half(top, left, 13, nyqutest, cf);
     int nystartrow = 0, nyendrow = 0, nystartcol = TS + 1, nyendcol = 0;
     for (int rr = 6; rr < rr1 - 6; ++rr)
       for (int cc = 6 + (color_at(top + rr, left + 2) & 1); cc < cc1 - 6; cc += 2) {
@@ -279,7 +329,9 @@ struct AmazeTile {
           nyendcol = std::max(nyendcol, cc);
         }
       }
+    trace_half(top, left, 14, nyquist, cf);
     const bool do_nyquist = nystartrow != nyendrow && nystartcol != nyendcol;
+    if (float* d = trace_plane(top, left, 15)) d[0] = do_nyquist ? 1.0f : 0.0f;
     if (do_nyquist) {
       nyendrow += 1;
       nyendcol += 1;

@@ -23,6 +23,15 @@ int oracle_amaze(const float* in, size_t w, size_t h, const char* cfa_pattern, s
   return 0;
 }
 
+// Test hooks of the AMaZE restatement: a planted mistake (0 = none) and the
+// per-pass trace of one tile (16 planes of 128 x 128 floats, or NULL).
+void oracle_amaze_hooks(int plant, float* trace, int tile_top, int tile_left) {
+  g_amaze_hooks.plant = plant;
+  g_amaze_hooks.trace = trace;
+  g_amaze_hooks.trace_top = tile_top;
+  g_amaze_hooks.trace_left = tile_left;
+}
+
 int oracle_markesteijn(const float* in, size_t w, size_t h, const char* cfa_pattern, size_t crop_x, size_t crop_y,
                        float* out) {
   Cfa base;

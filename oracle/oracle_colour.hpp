@@ -26,6 +26,15 @@ void gamma_encode(float* rgb, size_t n, float gamma);
 void cfa_coeffs(float* rgb, size_t n, const float wb[4]);
 void markesteijn(const float* in, size_t width, size_t height, const Cfa& cfa, float* out);
 
+// Test hooks of the AMaZE restatement (see amaze.cpp): trace receives 16
+// planes of 128 x 128 floats for the tile at (trace_top, trace_left).
+struct AmazeHooks {
+  int plant = 0;
+  float* trace = nullptr;
+  int trace_top = 0, trace_left = 0;
+};
+extern AmazeHooks g_amaze_hooks;
+
 bool amaze_size_supported(size_t width, size_t height);
 template <class T>
 void amaze_demosaic(const float* input, size_t width, size_t height, const Cfa& cfa, float* out);
