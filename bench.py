@@ -4,8 +4,9 @@ Workload: a 6016x4016 RGGB 14-bit raw (black 600, white 16383) from the seeded
 scene generator, resident in HBM as u16.  A step is one pass of
 
   Demosaic{amaze} -> BM3D{stride 3, 39x39 window, <=16 matches, lambda 2.7,
-  sigma 0.02} -> CST{AcesCg} -> Exp{ev = log2 800} -> Film{ColorNeg200} ->
-  CST{Srgb}
+  sigma 0.02} -> CST{AcesCg} -> Exp{ev = log2 800} -> Film{synthetic 3-layer
+  stock, 81 bins 380-780 nm, 5-knot curves, Philox grain sigma 0.01, halation
+  radius 40 px strength 0.15} -> CST{Srgb}
 
 from the u16 frame to the f32 interleaved sRGB output in HBM.  `value` is the
 plain FP32 form (the reference's arithmetic); the same stage list with
@@ -68,7 +69,13 @@ ev = {ev}
 
 [[pipeline_modules]]
 name = "Film"
-stock = "ColorNeg200"
+stock = "Synthetic"
+stock_seed = {stock_seed}
+spectral_bins = {bins}
+grain = "philox"
+grain_sigma = {grain_sigma}
+halation_radius = {hal_radius}
+halation_strength = {hal_strength}
 film_format = "Film35mm"
 seed = 1
 output = "NegativeLinear"
@@ -80,15 +87,27 @@ target_color_space = "Srgb"
 """
 
 
+# The workload's film: the seeded synthetic 3-layer colour negative (two-Gaussian
+# sensitivities, 5-knot characteristic curves) on the 81-bin 380..780 nm grid,
+# Philox grain of density sigma 0.01, halation of radius 40 px and strength 0.15.
+FILM_EXT = {"stock": "Synthetic", "stock_seed": 7, "spectral_bins": 81, "grain": "philox", "grain_sigma": 0.01,
+            "halation_radius": 40.0, "halation_strength": 0.15}
+
+
 def stages(precision: str) -> str:
     # the mixed form also runs the film's per-pixel spectral math in FP32
     return STAGES_TMPL.format(sigma=BM3D_SIGMA, lam=BM3D_LAMBDA, stride=BM3D_STRIDE, window=BM3D_WINDOW,
                               precision=precision, ev=repr(EV),
-                              film_precision="mixed" if precision == "mixed" else "fp64")
+                              film_precision="mixed" if precision == "mixed" else "fp64",
+                              stock_seed=FILM_EXT["stock_seed"], bins=FILM_EXT["spectral_bins"],
+                              grain_sigma=FILM_EXT["grain_sigma"], hal_radius=FILM_EXT["halation_radius"],
+                              hal_strength=FILM_EXT["halation_strength"])
 
 
 STAGE_NAMES = ["Demosaic{amaze}", f"BM3D{{stride {BM3D_STRIDE}, window {BM3D_WINDOW}, lambda {BM3D_LAMBDA}, "
-               f"sigma {BM3D_SIGMA}}}", "CST{AcesCg}", "Exp{log2 800}", "Film{ColorNeg200}", "CST{Srgb}"]
+               f"sigma {BM3D_SIGMA}}}", "CST{AcesCg}", "Exp{log2 800}",
+               "Film{Synthetic seed 7, 81 bins 380-780 nm, Philox grain sigma 0.01, halation radius 40 px strength 0.15}",
+               "CST{Srgb}"]
 
 
 def calibration_matrix(seed: int = 7) -> np.ndarray:
@@ -135,7 +154,7 @@ def oracle_chain(raw_crop: np.ndarray, cal: np.ndarray, fp64: bool, cfa: str = "
     x, _ = oracle.bm3d(x, p, fp64=fp64)
     x = oracle.camera_cst(x, cal.ravel(), oracle.CS_ACESCG)
     x = oracle.exp(x, np.float32(EV))
-    x = oracle.film(x, seed=1, fp64=fp64)
+    x = oracle.film(x, seed=1, fp64=fp64, **FILM_EXT)
     return oracle.cst(x, oracle.CS_ACESCG, oracle.CS_SRGB)
 
 

@@ -36,7 +36,7 @@ _ALGORITHMS = {"amaze": DEMOSAIC_AMAZE, "markesteijn": DEMOSAIC_MARKESTEIJN, "fa
                "superfast": _lib.DEMOSAIC_SUPERFAST, "supersuperfast": _lib.DEMOSAIC_SUPERSUPERFAST}
 # Film stage choices (modules.rs:800-830; StockId order of film/stock/mod.rs:124-133)
 STOCKS = {"BwStub": 0, "ColorNeg200": 1, "Portra400": 2, "Ektar100": 3, "FujiPro400H": 4, "EktachromeE100": 5,
-          "TriX400": 6}
+          "TriX400": 6, "Synthetic": 7}
 FILM_FORMATS = {"Film35mm": 0, "Film6x6": 1, "Film4x5": 2}
 FILM_OUTPUTS = {"NegativeLinear": 0, "PositiveLinear": 1}
 
@@ -304,10 +304,31 @@ class Image:
         return self
 
     def film(self, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
-             output: str = "NegativeLinear") -> "Image":
-        """Film simulation (film/mod.rs:77-129); the image must be absolute-luminance ACEScg."""
-        check(lib().imops_film(self.ctx.handle, self._h, STOCKS[stock], FILM_FORMATS[film_format], int(seed),
-                               FILM_OUTPUTS[output]))
+             output: str = "NegativeLinear", **ext) -> "Image":
+        """Film simulation (film/mod.rs:77-129); the image must be absolute-luminance ACEScg.
+
+        Extension keywords (imops_film_ex; absent: the reference's behaviour): precision
+        ("fp64" | "mixed"), spectral_bins (16 | 81), stock_seed (stock "Synthetic"),
+        grain ("splitmix" | "philox"), grain_sigma, halation_radius, halation_strength."""
+        if not ext:
+            check(lib().imops_film(self.ctx.handle, self._h, STOCKS[stock], FILM_FORMATS[film_format], int(seed),
+                                   FILM_OUTPUTS[output]))
+            return self
+        p = _lib.FilmParams()
+        check(lib().imops_film_default_params(C.byref(p)))
+        p.stock, p.film_format, p.seed, p.output = STOCKS[stock], FILM_FORMATS[film_format], int(seed), FILM_OUTPUTS[output]
+        for k, v in ext.items():
+            if k == "precision":
+                p.precision = {"fp64": 0, "mixed": 1}[v]
+            elif k == "grain":
+                p.grain = {"splitmix": 0, "philox": 1}[v]
+            elif k in ("spectral_bins", "stock_seed"):
+                setattr(p, k, int(v))
+            elif k in ("grain_sigma", "halation_radius", "halation_strength"):
+                setattr(p, k, float(v))
+            else:
+                raise TypeError(f"film: unknown parameter {k}")
+        check(lib().imops_film_ex(self.ctx.handle, self._h, C.byref(p)))
         return self
 
     def camera_cst(self, target, calibration_matrix: Sequence[float]) -> "Image":

@@ -50,6 +50,13 @@ class FilmFitParams(C.Structure):
                 ("gamma", C.c_float * 8), ("hal_strength", C.c_float), ("hal_radius", C.c_float)]
 
 
+class FilmParams(C.Structure):
+    """imops_film_params (include/imops_gpu.h)."""
+    _fields_ = [("stock", C.c_int), ("film_format", C.c_int), ("output", C.c_int), ("precision", C.c_int),
+                ("spectral_bins", C.c_int), ("grain", C.c_int), ("seed", C.c_uint64), ("stock_seed", C.c_uint64),
+                ("grain_sigma", C.c_float), ("halation_radius", C.c_float), ("halation_strength", C.c_float)]
+
+
 # name -> (restype, argtypes); every symbol the header declares.
 SIGNATURES = {
     "imops_last_error": (C.c_char_p, []),
@@ -93,6 +100,8 @@ SIGNATURES = {
     "imops_baseline_exposure": (_i, [_vp, _vp]),
     "imops_vignette": (_i, [_vp, _vp, _f]),
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
+    "imops_film_default_params": (_i, [C.POINTER(FilmParams)]),
+    "imops_film_ex": (_i, [_vp, _vp, C.POINTER(FilmParams)]),
     "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
     "imops_film_fit_state_floats": (_i, [_i, _sz, _sz, _sz, C.POINTER(_sz)]),
     "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp]),

@@ -38,10 +38,15 @@ __device__ __forceinline__ int reflect_index(long long i, long long n) {
 }
 
 // Real = double is the reference's arithmetic; Real = float is the mixed
-// form's (its tables are rounded to float once per pixel loop, hoisted).
-template <typename Real>
+// form's.  NB: bins of the spectral grid.  The walk runs bin by bin through
+// the layers and accumulates each emulsion's trapezoid as it goes (acc_e +=
+// (prev_e + cur_e) / 2 * dl in bin order: the reference's sum in its order),
+// so no per-bin arrays are kept.
+template <typename Real, int NB>
 __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const FilmDeviceTables* __restrict__ T,
                               float* __restrict__ planes) {
+  const int E = T->n_emulsions, NA = T->n_absorbing;
+  const Real dl = (Real)T->dl;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
     const Real r = (Real)rgb[p * 3], g = (Real)rgb[p * 3 + 1], b = (Real)rgb[p * 3 + 2];
     Real w[3];
@@ -49,26 +54,31 @@ __global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const Fil
       const Real v = (Real)T->minv[k][0] * r + (Real)T->minv[k][1] * g + (Real)T->minv[k][2] * b;
       w[k] = v < (Real)0 ? (Real)0 : v;
     }
-    Real phi[16];
-    for (int i = 0; i < 16; ++i) {
+    Real acc[kFilmMaxEmulsions], prev[kFilmMaxEmulsions];
+#pragma unroll
+    for (int e = 0; e < kFilmMaxEmulsions; ++e) acc[e] = prev[e] = 0;
+    for (int i = 0; i < NB; ++i) {
       const Real s = w[0] * (Real)T->basis[0][i] + w[1] * (Real)T->basis[1][i] + w[2] * (Real)T->basis[2][i];
-      phi[i] = s * (Real)T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp
-    }
-    int e = 0;
-    for (int l = 0; l < T->n_absorbing; ++l) {
-      Real absorbed[16];
-      for (int i = 0; i < 16; ++i) {
-        const Real pt = phi[i] * (Real)T->trans[l][i];
-        absorbed[i] = phi[i] - pt;
-        phi[i] = pt;
+      Real phi = s * (Real)T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp
+      int e = 0;
+      for (int l = 0; l < NA; ++l) {
+        const Real pt = phi * (Real)T->trans[l][i];
+        const Real absorbed = phi - pt;
+        phi = pt;
+        if (T->is_emulsion[l]) {
+#pragma unroll
+          for (int k = 0; k < kFilmMaxEmulsions; ++k)  // register-resident: static index
+            if (k == e) {
+              if (i > 0) acc[k] += (Real)0.5 * (prev[k] + absorbed) * dl;
+              prev[k] = absorbed;
+            }
+          ++e;
+        }
       }
-      if (T->is_emulsion[l]) {
-        Real acc = 0;
-        for (int i = 0; i < 15; ++i) acc += (Real)0.5 * (absorbed[i] + absorbed[i + 1]) * (Real)20.0;
-        planes[(size_t)e * n + p] = (float)(acc * (Real)(1.0 / 300.0));
-        ++e;
-      }
     }
+#pragma unroll
+    for (int e = 0; e < kFilmMaxEmulsions; ++e)
+      if (e < E) planes[(size_t)e * n + p] = (float)(acc[e] * (Real)T->band_inv);
   }
 }
 
@@ -144,12 +154,13 @@ __global__ void capture_reduce_kernel(const float* __restrict__ absorbed, const 
         f = (Real)E.lut_frac[0];
       } else {
         const Real lp = log10(x);
+        const int last = E.lut_n - 1;
         if (lp <= (Real)E.lut_log[0]) {
           f = (Real)E.lut_frac[0];
-        } else if (lp >= (Real)E.lut_log[63]) {
-          f = (Real)E.lut_frac[63];
+        } else if (lp >= (Real)E.lut_log[last]) {
+          f = (Real)E.lut_frac[last];
         } else {
-          int lo = 0, hi = 63;
+          int lo = 0, hi = last;
           while (hi - lo > 1) {
             const int mid = (lo + hi) / 2;
             if ((Real)E.lut_log[mid] <= lp) lo = mid;
@@ -226,6 +237,35 @@ __global__ void grain_noise_kernel(float* __restrict__ noise, int w, int h, uint
   }
 }
 
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as easy
+// as 1, 2, 3", SC 2011): ten rounds of two 32x32->64 multiplies with the
+// Weyl key schedule.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t h0 = (uint32_t)(p0 >> 32), l0 = (uint32_t)p0, h1 = (uint32_t)(p1 >> 32), l1 = (uint32_t)p1;
+    const uint32_t n0 = h1 ^ c[1] ^ k0, n2 = h0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = l1;
+    c[2] = n2;
+    c[3] = l0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Gaussian grain noise from Philox: counter (pixel index low, high, layer, 0),
+// key (seed low, high); Box-Muller on the first two words, in f64.
+__global__ void grain_philox_kernel(float* __restrict__ noise, size_t n, uint64_t seed, uint32_t layer) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    uint32_t c[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), layer, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)c[0] + 0.5) * 2.3283064365386963e-10;  // (x + 1/2) / 2^32, in (0, 1)
+    const double u2 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+    noise[i] = (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586477 * u2));
+  }
+}
+
 // Sum of squares in f64: per-block partials, then one ordered pass.
 __global__ void sumsq_partial_kernel(const float* __restrict__ x, size_t n, double* __restrict__ partial) {
   __shared__ double s[256];
@@ -253,23 +293,26 @@ __global__ void grain_norm_kernel(const double* __restrict__ partial, int nparts
 }
 
 // grain.rs:91-98
+// absolute: the density sigma is `kappa` itself (no taper, no sqrt(D (Dmax - D)))
 __global__ void grain_apply_kernel(float* __restrict__ dye, const float* __restrict__ noise, size_t n,
-                                   const float* __restrict__ norm_p, float d_max, float kappa) {
+                                   const float* __restrict__ norm_p, float d_max, float kappa, int absolute) {
   const float norm = *norm_p;
   const float eps_toe = 0.02f * d_max;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const float v0 = dye[i];
     const float dens = v0 < 0.0f ? 0.0f : (v0 > d_max ? d_max : v0);
     const float taper = fminf(dens / (dens + eps_toe), 1.0f);
-    const float sigma_d = taper * sqrtf(fmaxf(dens * (d_max - dens), 0.0f));
+    const float sigma_d = absolute ? 1.0f : taper * sqrtf(fmaxf(dens * (d_max - dens), 0.0f));
     float v = dens + kappa * sigma_d * noise[i] * norm;
     const float hi = d_max * 1.05f;
     dye[i] = v < 0.0f ? 0.0f : (v > hi ? hi : v);
   }
 }
 
-// Densitometric scan (densitometry.rs:47-73) and the positive invert (invert.rs:36-108).
-template <typename Real>
+// Densitometric scan (densitometry.rs:47-73) and the positive invert (invert.rs:36-108),
+// streamed bin by bin: the density of bin i (emulsions in order), its 10^-d, and
+// each CMF trapezoid's term in bin order (no per-bin arrays).
+template <typename Real, int NB>
 __global__ void scan_kernel(const float* __restrict__ image, const float* __restrict__ mask, size_t n,
                             const FilmDeviceTables* __restrict__ T, float* __restrict__ rgb) {
   const int E = T->n_emulsions;
@@ -277,29 +320,31 @@ __global__ void scan_kernel(const float* __restrict__ image, const float* __rest
   image += b * E * n;
   mask += b * E * n;
   rgb += b * 3 * n;
+  const Real dl = (Real)T->dl;
   for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    Real dens[16];
-    for (int i = 0; i < 16; ++i) dens[i] = 0;
-    for (int e = 0; e < E; ++e) {
-      const Real di = (Real)image[(size_t)e * n + p];
-      const Real dm = (Real)mask[(size_t)e * n + p];
-      const FilmEmulsionTables& M = T->em[e];
-      for (int l = 0; l < 16; ++l) {
-        dens[l] += di * (Real)M.eps[l];
-        if (M.has_mask) dens[l] += dm * (Real)M.mask_eps[l];
-      }
+    Real di[kFilmMaxEmulsions], dm[kFilmMaxEmulsions];
+#pragma unroll
+    for (int e = 0; e < kFilmMaxEmulsions; ++e) {
+      di[e] = e < E ? (Real)image[(size_t)e * n + p] : (Real)0;
+      dm[e] = e < E ? (Real)mask[(size_t)e * n + p] : (Real)0;
     }
-    Real t[16];
-    for (int i = 0; i < 16; ++i)
-      t[i] = exp(-dens[i] * (Real)2.302585092994045684) * (Real)T->scanner[i];  // 10^-d
-    Real xyz[3];
-    for (int c = 0; c < 3; ++c) {
-      Real acc = 0;
-      for (int i = 0; i < 15; ++i) {
-        const Real a = t[i] * (Real)T->cmf[c][i], b = t[i + 1] * (Real)T->cmf[c][i + 1];
-        acc += (Real)0.5 * (a + b) * (Real)20.0;
+    Real xyz[3] = {0, 0, 0}, prev[3] = {0, 0, 0};
+    for (int i = 0; i < NB; ++i) {
+      Real dens = 0;
+#pragma unroll
+      for (int e = 0; e < kFilmMaxEmulsions; ++e) {
+        if (e >= E) break;
+        const FilmEmulsionTables& M = T->em[e];
+        dens += di[e] * (Real)M.eps[i];
+        if (M.has_mask) dens += dm[e] * (Real)M.mask_eps[i];
       }
-      xyz[c] = acc;
+      const Real t = exp(-dens * (Real)2.302585092994045684) * (Real)T->scanner[i];  // 10^-d
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const Real a = t * (Real)T->cmf[c][i];
+        if (i > 0) xyz[c] += (Real)0.5 * (prev[c] + a) * dl;
+        prev[c] = a;
+      }
     }
     float out[3];
     for (int c = 0; c < 3; ++c) {
@@ -636,9 +681,16 @@ __global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, 
 }  // namespace
 
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
-                       bool fp32) {
-  if (fp32) hipLaunchKernelGGL(expose_kernel<float>, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, T, planes);
-  else hipLaunchKernelGGL(expose_kernel<double>, dim3(grid1d(n)), dim3(256), 0, s, rgb, n, T, planes);
+                       bool fp32, int bins) {
+  if (bins != 16 && bins != 81) return hipErrorInvalidValue;
+  const dim3 g(grid1d(n)), blk(256);
+  if (bins == 16) {
+    if (fp32) hipLaunchKernelGGL((expose_kernel<float, 16>), g, blk, 0, s, rgb, n, T, planes);
+    else hipLaunchKernelGGL((expose_kernel<double, 16>), g, blk, 0, s, rgb, n, T, planes);
+  } else {
+    if (fp32) hipLaunchKernelGGL((expose_kernel<float, 81>), g, blk, 0, s, rgb, n, T, planes);
+    else hipLaunchKernelGGL((expose_kernel<double, 81>), g, blk, 0, s, rgb, n, T, planes);
+  }
   return hipGetLastError();
 }
 
@@ -692,22 +744,34 @@ hipError_t film_dir_apply(float* image, const float* inh, size_t n, const FilmDe
 }
 
 hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64_t key0, const float* taps, int radius,
-                      double* partial, float d_max, float kappa, hipStream_t s) {
+                      double* partial, float d_max, float kappa, hipStream_t s, bool philox, uint64_t seed, int layer,
+                      bool absolute) {
   const size_t n = (size_t)w * h;
-  hipLaunchKernelGGL(grain_noise_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, w, h, key0);
+  if (philox)
+    hipLaunchKernelGGL(grain_philox_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, n, seed, (uint32_t)layer);
+  else
+    hipLaunchKernelGGL(grain_noise_kernel, dim3(grid1d(n)), dim3(256), 0, s, noise, w, h, key0);
   blur_passes(noise, tmp, noise, noise, w, h, taps, taps, radius, (int)BLUR_STORE, 0.0f, BlurPlanes{}, s);
   const int parts = kFilmReduceBlocks;
   hipLaunchKernelGGL(sumsq_partial_kernel, dim3(parts), dim3(256), 0, s, noise, n, partial);
   float* norm = reinterpret_cast<float*>(partial + parts);
   hipLaunchKernelGGL(grain_norm_kernel, dim3(1), dim3(64), 0, s, partial, parts, n, norm);
-  hipLaunchKernelGGL(grain_apply_kernel, dim3(grid1d(n)), dim3(256), 0, s, dye, noise, n, norm, d_max, kappa);
+  hipLaunchKernelGGL(grain_apply_kernel, dim3(grid1d(n)), dim3(256), 0, s, dye, noise, n, norm, d_max, kappa,
+                     absolute ? 1 : 0);
   return hipGetLastError();
 }
 
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s, int images, bool fp32) {
-  if (fp32) hipLaunchKernelGGL(scan_kernel<float>, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
-  else hipLaunchKernelGGL(scan_kernel<double>, dim3(grid1d(n), images), dim3(256), 0, s, image, mask, n, T, rgb);
+                     hipStream_t s, int images, bool fp32, int bins) {
+  if (bins != 16 && bins != 81) return hipErrorInvalidValue;
+  const dim3 g(grid1d(n), images), blk(256);
+  if (bins == 16) {
+    if (fp32) hipLaunchKernelGGL((scan_kernel<float, 16>), g, blk, 0, s, image, mask, n, T, rgb);
+    else hipLaunchKernelGGL((scan_kernel<double, 16>), g, blk, 0, s, image, mask, n, T, rgb);
+  } else {
+    if (fp32) hipLaunchKernelGGL((scan_kernel<float, 81>), g, blk, 0, s, image, mask, n, T, rgb);
+    else hipLaunchKernelGGL((scan_kernel<double, 81>), g, blk, 0, s, image, mask, n, T, rgb);
+  }
   return hipGetLastError();
 }
 

@@ -8,6 +8,10 @@ namespace imops {
 
 constexpr int kFilmMaxLayers = 12;
 constexpr int kFilmMaxEmulsions = 8;
+// Spectral grids: the reference's 16 samples 400..700 nm (step 20) and the
+// 81-sample extension 380..780 nm (step 5).
+constexpr int kFilmMaxBins = 81;
+constexpr int kFilmLutMax = 64;  // capture curve knots (the reference's LUT has 64)
 constexpr int kFilmReduceBlocks = 1024;  // f64 partial sums of the grain variance (+1 slot for the norm)
 
 enum FilmBlurMode : int { BLUR_STORE = 0, BLUR_MIX = 1, BLUR_ADJ = 2 };
@@ -21,8 +25,9 @@ struct BlurPlanes {
 };
 
 struct FilmEmulsionTables {
-  double lut_log[64], lut_frac[64];  // developable-fraction LUT (log10 fluence -> fraction)
-  double eps[16], mask_eps[16];      // dye / coloured-mask absorptivity
+  double lut_log[kFilmLutMax], lut_frac[kFilmLutMax];  // capture curve knots (log10 fluence -> developable fraction)
+  int lut_n;                                           // knots in use: 64 (crystal LUT) or a knot curve's count
+  double eps[kFilmMaxBins], mask_eps[kFilmMaxBins];    // dye / coloured-mask absorptivity
   float gain;                        // halation bounce gain r_e * bleed_e (<= 0: none)
   float eta;                         // reciprocity efficiency at the shutter time
   float d_max, inv_gamma, mask_scale;
@@ -30,19 +35,22 @@ struct FilmEmulsionTables {
 };
 
 struct FilmDeviceTables {
+  int n_bins;              // 16 or 81 samples of the spectral grid
+  double dl;               // grid step (nm): 20 or 5
+  double band_inv;         // 1 / (last - first wavelength): the band mean's divisor (1/300 on the reference grid)
   double minv[3][3];       // ACEScg -> basis weights
-  double basis[3][16];     // Gaussian basis spectra
-  double e_rel[16];        // 550 / lambda
-  double inv_e_rel[16];    // 1 / e_rel: the per-pixel division becomes a product (<= 1 ulp in f64)
+  double basis[3][kFilmMaxBins];   // Gaussian basis spectra
+  double e_rel[kFilmMaxBins];      // 550 / lambda
+  double inv_e_rel[kFilmMaxBins];  // 1 / e_rel: the per-pixel division becomes a product (<= 1 ulp in f64)
   int n_absorbing;         // absorbing layers, top to bottom
   int is_emulsion[kFilmMaxLayers];
-  double trans[kFilmMaxLayers][16];
+  double trans[kFilmMaxLayers][kFilmMaxBins];
   int n_emulsions;
   FilmEmulsionTables em[kFilmMaxEmulsions];
   float dir[kFilmMaxEmulsions][kFilmMaxEmulsions];  // [source][target]
   int dir_valid[kFilmMaxEmulsions][kFilmMaxEmulsions];
-  double scanner[16];
-  double cmf[3][16];
+  double scanner[kFilmMaxBins];
+  double cmf[3][kFilmMaxBins];
   double xyz2ap1[3][3];
   double dmin_scale;       // 1 / peak of the Dmin scan
   int invert;              // PositiveLinear
@@ -64,8 +72,9 @@ struct FitExposeConsts {
 };
 
 // fp32: the mixed form's FP32 per-pixel arithmetic instead of the reference's FP64 (expose, capture LUT, scan)
+// bins: the tables' spectral grid (16 or 81; kernels are specialised on it)
 hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
-                       bool fp32 = false);
+                       bool fp32 = false, int bins = 16);
 // Batched over `images` images: rgb image b at rgb + 3 b n, its E planes at planes + b E n.
 hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s,
                            int images = 1);
@@ -97,9 +106,15 @@ hipError_t film_blur2(const float* in, float* tmp, float* out, int w, int h, con
 hipError_t film_capture_reduce(const float* absorbed, const float* bounce, size_t n, const FilmDeviceTables* T,
                                float* image, float* mask, hipStream_t s, int images = 1, bool fp32 = false);
 hipError_t film_dir_apply(float* image, const float* inh, size_t n, const FilmDeviceTables* T, hipStream_t s);
+// Grain (grain.rs:53-100).  Noise: the reference's SplitMix64 Irwin-Hall rows
+// (philox = false, key0 = the row key of y = 0), or Philox4x32-10 normals
+// (philox = true; key = the film seed, counter = (pixel, layer)).  Amplitude:
+// the reference's kappa * taper * sqrt(D (Dmax - D)), or a constant density
+// sigma `kappa` when `absolute`.
 hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64_t key0, const float* taps, int radius,
-                      double* partial, float d_max, float kappa, hipStream_t s);
+                      double* partial, float d_max, float kappa, hipStream_t s, bool philox = false,
+                      uint64_t seed = 0, int layer = 0, bool absolute = false);
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
-                     hipStream_t s, int images = 1, bool fp32 = false);
+                     hipStream_t s, int images = 1, bool fp32 = false, int bins = 16);
 
 }  // namespace imops

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -23,45 +24,66 @@ using namespace film_data;
 
 namespace {
 
+// The reference's spectral grid (16 samples, 400..700 nm); the differentiable
+// model below works on it.
 constexpr int NB = 16;
 double lam(int i) { return 400.0 + 20.0 * i; }
 
-void sample_curve(const Curve& c, double* out) {
-  for (int i = 0; i < NB; ++i) {
-    const double l = lam(i);
+// A spectral grid: the reference's (16 bins) or the 81-bin extension.
+struct Grid {
+  int n;
+  double l0, dl;
+  double lam(int i) const { return l0 + dl * i; }
+};
+const Grid kGridRef{16, 400.0, 20.0};
+const Grid kGrid81{81, 380.0, 5.0};
+const Grid& grid_of(int bins) { return bins == 81 ? kGrid81 : kGridRef; }
+
+double gauss_at(double l, double peak, double sigma) {
+  const double d = (l - peak) / sigma;
+  return std::exp(-0.5 * d * d);
+}
+
+void sample_curve(const Curve& c, const Grid& g, double* out) {
+  for (int i = 0; i < g.n; ++i) {
+    const double l = g.lam(i);
     if (c.kind == CURVE_CONST) {
       out[i] = c.a;
     } else if (c.kind == CURVE_GAUSS) {  // gaussian_curve (color_neg_200.rs:38-50)
-      const double d = (l - c.b) / c.c;
-      out[i] = c.a * std::exp(-0.5 * d * d);
+      out[i] = c.a * gauss_at(l, c.b, c.c);
     } else if (c.kind == CURVE_ORANGE) {  // orange_mask_epsilon (color_neg_200.rs:51-65, portra_400.rs:28-40)
       const double u = (l - 450.0) / 50.0, v = (l - 520.0) / 60.0;
       out[i] = c.a * std::exp(-0.5 * (u * u)) + c.b * (c.c * std::exp(-0.5 * (v * v))) + c.d;
     } else if (c.kind == CURVE_PANCHRO) {  // panchromatic_sensitivity (bw_stub.rs:19-32, trix_400.rs:16-28)
       const double t = (l - 400.0) / 300.0;
       out[i] = c.a + c.b * std::fmax(std::sin(3.14159265358979323846 * t), 0.0);
+    } else if (c.kind == CURVE_GAUSS2) {  // two-component Gaussian mixture (synthetic stock)
+      out[i] = c.a * gauss_at(l, c.b, c.c) + c.d * gauss_at(l, c.e, c.f);
     } else {
       out[i] = 0.0;
     }
   }
 }
+void sample_curve(const Curve& c, double* out) { sample_curve(c, kGridRef, out); }
 
-double trapz(const double* s) {  // SpectralCurve::integrate (spectrum.rs:106-118)
+double trapz(const double* s, const Grid& g) {  // SpectralCurve::integrate (spectrum.rs:106-118)
   double acc = 0.0;
-  for (int i = 0; i < NB - 1; ++i) acc += 0.5 * (s[i] + s[i + 1]) * (lam(i + 1) - lam(i));
+  for (int i = 0; i < g.n - 1; ++i) acc += 0.5 * (s[i] + s[i + 1]) * (g.lam(i + 1) - g.lam(i));
   return acc;
 }
+double trapz(const double* s) { return trapz(s, kGridRef); }
 
-double eval_curve(const double* s, double l) {  // SpectralCurve::evaluate (spectrum.rs:60-78)
-  if (l <= lam(0)) return s[0];
-  if (l >= lam(NB - 1)) return s[NB - 1];
-  for (int i = 0; i < NB - 1; ++i)
-    if (l >= lam(i) && l <= lam(i + 1)) {
-      const double t = (l - lam(i)) / (lam(i + 1) - lam(i));
+double eval_curve(const double* s, const Grid& g, double l) {  // SpectralCurve::evaluate (spectrum.rs:60-78)
+  if (l <= g.lam(0)) return s[0];
+  if (l >= g.lam(g.n - 1)) return s[g.n - 1];
+  for (int i = 0; i < g.n - 1; ++i)
+    if (l >= g.lam(i) && l <= g.lam(i + 1)) {
+      const double t = (l - g.lam(i)) / (g.lam(i + 1) - g.lam(i));
       return s[i] * (1.0 - t) + s[i + 1] * t;
     }
-  return s[NB - 1];
+  return s[g.n - 1];
 }
+double eval_curve(const double* s, double l) { return eval_curve(s, kGridRef, l); }
 
 // CIE 1931 2-degree CMFs on the 20 nm grid and XYZ(D65) -> AP1 (upsample.rs:17-45).
 const double kCmf[3][16] = {
@@ -75,13 +97,37 @@ const double kXyz2Ap1[3][3] = {{1.6410233797, -0.3248032942, -0.2364246952},
                                {-0.6636628587, 1.6153315917, 0.0167563477},
                                {0.0117218943, -0.0082844165, 0.9883948585}};
 
-void spectrum_ap1(const double* s, double out[3]) {
+// CMFs on a grid: the reference's table on its own grid; on the 81-bin grid the
+// multi-lobe piecewise-Gaussian fit of the CIE 1931 2-degree observer (Wyman,
+// Sloan, Shirley, "Simple analytic approximations to the CIE XYZ colour matching
+// functions", JCGT 2(2), 2013), each lobe with separate widths below and above
+// its centre.
+double lobe(double l, double mu, double s_lo, double s_hi) {
+  const double t = (l - mu) / (l < mu ? s_lo : s_hi);
+  return std::exp(-0.5 * t * t);
+}
+void cmf_on(const Grid& g, double out[3][kFilmMaxBins]) {
+  for (int i = 0; i < g.n; ++i) {
+    if (g.n == 16) {
+      for (int c = 0; c < 3; ++c) out[c][i] = kCmf[c][i];
+      continue;
+    }
+    const double l = g.lam(i);
+    out[0][i] = 1.056 * lobe(l, 599.8, 37.9, 31.0) + 0.362 * lobe(l, 442.0, 16.0, 26.7) -
+                0.065 * lobe(l, 501.1, 20.4, 26.2);
+    out[1][i] = 0.821 * lobe(l, 568.8, 46.9, 40.5) + 0.286 * lobe(l, 530.9, 16.3, 31.1);
+    out[2][i] = 1.217 * lobe(l, 437.0, 11.8, 36.0) + 0.681 * lobe(l, 459.0, 26.0, 13.8);
+  }
+}
+
+// spectrum_to_acescg_rgb (upsample.rs:85-95, 179-184): CMF trapezoid -> XYZ -> AP1.
+void spectrum_ap1(const double* s, const Grid& g, const double cmf[3][kFilmMaxBins], double out[3]) {
   double xyz[3];
   for (int c = 0; c < 3; ++c) {
     double acc = 0.0;
-    for (int i = 0; i < 15; ++i) {
-      const double a = s[i] * kCmf[c][i], b = s[i + 1] * kCmf[c][i + 1];
-      acc += 0.5 * (a + b) * 20.0;
+    for (int i = 0; i < g.n - 1; ++i) {
+      const double a = s[i] * cmf[c][i], b = s[i + 1] * cmf[c][i + 1];
+      acc += 0.5 * (a + b) * g.dl;
     }
     xyz[c] = acc;
   }
@@ -111,33 +157,113 @@ void build_lut(double mu_ln, double sigma_ln, double k, double* logs, double* fr
   }
 }
 
+// A characteristic curve given by knots (log10 fluence, developable fraction),
+// sampled like the reference's LUT: linear in log10 fluence, clamped at the ends.
+struct KnotCurve {
+  int n = 0;
+  double log10_phi[kFilmLutMax], frac[kFilmLutMax];
+};
+
+// The seeded synthetic colour negative: the ColorNeg200 layer stack
+// (color_neg_200.rs:67-209) with each emulsion's sensitivity a two-Gaussian
+// mixture, its capture a 5-knot monotone characteristic curve, and no
+// reciprocity failure.  Draws, in order, from SplitMix64 (grain.rs:20-45
+// generator, state = seed), u = (next >> 11) * 2^-53, for emulsions blue, green,
+// red (base 450, 550, 650 nm):
+//   mu1 = base + 30 (u - 1/2), s1 = 25 + 20 u, mu2 = mu1 + 100 (u - 1/2),
+//   s2 = 20 + 30 u, a2 = 0.15 + 0.35 u;
+//   knot j = 0..4: log10 fluence -2.5 + j + 0.3 (u - 1/2);
+//   knot j = 1..4: increment 0.2 + u; fraction_0 = 0.005 (the toe: unexposed film is
+//   nearly the base), fraction_j = 0.005 + 0.965 (increments 1..j) / (increments 1..4).
+struct SplitMix {
+  uint64_t state;
+  uint64_t next() {
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+struct SyntheticStock {
+  Stock stock;
+  KnotCurve knots[3];
+};
+
+SyntheticStock make_synthetic_stock(uint64_t seed) {
+  SyntheticStock out{color_neg_200(), {}};
+  Stock& st = out.stock;
+  st.name = "Synthetic";
+  SplitMix rng{seed};
+  const double base[3] = {450.0, 550.0, 650.0};
+  int e = 0;
+  for (int l = 0; l < st.n_layers; ++l) {
+    Layer& L = st.layers[l];
+    if (L.kind != L_EMULSION) continue;
+    const double mu1 = base[e] + 30.0 * (rng.uniform() - 0.5);
+    const double s1 = 25.0 + 20.0 * rng.uniform();
+    const double mu2 = mu1 + 100.0 * (rng.uniform() - 0.5);
+    const double s2 = 20.0 + 30.0 * rng.uniform();
+    const double a2 = 0.15 + 0.35 * rng.uniform();
+    L.sensitivity = Curve{CURVE_GAUSS2, 1.0, mu1, s1, a2, mu2, s2};
+    L.reciprocity_p = 1.0f;
+    KnotCurve& K = out.knots[e];
+    K.n = 5;
+    for (int j = 0; j < 5; ++j) K.log10_phi[j] = -2.5 + (double)j + 0.3 * (rng.uniform() - 0.5);
+    double inc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, total = 0.0;
+    for (int j = 1; j < 5; ++j) {
+      inc[j] = 0.2 + rng.uniform();
+      total += inc[j];
+    }
+    double run = 0.0;
+    for (int j = 0; j < 5; ++j) {
+      run += inc[j];
+      K.frac[j] = 0.005 + 0.965 * (run / total);
+    }
+    ++e;
+  }
+  return out;
+}
+
 struct HostStock {
   const Stock* s = nullptr;
+  Stock own{};                 // the record itself, for a synthetic stock
+  Grid grid = kGridRef;
   FilmDeviceTables t{};        // per-stock part (no per-call fields yet)
   float kappa_ref[kFilmMaxEmulsions] = {};
   float recip_p[kFilmMaxEmulsions] = {};
+  float bleed[kFilmMaxEmulsions] = {};       // bleed_e of the sensitivity peak (halation.rs:102-132)
   float bleed_gain[kFilmMaxEmulsions] = {};  // r_e * bleed_e
   bool wide_reflect = false;                 // any backing reflectance > 0 at the four bands
   bool any_reversal = false;
   double dmin_raw[3] = {0, 0, 0};
   double alpha[kFilmMaxLayers] = {};  // emulsion layers: optical density per unit sensitivity (2 rho t)
   int layer_of[kFilmMaxLayers] = {};   // stock layer of each absorbing layer
-  double reflect[NB] = {};            // antihalation backing reflectance samples
+  double reflect[kFilmMaxBins] = {};  // antihalation backing reflectance samples
 };
 
-HostStock build_stock(const Stock* st) {
+// knots: per emulsion, a characteristic curve replacing the crystal LUT (or null)
+HostStock build_stock(const Stock* st, const Grid& g, const KnotCurve* knots = nullptr) {
   HostStock h;
   h.s = st;
+  h.grid = g;
   FilmDeviceTables& T = h.t;
+  T.n_bins = g.n;
+  T.dl = g.dl;
+  T.band_inv = 1.0 / (g.lam(g.n - 1) - g.lam(0));
+  double cmf[3][kFilmMaxBins];
+  cmf_on(g, cmf);
   // upsample basis (upsample.rs:104-143)
   const double peaks[3] = {450.0, 550.0, 650.0};
   double cols[3][3];
   for (int k = 0; k < 3; ++k) {
-    for (int i = 0; i < NB; ++i) {
-      const double d = (lam(i) - peaks[k]) / 40.0;
+    for (int i = 0; i < g.n; ++i) {
+      const double d = (g.lam(i) - peaks[k]) / 40.0;
       T.basis[k][i] = std::exp(-0.5 * d * d);
     }
-    spectrum_ap1(T.basis[k], cols[k]);
+    spectrum_ap1(T.basis[k], g, cmf, cols[k]);
   }
   double m[3][3];
   for (int r = 0; r < 3; ++r)
@@ -154,20 +280,20 @@ HostStock build_stock(const Stock* st) {
   T.minv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
   T.minv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
   T.minv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
-  for (int i = 0; i < NB; ++i) {
-    T.e_rel[i] = 550.0 / lam(i);
+  for (int i = 0; i < g.n; ++i) {
+    T.e_rel[i] = 550.0 / g.lam(i);
     T.inv_e_rel[i] = 1.0 / T.e_rel[i];
   }
   for (int c = 0; c < 3; ++c)
-    for (int i = 0; i < NB; ++i) T.cmf[c][i] = kCmf[c][i];
+    for (int i = 0; i < g.n; ++i) T.cmf[c][i] = cmf[c][i];
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) T.xyz2ap1[r][c] = kXyz2Ap1[r][c];
-  sample_curve(st->scanner_light, T.scanner);
+  sample_curve(st->scanner_light, g, T.scanner);
   double* reflect = h.reflect;
-  sample_curve(st->ah_reflectance, reflect);
+  sample_curve(st->ah_reflectance, g, reflect);
   const double bands[4] = {450.0, 550.0, 650.0, 700.0};
   float max_r = 0.0f;
-  for (double b : bands) max_r = std::max(max_r, (float)eval_curve(reflect, b));
+  for (double b : bands) max_r = std::max(max_r, (float)eval_curve(reflect, g, b));
   h.wide_reflect = max_r > 0.0f;
 
   int e = 0;
@@ -175,12 +301,12 @@ HostStock build_stock(const Stock* st) {
     const Layer& L = st->layers[l];
     if (L.is_reversal) h.any_reversal = true;
     if (L.kind == L_SUPPORT || L.sensitivity.kind == CURVE_NONE) continue;  // no absorption (absorption.rs:68-70)
-    double curve[NB];
-    sample_curve(L.sensitivity, curve);
+    double curve[kFilmMaxBins];
+    sample_curve(L.sensitivity, g, curve);
     const int a = T.n_absorbing++;
     h.layer_of[a] = l;
     const double thickness = (double)L.thickness_um;
-    for (int i = 0; i < NB; ++i) {
+    for (int i = 0; i < g.n; ++i) {
       const double od = L.kind == L_EMULSION ? (curve[i] * 2.0) * (double)L.silver_halide_fraction * thickness
                                              : curve[i] * thickness;
       T.trans[a][i] = std::exp(-od);
@@ -190,10 +316,19 @@ HostStock build_stock(const Stock* st) {
     h.alpha[a] = 2.0 * (double)L.silver_halide_fraction * thickness;
     FilmEmulsionTables& E = T.em[e];
     const double mu = std::log(L.crystal_mean_um), sg = L.crystal_sigma_ln;
-    build_lut(mu, sg, L.capture_k, E.lut_log, E.lut_frac);
-    sample_curve(L.dye_epsilon, E.eps);
+    if (knots && knots[e].n > 0) {
+      E.lut_n = knots[e].n;
+      for (int j = 0; j < E.lut_n; ++j) {
+        E.lut_log[j] = knots[e].log10_phi[j];
+        E.lut_frac[j] = knots[e].frac[j];
+      }
+    } else {
+      E.lut_n = 64;
+      build_lut(mu, sg, L.capture_k, E.lut_log, E.lut_frac);
+    }
+    sample_curve(L.dye_epsilon, g, E.eps);
     E.has_mask = L.mask_epsilon.kind != CURVE_NONE;
-    sample_curve(L.mask_epsilon, E.mask_eps);
+    sample_curve(L.mask_epsilon, g, E.mask_eps);
     E.reversal = L.is_reversal;
     E.d_max = L.d_max;
     E.inv_gamma = 1.0f / std::max(L.gamma_contrast, 1e-6f);
@@ -207,23 +342,24 @@ HostStock build_stock(const Stock* st) {
     h.kappa_ref[e] = (float)(1.0 / std::sqrt(rho));
     // halation bleed (halation.rs:102-132) and effective reflectance (155-161)
     double peak = 550.0, mx = -INFINITY;
-    for (int i = 0; i < NB; ++i)
-      if (curve[i] > mx) { mx = curve[i]; peak = lam(i); }
+    for (int i = 0; i < g.n; ++i)
+      if (curve[i] > mx) { mx = curve[i]; peak = g.lam(i); }
     const float wb = 0.20f, wg = 0.45f, wr = 1.0f;
     float bleed;
     if (peak <= 450.0) bleed = wb;
     else if (peak <= 550.0) bleed = wb + (wg - wb) * (float)((peak - 450.0) / 100.0);
     else if (peak <= 650.0) bleed = wg + (wr - wg) * (float)((peak - 550.0) / 100.0);
     else bleed = wr;
-    const double norm = trapz(curve);
+    const double norm = trapz(curve, g);
     float r_e;
     if (norm <= 1e-9) {
-      r_e = (float)eval_curve(reflect, 650.0);
+      r_e = (float)eval_curve(reflect, g, 650.0);
     } else {
-      double prod[NB];
-      for (int i = 0; i < NB; ++i) prod[i] = reflect[i] * curve[i];
-      r_e = (float)(trapz(prod) / norm);
+      double prod[kFilmMaxBins];
+      for (int i = 0; i < g.n; ++i) prod[i] = reflect[i] * curve[i];
+      r_e = (float)(trapz(prod, g) / norm);
     }
+    h.bleed[e] = bleed;
     h.bleed_gain[e] = r_e * bleed;
     ++e;
   }
@@ -235,29 +371,44 @@ HostStock build_stock(const Stock* st) {
       T.dir[i][j] = v ? st->dir[i][j] : 0.0f;
     }
   // Dmin reference scan: image dye 0, mask at d_max * 0.4 (densitometry.rs:76-105)
-  double dens[NB] = {0};
+  double dens[kFilmMaxBins] = {0};
   for (int k = 0; k < e; ++k) {
     const double dm = T.em[k].has_mask ? (double)(T.em[k].d_max * 0.4f) : 0.0;
-    for (int l = 0; l < NB; ++l) {
+    for (int l = 0; l < g.n; ++l) {
       dens[l] += 0.0 * T.em[k].eps[l];
       if (T.em[k].has_mask) dens[l] += dm * T.em[k].mask_eps[l];
     }
   }
-  double tt[NB];
-  for (int i = 0; i < NB; ++i) tt[i] = std::pow(10.0, -dens[i]) * T.scanner[i];
-  spectrum_ap1(tt, h.dmin_raw);
+  double tt[kFilmMaxBins];
+  for (int i = 0; i < g.n; ++i) tt[i] = std::pow(10.0, -dens[i]) * T.scanner[i];
+  spectrum_ap1(tt, g, cmf, h.dmin_raw);
   const double pk = std::max(std::max(std::max(h.dmin_raw[0], h.dmin_raw[1]), h.dmin_raw[2]), 1e-12);
   T.dmin_scale = 1.0 / pk;
   return h;
 }
 
-const HostStock& host_stock(int idx) {
-  static std::unique_ptr<HostStock> cache[STOCK_COUNT];
-  static std::once_flag once[STOCK_COUNT];
-  const Stock* st = stock_by_index(idx);
-  if (!st) throw Error{IMOPS_E_UNSUPPORTED, "film stock tables not available on the GPU path"};
-  std::call_once(once[idx], [&] { cache[idx] = std::make_unique<HostStock>(build_stock(st)); });
-  return *cache[idx];
+// Host tables of a stock on a grid (built once per (stock, bins, synthetic seed)).
+const HostStock& host_stock(int idx, int bins = 16, uint64_t stock_seed = 0) {
+  static std::mutex mu;
+  static std::map<std::string, std::unique_ptr<HostStock>> cache;
+  if (bins != 16 && bins != 81) throw Error{IMOPS_E_INVALID, "film: spectral_bins must be 16 or 81"};
+  const std::string key = std::to_string(idx) + "/" + std::to_string(bins) + "/" +
+                          (idx == SYNTHETIC ? std::to_string(stock_seed) : std::string("-"));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return *it->second;
+  std::unique_ptr<HostStock> h;
+  if (idx == SYNTHETIC) {
+    const SyntheticStock syn = make_synthetic_stock(stock_seed);
+    h = std::make_unique<HostStock>(build_stock(&syn.stock, grid_of(bins), syn.knots));
+    h->own = syn.stock;
+    h->s = &h->own;
+  } else {
+    const Stock* st = stock_by_index(idx);
+    if (!st) throw Error{IMOPS_E_UNSUPPORTED, "film stock tables not available on the GPU path"};
+    h = std::make_unique<HostStock>(build_stock(st, grid_of(bins)));
+  }
+  return *(cache[key] = std::move(h));
 }
 
 // Reflect-101 Gaussian taps of film/blur.rs:39-55.
@@ -313,19 +464,27 @@ struct FilmPrepared {
 // Taps and per-call table fields at this pitch and shutter (radiance.rs:43-49,
 // halation.rs:24-93, diffusion.rs, grain.rs:53-100); uploads both and waits
 // (the host sources are locals).
+// Halation extension: hal_radius >= 0 sets the wide blur's radius in pixels
+// (sigma = radius / 3, so the kernel's ceil(3 sigma) radius is that radius) and
+// hal_strength >= 0 its gain strength * bleed_e, applied whatever the backing's
+// reflectance; negative values keep the reference's psf_halation / pitch and
+// r_e * bleed_e.
 void prepare_film(hipStream_t s, const HostStock& hs, const FilmDeviceTables& base, float pitch, float shutter,
-                  FilmPrepared& P) {
+                  const FilmStageParams& fp, FilmPrepared& P) {
   const Stock& st = *hs.s;
   P.T = base;
   const int E = P.T.n_emulsions;
   const float sigma_local = std::max(st.psf_local_um / std::max(pitch, 1e-6f), 0.0f);
-  const float sigma_wide = std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
-  P.do_wide = sigma_wide >= 1e-3f && hs.wide_reflect && E > 0;
+  const float sigma_wide = fp.hal_radius >= 0.0f ? fp.hal_radius / 3.0f
+                                                 : std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
+  const bool reflects = fp.hal_strength >= 0.0f ? fp.hal_strength > 0.0f : hs.wide_reflect;
+  P.do_wide = sigma_wide >= 1e-3f && reflects && E > 0;
   for (int e = 0; e < E; ++e) {
     const double p = (double)hs.recip_p[e];
     P.T.em[e].eta = ((double)shutter <= 1.0 || p >= 0.9999) ? 1.0f
                                                             : (float)std::pow(std::max((double)shutter, 1.0), p - 1.0);
-    P.T.em[e].gain = P.do_wide ? hs.bleed_gain[e] : 0.0f;
+    const float gain = fp.hal_strength >= 0.0f ? fp.hal_strength * hs.bleed[e] : hs.bleed_gain[e];
+    P.T.em[e].gain = P.do_wide ? gain : 0.0f;
   }
   // blur tap sets: local scatter, halation, DIR, adjacency, grain
   P.sigmas[0] = sigma_local;
@@ -354,7 +513,7 @@ void prepare_film(hipStream_t s, const HostStock& hs, const FilmDeviceTables& ba
 // prepared tables carry it.  Kernels only: nothing is uploaded or awaited.
 // fp32: the mixed form's FP32 per-pixel spectral arithmetic (expose, capture LUT, scan)
 void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const float* rgb_in, float* rgb_out, int w,
-              int h, float pitch, uint64_t seed, FilmBuffers& B, bool fp32 = false) {
+              int h, float pitch, uint64_t seed, FilmBuffers& B, const FilmStageParams& fp, bool fp32 = false) {
   const size_t n = (size_t)w * h;
   const Stock& st = *hs.s;
   const int E = P.T.n_emulsions;
@@ -362,7 +521,8 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
   const FilmDeviceTables* T = P.tables.as<FilmDeviceTables>();
   const float sigma_local = P.sigmas[0], sig_dir = P.sigmas[2], sig_adj = P.sigmas[3];
 
-  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s, fp32));
+  const int bins = P.T.n_bins;
+  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s, fp32, bins));
   if (sigma_local >= 1e-3f)  // local gelatin scatter, mix 0.18 (halation.rs:38-44)
     for (int e = 0; e < E; ++e) {
       float* p = B.absorbed + (size_t)e * n;
@@ -389,15 +549,17 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
       IMOPS_HIP_CHECK(film_blur(p, B.tmp, p, p, w, h, taps + P.off[3], P.radius[3], BLUR_ADJ, st.adjacency_beta, s));
     }
   // grain (grain.rs:53-100)
+  // (grain_sigma >= 0: that constant density sigma; philox: Philox4x32-10 normals)
+  const bool absolute = fp.grain_sigma >= 0.0f;
   for (int e = 0; e < E; ++e) {
     const float d_max = P.T.em[e].d_max;
-    const float kappa = (hs.kappa_ref[e] / std::max(pitch, 1e-6f)) * 0.15f;
+    const float kappa = absolute ? fp.grain_sigma : (hs.kappa_ref[e] / std::max(pitch, 1e-6f)) * 0.15f;
     if (kappa <= 0.0f || d_max <= 0.0f) continue;
     const uint64_t key0 = seed * 0xD1B54A32D192ED03ull + (uint64_t)e * 0x9E3779B97F4A7C15ull;
     IMOPS_HIP_CHECK(film_grain(B.image + (size_t)e * n, B.bounce, B.tmp, w, h, key0, taps + P.off[4], P.radius[4],
-                               B.partial, d_max, kappa, s));
+                               B.partial, d_max, kappa, s, fp.philox, seed, e, absolute));
   }
-  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, T, rgb_out, s, 1, fp32));
+  IMOPS_HIP_CHECK(film_scan(B.image, B.mask, n, T, rgb_out, s, 1, fp32, bins));
 }
 
 // Prepared runs of a context, by (stock, format, width, shutter, output).
@@ -410,10 +572,10 @@ struct FilmCache : imops::StageCache {
 FilmStageParams film_stage_params_from_table(const std::string& stock, const std::string& format, uint64_t seed,
                                              const std::string& output) {
   static const char* stocks[] = {"BwStub", "ColorNeg200", "Portra400", "Ektar100", "FujiPro400H", "EktachromeE100",
-                                 "TriX400"};
+                                 "TriX400", "Synthetic"};
   FilmStageParams p;
   p.stock = -1;
-  for (int i = 0; i < 7; ++i)
+  for (int i = 0; i < 8; ++i)
     if (stock == stocks[i]) p.stock = i;
   if (p.stock < 0) throw Error{IMOPS_E_INVALID, "Unknown film stock: " + stock};
   if (format == "Film35mm") p.format = 0;
@@ -434,8 +596,8 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
     throw Error{IMOPS_E_STATE, "film: wrong color space: expected ACEScg"};  // film/mod.rs:78-87
   const int w = (int)img->width, h = (int)img->height;
   if (w <= 0 || h <= 0) throw Error{IMOPS_E_INVALID, "film: invalid dimensions"};
-  if (p.stock < 0 || p.stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film: unknown stock"};
-  const HostStock& hs = host_stock(p.stock);
+  if (p.stock < 0 || p.stock > SYNTHETIC) throw Error{IMOPS_E_INVALID, "film: unknown stock"};
+  const HostStock& hs = host_stock(p.stock, p.bins, p.stock_seed);
   if (hs.t.n_emulsions > kFilmMaxEmulsions) throw Error{IMOPS_E_UNSUPPORTED, "film: too many emulsion layers"};
   hipStream_t s = ctx->stream;
   const size_t n = (size_t)w * h;
@@ -462,8 +624,9 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   layout(ctx->scr[SCR_FILM_0].as<float>(), n, B);
 
   // prepared constants of this shape (computed once per context)
-  char keybuf[160];
-  snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%08x/%d", p.stock, p.format, w, bits_of(shutter), p.output);
+  char keybuf[256];
+  snprintf(keybuf, sizeof keybuf, "%d/%d/%d/%08x/%d/%d/%llu/%08x/%08x", p.stock, p.format, w, bits_of(shutter),
+           p.output, p.bins, (unsigned long long)p.stock_seed, bits_of(p.hal_radius), bits_of(p.hal_strength));
   const std::string key(keybuf);
   if (!ctx->film_cache) ctx->film_cache.reset(new FilmCache());
   FilmCache& cache = static_cast<FilmCache&>(*ctx->film_cache);
@@ -477,7 +640,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
     P = cache.runs.back().second.get();
     FilmDeviceTables T = hs.t;
     T.invert = 0;
-    prepare_film(s, hs, T, pitch, shutter, *P);
+    prepare_film(s, hs, T, pitch, shutter, p, *P);
     if (p.output == 1 && !hs.any_reversal) {
       // PositiveLinear: Dmin of the film base and the mean of a 32x32 flat
       // mid-gray run with seed 0 at this image's pitch and shutter (film/mod.rs:110-121, 136-151).
@@ -490,7 +653,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
       const float gv = (float)(12.5 * (double)0.185f * (8.0 * 8.0) / ((double)(1.0f / iso) * (double)iso));
       std::vector<float> hgray(mid_n * 3, gv), hout(mid_n * 3);
       IMOPS_HIP_CHECK(hipMemcpyAsync(gray, hgray.data(), hgray.size() * sizeof(float), hipMemcpyHostToDevice, s));
-      run_film(s, hs, *P, gray, gout, 32, 32, pitch, 0, M);
+      run_film(s, hs, *P, gray, gout, 32, 32, pitch, 0, M, p);
       IMOPS_HIP_CHECK(hipMemcpyAsync(hout.data(), gout, hout.size() * sizeof(float), hipMemcpyDeviceToHost, s));
       IMOPS_HIP_CHECK(hipStreamSynchronize(s));
       double acc[3] = {0, 0, 0};
@@ -523,7 +686,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   }
   // the scan writes straight into the image: every pixel's input was consumed by expose
   // the 32x32 mid-gray calibration above stays FP64 in both forms
-  run_film(s, hs, *P, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, p.seed, B, p.fp32);
+  run_film(s, hs, *P, img->rgb.as<float>(), img->rgb.as<float>(), w, h, pitch, p.seed, B, p, p.fp32);
   img->color_space = IMOPS_CS_ACESCG;
 }
 
@@ -832,6 +995,48 @@ extern "C" int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_
     p.output = output;
     if (stock < 0 || stock >= STOCK_COUNT || film_format < 0 || film_format > 2 || output < 0 || output > 1)
       throw Error{IMOPS_E_INVALID, "film: stock / format / output out of range"};
+    stage_film(ctx, img, p);
+  });
+}
+
+extern "C" int imops_film_default_params(imops_film_params* out) {
+  return guarded([&] {
+    if (!out) throw Error{IMOPS_E_INVALID, "film_default_params: null argument"};
+    const FilmStageParams d;
+    *out = imops_film_params{};
+    out->stock = d.stock;
+    out->film_format = d.format;
+    out->output = d.output;
+    out->precision = 0;
+    out->spectral_bins = d.bins;
+    out->grain = 0;
+    out->seed = d.seed;
+    out->stock_seed = d.stock_seed;
+    out->grain_sigma = d.grain_sigma;
+    out->halation_radius = d.hal_radius;
+    out->halation_strength = d.hal_strength;
+  });
+}
+
+extern "C" int imops_film_ex(imops_ctx* ctx, imops_image* img, const imops_film_params* q) {
+  return guarded([&] {
+    if (!ctx || !img || !q) throw Error{IMOPS_E_INVALID, "film_ex: null argument"};
+    if (q->stock < 0 || q->stock > SYNTHETIC || q->film_format < 0 || q->film_format > 2 || q->output < 0 ||
+        q->output > 1 || q->precision < 0 || q->precision > 1 || (q->spectral_bins != 16 && q->spectral_bins != 81) ||
+        q->grain < 0 || q->grain > 1)
+      throw Error{IMOPS_E_INVALID, "film_ex: parameter out of range"};
+    FilmStageParams p;
+    p.stock = q->stock;
+    p.format = q->film_format;
+    p.seed = q->seed;
+    p.output = q->output;
+    p.fp32 = q->precision == 1;
+    p.bins = q->spectral_bins;
+    p.stock_seed = q->stock_seed;
+    p.philox = q->grain == 1;
+    p.grain_sigma = q->grain_sigma;
+    p.hal_radius = q->halation_radius;
+    p.hal_strength = q->halation_strength;
     stage_film(ctx, img, p);
   });
 }

@@ -16,11 +16,12 @@ enum CurveKind : int {
   CURVE_GAUSS = 2,     // a * exp(-0.5 ((l - b) / c)^2): a = amplitude, b = peak, c = sigma
   CURVE_ORANGE = 3,    // a * exp(-0.5 ((l-450)/50)^2) + b * (c * exp(-0.5 ((l-520)/60)^2)) + d  (coloured mask)
   CURVE_PANCHRO = 4,   // a + b * max(sin(pi (l - 400) / 300), 0)  (panchromatic B&W sensitivity)
+  CURVE_GAUSS2 = 5,    // a exp(-0.5 ((l - b) / c)^2) + d exp(-0.5 ((l - e) / f)^2)  (synthetic stock sensitivities)
 };
 
 struct Curve {
   int kind;
-  double a, b, c, d;
+  double a, b, c, d, e, f;
 };
 
 enum LayerKindId : int { L_EMULSION = 0, L_FILTER = 1, L_OVERCOAT = 2, L_ANTIHALATION = 3, L_SUPPORT = 4 };
@@ -79,7 +80,8 @@ constexpr Layer passive(int kind, float depth, float thick, Curve absorption) {
 
 // Stock order of StockId (film/stock/mod.rs:124-133).
 enum StockIdx : int { BW_STUB = 0, COLOR_NEG_200 = 1, PORTRA_400 = 2, EKTAR_100 = 3, FUJI_PRO_400H = 4,
-                      EKTACHROME_E100 = 5, TRIX_400 = 6, STOCK_COUNT = 7 };
+                      EKTACHROME_E100 = 5, TRIX_400 = 6, STOCK_COUNT = 7,
+                      SYNTHETIC = 7 };  // seeded extension stock (not one of the reference's StockIds)
 
 // ColorNeg200: overcoat, blue emulsion, yellow filter, green emulsion, red emulsion, antihalation.
 inline const Stock& color_neg_200() {

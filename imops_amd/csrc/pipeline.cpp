@@ -415,6 +415,23 @@ Stage make_stage(const Table& t) {
     const std::string prec = str_param(t, "precision", "fp64", "Film");
     if (prec == "mixed") s.film.fp32 = true;
     else if (prec != "fp64") throw Error{IMOPS_E_INVALID, "Invalid Film config"};
+    // extensions (absent: the reference's behaviour)
+    s.film.stock_seed = u64_param(t, "stock_seed", 0, "Film");
+    const uint64_t bins = u64_param(t, "spectral_bins", 16, "Film");
+    if (bins != 16 && bins != 81) throw Error{IMOPS_E_INVALID, "Invalid Film config: spectral_bins must be 16 or 81"};
+    s.film.bins = (int)bins;
+    const std::string grain = str_param(t, "grain", "splitmix", "Film");
+    if (grain == "philox") s.film.philox = true;
+    else if (grain != "splitmix") throw Error{IMOPS_E_INVALID, "Invalid Film config"};
+    auto nonneg = [&](const char* key) -> float {
+      if (!param(t, key)) return -1.0f;
+      const double v = num_param(t, key, 0.0, "Film");
+      if (!(v >= 0.0 && v < 1e6)) throw Error{IMOPS_E_INVALID, std::string("Invalid Film config: ") + key};
+      return (float)v;
+    };
+    s.film.grain_sigma = nonneg("grain_sigma");
+    s.film.hal_radius = nonneg("halation_radius");
+    s.film.hal_strength = nonneg("halation_strength");
   } else {
     s.kind = K_OTHER;
   }

@@ -35,11 +35,18 @@ void stage_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity);
 void stage_luma_chroma_denoise(imops_ctx* ctx, imops_image* img, int radius, float epsilon);
 
 struct FilmStageParams {
-  int stock = 1;    // StockId order of film/stock/mod.rs (0 BwStub, 1 ColorNeg200, ...)
+  int stock = 1;    // StockId order of film/stock/mod.rs (0 BwStub, 1 ColorNeg200, ...; 7 the synthetic stock)
   int format = 0;   // 0 Film35mm, 1 Film6x6, 2 Film4x5
   uint64_t seed = 1;
   int output = 0;   // 0 NegativeLinear, 1 PositiveLinear
   bool fp32 = false;  // precision = "mixed": FP32 per-pixel spectral arithmetic (the reference's is FP64)
+  // Extensions (the defaults are the reference's behaviour):
+  int bins = 16;              // spectral grid: 16 (400..700 nm / 20) or 81 (380..780 nm / 5)
+  uint64_t stock_seed = 0;    // parameters of the synthetic stock
+  bool philox = false;        // grain noise from Philox4x32-10 normals instead of SplitMix64 Irwin-Hall rows
+  float grain_sigma = -1.0f;  // >= 0: constant grain density sigma instead of kappa * taper * sqrt(D (Dmax - D))
+  float hal_radius = -1.0f;   // >= 0: halation blur radius in pixels (sigma = radius / 3) instead of psf_halation / pitch
+  float hal_strength = -1.0f; // >= 0: halation gain strength * bleed_e instead of r_e * bleed_e
 };
 
 FilmStageParams film_stage_params_from_table(const std::string& stock, const std::string& format, uint64_t seed,

@@ -257,6 +257,27 @@ int imops_vignette(imops_ctx* ctx, imops_image* img, float strength);
  * IMOPS_E_UNSUPPORTED. */
 int imops_film(imops_ctx* ctx, imops_image* img, int stock, int film_format, uint64_t seed, int output);
 
+/* Film stage with the extension parameters (the Film stage fields of a stage
+ * list: stock, film_format, seed, output, precision, spectral_bins,
+ * stock_seed, grain, grain_sigma, halation_radius, halation_strength).
+ * imops_film_default_params fills the reference's behaviour; the extensions:
+ *   stock 7 = the seeded synthetic colour negative (stock_seed);
+ *   spectral_bins 16 (the reference's 400..700 nm / 20 nm grid) or 81
+ *     (380..780 nm / 5 nm);
+ *   grain 0 = the reference's SplitMix64 Irwin-Hall rows, 1 = Philox4x32-10
+ *     normals; grain_sigma >= 0 replaces the density-dependent amplitude by a
+ *     constant density sigma;
+ *   halation_radius >= 0 (pixels) and halation_strength >= 0 replace the
+ *     stock's halation width (psf_halation / pitch) and gains (r_e bleed_e);
+ *   precision 0 = the reference's FP64 spectral arithmetic, 1 = FP32. */
+typedef struct imops_film_params {
+  int stock, film_format, output, precision, spectral_bins, grain;
+  uint64_t seed, stock_seed;
+  float grain_sigma, halation_radius, halation_strength;
+} imops_film_params;
+int imops_film_default_params(imops_film_params* out);
+int imops_film_ex(imops_ctx* ctx, imops_image* img, const imops_film_params* params);
+
 /* Differentiable film model --------------------------------------------------
  * No counterpart in the reference (its film simulation has no gradients); it is
  * film::process (film/mod.rs:77-129) in the NegativeLinear form with shutter

@@ -324,7 +324,7 @@ def refined_sum(rgb: np.ndarray, channel: int) -> float:
 
 
 STOCKS = {"BwStub": 0, "ColorNeg200": 1, "Portra400": 2, "Ektar100": 3, "FujiPro400H": 4, "EktachromeE100": 5,
-          "TriX400": 6}
+          "TriX400": 6, "Synthetic": 7}
 FORMATS = {"Film35mm": 0, "Film6x6": 1, "Film4x5": 2}
 _dp = C.POINTER(C.c_double)
 
@@ -332,15 +332,23 @@ _dp = C.POINTER(C.c_double)
 
 def film(rgb: np.ndarray, stock: str = "ColorNeg200", film_format: str = "Film35mm", seed: int = 1,
          output: str = "NegativeLinear", shutter: float = 0.0, fp64: bool = False, n_emulsions: int = 3,
-         trace: bool = False):
+         trace: bool = False, spectral_bins: int = 16, stock_seed: int = 0, grain: str = "splitmix",
+         grain_sigma: float = -1.0, halation_radius: float = -1.0, halation_strength: float = -1.0):
     """film::process (film/mod.rs:77-129) on an (H, W, 3) absolute-luminance ACEScg image.
 
+    Extension keywords as the film stage's (spectral_bins 16 | 81, stock "Synthetic" with
+    stock_seed, grain "splitmix" | "philox", grain_sigma, halation_radius,
+    halation_strength; negative: the reference's behaviour).
     Returns the scanned ACEScg image, or (image, trace) with the latent, image-dye and
     mask-dye planes (n_emulsions x H x W, f64) and the PositiveLinear mid/Dmin."""
     L = lib()
     L.oracle_film.restype = C.c_int
     L.oracle_film.argtypes = [_fp, _sz, _sz, C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.c_float, C.c_int,
-                              _dp, _dp, _dp, _fp]
+                              _dp, _dp, _dp, _fp, _dp]
+    ext = np.array([spectral_bins, float(stock_seed), 1.0 if grain == "philox" else 0.0, grain_sigma,
+                    halation_radius, halation_strength], np.float64)
+    if stock_seed >= 2 ** 53:
+        raise ValueError("oracle.film: stock_seed must be below 2^53")
     out = np.array(rgb, np.float32, copy=True, order="C")
     h, w, _ = out.shape
     planes = [np.zeros((n_emulsions, h, w), np.float64) for _ in range(3)] if trace else [None] * 3
@@ -348,7 +356,7 @@ def film(rgb: np.ndarray, stock: str = "ColorNeg200", film_format: str = "Film35
     ptr = lambda a: a.ctypes.data_as(_dp) if a is not None else None
     _rc(L.oracle_film(_f(out), w, h, STOCKS[stock], FORMATS[film_format], seed,
                       1 if output == "PositiveLinear" else 0, shutter, int(fp64), ptr(planes[0]), ptr(planes[1]),
-                      ptr(planes[2]), _f(md) if trace else None), "film")
+                      ptr(planes[2]), _f(md) if trace else None, ext.ctypes.data_as(_dp)), "film")
     if not trace:
         return out
     return out, {"latent": planes[0], "image_dye": planes[1], "mask_dye": planes[2], "mid": md[:3].copy(),
@@ -392,6 +400,62 @@ def film_grain_noise(w: int, h: int, seed: int, layer: int) -> np.ndarray:
     out = np.zeros((h, w), np.float32)
     _film_sigs(lib()).oracle_film_grain_noise(_f(out), w, h, seed, layer)
     return out
+
+
+def film_philox(counter, key) -> np.ndarray:
+    """Philox4x32-10 of a 4-word counter under a 2-word key (the grain extension's generator)."""
+    c = np.ascontiguousarray(counter, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    L = lib()
+    L.oracle_film_philox.restype = None
+    L.oracle_film_philox.argtypes = [C.c_void_p] * 3
+    L.oracle_film_philox(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+def film_grain_noise_philox(n: int, seed: int, layer: int) -> np.ndarray:
+    """Philox grain noise (before its blur) of n pixels."""
+    out = np.zeros(n, np.float32)
+    L = lib()
+    L.oracle_film_grain_noise_philox.restype = None
+    L.oracle_film_grain_noise_philox.argtypes = [_fp, _sz, C.c_ulonglong, _sz]
+    L.oracle_film_grain_noise_philox(_f(out), n, seed, layer)
+    return out
+
+
+def film_stock_knots(stock: str, emulsion: int, spectral_bins: int = 16, stock_seed: int = 0):
+    """(log10 fluence knots, developable fractions) of an emulsion's characteristic curve."""
+    logs, fracs = np.zeros(64), np.zeros(64)
+    L = lib()
+    L.oracle_film_stock_knots.restype = C.c_int
+    L.oracle_film_stock_knots.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, _dp, _dp]
+    n = L.oracle_film_stock_knots(STOCKS[stock], spectral_bins, stock_seed, emulsion, logs.ctypes.data_as(_dp),
+                                  fracs.ctypes.data_as(_dp))
+    if n <= 0:
+        raise ValueError("oracle film_stock_knots failed")
+    return logs[:n], fracs[:n]
+
+
+def film_stock_sensitivity(stock: str, emulsion: int, spectral_bins: int = 16, stock_seed: int = 0) -> np.ndarray:
+    out = np.zeros(81)
+    L = lib()
+    L.oracle_film_stock_sensitivity.restype = C.c_int
+    L.oracle_film_stock_sensitivity.argtypes = [C.c_int, C.c_int, C.c_ulonglong, C.c_int, _dp]
+    n = L.oracle_film_stock_sensitivity(STOCKS[stock], spectral_bins, stock_seed, emulsion, out.ctypes.data_as(_dp))
+    if n <= 0:
+        raise ValueError("oracle film_stock_sensitivity failed")
+    return out[:n]
+
+
+def film_cmf(spectral_bins: int) -> np.ndarray:
+    """CIE 1931 CMFs (3 x bins) on the grid: the reference's table at 16 bins, the fit at 81."""
+    out = np.zeros(3 * spectral_bins)
+    L = lib()
+    L.oracle_film_cmf.restype = None
+    L.oracle_film_cmf.argtypes = [C.c_int, _dp]
+    L.oracle_film_cmf(spectral_bins, out.ctypes.data_as(_dp))
+    return out.reshape(3, spectral_bins)
 
 
 def film_capture_lut(stock: str, emulsion: int):

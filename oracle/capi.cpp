@@ -166,12 +166,24 @@ double oracle_refined_sum(const float* rgb, size_t n, int channel) {
 
 // Film simulation; trace planes (latent, image dye, mask dye: E*w*h each as f64)
 // are written when the pointers are non-null.
+// ext (may be null: the reference's behaviour) = {bins, stock_seed, philox, grain_sigma,
+// halation_radius, halation_strength} as doubles.
 int oracle_film(float* rgb, size_t w, size_t h, int stock, int format, unsigned long long seed, int output,
-                float shutter, int fp64, double* latent, double* image_dye, double* mask_dye, float* mid_dmin6) {
+                float shutter, int fp64, double* latent, double* image_dye, double* mask_dye, float* mid_dmin6,
+                const double* ext) {
   FilmTrace tr;
+  FilmExt x;
+  if (ext) {
+    x.bins = (int)ext[0];
+    x.stock_seed = (uint64_t)ext[1];
+    x.philox = ext[2] != 0.0;
+    x.grain_sigma = (float)ext[3];
+    x.hal_radius = (float)ext[4];
+    x.hal_strength = (float)ext[5];
+  }
   const bool want = latent || image_dye || mask_dye || mid_dmin6;
-  int rc = fp64 ? film_process<double>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr)
-                : film_process<float>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr);
+  int rc = fp64 ? film_process<double>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr, x)
+                : film_process<float>(rgb, w, h, stock, format, seed, output, shutter, want ? &tr : nullptr, x);
   if (rc != 0) return rc;
   if (latent) std::copy(tr.latent.begin(), tr.latent.end(), latent);
   if (image_dye) std::copy(tr.image_dye.begin(), tr.image_dye.end(), image_dye);
@@ -185,6 +197,18 @@ void oracle_film_blur(float* buf, size_t w, size_t h, float sigma) { film_blur(b
 void oracle_film_grain_noise(float* out, size_t w, size_t h, unsigned long long seed, size_t layer) {
   film_grain_noise(out, w, h, seed, layer);
 }
+void oracle_film_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) { film_philox(ctr, key, out); }
+void oracle_film_grain_noise_philox(float* out, size_t n, unsigned long long seed, size_t layer) {
+  film_grain_noise_philox(out, n, seed, layer);
+}
+int oracle_film_stock_knots(int stock, int bins, unsigned long long stock_seed, int emulsion, double* logs,
+                            double* fracs) {
+  return film_stock_knots(stock, bins, stock_seed, emulsion, logs, fracs);
+}
+int oracle_film_stock_sensitivity(int stock, int bins, unsigned long long stock_seed, int emulsion, double* out) {
+  return film_stock_sensitivity(stock, bins, stock_seed, emulsion, out);
+}
+void oracle_film_cmf(int bins, double* out) { film_cmf(bins, out); }
 int oracle_film_capture_lut(int stock, int emulsion, double* logs, double* fracs, double* params) {
   return film_capture_lut(stock, emulsion, logs, fracs, params);
 }

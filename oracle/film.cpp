@@ -17,78 +17,75 @@
 // reference's rounding; double: the FP64 oracle).  The reference's f64 islands
 // (spectral exposure, LUT, scan) are f64 in both.
 //
+// Extensions (the film stage's optional fields): an 81-bin spectral grid
+// 380..780 nm (CIE 1931 CMFs from the multi-lobe Gaussian fit of Wyman, Sloan
+// and Shirley, JCGT 2(2) 2013), the seeded synthetic stock (ColorNeg200's stack
+// with two-Gaussian sensitivities and 5-knot characteristic curves), Philox4x32-10
+// grain noise (Salmon et al., SC 2011) with a constant density sigma, and the
+// halation radius / strength mapping.
+//
+// Stock data: oracle_stock_tables.hpp, generated from the reference's
+// film/stock/*.rs by tests/golden/make_stock_tables.py (not the GPU library's
+// records).
+//
 // Pins: grain noise = SplitMix64 row streams (Python-integer brute force plus
-// the published test vector); blur = reflect-101 brute force; capture LUT =
-// adaptive quadrature; upsample/scan round trip; the reference's end-to-end
-// film test gates.  The per-stock absolute outputs are otherwise parity
-// unpinned (the reference's own tests only gate them loosely).
+// the published test vector); Philox = the Random123 known-answer vectors;
+// blur = reflect-101 brute force; capture LUT = adaptive quadrature;
+// upsample/scan round trip; the CMF fit against the reference's 20 nm table;
+// the reference's end-to-end film test gates.  The per-stock absolute outputs
+// are otherwise pinned only by those gates (the reference's own tests gate them
+// loosely).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
 #include <vector>
 
-#include "../imops_amd/csrc/film_stock_data.hpp"
 #include "oracle_common.hpp"
 #include "oracle_film.hpp"
+#include "oracle_stock_tables.hpp"
 
 namespace oracle {
 
-using namespace film_data;
-
 namespace {
 
-constexpr int NB = 16;
+constexpr int NB = 16;       // the reference's grid (the differentiable model's)
+constexpr int NBMAX = 81;
+
+struct Grid {
+  int n;
+  double l0, dl;
+  double lam(int i) const { return l0 + dl * i; }
+};
+const Grid kG16{16, 400.0, 20.0};  // WavelengthGrid::mvp (spectrum.rs:17-26)
+const Grid kG81{81, 380.0, 5.0};
+const Grid& grid_for(int bins) { return bins == 81 ? kG81 : kG16; }
 
 // The reference rounds its f64 islands to f32 where it stores planes; the FP64
 // oracle keeps the f64 value.
 template <class P> inline P store(double v) { return (P)(float)v; }
 template <> inline double store<double>(double v) { return v; }
 
-double lambda_nm(int i) { return 400.0 + 20.0 * i; }
+double lambda_nm(int i) { return kG16.lam(i); }
 
-void curve_samples(const Curve& c, double* out) {
-  for (int i = 0; i < NB; ++i) {
-    const double l = lambda_nm(i);
-    switch (c.kind) {
-      case CURVE_CONST: out[i] = c.a; break;
-      case CURVE_GAUSS: {
-        const double d = (l - c.b) / c.c;
-        out[i] = c.a * std::exp(-0.5 * d * d);
-        break;
-      }
-      case CURVE_ORANGE: {  // orange_mask_epsilon (color_neg_200.rs:51-65 and the other colour negatives)
-        const double u = (l - 450.0) / 50.0, v = (l - 520.0) / 60.0;
-        const double blue = std::exp(-0.5 * (u * u));
-        const double green = c.c * std::exp(-0.5 * (v * v));
-        out[i] = c.a * blue + c.b * green + c.d;
-        break;
-      }
-      case CURVE_PANCHRO: {  // panchromatic_sensitivity (bw_stub.rs:19-32, trix_400.rs:16-28)
-        const double t = (l - 400.0) / 300.0;
-        const double envelope = std::fmax(std::sin(M_PI * t), 0.0);
-        out[i] = c.a + c.b * envelope;
-        break;
-      }
-      default: out[i] = 0.0;
-    }
-  }
-}
-
-double integrate(const double* s) {  // SpectralCurve::integrate (spectrum.rs:106-118)
+double integrate(const double* s, const Grid& g) {  // SpectralCurve::integrate (spectrum.rs:106-118)
   double acc = 0.0;
-  for (int i = 0; i < NB - 1; ++i) acc += 0.5 * (s[i] + s[i + 1]) * (lambda_nm(i + 1) - lambda_nm(i));
+  for (int i = 0; i < g.n - 1; ++i) acc += 0.5 * (s[i] + s[i + 1]) * (g.lam(i + 1) - g.lam(i));
   return acc;
 }
 
-double evaluate(const double* s, double l) {  // SpectralCurve::evaluate (spectrum.rs:60-78)
-  if (l <= lambda_nm(0)) return s[0];
-  if (l >= lambda_nm(NB - 1)) return s[NB - 1];
-  for (int i = 0; i < NB - 1; ++i)
-    if (l >= lambda_nm(i) && l <= lambda_nm(i + 1)) {
-      const double t = (l - lambda_nm(i)) / (lambda_nm(i + 1) - lambda_nm(i));
+double evaluate(const double* s, const Grid& g, double l) {  // SpectralCurve::evaluate (spectrum.rs:60-78)
+  if (l <= g.lam(0)) return s[0];
+  if (l >= g.lam(g.n - 1)) return s[g.n - 1];
+  for (int i = 0; i < g.n - 1; ++i)
+    if (l >= g.lam(i) && l <= g.lam(i + 1)) {
+      const double t = (l - g.lam(i)) / (g.lam(i + 1) - g.lam(i));
       return s[i] * (1.0 - t) + s[i + 1] * t;
     }
-  return s[NB - 1];
+  return s[g.n - 1];
 }
 
 const double XBAR[16] = {0.01431, 0.13438, 0.34828, 0.29080, 0.09564, 0.00490, 0.06327, 0.29040,
@@ -101,17 +98,48 @@ const double XYZ2AP1[3][3] = {{1.6410233797, -0.3248032942, -0.2364246952},
                               {-0.6636628587, 1.6153315917, 0.0167563477},
                               {0.0117218943, -0.0082844165, 0.9883948585}};
 
-double integrate_cmf(const double* s, const double* cmf) {  // upsample.rs:85-95
+// CIE 1931 CMFs on a grid: the reference's table (upsample.rs:17-30) on its grid,
+// else the Wyman-Sloan-Shirley multi-lobe fit: sums of Gaussian lobes whose
+// width is s1 below the centre and s2 above it.
+struct Cmf {
+  double c[3][NBMAX];
+  explicit Cmf(const Grid& g) {
+    auto lobe = [](double l, double m, double s1, double s2) {
+      const double t = (l - m) / (l < m ? s1 : s2);
+      return std::exp(-0.5 * t * t);
+    };
+    for (int i = 0; i < g.n; ++i) {
+      if (g.n == 16) {
+        c[0][i] = XBAR[i];
+        c[1][i] = YBAR[i];
+        c[2][i] = ZBAR[i];
+        continue;
+      }
+      const double l = g.lam(i);
+      c[0][i] = 1.056 * lobe(l, 599.8, 37.9, 31.0) + 0.362 * lobe(l, 442.0, 16.0, 26.7) -
+                0.065 * lobe(l, 501.1, 20.4, 26.2);
+      c[1][i] = 0.821 * lobe(l, 568.8, 46.9, 40.5) + 0.286 * lobe(l, 530.9, 16.3, 31.1);
+      c[2][i] = 1.217 * lobe(l, 437.0, 11.8, 36.0) + 0.681 * lobe(l, 459.0, 26.0, 13.8);
+    }
+  }
+};
+const Cmf& cmf_for(const Grid& g) {
+  static const Cmf c16(kG16), c81(kG81);
+  return g.n == 81 ? c81 : c16;
+}
+
+double integrate_cmf(const double* s, const double* cmf, const Grid& g) {  // upsample.rs:85-95
   double acc = 0.0;
-  for (int i = 0; i < 15; ++i) {
+  for (int i = 0; i < g.n - 1; ++i) {
     const double a = s[i] * cmf[i], b = s[i + 1] * cmf[i + 1];
-    acc += 0.5 * (a + b) * 20.0;
+    acc += 0.5 * (a + b) * g.dl;
   }
   return acc;
 }
 
-void spectrum_to_acescg(const double* s, double out[3]) {
-  const double x = integrate_cmf(s, XBAR), y = integrate_cmf(s, YBAR), z = integrate_cmf(s, ZBAR);
+void spectrum_to_acescg(const double* s, const Grid& g, double out[3]) {
+  const Cmf& C = cmf_for(g);
+  const double x = integrate_cmf(s, C.c[0], g), y = integrate_cmf(s, C.c[1], g), z = integrate_cmf(s, C.c[2], g);
   for (int i = 0; i < 3; ++i) out[i] = XYZ2AP1[i][0] * x + XYZ2AP1[i][1] * y + XYZ2AP1[i][2] * z;
 }
 
@@ -130,18 +158,18 @@ void inv3(const double m[3][3], double o[3][3]) {  // upsample.rs:56-79
   o[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
 }
 
-struct Basis {  // UpsampleBasis::new (upsample.rs:123-143)
-  double b[3][NB];
+struct Basis {  // UpsampleBasis::new (upsample.rs:123-143) on a grid
+  double b[3][NBMAX];
   double minv[3][3];
-  Basis() {
+  explicit Basis(const Grid& g) {
     const double peaks[3] = {450.0, 550.0, 650.0};
     double cols[3][3];
     for (int k = 0; k < 3; ++k) {
-      for (int i = 0; i < NB; ++i) {
-        const double d = (lambda_nm(i) - peaks[k]) / 40.0;
+      for (int i = 0; i < g.n; ++i) {
+        const double d = (g.lam(i) - peaks[k]) / 40.0;
         b[k][i] = std::exp(-0.5 * d * d);
       }
-      spectrum_to_acescg(b[k], cols[k]);
+      spectrum_to_acescg(b[k], g, cols[k]);
     }
     double m[3][3];
     for (int r = 0; r < 3; ++r)
@@ -150,14 +178,19 @@ struct Basis {  // UpsampleBasis::new (upsample.rs:123-143)
   }
 };
 
-const Basis& basis() {
-  static Basis b;
-  return b;
+const Basis& basis(const Grid& g) {
+  static const Basis b16(kG16), b81(kG81);
+  return g.n == 81 ? b81 : b16;
 }
 
-struct Lut {  // DevelopableFractionLut (capture.rs:37-88)
+// Capture curve: the reference's DevelopableFractionLut (capture.rs:37-88) with
+// 64 knots, or a characteristic curve given by knots (the synthetic stock); both
+// sampled linearly in log10 fluence, clamped at the ends.
+struct Lut {
+  int n = 64;
   double logs[64], fracs[64];
   void build(double mu_ln, double sigma_ln, double k) {
+    n = 64;
     for (int i = 0; i < 64; ++i) {
       const double t = (double)i / 63.0;
       const double log_phi = -4.0 + t * (8.0 - (-4.0));
@@ -182,8 +215,8 @@ struct Lut {  // DevelopableFractionLut (capture.rs:37-88)
     if (!(phi > 0.0)) return fracs[0];
     const double lp = std::log10(phi);
     if (lp <= logs[0]) return fracs[0];
-    if (lp >= logs[63]) return fracs[63];
-    int lo = 0, hi = 63;
+    if (lp >= logs[n - 1]) return fracs[n - 1];
+    int lo = 0, hi = n - 1;
     while (hi - lo > 1) {
       const int mid = (lo + hi) / 2;
       if (logs[mid] <= lp) lo = mid;
@@ -194,9 +227,136 @@ struct Lut {  // DevelopableFractionLut (capture.rs:37-88)
   }
 };
 
+// ---- stock source: a stock's parameters with every curve sampled on a grid
+struct SrcLayer {
+  int kind;  // 0 emulsion, 1 filter, 2 overcoat, 3 antihalation, 4 support
+  float thickness_um;
+  bool has_sens;
+  double sens[NBMAX];
+  double mu_ln, sigma_ln;
+  float agx;
+  double eps[NBMAX], mask[NBMAX];
+  bool has_mask;
+  float d_max, gamma;
+  double capture_k;
+  float recip_p;
+  bool reversal;
+  int knots = 0;  // > 0: characteristic curve knots instead of the crystal LUT
+  double knot_x[8], knot_f[8];
+};
+
+struct Src {
+  Grid g;
+  float box_iso;
+  std::vector<SrcLayer> layers;
+  double reflect[NBMAX], scanner[NBMAX];
+  float psf_local_um, psf_halation_um, developer_diffusion_um, adjacency_beta, dir_diffusion_um;
+  int n_dir;
+  float dir[8][8];
+};
+
+void take(const oracle_stocks::OCurve& c, const Grid& g, double* out) {
+  for (int i = 0; i < g.n; ++i) out[i] = c.present ? (g.n == 81 ? c.s81[i] : c.s16[i]) : 0.0;
+}
+
+Src src_from_tables(int idx, const Grid& g) {
+  const oracle_stocks::OStock& S = oracle_stocks::kStocks[idx];
+  Src r;
+  r.g = g;
+  r.box_iso = S.box_iso;
+  for (int l = 0; l < S.n_layers; ++l) {
+    const oracle_stocks::OLayer& L = S.layers[l];
+    SrcLayer o{};
+    o.kind = L.kind;
+    o.thickness_um = L.thickness_um;
+    o.has_sens = L.sensitivity.present;
+    take(L.sensitivity, g, o.sens);
+    o.mu_ln = L.mu_ln;
+    o.sigma_ln = L.sigma_ln;
+    o.agx = L.silver_halide_fraction;
+    take(L.epsilon, g, o.eps);
+    o.has_mask = L.has_coupler && L.mask_epsilon.present;
+    take(L.mask_epsilon, g, o.mask);
+    o.d_max = L.d_max;
+    o.gamma = L.gamma_contrast;
+    o.capture_k = L.capture_k;
+    o.recip_p = L.reciprocity_p;
+    o.reversal = L.is_reversal;
+    r.layers.push_back(o);
+  }
+  take(S.ah_reflectance, g, r.reflect);
+  take(S.scanner_light, g, r.scanner);
+  r.psf_local_um = S.psf_local_um;
+  r.psf_halation_um = S.psf_halation_um;
+  r.developer_diffusion_um = S.developer_diffusion_um;
+  r.adjacency_beta = S.adjacency_beta;
+  r.dir_diffusion_um = S.dir_diffusion_um;
+  r.n_dir = S.n_dir;
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) r.dir[i][j] = S.dir[i][j];
+  return r;
+}
+
+// The synthetic stock: ColorNeg200's stack; emulsion k (blue, green, red;
+// base 450, 550, 650 nm) gets the sensitivity
+//   exp(-(l - mu1)^2 / (2 s1^2)) + a2 exp(-(l - mu2)^2 / (2 s2^2))
+// and a 5-knot characteristic curve, with no reciprocity failure.  Uniforms
+// from a SplitMix64 stream started at `seed`
+// (u = top 53 bits / 2^53): mu1 = base + 30 (u - 1/2), s1 = 25 + 20 u,
+// mu2 = mu1 + 100 (u - 1/2), s2 = 20 + 30 u, a2 = 0.15 + 0.35 u, five knot
+// positions -2.5 + j + 0.3 (u - 1/2), then four increments 0.2 + u (knots 1..4)
+// whose running share c_j gives the fraction 0.005 + 0.965 c_j (c_0 = 0: the toe).
+// Fourteen uniforms per emulsion.
+Src src_synthetic(uint64_t seed, const Grid& g) {
+  Src r = src_from_tables(1, g);  // ColorNeg200
+  uint64_t state = seed;
+  auto uniform = [&]() {
+    state += 0x9E3779B97F4A7C15ull;
+    uint64_t z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) / 9007199254740992.0;
+  };
+  const double base[3] = {450.0, 550.0, 650.0};
+  int k = 0;
+  for (SrcLayer& L : r.layers) {
+    if (L.kind != 0) continue;
+    const double mu1 = base[k] + 30.0 * (uniform() - 0.5);
+    const double s1 = 25.0 + 20.0 * uniform();
+    const double mu2 = mu1 + 100.0 * (uniform() - 0.5);
+    const double s2 = 20.0 + 30.0 * uniform();
+    const double a2 = 0.15 + 0.35 * uniform();
+    for (int i = 0; i < g.n; ++i) {
+      const double d1 = (g.lam(i) - mu1) / s1, d2 = (g.lam(i) - mu2) / s2;
+      L.sens[i] = std::exp(-0.5 * d1 * d1) + a2 * std::exp(-0.5 * d2 * d2);
+    }
+    L.recip_p = 1.0f;
+    L.knots = 5;
+    for (int j = 0; j < 5; ++j) L.knot_x[j] = -2.5 + (double)j + 0.3 * (uniform() - 0.5);
+    double inc[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, total = 0.0;
+    for (int j = 1; j < 5; ++j) {
+      inc[j] = 0.2 + uniform();
+      total += inc[j];
+    }
+    double run = 0.0;
+    for (int j = 0; j < 5; ++j) {
+      run += inc[j];
+      L.knot_f[j] = 0.005 + 0.965 * (run / total);
+    }
+    ++k;
+  }
+  return r;
+}
+
+Src load_src(int idx, int bins, uint64_t stock_seed) {
+  const Grid& g = grid_for(bins);
+  return idx == 7 ? src_synthetic(stock_seed, g) : src_from_tables(idx, g);
+}
+
 struct Emulsion {
   int layer;
-  double sens[NB], eps[NB], mask[NB];
+  double sens[NBMAX], eps[NBMAX], mask[NBMAX];
   bool has_mask, reversal;
   float d_max, gamma, recip_p;
   Lut lut;
@@ -204,58 +364,76 @@ struct Emulsion {
 };
 
 struct Derived {
-  const Stock* s;
+  Src src;
+  const Src* s;
+  Grid g;
   std::vector<Emulsion> em;
-  double trans[12][NB];  // per-layer Beer-Lambert transmittance (absorbing layers)
+  double trans[12][NBMAX];  // per-layer Beer-Lambert transmittance (absorbing layers)
   bool absorbs[12], is_emulsion[12];
-  double reflect[NB], scanner[NB];
+  double reflect[NBMAX], scanner[NBMAX];
   bool any_reversal = false;
   // differentiable-model switches (FilmFitParams); defaults reproduce film::process
   double hal_strength = 1.0, hal_radius = 1.0, scan_scale = 0.0;  // scan_scale <= 0: from this stock's Dmin
   bool grain = true, dir = true, adjacency = true;
-  explicit Derived(const Stock* st, const FilmFitParams* fit = nullptr) : s(st) {
-    curve_samples(st->ah_reflectance, reflect);
-    curve_samples(st->scanner_light, scanner);
+  FilmExt ext;  // extension parameters (defaults: the reference's behaviour)
+  Derived(const Derived&) = delete;  // `s` points into this object
+  Derived& operator=(const Derived&) = delete;
+  explicit Derived(const Src& source, const FilmFitParams* fit = nullptr) : src(source), s(&src), g(source.g) {
+    const Src* st = &src;
+    for (int i = 0; i < g.n; ++i) {
+      reflect[i] = st->reflect[i];
+      scanner[i] = st->scanner[i];
+    }
     int ei = 0;
-    for (int l = 0; l < st->n_layers; ++l) {
-      const Layer& L = st->layers[l];
-      if (L.is_reversal) any_reversal = true;
-      double curve[NB];
-      curve_samples(L.sensitivity, curve);
-      if (fit && L.kind == L_EMULSION && L.sensitivity.kind != CURVE_NONE)
-        for (int i = 0; i < NB; ++i) curve[i] = fit->sens[ei][i];
-      is_emulsion[l] = L.kind == L_EMULSION;
-      absorbs[l] = L.kind != L_SUPPORT && L.sensitivity.kind != CURVE_NONE;
+    for (size_t l = 0; l < st->layers.size(); ++l) {
+      const SrcLayer& L = st->layers[l];
+      if (L.reversal) any_reversal = true;
+      double curve[NBMAX];
+      for (int i = 0; i < g.n; ++i) curve[i] = L.sens[i];
+      if (fit && L.kind == 0 && L.has_sens)
+        for (int i = 0; i < g.n; ++i) curve[i] = fit->sens[ei][i];
+      is_emulsion[l] = L.kind == 0;
+      absorbs[l] = L.kind != 4 && L.has_sens;
       const double thickness = (double)L.thickness_um;
-      for (int i = 0; i < NB; ++i) {
+      for (int i = 0; i < g.n; ++i) {
         double od;
-        if (L.kind == L_EMULSION) {
+        if (L.kind == 0) {
           const double sigma = curve[i] * 2.0;  // ABSORPTION_SIGMA_SCALE_PER_UM (constants.rs:51)
-          od = sigma * (double)L.silver_halide_fraction * thickness;
+          od = sigma * (double)L.agx * thickness;
         } else {
           od = curve[i] * thickness;
         }
         trans[l][i] = std::exp(-od);
       }
-      if (L.kind != L_EMULSION) continue;
+      if (L.kind != 0) continue;
       Emulsion e;
-      e.layer = l;
-      for (int i = 0; i < NB; ++i) e.sens[i] = curve[i];
-      curve_samples(L.dye_epsilon, e.eps);
-      e.has_mask = L.mask_epsilon.kind != CURVE_NONE;
-      curve_samples(L.mask_epsilon, e.mask);
-      e.reversal = L.is_reversal;
+      e.layer = (int)l;
+      for (int i = 0; i < g.n; ++i) {
+        e.sens[i] = curve[i];
+        e.eps[i] = L.eps[i];
+        e.mask[i] = L.mask[i];
+      }
+      e.has_mask = L.has_mask;
+      e.reversal = L.reversal;
       e.d_max = fit ? (float)fit->d_max[ei] : L.d_max;
-      e.gamma = fit ? (float)fit->gamma[ei] : L.gamma_contrast;
+      e.gamma = fit ? (float)fit->gamma[ei] : L.gamma;
       ++ei;
-      e.recip_p = L.reciprocity_p;
-      const double mu = std::log(L.crystal_mean_um), sg = L.crystal_sigma_ln;
-      e.lut.build(mu, sg, L.capture_k);
+      e.recip_p = L.recip_p;
+      const double mu = L.mu_ln, sg = L.sigma_ln;
+      if (L.knots > 0) {
+        e.lut.n = L.knots;
+        for (int j = 0; j < L.knots; ++j) {
+          e.lut.logs[j] = L.knot_x[j];
+          e.lut.fracs[j] = L.knot_f[j];
+        }
+      } else {
+        e.lut.build(mu, sg, L.capture_k);
+      }
       // grain kappa at 1 um pitch (stock/mod.rs:193-211)
       const double mean_s = std::exp(mu + 0.5 * sg * sg);
       const double r = std::max(mean_s * 0.5, 1e-6);
       const double volume = 1.04719755119659774615 * 4.0 * r * r * r;  // FRAC_PI_3 * 4 r^3
-      const double rho = (double)L.silver_halide_fraction * thickness / std::max(volume, 1e-18);
+      const double rho = (double)L.agx * thickness / std::max(volume, 1e-18);
       e.kappa_ref = (float)(1.0 / std::sqrt(rho));
       em.push_back(e);
     }
@@ -306,13 +484,18 @@ void blur(std::vector<P>& buf, size_t w, size_t h, float sigma) {
 template <class P>
 using Planes = std::vector<std::vector<P>>;
 
-// exposure/mod.rs:45-122
+// exposure/mod.rs:45-122 (band mean over the grid's span: 300 nm on the
+// reference grid).  Halation extension: hal_radius >= 0 px gives sigma =
+// radius / 3; hal_strength >= 0 gives the gain strength * bleed_e whatever the
+// backing reflectance.
 template <class P>
 Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float pitch, float shutter) {
   const size_t n = w * h;
   const size_t E = d.em.size();
-  const Stock& st = *d.s;
-  const Basis& B = basis();
+  const Src& st = *d.s;
+  const Grid& g = d.g;
+  const Basis& B = basis(g);
+  const double span = g.lam(g.n - 1) - g.lam(0);
   Planes<P> abs(E, std::vector<P>(n));
 #pragma omp parallel for schedule(static)
   for (long long p = 0; p < (long long)n; ++p) {
@@ -322,25 +505,25 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
       wgt[r] = B.minv[r][0] * px[0] + B.minv[r][1] * px[1] + B.minv[r][2] * px[2];
       if (wgt[r] < 0.0) wgt[r] = 0.0;
     }
-    double phi[NB];
-    for (int i = 0; i < NB; ++i) {
+    double phi[NBMAX];
+    for (int i = 0; i < g.n; ++i) {
       const double s = wgt[0] * B.b[0][i] + wgt[1] * B.b[1][i] + wgt[2] * B.b[2][i];
-      const double e_rel = 550.0 / lambda_nm(i);
+      const double e_rel = 550.0 / g.lam(i);
       phi[i] = s / e_rel;
     }
     size_t e = 0;
-    for (int l = 0; l < st.n_layers; ++l) {
+    for (size_t l = 0; l < st.layers.size(); ++l) {
       if (!d.absorbs[l]) continue;
-      double absorbed[NB];
-      for (int i = 0; i < NB; ++i) {
+      double absorbed[NBMAX];
+      for (int i = 0; i < g.n; ++i) {
         const double phi_t = phi[i] * d.trans[l][i];
         absorbed[i] = phi[i] - phi_t;
         phi[i] = phi_t;
       }
       if (d.is_emulsion[l]) {
         double acc = 0.0;
-        for (int i = 0; i < 15; ++i) acc += 0.5 * (absorbed[i] + absorbed[i + 1]) * 20.0;
-        abs[e++][p] = store<P>(acc / 300.0);
+        for (int i = 0; i < g.n - 1; ++i) acc += 0.5 * (absorbed[i] + absorbed[i + 1]) * g.dl;
+        abs[e++][p] = store<P>(acc / span);
       }
     }
   }
@@ -354,13 +537,20 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
       for (size_t i = 0; i < n; ++i) plane[i] = keep * plane[i] + f * sc[i];
     }
   }
-  const float sigma_wide = (float)((double)std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f) * d.hal_radius);
+  const FilmExt& x = d.ext;
+  const float sigma_wide =
+      x.hal_radius >= 0.0f ? x.hal_radius / 3.0f
+                           : (float)((double)std::max(st.psf_halation_um / std::max(pitch, 1e-6f), 0.0f) * d.hal_radius);
   bool do_wide = sigma_wide >= 1e-3f;
   if (do_wide) {
-    float max_r = 0.0f;
-    const double bands[4] = {450.0, 550.0, 650.0, 700.0};
-    for (double bl : bands) max_r = std::max(max_r, (float)evaluate(d.reflect, bl));
-    do_wide = max_r > 0.0f;
+    if (x.hal_strength >= 0.0f) {
+      do_wide = x.hal_strength > 0.0f;
+    } else {
+      float max_r = 0.0f;
+      const double bands[4] = {450.0, 550.0, 650.0, 700.0};
+      for (double bl : bands) max_r = std::max(max_r, (float)evaluate(d.reflect, g, bl));
+      do_wide = max_r > 0.0f;
+    }
   }
   if (do_wide && E > 0) {
     std::vector<P> bounce = abs[E - 1];
@@ -369,25 +559,30 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
       const Emulsion& em = d.em[e];
       // bleed weight from the sensitivity peak (halation.rs:102-132)
       double peak = 550.0, mx = -INFINITY;
-      for (int i = 0; i < NB; ++i)
-        if (em.sens[i] > mx) { mx = em.sens[i]; peak = lambda_nm(i); }
+      for (int i = 0; i < g.n; ++i)
+        if (em.sens[i] > mx) { mx = em.sens[i]; peak = g.lam(i); }
       const float wb = 0.20f, wg = 0.45f, wr = 1.0f;
       float bleed;
       if (peak <= 450.0) bleed = wb;
       else if (peak <= 550.0) bleed = wb + (wg - wb) * (float)((peak - 450.0) / 100.0);
       else if (peak <= 650.0) bleed = wg + (wr - wg) * (float)((peak - 550.0) / 100.0);
       else bleed = wr;
-      // effective_reflectance (halation.rs:155-161)
-      float r_e;
-      const double norm = integrate(em.sens);
-      if (norm <= 1e-9) {
-        r_e = (float)evaluate(d.reflect, 650.0);
+      float gain;
+      if (x.hal_strength >= 0.0f) {
+        gain = x.hal_strength * bleed;
       } else {
-        double prod[NB];
-        for (int i = 0; i < NB; ++i) prod[i] = d.reflect[i] * em.sens[i];
-        r_e = (float)(integrate(prod) / norm);
+        // effective_reflectance (halation.rs:155-161)
+        float r_e;
+        const double norm = integrate(em.sens, g);
+        if (norm <= 1e-9) {
+          r_e = (float)evaluate(d.reflect, g, 650.0);
+        } else {
+          double prod[NBMAX];
+          for (int i = 0; i < g.n; ++i) prod[i] = d.reflect[i] * em.sens[i];
+          r_e = (float)(integrate(prod, g) / norm);
+        }
+        gain = (float)((double)(r_e * bleed) * d.hal_strength);
       }
-      const float gain = (float)((double)(r_e * bleed) * d.hal_strength);
       if (gain <= 0.0f) continue;
       for (size_t i = 0; i < n; ++i) abs[e][i] += (P)gain * bounce[i];
     }
@@ -404,6 +599,38 @@ Planes<P> expose(const float* rgb, size_t w, size_t h, const Derived& d, float p
       plane[i] = store<P>(em.lut.sample((double)(P)(plane[i] * (P)eta)));
   }
   return abs;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC 2011): per round the products
+// M0 c0 and M1 c2 (32x32 -> 64 bits) are folded with the key, and the key
+// advances by the Weyl increments.
+void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t out0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+    const uint32_t out2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[1] = (uint32_t)p1;
+    c[3] = (uint32_t)p0;
+    c[0] = out0;
+    c[2] = out2;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Grain noise of the Philox extension: counter (pixel index low / high 32 bits,
+// layer, 0), key (seed low / high 32 bits), a standard normal from the first
+// two words by Box-Muller in f64 ((x + 1/2) / 2^32 keeps both uniforms in (0, 1)).
+template <class P>
+void grain_noise_philox(P* noise, size_t n, uint64_t seed, size_t layer) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    uint32_t c[4] = {(uint32_t)(uint64_t)i, (uint32_t)((uint64_t)i >> 32), (uint32_t)layer, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = ((double)c[0] + 0.5) / 4294967296.0, u2 = ((double)c[1] + 0.5) / 4294967296.0;
+    noise[i] = (P)(float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * 3.14159265358979323846 * u2));
+  }
 }
 
 // Unit grain noise n(x) = sum of 12 uniforms - 6 from one SplitMix64 stream
@@ -437,7 +664,7 @@ struct Dyes {
 template <class P>
 Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, uint64_t seed, float pitch) {
   const size_t n = w * h, E = d.em.size();
-  const Stock& st = *d.s;
+  const Src& st = *d.s;
   Dyes<P> dy;
   // reduce (reduction.rs:16-68)
   for (size_t e = 0; e < E; ++e) {
@@ -489,12 +716,15 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
   }
   // grain (grain.rs:53-100)
   const float sigma_g = 3.0f / std::max(pitch, 1e-6f);
+  // extension: grain_sigma >= 0 is a constant density sigma; philox: Philox normals
+  const bool absolute = d.ext.grain_sigma >= 0.0f;
   for (size_t e = 0; e < (d.grain ? E : 0); ++e) {
     const P d_max = (P)d.em[e].d_max;
-    const P kappa = (P)((d.em[e].kappa_ref / std::max(pitch, 1e-6f)) * 0.15f);
+    const P kappa = absolute ? (P)d.ext.grain_sigma : (P)((d.em[e].kappa_ref / std::max(pitch, 1e-6f)) * 0.15f);
     if (kappa <= (P)0 || d_max <= (P)0) continue;
     std::vector<P> noise(n);
-    grain_noise(noise.data(), w, h, seed, e);
+    if (d.ext.philox) grain_noise_philox(noise.data(), n, seed, e);
+    else grain_noise(noise.data(), w, h, seed, e);
     blur(noise, w, h, std::max(sigma_g, 1.0f));
     double var = 0.0;
     for (size_t i = 0; i < n; ++i) var += (double)noise[i] * (double)noise[i];
@@ -505,7 +735,7 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
       const P dens = rclamp(plane[i], (P)0, d_max);
       const P eps_toe = (P)0.02f * d_max;
       const P taper = std::min(dens / (dens + eps_toe), (P)1);
-      const P sigma_d = taper * std::sqrt(std::max(dens * (d_max - dens), (P)0));
+      const P sigma_d = absolute ? (P)1 : taper * std::sqrt(std::max(dens * (d_max - dens), (P)0));
       P v = dens + kappa * sigma_d * noise[i] * norm;
       plane[i] = rclamp(v, (P)0, d_max * (P)1.05f);
     }
@@ -515,15 +745,16 @@ Dyes<P> develop(const Derived& d, const Planes<P>& latent, size_t w, size_t h, u
 
 // densitometry.rs:13-73 (before the Dmin normalisation)
 void scan_pixel(const Derived& d, const double* di, const double* dm, double out[3]) {
-  double dens[NB] = {0};
+  const Grid& g = d.g;
+  double dens[NBMAX] = {0};
   for (size_t e = 0; e < d.em.size(); ++e)
-    for (int l = 0; l < NB; ++l) {
+    for (int l = 0; l < g.n; ++l) {
       dens[l] += di[e] * d.em[e].eps[l];
       if (d.em[e].has_mask) dens[l] += dm[e] * d.em[e].mask[l];
     }
-  double t[NB];
-  for (int i = 0; i < NB; ++i) t[i] = std::pow(10.0, -dens[i]) * d.scanner[i];
-  spectrum_to_acescg(t, out);
+  double t[NBMAX];
+  for (int i = 0; i < g.n; ++i) t[i] = std::pow(10.0, -dens[i]) * d.scanner[i];
+  spectrum_to_acescg(t, g, out);
 }
 
 void dmin_reference(const Derived& d, double out[3]) {
@@ -614,17 +845,17 @@ void run_negative(const Derived& d, const float* rgb_in, size_t w, size_t h, flo
 
 }  // namespace
 
-const film_data::Stock* film_stock(int idx) { return stock_by_index(idx); }
+bool stock_ok(int idx) { return idx >= 0 && idx <= 7; }
 
-// film/mod.rs:77-129
+// film/mod.rs:77-129 (and the extension parameters of `ext`)
 template <class P>
 int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint64_t seed, int output,
-                 float shutter_seconds, FilmTrace* trace) {
-  const Stock* st = stock_by_index(stock_idx);
-  if (!st || w == 0 || h == 0) return -1;
-  const Derived d(st);
+                 float shutter_seconds, FilmTrace* trace, const FilmExt& ext) {
+  if (!stock_ok(stock_idx) || w == 0 || h == 0 || (ext.bins != 16 && ext.bins != 81)) return -1;
+  Derived d(load_src(stock_idx, ext.bins, ext.stock_seed));
+  d.ext = ext;
   const float pitch = format_width_mm(format) * 1000.0f / (float)w;
-  const float shutter = shutter_seconds > 0.0f ? shutter_seconds : 1.0f / st->box_iso;
+  const float shutter = shutter_seconds > 0.0f ? shutter_seconds : 1.0f / d.s->box_iso;
   std::vector<float> out(w * h * 3);
   run_negative<P>(d, rgb, w, h, pitch, shutter, seed, out.data(), trace);
   if (output == 1 && !d.any_reversal) {
@@ -633,7 +864,7 @@ int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint
     // mid-gray negative: 32x32 flat absolute-luminance gray, seed 0 (film/mod.rs:136-163)
     const int N = 32;
     // relative_to_absolute_y with sunny-16 metering (film/mod.rs:154-163, radiance.rs:19-25, 52-59)
-    const float iso = std::max(st->box_iso, 1.0f);
+    const float iso = std::max(d.s->box_iso, 1.0f);
     const double t = (double)(1.0f / iso), fn = 8.0, v = (double)0.185f;
     const float gv = (float)(12.5 * v * (fn * fn) / (t * (double)iso));
     std::vector<float> gray(N * N * 3, gv), gout(N * N * 3);
@@ -651,8 +882,8 @@ int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint
   return 0;
 }
 
-template int film_process<float>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*);
-template int film_process<double>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*);
+template int film_process<float>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*, const FilmExt&);
+template int film_process<double>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*, const FilmExt&);
 
 // The differentiable film model: film::process (NegativeLinear, shutter 1/box
 // ISO, a given pixel pitch) with grain, DIR inhibition and adjacency off, the emulsion sensitivities,
@@ -661,22 +892,22 @@ template int film_process<double>(float*, size_t, size_t, int, int, uint64_t, in
 // stock's Dmin (so the normalisation is not a function of the parameters).
 template <class P>
 int film_fit_process(float* rgb, size_t w, size_t h, int stock_idx, float pitch_um, const FilmFitParams& fit) {
-  const Stock* st = stock_by_index(stock_idx);
-  if (!st || w == 0 || h == 0) return -1;
+  if (!stock_ok(stock_idx) || w == 0 || h == 0) return -1;
+  const Src src = load_src(stock_idx, 16, 0);
   double nominal_scale;
   {
-    const Derived nom(st);
+    const Derived nom(src);
     double dmin[3];
     dmin_reference(nom, dmin);
     nominal_scale = 1.0 / std::max(std::max(std::max(dmin[0], dmin[1]), dmin[2]), 1e-12);
   }
-  Derived d(st, &fit);
+  Derived d(src, &fit);
   d.hal_strength = fit.hal_strength;
   d.hal_radius = fit.hal_radius;
   d.scan_scale = nominal_scale;
   d.grain = d.dir = d.adjacency = false;
   std::vector<float> out(w * h * 3);
-  run_negative<P>(d, rgb, w, h, pitch_um, 1.0f / st->box_iso, 0, out.data(), nullptr);
+  run_negative<P>(d, rgb, w, h, pitch_um, 1.0f / src.box_iso, 0, out.data(), nullptr);
   std::copy(out.begin(), out.end(), rgb);
   return 0;
 }
@@ -684,9 +915,8 @@ template int film_fit_process<float>(float*, size_t, size_t, int, float, const F
 template int film_fit_process<double>(float*, size_t, size_t, int, float, const FilmFitParams&);
 
 int film_fit_nominal(int stock_idx, FilmFitParams* out) {
-  const Stock* st = stock_by_index(stock_idx);
-  if (!st) return -1;
-  const Derived d(st);
+  if (!stock_ok(stock_idx)) return -1;
+  const Derived d(load_src(stock_idx, 16, 0));
   *out = FilmFitParams{};
   out->n_emulsions = (int)d.em.size();
   for (size_t e = 0; e < d.em.size(); ++e) {
@@ -711,28 +941,63 @@ void film_grain_noise(float* out, size_t w, size_t h, uint64_t seed, size_t laye
   grain_noise(out, w, h, seed, layer);
 }
 
+void film_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  for (int i = 0; i < 4; ++i) out[i] = ctr[i];
+  philox4x32_10(out, key[0], key[1]);
+}
+
+void film_grain_noise_philox(float* out, size_t n, uint64_t seed, size_t layer) {
+  grain_noise_philox(out, n, seed, layer);
+}
+
 int film_capture_lut(int stock_idx, int emulsion, double* logs, double* fracs, double params[3]) {
-  const Stock* st = stock_by_index(stock_idx);
-  if (!st) return -1;
-  const Derived d(st);
+  if (!stock_ok(stock_idx) || stock_idx == 7) return -1;
+  const Derived d(load_src(stock_idx, 16, 0));
   if (emulsion < 0 || emulsion >= (int)d.em.size()) return -1;
   const Emulsion& e = d.em[emulsion];
   std::copy(e.lut.logs, e.lut.logs + 64, logs);
   std::copy(e.lut.fracs, e.lut.fracs + 64, fracs);
-  const Layer& L = st->layers[e.layer];
-  params[0] = std::log(L.crystal_mean_um);
-  params[1] = L.crystal_sigma_ln;
+  const SrcLayer& L = d.s->layers[e.layer];
+  params[0] = L.mu_ln;
+  params[1] = L.sigma_ln;
   params[2] = L.capture_k;
   return 0;
 }
 
+// The characteristic curve knots of an emulsion of a stock (64 LUT knots, or a knot curve's).
+int film_stock_knots(int stock_idx, int bins, uint64_t stock_seed, int emulsion, double* logs, double* fracs) {
+  if (!stock_ok(stock_idx) || (bins != 16 && bins != 81)) return -1;
+  const Derived d(load_src(stock_idx, bins, stock_seed));
+  if (emulsion < 0 || emulsion >= (int)d.em.size()) return -1;
+  const Lut& L = d.em[emulsion].lut;
+  std::copy(L.logs, L.logs + L.n, logs);
+  std::copy(L.fracs, L.fracs + L.n, fracs);
+  return L.n;
+}
+
+// An emulsion's sensitivity samples on the grid.
+int film_stock_sensitivity(int stock_idx, int bins, uint64_t stock_seed, int emulsion, double* out) {
+  if (!stock_ok(stock_idx) || (bins != 16 && bins != 81)) return -1;
+  const Derived d(load_src(stock_idx, bins, stock_seed));
+  if (emulsion < 0 || emulsion >= (int)d.em.size()) return -1;
+  std::copy(d.em[emulsion].sens, d.em[emulsion].sens + d.g.n, out);
+  return d.g.n;
+}
+
+void film_cmf(int bins, double* out) {
+  const Grid& g = grid_for(bins);
+  const Cmf& c = cmf_for(g);
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < g.n; ++i) out[k * g.n + i] = c.c[k][i];
+}
+
 double film_lut_sample(int stock_idx, int emulsion, double phi) {
-  const Derived d(stock_by_index(stock_idx));
+  const Derived d(load_src(stock_idx, 16, 0));
   return d.em[emulsion].lut.sample(phi);
 }
 
 void film_upsample(const float rgb[3], double out[16]) {
-  const Basis& B = basis();
+  const Basis& B = basis(kG16);
   double wgt[3];
   for (int r = 0; r < 3; ++r) {
     wgt[r] = B.minv[r][0] * rgb[0] + B.minv[r][1] * rgb[1] + B.minv[r][2] * rgb[2];
@@ -741,12 +1006,11 @@ void film_upsample(const float rgb[3], double out[16]) {
   for (int i = 0; i < NB; ++i) out[i] = wgt[0] * B.b[0][i] + wgt[1] * B.b[1][i] + wgt[2] * B.b[2][i];
 }
 
-void film_spectrum_to_acescg(const double s[16], double out[3]) { spectrum_to_acescg(s, out); }
+void film_spectrum_to_acescg(const double s[16], double out[3]) { spectrum_to_acescg(s, kG16, out); }
 
 int film_normalized_dmin(int stock_idx, float out[3]) {
-  const Stock* st = stock_by_index(stock_idx);
-  if (!st) return -1;
-  const Derived d(st);
+  if (!stock_ok(stock_idx)) return -1;
+  const Derived d(load_src(stock_idx, 16, 0));
   normalized_dmin(d, out);
   return 0;
 }

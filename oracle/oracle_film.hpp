@@ -4,8 +4,6 @@
 #include <cstdint>
 #include <vector>
 
-namespace film_data { struct Stock; }
-
 namespace oracle {
 
 // Intermediate planes of one run (emulsion-major, width*height each).
@@ -14,7 +12,15 @@ struct FilmTrace {
   float mid[3] = {0, 0, 0}, dmin[3] = {0, 0, 0};
 };
 
-const film_data::Stock* film_stock(int idx);
+// Extension parameters of the film stage (defaults: the reference's behaviour).
+struct FilmExt {
+  int bins = 16;               // spectral grid: 16 (400..700 / 20 nm) or 81 (380..780 / 5 nm)
+  uint64_t stock_seed = 0;     // the synthetic stock (stock index 7)
+  bool philox = false;         // Philox4x32-10 normal grain noise
+  float grain_sigma = -1.0f;   // >= 0: constant grain density sigma
+  float hal_radius = -1.0f;    // >= 0: halation radius in pixels (sigma = radius / 3)
+  float hal_strength = -1.0f;  // >= 0: halation gain strength * bleed_e
+};
 
 // Parameters of the differentiable film model (emulsions top to bottom).
 struct FilmFitParams {
@@ -33,10 +39,15 @@ int film_fit_nominal(int stock_idx, FilmFitParams* out);
 // output: 0 NegativeLinear, 1 PositiveLinear.  shutter_seconds <= 0: 1/box ISO.
 template <class P>
 int film_process(float* rgb, size_t w, size_t h, int stock_idx, int format, uint64_t seed, int output,
-                 float shutter_seconds, FilmTrace* trace);
+                 float shutter_seconds, FilmTrace* trace, const FilmExt& ext = FilmExt{});
 
 void film_blur(float* buf, size_t w, size_t h, float sigma);
 void film_grain_noise(float* out, size_t w, size_t h, uint64_t seed, size_t layer);
+void film_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void film_grain_noise_philox(float* out, size_t n, uint64_t seed, size_t layer);
+int film_stock_knots(int stock_idx, int bins, uint64_t stock_seed, int emulsion, double* logs, double* fracs);
+int film_stock_sensitivity(int stock_idx, int bins, uint64_t stock_seed, int emulsion, double* out);
+void film_cmf(int bins, double* out);
 int film_capture_lut(int stock_idx, int emulsion, double* logs, double* fracs, double params[3]);
 double film_lut_sample(int stock_idx, int emulsion, double phi);
 void film_upsample(const float rgb[3], double out[16]);

@@ -113,6 +113,12 @@ target_color_space = "Srgb"
     ('[[pipeline_modules]]\nname = "Film"\nseed = 1.5\n', "Invalid Film config"),
     ('[[pipeline_modules]]\nname = "Film"\nseed = 1e3\n', "Invalid Film config"),
     ('{"pipeline_modules": [{"name": "Film", "seed": 18446744073709551616}]}', "cant decode"),
+    # film extension fields
+    ('[[pipeline_modules]]\nname = "Film"\nspectral_bins = 40\n', "spectral_bins"),
+    ('[[pipeline_modules]]\nname = "Film"\ngrain = "mt19937"\n', "Invalid Film config"),
+    ('[[pipeline_modules]]\nname = "Film"\ngrain_sigma = -1\n', "grain_sigma"),
+    ('[[pipeline_modules]]\nname = "Film"\nhalation_radius = -3\n', "halation_radius"),
+    ('[[pipeline_modules]]\nname = "Film"\nstock = "Velvia"\n', "Unknown film stock"),
 ])
 def test_parse_errors(bad, msg):
     with pytest.raises(im.ImopsError, match=msg):
