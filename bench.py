@@ -149,6 +149,8 @@ def oracle_chain(raw_crop: np.ndarray, cal: np.ndarray, fp64: bool, cfa: str = "
     plane = oracle.normalize_raw(raw_crop, BLACK, WHITE)
     if demosaic == "amaze":
         x = oracle.amaze(plane, cfa, fp64=fp64)
+    elif demosaic == "markesteijn3":
+        x = oracle.markesteijn3(plane, cfa, fp64=fp64)
     else:
         x = oracle.markesteijn(plane, cfa)
     x, _ = oracle.bm3d(x, p, fp64=fp64)

@@ -33,7 +33,8 @@ __all__ = [
 # ColorSpaceTag names accepted by CST (modules.rs:107-112, serde_plain names)
 COLOR_SPACES = {"Srgb": CS_SRGB, "LinearSrgb": CS_LINEAR_SRGB, "AcesCg": CS_ACESCG, "XyzD65": CS_XYZD65}
 _ALGORITHMS = {"amaze": DEMOSAIC_AMAZE, "markesteijn": DEMOSAIC_MARKESTEIJN, "fast": _lib.DEMOSAIC_FAST,
-               "superfast": _lib.DEMOSAIC_SUPERFAST, "supersuperfast": _lib.DEMOSAIC_SUPERSUPERFAST}
+               "superfast": _lib.DEMOSAIC_SUPERFAST, "supersuperfast": _lib.DEMOSAIC_SUPERSUPERFAST,
+               "markesteijn3": _lib.DEMOSAIC_MARKESTEIJN3, "markesteijn3_1pass": _lib.DEMOSAIC_MARKESTEIJN3_1PASS}
 # Film stage choices (modules.rs:800-830; StockId order of film/stock/mod.rs:124-133)
 STOCKS = {"BwStub": 0, "ColorNeg200": 1, "Portra400": 2, "Ektar100": 3, "FujiPro400H": 4, "EktachromeE100": 5,
           "TriX400": 6, "Synthetic": 7}

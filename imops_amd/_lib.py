@@ -27,6 +27,8 @@ DEMOSAIC_MARKESTEIJN = 1
 DEMOSAIC_FAST = 2
 DEMOSAIC_SUPERFAST = 3
 DEMOSAIC_SUPERSUPERFAST = 4
+DEMOSAIC_MARKESTEIJN3 = 5
+DEMOSAIC_MARKESTEIJN3_1PASS = 6
 BM3D_FP32 = 0
 BM3D_MIXED = 1
 BM3D_MIXED_MATCH = 2

@@ -1,11 +1,11 @@
 // Film stock parameter tables (data only).
 //
 // The values are the stock definitions of pichromatic/src/film/stock/*.rs
-// (e.g. ColorNeg200: color_neg_200.rs:67-209), written as plain records so
-// both the GPU library's host code and the oracle can build their own derived
-// tables (curve samples, capture LUTs, grain kappa) from the same inputs.
-// Spectral curves are given by their analytic form on the 16-sample grid
-// 400..700 nm (step 20 nm).
+// (e.g. ColorNeg200: color_neg_200.rs:67-209), written as plain records from
+// which the library's host code builds its derived tables (curve samples on
+// the spectral grid, capture LUTs, grain kappa).  Spectral curves are given by
+// their analytic form, so they can be sampled on any grid (the reference's 16
+// samples 400..700 nm, or 81 samples 380..780 nm).
 #pragma once
 
 namespace film_data {

@@ -352,6 +352,11 @@ Stage make_stage(const Table& t) {
     std::string a = str_param(t, "algorithm", "amaze", "Demosaic");  // rename_all = lowercase
     if (a == "amaze") s.algorithm = IMOPS_DEMOSAIC_AMAZE;
     else if (a == "markesteijn") s.algorithm = IMOPS_DEMOSAIC_MARKESTEIJN;
+    else if (a == "markesteijn3") {
+      const uint64_t passes = u64_param(t, "passes", 3, "Demosaic");
+      if (passes != 1 && passes != 3) throw Error{IMOPS_E_INVALID, "Invalid Demosaic config: passes must be 1 or 3"};
+      s.algorithm = passes == 3 ? IMOPS_DEMOSAIC_MARKESTEIJN3 : IMOPS_DEMOSAIC_MARKESTEIJN3_1PASS;
+    }
     else if (a == "fast") s.algorithm = IMOPS_DEMOSAIC_FAST;
     else if (a == "superfast") s.algorithm = IMOPS_DEMOSAIC_SUPERFAST;
     else if (a == "supersuperfast") s.algorithm = IMOPS_DEMOSAIC_SUPERSUPERFAST;

@@ -47,6 +47,11 @@ hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_
 // colour of CFA site (k/2, k%2), written in that order (a repeated colour keeps the later site).
 hipError_t preview_demosaic_launch(const float* in, int width, int height, int factor, const int cell[4], float* out,
                                    hipStream_t s);
+// Three-pass Markesteijn (markesteijn3.hip): passes 1 or 3; scratch of
+// markesteijn3_scratch_bytes.
+size_t markesteijn3_scratch_bytes(int width, int height, int passes);
+hipError_t markesteijn3_launch(const float* in, int width, int height, const CfaPattern& cfa, int passes,
+                               void* scratch, float* out, hipStream_t s);
 hipError_t markesteijn_launch(const float* in, int width, int height, const CfaPattern& cfa, float* out,
                               hipStream_t s);
 hipError_t pointwise_launch(float* rgb, size_t npix, const PointwiseProgram& prog, hipStream_t s);

@@ -459,6 +459,21 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
       for (int c = 0; c < 12; ++c) p.base[r][c] = cfa.pat[r][c];
     TimedRegion tr(ctx, "markesteijn");
     IMOPS_HIP_CHECK(markesteijn_launch(img->raw.as<float>(), (int)w, (int)h, p, img->rgb.as<float>(), ctx->stream));
+  } else if (algorithm == IMOPS_DEMOSAIC_MARKESTEIJN3 || algorithm == IMOPS_DEMOSAIC_MARKESTEIJN3_1PASS) {
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c)
+        if (cfa.pat[r][c] > 2) throw Error{IMOPS_E_UNSUPPORTED, "markesteijn3 needs an RGB pattern, got " + cfa.name};
+    if (w > (1u << 30) / 3 || h > (1u << 20)) throw Error{IMOPS_E_INVALID, "markesteijn3: image too large"};
+    CfaPattern p{};
+    p.w = cfa.width;
+    p.h = cfa.height;
+    for (int r = 0; r < 12; ++r)
+      for (int c = 0; c < 12; ++c) p.base[r][c] = cfa.pat[r][c];
+    const int passes = algorithm == IMOPS_DEMOSAIC_MARKESTEIJN3 ? 3 : 1;
+    ctx->scr[SCR_DEMOSAIC].reserve(markesteijn3_scratch_bytes((int)w, (int)h, passes));
+    TimedRegion tr(ctx, "markesteijn3");
+    IMOPS_HIP_CHECK(markesteijn3_launch(img->raw.as<float>(), (int)w, (int)h, p, passes, ctx->scr[SCR_DEMOSAIC].ptr,
+                                        img->rgb.as<float>(), ctx->stream));
   } else if (algorithm >= IMOPS_DEMOSAIC_FAST && algorithm <= IMOPS_DEMOSAIC_SUPERSUPERFAST) {
     const int f = algorithm == IMOPS_DEMOSAIC_FAST ? 2 : (algorithm == IMOPS_DEMOSAIC_SUPERFAST ? 4 : 8);
     const size_t nw = w / f, nh = h / f;

@@ -52,7 +52,7 @@ struct DevBuf {
 
 // Scratch buffers a context owns, by use.
 enum ScratchId : int { SCR_AMAZE = 0, SCR_STAGING, SCR_BM3D_0, SCR_BM3D_1, SCR_BM3D_2, SCR_BM3D_3, SCR_FILM_0,
-                       SCR_FILM_1, SCR_FILM_2, SCR_FILM_3, SCR_COUNT };
+                       SCR_FILM_1, SCR_FILM_2, SCR_FILM_3, SCR_DEMOSAIC, SCR_COUNT };
 
 namespace imops {
 // Optional per-region GPU timing: HIP events recorded around launches on the

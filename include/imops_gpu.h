@@ -57,6 +57,13 @@ extern "C" {
 #define IMOPS_DEMOSAIC_FAST 2
 #define IMOPS_DEMOSAIC_SUPERFAST 3
 #define IMOPS_DEMOSAIC_SUPERSUPERFAST 4
+/* The three-pass Markesteijn X-Trans demosaic (an extension: the reference's
+ * "markesteijn" is the 3x3 same-colour mean): directional green clamped to the
+ * local green range, green recomputed from the closer pixels, colour-difference
+ * red / blue, Y Pb Pr homogeneity selection; any CFA.  MARKESTEIJN3 runs the
+ * three passes (eight candidates), MARKESTEIJN3_1PASS one pass (four). */
+#define IMOPS_DEMOSAIC_MARKESTEIJN3 5
+#define IMOPS_DEMOSAIC_MARKESTEIJN3_1PASS 6
 
 typedef struct imops_ctx imops_ctx;
 typedef struct imops_image imops_image;

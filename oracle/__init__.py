@@ -7,7 +7,7 @@ baseline.  The product library never links or calls it.
 Functions and the reference code they restate:
 
 * ``amaze``        pichromatic/src/demosaic.rs:358-1379 (f32 exact order; fp64 variant for error statements)
-* ``markesteijn``  pichromatic/src/demosaic.rs:108-186; ``preview_demosaic`` demosaic.rs:189-354 (Fast/SuperFast/SuperSuperFast)
+* ``markesteijn``  pichromatic/src/demosaic.rs:108-186; ``markesteijn3`` the three-pass extension (markesteijn3.cpp); ``preview_demosaic`` demosaic.rs:189-354 (Fast/SuperFast/SuperSuperFast)
 * ``normalize_raw`` demosaic.rs:61-91 + (DN-black)/(white-black) scaling (rawler apply_scaling: parity unpinned)
 * ``camera_cst``   pichromatic/src/cst.rs:12-95 (colour-crate matrices: parity unpinned)
 * ``cst``, ``exp``, ``gamma``, ``cfa_coeffs``  cst.rs:6-10, exp.rs:4-11, tone_map.rs:38-49, cfa_coeffs.rs:4-12
@@ -134,6 +134,27 @@ def markesteijn(cfa_plane: np.ndarray, pattern: str, crop_origin=(0, 0)) -> np.n
     out = np.zeros((h, w, 3), np.float32)
     _rc(lib().oracle_markesteijn(_f(x), w, h, pattern.encode(), crop_origin[0], crop_origin[1], _f(out)),
         "markesteijn")
+    return out
+
+
+def markesteijn3(cfa_plane: np.ndarray, pattern: str, crop_origin=(0, 0), passes: int = 3,
+                 fp64: bool = False, plant: int = 0) -> np.ndarray:
+    """The three-pass Markesteijn X-Trans demosaic (the markesteijn3 extension);
+    `plant` selects a deliberate mistake (test hook, 0 = none)."""
+    x = np.ascontiguousarray(cfa_plane, np.float32)
+    h, w = x.shape
+    out = np.zeros((h, w, 3), np.float32)
+    L = lib()
+    L.oracle_markesteijn3.restype = C.c_int
+    L.oracle_markesteijn3.argtypes = [_fp, _sz, _sz, C.c_char_p, _sz, _sz, C.c_int, C.c_int, _fp]
+    L.oracle_markesteijn3_plant.restype = None
+    L.oracle_markesteijn3_plant.argtypes = [C.c_int]
+    L.oracle_markesteijn3_plant(plant)
+    try:
+        _rc(L.oracle_markesteijn3(_f(x), w, h, pattern.encode(), crop_origin[0], crop_origin[1], passes, int(fp64),
+                                  _f(out)), "markesteijn3")
+    finally:
+        L.oracle_markesteijn3_plant(0)
     return out
 
 

@@ -41,6 +41,18 @@ int oracle_markesteijn(const float* in, size_t w, size_t h, const char* cfa_patt
   return 0;
 }
 
+void oracle_markesteijn3_plant(int plant) { oracle::g_m3_plant = plant; }
+
+int oracle_markesteijn3(const float* in, size_t w, size_t h, const char* cfa_pattern, size_t crop_x, size_t crop_y,
+                        int passes, int fp64, float* out) {
+  Cfa base;
+  if (!base.parse(cfa_pattern) || (passes != 1 && passes != 3) || w == 0 || h == 0) return -1;
+  Cfa cfa = base.shift(crop_x, crop_y);
+  if (fp64) markesteijn3<double>(in, w, h, cfa, passes, out);
+  else markesteijn3<float>(in, w, h, cfa, passes, out);
+  return 0;
+}
+
 // Fast / SuperFast / SuperSuperFast (demosaic.rs:189-354): factor 2, 4 or 8; out (w/f) x (h/f) x 3.
 int oracle_preview_demosaic(const float* in, size_t w, size_t h, const char* cfa_pattern, size_t crop_x,
                             size_t crop_y, int factor, float* out) {

@@ -25,6 +25,10 @@ void exp_ev(float* rgb, size_t n, float ev);
 void gamma_encode(float* rgb, size_t n, float gamma);
 void cfa_coeffs(float* rgb, size_t n, const float wb[4]);
 void markesteijn(const float* in, size_t width, size_t height, const Cfa& cfa, float* out);
+// The three-pass Markesteijn extension (markesteijn3.cpp); passes 1 or 3.
+extern int g_m3_plant;  // test hook: a planted mistake (0 = none)
+template <class T>
+void markesteijn3(const float* in, size_t width, size_t height, const Cfa& cfa, int passes, float* out_rgb);
 
 // Test hooks of the AMaZE restatement (see amaze.cpp): trace receives 16
 // planes of 128 x 128 floats for the tile at (trace_top, trace_left).

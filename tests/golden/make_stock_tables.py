@@ -26,7 +26,7 @@ import sys
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parents[2]
-REF = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
+REF = Path(sys.argv[1]) if len(sys.argv) > 1 and __name__ == "__main__" else Path("/root/reference")
 STOCK_DIR = REF / "pichromatic" / "src" / "film" / "stock"
 OUT = ROOT / "oracle" / "oracle_stock_tables.hpp"
 
@@ -360,7 +360,8 @@ def stock_cpp(s) -> str:
     return "  {" + ",\n    ".join(parts) + "}"
 
 
-def main():
+def render() -> str:
+    """The header text generated from the reference sources."""
     stocks = []
     for name, fname in STOCKS:
         s = load_stock(STOCK_DIR / fname)
@@ -417,8 +418,12 @@ static const OStock kStocks[{len(stocks)}] = {{
 
 }  // namespace oracle_stocks
 """
-    OUT.write_text(text)
-    print(f"wrote {OUT} ({len(stocks)} stocks)")
+    return text
+
+
+def main():
+    OUT.write_text(render())
+    print(f"wrote {OUT}")
 
 
 if __name__ == "__main__":

@@ -58,14 +58,17 @@ def test_c2_headline_stage_list_on_a_crop_of_the_24mp_frame(gpu_ctx, cal, precis
     _check(got, crop, "RGGB", cal, precision, "amaze", f"C2 {precision}")
 
 
+@pytest.mark.parametrize("demosaic", ["markesteijn", "markesteijn3"])
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
-def test_c3_xtrans_stage_list_on_a_crop_of_the_26mp_frame(gpu_ctx, cal, precision):
+def test_c3_xtrans_stage_list_on_a_crop_of_the_26mp_frame(gpu_ctx, cal, precision, demosaic):
+    """Demosaic{markesteijn} (the reference's 3x3 mean) and {markesteijn3} (the
+    three-pass extension) on X-Trans, then the headline stage list."""
     raw = im.synth_raw_u16(bench.XT_W, bench.XT_H, bench.XTRANS, 3)
     # a crop at the origin keeps the 6x6 pattern's phase; a second one further in
     for y0, x0 in ((0, 0), (1800, 3000)):
         crop = np.ascontiguousarray(raw[y0:y0 + CH, x0:x0 + CW])
-        got = _gpu_chain(gpu_ctx, crop, bench.XTRANS, cal, precision, "markesteijn")
-        _check(got, crop, bench.XTRANS, cal, precision, "markesteijn", f"C3 {precision} crop ({x0},{y0})")
+        got = _gpu_chain(gpu_ctx, crop, bench.XTRANS, cal, precision, demosaic)
+        _check(got, crop, bench.XTRANS, cal, precision, demosaic, f"C3 {precision} {demosaic} crop ({x0},{y0})")
 
 
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])

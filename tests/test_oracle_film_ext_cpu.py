@@ -129,3 +129,17 @@ def test_halation_strength_zero_and_radius_mapping():
     _, wide = oracle.film(rgb, trace=True, halation_radius=60.0, halation_strength=0.15, **kw)
     assert np.all(on["latent"] >= off["latent"]) and np.any(on["latent"] > off["latent"])
     assert not np.array_equal(on["latent"], wide["latent"])
+
+
+def test_generated_stock_tables_are_current():
+    """oracle/oracle_stock_tables.hpp is what tests/golden/make_stock_tables.py makes from
+    the reference's stock sources (checked where the reference checkout is present)."""
+    import importlib.util
+    from pathlib import Path
+    root = Path(__file__).resolve().parent
+    spec = importlib.util.spec_from_file_location("make_stock_tables", root / "golden" / "make_stock_tables.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    if not gen.STOCK_DIR.is_dir():
+        pytest.skip("reference stock sources not present here")
+    assert gen.render() == gen.OUT.read_text()
