@@ -356,7 +356,8 @@ def main() -> int:
         cfg = im.parse_config(stages(precision))
 
         def step():
-            img.set_raw_u16(d_raw, "RGGB", BLACK, WHITE)
+            # the u16 frame is bound, not copied: AMaZE normalises it in its tile fill
+            img.bind_raw_u16(d_raw, "RGGB", BLACK, WHITE)
             im.run_pixel_pipeline(img, cfg)
 
         for _ in range(args.warmup):

@@ -142,6 +142,23 @@ class Image:
                                             float(black), float(white), cfa.encode()))
         return self
 
+    def bind_raw_u16(self, raw, cfa: str, black: float, white: float, crop=None) -> "Image":
+        """Binds a device u16 frame (a torch tensor kept alive by the caller until the
+        demosaic has run) without copying: AMaZE normalises it in its tile fill."""
+        on_dev, ptr, (h, w) = _buffer(raw, np.uint16)
+        if not on_dev:
+            raise ValueError("bind_raw_u16: the frame must be a device tensor")
+        cx, cy, cw, ch = crop if crop is not None else (0, 0, w, h)
+        self._bound = raw  # keeps the tensor alive while bound
+        check(lib().imops_image_bind_raw_u16(self.ctx.handle, self._h, ptr, w, h, cx, cy, cw, ch, float(black),
+                                             float(white), cfa.encode()))
+        return self
+
+    def crop_bayer_center(self, crop_factor: int) -> "Image":
+        """crop_bayer_center (extern_pipeline.rs:113-179) on the raw plane."""
+        check(lib().imops_crop_bayer_center(self.ctx.handle, self._h, int(crop_factor)))
+        return self
+
     def set_raw_f32(self, raw, cfa: str, crop_origin=(0, 0)) -> "Image":
         on_dev, ptr, (h, w) = _buffer(raw, np.float32)
         check(lib().imops_image_set_raw_f32(self.ctx.handle, self._h, ptr, on_dev, w, h, cfa.encode(),
@@ -379,6 +396,22 @@ def parse_config(text: str) -> PipelineConfig:
 def run_pixel_pipeline(image: Image, config: PipelineConfig) -> Image:
     check(lib().imops_pipeline_run(image.ctx.handle, config._h, image.handle))
     return image
+
+
+def dng_calibration_matrix(color_matrix_a=None, color_matrix_d65=None, color_matrix_other=None,
+                           wb_coeffs=None) -> np.ndarray:
+    """The calibration matrix parse_raw_image picks or interpolates (extern_pipeline.rs:254-270, 457-495)."""
+    def arg(m, k):
+        if m is None:
+            return None, None
+        a = np.ascontiguousarray(m, np.float32).ravel()
+        if a.size != k:
+            raise ValueError(f"expected {k} values")
+        return a, fptr(a)
+    keep = [arg(color_matrix_a, 9), arg(color_matrix_d65, 9), arg(color_matrix_other, 9), arg(wb_coeffs, 4)]
+    out = np.zeros(9, np.float32)
+    check(lib().imops_dng_calibration_matrix(keep[0][1], keep[1][1], keep[2][1], keep[3][1], fptr(out)))
+    return out
 
 
 def demosaic_amaze_noalloc(ctx: Context, cfa_plane_ptr: int, rgb_out_ptr: int, width: int, height: int,

@@ -72,6 +72,9 @@ SIGNATURES = {
     "imops_image_free": (_i, [_vp]),
     "imops_image_set_raw_u16": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _sz, _sz, _sz, _sz, _f, _f, C.c_char_p]),
     "imops_image_set_raw_f32": (_i, [_vp, _vp, _vp, _i, _sz, _sz, C.c_char_p, _sz, _sz]),
+    "imops_image_bind_raw_u16": (_i, [_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz, _sz, _f, _f, C.c_char_p]),
+    "imops_crop_bayer_center": (_i, [_vp, _vp, _sz]),
+    "imops_dng_calibration_matrix": (_i, [_fp, _fp, _fp, _fp, _fp]),
     "imops_image_set_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz, _sz, _i]),
     "imops_image_get_rgb_f32": (_i, [_vp, _vp, _vp, _i, _sz]),
     "imops_image_get_info": (_i, [_vp, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_i), C.POINTER(_i)]),

@@ -92,6 +92,8 @@ struct Ctx {
   __device__ float get_in(int r, int c) const {
     r = r < 0 ? 0 : (r > a.height - 1 ? a.height - 1 : r);
     c = c < 0 ? 0 : (c > a.width - 1 ? a.width - 1 : c);
+    if (a.in16)  // raw ingest fused into the tile fill: the normalisation of crop_and_normalize
+      return ((float)a.in16[(a.in16_y + (size_t)r) * a.in16_pitch + a.in16_x + (size_t)c] - a.black) / a.range;
     return a.in[(size_t)r * a.width + c];
   }
   __device__ float* P(int off) const { return s + off; }

@@ -11,7 +11,12 @@ namespace imops {
 constexpr size_t kAmazeSlotFloats = 11 * 16384 + 11 * 8192;
 
 struct AmazeArgs {
-  const float* in;   // normalised CFA plane, width*height
+  const float* in;   // normalised CFA plane, width*height (or null: read in16)
+  // u16 frame read directly by the tile fill: sample (r, c) of the crop window is
+  // (in16[(in16_y + r) * in16_pitch + in16_x + c] - black) / range
+  const uint16_t* in16;
+  size_t in16_pitch, in16_x, in16_y;
+  float black, range;
   float* out;        // interleaved RGB f32, width*height*3
   int width, height;
   int tiles_x, tiles_y;

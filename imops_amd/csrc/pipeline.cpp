@@ -559,6 +559,8 @@ struct BurstShape {
 };
 
 void run_burst_graphs(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* const* frames, size_t n) {
+  // bound u16 frames become f32 raw planes: the captured graphs read each frame's copy
+  for (size_t i = 0; i < n; ++i) materialize_raw(ctx, frames[i]);
   // every frame's state before the stage list runs (the eager first frames change theirs)
   std::vector<imops_image> pre(n);
   std::vector<BurstShape> shapes;

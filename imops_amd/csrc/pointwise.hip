@@ -26,6 +26,20 @@ __global__ void normalize_kernel(const uint16_t* __restrict__ raw, size_t full_w
   }
 }
 
+// crop_bayer_center (extern_pipeline.rs:113-179): destination (x, y) takes
+// source 2x2 block (x/2, y/2) * factor, site (x%2, y%2): every factor-th CFA
+// cell, so the Bayer phase is kept; samples past the plane read 0.
+__global__ void crop_bayer_center_kernel(const float* __restrict__ in, size_t width, size_t n_in, size_t factor,
+                                         size_t nw, size_t nh, float* __restrict__ out) {
+  const size_t n = nw * nh;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+    const size_t y = k / nw, x = k - y * nw;
+    const size_t sy = (y / 2) * factor * 2 + y % 2, sx = (x / 2) * factor * 2 + x % 2;
+    const size_t idx = sy * width + sx;
+    out[k] = idx < n_in ? in[idx] : 0.0f;
+  }
+}
+
 // demosaic.rs:108-186: mean of the same-colour pixels in the clipped 3x3
 // window (row-major order, centre skipped); native channel copied.
 __global__ void markesteijn_kernel(const float* __restrict__ in, int width, int height, CfaPattern cfa,
@@ -304,6 +318,13 @@ hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_
   const float range = white - black;
   hipLaunchKernelGGL(normalize_kernel, dim3(grid_for(cw * ch, 256)), dim3(256), 0, s, raw, full_w, cx, cy, cw, ch,
                      black, range, out);
+  return hipGetLastError();
+}
+
+hipError_t crop_bayer_center_launch(const float* in, size_t width, size_t height, size_t factor, size_t nw, size_t nh,
+                                   float* out, hipStream_t s) {
+  hipLaunchKernelGGL(crop_bayer_center_kernel, dim3(grid_for(nw * nh, 256)), dim3(256), 0, s, in, width,
+                     width * height, factor, nw, nh, out);
   return hipGetLastError();
 }
 

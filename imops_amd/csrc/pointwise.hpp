@@ -43,6 +43,9 @@ struct PointwiseProgram {
 
 hipError_t normalize_launch(const uint16_t* raw, size_t full_w, size_t cx, size_t cy, size_t cw, size_t ch,
                             float black, float white, float* out, hipStream_t s);
+// crop_bayer_center (extern_pipeline.rs:113-179): out is nw x nh.
+hipError_t crop_bayer_center_launch(const float* in, size_t width, size_t height, size_t factor, size_t nw, size_t nh,
+                                   float* out, hipStream_t s);
 // Fast / SuperFast / SuperSuperFast (demosaic.rs:189-354): output (w/f) x (h/f); cell[k] is the
 // colour of CFA site (k/2, k%2), written in that order (a repeated colour keeps the later site).
 hipError_t preview_demosaic_launch(const float* in, int width, int height, int factor, const int cell[4], float* out,

@@ -436,14 +436,33 @@ AmazeArgs amaze_args(const CfaTable& cfa, size_t w, size_t h, const float* in, f
   return a;
 }
 
+void materialize_raw(imops_ctx* ctx, imops_image* img) {
+  if (!img->raw16) return;
+  const size_t w = img->width, h = img->height;
+  img->raw.reserve(w * h * sizeof(float));
+  TimedRegion tr(ctx, "normalize");
+  IMOPS_HIP_CHECK(normalize_launch(img->raw16, img->raw16_pitch, img->raw16_x, img->raw16_y, w, h, img->raw16_black,
+                                   img->raw16_white, img->raw.as<float>(), ctx->stream));
+  img->raw16 = nullptr;
+}
+
 void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
   if (!img->has_raw) throw Error{IMOPS_E_STATE, "Demosaic: image has no raw CFA data"};
+  if (algorithm != IMOPS_DEMOSAIC_AMAZE) materialize_raw(ctx, img);
   if (!img->has_cfa) throw Error{IMOPS_E_STATE, "A CFA must be set to demosaic raw images"};
   const size_t w = img->width, h = img->height;
   const CfaTable cfa = img->cfa.shift(img->crop_x, img->crop_y);  // demosaic.rs:93-98
   img->rgb.reserve(w * h * 3 * sizeof(float));
   if (algorithm == IMOPS_DEMOSAIC_AMAZE) {
-    AmazeArgs a = amaze_args(cfa, w, h, img->raw.as<float>(), img->rgb.as<float>());
+    AmazeArgs a = amaze_args(cfa, w, h, img->raw16 ? nullptr : img->raw.as<float>(), img->rgb.as<float>());
+    if (img->raw16) {  // fused ingest: the tile fill reads and normalises the bound u16 frame
+      a.in16 = img->raw16;
+      a.in16_pitch = img->raw16_pitch;
+      a.in16_x = img->raw16_x;
+      a.in16_y = img->raw16_y;
+      a.black = img->raw16_black;
+      a.range = img->raw16_white - img->raw16_black;  // as normalize_launch forms it
+    }
     const size_t ntiles = (size_t)a.tiles_x * a.tiles_y;
     int grid = ctx->num_cus > 0 ? ctx->num_cus : 256;  // one 144 KiB-LDS workgroup per CU
     if ((size_t)grid > ntiles) grid = (int)ntiles;
@@ -489,6 +508,7 @@ void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm) {
     img->width = nw;
     img->height = nh;
     img->has_raw = false;  // the raw plane keeps the old size
+    img->raw16 = nullptr;
   } else {
     throw Error{IMOPS_E_INVALID, "unknown demosaic algorithm"};
   }
@@ -632,6 +652,7 @@ int imops_ctx_destroy(imops_ctx* ctx) {
       img->raw.release();
       img->rgb.release();
       img->tmp.release();
+      img->raw16 = nullptr;
       img->has_raw = img->has_rgb = false;
       img->ctx = nullptr;
     }
@@ -699,6 +720,7 @@ int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* sr
       dsrc = ctx->scr[SCR_STAGING].as<uint16_t>();
     }
     img->raw.reserve(cw * ch * sizeof(float));
+    img->raw16 = nullptr;
     TimedRegion tr(ctx, "normalize");
     IMOPS_HIP_CHECK(normalize_launch(dsrc, full_w, cx, cy, cw, ch, black, white, img->raw.as<float>(), ctx->stream));
     img->width = cw;
@@ -714,6 +736,104 @@ int imops_image_set_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* sr
   });
 }
 
+int imops_image_bind_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* dev_src, size_t full_w, size_t full_h,
+                             size_t cx, size_t cy, size_t cw, size_t ch, float black, float white,
+                             const char* cfa_pattern) {
+  return guarded([&] {
+    if (!ctx || !img || !dev_src || !cfa_pattern) throw Error{IMOPS_E_INVALID, "bind_raw_u16: null argument"};
+    if (!img->ctx) throw Error{IMOPS_E_STATE, "bind_raw_u16: the image's context was destroyed"};
+    if (cw == 0 || ch == 0 || cx + cw > full_w || cy + ch > full_h)
+      throw Error{IMOPS_E_INVALID, "bind_raw_u16: crop window outside the frame"};
+    if (!(white > black)) throw Error{IMOPS_E_INVALID, "bind_raw_u16: white level must exceed black level"};
+    CfaTable cfa;
+    if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, dev_src) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+      (void)hipGetLastError();
+      throw Error{IMOPS_E_INVALID, "bind_raw_u16: the frame must be device memory"};
+    }
+    img->raw16 = dev_src;
+    img->raw16_pitch = full_w;
+    img->raw16_x = cx;
+    img->raw16_y = cy;
+    img->raw16_black = black;
+    img->raw16_white = white;
+    img->width = cw;
+    img->height = ch;
+    img->cfa = cfa;
+    img->has_cfa = true;
+    img->crop_x = cx;
+    img->crop_y = cy;
+    img->has_raw = true;
+    img->has_rgb = false;
+    img->color_space = IMOPS_CS_NONE;
+  });
+}
+
+int imops_crop_bayer_center(imops_ctx* ctx, imops_image* img, size_t crop_factor) {
+  return guarded([&] {
+    if (!ctx || !img) throw Error{IMOPS_E_INVALID, "crop_bayer_center: null argument"};
+    if (!img->has_raw) throw Error{IMOPS_E_STATE, "crop_bayer_center: image has no raw CFA data"};
+    if (crop_factor <= 1) return;  // the image itself (extern_pipeline.rs:118-120)
+    materialize_raw(ctx, img);
+    const size_t w = img->width, h = img->height;
+    const size_t nw = (w / crop_factor) & ~(size_t)1, nh = (h / crop_factor) & ~(size_t)1;
+    DevBuf out;
+    out.reserve(std::max<size_t>(nw * nh, 1) * sizeof(float));
+    if (nw * nh > 0)
+      IMOPS_HIP_CHECK(crop_bayer_center_launch(img->raw.as<float>(), w, h, crop_factor, nw, nh, out.as<float>(),
+                                               ctx->stream));
+    IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    std::swap(img->raw, out);
+    out.release();
+    img->width = nw;  // the metadata (CFA, crop origin, white balance, ...) is kept
+    img->height = nh;
+    img->has_rgb = false;
+  });
+}
+
+// interpolate_matrices (extern_pipeline.rs:457-495) in the reference's f32 order.
+static void interpolate_matrices(const float* m1, const float* m2, const float wb[4], float out[9]) {
+  const float xyz_a[3] = {1.09850f, 1.0f, 0.35585f};
+  const float xyz_d65[3] = {0.95047f, 1.0f, 1.08883f};
+  float s1[3], s2[3];
+  for (int i = 0; i < 3; ++i) {
+    s1[i] = m1[i * 3] * xyz_a[0] + m1[i * 3 + 1] * xyz_a[1] + m1[i * 3 + 2] * xyz_a[2];
+    s2[i] = m2[i * 3] * xyz_d65[0] + m2[i * 3 + 1] * xyz_d65[1] + m2[i * 3 + 2] * xyz_d65[2];
+  }
+  const float ratio1 = s1[2] != 0.0f ? s1[0] / s1[2] : 1.0f;
+  const float ratio2 = s2[2] != 0.0f ? s2[0] / s2[2] : 1.0f;
+  const float ratio_s = (wb[0] > 0.0f && wb[2] > 0.0f) ? wb[2] / wb[0] : ratio2;
+  float w = 0.0f;
+  if (ratio1 != ratio2 && ratio1 > 0.0f && ratio2 > 0.0f && ratio_s > 0.0f) {
+    const float v = (std::log(ratio_s) - std::log(ratio2)) / (std::log(ratio1) - std::log(ratio2));
+    w = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);  // f32::clamp (NaN passes through)
+  }
+  for (int i = 0; i < 9; ++i) out[i] = w * m1[i] + (1.0f - w) * m2[i];
+}
+
+int imops_dng_calibration_matrix(const float* color_matrix_a, const float* color_matrix_d65,
+                                 const float* color_matrix_other, const float* wb_coeffs, float out[9]) {
+  return guarded([&] {
+    if (!out) throw Error{IMOPS_E_INVALID, "dng_calibration_matrix: null output"};
+    float wb[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 4 && wb_coeffs; ++i) wb[i] = std::isnan(wb_coeffs[i]) ? 0.0f : wb_coeffs[i];
+    const float* pick = nullptr;
+    if (color_matrix_a && color_matrix_d65) {
+      interpolate_matrices(color_matrix_a, color_matrix_d65, wb, out);
+      return;
+    }
+    if (color_matrix_a) pick = color_matrix_a;
+    else if (color_matrix_d65) pick = color_matrix_d65;
+    else if (color_matrix_other) pick = color_matrix_other;
+    if (pick) {
+      for (int i = 0; i < 9; ++i) out[i] = pick[i];
+    } else {
+      for (int i = 0; i < 9; ++i) out[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+    }
+  });
+}
+
 int imops_image_set_raw_f32(imops_ctx* ctx, imops_image* img, const float* src, int src_on_device, size_t w,
                             size_t h, const char* cfa_pattern, size_t crop_x, size_t crop_y) {
   return guarded([&] {
@@ -723,6 +843,7 @@ int imops_image_set_raw_f32(imops_ctx* ctx, imops_image* img, const float* src, 
     CfaTable cfa;
     if (!cfa.parse(cfa_pattern)) throw Error{IMOPS_E_INVALID, std::string("Unknown CFA pattern \"") + cfa_pattern + "\""};
     img->raw.reserve(w * h * sizeof(float));
+    img->raw16 = nullptr;
     IMOPS_HIP_CHECK(hipMemcpyAsync(img->raw.ptr, src, w * h * sizeof(float),
                                    src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
     IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -753,6 +874,7 @@ int imops_image_set_rgb_f32(imops_ctx* ctx, imops_image* img, const float* src, 
     img->height = h;
     img->has_rgb = true;
     img->has_raw = false;
+    img->raw16 = nullptr;
     img->color_space = cs;
   });
 }

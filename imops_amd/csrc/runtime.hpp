@@ -131,6 +131,12 @@ struct imops_image {
   size_t width = 0, height = 0;
   bool has_raw = false, has_rgb = false;
   imops::DevBuf raw;   // f32 CFA plane (width*height)
+  // A caller's device u16 frame bound without a copy (imops_image_bind_raw_u16):
+  // the raw plane is pending until a stage needs it as f32 (AMaZE reads the
+  // u16 samples directly).  Not part of copy_state_from: it is buffer state.
+  const uint16_t* raw16 = nullptr;
+  size_t raw16_pitch = 0, raw16_x = 0, raw16_y = 0;
+  float raw16_black = 0.0f, raw16_white = 1.0f;
   imops::DevBuf rgb;   // f32 interleaved RGB (width*height*3)
   imops::DevBuf tmp;   // second RGB buffer for out-of-place stages
   imops::CfaTable cfa;
@@ -185,6 +191,9 @@ namespace imops {
 // same result; the caller runs the chosen one afterwards.
 int tuned_choice(imops_ctx* ctx, const std::string& key, const std::vector<int>& candidates, int fallback,
                  const std::function<void(int)>& run, const std::function<void()>& reset);
+
+// Turns a bound u16 frame into the f32 raw plane (no-op when none is pending).
+void materialize_raw(imops_ctx* ctx, imops_image* img);
 
 // Stage implementations shared by the per-stage C calls and the pipeline runner.
 void stage_demosaic(imops_ctx* ctx, imops_image* img, int algorithm);

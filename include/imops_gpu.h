@@ -106,6 +106,33 @@ int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* 
 int imops_image_create(imops_ctx* ctx, imops_image** out);
 int imops_image_free(imops_image* img);
 
+/* Raw ingest fused into the demosaic load: binds the caller's DEVICE u16 frame
+ * (full_w x full_h, the crop window cx, cy, cw, ch, black / white levels as
+ * imops_image_set_raw_u16) without copying or normalising it.  AMaZE then
+ * reads and normalises the u16 samples in its tile fill (crop_and_normalize,
+ * demosaic.rs:61-91, and the CFA shift of the crop origin, demosaic.rs:93-98,
+ * folded into the load); any other consumer of the raw plane first converts
+ * it as imops_image_set_raw_u16 would.  The frame must stay valid and
+ * unchanged until the demosaic has run (synchronise the context). */
+int imops_image_bind_raw_u16(imops_ctx* ctx, imops_image* img, const uint16_t* dev_src, size_t full_w, size_t full_h,
+                             size_t cx, size_t cy, size_t cw, size_t ch, float black, float white,
+                             const char* cfa_pattern);
+
+/* crop_bayer_center (extern_pipeline.rs:113-179): keeps every crop_factor-th
+ * 2x2 CFA cell of the raw plane, so the frame becomes
+ * ((w / f) & ~1) x ((h / f) & ~1) with the same CFA phase; metadata kept.
+ * crop_factor <= 1 leaves the image unchanged. */
+int imops_crop_bayer_center(imops_ctx* ctx, imops_image* img, size_t crop_factor);
+
+/* The calibration matrix parse_raw_image chooses (extern_pipeline.rs:254-270,
+ * interpolate_matrices 457-495): with both the illuminant-A and D65 colour
+ * matrices, their mix weighted by where the white balance's B/R ratio falls
+ * between the two illuminants' (in log space, clamped to [0, 1]); else
+ * whichever exists (A, then D65, then any other), else the identity.  Any
+ * matrix may be NULL; wb_coeffs (4 values, NaN read as 0) may be NULL. */
+int imops_dng_calibration_matrix(const float* color_matrix_a, const float* color_matrix_d65,
+                                 const float* color_matrix_other, const float* wb_coeffs, float out[9]);
+
 /* Raw ingest: u16 sensor frame -> normalised f32 CFA plane of the crop window,
  * value (DN - black) / (white - black).  `src` is a host pointer, or a device
  * pointer when src_on_device != 0.  `cfa_pattern` is the sensor pattern string
