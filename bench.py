@@ -318,6 +318,9 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the X-Trans and burst workloads")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain f32 copy kernel of this size first (calibrates HBM byte counters)")
+    ap.add_argument("--profile-form", choices=("fp32", "mixed"), default=None,
+                    help="time only this form of the 24 MP chain (warm-up + steps) and print its stage times "
+                         "(the workload the kernel profiles are taken on)")
     ap.add_argument("--adjoint-only", type=int, default=0, metavar="STEPS",
                     help="run only the film-model adjoint workload for STEPS Adam steps and print its JSON")
     args = ap.parse_args()
@@ -387,6 +390,11 @@ def main() -> int:
               f"regions {json.dumps(regions)}", file=sys.stderr, flush=True)
         return ms, regions, extra, img.rgb()
 
+    if args.profile_form:
+        ms_p, regions_p, extra_p, _ = run_form(args.profile_form)
+        print(json.dumps({"form": args.profile_form, "ms_per_step": round(ms_p, 3), "steps": args.steps,
+                          "stage_ms": regions_p, **extra_p}), flush=True)
+        return 0
     ms, regions, _, out_fp32 = run_form("fp32")
     ms_mx, regions_mx, extra_mx, out_mixed = run_form("mixed")
     psnr_full = psnr(out_mixed, out_fp32)

@@ -29,6 +29,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bm3d.hpp"
 
@@ -73,23 +74,51 @@ __device__ __forceinline__ void load8g(const float* p, float* v) {
   v[4] = b[0]; v[5] = b[1]; v[6] = b[2]; v[7] = b[3];
 }
 
+// One channel's 8x8 patches of the candidate and the reference, every row
+// loaded before any is used (16 independent loads per patch in flight).
+__device__ __forceinline__ void load_pair(const float* plane, int W, int rx, int ry, int cx, int cy, float (&c)[8][8],
+                                          float (&r)[8][8]) {
+#pragma unroll
+  for (int dy = 0; dy < 8; ++dy) {
+    load8g(plane + (size_t)(cy + dy) * W + cx, c[dy]);
+    load8g(plane + (size_t)(ry + dy) * W + rx, r[dy]);
+  }
+}
+
 // The plain matcher's acceptance of one candidate (early exit per row, per-channel
-// stop thresholds) on the FP32 guide planes.
+// stop thresholds) on the FP32 guide planes.  kPreload: every row of a channel is
+// loaded before the sum starts (for callers outside the candidate sweeps, which
+// have the registers to spare); otherwise row by row.
+template <bool kPreload>
 __device__ bool exact_accept(const float* const g[3], int W, int rx, int ry, int cx, int cy, float thr) {
   float d[3] = {0.0f, 0.0f, 0.0f};
   float stop = thr;
   for (int ch = 0; ch < 3; ++ch) {
     float acc = 0.0f;
-    for (int dy = 0; dy < 8; ++dy) {
-      float c[8], r[8];
-      load8g(g[ch] + (size_t)(cy + dy) * W + cx, c);
-      load8g(g[ch] + (size_t)(ry + dy) * W + rx, r);
+    if (kPreload) {
+      float c[8][8], r[8][8];
+      load_pair(g[ch], W, rx, ry, cx, cy, c, r);
 #pragma unroll
-      for (int dx = 0; dx < 8; ++dx) {
-        const float diff = c[dx] - r[dx];
-        acc += diff * diff;
+      for (int dy = 0; dy < 8; ++dy) {
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) {
+          const float diff = c[dy][dx] - r[dy][dx];
+          acc += diff * diff;
+        }
+        if (acc > stop) break;
       }
-      if (acc > stop) break;
+    } else {
+      for (int dy = 0; dy < 8; ++dy) {
+        float c[8], r[8];
+        load8g(g[ch] + (size_t)(cy + dy) * W + cx, c);
+        load8g(g[ch] + (size_t)(ry + dy) * W + rx, r);
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) {
+          const float diff = c[dx] - r[dx];
+          acc += diff * diff;
+        }
+        if (acc > stop) break;
+      }
     }
     d[ch] = acc;
     if (ch == 0) {
@@ -109,17 +138,16 @@ __device__ float exact_total(const float* const g[3], int W, int rx, int ry, int
   float d[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
+    float c[8][8], r[8][8];
+    load_pair(g[ch], W, rx, ry, cx, cy, c, r);
     float acc = 0.0f;
-    for (int dy = 0; dy < 8; ++dy) {
-      float c[8], r[8];
-      load8g(g[ch] + (size_t)(cy + dy) * W + cx, c);
-      load8g(g[ch] + (size_t)(ry + dy) * W + rx, r);
+#pragma unroll
+    for (int dy = 0; dy < 8; ++dy)
 #pragma unroll
       for (int dx = 0; dx < 8; ++dx) {
-        const float diff = c[dx] - r[dx];
+        const float diff = c[dy][dx] - r[dy][dx];
         acc += diff * diff;
       }
-    }
     d[ch] = acc;
   }
   return d[0] + d[1] + d[2];
@@ -128,6 +156,9 @@ __device__ float exact_total(const float* const g[3], int W, int rx, int ry, int
 // Interval arithmetic below uses the hardware square root (<= 1 ulp); the
 // (1 +- 1e-6) factors absorb it.
 __device__ __forceinline__ float hsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// median of three (v_med3_i32)
+__device__ __forceinline__ int med3i(int a, int b, int c) { return max(min(a, b), min(max(a, b), c)); }
 
 __device__ __forceinline__ float wave_sum(float v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -158,8 +189,8 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   _Float16* c0 = reinterpret_cast<_Float16*>(smem + a.off_w16);
   _Float16* c1 = reinterpret_cast<_Float16*>(smem + a.off_w16b);
   float* nb = reinterpret_cast<float*>(smem + a.off_nb);
-  float* hs = reinterpret_cast<float*>(smem + a.off_w16b);   // setup scratch, before c1 is staged
-  float* tot = reinterpret_cast<float*>(smem + a.off_w16b);  // exact phase, after pass 2
+  float* hs = reinterpret_cast<float*>(smem + a.off_w16b);         // setup scratch, before c1 is staged
+  uint64_t* keys = reinterpret_cast<uint64_t*>(smem + a.off_w16b);  // exact phase, after pass 2: rank keys
   uint16_t* list = reinterpret_cast<uint16_t*>(smem + a.off_list);
   uint32_t* rejrow = reinterpret_cast<uint32_t*>(smem + a.off_list);  // pass 1: rejections per (reference, window row)
   int* lcnt = reinterpret_cast<int*>(smem + a.off_cnt);
@@ -168,7 +199,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   RefMeta* meta = reinterpret_cast<RefMeta*>(smem + a.off_meta);
   float* red = reinterpret_cast<float*>(smem + a.off_red);
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int q = lane >> 4, n = lane & 15;
 
   // Workgroups are dealt round-robin to the 8 XCDs: give each XCD a contiguous
@@ -191,13 +222,6 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
 
   // ---- shift = mean of the middle reference patch (per channel); norms = 8x8
   // box sums of the squared FP16 values; then the two FP16 copies
-  if (wave == 0) {
-    const int mx = (r0x + (nvx - 1) / 2) * st, my = (r0y + (nvy - 1) / 2) * st;
-    for (int ch = 0; ch < 3; ++ch) {
-      const float t = wave_sum(a.guide[ch][(size_t)(my + (lane >> 3)) * W + mx + (lane & 7)]);
-      if (lane == 0) red[ch] = t * (1.0f / 64.0f);
-    }
-  }
   for (int k = tid; k < XW * 16 * kRowWords; k += XT) rejrow[k] = 0u;
   if (tid < XW * 16) {
     lcnt[tid] = 0;
@@ -208,48 +232,107 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     meta[tid].g = 1;
     meta[tid].pad = 0;
   }
-  int* cap_next = reinterpret_cast<int*>(red + 4);  // next reference slot of the cap walk
-  if (tid == 0) *cap_next = 0;
-  __syncthreads();
-  const float shift[3] = {red[0], red[1], red[2]};
-  for (int row = wave; row < SY; row += XW) {
-    for (int xx = lane; xx < NX; xx += 64) {
-      float acc = 0.0f;
+  // every wave computes the shift itself (the same loads and sums, so the same values)
+  float shift[3];
+  {
+    const int mx = (r0x + (nvx - 1) / 2) * st, my = (r0y + (nvy - 1) / 2) * st;
+    const size_t at = (size_t)(my + (lane >> 3)) * W + mx + (lane & 7);
+    const float g0 = a.guide[0][at], g1 = a.guide[1][at], g2 = a.guide[2][at];
+    shift[0] = wave_sum(g0) * (1.0f / 64.0f);
+    shift[1] = wave_sum(g1) * (1.0f / 64.0f);
+    shift[2] = wave_sum(g2) * (1.0f / 64.0f);
+  }
+  // the FP16 copy c0, straight from the guide planes: the workgroup's threads
+  // take consecutive elements of the 3 x SY x SX region, 16 loads in flight each
+  {
+    const int tot_el = 3 * SY * SX;
+    for (int base = 0; base < tot_el; base += 16 * XT) {
+      float v[16];
+      int dst[16], chs[16];
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float* src = a.guide[ch] + (size_t)(Y0 + row) * W + X0 + xx;
-#pragma unroll
-        for (int dx = 0; dx < 8; ++dx) {
-          const float f = (float)(_Float16)(src[dx] - shift[ch]);
-          acc += f * f;
-        }
+      for (int u = 0; u < 16; ++u) {
+        const int e = base + u * XT + tid;
+        const int r = e / SX, xx = e - r * SX;
+        const int ch = r >= 2 * SY ? 2 : (r >= SY ? 1 : 0), yy = r - ch * SY;
+        chs[u] = ch;
+        dst[u] = e < tot_el ? ch * plane + yy * P + xx : -1;
+        v[u] = e < tot_el ? a.guide[ch][(size_t)(Y0 + yy) * W + X0 + xx] : 0.0f;
       }
-      hs[row * PN + xx] = acc;
+#pragma unroll
+      for (int u = 0; u < 16; ++u)
+        if (dst[u] >= 0) c0[dst[u]] = (_Float16)(v[u] - shift[chs[u]]);
     }
   }
   __syncthreads();
+  // norms: per pixel row, the sum over channels, then over 8 columns, of the squared
+  // FP16 values (read back from the copy) into the row-sum scratch hs (c1's area);
+  // a lane takes four adjacent columns from twelve loaded values per channel
+  const int ngx = (NX + 3) / 4;
+  for (int it = tid; it < SY * ngx; it += XT) {
+    const int row = it / ngx, x4 = 4 * (it - row * ngx);
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+      const _Float16* src = c0 + ch * plane + row * P + x4;  // x4 even: 4-byte aligned
+      float f[12];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        const uint32_t w2 = *reinterpret_cast<const uint32_t*>(src + 2 * k);
+        f[2 * k] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w2 & 0xffffu));
+        f[2 * k + 1] = (float)__builtin_bit_cast(_Float16, (uint16_t)(w2 >> 16));
+      }
+#pragma unroll
+      for (int o = 0; o < 4; ++o)
+#pragma unroll
+        for (int dx = 0; dx < 8; ++dx) acc[o] += f[o + dx] * f[o + dx];
+    }
+#pragma unroll
+    for (int o = 0; o < 4; ++o)
+      if (x4 + o < NX) hs[row * PN + x4 + o] = acc[o];
+  }
+  __syncthreads();
+  // 8-row sums: a thread walks down one column over a quarter of the rows
   float nbm = 0.0f;
-  for (int row = wave; row < NY; row += XW) {
-    for (int xx = lane; xx < NX; xx += 64) {
+  const int seg = (NY + 3) / 4;
+  for (int it = tid; it < 4 * NX; it += XT) {
+    const int xx = it % NX, r0 = (it / NX) * seg, r1 = min(r0 + seg, NY);
+    if (r0 >= r1) continue;
+    float win8[8];
+#pragma unroll
+    for (int dy = 0; dy < 7; ++dy) win8[dy] = hs[(r0 + dy) * PN + xx];
+    for (int row = r0; row < r1; ++row) {
+      win8[7] = hs[(row + 7) * PN + xx];
       float acc = 0.0f;
 #pragma unroll
-      for (int dy = 0; dy < 8; ++dy) acc += hs[(row + dy) * PN + xx];
+      for (int dy = 0; dy < 8; ++dy) acc += win8[dy];
       nb[row * PN + xx] = acc;
       nbm = fmaxf(nbm, acc);
+#pragma unroll
+      for (int dy = 0; dy < 7; ++dy) win8[dy] = win8[dy + 1];
     }
   }
   nbm = wave_max(nbm);
   if (lane == 0) red[3 * XW + wave] = nbm;
-  __syncthreads();  // hs is dead: stage both copies
-  for (int row = wave; row < 3 * SY; row += XW) {
-    const int ch = row / SY, yy = row - ch * SY;
-    const float* src = a.guide[ch] + (size_t)(Y0 + yy) * W + X0;
-    _Float16* d0 = c0 + ch * plane + yy * P;
-    _Float16* d1 = c1 + ch * plane + yy * P;
-    for (int xx = lane; xx < SX; xx += 64) {
-      const _Float16 v = (_Float16)(src[xx] - shift[ch]);
-      d0[xx] = v;
-      if (xx > 0) d1[xx - 1] = v;
+  __syncthreads();  // hs is dead: c1[x] = c0[x + 1], a dword (two halves) per lane
+  {
+    const int pw = P / 2;  // dwords per row (P and plane are even)
+    const int tot_dw = 3 * SY * pw;
+    for (int base = 0; base < tot_dw; base += 8 * XT) {
+      uint32_t lo[8], hi[8];
+      int at[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = base + u * XT + tid;
+        const int r = k / pw, c = k - r * pw;
+        const int ch = r >= 2 * SY ? 2 : (r >= SY ? 1 : 0), yy = r - ch * SY;
+        at[u] = k < tot_dw ? (ch * plane + yy * P) / 2 + c : -1;
+        const uint32_t* s0 = reinterpret_cast<const uint32_t*>(c0);
+        lo[u] = at[u] >= 0 ? s0[at[u]] : 0u;
+        hi[u] = at[u] >= 0 && c + 1 < pw ? s0[at[u] + 1] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (at[u] >= 0) reinterpret_cast<uint32_t*>(c1)[at[u]] = __builtin_amdgcn_alignbit(hi[u], lo[u], 16);
     }
   }
   __syncthreads();
@@ -270,7 +353,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   const int wy0 = max(ry - half, 0), wy1 = min(ry + half, hmax);
   int rxa[4], wx0[4], ew[4], ewv[4], limgeo[4], npos[4];
   bool vj[4];
-  float na[4], ra[4], asure[4], epsmax[4];
+  float na[4], ra[4], asure[4], arej[4], epsmax[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     rxa[j] = (wrx0 + j) * st;
@@ -288,6 +371,10 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     epsmax[j] = kEpsRel * (na[j] + nbmax) + kEpsAbs;
     const float root = hsqrt(thr / (1.0f + kSumRel)) * (1.0f - 1e-6f) - ra[j] - rbmax;
     asure[j] = root > 0.0f ? root * root * (1.0f - 1e-6f) - epsmax[j] : -kInf;
+    // d > arej: every candidate's lower bound exceeds the threshold (surely rejected;
+    // the per-pair classification would reject it too)
+    const float rfar = hsqrt(thr / ((1.0f - kSumRel) * (1.0f - 1e-6f))) * (1.0f + 1e-6f) + ra[j] + rbmax;
+    arej[j] = rfar * rfar * ((1.0f + 1e-5f) / ((1.0f - 1e-6f) * (1.0f - 1e-6f))) + epsmax[j];
   }
 
   int ux0 = 0, ux1 = 0, uy0 = 0, uy1 = -1;
@@ -318,121 +405,193 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     if (hi * hi * (1.0f + kSumRel) * (1.0f + 1e-6f) < thr) return true;
     const float lo = fmaxf(hsqrt(fmaxf(d - eps, 0.0f)) * (1.0f - 1e-6f) - r, 0.0f);
     if (lo * lo * (1.0f - kSumRel) * (1.0f - 1e-6f) > thr) return false;
-    return exact_accept(a.guide, W, rxa[j], ry, cx, cy, thr);
+    return exact_accept<false>(a.guide, W, rxa[j], ry, cx, cy, thr);
   };
 
-  // Candidate column of this lane in tile (cy, tc).  The lane order within a
-  // tile rotates with the row, so candidates along a vertical or diagonal
-  // structure spread over the lanes (each lane tracks its own best two).
-  auto col_of = [&](int cy, int tc) { return ux0 + 16 * tc + ((n + 7 * cy) & 15); };
-  // MFMA cross terms of the 16 references against the 16 candidates of tile
-  // (cy, tc).  Columns past the union are clamped to one of the same parity
-  // (their results are masked by the window tests).
-  // Tiles are visited row by row; the B fragments and norm of the next tile are
-  // loaded while the current one is evaluated.
-  struct TileLoad {
-    half8 B[6];
-    float nbv;
-  };
-  auto load_tile = [&](int cy, int tc, TileLoad& L) {
-    const int cx = col_of(cy, tc);
-    int cxl = cx;
-    if (cx > ux1) {
-      cxl = ux1 - ((cx - ux1) & 1);
-      if (cxl < X0) cxl += 2;  // one-column union: read past it (inside the pitch)
+  // ---- tile walk.  Lane n evaluates candidate column cx = ux0 + 16 tc + n in
+  // every row: tile columns tc outermost, rows innermost.  A column's window
+  // tests (xin) are fixed for the whole column; a row's tests reduce to its
+  // window row index rr = cy - wy0 against row counts, except in "slow" rows
+  // (a reference's own row, or the row where its cap falls mid-row), which
+  // take the scan-index tests.  Columns past the union are clamped to one of
+  // the same parity for the loads and fail every window test.
+  const int nrows = yvalid ? wy1 - wy0 + 1 : 0;
+  const int sr = ry - wy0;  // window row of the references themselves
+  int sx[4];                // window column of reference j itself
+#pragma unroll
+  for (int j = 0; j < 4; ++j) sx[j] = rxa[j] - wx0[j];
+  int cxc = 0;              // this lane's candidate column in the current tile column
+  int oxc[4];               // its window column, per reference
+  bool xin[4];
+  auto column = [&](int cx) {
+    cxc = cx;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      oxc[j] = cx - wx0[j];
+      xin[j] = (unsigned)oxc[j] < (unsigned)ewv[j];
     }
-    const int odd = (cxl - X0) & 1;
-    const _Float16* row = (odd ? c1 : c0) + (cy - Y0) * P + (cxl - X0 - odd);
+  };
+  // Rows of the window wholly before the cap position `lim` (scan index of
+  // the last capped candidate); the row holding it mid-row is slow.  Slow rows
+  // of the wave as a 128-bit map over the union rows (wave-uniform).
+  auto cap_rows = [&](const int* lim, int* full, uint64_t& slow_lo, uint64_t& slow_hi) {
+    uint32_t m[4] = {0u, 0u, 0u, 0u};
+    auto mark = [&](int rr) {
+      const int u = wy0 + rr - uy0;
+      m[u >> 5] |= 1u << (u & 31);
+    };
+    if (yvalid) mark(sr);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      full[j] = 0;
+      if (vj[j]) {
+        const bool all = lim[j] >= nrows * ew[j];
+        full[j] = all ? nrows : (lim[j] + 1) / ew[j];
+        if (!all && (lim[j] + 1) % ew[j] != 0) mark(full[j]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      for (int off = 1; off < 64; off <<= 1) m[k] |= (uint32_t)__shfl_xor((int)m[k], off, 64);
+    slow_lo = ((uint64_t)__builtin_amdgcn_readfirstlane(m[1]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[0]);
+    slow_hi = ((uint64_t)__builtin_amdgcn_readfirstlane(m[3]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[2]);
+  };
+  auto is_slow = [&](int cy, uint64_t lo, uint64_t hi) {
+    const int u = cy - uy0;
+    return (((u < 64 ? lo >> (u & 63) : hi >> (u & 63))) & 1ull) != 0;
+  };
+  // MFMA cross terms of the 16 references against the 16 candidates of one row
+  // of a tile column.  The row's B fragments start at `row` (this lane's
+  // column, copy c0 or c1 by parity).
+  auto load_b = [&](const _Float16* row, half8* B) {
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
       const int t = 4 * s + q, ch = t >> 3, dy = t & 7;
-      L.B[s] = load8a(row + ch * plane + dy * P);
+      B[s] = load8a(row + ch * plane + dy * P);
     }
-    L.nbv = nb[(cy - Y0) * PN + (min(cxl, X1) - X0)];
   };
-  auto mfma_tile = [&](const TileLoad& L) -> floatx4 {
+  auto mfma_row = [&](const half8* B) -> floatx4 {
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int s = 0; s < 6; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[s], L.B[s], acc, 0, 0, 0);
+    for (int s = 0; s < 6; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[s], B[s], acc, 0, 0, 0);
     return acc;
   };
-  // visits every tile (cy, tc) of the union in scan order with its cross
-  // terms; two load buffers alternate so tile t+1 loads while t is evaluated
-  auto sweep = [&](auto&& body) {
-    const int nt = (uy1 - uy0 + 1) * ntc;
-    TileLoad L0, L1;
-    int cy = uy0, tc = 0;  // tile t
-    auto next = [&](int& y, int& c) {
-      if (++c == ntc) {
-        c = 0;
-        ++y;
+  // Visits every tile of the union: on_column(cx) once per tile column, then
+  // on_tile(cy, acc, nbv) per row.  The products of row r + 1 are issued
+  // before row r is evaluated, and the loads of row r + 2 after them.
+  auto sweep = [&](auto&& on_column, auto&& on_tile) {
+    const int nr = uy1 - uy0 + 1;
+    for (int tc = 0; tc < ntc; ++tc) {
+      const int cx = ux0 + 16 * tc + n;
+      int cxl = cx;
+      if (cx > ux1) {
+        cxl = ux1 - ((cx - ux1) & 1);
+        if (cxl < X0) cxl += 2;  // one-column union: read past it (inside the pitch)
       }
-    };
-    load_tile(cy, tc, L0);
-    for (int t = 0; t < nt; t += 2) {
-      int cy1 = cy, tc1 = tc;
-      next(cy1, tc1);
-      if (t + 1 < nt) load_tile(cy1, tc1, L1);
-      {
-        const floatx4 acc = mfma_tile(L0);
-        body(cy, tc, col_of(cy, tc), acc, L0.nbv);
+      const int odd = (cxl - X0) & 1;
+      const _Float16* bp = (odd ? c1 : c0) + (uy0 - Y0) * P + (cxl - X0 - odd);
+      const float* np = nb + (uy0 - Y0) * PN + (min(cxl, X1) - X0);
+      on_column(cx);
+      half8 B[6];
+      load_b(bp, B);
+      float nb_next = np[0];
+      floatx4 acc_next = mfma_row(B);
+      if (nr > 1) load_b(bp + P, B);
+      for (int r = 0; r < nr; ++r) {
+        const floatx4 acc = acc_next;
+        const float nbv = nb_next;
+        if (r + 1 < nr) {
+          nb_next = np[(r + 1) * PN];
+          acc_next = mfma_row(B);
+          if (r + 2 < nr) load_b(bp + (r + 2) * P, B);
+        }
+        on_tile(uy0 + r, acc, nbv);
       }
-      if (t + 1 >= nt) break;
-      int cy2 = cy1, tc2 = tc1;
-      next(cy2, tc2);
-      if (t + 2 < nt) load_tile(cy2, tc2, L0);
-      {
-        const floatx4 acc = mfma_tile(L1);
-        body(cy1, tc1, col_of(cy1, tc1), acc, L1.nbv);
-      }
-      cy = cy2;
-      tc = tc2;
     }
   };
+  // distance of pair j from the cross term: (na + nb) - 2 a.b, one rounding after the sum
+  auto dist = [&](int j, float acc, float nbv) { return fmaf(-2.0f, acc, na[j] + nbv); };
 
   // ---- pass 1: rejections, and per lane the two smallest d among accepted
-  // candidates inside the geometric cap.  Pairs that are surely accepted and
-  // cannot enter a lane's two smallest take no further work.
-  float b1d[4], b2d[4];
-  int rejl[4];
-  bool rejpre[4];
+  // candidates inside the geometric cap.  Pairs that are surely accepted take
+  // no further work; one branch per tile handles the rest.
+  // The two smallest are tracked on the distances' bit patterns in integer
+  // order (= float order for d >= 0; among negative d, whose tracked order is
+  // immaterial, reversed), b1 <= b2.  Every fourth row the trackers rotate
+  // within the 16-lane group, so candidates along a vertical structure spread
+  // over several lanes' pairs.
+  int b1[4], b2[4];
+  constexpr int kInfBits = 0x7f800000;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    b1d[j] = b2d[j] = kInf;
-    rejl[j] = 0;
-    rejpre[j] = false;
+    b1[j] = b2[j] = kInfBits;
   }
-  // the lane's two smallest distances (min / max keep them in order)
   auto track = [&](int j, bool on, float d) {
-    const float dv = on ? d : kInf;
-    b2d[j] = fminf(b2d[j], fmaxf(b1d[j], dv));
-    b1d[j] = fminf(b1d[j], dv);
+    const int dv = on ? __float_as_int(d) : kInfBits;
+    b2[j] = med3i(b1[j], b2[j], dv);
+    b1[j] = min(b1[j], dv);
   };
+  int full1[4] = {0, 0, 0, 0};  // window rows wholly before the geometric cap
   if (wave_active) {
-    sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
-      // the union's columns past ux1 fail every reference's column test
-      const bool yin = yvalid & (cy >= wy0) & (cy <= wy1);
-      const bool selfrow = cy == ry;
+    uint64_t slow_lo, slow_hi;
+    cap_rows(limgeo, full1, slow_lo, slow_hi);
+    // kSlow: the row holds a reference's own position or a cap mid-row
+    auto tile = [&](auto slow_tag, int cy, const floatx4& acc, float nbv) {
+      constexpr bool kSlow = decltype(slow_tag)::value;
+      const unsigned rr = (unsigned)(cy - wy0);
+      const bool yin = rr < (unsigned)nrows;
+      bool amb = false;
+      float d[4];
+      bool win[4], pre[4], sure[4], rej[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        // the cheap tests combine as masks (no branches); only the rare
-        // classification below branches
-        const int ox = cx - wx0[j];
-        const bool inw = yin & ((unsigned)ox < (unsigned)ewv[j]) & !(selfrow & (cx == rxa[j]));
-        const float d = (na[j] + nbv) - 2.0f * acc[j];
-        const bool pre = (int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limgeo[j];
-        const bool sure = d <= asure[j];
-        track(j, inw & sure & pre, d);
-        if (inw && !sure) {  // rare: the threshold is within reach
-          const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
-          const bool ok = classify(j, d, nbv, rb, cx, cy);
-          track(j, ok && pre, d);
-          if (!ok) {
-            ++rejl[j];
-            rejpre[j] = rejpre[j] || pre;
-            const int rr = cy - wy0;  // two 16-bit row counters per word
-            atomicAdd(&rejrow[(wave * 16 + 4 * q + j) * kRowWords + (rr >> 1)], (rr & 1) ? 65536u : 1u);
+        d[j] = dist(j, acc[j], nbv);
+        sure[j] = d[j] <= asure[j];
+        rej[j] = d[j] > arej[j];
+        if (kSlow) {
+          win[j] = xin[j] & yin & !(((int)rr == sr) & (oxc[j] == sx[j]));
+          pre[j] = win[j] & ((int)rr * ew[j] + oxc[j] <= limgeo[j]);
+        } else {
+          win[j] = xin[j] & yin;
+          pre[j] = xin[j] & (rr < (unsigned)full1[j]);
+        }
+        track(j, pre[j] & sure[j], d[j]);
+        amb |= win[j] & !sure[j] & !rej[j];
+      }
+      // surely rejected pairs: lane n < 4 of each quadrant adds the quadrant's
+      // count for reference (q, n) to its row counter (two 16-bit row counters per word)
+      bool r[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) r[j] = win[j] & rej[j];
+      if (__ballot(r[0] | r[1] | r[2] | r[3])) {
+        uint64_t mr[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mr[j] = __ballot(r[j]);
+        const uint64_t msel = n == 0 ? mr[0] : (n == 1 ? mr[1] : (n == 2 ? mr[2] : mr[3]));
+        const uint32_t part = (uint32_t)(msel >> (16 * q)) & 0xffffu;
+        if (n < 4 && part)
+          atomicAdd(&rejrow[(wave * 16 + 4 * q + n) * kRowWords + (rr >> 1)], (uint32_t)__popc(part) << (16 * (rr & 1)));
+      }
+      if (amb) {  // the threshold is within reach of some pair
+        const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (win[j] && !sure[j] && !rej[j]) {
+            const bool ok = classify(j, d[j], nbv, rb, cxc, cy);
+            track(j, ok && pre[j], d[j]);
+            if (!ok) atomicAdd(&rejrow[(wave * 16 + 4 * q + j) * kRowWords + (rr >> 1)], (rr & 1) ? 65536u : 1u);
           }
+        }
+      }
+    };
+    sweep(column, [&](int cy, const floatx4& acc, float nbv) {
+      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, cy, acc, nbv);
+      else tile(std::false_type{}, cy, acc, nbv);
+      if ((cy - uy0) % 4 == 3) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          b1[j] = __builtin_amdgcn_update_dpp(0, b1[j], 0x127, 0xf, 0xf, false);  // row_ror:7
+          b2[j] = __builtin_amdgcn_update_dpp(0, b2[j], 0x127, 0xf, 0xf, false);
         }
       }
     });
@@ -442,8 +601,18 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   bool anyrej[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    nacc[j] = npos[j] - group_sum(rejl[j]);
-    anyrej[j] = (__ballot(rejpre[j]) >> (16 * q)) & 0xffffull;
+    // rejections of reference (q, j): its row counters, lane n reads words n and n + 16
+    const uint32_t* rc = rejrow + (wave * 16 + 4 * q + j) * kRowWords;
+    const uint32_t w0 = rc[n], w1 = n + 16 < kRowWords ? rc[n + 16] : 0u;
+    const int rj = (int)((w0 & 0xffffu) + (w0 >> 16) + (w1 & 0xffffu) + (w1 >> 16));
+    nacc[j] = npos[j] - group_sum(rj);
+    // rejections up to the row holding the geometric cap (all of that row: a
+    // reference flagged without need gets the geometric cap from the walk)
+    const int cr = min(full1[j], (yvalid ? wy1 - wy0 + 1 : 0) - 1);
+    auto upto = [&](uint32_t w, int row) {
+      return (row <= cr ? (int)(w & 0xffffu) : 0) + (row + 1 <= cr ? (int)(w >> 16) : 0);
+    };
+    anyrej[j] = group_sum(upto(w0, 2 * n) + (n + 16 < kRowWords ? upto(w1, 2 * n + 32) : 0)) > 0;
   }
   if (wave_active && n == 0) {
 #pragma unroll
@@ -457,38 +626,75 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     }
   }
   __syncthreads();
-  // ---- the cap when candidates before the geometric cap were rejected: walk
-  // the per-row rejection counts to the row holding the cap-th acceptance, then
-  // settle that row with the reference's own acceptance test (lane = column).
-  // The workgroup's waves take the 64 reference slots from a shared counter, so
-  // a sub-block with many such references does not hold the others at the barrier.
-  for (;;) {
-    int slot = 0;
-    if (lane == 0) slot = atomicAdd(cap_next, 1);
-    slot = __builtin_amdgcn_readfirstlane(__shfl(slot, 0, 64));
-    if (slot >= XW * 16) break;
-    if (!meta[slot].pad) continue;  // uniform: every lane reads the same flag
-    {
-      const int m = slot & 15, ow = slot >> 4;  // reference m of wave ow's sub-block
-      const int mrx = (r0x + 4 * (ow & 1) + (m & 3)) * st, mry = (r0y + 4 * (ow >> 1) + (m >> 2)) * st;
+  // ---- the cap when candidates before the geometric cap were rejected.  Each
+  // wave settles its own sub-block's references: a prefix sum over the window
+  // rows (lane = row) of the accepted counts finds the row holding the cap-th
+  // acceptance; that row's candidates are re-evaluated from the same products
+  // as pass 1 (same operands, so the same distances and classifications), the
+  // rare undecided pairs by the reference's own test.
+  if (wave_active) {
+    const bool padl = lane < 16 && meta[wave * 16 + lane].pad;
+    uint64_t padm = __ballot(padl);
+    while (padm) {
+      const int m = __builtin_ctzll(padm);  // reference m = (row m >> 2, column m & 3) of the sub-block
+      padm &= padm - 1;
+      const int mq = m >> 2, mj = m & 3, slot = wave * 16 + m;
+      const int mrx = (wrx0 + mj) * st, mry = (wry0 + mq) * st;
       const int mx0 = max(mrx - half, 0), mw = min(mrx + half, wmax) - mx0 + 1;
       const int my0 = max(mry - half, 0), mh = min(mry + half, hmax) - my0 + 1;
-      int seen = 0, row = 0;
-      for (; row < mh; ++row) {
-        const int inrow = mw - (my0 + row == mry ? 1 : 0) - (int)((rejrow[slot * kRowWords + (row >> 1)] >> (16 * (row & 1))) & 0xffffu);
-        if (seen + inrow >= cap) break;
-        seen += inrow;
+      auto rejs = [&](int row) { return (int)((rejrow[slot * kRowWords + (row >> 1)] >> (16 * (row & 1))) & 0xffffu); };
+      int inrow = 0;
+      if (lane < mh) inrow = mw - (my0 + lane == mry ? 1 : 0) - rejs(lane);
+      int pref = inrow;  // inclusive prefix over rows
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(pref, off, 64);
+        if (lane >= off) pref += v;
       }
-      // row < mh: the cap-th acceptance lies in this row; a row without rejections
-      // accepts every candidate, otherwise the reference's own test settles it
-      const int cy = my0 + row, cx = mx0 + lane;
-      const bool row_rej = row < mh && ((rejrow[slot * kRowWords + (row >> 1)] >> (16 * (row & 1))) & 0xffffu) != 0;
-      bool ok = false;
-      if (row < mh && lane < mw && !(cy == mry && cx == mrx))
-        ok = row_rej ? exact_accept(a.guide, W, mrx, mry, cx, cy, thr) : true;
-      const unsigned long long bal = __ballot(ok);
-      const int before = seen + __popcll(bal & ((1ull << lane) - 1ull));
-      if (ok && before == cap - 1) meta[slot].limit = row * mw + lane;
+      const uint64_t reach = __ballot(lane < mh && pref >= cap);
+      if (!reach) continue;  // fewer than cap acceptances (not the case for a flagged reference)
+      const int row = __builtin_ctzll(reach);
+      const int seen = __shfl(pref - inrow, row, 64);  // acceptances before the row
+      const int cy = my0 + row;
+      // accepted candidates of the row as a column mask (bit c = column mx0 + c)
+      uint64_t acc_cols = 0;
+      if (rejs(row) == 0) {
+        acc_cols = mw >= 64 ? ~0ull : ((1ull << mw) - 1ull);
+      } else {
+        auto eval = [&](auto jtag) {
+          constexpr int J = decltype(jtag)::value;
+          for (int tt = 0; tt * 16 < mw; ++tt) {
+            const int cx = mx0 + 16 * tt + n;
+            int cxl = cx;
+            if (cx > ux1) {
+              cxl = ux1 - ((cx - ux1) & 1);
+              if (cxl < X0) cxl += 2;
+            }
+            const int odd = (cxl - X0) & 1;
+            half8 B[6];
+            load_b((odd ? c1 : c0) + (cy - Y0) * P + (cxl - X0 - odd), B);
+            const float nbv = nb[(cy - Y0) * PN + (min(cxl, X1) - X0)];
+            const floatx4 acc = mfma_row(B);
+            bool ok = false;
+            if (q == mq && 16 * tt + n < mw) {
+              const float d = dist(J, acc[J], nbv);
+              if (d <= asure[J]) ok = true;
+              else if (d <= arej[J]) ok = classify(J, d, nbv, kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f, cx, cy);
+            }
+            acc_cols |= ((__ballot(ok) >> (16 * mq)) & 0xffffull) << (16 * tt);
+          }
+        };
+        switch (mj) {
+          case 0: eval(std::integral_constant<int, 0>{}); break;
+          case 1: eval(std::integral_constant<int, 1>{}); break;
+          case 2: eval(std::integral_constant<int, 2>{}); break;
+          default: eval(std::integral_constant<int, 3>{}); break;
+        }
+      }
+      if (cy == mry) acc_cols &= ~(1ull << (mrx - mx0));  // the reference itself
+      // lane c: column c holds the (cap - seen)-th acceptance of the row
+      const bool mine = lane < mw && ((acc_cols >> lane) & 1ull);
+      const int before = seen + __popcll(acc_cols & ((1ull << lane) - 1ull));
+      if (mine && before == cap - 1) meta[slot].limit = row * mw + lane;
     }
   }
   // ---- T: K-th smallest upper bound among the tracked candidates.  The bound
@@ -500,16 +706,16 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     for (int j = 0; j < 4; ++j) {
       float u1 = kInf, u2 = kInf;
       // upper bounds with the block's largest candidate norm (each term grows with the norm)
-      if (b1d[j] != kInf) {
-        const float f = hsqrt(fmaxf(b1d[j] + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
+      const float e1 = __int_as_float(b1[j]), e2 = __int_as_float(b2[j]);
+      if (b1[j] != kInfBits) {
+        const float f = hsqrt(fmaxf(e1 + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
         u1 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
       }
-      if (b2d[j] != kInf) {
-        const float f = hsqrt(fmaxf(b2d[j] + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
+      if (b2[j] != kInfBits) {
+        const float f = hsqrt(fmaxf(e2 + epsmax[j], 0.0f)) * (1.0f + 1e-6f) + ra[j] + rbmax;
         u2 = f * f * ((1.0f + kSumRel) * (1.0f + 1e-6f));
       }
       int s1 = 0, s2 = 0;
-      const float e1 = b1d[j], e2 = b2d[j];
       for (int k = 0; k < 16; ++k) {
         const float p1 = __shfl(e1, 16 * q + k, 64), p2 = __shfl(e2, 16 * q + k, 64);
         s1 += (p1 < e1 || (p1 == e1 && k < n)) + (p2 < e1 || (p2 == e1 && k < n));
@@ -552,33 +758,75 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         dA[j] = f * f * (1.0f + 1e-5f) + epsmax[j];
       }
     }
-    sweep([&](int cy, int tc, int cx, const floatx4& acc, float nbv) {
-      const bool yin = yvalid & (cy >= wy0) & (cy <= wy1);
-      const bool selfrow = cy == ry;
+    // list lengths of reference (q, j), the same in the quadrant's 16 lanes;
+    // a selected lane's slot is the count plus the selected lanes below it in the quadrant
+    int cntA[4] = {0, 0, 0, 0}, cntB[4] = {0, 0, 0, 0};
+    const uint64_t qmask = 0xffffull << (16 * q);
+    auto append = [&](bool sel, int& cnt, uint16_t* lst, int capn, uint16_t li) {
+      const uint64_t m = __ballot(sel);
+      if (m) {
+        const uint64_t mq = m & qmask;
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mq >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mq, 0u));
+        const int pos = cnt + below;
+        if (sel && pos < capn) lst[pos] = li;
+        cnt += __popcll(mq);
+      }
+    };
+    // capped candidates: the window rows wholly before the cap position, and
+    // slow rows (own row, cap mid-row) by scan index
+    int full[4];
+    float dAB[4];
+    uint64_t slow_lo, slow_hi;
+    cap_rows(limit, full, slow_lo, slow_hi);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dAB[j] = fmaxf(dA[j], dK[j]);
+    auto tile = [&](auto slow_tag, int cy, const floatx4& acc, float nbv) {
+      constexpr bool kSlow = decltype(slow_tag)::value;
+      const unsigned rr = (unsigned)(cy - wy0);
+      float d[4];
+      bool cand[4];
+      bool any = false;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int ox = cx - wx0[j];
-        const bool inw = yin & ((unsigned)ox < (unsigned)ewv[j]) & !(selfrow & (cx == rxa[j]));
-        const float d = (na[j] + nbv) - 2.0f * acc[j];
-        const bool inA = d <= dA[j];  // lower bound may reach T
-        const bool inB = d <= dK[j];  // approximate top K
-        const bool cand = inw & (inA | inB) & ((int)__umul24((unsigned)(cy - wy0), (unsigned)ew[j]) + ox <= limit[j]);
-        if (cand && (d <= asure[j] || classify(j, d, nbv, kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f, cx, cy))) {
+        d[j] = dist(j, acc[j], nbv);
+        bool pre;
+        if (kSlow) {
+          const bool win = xin[j] & (rr < (unsigned)nrows) & !(((int)rr == sr) & (oxc[j] == sx[j]));
+          pre = win & ((int)rr * ew[j] + oxc[j] <= limit[j]);
+        } else {
+          pre = xin[j] & (rr < (unsigned)full[j]);
+        }
+        cand[j] = pre & (d[j] <= dAB[j]);  // lower bound may reach T, or approximate top K
+        any |= cand[j];
+      }
+      // wave-uniform: every lane of a quadrant takes part in the appends, so the
+      // quadrant's list lengths stay equal in its 16 lanes
+      if (__ballot(any)) {
+        const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
+        const uint16_t li = (uint16_t)((cy - uy0) * UW + (cxc - ux0));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          bool ok = cand[j] && d[j] <= asure[j];
+          if (cand[j] && !ok && d[j] <= arej[j]) ok = classify(j, d[j], nbv, rb, cxc, cy);
           const int slot = wave * 16 + 4 * q + j;
-          const uint16_t li = (uint16_t)((cy - uy0) * UW + (cx - ux0));
-          if (inA) {
-            const int pos = atomicAdd(&lcnt[slot], 1);
-            if (pos < kListCap) list[slot * kListCap + pos] = li;
-          }
-          if (inB) {
-            const int pos = atomicAdd(&lcntb[slot], 1);
-            if (pos < kListCapB) listb[slot * kListCapB + pos] = li;
-          }
+          append(ok && d[j] <= dA[j], cntA[j], list + slot * kListCap, kListCap, li);
+          append(ok && d[j] <= dK[j], cntB[j], listb + slot * kListCapB, kListCapB, li);
         }
       }
+    };
+    sweep(column, [&](int cy, const floatx4& acc, float nbv) {
+      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, cy, acc, nbv);
+      else tile(std::false_type{}, cy, acc, nbv);
     });
+    if (n == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        lcnt[wave * 16 + 4 * q + j] = cntA[j];
+        lcntb[wave * 16 + 4 * q + j] = cntB[j];
+      }
+    }
   }
-  __syncthreads();  // c1 is dead: its space holds the exact distances
+  __syncthreads();  // c1 is dead: its space holds the rank keys
 
   // ---- exact re-rank of the lists; entries of the wave's 16 references are
   // spread over the 64 lanes.  A reference uses its exact-band list when that
@@ -623,9 +871,14 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       if (e < total) {
         const int slot = wave * 16 + rm;
         const int li = ua ? list[slot * kListCap + idx] : listb[slot * kListCapB + idx];
-        const int py = uy0 + li / UW, px = ux0 + li % UW;
+        const int py = min(uy0 + li / UW, hmax), px = min(ux0 + li % UW, wmax);  // inside the image whatever the entry
         const int rrx = (wrx0 + (rm & 3)) * st, rry = (wry0 + (rm >> 2)) * st;
-        tot[slot * kListCap + idx] = exact_total(a.guide, W, rrx, rry, px, py);
+        const float t = exact_total(a.guide, W, rrx, rry, px, py);
+        // rank key (distance, scan index): a sum of squares is >= +0, whose bit
+        // patterns order as the values
+        const int ex0 = max(rrx - half, 0), ex1 = min(rrx + half, wmax), ey0 = max(rry - half, 0);
+        const uint32_t scan = (uint32_t)((py - ey0) * (ex1 - ex0 + 1) + (px - ex0));
+        keys[slot * kListCap + idx] = ((uint64_t)__float_as_uint(t) << 32) | scan;
       }
     }
   }
@@ -639,19 +892,12 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         const int slot = wave * 16 + rm;
         const uint16_t* lst = ua ? list + slot * kListCap : listb + slot * kListCapB;
         const int li = lst[idx];
-        const int py = uy0 + li / UW, px = ux0 + li % UW;
-        const float t = tot[slot * kListCap + idx];
-        const int rrx = (wrx0 + (rm & 3)) * st, rry = (wry0 + (rm >> 2)) * st;
-        const int ex0 = max(rrx - half, 0), ex1 = min(rrx + half, wmax), ey0 = max(rry - half, 0);
-        const int eww = ex1 - ex0 + 1;
-        const int myscan = (py - ey0) * eww + (px - ex0);
+        const int py = min(uy0 + li / UW, hmax), px = min(ux0 + li % UW, wmax);  // inside the image whatever the entry
+        // rank by (distance, scan order)
+        const uint64_t* kk = keys + slot * kListCap;
+        const uint64_t mine = kk[idx];
         int rank = 0;
-        for (int k = 0; k < nm; ++k) {
-          const int lk = lst[k];
-          const float tk = tot[slot * kListCap + k];
-          const int sk = (uy0 + lk / UW - ey0) * eww + (ux0 + lk % UW - ex0);
-          rank += (tk < t || (tk == t && sk < myscan)) ? 1 : 0;
-        }
+        for (int k = 0; k < nm; ++k) rank += kk[k] < mine ? 1 : 0;
         const int g = meta[slot].g;
         const int ref_id = (wry0 + (rm >> 2)) * a.refs_x + wrx0 + (rm & 3);
         if (rank < g - 1) a.groups[(size_t)ref_id * 17 + 2 + rank] = (uint32_t)px | ((uint32_t)py << 16);
@@ -937,13 +1183,12 @@ Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window) {
   a.pitch = (a.spx + 1 + 1) & ~1;  // even, with room for reads one column past the staged span
   auto up16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
   const size_t copy = (size_t)3 * a.spy * a.pitch * 2;
-  const size_t hs = (size_t)a.spy * (a.spx - 7) * 4;
-  const size_t tot = (size_t)XW * 16 * kListCap * 4;
+  const size_t tot = (size_t)XW * 16 * kListCap * 8;  // rank keys of the exact phase
   size_t off = 0;
   a.off_w16 = 0;
   off = up16(copy);
-  a.off_w16b = (int)off;  // second copy; also the setup scratch and the exact distances
-  off = up16(off + std::max(copy, std::max(hs, tot)));
+  a.off_w16b = (int)off;  // second copy; also the setup row sums and the exact-phase rank keys
+  off = up16(off + std::max(copy, tot));
   a.off_nb = (int)off;
   off = up16(off + (size_t)(a.spy - 7) * (a.spx - 7) * 4);
   a.off_list = (int)off;
@@ -955,8 +1200,8 @@ Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window) {
   a.off_red = (int)off;
   off = up16(off + 4 * XW * 4);
   a.smem = (int)off;
-  // tile columns are tracked in a 32-bit mask (8 tiles of 16 candidates);
-  // list entries index the wave's candidate union in 16 bits
+  // the union's rows are tracked in a 128-bit map; list entries index the
+  // wave's candidate union in 16 bits
   const int uw = 3 * stride + 2 * half + 1;
   a.max_tiles_ok = uw <= 128 && (size_t)uw * uw < 65536;
   return a;
