@@ -775,13 +775,15 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     // capped candidates: the window rows wholly before the cap position, and
     // slow rows (own row, cap mid-row) by scan index
     int full[4];
-    float dAB[4];
     uint64_t slow_lo, slow_hi;
     cap_rows(limit, full, slow_lo, slow_hi);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) dAB[j] = fmaxf(dA[j], dK[j]);
-    auto tile = [&](auto slow_tag, int cy, const floatx4& acc, float nbv) {
-      constexpr bool kSlow = decltype(slow_tag)::value;
+    // The approximate top-K criterion (d <= dK) selects a subset of the exact
+    // band (dK <= dA), and its list is only used where the band's list
+    // overflowed: the sweep collects the band (kB false), and a second sweep
+    // collects the top-K list of such references (kB true, dsel = dK there).
+    float dsel[4];
+    auto tile = [&](auto slow_tag, auto b_tag, int cy, const floatx4& acc, float nbv) {
+      constexpr bool kSlow = decltype(slow_tag)::value, kB = decltype(b_tag)::value;
       const unsigned rr = (unsigned)(cy - wy0);
       float d[4];
       bool cand[4];
@@ -796,7 +798,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         } else {
           pre = xin[j] & (rr < (unsigned)full[j]);
         }
-        cand[j] = pre & (d[j] <= dAB[j]);  // lower bound may reach T, or approximate top K
+        cand[j] = pre & (d[j] <= dsel[j]);
         any |= cand[j];
       }
       // wave-uniform: every lane of a quadrant takes part in the appends, so the
@@ -809,15 +811,29 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
           bool ok = cand[j] && d[j] <= asure[j];
           if (cand[j] && !ok && d[j] <= arej[j]) ok = classify(j, d[j], nbv, rb, cxc, cy);
           const int slot = wave * 16 + 4 * q + j;
-          append(ok && d[j] <= dA[j], cntA[j], list + slot * kListCap, kListCap, li);
-          append(ok && d[j] <= dK[j], cntB[j], listb + slot * kListCapB, kListCapB, li);
+          if (kB) append(ok, cntB[j], listb + slot * kListCapB, kListCapB, li);
+          else append(ok, cntA[j], list + slot * kListCap, kListCap, li);
         }
       }
     };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dsel[j] = dA[j];
     sweep(column, [&](int cy, const floatx4& acc, float nbv) {
-      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, cy, acc, nbv);
-      else tile(std::false_type{}, cy, acc, nbv);
+      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, std::false_type{}, cy, acc, nbv);
+      else tile(std::false_type{}, std::false_type{}, cy, acc, nbv);
     });
+    bool needb = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      dsel[j] = cntA[j] > kListCap ? dK[j] : -kInf;
+      needb |= cntA[j] > kListCap;
+    }
+    if (__ballot(needb)) {
+      sweep(column, [&](int cy, const floatx4& acc, float nbv) {
+        if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, std::true_type{}, cy, acc, nbv);
+        else tile(std::false_type{}, std::true_type{}, cy, acc, nbv);
+      });
+    }
     if (n == 0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
