@@ -246,21 +246,34 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   // take consecutive elements of the 3 x SY x SX region, 16 loads in flight each
   {
     const int tot_el = 3 * SY * SX;
+    const float* const g0 = a.guide[0];
+    const float* const g1 = a.guide[1];
+    const float* const g2 = a.guide[2];
+    const int drow = XT / SX, dcol = XT - drow * SX;  // a step of XT elements
     for (int base = 0; base < tot_el; base += 16 * XT) {
       float v[16];
       int dst[16], chs[16];
+      int e = base + tid, r = e / SX, xx = e - r * SX;
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
-        const int e = base + u * XT + tid;
-        const int r = e / SX, xx = e - r * SX;
         const int ch = r >= 2 * SY ? 2 : (r >= SY ? 1 : 0), yy = r - ch * SY;
         chs[u] = ch;
         dst[u] = e < tot_el ? ch * plane + yy * P + xx : -1;
-        v[u] = e < tot_el ? a.guide[ch][(size_t)(Y0 + yy) * W + X0 + xx] : 0.0f;
+        // plane pointer by select: indexing the argument array with a per-lane
+        // channel would load the pointer from memory before every element
+        const float* gp = ch == 0 ? g0 : (ch == 1 ? g1 : g2);
+        v[u] = e < tot_el ? gp[(size_t)(Y0 + yy) * W + X0 + xx] : 0.0f;
+        e += XT;
+        xx += dcol;
+        r += drow;
+        if (xx >= SX) {
+          xx -= SX;
+          ++r;
+        }
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u)
-        if (dst[u] >= 0) c0[dst[u]] = (_Float16)(v[u] - shift[chs[u]]);
+        if (dst[u] >= 0) c0[dst[u]] = (_Float16)(v[u] - (chs[u] == 0 ? shift[0] : (chs[u] == 1 ? shift[1] : shift[2])));
     }
   }
   __syncthreads();
