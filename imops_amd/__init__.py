@@ -85,6 +85,15 @@ class Context:
         new shape and keep the fastest (results are identical for every candidate)."""
         check(lib().imops_ctx_set_autotune(self._h, int(bool(enabled))))
 
+    def measure_peaks(self) -> dict:
+        """Measured rates of this context's device (HBM copy, FP32 non-fused VALU issue,
+        FP16 MFMA); allocates two 1 GiB buffers for the call."""
+        out = (C.c_double * 6)()
+        check(lib().imops_measure_peaks(self._h, out, 6))
+        return {"hbm_copy_GBps": round(out[0], 1), "valu_fp32_nonfused_instr_per_clk_per_simd": round(out[1], 4),
+                "valu_fp32_nonfused_TFLOPs": round(out[2], 2), "mfma_f16_TFLOPs": round(out[3], 1),
+                "rated_clock_GHz": round(out[4], 3), "compute_units": int(out[5])}
+
     def tuning(self) -> dict:
         """Launch-shape choices made so far: {key: value}."""
         need = C.c_size_t()

@@ -120,6 +120,8 @@ SIGNATURES = {
     "imops_pipeline_run_burst": (_i, [_vp, _vp, C.POINTER(_vp), _sz, _i]),
     "imops_ctx_set_autotune": (_i, [_vp, _i]),
     "imops_ctx_tuning": (_i, [_vp, C.c_char_p, _sz, C.POINTER(_sz)]),
+    "imops_measure_peaks": (_i, [_vp, C.POINTER(C.c_double), _i]),
+    "imops_debug_fail_allocations": (_i, [_i]),
     "imops_synth_scene_rgb": (_i, [_fp, _sz, _sz, C.c_uint64]),
     "imops_synth_raw_u16": (_i, [_vp, _sz, _sz, C.c_char_p, C.c_uint64, _f, _f, _f, _f]),
 }

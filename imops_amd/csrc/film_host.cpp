@@ -636,8 +636,10 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
   if (!P) {
     if (ctx->capturing) throw Error{IMOPS_E_STATE, "film: shape not prepared before stream capture"};
     if (cache.runs.size() >= 64 && !ctx->in_burst) cache.runs.clear();  // a burst keeps every shape it prepared
-    cache.runs.emplace_back(key, std::unique_ptr<FilmPrepared>(new FilmPrepared()));
-    P = cache.runs.back().second.get();
+    // prepared in full before it enters the cache: a failure part-way (an
+    // allocation, a copy) must not leave an entry with missing device tables
+    std::unique_ptr<FilmPrepared> fresh(new FilmPrepared());
+    P = fresh.get();
     FilmDeviceTables T = hs.t;
     T.invert = 0;
     prepare_film(s, hs, T, pitch, shutter, p, *P);
@@ -683,6 +685,7 @@ void stage_film(imops_ctx* ctx, imops_image* img, const FilmStageParams& p) {
       IMOPS_HIP_CHECK(hipMemcpyAsync(P->tables.ptr, &P->T, sizeof(P->T), hipMemcpyHostToDevice, s));
       IMOPS_HIP_CHECK(hipStreamSynchronize(s));
     }
+    cache.runs.emplace_back(key, std::move(fresh));
   }
   // the scan writes straight into the image: every pixel's input was consumed by expose
   // the 32x32 mid-gray calibration above stays FP64 in both forms

@@ -4,11 +4,13 @@
 // imops_last_error() (the reference's catch_panic + LAST_ERROR,
 // pichromatic-pipeline/src/extern_pipeline.rs:18-58).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <string>
 
 #include "amaze.hpp"
+#include "peaks.hpp"
 #include "runtime.hpp"
 
 namespace imops {
@@ -23,11 +25,18 @@ void set_error(int code, const std::string& msg) {
 }
 void clear_error() {}
 
+// Allocation failures injected by imops_debug_fail_allocations (a test hook):
+// that many of the next device allocations fail as hipMalloc would.
+static std::atomic<int> g_injected_alloc_failures{0};
+
 void DevBuf::reserve(size_t n) {
   if (n <= bytes) return;
   release();
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, n);
+  int pending = g_injected_alloc_failures.load();
+  while (pending > 0 && !g_injected_alloc_failures.compare_exchange_weak(pending, pending - 1)) {
+  }
+  hipError_t e = pending > 0 ? hipErrorOutOfMemory : hipMalloc(&p, n);
   if (e != hipSuccess)
     throw Error{IMOPS_E_HIP, "hipMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e)};
   ptr = p;
@@ -671,6 +680,39 @@ int imops_ctx_synchronize(imops_ctx* ctx) {
   return guarded([&] {
     if (!ctx) throw Error{IMOPS_E_INVALID, "null context"};
     IMOPS_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int imops_debug_fail_allocations(int count) {
+  return guarded([&] {
+    if (count < 0) throw Error{IMOPS_E_INVALID, "imops_debug_fail_allocations: count < 0"};
+    g_injected_alloc_failures.store(count);
+  });
+}
+
+int imops_measure_peaks(imops_ctx* ctx, double* out, int n) {
+  return guarded([&] {
+    if (!ctx || !out || n < 6) throw Error{IMOPS_E_INVALID, "imops_measure_peaks: null argument or n < 6"};
+    IMOPS_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)1 << 30;
+    struct Held {  // released on every exit
+      imops::DevBuf b;
+      ~Held() { b.release(); }
+    } a, b;
+    a.b.reserve(bytes);
+    b.b.reserve(bytes);
+    IMOPS_HIP_CHECK(hipMemsetAsync(a.b.ptr, 0, bytes, ctx->stream));
+    IMOPS_HIP_CHECK(hipMemsetAsync(b.b.ptr, 0, bytes, ctx->stream));
+    imops::MeasuredPeaks p;
+    const hipError_t e = imops::measure_peaks(ctx->stream, a.b.ptr, b.b.ptr, bytes, ctx->device, &p);
+    (void)hipStreamSynchronize(ctx->stream);
+    IMOPS_HIP_CHECK(e);
+    out[0] = p.copy_GBps;
+    out[1] = p.valu_per_clk_per_simd;
+    out[2] = p.valu_nonfused_TFLOPs;
+    out[3] = p.mfma_f16_TFLOPs;
+    out[4] = p.clock_GHz;
+    out[5] = p.compute_units;
   });
 }
 

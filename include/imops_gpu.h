@@ -101,6 +101,21 @@ int imops_ctx_set_autotune(imops_ctx* ctx, int enabled);
 int imops_ctx_tuning(imops_ctx* ctx, char* buf, size_t len, size_t* needed);
 int imops_ctx_timing(imops_ctx* ctx, const char* region, double* total_ms, int* count);
 
+/* Measured rates of the context's device (no counterpart in the reference: the
+ * benchmark reports them beside the published peaks).  Allocates two 1 GiB
+ * buffers for the call and writes n >= 6 values: out[0] HBM copy GB/s (read +
+ * written bytes), out[1] FP32 non-fused VALU wave64 instructions per clock per
+ * SIMD at the rated clock, out[2] the same stream in TFLOP/s (one flop per lane
+ * per mul or add), out[3] dense v_mfma_f32_16x16x32_f16 TFLOP/s, out[4] the rated
+ * clock in GHz, out[5] the number of compute units. */
+int imops_measure_peaks(imops_ctx* ctx, double* out, int n);
+
+/* Test hook (no counterpart in the reference): the next `count` device
+ * allocations the library makes fail as hipMalloc would (the call returns
+ * IMOPS_E_HIP and imops_last_error names the allocation); 0 clears it.  Lets
+ * tests reach the device-allocation error path from the host. */
+int imops_debug_fail_allocations(int count);
+
 /* Images: device-resident replacement of pichromatic::pixel::Image
  * (pixel.rs:22-27, image.rs:7-43). */
 int imops_image_create(imops_ctx* ctx, imops_image** out);
