@@ -3,12 +3,12 @@
 // Plane layout: every emulsion quantity is a planar f32 W x H array
 // (emulsion-major), so each stage streams unit-stride rows.
 //
-//  * expose: per pixel, the reference's f64 spectral walk (upsample ->
-//    photon weighting -> Beer-Lambert through every absorbing layer ->
-//    trapezoid band mean), with the per-stock transmittances precomputed on
-//    the host exactly as the reference evaluates them, so the walk is the same
-//    sequence of f64 multiplies/adds/divides (exposure/mod.rs:68-81,
-//    absorption.rs:24-108, upsample.rs:145-170);
+//  * expose: the reference's spectral walk (upsample -> photon weighting ->
+//    Beer-Lambert through every absorbing layer -> trapezoid band mean,
+//    exposure/mod.rs:68-81, absorption.rs:24-108, upsample.rs:145-170) is linear
+//    in the three basis weights, so the host collapses it over the grid into a
+//    3 x E matrix and the device evaluates relu(M^-1 rgb) . C per pixel in f64
+//    (film_fit_expose in film_fit.hip);
 //  * blur: reflect-101 separable Gaussian, one output per thread, taps summed
 //    in the reference's order (film/blur.rs:10-89), with fused epilogues for
 //    local scatter, adjacency and plain stores;
@@ -17,8 +17,9 @@
 //    reduction.rs:16-68);
 //  * DIR inhibition, grain (SplitMix64 in counter form: value k of the row
 //    stream is mix(key + k * golden), bit-identical to the reference's
-//    sequential generator), and the f64 densitometric scan + optional
-//    positive invert.
+//    sequential generator), and the densitometric scan + optional positive
+//    invert (f64 per bin in the reference's order; the mixed form in f32 with
+//    log2(10)-scaled absorptivities and folded trapezoid weights).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -34,51 +35,6 @@ __device__ __forceinline__ int reflect_index(long long i, long long n) {
     else if (x >= n) x = 2 * n - 2 - x;
     else return (int)x;
     if (n == 1) return 0;
-  }
-}
-
-// Real = double is the reference's arithmetic; Real = float is the mixed
-// form's.  NB: bins of the spectral grid.  The walk runs bin by bin through
-// the layers and accumulates each emulsion's trapezoid as it goes (acc_e +=
-// (prev_e + cur_e) / 2 * dl in bin order: the reference's sum in its order),
-// so no per-bin arrays are kept.
-template <typename Real, int NB>
-__global__ void expose_kernel(const float* __restrict__ rgb, size_t n, const FilmDeviceTables* __restrict__ T,
-                              float* __restrict__ planes) {
-  const int E = T->n_emulsions, NA = T->n_absorbing;
-  const Real dl = (Real)T->dl;
-  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    const Real r = (Real)rgb[p * 3], g = (Real)rgb[p * 3 + 1], b = (Real)rgb[p * 3 + 2];
-    Real w[3];
-    for (int k = 0; k < 3; ++k) {
-      const Real v = (Real)T->minv[k][0] * r + (Real)T->minv[k][1] * g + (Real)T->minv[k][2] * b;
-      w[k] = v < (Real)0 ? (Real)0 : v;
-    }
-    Real acc[kFilmMaxEmulsions], prev[kFilmMaxEmulsions];
-#pragma unroll
-    for (int e = 0; e < kFilmMaxEmulsions; ++e) acc[e] = prev[e] = 0;
-    for (int i = 0; i < NB; ++i) {
-      const Real s = w[0] * (Real)T->basis[0][i] + w[1] * (Real)T->basis[1][i] + w[2] * (Real)T->basis[2][i];
-      Real phi = s * (Real)T->inv_e_rel[i];  // s / e_rel (exposure/mod.rs:124-137), within 1 ulp
-      int e = 0;
-      for (int l = 0; l < NA; ++l) {
-        const Real pt = phi * (Real)T->trans[l][i];
-        const Real absorbed = phi - pt;
-        phi = pt;
-        if (T->is_emulsion[l]) {
-#pragma unroll
-          for (int k = 0; k < kFilmMaxEmulsions; ++k)  // register-resident: static index
-            if (k == e) {
-              if (i > 0) acc[k] += (Real)0.5 * (prev[k] + absorbed) * dl;
-              prev[k] = absorbed;
-            }
-          ++e;
-        }
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < kFilmMaxEmulsions; ++e)
-      if (e < E) planes[(size_t)e * n + p] = (float)(acc[e] * (Real)T->band_inv);
   }
 }
 
@@ -680,20 +636,6 @@ __global__ void blur_hT_kernel(const float* __restrict__ in, const float* orig, 
 
 }  // namespace
 
-hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
-                       bool fp32, int bins) {
-  if (bins != 16 && bins != 81) return hipErrorInvalidValue;
-  const dim3 g(grid1d(n)), blk(256);
-  if (bins == 16) {
-    if (fp32) hipLaunchKernelGGL((expose_kernel<float, 16>), g, blk, 0, s, rgb, n, T, planes);
-    else hipLaunchKernelGGL((expose_kernel<double, 16>), g, blk, 0, s, rgb, n, T, planes);
-  } else {
-    if (fp32) hipLaunchKernelGGL((expose_kernel<float, 81>), g, blk, 0, s, rgb, n, T, planes);
-    else hipLaunchKernelGGL((expose_kernel<double, 81>), g, blk, 0, s, rgb, n, T, planes);
-  }
-  return hipGetLastError();
-}
-
 hipError_t film_blur(const float* in, float* tmp, const float* orig, float* out, int w, int h, const float* taps,
                      int radius, int mode, float a, hipStream_t s, const BlurPlanes& pl) {
   blur_passes(in, tmp, orig, out, w, h, taps, taps, radius, mode, a, pl, s);
@@ -761,15 +703,79 @@ hipError_t film_grain(float* dye, float* noise, float* tmp, int w, int h, uint64
   return hipGetLastError();
 }
 
+namespace {
+// The mixed form's scan in FP32: per bin one density sum over the emulsions'
+// dye and mask terms (pre-scaled by log2 10), one v_exp_f32 and three
+// products with scanner x CMF x trapezoid weight; the XYZ -> AP1 step, the
+// Dmin normalisation and the inversion as scan_kernel.
+template <int NB>
+__global__ void __launch_bounds__(256) scan_f32_kernel(const float* __restrict__ image, const float* __restrict__ mask,
+                                                       size_t n, const FilmDeviceTables* __restrict__ T,
+                                                       float* __restrict__ rgb) {
+  const int E = T->n_emulsions;
+  const size_t b = blockIdx.y;
+  image += b * E * n;
+  mask += b * E * n;
+  rgb += b * 3 * n;
+  for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    float di[kFilmMaxEmulsions], dm[kFilmMaxEmulsions];
+#pragma unroll
+    for (int e = 0; e < kFilmMaxEmulsions; ++e) {
+      di[e] = e < E ? image[(size_t)e * n + p] : 0.0f;
+      dm[e] = e < E ? mask[(size_t)e * n + p] : 0.0f;
+    }
+    float x0 = 0.0f, x1 = 0.0f, x2 = 0.0f;
+    for (int i = 0; i < NB; ++i) {
+      float d = 0.0f;
+#pragma unroll
+      for (int e = 0; e < kFilmMaxEmulsions; ++e) {
+        if (e >= E) break;
+        d = fmaf(di[e], T->scan_eps2[e][i], d);
+        d = fmaf(dm[e], T->scan_mask2[e][i], d);
+      }
+      const float t = __builtin_amdgcn_exp2f(-d);
+      x0 = fmaf(t, T->scan_w[0][i], x0);
+      x1 = fmaf(t, T->scan_w[1][i], x1);
+      x2 = fmaf(t, T->scan_w[2][i], x2);
+    }
+    const float ds = (float)T->dmin_scale;
+    float out[3];
+    for (int c = 0; c < 3; ++c)
+      out[c] = (T->scan_xyz2ap1[c][0] * x0 + T->scan_xyz2ap1[c][1] * x1 + T->scan_xyz2ap1[c][2] * x2) * ds;
+    if (T->invert) {
+      const float eps = 1e-6f;
+      float raw[3];
+      for (int c = 0; c < 3; ++c) {
+        float tt = out[c] / T->inv_dmin[c];
+        tt = tt < eps ? eps : (tt > 1.0f ? 1.0f : tt);
+        const float inv = fmaxf(1.0f / tt - 1.0f, 0.0f);
+        const float iv = fmaxf(inv, 0.0f);
+        const float sh = fmaxf(T->inv_shoulder[c], 1e-6f);
+        raw[c] = fmaxf(T->inv_gain[c] * (iv / (1.0f + iv / sh)), 0.0f);
+      }
+      const float y = fmaxf(0.2722287168f * raw[0] + 0.6740817658f * raw[1] + 0.0536895174f * raw[2], 0.0f);
+      float k;
+      if (y <= 0.0f) k = 0.0f;
+      else if (y >= 0.02f) k = 0.55f;
+      else k = 0.55f * (y / 0.02f);
+      for (int c = 0; c < 3; ++c) out[c] = fmaxf(y + k * (raw[c] - y), 0.0f);
+    }
+    rgb[p * 3] = out[0];
+    rgb[p * 3 + 1] = out[1];
+    rgb[p * 3 + 2] = out[2];
+  }
+}
+}  // namespace
+
 hipError_t film_scan(const float* image, const float* mask, size_t n, const FilmDeviceTables* T, float* rgb,
                      hipStream_t s, int images, bool fp32, int bins) {
   if (bins != 16 && bins != 81) return hipErrorInvalidValue;
   const dim3 g(grid1d(n), images), blk(256);
   if (bins == 16) {
-    if (fp32) hipLaunchKernelGGL((scan_kernel<float, 16>), g, blk, 0, s, image, mask, n, T, rgb);
+    if (fp32) hipLaunchKernelGGL((scan_f32_kernel<16>), g, blk, 0, s, image, mask, n, T, rgb);
     else hipLaunchKernelGGL((scan_kernel<double, 16>), g, blk, 0, s, image, mask, n, T, rgb);
   } else {
-    if (fp32) hipLaunchKernelGGL((scan_kernel<float, 81>), g, blk, 0, s, image, mask, n, T, rgb);
+    if (fp32) hipLaunchKernelGGL((scan_f32_kernel<81>), g, blk, 0, s, image, mask, n, T, rgb);
     else hipLaunchKernelGGL((scan_kernel<double, 81>), g, blk, 0, s, image, mask, n, T, rgb);
   }
   return hipGetLastError();

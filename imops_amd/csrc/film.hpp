@@ -55,6 +55,12 @@ struct FilmDeviceTables {
   double dmin_scale;       // 1 / peak of the Dmin scan
   int invert;              // PositiveLinear
   float inv_dmin[3], inv_shoulder[3], inv_gain[3];
+  // The mixed form's FP32 scan: absorptivities pre-scaled by log2(10), so that
+  // 10^-d = exp2(-d'), and scanner x CMF x trapezoid weight per bin (0 where unused)
+  float scan_eps2[kFilmMaxEmulsions][kFilmMaxBins];
+  float scan_mask2[kFilmMaxEmulsions][kFilmMaxBins];
+  float scan_w[3][kFilmMaxBins];
+  float scan_xyz2ap1[3][3];
 };
 
 // Differentiable film model (film_fit.hip).  Parameter gradients are f64 sums:
@@ -71,11 +77,10 @@ struct FitExposeConsts {
   int n_emulsions;
 };
 
-// fp32: the mixed form's FP32 per-pixel arithmetic instead of the reference's FP64 (expose, capture LUT, scan)
-// bins: the tables' spectral grid (16 or 81; kernels are specialised on it)
-hipError_t film_expose(const float* rgb, size_t n, const FilmDeviceTables* T, float* planes, hipStream_t s,
-                       bool fp32 = false, int bins = 16);
-// Batched over `images` images: rgb image b at rgb + 3 b n, its E planes at planes + b E n.
+// fp32: the mixed form's FP32 per-pixel arithmetic instead of the reference's FP64 (capture LUT, scan)
+// bins: the tables' spectral grid (16 or 81; the scan kernels are specialised on it)
+// The exposure, collapsed over the spectral grid to relu(M^-1 rgb) . C in f64 (the stage and the
+// differentiable model), batched over `images` images: rgb image b at rgb + 3 b n, its E planes at planes + b E n.
 hipError_t film_fit_expose(const float* rgb, size_t n, const FitExposeConsts& k, float* planes, hipStream_t s,
                            int images = 1);
 // out = c0 * orig + c1 * (V o H)^T in   (the transpose of film_blur's reflect-101 Gaussian; radius <= w-1, h-1);

@@ -451,6 +451,7 @@ unsigned bits_of(float f) {
 // per-call fields (reciprocity, bounce gains, PositiveLinear inversion) filled in.
 struct FilmPrepared {
   imops::DevBuf taps, tables;
+  FitExposeConsts ex{};  // the exposure collapsed over the spectral grid (collapse_exposure)
   int radius[5] = {}, off[5] = {};
   float sigmas[5] = {};
   bool do_wide = false;
@@ -469,10 +470,64 @@ struct FilmPrepared {
 // hal_strength >= 0 its gain strength * bleed_e, applied whatever the backing's
 // reflectance; negative values keep the reference's psf_halation / pitch and
 // r_e * bleed_e.
+// The exposure (exposure/mod.rs:45-137) is linear in the three basis weights:
+// emulsion e absorbs band_inv * sum_i w_i s(i)/e_rel(i) * (transmittance of the
+// layers above it)(i) * (1 - its own transmittance)(i), with s(i) = sum_k
+// w_k basis_k(i) and trapezoid weights w_i, so it is a 3 x E matrix per stock
+// and grid, formed here in f64; the device evaluates relu(M^-1 rgb) . C per
+// pixel in f64 (film_fit_expose), which differs from the per-bin walk only by
+// the order of f64 roundings.
+FitExposeConsts collapse_exposure(const FilmDeviceTables& T) {
+  FitExposeConsts k{};
+  for (int m = 0; m < 3; ++m)
+    for (int c = 0; c < 3; ++c) k.minv[m][c] = T.minv[m][c];
+  k.n_emulsions = T.n_emulsions;
+  const int nb = T.n_bins;
+  std::vector<double> above(nb, 1.0);
+  int e = 0;
+  for (int l = 0; l < T.n_absorbing; ++l) {
+    if (T.is_emulsion[l]) {
+      for (int kk = 0; kk < 3; ++kk) {
+        double acc = 0.0;
+        for (int i = 0; i < nb; ++i) {
+          const double wi = (i == 0 || i == nb - 1) ? 0.5 * T.dl : T.dl;
+          acc += wi * T.basis[kk][i] * T.inv_e_rel[i] * above[i] * (1.0 - T.trans[l][i]);
+        }
+        k.c[kk][e] = acc * T.band_inv;
+      }
+      ++e;
+    }
+    for (int i = 0; i < nb; ++i) above[i] *= T.trans[l][i];
+  }
+  return k;
+}
+
+// The FP32 scan's tables (scan_f32_kernel): log2(10)-scaled absorptivities and
+// scanner x CMF x trapezoid weight per bin.
+void scan_f32_tables(FilmDeviceTables& T) {
+  const int nb = T.n_bins;
+  const double l2_10 = 3.321928094887362347870;
+  for (int e = 0; e < kFilmMaxEmulsions; ++e)
+    for (int i = 0; i < kFilmMaxBins; ++i) {
+      const bool on = e < T.n_emulsions && i < nb;
+      T.scan_eps2[e][i] = on ? (float)(T.em[e].eps[i] * l2_10) : 0.0f;
+      T.scan_mask2[e][i] = on && T.em[e].has_mask ? (float)(T.em[e].mask_eps[i] * l2_10) : 0.0f;
+    }
+  for (int c = 0; c < 3; ++c)
+    for (int i = 0; i < kFilmMaxBins; ++i) {
+      const double wi = i >= nb ? 0.0 : ((i == 0 || i == nb - 1) ? 0.5 * T.dl : T.dl);
+      T.scan_w[c][i] = (float)(T.scanner[i] * T.cmf[c][i] * wi);
+    }
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) T.scan_xyz2ap1[r][c] = (float)T.xyz2ap1[r][c];
+}
+
 void prepare_film(hipStream_t s, const HostStock& hs, const FilmDeviceTables& base, float pitch, float shutter,
                   const FilmStageParams& fp, FilmPrepared& P) {
   const Stock& st = *hs.s;
   P.T = base;
+  P.ex = collapse_exposure(P.T);
+  scan_f32_tables(P.T);
   const int E = P.T.n_emulsions;
   const float sigma_local = std::max(st.psf_local_um / std::max(pitch, 1e-6f), 0.0f);
   const float sigma_wide = fp.hal_radius >= 0.0f ? fp.hal_radius / 3.0f
@@ -522,7 +577,8 @@ void run_film(hipStream_t s, const HostStock& hs, const FilmPrepared& P, const f
   const float sigma_local = P.sigmas[0], sig_dir = P.sigmas[2], sig_adj = P.sigmas[3];
 
   const int bins = P.T.n_bins;
-  IMOPS_HIP_CHECK(film_expose(rgb_in, n, T, B.absorbed, s, fp32, bins));
+  (void)bins;
+  IMOPS_HIP_CHECK(film_fit_expose(rgb_in, n, P.ex, B.absorbed, s, 1));
   if (sigma_local >= 1e-3f)  // local gelatin scatter, mix 0.18 (halation.rs:38-44)
     for (int e = 0; e < E; ++e) {
       float* p = B.absorbed + (size_t)e * n;
