@@ -156,3 +156,60 @@ def test_batch_and_saved_state_match_single_images(model):
           f"batch vs single sum {np.abs(g_re - g_sum).max() / scale:.2e}", flush=True)
     assert np.allclose(g_st, g_re, rtol=1e-6, atol=1e-7 * scale)
     assert np.allclose(g_sum, g_re, rtol=1e-5, atol=1e-6 * scale)
+
+
+def test_gradients_at_the_c4_shape_match_fp64_central_differences(model):
+    """The same check at the adjoint workload's shape: 8 ACEScg crops of 512x512.  Per
+    parameter (d_max, gamma, halation, four sensitivity bins per emulsion) and along two
+    random directions over every parameter, the GPU adjoint's gradient of the L2 loss
+    against FP64 central differences (h = 1e-3) of the FP64 oracle's loss."""
+    E = model.n_emulsions
+    print("film fit: gradients at 8x512x512", flush=True)
+    imgs = _images(8, 512, 512, 60)
+    hidden = _perturbed(model.nominal(), 61, 1.0)
+    target = _oracle_forward(imgs, hidden, E)
+    theta = _perturbed(model.nominal(), 63, 0.4)
+
+    x = torch.from_numpy(imgs).cuda()
+    t = torch.from_numpy(target).cuda()
+    params = {k: v.clone().requires_grad_(True) for k, v in theta.items()}
+    loss = ((model(x, **params) - t) ** 2).sum()
+    loss.backward()
+    grads = {k: v.grad.detach().double() for k, v in params.items()}
+
+    def oracle_loss(p):
+        out = _oracle_forward(imgs, p, E)
+        return float(np.sum((out.astype(np.float64) - target.astype(np.float64)) ** 2))
+
+    def moved(direction, sign, h):
+        return {k: (v.double() + sign * h * direction[k]).float() for k, v in theta.items()}
+
+    h = 1e-3
+    checks = [("d_max", (e,)) for e in range(E)] + [("gamma", (e,)) for e in range(E)] + \
+             [("hal_strength", ()), ("hal_radius", ())] + \
+             [("sens", (e, i)) for e in range(E) for i in (2, 5, 8, 12)]
+    adj, fds = [], []
+    for name, idx in checks:
+        d = {k: torch.zeros_like(v, dtype=torch.float64) for k, v in theta.items()}
+        d[name][idx] = 1.0
+        fds.append((oracle_loss(moved(d, 1, h)) - oracle_loss(moved(d, -1, h))) / (2 * h))
+        adj.append(float(grads[name][idx]))
+    adj, fds = np.array(adj), np.array(fds)
+    floor = 1e-2 * np.abs(fds).max()
+    rel = np.abs(adj - fds) / np.maximum(np.abs(fds), floor)
+    for (name, idx), g, fd, r in zip(checks, adj, fds, rel):
+        print(f"{name}{list(idx)}: adjoint {g:.6g} fd {fd:.6g} rel {r:.2e}", flush=True)
+    vec = np.linalg.norm(adj - fds) / np.linalg.norm(fds)
+    print(f"worst entry {rel.max():.2e}, vector relative error {vec:.2e}", flush=True)
+    assert rel.max() <= 1e-3 and vec <= 1e-3
+    # directional derivatives along random directions over every parameter (sensitivities scaled
+    # to their magnitude so each coordinate moves by a comparable relative amount)
+    rng = np.random.default_rng(65)
+    for k in range(2):
+        d = {name: torch.from_numpy(rng.uniform(-1, 1, tuple(v.shape))) * (v.double().abs() + 0.1)
+             for name, v in theta.items()}
+        fd = (oracle_loss(moved(d, 1, h)) - oracle_loss(moved(d, -1, h))) / (2 * h)
+        g = sum(float((grads[name] * d[name]).sum()) for name in d)
+        r = abs(g - fd) / abs(fd)
+        print(f"direction {k}: adjoint {g:.6g} fd {fd:.6g} rel {r:.2e}", flush=True)
+        assert r <= 1e-3
