@@ -112,6 +112,8 @@ SIGNATURES = {
     "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp]),
     "imops_film_fit_backward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp,
                                      C.POINTER(FilmFitParams)]),
+    "imops_film_fit_l2_step": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz,
+                                    C.POINTER(FilmFitParams), C.POINTER(C.c_double)]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
     "imops_pipeline_num_stages": (_sz, [_vp]),

@@ -369,6 +369,54 @@ int blur_col_height(int w, int h) {
   return th;
 }
 
+// a[j] = sum_k tp[k] bp[j + k], j < 4, over a 16-byte-aligned LDS row (tp zero-padded
+// to a multiple of 4; reads up to bp[nt + 6]).  q holds bp[k .. k+3], nq the next quad.
+__device__ __forceinline__ void row_conv4(const float* bp, const float* tp, int nt, float (&a)[4]) {
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  float4 q = *reinterpret_cast<const float4*>(bp);
+  int k = 0;
+  for (; k + 4 <= nt; k += 4) {
+    const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
+    const float4 tk = *reinterpret_cast<const float4*>(tp + k);
+    a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
+    a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
+    a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
+    a0 += q.w * tk.w; a1 += nq.x * tk.w; a2 += nq.y * tk.w; a3 += nq.z * tk.w;
+    q = nq;
+  }
+  if (k < nt) {  // 1 or 3 taps left (nt is odd)
+    const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
+    const float4 tk = *reinterpret_cast<const float4*>(tp + k);
+    a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
+    if (k + 1 < nt) {
+      a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
+      a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
+    }
+  }
+  a[0] = a0;
+  a[1] = a1;
+  a[2] = a2;
+  a[3] = a3;
+}
+
+// a[j] = sum_k tp[k] bp[(j + k) 64], j < 4, down one lane's column of a 64-wide
+// LDS tile; the window w0..w3 slides down one row per tap (reads up to row nt + 2).
+__device__ __forceinline__ void col_conv4(const float* bp, const float* tp, int nt, float (&a)[4]) {
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+  float w0 = bp[0], w1 = bp[64], w2 = bp[128];
+#pragma unroll 4
+  for (int k = 0; k < nt; ++k) {
+    const float w3 = bp[(k + 3) * 64];
+    const float tk = tp[k];
+    a0 += w0 * tk; a1 += w1 * tk; a2 += w2 * tk; a3 += w3 * tk;
+    w0 = w1; w1 = w2; w2 = w3;
+  }
+  a[0] = a0;
+  a[1] = a1;
+  a[2] = a2;
+  a[3] = a3;
+}
+
 __global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
                                                       int h, const float* __restrict__ taps, int radius,
                                                       BlurPlanes pl) {
@@ -386,35 +434,14 @@ __global__ void __launch_bounds__(BT) blur_row_kernel(const float* __restrict__ 
   for (int j = tid; j < nx + 2 * radius; j += BT) buf[j] = row[reflect_index((long long)x0 - radius + j, w)];
   __syncthreads();
   float* orow = out + (size_t)y * w + x0;
-  // outputs 4u..4u+3 read buf[4u + j + k]; q holds buf[4u + k .. +3], nq the next quad
   for (int u = tid; 4 * u < nx; u += BT) {
-    const float* bp = buf + 4 * u;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    float4 q = *reinterpret_cast<const float4*>(bp);
-    int k = 0;
-    for (; k + 4 <= nt; k += 4) {
-      const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
-      const float4 tk = *reinterpret_cast<const float4*>(tp + k);
-      a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
-      a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
-      a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
-      a0 += q.w * tk.w; a1 += nq.x * tk.w; a2 += nq.y * tk.w; a3 += nq.z * tk.w;
-      q = nq;
-    }
-    if (k < nt) {  // 1 or 3 taps left (nt is odd)
-      const float4 nq = *reinterpret_cast<const float4*>(bp + k + 4);
-      const float4 tk = *reinterpret_cast<const float4*>(tp + k);
-      a0 += q.x * tk.x; a1 += q.y * tk.x; a2 += q.z * tk.x; a3 += q.w * tk.x;
-      if (k + 1 < nt) {
-        a0 += q.y * tk.y; a1 += q.z * tk.y; a2 += q.w * tk.y; a3 += nq.x * tk.y;
-        a0 += q.z * tk.z; a1 += q.w * tk.z; a2 += nq.x * tk.z; a3 += nq.y * tk.z;
-      }
-    }
+    float a[4];
+    row_conv4(buf + 4 * u, tp, nt, a);
     const int o = 4 * u;
-    orow[o] = a0;
-    if (o + 1 < nx) orow[o + 1] = a1;
-    if (o + 2 < nx) orow[o + 2] = a2;
-    if (o + 3 < nx) orow[o + 3] = a3;
+    orow[o] = a[0];
+    if (o + 1 < nx) orow[o + 1] = a[1];
+    if (o + 2 < nx) orow[o + 2] = a[2];
+    if (o + 3 < nx) orow[o + 3] = a[3];
   }
 }
 
@@ -453,23 +480,15 @@ __global__ void __launch_bounds__(BT) blur_col_kernel(const float* __restrict__ 
     for (int j = g; j < rows; j += BT / 64) buf[j * 64 + c] = tmp[(size_t)reflect_index((long long)ys - radius + j, h) * w + x];
   __syncthreads();
   if (x >= w) return;
-  // rows t0..t0+3 read buf rows t0 + j + k; the window w0..w3 slides down one row per tap
+  // rows t0..t0+3 read buf rows t0 + j + k
   for (int t0 = 4 * g; t0 < nyo; t0 += 4 * (BT / 64)) {
-    const float* bp = buf + t0 * 64 + c;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    float w0 = bp[0], w1 = bp[64], w2 = bp[128];
-#pragma unroll 4
-    for (int k = 0; k < nt; ++k) {
-      const float w3 = bp[(k + 3) * 64];
-      const float tk = tp[k];
-      a0 += w0 * tk; a1 += w1 * tk; a2 += w2 * tk; a3 += w3 * tk;
-      w0 = w1; w1 = w2; w2 = w3;
-    }
+    float acc[4];
+    col_conv4(buf + t0 * 64 + c, tp, nt, acc);
     const size_t i = (size_t)(ys + t0) * w + x;
-    out[i] = blur_epilogue(mode, a, orig, i, a0);
-    if (t0 + 1 < nyo) out[i + w] = blur_epilogue(mode, a, orig, i + w, a1);
-    if (t0 + 2 < nyo) out[i + 2 * (size_t)w] = blur_epilogue(mode, a, orig, i + 2 * (size_t)w, a2);
-    if (t0 + 3 < nyo) out[i + 3 * (size_t)w] = blur_epilogue(mode, a, orig, i + 3 * (size_t)w, a3);
+    out[i] = blur_epilogue(mode, a, orig, i, acc[0]);
+    if (t0 + 1 < nyo) out[i + w] = blur_epilogue(mode, a, orig, i + w, acc[1]);
+    if (t0 + 2 < nyo) out[i + 2 * (size_t)w] = blur_epilogue(mode, a, orig, i + 2 * (size_t)w, acc[2]);
+    if (t0 + 3 < nyo) out[i + 3 * (size_t)w] = blur_epilogue(mode, a, orig, i + 3 * (size_t)w, acc[3]);
   }
 }
 
@@ -530,8 +549,24 @@ __device__ __forceinline__ float adj_sum(const float* __restrict__ v, size_t str
   return adj_edges(v, stride, n, q, taps, r, acc);
 }
 
+// The reflected sources of a transposed blur, read from an LDS window: v[(p -
+// base) stride] holds source p (zero outside [0, n)); tr holds the taps
+// reversed (tr[m] = tap 2r - m).  Sources -q (q <= r) and 2n - 2 - q (q >= n -
+// 1 - r) reflect onto q; the window of the tile holding q always covers the
+// sources they read (p in [0, r - q] and [n - r, n - 1]).
+__device__ __forceinline__ float fold_edges(const float* v, int stride, int base, int n, int q, const float* tr, int r,
+                                            float acc) {
+  if (q >= 1 && q <= r)  // p = r - q - j >= 0
+    for (int j = 0; j <= r - q; ++j) acc += tr[2 * r - j] * v[(r - q - j - base) * stride];
+  if (q <= n - 2 && q >= n - 1 - r)  // p = 2n - 2 - q + r - j < n
+    for (int j = n - 1 - q + r; j <= 2 * r; ++j) acc += tr[2 * r - j] * v[(2 * n - 2 - q + r - j - base) * stride];
+  return acc;
+}
+
 // Column transpose, LDS-tiled like blur_col_kernel: the tile's rows and a halo
 // of r above and below, zero outside the plane (those sources do not exist).
+// The transpose at row q sums tap k times source q + r - k, i.e. the forward
+// column loop with the taps reversed, plus the folded reflections.
 __global__ void __launch_bounds__(BT) blur_colT_kernel(const float* __restrict__ in, float* __restrict__ out, int w,
                                                        int h, const float* __restrict__ taps, int radius, int th,
                                                        BlurPlanes pl) {
@@ -543,9 +578,9 @@ __global__ void __launch_bounds__(BT) blur_colT_kernel(const float* __restrict__
   const int xs = (blockIdx.x % tiles_x) * 64, ys = (blockIdx.x / tiles_x) * th;
   const int nyo = min(th, h - ys);
   const int c = tid & 63, g = tid >> 6, x = xs + c;
-  float* tp = sh;
+  float* tr = sh;
   float* buf = sh + nt;
-  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  for (int k = tid; k < nt; k += BT) tr[k] = taps[nt - 1 - k];
   const int rows = nyo + 2 * radius;
   if (x < w)
     for (int j = g; j < rows; j += BT / 64) {
@@ -554,30 +589,22 @@ __global__ void __launch_bounds__(BT) blur_colT_kernel(const float* __restrict__
     }
   __syncthreads();
   if (x >= w) return;
-  // rows t0..t0+3 read buf rows t0 + j + 2r - k; the window slides up one row per tap
   for (int t0 = 4 * g; t0 < nyo; t0 += 4 * (BT / 64)) {
-    const float* bp = buf + (t0 + 2 * radius) * 64 + c;
-    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    float w1 = bp[64], w2 = bp[128], w3 = bp[192];
-#pragma unroll 4
-    for (int k = 0; k < nt; ++k) {
-      const float w0 = bp[-k * 64];
-      const float tk = tp[k];
-      a[0] += tk * w0; a[1] += tk * w1; a[2] += tk * w2; a[3] += tk * w3;
-      w3 = w2; w2 = w1; w1 = w0;
-    }
+    float a[4];
+    col_conv4(buf + t0 * 64 + c, tr, nt, a);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int q = ys + t0 + j;
       if (t0 + j >= nyo) break;
       float acc = a[j];
-      if (q <= radius || q >= h - 1 - radius) acc = adj_edges(in + x, (size_t)w, h, q, tp, radius, acc);
+      if (q <= radius || q >= h - 1 - radius) acc = fold_edges(buf + c, 64, ys - radius, h, q, tr, radius, acc);
       out[(size_t)q * w + x] = acc;
     }
   }
 }
 
-// Row transpose, LDS-tiled like blur_row_kernel, then out = c0 orig + c1 acc
+// Row transpose, LDS-tiled like blur_row_kernel (taps reversed, zero halo, the
+// reflections folded from the same row buffer), then out = c0 orig + c1 acc
 // (orig may alias out).
 __global__ void __launch_bounds__(BT) blur_rowT_kernel(const float* __restrict__ in, const float* orig, float* out,
                                                        int w, int h, const float* __restrict__ taps, int radius,
@@ -590,22 +617,28 @@ __global__ void __launch_bounds__(BT) blur_rowT_kernel(const float* __restrict__
   const int segs = (w + kRowSeg - 1) / kRowSeg;
   const int y = blockIdx.x / segs, x0 = (blockIdx.x % segs) * kRowSeg;
   const int nx = min(kRowSeg, w - x0);
-  float* tp = sh;
-  float* buf = sh + nt;
-  for (int k = tid; k < nt; k += BT) tp[k] = taps[k];
+  float* tr = sh;
+  float* buf = sh + row_buf_off(nt);
+  for (int k = tid; k < row_buf_off(nt); k += BT) tr[k] = k < nt ? taps[nt - 1 - k] : 0.0f;
   const float* row = in + (size_t)y * w;
   for (int j = tid; j < nx + 2 * radius; j += BT) {
     const int xx = x0 - radius + j;
     buf[j] = (xx >= 0 && xx < w) ? row[xx] : 0.0f;
   }
   __syncthreads();
-  for (int t = tid; t < nx; t += BT) {
-    float acc = 0.0f;
-    for (int k = 0; k < nt; ++k) acc += tp[k] * buf[t + 2 * radius - k];
-    const int q = x0 + t;
-    if (q <= radius || q >= w - 1 - radius) acc = adj_edges(row, 1, w, q, tp, radius, acc);
-    const size_t i = (size_t)y * w + q;
-    out[i] = (c0 != 0.0f ? c0 * orig[i] : 0.0f) + c1 * acc;
+  for (int u = tid; 4 * u < nx; u += BT) {
+    float a[4];
+    row_conv4(buf + 4 * u, tr, nt, a);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = 4 * u + j;
+      if (t >= nx) break;
+      const int q = x0 + t;
+      float acc = a[j];
+      if (q <= radius || q >= w - 1 - radius) acc = fold_edges(buf, 1, x0 - radius, w, q, tr, radius, acc);
+      const size_t i = (size_t)y * w + q;
+      out[i] = (c0 != 0.0f ? c0 * orig[i] : 0.0f) + c1 * acc;
+    }
   }
 }
 

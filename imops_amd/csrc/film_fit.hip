@@ -80,7 +80,10 @@ __device__ void block_partials(double (&v)[kGroup * kGroups], double* part_rows,
 // dL/dd_max, dL/dgamma, dL/dgain (3 x E).  f32 arithmetic on f32 copies of
 // the capture LUTs and spectral tables (staged in LDS), with the per-block
 // parameter sums in f64.
-template <int kE>
+// kL2: `gout` holds a target image instead; the kernel evaluates the model
+// output from the same per-pixel quantities (f32), takes dL/dout = 2 (out -
+// target) and adds the f64 sum of (out - target)^2 at kFitSlotLoss.
+template <int kE, bool kL2>
 __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __restrict__ S, const float* __restrict__ B,
                                                             const FilmDeviceTables* __restrict__ T,
                                                             const float* __restrict__ gout, size_t n,
@@ -113,6 +116,7 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
   gout += b * 3 * n;
   double part[3 * kE];
   for (int k = 0; k < 3 * kE; ++k) part[k] = 0.0;
+  double loss[1] = {0.0};
   const float ln10f = 2.30258512f, log2_10f = 3.32192802f;
   FIT_PIXELS(p, n) {
     float D[kE], M[kE], dDdf[kE], eff[kE], slope[kE];
@@ -162,21 +166,53 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
     }
     // reverse: out_r = sum_c X[r][c] xyz_c,  xyz_c = sum_i trap_i t_i cmf_c(i)
     float gx[3];
-    for (int c = 0; c < 3; ++c)
-      gx[c] = x2a_s[0][c] * gout[p * 3] + x2a_s[1][c] * gout[p * 3 + 1] + x2a_s[2][c] * gout[p * 3 + 2];
-    // per wavelength: the forward transmittance t = scanner 10^-dens, then
-    // dL/ddens = -ln10 t (gx . cmf) into the per-emulsion dye and mask sums
     float gDs[kE], gMs[kE];
     for (int e = 0; e < kE; ++e) gDs[e] = gMs[e] = 0.0f;
+    if constexpr (kL2) {
+      // forward scan first (each thread's t per wavelength parked in its own LDS
+      // column: fully unrolled register copies would hoist every table read out
+      // of the pixel loop and spill), then the loss gradient
+      __shared__ float tv[16][256];
+      float xyz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 2
-    for (int i = 0; i < 16; ++i) {
-      float d = 0.0f;
-      for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
-      const float t = exp2f(-d * log2_10f) * scan_s[i];
-      const float gd = -ln10f * t * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
-      for (int e = 0; e < kE; ++e) {
-        gDs[e] += gd * eps_s[e][i];
-        gMs[e] += gd * meps_s[e][i];
+      for (int i = 0; i < 16; ++i) {
+        float d = 0.0f;
+        for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
+        const float t = exp2f(-d * log2_10f) * scan_s[i];
+        tv[i][threadIdx.x] = t;
+        for (int c = 0; c < 3; ++c) xyz[c] += t * cmf_s[c][i];
+      }
+      float g[3];
+      for (int r = 0; r < 3; ++r) {
+        const float o = x2a_s[r][0] * xyz[0] + x2a_s[r][1] * xyz[1] + x2a_s[r][2] * xyz[2];
+        const float diff = o - gout[p * 3 + r];
+        loss[0] += (double)diff * (double)diff;
+        g[r] = 2.0f * diff;
+      }
+      for (int c = 0; c < 3; ++c) gx[c] = x2a_s[0][c] * g[0] + x2a_s[1][c] * g[1] + x2a_s[2][c] * g[2];
+#pragma unroll 2
+      for (int i = 0; i < 16; ++i) {
+        const float gd = -ln10f * tv[i][threadIdx.x] * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
+        for (int e = 0; e < kE; ++e) {
+          gDs[e] += gd * eps_s[e][i];
+          gMs[e] += gd * meps_s[e][i];
+        }
+      }
+    } else {
+      for (int c = 0; c < 3; ++c)
+        gx[c] = x2a_s[0][c] * gout[p * 3] + x2a_s[1][c] * gout[p * 3 + 1] + x2a_s[2][c] * gout[p * 3 + 2];
+      // per wavelength: the forward transmittance t = scanner 10^-dens, then
+      // dL/ddens = -ln10 t (gx . cmf) into the per-emulsion dye and mask sums
+#pragma unroll 2
+      for (int i = 0; i < 16; ++i) {
+        float d = 0.0f;
+        for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
+        const float t = exp2f(-d * log2_10f) * scan_s[i];
+        const float gd = -ln10f * t * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
+        for (int e = 0; e < kE; ++e) {
+          gDs[e] += gd * eps_s[e][i];
+          gMs[e] += gd * meps_s[e][i];
+        }
       }
     }
     float gB = 0.0f;
@@ -209,6 +245,7 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
     if (bbar) bbar[p] = gB;
   }
   block_partials<kE, 3>(part, part_rows, kFitSlotDmax);
+  if constexpr (kL2) block_partials<1, 1>(loss, part_rows, kFitSlotLoss);
 }
 
 // dL/dC[k][e] = sum_p w_k(p) Abar_e(p)
@@ -249,21 +286,22 @@ __global__ void fill_f64_kernel(double* p, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
-// acc[slot] = sum over the partial rows: kReduceGroups groups of kFitSlots
-// threads each add every kReduceGroups-th row, then group sums are added in
-// group order (a fixed order, whatever the timing).
-constexpr int kReduceGroups = 16;
-__global__ void __launch_bounds__(kFitSlots * kReduceGroups) fit_reduce_kernel(const double* __restrict__ part_rows,
-                                                                              int rows, double* __restrict__ acc) {
-  __shared__ double grp[kReduceGroups][kFitSlots];
-  const int slot = threadIdx.x % kFitSlots, g = threadIdx.x / kFitSlots;
+// acc[slot] = sum over the partial rows, one block per slot: thread t adds rows
+// t, t + 256, ... in order, then a fixed butterfly within each wave and the four
+// wave sums in wave order (a fixed order, whatever the timing).
+constexpr int kReduceThreads = 256;
+__global__ void __launch_bounds__(kReduceThreads) fit_reduce_kernel(const double* __restrict__ part_rows, int rows,
+                                                                    double* __restrict__ acc) {
+  __shared__ double wsum[kReduceThreads / 64];
+  const int slot = blockIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double s = 0.0;
-  for (int r = g; r < rows; r += kReduceGroups) s += part_rows[(size_t)r * kFitSlots + slot];
-  grp[g][slot] = s;
+  for (int r = threadIdx.x; r < rows; r += kReduceThreads) s += part_rows[(size_t)r * kFitSlots + slot];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) wsum[wave] = s;
   __syncthreads();
-  if (g == 0) {
+  if (threadIdx.x == 0) {
     double t = 0.0;
-    for (int k = 0; k < kReduceGroups; ++k) t += grp[k][slot];
+    for (int k = 0; k < kReduceThreads / 64; ++k) t += wsum[k];
     acc[slot] = t;
   }
 }
@@ -302,11 +340,15 @@ hipError_t with_emulsions(int emulsions, F&& f) {
 
 hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions,
                               const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, hipStream_t s,
-                              int images) {
+                              int images, bool l2_target) {
   const dim3 grid(fit_blocks_per_image(n, images), images);
   return with_emulsions(emulsions, [&](auto e) {
-    hipLaunchKernelGGL(fit_back_scan_kernel<decltype(e)::value>, grid, dim3(256), 0, s, S, B, T, gout, n, sbar, bbar,
-                       part_rows);
+    if (l2_target)
+      hipLaunchKernelGGL((fit_back_scan_kernel<decltype(e)::value, true>), grid, dim3(256), 0, s, S, B, T, gout, n,
+                         sbar, bbar, part_rows);
+    else
+      hipLaunchKernelGGL((fit_back_scan_kernel<decltype(e)::value, false>), grid, dim3(256), 0, s, S, B, T, gout, n,
+                         sbar, bbar, part_rows);
   });
 }
 
@@ -331,7 +373,7 @@ hipError_t film_fit_zero(double* p, size_t n, hipStream_t s) {
 }
 
 hipError_t film_fit_reduce(const double* part_rows, int rows, double* acc, hipStream_t s) {
-  hipLaunchKernelGGL(fit_reduce_kernel, dim3(1), dim3(kFitSlots * kReduceGroups), 0, s, part_rows, rows, acc);
+  hipLaunchKernelGGL(fit_reduce_kernel, dim3(kFitSlots), dim3(kReduceThreads), 0, s, part_rows, rows, acc);
   return hipGetLastError();
 }
 

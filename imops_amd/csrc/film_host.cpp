@@ -1144,71 +1144,97 @@ extern "C" int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um,
   });
 }
 
+namespace imops {
+namespace {
+
+// The reverse pass of one batched call: dL/dparams into *grad for dL/dout =
+// gout, or, with l2_target, for L = sum (out - gout)^2 (its value into *loss).
+// front: run the exposure and blurs first (no saved state).
+void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in, const float* gout, bool l2_target,
+                 bool front, imops_film_fit_params* grad, double* loss) {
+  imops_ctx* ctx = R.ctx;
+  hipStream_t s = ctx->stream;
+  const int E = R.E;
+  const size_t n = R.n, En = (size_t)E * n;
+  {
+    TimedRegion tr(ctx, l2_target ? "film_fit_l2_step" : "film_fit_backward");
+    IMOPS_HIP_CHECK(film_fit_zero(R.part, (size_t)R.rows * kFitSlots, s));
+    if (front) fit_front(R, rgb_in);
+    IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, E, gout, n, R.Sbar,
+                                       R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N, l2_target));
+    if (R.f.do_wide) {
+      const float* deep = R.S + (size_t)(E - 1) * n;
+      float* deep_bar = R.Sbar + (size_t)(E - 1) * n;
+      const float* k = R.taps + R.off_h;
+      const float* kd = R.taps + R.off_hd;
+      // d/dsigma of V(H(x)) = V'(H x) + V(H' x)
+      IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u1, R.w, R.h, k, kd, R.rh, s, BlurPlanes{R.N, En, 0, n}));
+      IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u2, R.w, R.h, kd, k, R.rh, s, BlurPlanes{R.N, En, 0, n}));
+      IMOPS_HIP_CHECK(film_fit_dot(R.Bbar, R.u1, R.u2, n, R.part, s, R.N));
+      IMOPS_HIP_CHECK(film_blur_adjoint(R.Bbar, R.tmp, deep_bar, deep_bar, R.w, R.h, k, R.rh, 1.0f, 1.0f, s,
+                                        BlurPlanes{R.N, n, En, En}));
+    }
+    if (R.f.do_local)
+      IMOPS_HIP_CHECK(film_blur_adjoint(R.Sbar, R.tmp, R.Sbar, R.Sbar, R.w, R.h, R.taps + R.off_l, R.rl,
+                                        1.0f - 0.18f, 0.18f, s, BlurPlanes{R.N * E, n, n, n}));
+    IMOPS_HIP_CHECK(film_fit_back_expose(rgb_in, R.Sbar, n, R.f.k, R.part, s, R.N));
+    IMOPS_HIP_CHECK(film_fit_reduce(R.part, R.rows, R.acc, s));
+  }
+  double acc[kFitSlots];
+  IMOPS_HIP_CHECK(hipMemcpyAsync(acc, R.acc, sizeof acc, hipMemcpyDeviceToHost, s));
+  IMOPS_HIP_CHECK(hipStreamSynchronize(s));
+  const FitSetup& f = R.f;
+  *grad = imops_film_fit_params{};
+  grad->n_emulsions = E;
+  double g_hs = 0.0;
+  for (int e = 0; e < E; ++e) {
+    grad->d_max[e] = (float)acc[e];
+    grad->gamma[e] = (float)acc[8 + e];
+    g_hs += acc[16 + e] * (double)f.rb[e];
+  }
+  grad->hal_strength = f.do_wide ? (float)g_hs : 0.0f;
+  grad->hal_radius = f.do_wide ? (float)(acc[48] * (double)f.sigma_h_unit) : 0.0f;
+  // sensitivities: through the exposure contraction C and the halation reflectance r_e
+  for (int e2 = 0; e2 < E; ++e2)
+    for (int i = 0; i < NB; ++i) {
+      double g = 0.0;
+      for (int k = 0; k < 3; ++k)
+        for (int e = 0; e < E; ++e) {
+          const double cbar = acc[24 + k * 8 + e];
+          double d = 0.0;
+          if (e2 == e) d += f.tab[e][i] * f.alpha[e] * f.te[e][i];
+          if (f.above[e2][e]) d -= f.alpha[e2] * f.tab[e][i] * (1.0 - f.te[e][i]);
+          g += cbar * f.W[k][i] * d;
+        }
+      if (f.do_wide && f.rden[e2] > 1e-9) {
+        const double tw = trap_w(i);
+        const double dr = tw * (R.hs->reflect[i] * f.rden[e2] - f.rnum[e2]) / (f.rden[e2] * f.rden[e2]);
+        g += acc[16 + e2] * (double)p.hal_strength * f.bleed[e2] * dr;
+      }
+      grad->sens[e2][i] = (float)g;
+    }
+  if (loss) *loss = acc[kFitSlotLoss];
+}
+
+}  // namespace
+}  // namespace imops
+
 extern "C" int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
                                        const float* rgb_in, const float* grad_out, size_t n_images, size_t width,
                                        size_t height, const float* state, imops_film_fit_params* grad) {
   return guarded([&] {
     if (!p || !rgb_in || !grad_out || !grad) throw Error{IMOPS_E_INVALID, "film_fit_backward: null argument"};
     FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, state);
-    hipStream_t s = ctx->stream;
-    const int E = R.E;
-    const size_t n = R.n, En = (size_t)E * n;
-    {
-      TimedRegion tr(ctx, "film_fit_backward");
-      IMOPS_HIP_CHECK(film_fit_zero(R.part, (size_t)R.rows * kFitSlots, s));
-      if (!state) fit_front(R, rgb_in);
-      IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, E, grad_out, n, R.Sbar,
-                                         R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N));
-      if (R.f.do_wide) {
-        const float* deep = R.S + (size_t)(E - 1) * n;
-        float* deep_bar = R.Sbar + (size_t)(E - 1) * n;
-        const float* k = R.taps + R.off_h;
-        const float* kd = R.taps + R.off_hd;
-        // d/dsigma of V(H(x)) = V'(H x) + V(H' x)
-        IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u1, R.w, R.h, k, kd, R.rh, s, BlurPlanes{R.N, En, 0, n}));
-        IMOPS_HIP_CHECK(film_blur2(deep, R.tmp, R.u2, R.w, R.h, kd, k, R.rh, s, BlurPlanes{R.N, En, 0, n}));
-        IMOPS_HIP_CHECK(film_fit_dot(R.Bbar, R.u1, R.u2, n, R.part, s, R.N));
-        IMOPS_HIP_CHECK(film_blur_adjoint(R.Bbar, R.tmp, deep_bar, deep_bar, R.w, R.h, k, R.rh, 1.0f, 1.0f, s,
-                                          BlurPlanes{R.N, n, En, En}));
-      }
-      if (R.f.do_local)
-        IMOPS_HIP_CHECK(film_blur_adjoint(R.Sbar, R.tmp, R.Sbar, R.Sbar, R.w, R.h, R.taps + R.off_l, R.rl,
-                                          1.0f - 0.18f, 0.18f, s, BlurPlanes{R.N * E, n, n, n}));
-      IMOPS_HIP_CHECK(film_fit_back_expose(rgb_in, R.Sbar, n, R.f.k, R.part, s, R.N));
-      IMOPS_HIP_CHECK(film_fit_reduce(R.part, R.rows, R.acc, s));
-    }
-    double acc[kFitSlots];
-    IMOPS_HIP_CHECK(hipMemcpyAsync(acc, R.acc, sizeof acc, hipMemcpyDeviceToHost, s));
-    IMOPS_HIP_CHECK(hipStreamSynchronize(s));
-    const FitSetup& f = R.f;
-    *grad = imops_film_fit_params{};
-    grad->n_emulsions = E;
-    double g_hs = 0.0;
-    for (int e = 0; e < E; ++e) {
-      grad->d_max[e] = (float)acc[e];
-      grad->gamma[e] = (float)acc[8 + e];
-      g_hs += acc[16 + e] * (double)f.rb[e];
-    }
-    grad->hal_strength = f.do_wide ? (float)g_hs : 0.0f;
-    grad->hal_radius = f.do_wide ? (float)(acc[48] * (double)f.sigma_h_unit) : 0.0f;
-    // sensitivities: through the exposure contraction C and the halation reflectance r_e
-    for (int e2 = 0; e2 < E; ++e2)
-      for (int i = 0; i < NB; ++i) {
-        double g = 0.0;
-        for (int k = 0; k < 3; ++k)
-          for (int e = 0; e < E; ++e) {
-            const double cbar = acc[24 + k * 8 + e];
-            double d = 0.0;
-            if (e2 == e) d += f.tab[e][i] * f.alpha[e] * f.te[e][i];
-            if (f.above[e2][e]) d -= f.alpha[e2] * f.tab[e][i] * (1.0 - f.te[e][i]);
-            g += cbar * f.W[k][i] * d;
-          }
-        if (f.do_wide && f.rden[e2] > 1e-9) {
-          const double tw = trap_w(i);
-          const double dr = tw * (R.hs->reflect[i] * f.rden[e2] - f.rnum[e2]) / (f.rden[e2] * f.rden[e2]);
-          g += acc[16 + e2] * (double)p->hal_strength * f.bleed[e2] * dr;
-        }
-        grad->sens[e2][i] = (float)g;
-      }
+    fit_reverse(R, *p, rgb_in, grad_out, false, !state, grad, nullptr);
+  });
+}
+
+extern "C" int imops_film_fit_l2_step(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+                                      const float* rgb_in, const float* target, size_t n_images, size_t width,
+                                      size_t height, imops_film_fit_params* grad, double* loss) {
+  return guarded([&] {
+    if (!p || !rgb_in || !target || !grad || !loss) throw Error{IMOPS_E_INVALID, "film_fit_l2_step: null argument"};
+    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, nullptr);
+    fit_reverse(R, *p, rgb_in, target, true, true, grad, loss);
   });
 }

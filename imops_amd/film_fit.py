@@ -7,6 +7,8 @@ characteristic curve (d_max, gamma per emulsion) and two halation multipliers
 (gain and PSF width).  Forward and reverse passes are the library's HIP
 kernels (`imops_film_fit_forward` / `imops_film_fit_backward`); `FilmFit`
 wraps them as a torch.autograd.Function so an optimiser can drive them.
+`fit` runs Adam on the L2 loss through `imops_film_fit_l2_step` (loss and
+gradient in one device pass per iteration) with the update on the host.
 
     model = FilmModel(im.Context(0))
     params = model.nominal()                       # dict of float32 CPU tensors
@@ -109,6 +111,26 @@ class FilmModel:
                                             C.byref(g)))  # returns after the device work (it reads the sums back)
         return g
 
+    def l2_step(self, rgb: torch.Tensor, target: torch.Tensor, p: FilmFitParams,
+                grad: FilmFitParams | None = None, order: bool = True) -> tuple[float, FilmFitParams]:
+        """L = sum (model(rgb) - target)^2 and dL/dparams in one device pass
+        (imops_film_fit_l2_step: the model output is evaluated inside the reverse kernel,
+        never written).  `grad`, when given, is filled in place.  order=False skips
+        ordering the library's stream after torch's (the caller has done it once for
+        inputs that no torch work changes since)."""
+        _check_images(rgb)
+        _check_images(target)
+        if target.shape != rgb.shape:
+            raise ValueError("film fit target must have the images' shape")
+        n, h, w, _ = rgb.shape
+        g = FilmFitParams() if grad is None else grad
+        loss = C.c_double()
+        if order or self._lib_stream is None:
+            self._after_torch(rgb.device)
+        check(lib().imops_film_fit_l2_step(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
+                                           target.data_ptr(), n, w, h, C.byref(g), C.byref(loss)))
+        return loss.value, g
+
     def __call__(self, rgb, sens, d_max, gamma, hal_strength, hal_radius) -> torch.Tensor:
         return FilmFit.apply(rgb, sens, d_max, gamma, hal_strength, hal_radius, self)
 
@@ -143,10 +165,110 @@ class FilmFit(torch.autograd.Function):
                 as_t(g.hal_radius, hr).reshape(hr.shape), None)
 
 
+# lower bounds of the parameters after every optimiser step
+LOWER = {"sens": 0.0, "d_max": 0.05, "gamma": 0.05, "hal_strength": 0.0, "hal_radius": 0.05}
+
+
+class Adam:
+    """torch.optim.Adam's update (default betas and eps, no weight decay) on one flat
+    float32 numpy vector: m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
+    theta -= lr / (1 - b1^t) * m / (sqrt(v) / sqrt(1 - b2^t) + eps)."""
+
+    def __init__(self, theta: np.ndarray, lr: float, betas=(0.9, 0.999), eps: float = 1e-8):
+        self.theta = theta
+        self.lr, self.b1, self.b2, self.eps = np.float32(lr), betas[0], betas[1], np.float32(eps)
+        self.m = np.zeros_like(theta)
+        self.v = np.zeros_like(theta)
+        self.t = 0
+
+    def step(self, g: np.ndarray) -> None:
+        self.t += 1
+        self.m += (g - self.m) * np.float32(1.0 - self.b1)
+        self.v *= np.float32(self.b2)
+        self.v += g * g * np.float32(1.0 - self.b2)
+        bc1 = 1.0 - self.b1 ** self.t
+        bc2_sqrt = np.float32((1.0 - self.b2 ** self.t) ** 0.5)
+        denom = np.sqrt(self.v) / bc2_sqrt + self.eps
+        self.theta -= np.float32(self.lr / bc1) * self.m / denom
+
+
+class FlatParams:
+    """The fitted parameters as one float32 vector (sens E x 16, d_max E, gamma E,
+    hal_strength, hal_radius) packed in place into one imops_film_fit_params, and the
+    gradient struct read back as a vector of the same layout."""
+
+    def __init__(self, model: FilmModel, init: dict):
+        E = self.E = model.n_emulsions
+        self.theta = np.concatenate([np.asarray(init[k].detach().cpu().numpy(), np.float32).ravel()
+                                     for k in PARAM_NAMES])
+        self.lower = np.concatenate([np.full(E * 16, LOWER["sens"]), np.full(E, LOWER["d_max"]),
+                                     np.full(E, LOWER["gamma"]), [LOWER["hal_strength"], LOWER["hal_radius"]]]
+                                    ).astype(np.float32)
+        self.p = model.pack(**init)
+        self.g = FilmFitParams()
+        self._p_sens = np.ctypeslib.as_array(self.p.sens)[:E]
+        self._p_dmax = np.ctypeslib.as_array(self.p.d_max)[:E]
+        self._p_gamma = np.ctypeslib.as_array(self.p.gamma)[:E]
+        self._g_sens = np.ctypeslib.as_array(self.g.sens)[:E]
+        self._g_dmax = np.ctypeslib.as_array(self.g.d_max)[:E]
+        self._g_gamma = np.ctypeslib.as_array(self.g.gamma)[:E]
+        self.grad = np.zeros_like(self.theta)
+
+    def pack(self) -> FilmFitParams:
+        E, t = self.E, self.theta
+        self._p_sens[:] = t[:16 * E].reshape(E, 16)
+        self._p_dmax[:] = t[16 * E:17 * E]
+        self._p_gamma[:] = t[17 * E:18 * E]
+        self.p.hal_strength = float(t[18 * E])
+        self.p.hal_radius = float(t[18 * E + 1])
+        return self.p
+
+    def read_grad(self) -> np.ndarray:
+        E, g = self.E, self.grad
+        g[:16 * E] = self._g_sens.ravel()
+        g[16 * E:17 * E] = self._g_dmax
+        g[17 * E:18 * E] = self._g_gamma
+        g[18 * E] = self.g.hal_strength
+        g[18 * E + 1] = self.g.hal_radius
+        return g
+
+    def as_dict(self) -> dict:
+        E, t = self.E, self.theta
+        return {"sens": torch.from_numpy(t[:16 * E].reshape(E, 16).copy()),
+                "d_max": torch.from_numpy(t[16 * E:17 * E].copy()),
+                "gamma": torch.from_numpy(t[17 * E:18 * E].copy()),
+                "hal_strength": torch.tensor(float(t[18 * E])),
+                "hal_radius": torch.tensor(float(t[18 * E + 1]))}
+
+
 def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, steps: int = 200, lr: float = 0.01,
-        log_every: int = 0) -> dict:
-    """Adam on the L2 loss between the model output and `target`; returns the fitted
-    parameters, the loss history and the mean forward+backward time per step (ms)."""
+        log_every: int = 0, fused: bool = True) -> dict:
+    """Adam on the L2 loss between the model output and `target`, the parameters clamped
+    to LOWER after each step; returns the fitted parameters, the loss history (the loss
+    at each step's parameters before its update) and the median wall time of one
+    iteration (ms; forward, adjoint and optimiser step).
+
+    fused: each iteration is one imops_film_fit_l2_step (loss and gradient in one device
+    pass, one synchronisation) followed by Adam on the host on a 16E + 2E + 2 vector;
+    otherwise the forward, a torch loss, autograd through FilmFit and torch.optim.Adam."""
+    if not fused:
+        return _fit_autograd(model, rgb, target, init, steps, lr, log_every)
+    fp = FlatParams(model, init)
+    opt = Adam(fp.theta, lr)
+    history, times = [], []
+    for it in range(steps):
+        t0 = time.perf_counter()
+        loss, _ = model.l2_step(rgb, target, fp.pack(), fp.g, order=it == 0)
+        opt.step(fp.read_grad())
+        np.maximum(fp.theta, fp.lower, out=fp.theta)
+        times.append(time.perf_counter() - t0)
+        history.append(loss)
+        if log_every and it % log_every == 0:
+            print(f"[fit] step {it} loss {loss:.6g}", flush=True)
+    return {"params": fp.as_dict(), "loss": history, "ms_per_iter": 1e3 * float(np.median(times)) if times else 0.0}
+
+
+def _fit_autograd(model, rgb, target, init, steps, lr, log_every) -> dict:
     params = {k: v.detach().clone().requires_grad_(True) for k, v in init.items()}
     opt = torch.optim.Adam(params.values(), lr=lr)
     history, times = [], []
@@ -156,17 +278,14 @@ def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, s
         out = model(rgb, **params)
         loss = ((out - target) ** 2).sum()
         loss.backward()
-        times.append(time.perf_counter() - t0)
         opt.step()
         with torch.no_grad():
-            params["sens"].clamp_(min=0.0)
-            params["gamma"].clamp_(min=0.05)
-            params["d_max"].clamp_(min=0.05)
-            params["hal_strength"].clamp_(min=0.0)
-            params["hal_radius"].clamp_(min=0.05)
+            for k, v in params.items():
+                v.clamp_(min=LOWER[k])
+        times.append(time.perf_counter() - t0)
         history.append(loss.detach())  # read back once at the end, not every step
         if log_every and it % log_every == 0:
             print(f"[fit] step {it} loss {float(history[-1]):.6g}", flush=True)
     history = torch.stack(history).cpu().tolist() if history else []
     return {"params": {k: v.detach() for k, v in params.items()}, "loss": history,
-            "ms_per_iter": 1e3 * float(np.median(times))}
+            "ms_per_iter": 1e3 * float(np.median(times)) if times else 0.0}

@@ -361,6 +361,16 @@ int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imop
 int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
                             const float* rgb_in, const float* grad_out, size_t n_images, size_t width, size_t height,
                             const float* state, imops_film_fit_params* grad);
+/* One fitting step of the L2 loss L = sum over images, pixels and channels of
+ * (model(rgb_in) - target)^2: the exposure and blurs, then the reverse pass,
+ * whose per-pixel kernel evaluates the model output (f32) from the quantities
+ * it already forms, so the output image is never written.  Returns L in *loss
+ * and dL/dparams in *grad (the same gradient as imops_film_fit_backward with
+ * grad_out = 2 (out - target), up to the f32 evaluation of out); one
+ * synchronisation, for the readback.  No counterpart in the reference. */
+int imops_film_fit_l2_step(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+                           const float* rgb_in, const float* target, size_t n_images, size_t width, size_t height,
+                           imops_film_fit_params* grad, double* loss);
 
 /* Pipeline --------------------------------------------------------------- */
 

@@ -114,15 +114,59 @@ def test_gradients_match_fp64_central_differences(model):
     assert rel.max() <= 1e-3 and vec <= 1e-3
 
 
-def test_fit_reduces_loss(model):
+@pytest.mark.parametrize("fused", [True, False])
+def test_fit_reduces_loss(model, fused):
     imgs = _images(2, 96, 88, 7)
     hidden = _perturbed(model.nominal(), 41, 1.0)
     x = torch.from_numpy(imgs).cuda()
     with torch.no_grad():
         target = model(x, **hidden)
-    res = fit(model, x, target, model.nominal(), steps=30, lr=0.01)
-    print(f"loss {res['loss'][0]:.4g} -> {res['loss'][-1]:.4g}, {res['ms_per_iter']:.2f} ms/iter", flush=True)
+    res = fit(model, x, target, model.nominal(), steps=30, lr=0.01, fused=fused)
+    print(f"fused={fused}: loss {res['loss'][0]:.4g} -> {res['loss'][-1]:.4g}, {res['ms_per_iter']:.2f} ms/iter",
+          flush=True)
     assert res["loss"][-1] < 0.5 * res["loss"][0]
+
+
+def test_fused_and_autograd_fits_follow_one_trajectory(model):
+    """The fused loop (one imops_film_fit_l2_step per iteration, host Adam) and the
+    autograd loop (forward, torch loss, FilmFit backward, torch.optim.Adam) take the same
+    steps: their losses and final parameters agree to the f32 evaluation of the output."""
+    imgs = _images(2, 96, 88, 17)
+    hidden = _perturbed(model.nominal(), 43, 1.0)
+    x = torch.from_numpy(imgs).cuda()
+    with torch.no_grad():
+        target = model(x, **hidden)
+    a = fit(model, x, target, model.nominal(), steps=15, lr=0.01, fused=True)
+    b = fit(model, x, target, model.nominal(), steps=15, lr=0.01, fused=False)
+    la, lb = np.array(a["loss"]), np.array(b["loss"])
+    print(f"loss rel diff {np.abs(la - lb).max() / lb.max():.2e}", flush=True)
+    np.testing.assert_allclose(la, lb, rtol=1e-4, atol=1e-6 * lb.max())
+    for k in a["params"]:
+        np.testing.assert_allclose(a["params"][k].numpy(), b["params"][k].numpy(), rtol=1e-4, atol=1e-5)
+
+
+def test_l2_step_matches_forward_and_backward(model):
+    """imops_film_fit_l2_step returns the L2 loss of the forward's output against the
+    target and the backward's gradient for dL/dout = 2 (out - target); the step evaluates
+    the output in f32 inside the reverse kernel, so both agree to f32 rounding.  Two images
+    of different content, plus a shape whose pixel count is not a multiple of the block."""
+    E = model.n_emulsions
+    for n, w, h in ((2, 96, 88), (1, 131, 77)):
+        imgs = _images(n, w, h, 21)
+        x = torch.from_numpy(imgs).cuda()
+        p_hidden = model.pack(**_perturbed(model.nominal(), 51, 1.0))
+        target = model.forward_raw(x, p_hidden)
+        p = model.pack(**_perturbed(model.nominal(), 53, 0.5))
+        out = model.forward_raw(x, p)
+        want_loss = float(((out.double() - target.double()) ** 2).sum())
+        want_g = _grad_vector(model.backward_raw(x, (2.0 * (out - target)).contiguous(), p), E)
+        loss, g = model.l2_step(x, target, p)
+        got_g = _grad_vector(g, E)
+        rel = np.abs(got_g - want_g).max() / np.abs(want_g).max()
+        print(f"{n}x{w}x{h}: loss {loss:.8g} vs {want_loss:.8g}, gradient max rel {rel:.2e}", flush=True)
+        assert abs(loss - want_loss) <= 1e-5 * want_loss
+        assert rel <= 1e-4
+        assert np.linalg.norm(got_g - want_g) <= 1e-5 * np.linalg.norm(want_g)
 
 
 def _grad_vector(g, E):
@@ -213,3 +257,17 @@ def test_gradients_at_the_c4_shape_match_fp64_central_differences(model):
         r = abs(g - fd) / abs(fd)
         print(f"direction {k}: adjoint {g:.6g} fd {fd:.6g} rel {r:.2e}", flush=True)
         assert r <= 1e-3
+    # the fused L2 step at the same point: its loss against the oracle's, its gradient
+    # against the same central differences
+    loss, g = model.l2_step(x, t, model.pack(**theta))
+    want = oracle_loss(theta)
+    gv = {"d_max": np.ctypeslib.as_array(g.d_max)[:E], "gamma": np.ctypeslib.as_array(g.gamma)[:E],
+          "hal_strength": np.float32(g.hal_strength), "hal_radius": np.float32(g.hal_radius),
+          "sens": np.ctypeslib.as_array(g.sens)[:E]}
+    fused = np.array([float(np.asarray(gv[name])[idx]) for name, idx in checks])
+    rel = np.abs(fused - fds) / np.maximum(np.abs(fds), floor)
+    vec = np.linalg.norm(fused - fds) / np.linalg.norm(fds)
+    print(f"fused step: loss {loss:.8g} vs oracle {want:.8g}; worst entry {rel.max():.2e}, vector {vec:.2e}",
+          flush=True)
+    assert abs(loss - want) <= 1e-4 * want
+    assert rel.max() <= 1e-3 and vec <= 1e-3

@@ -145,3 +145,28 @@ def test_synth_deterministic_and_in_range():
     sc = im.synth_scene_rgb(512, 512, 1)
     pred = 600 + sc[0::2, 0::2, 0] * 15783
     assert abs((big[0::2, 0::2] - pred).mean()) < 2.0
+
+
+def test_fit_adam_matches_torch_adam():
+    """The film fit's host optimiser takes torch.optim.Adam's steps (default betas and eps)
+    on the same gradient sequence, including the clamp to the lower bounds after a step."""
+    import torch
+    from imops_amd.film_fit import Adam
+
+    rng = np.random.default_rng(11)
+    theta0 = rng.uniform(0.0, 2.0, 50).astype(np.float32)
+    lower = np.full(50, 0.05, np.float32)
+    ours = theta0.copy()
+    opt = Adam(ours, lr=0.01)
+    ref = torch.tensor(theta0.copy(), requires_grad=True)
+    topt = torch.optim.Adam([ref], lr=0.01, foreach=False)
+    for _ in range(200):
+        g = rng.normal(size=50).astype(np.float32) * np.float32(rng.uniform(0.01, 100.0))
+        opt.step(g)
+        np.maximum(ours, lower, out=ours)
+        ref.grad = torch.from_numpy(g.copy())
+        topt.step()
+        with torch.no_grad():
+            ref.clamp_(min=0.05)
+    np.testing.assert_allclose(ours, ref.detach().numpy(), rtol=2e-5, atol=2e-6)
+    assert not np.array_equal(ours, theta0)
