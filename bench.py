@@ -230,22 +230,32 @@ def io_rates(d_raw, img) -> dict:
             "note": "pinned host buffers; raw u16 in (48 MB), f32 sRGB out (290 MB); value excludes them"}
 
 
-def film_adjoint_bench(ctx, n=8, size=512, steps=200) -> dict:
-    """Film-model adjoint: 8 ACEScg 512x512 crops of seeded scenes, target rendered with
-    hidden perturbed parameters, Adam from the nominal stock on the L2 loss.  Reports the
-    forward + adjoint time per iteration (HIP events) and the loss reduction."""
+def film_adjoint_bench(ctx, n=8, size=512, steps=200, stock="Synthetic", bins=81, stock_seed=7) -> dict:
+    """Film-model adjoint (the C4 workload by default): 8 linear-XYZ 512x512 crops of seeded
+    scenes, taken to ACEScg by the library's CST stage; the seeded synthetic stock on the
+    81-bin grid, parameters 3 x 81 sensitivities, 3 x 5 curve knots, d_max, gamma and the
+    halation radius / strength; target rendered with hidden seeded parameters; Adam from
+    the nominal stock on the L2 loss.  Reports the time per iteration of the fused loop
+    and of the autograd loop (HIP events around the whole loop) and the loss reduction."""
     import torch
     import imops_amd as im
     from imops_amd.film_fit import FilmModel, fit
     g = 12.5 * 64.0 / ((1.0 / 200.0) * 200.0)  # sunny-16 absolute luminance of relative 1.0 at ISO 200
-    x = torch.from_numpy(np.stack([im.synth_scene_rgb(size, size, 100 + i) * np.float32(g) for i in range(n)])).cuda()
-    model = FilmModel(ctx)
+    crops = []
+    for i in range(n):
+        xyz = (im.synth_scene_rgb(size, size, 100 + i) * np.float32(g)).astype(np.float32)
+        crops.append(im.Image(ctx).set_rgb(xyz, im.CS_XYZD65).cst("AcesCg").rgb())
+    x = torch.from_numpy(np.stack(crops)).cuda()
+    model = FilmModel(ctx, stock, spectral_bins=bins, stock_seed=stock_seed)
     nom = model.nominal()
     rng = np.random.default_rng(7)
     hidden = {"sens": nom["sens"] * torch.tensor(1 + 0.15 * rng.uniform(-1, 1, nom["sens"].shape), dtype=torch.float32),
               "d_max": nom["d_max"] + 0.2 * torch.tensor(rng.uniform(-1, 1, nom["d_max"].shape), dtype=torch.float32),
               "gamma": nom["gamma"] * torch.tensor(1 + 0.1 * rng.uniform(-1, 1, nom["gamma"].shape), dtype=torch.float32),
               "hal_strength": torch.tensor(1.3), "hal_radius": torch.tensor(0.95)}
+    if "knots" in nom:
+        hidden["knots"] = (nom["knots"] + 0.03 * torch.tensor(rng.uniform(-1, 1, nom["knots"].shape),
+                                                              dtype=torch.float32)).clamp(0.001, 0.999)
     with torch.no_grad():
         target = model(x, **hidden)
     stream = torch.cuda.ExternalStream(ctx.stream)
@@ -268,19 +278,24 @@ def film_adjoint_bench(ctx, n=8, size=512, steps=200) -> dict:
     E = model.n_emulsions
     px = n * size * size
     step_bytes = px * (12 + 4 * (E + 1) * 2 + 12 + 12)
+    nparams = sum(int(np.prod(v.shape)) for v in nom.values())
     return {"metric": "film-model adjoint ms/iter (forward + hand-written adjoint + Adam step)",
             "ms_per_iter": round(ms, 3),
             "form": "fused: one imops_film_fit_l2_step per iteration (exposure, blurs, reverse pass with the "
                     "model output and L2 loss evaluated in its per-pixel kernel), Adam on the host",
+            "model": f"{stock} stock{f' (seed {stock_seed})' if stock == 'Synthetic' else ''}, {bins} bins, "
+                     f"{f'{model.n_knots}-knot curves' if model.n_knots else 'crystal-LUT curves'}, "
+                     f"{nparams} parameters ({' + '.join(f'{k} {tuple(v.shape)}' for k, v in nom.items())})",
             "compulsory_bytes_per_iter": step_bytes,
             "achieved_GBps": round(step_bytes / (ms * 1e-3) / 1e9, 1),
             "ms_per_iter_wall_median": round(res["ms_per_iter"], 3),
             "ms_per_iter_autograd": round(ms_auto, 3),
             "autograd_form": "forward (FP64 capture and scan) -> torch L2 loss -> FilmFit backward -> torch.optim.Adam",
-            "batch": f"{n}x{size}x{size} ACEScg crops", "steps": steps,
+            "batch": f"{n}x{size}x{size} linear-XYZ crops -> CST{{AcesCg}}", "steps": steps,
             "loss_first": res["loss"][0], "loss_last": res["loss"][-1],
             "loss_last_autograd": res_auto["loss"][-1],
-            "gradient_check": "tests/test_gpu_film_fit.py: vector relative error vs FP64 central differences"}
+            "gradient_check": "tests/test_gpu_film_fit.py: against FP64 central differences and the oracle's "
+                              "FP64 reverse pass at this shape"}
 
 
 XTRANS = "GGRGGB" "GGBGGR" "BRGRBG" "GGBGGR" "GGRGGB" "RBGBRG"
@@ -399,6 +414,7 @@ def main() -> int:
     ctx = im.Context(0)
     if args.adjoint_only:
         print(json.dumps(film_adjoint_bench(ctx, steps=args.adjoint_only)), flush=True)
+        print(json.dumps(film_adjoint_bench(ctx, steps=args.adjoint_only, stock="ColorNeg200", bins=16)), flush=True)
         return 0
     ctx.set_autotune(True)  # launch shapes timed during the warm-up steps; results do not depend on them
     if args.copy_probe_mib:
@@ -530,6 +546,7 @@ def main() -> int:
     out["io"]["pcie_inclusive_MPps"] = round(W * H / 1e6 / (ms * 1e-3 + (W * H * 2 / out["io"]["h2d_GBps"]
                                                                    + W * H * 12 / out["io"]["d2h_GBps"]) / 1e9), 3)
     out["film_adjoint"] = film_adjoint_bench(ctx)
+    out["film_adjoint_colorneg200_16bin"] = film_adjoint_bench(ctx, stock="ColorNeg200", bins=16)
     if not args.no_extra:
         out["xtrans"] = xtrans_bench(ctx, cal)
         out["burst"] = burst_bench(cal)

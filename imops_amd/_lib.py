@@ -46,10 +46,16 @@ class Bm3dParams(C.Structure):
                 ("max_candidates", C.c_int), ("precision", C.c_int)]
 
 
+class FilmFitModel(C.Structure):
+    """imops_film_fit_model (include/imops_gpu.h)."""
+    _fields_ = [("stock", C.c_int), ("spectral_bins", C.c_int), ("stock_seed", C.c_uint64), ("pitch_um", C.c_float)]
+
+
 class FilmFitParams(C.Structure):
     """imops_film_fit_params (include/imops_gpu.h)."""
-    _fields_ = [("n_emulsions", C.c_int), ("sens", (C.c_float * 16) * 8), ("d_max", C.c_float * 8),
-                ("gamma", C.c_float * 8), ("hal_strength", C.c_float), ("hal_radius", C.c_float)]
+    _fields_ = [("n_emulsions", C.c_int), ("n_bins", C.c_int), ("n_knots", C.c_int), ("sens", (C.c_float * 81) * 8),
+                ("knot_frac", (C.c_float * 8) * 8), ("d_max", C.c_float * 8), ("gamma", C.c_float * 8),
+                ("hal_strength", C.c_float), ("hal_radius", C.c_float)]
 
 
 class FilmParams(C.Structure):
@@ -107,13 +113,15 @@ SIGNATURES = {
     "imops_film": (_i, [_vp, _vp, _i, _i, C.c_uint64, _i]),
     "imops_film_default_params": (_i, [C.POINTER(FilmParams)]),
     "imops_film_ex": (_i, [_vp, _vp, C.POINTER(FilmParams)]),
-    "imops_film_fit_nominal": (_i, [_i, C.POINTER(FilmFitParams)]),
-    "imops_film_fit_state_floats": (_i, [_i, _sz, _sz, _sz, C.POINTER(_sz)]),
-    "imops_film_fit_forward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp]),
-    "imops_film_fit_backward": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz, _vp,
-                                     C.POINTER(FilmFitParams)]),
-    "imops_film_fit_l2_step": (_i, [_vp, _i, _f, C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz,
-                                    C.POINTER(FilmFitParams), C.POINTER(C.c_double)]),
+    "imops_film_fit_nominal": (_i, [C.POINTER(FilmFitModel), C.POINTER(FilmFitParams)]),
+    "imops_film_fit_knots": (_i, [C.POINTER(FilmFitModel), _i, C.POINTER(C.c_double)]),
+    "imops_film_fit_state_floats": (_i, [C.POINTER(FilmFitModel), _sz, _sz, _sz, C.POINTER(_sz)]),
+    "imops_film_fit_forward": (_i, [_vp, C.POINTER(FilmFitModel), C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz, _sz,
+                                    _vp]),
+    "imops_film_fit_backward": (_i, [_vp, C.POINTER(FilmFitModel), C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz,
+                                     _sz, _vp, C.POINTER(FilmFitParams)]),
+    "imops_film_fit_l2_step": (_i, [_vp, C.POINTER(FilmFitModel), C.POINTER(FilmFitParams), _vp, _vp, _sz, _sz,
+                                    _sz, C.POINTER(FilmFitParams), C.POINTER(C.c_double)]),
     "imops_pipeline_parse": (_i, [C.c_char_p, _sz, C.POINTER(_vp)]),
     "imops_pipeline_free": (_i, [_vp]),
     "imops_pipeline_num_stages": (_sz, [_vp]),

@@ -67,10 +67,13 @@ struct FilmDeviceTables {
 // each reducing kernel stores one row of kFitSlots partial sums per block
 // (d_max, gamma, gain per emulsion at kFitSlotDmax + {0, 8, 16}; the exposure
 // contraction C[k][e] at kFitSlotC + 8 k + e; the halation width at
-// kFitSlotSigma; the L2 loss at kFitSlotLoss), and film_fit_reduce adds the
+// kFitSlotSigma; the L2 loss at kFitSlotLoss; the developable fraction at knot
+// k of a knot curve at kFitSlotKnot + 8 k + e), and film_fit_reduce adds the
 // rows in order into acc.
-constexpr int kFitSlots = 64;
-constexpr int kFitSlotDmax = 0, kFitSlotC = 24, kFitSlotSigma = 48, kFitSlotLoss = 49;
+constexpr int kFitSlots = 128;
+constexpr int kFitSlotDmax = 0, kFitSlotC = 24, kFitSlotSigma = 48, kFitSlotLoss = 49, kFitSlotKnot = 64;
+constexpr int kFitMaxKnots = 8;
+constexpr int kFitMaxEmulsionsExt = 4;  // emulsions of the 81-bin / knot-curve reverse kernels
 constexpr int kFitReduceRows = 1024;  // blocks of a reducing launch (whole batch), when the pixels allow
 struct FitExposeConsts {
   double minv[3][3];                 // ACEScg -> basis weights
@@ -92,11 +95,13 @@ hipError_t film_blur_adjoint(const float* in, float* tmp, const float* orig, flo
 // Rows of partial sums a reducing launch over `images` images of n pixels writes.
 int fit_partial_rows(size_t n, int images);
 // S, sbar: E planes per image; B, bbar: one plane per image; gout: 3n per image
-// (emulsions = T->n_emulsions, 1..kFilmMaxEmulsions).  l2_target: gout is a
-// target image and dL/dout = 2 (model output - target), the loss summed at kFitSlotLoss.
-hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions,
-                              const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, hipStream_t s,
-                              int images = 1, bool l2_target = false);
+// (emulsions = T->n_emulsions, 1..kFilmMaxEmulsions; bins = T->n_bins, 16 or
+// 81; knots: 0 for the crystal LUT, 5 for a 5-knot curve; the 81-bin grid and
+// the knot curve take up to kFitMaxEmulsionsExt emulsions).  l2_target: gout
+// is a target image and dL/dout = 2 (model output - target), the loss summed at kFitSlotLoss.
+hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions, int bins,
+                              int knots, const float* gout, size_t n, float* sbar, float* bbar, double* part_rows,
+                              hipStream_t s, int images = 1, bool l2_target = false);
 hipError_t film_fit_back_expose(const float* rgb, const float* abar, size_t n, const FitExposeConsts& k,
                                 double* part_rows, hipStream_t s, int images = 1);
 // sum_p a (b + c) over one plane per image (planes of consecutive images n apart)

@@ -78,34 +78,41 @@ __device__ void block_partials(double (&v)[kGroup * kGroups], double* part_rows,
 // Reverse of capture + reduction + scan for one image of a batch.  In: S planes,
 // bounce B, dL/dout.  Out: dL/dS planes (into sbar), dL/dB, and f64 sums of
 // dL/dd_max, dL/dgamma, dL/dgain (3 x E).  f32 arithmetic on f32 copies of
-// the capture LUTs and spectral tables (staged in LDS), with the per-block
+// the capture curves and spectral tables (staged in LDS), with the per-block
 // parameter sums in f64.
+// kNB: bins of the spectral grid (16 or 81).
+// kK: 0 for the 64-entry crystal LUT, else the knots of a knot curve; then the
+// f64 sums of dL/d(developable fraction at knot k) go to kFitSlotKnot + 8 k + e
+// (linear interpolation in log10 fluence: weights 1 - t and t at the bracketing
+// knots, weight 1 at the clamped end).
 // kL2: `gout` holds a target image instead; the kernel evaluates the model
 // output from the same per-pixel quantities (f32), takes dL/dout = 2 (out -
 // target) and adds the f64 sum of (out - target)^2 at kFitSlotLoss.
-template <int kE, bool kL2>
+template <int kE, bool kL2, int kNB, int kK>
 __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __restrict__ S, const float* __restrict__ B,
                                                             const FilmDeviceTables* __restrict__ T,
                                                             const float* __restrict__ gout, size_t n,
                                                             float* __restrict__ sbar, float* __restrict__ bbar,
                                                             double* __restrict__ part_rows) {
-  __shared__ float eps_s[kFilmMaxEmulsions][16], meps_s[kFilmMaxEmulsions][16], cmf_s[3][16], scan_s[16], x2a_s[3][3];
-  __shared__ float llog_s[kFilmMaxEmulsions][64], lfrac_s[kFilmMaxEmulsions][64];
+  constexpr int kN = kK > 0 ? kK : 64;  // curve samples
+  __shared__ float eps_s[kE][kNB], meps_s[kE][kNB], cmf_s[3][kNB], scan_s[kNB], x2a_s[3][3];
+  __shared__ float llog_s[kE][kN], lfrac_s[kE][kN];
   constexpr int E = kE;  // == T->n_emulsions
-  for (int i = threadIdx.x; i < E * 64; i += blockDim.x) {
-    llog_s[i / 64][i % 64] = (float)T->em[i / 64].lut_log[i % 64];
-    lfrac_s[i / 64][i % 64] = (float)T->em[i / 64].lut_frac[i % 64];
+  for (int i = threadIdx.x; i < E * kN; i += blockDim.x) {
+    llog_s[i / kN][i % kN] = (float)T->em[i / kN].lut_log[i % kN];
+    lfrac_s[i / kN][i % kN] = (float)T->em[i / kN].lut_frac[i % kN];
   }
-  for (int i = threadIdx.x; i < kFilmMaxEmulsions * 16; i += blockDim.x) {
-    const int e = i / 16, l = i % 16;
-    eps_s[e][l] = e < E ? (float)T->em[e].eps[l] : 0.0f;
-    meps_s[e][l] = e < E && T->em[e].has_mask ? (float)T->em[e].mask_eps[l] : 0.0f;
+  for (int i = threadIdx.x; i < kE * kNB; i += blockDim.x) {
+    const int e = i / kNB, l = i % kNB;
+    eps_s[e][l] = (float)T->em[e].eps[l];
+    meps_s[e][l] = T->em[e].has_mask ? (float)T->em[e].mask_eps[l] : 0.0f;
   }
-  for (int i = threadIdx.x; i < 48; i += blockDim.x) {
-    const float tw = (i % 16 == 0 || i % 16 == 15) ? 10.0f : 20.0f;  // trapezoid weights of the scan integral
-    cmf_s[i / 16][i % 16] = tw * (float)T->cmf[i / 16][i % 16];
+  for (int i = threadIdx.x; i < 3 * kNB; i += blockDim.x) {
+    const int c = i / kNB, l = i % kNB;
+    const double tw = (l == 0 || l == kNB - 1) ? 0.5 * T->dl : T->dl;  // trapezoid weights of the scan integral
+    cmf_s[c][l] = (float)(tw * T->cmf[c][l]);
   }
-  if (threadIdx.x < 16) scan_s[threadIdx.x] = (float)T->scanner[threadIdx.x];
+  for (int i = threadIdx.x; i < kNB; i += blockDim.x) scan_s[i] = (float)T->scanner[i];
   if (threadIdx.x < 9) x2a_s[threadIdx.x / 3][threadIdx.x % 3] = (float)(T->xyz2ap1[threadIdx.x / 3][threadIdx.x % 3] * T->dmin_scale);
   __syncthreads();
   const size_t b = blockIdx.y;  // image b of a batch
@@ -116,10 +123,14 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
   gout += b * 3 * n;
   double part[3 * kE];
   for (int k = 0; k < 3 * kE; ++k) part[k] = 0.0;
+  double kpart[kK > 0 ? kK * kE : 1];
+  for (int k = 0; k < (kK > 0 ? kK * kE : 1); ++k) kpart[k] = 0.0;
   double loss[1] = {0.0};
   const float ln10f = 2.30258512f, log2_10f = 3.32192802f;
   FIT_PIXELS(p, n) {
     float D[kE], M[kE], dDdf[kE], eff[kE], slope[kE];
+    int klo[kE];
+    float kt[kE];
     const float bv = B ? B[p] : 0.0f;
     for (int e = 0; e < E; ++e) {
       const FilmEmulsionTables& Em = T->em[e];
@@ -128,27 +139,33 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
       const float* ll = llog_s[e];
       const float* lf = lfrac_s[e];
       float fr, sl = 0.0f;
+      int lo = 0;
+      float t = 0.0f;
       if (!(x > 0.0f)) {
         fr = lf[0];
       } else {
         const float lp = log10f(x);
         if (lp <= ll[0]) {
           fr = lf[0];
-        } else if (lp >= ll[63]) {
-          fr = lf[63];
+        } else if (lp >= ll[kN - 1]) {
+          fr = lf[kN - 1];
+          lo = kN - 2;
+          t = 1.0f;
         } else {
-          int lo = 0, hi = 63;
+          int hi = kN - 1;
           while (hi - lo > 1) {
             const int mid = (lo + hi) / 2;
             if (ll[mid] <= lp) lo = mid;
             else hi = mid;
           }
           const float dl = ll[hi] - ll[lo];
-          const float t = (lp - ll[lo]) / dl;
+          t = (lp - ll[lo]) / dl;
           fr = lf[lo] * (1.0f - t) + lf[hi] * t;
           sl = (lf[hi] - lf[lo]) / dl / (x * ln10f) * Em.eta;  // df/dP
         }
       }
+      klo[e] = lo;
+      kt[e] = t;
       slope[e] = sl;
       const bool inside = fr > 0.0f && fr < 1.0f;
       const float c = fr < 0.0f ? 0.0f : (fr > 1.0f ? 1.0f : fr);
@@ -169,18 +186,26 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
     float gDs[kE], gMs[kE];
     for (int e = 0; e < kE; ++e) gDs[e] = gMs[e] = 0.0f;
     if constexpr (kL2) {
-      // forward scan first (each thread's t per wavelength parked in its own LDS
-      // column: fully unrolled register copies would hoist every table read out
-      // of the pixel loop and spill), then the loss gradient
-      __shared__ float tv[16][256];
-      float xyz[3] = {0.0f, 0.0f, 0.0f};
+      // one pass over the grid: xyz_c = sum_i u_c(i) with u_c(i) = trap_i t_i cmf_c(i),
+      // and the per-channel parts of the reverse, K_c,e = sum_i u_c(i) eps_e(i) (and the
+      // mask's), so that once the residual gives gx after the loop,
+      // dL/dD_e = -ln10 sum_c gx_c K_c,e (one 10^-d per bin, every table read once)
+      float xyz[3] = {0.0f, 0.0f, 0.0f}, Kd[3][kE], Km[3][kE];
+      for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < kE; ++e) Kd[c][e] = Km[c][e] = 0.0f;
 #pragma unroll 2
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < kNB; ++i) {
         float d = 0.0f;
         for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
         const float t = exp2f(-d * log2_10f) * scan_s[i];
-        tv[i][threadIdx.x] = t;
-        for (int c = 0; c < 3; ++c) xyz[c] += t * cmf_s[c][i];
+        for (int c = 0; c < 3; ++c) {
+          const float u = t * cmf_s[c][i];
+          xyz[c] += u;
+          for (int e = 0; e < kE; ++e) {
+            Kd[c][e] += u * eps_s[e][i];
+            Km[c][e] += u * meps_s[e][i];
+          }
+        }
       }
       float g[3];
       for (int r = 0; r < 3; ++r) {
@@ -190,13 +215,9 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
         g[r] = 2.0f * diff;
       }
       for (int c = 0; c < 3; ++c) gx[c] = x2a_s[0][c] * g[0] + x2a_s[1][c] * g[1] + x2a_s[2][c] * g[2];
-#pragma unroll 2
-      for (int i = 0; i < 16; ++i) {
-        const float gd = -ln10f * tv[i][threadIdx.x] * (gx[0] * cmf_s[0][i] + gx[1] * cmf_s[1][i] + gx[2] * cmf_s[2][i]);
-        for (int e = 0; e < kE; ++e) {
-          gDs[e] += gd * eps_s[e][i];
-          gMs[e] += gd * meps_s[e][i];
-        }
+      for (int e = 0; e < kE; ++e) {
+        gDs[e] = -ln10f * (gx[0] * Kd[0][e] + gx[1] * Kd[1][e] + gx[2] * Kd[2][e]);
+        gMs[e] = -ln10f * (gx[0] * Km[0][e] + gx[1] * Km[1][e] + gx[2] * Km[2][e]);
       }
     } else {
       for (int c = 0; c < 3; ++c)
@@ -204,7 +225,7 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
       // per wavelength: the forward transmittance t = scanner 10^-dens, then
       // dL/ddens = -ln10 t (gx . cmf) into the per-emulsion dye and mask sums
 #pragma unroll 2
-      for (int i = 0; i < 16; ++i) {
+      for (int i = 0; i < kNB; ++i) {
         float d = 0.0f;
         for (int e = 0; e < kE; ++e) d += D[e] * eps_s[e][i] + M[e] * meps_s[e][i];
         const float t = exp2f(-d * log2_10f) * scan_s[i];
@@ -233,7 +254,8 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
       const float ef = eff[e];
       gdmax += gD * (D[e] / Em.d_max);  // dD/dd_max = ef^(1/g)
       const float ggamma = ef > 0.0f ? -gD * D[e] * logf(ef) * Em.inv_gamma * Em.inv_gamma : 0.0f;
-      const float gP = gD * dDdf[e] * slope[e];
+      const float gF = gD * dDdf[e];  // dL/d(developable fraction)
+      const float gP = gF * slope[e];
       sbar[(size_t)e * n + p] = gP;
       part[e] += (double)gdmax;
       part[kE + e] += (double)ggamma;
@@ -241,10 +263,17 @@ __global__ void __launch_bounds__(256, 2) fit_back_scan_kernel(const float* __re
         part[2 * kE + e] += (double)gP * (double)bv;
         gB += gP * Em.gain;
       }
+      if constexpr (kK > 0) {
+        const double w1 = (double)gF * (double)kt[e], w0 = (double)gF - w1;
+#pragma unroll
+        for (int k = 0; k < kK; ++k)
+          kpart[k * kE + e] += k == klo[e] ? w0 : (k == klo[e] + 1 ? w1 : 0.0);
+      }
     }
     if (bbar) bbar[p] = gB;
   }
   block_partials<kE, 3>(part, part_rows, kFitSlotDmax);
+  if constexpr (kK > 0) block_partials<kE, kK>(kpart, part_rows, kFitSlotKnot);
   if constexpr (kL2) block_partials<1, 1>(loss, part_rows, kFitSlotLoss);
 }
 
@@ -338,17 +367,33 @@ hipError_t with_emulsions(int emulsions, F&& f) {
   return hipGetLastError();
 }
 
-hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions,
-                              const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, hipStream_t s,
-                              int images, bool l2_target) {
+template <int kE, int kNB, int kK>
+void launch_back_scan(dim3 grid, hipStream_t s, const float* S, const float* B, const FilmDeviceTables* T,
+                      const float* gout, size_t n, float* sbar, float* bbar, double* part_rows, bool l2_target) {
+  if (l2_target)
+    hipLaunchKernelGGL((fit_back_scan_kernel<kE, true, kNB, kK>), grid, dim3(256), 0, s, S, B, T, gout, n, sbar, bbar,
+                       part_rows);
+  else
+    hipLaunchKernelGGL((fit_back_scan_kernel<kE, false, kNB, kK>), grid, dim3(256), 0, s, S, B, T, gout, n, sbar, bbar,
+                       part_rows);
+}
+
+hipError_t film_fit_back_scan(const float* S, const float* B, const FilmDeviceTables* T, int emulsions, int bins,
+                              int knots, const float* gout, size_t n, float* sbar, float* bbar, double* part_rows,
+                              hipStream_t s, int images, bool l2_target) {
   const dim3 grid(fit_blocks_per_image(n, images), images);
+  if (bins == 16 && knots == 0)
+    return with_emulsions(emulsions, [&](auto e) {
+      launch_back_scan<decltype(e)::value, 16, 0>(grid, s, S, B, T, gout, n, sbar, bbar, part_rows, l2_target);
+    });
+  // the 81-bin grid and the 5-knot curve: up to kFitMaxEmulsionsExt emulsions
+  if (emulsions < 1 || emulsions > kFitMaxEmulsionsExt || (bins != 16 && bins != 81) || (knots != 0 && knots != 5))
+    return hipErrorInvalidValue;
   return with_emulsions(emulsions, [&](auto e) {
-    if (l2_target)
-      hipLaunchKernelGGL((fit_back_scan_kernel<decltype(e)::value, true>), grid, dim3(256), 0, s, S, B, T, gout, n,
-                         sbar, bbar, part_rows);
-    else
-      hipLaunchKernelGGL((fit_back_scan_kernel<decltype(e)::value, false>), grid, dim3(256), 0, s, S, B, T, gout, n,
-                         sbar, bbar, part_rows);
+    constexpr int kE = decltype(e)::value <= kFitMaxEmulsionsExt ? decltype(e)::value : 1;
+    if (bins == 81 && knots == 5) launch_back_scan<kE, 81, 5>(grid, s, S, B, T, gout, n, sbar, bbar, part_rows, l2_target);
+    else if (bins == 81) launch_back_scan<kE, 81, 0>(grid, s, S, B, T, gout, n, sbar, bbar, part_rows, l2_target);
+    else launch_back_scan<kE, 16, 5>(grid, s, S, B, T, gout, n, sbar, bbar, part_rows, l2_target);
   });
 }
 

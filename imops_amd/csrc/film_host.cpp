@@ -242,6 +242,7 @@ struct HostStock {
   double alpha[kFilmMaxLayers] = {};  // emulsion layers: optical density per unit sensitivity (2 rho t)
   int layer_of[kFilmMaxLayers] = {};   // stock layer of each absorbing layer
   double reflect[kFilmMaxBins] = {};  // antihalation backing reflectance samples
+  int knots = 0;                      // knot-curve stock: knots per emulsion (0: crystal LUTs)
 };
 
 // knots: per emulsion, a characteristic curve replacing the crystal LUT (or null)
@@ -317,6 +318,7 @@ HostStock build_stock(const Stock* st, const Grid& g, const KnotCurve* knots = n
     FilmEmulsionTables& E = T.em[e];
     const double mu = std::log(L.crystal_mean_um), sg = L.crystal_sigma_ln;
     if (knots && knots[e].n > 0) {
+      h.knots = knots[e].n;
       E.lut_n = knots[e].n;
       for (int j = 0; j < E.lut_n; ++j) {
         E.lut_log[j] = knots[e].log10_phi[j];
@@ -760,9 +762,11 @@ namespace {
 struct FitSetup {
   FilmDeviceTables T;                 // stock tables with the parameter-dependent fields
   FitExposeConsts k;
-  double W[3][NB];                    // trapezoid weight * basis_k / e_rel / 300
-  double tab[kFilmMaxEmulsions][NB];  // transmittance of every absorbing layer above emulsion e
-  double te[kFilmMaxEmulsions][NB];   // transmittance of emulsion e
+  int nb = 0;                         // bins of the spectral grid
+  double tw[kFilmMaxBins];            // trapezoid weights of the grid
+  double W[3][kFilmMaxBins];          // trapezoid weight * basis_k / e_rel / band
+  double tab[kFilmMaxEmulsions][kFilmMaxBins];  // transmittance of every absorbing layer above emulsion e
+  double te[kFilmMaxEmulsions][kFilmMaxBins];   // transmittance of emulsion e
   double alpha[kFilmMaxEmulsions];
   int above[kFilmMaxEmulsions][kFilmMaxEmulsions];  // [e'][e]: emulsion e' lies above emulsion e
   float rb[kFilmMaxEmulsions];        // r_e * bleed_e
@@ -771,45 +775,49 @@ struct FitSetup {
   bool do_local, do_wide;
 };
 
-double trap_w(int i) { return (i == 0 || i == NB - 1) ? 10.0 : 20.0; }
-
 FitSetup fit_setup(const HostStock& hs, const imops_film_fit_params& p, float pitch) {
   FitSetup f{};
   const Stock& st = *hs.s;
-  const int E = hs.t.n_emulsions;
+  const Grid& g = hs.grid;
+  const int E = hs.t.n_emulsions, nb = g.n;
   if (p.n_emulsions != E) throw Error{IMOPS_E_INVALID, "film_fit: n_emulsions does not match the stock"};
+  if (p.n_bins != nb) throw Error{IMOPS_E_INVALID, "film_fit: n_bins does not match the model's spectral grid"};
+  if (p.n_knots != hs.knots)
+    throw Error{IMOPS_E_INVALID, "film_fit: n_knots must be the stock's knot count (0 for crystal LUTs)"};
+  f.nb = nb;
   f.T = hs.t;
   f.T.invert = 0;
+  for (int i = 0; i < nb; ++i) f.tw[i] = (i == 0 || i == nb - 1) ? 0.5 * g.dl : g.dl;
   // sensitivities -> per-layer transmittances, C matrix (exposure/mod.rs:68-81 collapsed over the basis)
-  double trans[kFilmMaxLayers][NB];
+  double trans[kFilmMaxLayers][kFilmMaxBins];
   int emul_of_layer[kFilmMaxLayers];
   int e = 0;
   for (int a = 0; a < hs.t.n_absorbing; ++a) {
     emul_of_layer[a] = -1;
     if (!hs.t.is_emulsion[a]) {
-      for (int i = 0; i < NB; ++i) trans[a][i] = hs.t.trans[a][i];
+      for (int i = 0; i < nb; ++i) trans[a][i] = hs.t.trans[a][i];
       continue;
     }
     const Layer& L = st.layers[hs.layer_of[a]];
-    for (int i = 0; i < NB; ++i)
+    for (int i = 0; i < nb; ++i)
       trans[a][i] = std::exp(-((double)p.sens[e][i] * 2.0) * (double)L.silver_halide_fraction * (double)L.thickness_um);
     f.alpha[e] = hs.alpha[a];
     emul_of_layer[a] = e++;
   }
   for (int k = 0; k < 3; ++k)
-    for (int i = 0; i < NB; ++i) f.W[k][i] = trap_w(i) * hs.t.basis[k][i] / hs.t.e_rel[i] / 300.0;
-  double above_t[NB];
-  for (int i = 0; i < NB; ++i) above_t[i] = 1.0;
+    for (int i = 0; i < nb; ++i) f.W[k][i] = f.tw[i] * hs.t.basis[k][i] / hs.t.e_rel[i] * hs.t.band_inv;
+  double above_t[kFilmMaxBins];
+  for (int i = 0; i < nb; ++i) above_t[i] = 1.0;
   for (int a = 0; a < hs.t.n_absorbing; ++a) {
     const int em = emul_of_layer[a];
     if (em >= 0) {
-      for (int i = 0; i < NB; ++i) {
+      for (int i = 0; i < nb; ++i) {
         f.tab[em][i] = above_t[i];
         f.te[em][i] = trans[a][i];
       }
       for (int e2 = 0; e2 < em; ++e2) f.above[e2][em] = 1;
     }
-    for (int i = 0; i < NB; ++i) above_t[i] *= trans[a][i];
+    for (int i = 0; i < nb; ++i) above_t[i] *= trans[a][i];
   }
   for (int m = 0; m < 3; ++m)
     for (int c = 0; c < 3; ++c) f.k.minv[m][c] = hs.t.minv[m][c];
@@ -817,7 +825,7 @@ FitSetup fit_setup(const HostStock& hs, const imops_film_fit_params& p, float pi
   for (int k = 0; k < 3; ++k)
     for (int em = 0; em < E; ++em) {
       double acc = 0.0;
-      for (int i = 0; i < NB; ++i) acc += f.W[k][i] * f.tab[em][i] * (1.0 - f.te[em][i]);
+      for (int i = 0; i < nb; ++i) acc += f.W[k][i] * f.tab[em][i] * (1.0 - f.te[em][i]);
       f.k.c[k][em] = acc;
     }
   // halation (halation.rs:24-132): bleed from the sensitivity peak, reflectance weighted by the sensitivity
@@ -827,24 +835,24 @@ FitSetup fit_setup(const HostStock& hs, const imops_film_fit_params& p, float pi
   f.do_local = f.sigma_l >= 1e-3f;
   f.do_wide = f.sigma_h >= 1e-3f && hs.wide_reflect && E > 0;
   for (int em = 0; em < E; ++em) {
-    double curve[NB];
-    for (int i = 0; i < NB; ++i) curve[i] = p.sens[em][i];
+    double curve[kFilmMaxBins];
+    for (int i = 0; i < nb; ++i) curve[i] = p.sens[em][i];
     double peak = 550.0, mx = -INFINITY;
-    for (int i = 0; i < NB; ++i)
-      if (curve[i] > mx) { mx = curve[i]; peak = lam(i); }
+    for (int i = 0; i < nb; ++i)
+      if (curve[i] > mx) { mx = curve[i]; peak = g.lam(i); }
     const float wb = 0.20f, wg = 0.45f, wr = 1.0f;
     float bl;
     if (peak <= 450.0) bl = wb;
     else if (peak <= 550.0) bl = wb + (wg - wb) * (float)((peak - 450.0) / 100.0);
     else if (peak <= 650.0) bl = wg + (wr - wg) * (float)((peak - 550.0) / 100.0);
     else bl = wr;
-    const double norm = trapz(curve);
+    const double norm = trapz(curve, g);
     float r;
-    double prod[NB];
-    for (int i = 0; i < NB; ++i) prod[i] = hs.reflect[i] * curve[i];
-    f.rnum[em] = trapz(prod);
+    double prod[kFilmMaxBins];
+    for (int i = 0; i < nb; ++i) prod[i] = hs.reflect[i] * curve[i];
+    f.rnum[em] = trapz(prod, g);
     f.rden[em] = norm;
-    r = norm <= 1e-9 ? (float)eval_curve(hs.reflect, 650.0) : (float)(f.rnum[em] / norm);
+    r = norm <= 1e-9 ? (float)eval_curve(hs.reflect, g, 650.0) : (float)(f.rnum[em] / norm);
     f.bleed[em] = bl;
     f.rb[em] = r * bl;
     FilmEmulsionTables& Em = f.T.em[em];
@@ -853,6 +861,7 @@ FitSetup fit_setup(const HostStock& hs, const imops_film_fit_params& p, float pi
     Em.d_max = p.d_max[em];
     Em.inv_gamma = 1.0f / std::max(p.gamma[em], 1e-6f);
     Em.mask_scale = p.d_max[em] * 0.4f;
+    for (int j = 0; j < p.n_knots; ++j) Em.lut_frac[j] = p.knot_frac[em][j];  // knot positions stay the stock's
   }
   return f;
 }
@@ -927,17 +936,32 @@ void append_bytes(std::vector<unsigned char>& v, const T& x) {
   v.insert(v.end(), p, p + sizeof(T));
 }
 
-FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_params& p, size_t w, size_t h,
+// The model's stock tables (validated).
+const HostStock& fit_stock(const imops_film_fit_model* m) {
+  if (!m) throw Error{IMOPS_E_INVALID, "film_fit: null model"};
+  if (m->stock < 0 || m->stock > SYNTHETIC) throw Error{IMOPS_E_INVALID, "film_fit: unknown stock"};
+  if (m->spectral_bins != 16 && m->spectral_bins != 81)
+    throw Error{IMOPS_E_INVALID, "film_fit: spectral_bins must be 16 or 81"};
+  const HostStock& hs = host_stock(m->stock, m->spectral_bins, m->stock_seed);
+  if (hs.knots > kFitMaxKnots) throw Error{IMOPS_E_UNSUPPORTED, "film_fit: more than 8 curve knots"};
+  return hs;
+}
+
+FitRun fit_prepare(imops_ctx* ctx, const imops_film_fit_model* m, const imops_film_fit_params& p, size_t w, size_t h,
                    size_t images, const float* state) {
   if (!ctx) throw Error{IMOPS_E_INVALID, "film_fit: null context"};
-  if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit: unknown stock"};
+  const HostStock& hs = fit_stock(m);
+  const float pitch = m->pitch_um;
   if (w < 2 || h < 2 || !(pitch > 0.0f)) throw Error{IMOPS_E_INVALID, "film_fit: invalid dimensions or pitch"};
   if (images < 1 || images > 65535 || w > (size_t)INT32_MAX / 2 || h > (size_t)INT32_MAX / 2)
     throw Error{IMOPS_E_INVALID, "film_fit: image count must be 1..65535"};
   FitRun R{};
   R.ctx = ctx;
-  R.hs = &host_stock(stock);
+  R.hs = &hs;
   if (R.hs->any_reversal) throw Error{IMOPS_E_UNSUPPORTED, "film_fit: reversal stocks are not supported"};
+  const bool ext = m->spectral_bins != 16 || hs.knots != 0;
+  if (ext && (hs.t.n_emulsions > kFitMaxEmulsionsExt || (hs.knots != 0 && hs.knots != 5)))
+    throw Error{IMOPS_E_UNSUPPORTED, "film_fit: the 81-bin grid and knot curves take at most 4 emulsions and 5 knots"};
   R.f = fit_setup(*R.hs, p, pitch);
   R.w = (int)w;
   R.h = (int)h;
@@ -975,8 +999,10 @@ FitRun fit_prepare(imops_ctx* ctx, int stock, float pitch, const imops_film_fit_
   const size_t tab_bytes = sizeof(FilmDeviceTables) + (size_t)(1 + R.rows) * kFitSlots * sizeof(double);
   const size_t taps_bytes = std::max<size_t>(all.size(), 1) * sizeof(float);
   std::vector<unsigned char> key;
-  append_bytes(key, stock);
-  append_bytes(key, pitch);
+  append_bytes(key, m->stock);
+  append_bytes(key, m->spectral_bins);
+  append_bytes(key, m->stock_seed);
+  append_bytes(key, m->pitch_um);
   append_bytes(key, p);
   const void* old_tables = U.tables.ptr;
   const void* old_taps = U.taps.ptr;
@@ -1100,19 +1126,21 @@ extern "C" int imops_film_ex(imops_ctx* ctx, imops_image* img, const imops_film_
   });
 }
 
-extern "C" int imops_film_fit_nominal(int stock, imops_film_fit_params* out) {
+extern "C" int imops_film_fit_nominal(const imops_film_fit_model* m, imops_film_fit_params* out) {
   return guarded([&] {
     if (!out) throw Error{IMOPS_E_INVALID, "film_fit_nominal: null argument"};
-    if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit_nominal: unknown stock"};
-    const HostStock& hs = host_stock(stock);
+    const HostStock& hs = fit_stock(m);
     *out = imops_film_fit_params{};
     out->n_emulsions = hs.t.n_emulsions;
+    out->n_bins = hs.grid.n;
+    out->n_knots = hs.knots;
     for (int a = 0, e = 0; a < hs.t.n_absorbing; ++a) {
       if (!hs.t.is_emulsion[a]) continue;
       const Layer& L = hs.s->layers[hs.layer_of[a]];
-      double curve[NB];
-      sample_curve(L.sensitivity, curve);
-      for (int i = 0; i < NB; ++i) out->sens[e][i] = (float)curve[i];
+      double curve[kFilmMaxBins];
+      sample_curve(L.sensitivity, hs.grid, curve);
+      for (int i = 0; i < hs.grid.n; ++i) out->sens[e][i] = (float)curve[i];
+      for (int j = 0; j < hs.knots; ++j) out->knot_frac[e][j] = (float)hs.t.em[e].lut_frac[j];
       out->d_max[e] = L.d_max;
       out->gamma[e] = L.gamma_contrast;
       ++e;
@@ -1122,25 +1150,34 @@ extern "C" int imops_film_fit_nominal(int stock, imops_film_fit_params* out) {
   });
 }
 
-extern "C" int imops_film_fit_state_floats(int stock, size_t n_images, size_t width, size_t height, size_t* floats) {
+extern "C" int imops_film_fit_knots(const imops_film_fit_model* m, int emulsion, double* log10_fluence) {
   return guarded([&] {
-    if (!floats) throw Error{IMOPS_E_INVALID, "film_fit_state_floats: null argument"};
-    if (stock < 0 || stock >= STOCK_COUNT) throw Error{IMOPS_E_INVALID, "film_fit_state_floats: unknown stock"};
-    *floats = fit_state_floats(host_stock(stock).t.n_emulsions, n_images, width * height);
+    if (!log10_fluence) throw Error{IMOPS_E_INVALID, "film_fit_knots: null argument"};
+    const HostStock& hs = fit_stock(m);
+    if (emulsion < 0 || emulsion >= hs.t.n_emulsions) throw Error{IMOPS_E_INVALID, "film_fit_knots: no such emulsion"};
+    for (int j = 0; j < hs.knots; ++j) log10_fluence[j] = hs.t.em[emulsion].lut_log[j];
   });
 }
 
-extern "C" int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+extern "C" int imops_film_fit_state_floats(const imops_film_fit_model* m, size_t n_images, size_t width, size_t height,
+                                           size_t* floats) {
+  return guarded([&] {
+    if (!floats) throw Error{IMOPS_E_INVALID, "film_fit_state_floats: null argument"};
+    *floats = fit_state_floats(fit_stock(m).t.n_emulsions, n_images, width * height);
+  });
+}
+
+extern "C" int imops_film_fit_forward(imops_ctx* ctx, const imops_film_fit_model* m, const imops_film_fit_params* p,
                                       const float* rgb_in, float* rgb_out, size_t n_images, size_t width,
                                       size_t height, float* state) {
   return guarded([&] {
     if (!p || !rgb_in || !rgb_out) throw Error{IMOPS_E_INVALID, "film_fit_forward: null argument"};
-    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, state);
+    FitRun R = fit_prepare(ctx, m, *p, width, height, n_images, state);
     hipStream_t s = ctx->stream;
     TimedRegion tr(ctx, "film_fit_forward");
     fit_front(R, rgb_in);
     IMOPS_HIP_CHECK(film_capture_reduce(R.S, R.f.do_wide ? R.Bn : nullptr, R.n, R.T, R.img, R.msk, s, R.N));
-    IMOPS_HIP_CHECK(film_scan(R.img, R.msk, R.n, R.T, rgb_out, s, R.N));
+    IMOPS_HIP_CHECK(film_scan(R.img, R.msk, R.n, R.T, rgb_out, s, R.N, false, R.f.nb));
   });
 }
 
@@ -1160,8 +1197,8 @@ void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in,
     TimedRegion tr(ctx, l2_target ? "film_fit_l2_step" : "film_fit_backward");
     IMOPS_HIP_CHECK(film_fit_zero(R.part, (size_t)R.rows * kFitSlots, s));
     if (front) fit_front(R, rgb_in);
-    IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, E, gout, n, R.Sbar,
-                                       R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N, l2_target));
+    IMOPS_HIP_CHECK(film_fit_back_scan(R.S, R.f.do_wide ? R.Bn : nullptr, R.T, E, R.f.nb, R.hs->knots, gout, n,
+                                       R.Sbar, R.f.do_wide ? R.Bbar : nullptr, R.part, s, R.N, l2_target));
     if (R.f.do_wide) {
       const float* deep = R.S + (size_t)(E - 1) * n;
       float* deep_bar = R.Sbar + (size_t)(E - 1) * n;
@@ -1186,6 +1223,10 @@ void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in,
   const FitSetup& f = R.f;
   *grad = imops_film_fit_params{};
   grad->n_emulsions = E;
+  grad->n_bins = f.nb;
+  grad->n_knots = p.n_knots;
+  for (int k = 0; k < p.n_knots; ++k)
+    for (int e = 0; e < E; ++e) grad->knot_frac[e][k] = (float)acc[kFitSlotKnot + k * kFilmMaxEmulsions + e];
   double g_hs = 0.0;
   for (int e = 0; e < E; ++e) {
     grad->d_max[e] = (float)acc[e];
@@ -1196,7 +1237,7 @@ void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in,
   grad->hal_radius = f.do_wide ? (float)(acc[48] * (double)f.sigma_h_unit) : 0.0f;
   // sensitivities: through the exposure contraction C and the halation reflectance r_e
   for (int e2 = 0; e2 < E; ++e2)
-    for (int i = 0; i < NB; ++i) {
+    for (int i = 0; i < f.nb; ++i) {
       double g = 0.0;
       for (int k = 0; k < 3; ++k)
         for (int e = 0; e < E; ++e) {
@@ -1207,8 +1248,7 @@ void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in,
           g += cbar * f.W[k][i] * d;
         }
       if (f.do_wide && f.rden[e2] > 1e-9) {
-        const double tw = trap_w(i);
-        const double dr = tw * (R.hs->reflect[i] * f.rden[e2] - f.rnum[e2]) / (f.rden[e2] * f.rden[e2]);
+        const double dr = f.tw[i] * (R.hs->reflect[i] * f.rden[e2] - f.rnum[e2]) / (f.rden[e2] * f.rden[e2]);
         g += acc[16 + e2] * (double)p.hal_strength * f.bleed[e2] * dr;
       }
       grad->sens[e2][i] = (float)g;
@@ -1219,22 +1259,22 @@ void fit_reverse(FitRun& R, const imops_film_fit_params& p, const float* rgb_in,
 }  // namespace
 }  // namespace imops
 
-extern "C" int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+extern "C" int imops_film_fit_backward(imops_ctx* ctx, const imops_film_fit_model* m, const imops_film_fit_params* p,
                                        const float* rgb_in, const float* grad_out, size_t n_images, size_t width,
                                        size_t height, const float* state, imops_film_fit_params* grad) {
   return guarded([&] {
     if (!p || !rgb_in || !grad_out || !grad) throw Error{IMOPS_E_INVALID, "film_fit_backward: null argument"};
-    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, state);
+    FitRun R = fit_prepare(ctx, m, *p, width, height, n_images, state);
     fit_reverse(R, *p, rgb_in, grad_out, false, !state, grad, nullptr);
   });
 }
 
-extern "C" int imops_film_fit_l2_step(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* p,
+extern "C" int imops_film_fit_l2_step(imops_ctx* ctx, const imops_film_fit_model* m, const imops_film_fit_params* p,
                                       const float* rgb_in, const float* target, size_t n_images, size_t width,
                                       size_t height, imops_film_fit_params* grad, double* loss) {
   return guarded([&] {
     if (!p || !rgb_in || !target || !grad || !loss) throw Error{IMOPS_E_INVALID, "film_fit_l2_step: null argument"};
-    FitRun R = fit_prepare(ctx, stock, pitch_um, *p, width, height, n_images, nullptr);
+    FitRun R = fit_prepare(ctx, m, *p, width, height, n_images, nullptr);
     fit_reverse(R, *p, rgb_in, target, true, true, grad, loss);
   });
 }

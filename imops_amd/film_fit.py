@@ -1,16 +1,20 @@
 """Differentiable film model on the GPU, for fitting film-stock parameters.
 
 The model is the film stage (pichromatic/src/film/mod.rs:77-129) in its
-NegativeLinear form with grain, DIR inhibition and adjacency switched off,
-parameterised by the emulsion sensitivities (E x 16 samples, 400..700 nm), the
-characteristic curve (d_max, gamma per emulsion) and two halation multipliers
-(gain and PSF width).  Forward and reverse passes are the library's HIP
-kernels (`imops_film_fit_forward` / `imops_film_fit_backward`); `FilmFit`
-wraps them as a torch.autograd.Function so an optimiser can drive them.
-`fit` runs Adam on the L2 loss through `imops_film_fit_l2_step` (loss and
-gradient in one device pass per iteration) with the update on the host.
+NegativeLinear form with grain, DIR inhibition and adjacency switched off, on
+the stock's spectral grid (16 samples 400..700 nm as the reference, or 81
+samples 380..780 nm), parameterised by the emulsion sensitivities on that grid,
+the characteristic curve (d_max, gamma per emulsion, and for a knot-curve stock
+such as the seeded synthetic one the developable fraction at each of its
+knots) and two halation multipliers (gain and PSF width).  Forward and reverse
+passes are the library's HIP kernels (`imops_film_fit_forward` /
+`imops_film_fit_backward`); `FilmFit` wraps them as a torch.autograd.Function
+so an optimiser can drive them.  `fit` runs Adam on the L2 loss through
+`imops_film_fit_l2_step` (loss and gradient in one device pass per iteration)
+with the update on the host.
 
-    model = FilmModel(im.Context(0))
+    model = FilmModel(im.Context(0))                          # ColorNeg200, 16 bins
+    model = FilmModel(ctx, "Synthetic", spectral_bins=81, stock_seed=7)  # 3 x 81 sensitivities, 3 x 5 knots
     params = model.nominal()                       # dict of float32 CPU tensors
     out = model(rgb, **params)                     # rgb: (N, H, W, 3) float32 CUDA, ACEScg absolute
 """
@@ -23,19 +27,20 @@ import numpy as np
 import torch
 
 from . import Context, STOCKS
-from ._lib import FilmFitParams, check, lib
+from ._lib import FilmFitModel, FilmFitParams, check, lib
 
-PARAM_NAMES = ("sens", "d_max", "gamma", "hal_strength", "hal_radius")
+PARAM_NAMES = ("sens", "d_max", "gamma", "hal_strength", "hal_radius", "knots")
 
 
 class FilmModel:
-    def __init__(self, ctx: Context, stock: str = "ColorNeg200", pitch_um: float = 36000.0 / 6016):
+    def __init__(self, ctx: Context, stock: str = "ColorNeg200", pitch_um: float = 36000.0 / 6016,
+                 spectral_bins: int = 16, stock_seed: int = 0):
         self.ctx = ctx
-        self.stock = STOCKS[stock]
-        self.pitch = float(pitch_um)
+        self.model = FilmFitModel(STOCKS[stock], int(spectral_bins), int(stock_seed), float(pitch_um))
         p = FilmFitParams()
-        check(lib().imops_film_fit_nominal(self.stock, C.byref(p)))
-        self.n_emulsions = p.n_emulsions
+        check(lib().imops_film_fit_nominal(C.byref(self.model), C.byref(p)))
+        self.n_emulsions, self.n_bins, self.n_knots = p.n_emulsions, p.n_bins, p.n_knots
+        self.names = PARAM_NAMES if self.n_knots else PARAM_NAMES[:5]
         self._lib_stream = None
 
     def _after_torch(self, device) -> None:
@@ -55,27 +60,43 @@ class FilmModel:
 
     def nominal(self) -> dict:
         p = FilmFitParams()
-        check(lib().imops_film_fit_nominal(self.stock, C.byref(p)))
-        E = p.n_emulsions
-        return {"sens": torch.tensor(np.ctypeslib.as_array(p.sens)[:E].copy()),
-                "d_max": torch.tensor(np.ctypeslib.as_array(p.d_max)[:E].copy()),
-                "gamma": torch.tensor(np.ctypeslib.as_array(p.gamma)[:E].copy()),
-                "hal_strength": torch.tensor(p.hal_strength, dtype=torch.float32),
-                "hal_radius": torch.tensor(p.hal_radius, dtype=torch.float32)}
+        check(lib().imops_film_fit_nominal(C.byref(self.model), C.byref(p)))
+        return self.unpack(p)
 
-    def pack(self, sens, d_max, gamma, hal_strength, hal_radius) -> FilmFitParams:
+    def knot_positions(self) -> np.ndarray:
+        """log10 fluence of every knot (E x n_knots; empty for crystal-LUT stocks)."""
+        out = np.zeros((self.n_emulsions, self.n_knots))
+        for e in range(self.n_emulsions):
+            row = np.zeros(max(self.n_knots, 1))
+            check(lib().imops_film_fit_knots(C.byref(self.model), e, row.ctypes.data_as(C.POINTER(C.c_double))))
+            out[e] = row[:self.n_knots]
+        return out
+
+    def unpack(self, p: FilmFitParams) -> dict:
+        """A parameter (or gradient) struct as a dict of float32 CPU tensors."""
+        E, nb, K = self.n_emulsions, self.n_bins, self.n_knots
+        d = {"sens": torch.tensor(np.ctypeslib.as_array(p.sens)[:E, :nb].copy()),
+             "d_max": torch.tensor(np.ctypeslib.as_array(p.d_max)[:E].copy()),
+             "gamma": torch.tensor(np.ctypeslib.as_array(p.gamma)[:E].copy()),
+             "hal_strength": torch.tensor(p.hal_strength, dtype=torch.float32),
+             "hal_radius": torch.tensor(p.hal_radius, dtype=torch.float32)}
+        if K:
+            d["knots"] = torch.tensor(np.ctypeslib.as_array(p.knot_frac)[:E, :K].copy())
+        return d
+
+    def pack(self, sens, d_max, gamma, hal_strength, hal_radius, knots=None) -> FilmFitParams:
         p = FilmFitParams()
-        E = self.n_emulsions
-        p.n_emulsions = E
-        s = np.zeros((8, 16), np.float32)
-        s[:E] = sens.detach().cpu().numpy()
-        dm = np.zeros(8, np.float32)
-        dm[:E] = d_max.detach().cpu().numpy()
-        gm = np.ones(8, np.float32)
-        gm[:E] = gamma.detach().cpu().numpy()
-        C.memmove(p.sens, s.ctypes.data, s.nbytes)
-        C.memmove(p.d_max, dm.ctypes.data, dm.nbytes)
-        C.memmove(p.gamma, gm.ctypes.data, gm.nbytes)
+        E, nb, K = self.n_emulsions, self.n_bins, self.n_knots
+        p.n_emulsions, p.n_bins, p.n_knots = E, nb, K
+        np.ctypeslib.as_array(p.sens)[:E, :nb] = np.asarray(sens.detach().cpu().numpy(), np.float32)
+        np.ctypeslib.as_array(p.d_max)[:E] = np.asarray(d_max.detach().cpu().numpy(), np.float32)
+        g = np.ctypeslib.as_array(p.gamma)
+        g[:] = 1.0
+        g[:E] = np.asarray(gamma.detach().cpu().numpy(), np.float32)
+        if K:
+            if knots is None:
+                raise ValueError("this stock's curve has knots: pass `knots` (E x n_knots)")
+            np.ctypeslib.as_array(p.knot_frac)[:E, :K] = np.asarray(knots.detach().cpu().numpy(), np.float32)
         p.hal_strength = float(hal_strength)
         p.hal_radius = float(hal_radius)
         return p
@@ -84,7 +105,7 @@ class FilmModel:
         """Device buffer for the forward state that a backward pass reuses."""
         n, h, w, _ = rgb.shape
         floats = C.c_size_t()
-        check(lib().imops_film_fit_state_floats(self.stock, n, w, h, C.byref(floats)))
+        check(lib().imops_film_fit_state_floats(C.byref(self.model), n, w, h, C.byref(floats)))
         return torch.empty(floats.value, dtype=torch.float32, device=rgb.device)
 
     def forward_raw(self, rgb: torch.Tensor, p: FilmFitParams, state: torch.Tensor | None = None) -> torch.Tensor:
@@ -92,7 +113,7 @@ class FilmModel:
         n, h, w, _ = rgb.shape
         out = torch.empty_like(rgb)
         self._after_torch(rgb.device)
-        check(lib().imops_film_fit_forward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
+        check(lib().imops_film_fit_forward(self.ctx.handle, C.byref(self.model), C.byref(p), rgb.data_ptr(),
                                            out.data_ptr(), n, w, h, None if state is None else state.data_ptr()))
         self._before_torch(rgb.device)
         return out
@@ -106,7 +127,7 @@ class FilmModel:
         n, h, w, _ = rgb.shape
         g = FilmFitParams()
         self._after_torch(rgb.device)
-        check(lib().imops_film_fit_backward(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
+        check(lib().imops_film_fit_backward(self.ctx.handle, C.byref(self.model), C.byref(p), rgb.data_ptr(),
                                             grad_out.data_ptr(), n, w, h, None if state is None else state.data_ptr(),
                                             C.byref(g)))  # returns after the device work (it reads the sums back)
         return g
@@ -127,12 +148,12 @@ class FilmModel:
         loss = C.c_double()
         if order or self._lib_stream is None:
             self._after_torch(rgb.device)
-        check(lib().imops_film_fit_l2_step(self.ctx.handle, self.stock, self.pitch, C.byref(p), rgb.data_ptr(),
+        check(lib().imops_film_fit_l2_step(self.ctx.handle, C.byref(self.model), C.byref(p), rgb.data_ptr(),
                                            target.data_ptr(), n, w, h, C.byref(g), C.byref(loss)))
         return loss.value, g
 
-    def __call__(self, rgb, sens, d_max, gamma, hal_strength, hal_radius) -> torch.Tensor:
-        return FilmFit.apply(rgb, sens, d_max, gamma, hal_strength, hal_radius, self)
+    def __call__(self, rgb, sens, d_max, gamma, hal_strength, hal_radius, knots=None) -> torch.Tensor:
+        return FilmFit.apply(rgb, sens, d_max, gamma, hal_strength, hal_radius, knots, self)
 
 
 def _check_images(rgb: torch.Tensor) -> None:
@@ -142,31 +163,34 @@ def _check_images(rgb: torch.Tensor) -> None:
 
 
 class FilmFit(torch.autograd.Function):
-    """out = film model(rgb; sens, d_max, gamma, hal_strength, hal_radius); gradients w.r.t. the parameters."""
+    """out = film model(rgb; sens, d_max, gamma, hal_strength, hal_radius[, knots]); gradients
+    w.r.t. the parameters."""
 
     @staticmethod
-    def forward(fctx, rgb, sens, d_max, gamma, hal_strength, hal_radius, model):
-        p = model.pack(sens, d_max, gamma, hal_strength, hal_radius)
+    def forward(fctx, rgb, sens, d_max, gamma, hal_strength, hal_radius, knots, model):
+        p = model.pack(sens, d_max, gamma, hal_strength, hal_radius, knots)
         fctx.model, fctx.packed = model, p
+        fctx.has_knots = knots is not None
         # keep the forward's exposure planes for the backward when gradients are wanted
-        state = model.state_like(rgb) if any(fctx.needs_input_grad[1:6]) else None
+        state = model.state_like(rgb) if any(fctx.needs_input_grad[1:7]) else None
         out = model.forward_raw(rgb, p, state)
-        fctx.save_for_backward(rgb, sens, d_max, gamma, hal_strength, hal_radius, state)
+        fctx.save_for_backward(rgb, state)
+        fctx.like = (sens, d_max, gamma, hal_strength, hal_radius, knots)
         return out
 
     @staticmethod
     def backward(fctx, grad_out):
-        rgb, sens, d_max, gamma, hs, hr, state = fctx.saved_tensors
-        g = fctx.model.backward_raw(rgb, grad_out, fctx.packed, state)
-        E = fctx.model.n_emulsions
-        as_t = lambda a, like: torch.tensor(np.asarray(a, np.float32), dtype=like.dtype, device=like.device)
-        return (None, as_t(np.ctypeslib.as_array(g.sens)[:E], sens), as_t(np.ctypeslib.as_array(g.d_max)[:E], d_max),
-                as_t(np.ctypeslib.as_array(g.gamma)[:E], gamma), as_t(g.hal_strength, hs).reshape(hs.shape),
-                as_t(g.hal_radius, hr).reshape(hr.shape), None)
+        rgb, state = fctx.saved_tensors
+        g = fctx.model.unpack(fctx.model.backward_raw(rgb, grad_out, fctx.packed, state))
+        sens, d_max, gamma, hs, hr, knots = fctx.like
+        as_like = lambda t, like: t.to(dtype=like.dtype, device=like.device).reshape(like.shape)
+        return (None, as_like(g["sens"], sens), as_like(g["d_max"], d_max), as_like(g["gamma"], gamma),
+                as_like(g["hal_strength"], hs), as_like(g["hal_radius"], hr),
+                as_like(g["knots"], knots) if fctx.has_knots else None, None)
 
 
 # lower bounds of the parameters after every optimiser step
-LOWER = {"sens": 0.0, "d_max": 0.05, "gamma": 0.05, "hal_strength": 0.0, "hal_radius": 0.05}
+LOWER = {"sens": 0.0, "d_max": 0.05, "gamma": 0.05, "hal_strength": 0.0, "hal_radius": 0.05, "knots": 0.0}
 
 
 class Adam:
@@ -193,52 +217,61 @@ class Adam:
 
 
 class FlatParams:
-    """The fitted parameters as one float32 vector (sens E x 16, d_max E, gamma E,
-    hal_strength, hal_radius) packed in place into one imops_film_fit_params, and the
-    gradient struct read back as a vector of the same layout."""
+    """The fitted parameters as one float32 vector (in `model.names` order: sens E x bins,
+    d_max E, gamma E, hal_strength, hal_radius, knots E x n_knots) packed in place into one
+    imops_film_fit_params, and the gradient struct read back as a vector of the same layout."""
 
     def __init__(self, model: FilmModel, init: dict):
-        E = self.E = model.n_emulsions
+        E, nb, K = model.n_emulsions, model.n_bins, model.n_knots
+        self.names = model.names
         self.theta = np.concatenate([np.asarray(init[k].detach().cpu().numpy(), np.float32).ravel()
-                                     for k in PARAM_NAMES])
-        self.lower = np.concatenate([np.full(E * 16, LOWER["sens"]), np.full(E, LOWER["d_max"]),
-                                     np.full(E, LOWER["gamma"]), [LOWER["hal_strength"], LOWER["hal_radius"]]]
+                                     for k in self.names])
+        self.lower = np.concatenate([np.full(int(np.prod(init[k].shape)), LOWER[k]) for k in self.names]
                                     ).astype(np.float32)
-        self.p = model.pack(**init)
+        self.p = model.pack(**{k: init[k] for k in self.names})
         self.g = FilmFitParams()
-        self._p_sens = np.ctypeslib.as_array(self.p.sens)[:E]
-        self._p_dmax = np.ctypeslib.as_array(self.p.d_max)[:E]
-        self._p_gamma = np.ctypeslib.as_array(self.p.gamma)[:E]
-        self._g_sens = np.ctypeslib.as_array(self.g.sens)[:E]
-        self._g_dmax = np.ctypeslib.as_array(self.g.d_max)[:E]
-        self._g_gamma = np.ctypeslib.as_array(self.g.gamma)[:E]
+        # (struct view, offset into the vector, length) of every field
+        self._views = []
+        off = 0
+        for name, pv, gv, shape in (
+                ("sens", np.ctypeslib.as_array(self.p.sens)[:E, :nb], np.ctypeslib.as_array(self.g.sens)[:E, :nb],
+                 (E, nb)),
+                ("d_max", np.ctypeslib.as_array(self.p.d_max)[:E], np.ctypeslib.as_array(self.g.d_max)[:E], (E,)),
+                ("gamma", np.ctypeslib.as_array(self.p.gamma)[:E], np.ctypeslib.as_array(self.g.gamma)[:E], (E,)),
+                ("hal_strength", None, None, ()), ("hal_radius", None, None, ()),
+                ("knots", np.ctypeslib.as_array(self.p.knot_frac)[:E, :K],
+                 np.ctypeslib.as_array(self.g.knot_frac)[:E, :K], (E, K))):
+            if name not in self.names:
+                continue
+            size = int(np.prod(shape))
+            self._views.append((name, pv, gv, shape, off, size))
+            off += size
         self.grad = np.zeros_like(self.theta)
 
     def pack(self) -> FilmFitParams:
-        E, t = self.E, self.theta
-        self._p_sens[:] = t[:16 * E].reshape(E, 16)
-        self._p_dmax[:] = t[16 * E:17 * E]
-        self._p_gamma[:] = t[17 * E:18 * E]
-        self.p.hal_strength = float(t[18 * E])
-        self.p.hal_radius = float(t[18 * E + 1])
+        t = self.theta
+        for name, pv, _, shape, off, size in self._views:
+            if pv is None:
+                setattr(self.p, name, float(t[off]))
+            else:
+                pv[...] = t[off:off + size].reshape(shape)
         return self.p
 
     def read_grad(self) -> np.ndarray:
-        E, g = self.E, self.grad
-        g[:16 * E] = self._g_sens.ravel()
-        g[16 * E:17 * E] = self._g_dmax
-        g[17 * E:18 * E] = self._g_gamma
-        g[18 * E] = self.g.hal_strength
-        g[18 * E + 1] = self.g.hal_radius
+        g = self.grad
+        for name, _, gv, shape, off, size in self._views:
+            if gv is None:
+                g[off] = getattr(self.g, name)
+            else:
+                g[off:off + size] = gv.ravel()
         return g
 
     def as_dict(self) -> dict:
-        E, t = self.E, self.theta
-        return {"sens": torch.from_numpy(t[:16 * E].reshape(E, 16).copy()),
-                "d_max": torch.from_numpy(t[16 * E:17 * E].copy()),
-                "gamma": torch.from_numpy(t[17 * E:18 * E].copy()),
-                "hal_strength": torch.tensor(float(t[18 * E])),
-                "hal_radius": torch.tensor(float(t[18 * E + 1]))}
+        t = self.theta
+        out = {}
+        for name, _, _, shape, off, size in self._views:
+            out[name] = torch.from_numpy(t[off:off + size].reshape(shape).copy())
+        return out
 
 
 def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, steps: int = 200, lr: float = 0.01,
@@ -249,7 +282,7 @@ def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, s
     iteration (ms; forward, adjoint and optimiser step).
 
     fused: each iteration is one imops_film_fit_l2_step (loss and gradient in one device
-    pass, one synchronisation) followed by Adam on the host on a 16E + 2E + 2 vector;
+    pass, one synchronisation) followed by Adam on the host on the flat parameter vector;
     otherwise the forward, a torch loss, autograd through FilmFit and torch.optim.Adam."""
     if not fused:
         return _fit_autograd(model, rgb, target, init, steps, lr, log_every)
@@ -269,7 +302,7 @@ def fit(model: FilmModel, rgb: torch.Tensor, target: torch.Tensor, init: dict, s
 
 
 def _fit_autograd(model, rgb, target, init, steps, lr, log_every) -> dict:
-    params = {k: v.detach().clone().requires_grad_(True) for k, v in init.items()}
+    params = {k: init[k].detach().clone().requires_grad_(True) for k in model.names}
     opt = torch.optim.Adam(params.values(), lr=lr)
     history, times = [], []
     for it in range(steps):

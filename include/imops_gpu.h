@@ -331,34 +331,51 @@ int imops_film_ex(imops_ctx* ctx, imops_image* img, const imops_film_params* par
  * No counterpart in the reference (its film simulation has no gradients); it is
  * film::process (film/mod.rs:77-129) in the NegativeLinear form with shutter
  * 1/box ISO and a given pixel pitch, grain, DIR inhibition and adjacency off
- * (development/mod.rs:18-48), parameterised by the emulsion sensitivities on the
- * 16-sample 400..700 nm grid (stock/*.rs layer curves), the characteristic
- * curve D = d_max f^(1/gamma) (reduction.rs:16-68), a halation gain multiplier
- * and a halation PSF width multiplier (halation.rs:24-132); the scan is
- * normalised by the nominal stock's Dmin.  Images are N x H x W x 3 ACEScg
- * (absolute luminance) f32 on the device.  imops_film_fit_backward returns
- * dL/dparams for dL/dout = grad_out, summed over the images.  Every stage runs
- * once over all N images (one launch per stage, not per image).
+ * (development/mod.rs:18-48), on the stock's spectral grid (16 samples 400..700
+ * nm as the reference, or 81 samples 380..780 nm), parameterised by the emulsion
+ * sensitivities on that grid (stock/*.rs layer curves), the characteristic curve
+ * D = d_max f^(1/gamma) (reduction.rs:16-68) with, for a knot-curve stock (the
+ * seeded synthetic stock: 5 knots), the developable fraction f at each knot, a
+ * halation gain multiplier and a halation PSF width multiplier
+ * (halation.rs:24-132); the scan is normalised by the nominal stock's Dmin.
+ * Images are N x H x W x 3 ACEScg (absolute luminance) f32 on the device.
+ * imops_film_fit_backward returns dL/dparams for dL/dout = grad_out, summed over
+ * the images.  Every stage runs once over all N images (one launch per stage,
+ * not per image).  The 81-bin grid and knot curves take stocks of up to 4
+ * emulsions.
  *
  * `state` (optional, device, imops_film_fit_state_floats floats): the forward
  * pass saves the blurred exposure planes and the halation bounce there, and a
  * backward pass given the same state, parameters and images reads them instead
  * of running the exposure and blurs again.  NULL: nothing saved / recomputed. */
+typedef struct imops_film_fit_model {
+  int stock;             /* as imops_film_ex: 0..6 the reference's stocks, 7 the seeded synthetic stock */
+  int spectral_bins;     /* 16 or 81 */
+  uint64_t stock_seed;   /* stock 7 */
+  float pitch_um;        /* pixel pitch */
+} imops_film_fit_model;
+
 typedef struct imops_film_fit_params {
   int n_emulsions;       /* emulsion layers top to bottom (the stock's count) */
-  float sens[8][16];     /* sensitivity samples 400, 420, ..., 700 nm */
+  int n_bins;            /* the model's spectral_bins */
+  int n_knots;           /* the stock's curve knots (0: crystal LUTs) */
+  float sens[8][81];     /* sensitivity samples on the grid (n_bins used) */
+  float knot_frac[8][8]; /* developable fraction at each knot (knot positions are the stock's) */
   float d_max[8];
   float gamma[8];
   float hal_strength;    /* 1 = the stock's halation gain */
   float hal_radius;      /* 1 = the stock's halation PSF width */
 } imops_film_fit_params;
 
-int imops_film_fit_nominal(int stock, imops_film_fit_params* out);
-int imops_film_fit_state_floats(int stock, size_t n_images, size_t width, size_t height, size_t* floats);
-int imops_film_fit_forward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+int imops_film_fit_nominal(const imops_film_fit_model* model, imops_film_fit_params* out);
+/* log10 fluence of each knot of one emulsion's curve (n_knots values) */
+int imops_film_fit_knots(const imops_film_fit_model* model, int emulsion, double* log10_fluence);
+int imops_film_fit_state_floats(const imops_film_fit_model* model, size_t n_images, size_t width, size_t height,
+                                size_t* floats);
+int imops_film_fit_forward(imops_ctx* ctx, const imops_film_fit_model* model, const imops_film_fit_params* params,
                            const float* rgb_in, float* rgb_out, size_t n_images, size_t width, size_t height,
                            float* state);
-int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+int imops_film_fit_backward(imops_ctx* ctx, const imops_film_fit_model* model, const imops_film_fit_params* params,
                             const float* rgb_in, const float* grad_out, size_t n_images, size_t width, size_t height,
                             const float* state, imops_film_fit_params* grad);
 /* One fitting step of the L2 loss L = sum over images, pixels and channels of
@@ -368,7 +385,7 @@ int imops_film_fit_backward(imops_ctx* ctx, int stock, float pitch_um, const imo
  * and dL/dparams in *grad (the same gradient as imops_film_fit_backward with
  * grad_out = 2 (out - target), up to the f32 evaluation of out); one
  * synchronisation, for the readback.  No counterpart in the reference. */
-int imops_film_fit_l2_step(imops_ctx* ctx, int stock, float pitch_um, const imops_film_fit_params* params,
+int imops_film_fit_l2_step(imops_ctx* ctx, const imops_film_fit_model* model, const imops_film_fit_params* params,
                            const float* rgb_in, const float* target, size_t n_images, size_t width, size_t height,
                            imops_film_fit_params* grad, double* loss);
 

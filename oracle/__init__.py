@@ -514,28 +514,98 @@ def film_normalized_dmin(stock: str = "ColorNeg200") -> np.ndarray:
     return out
 
 
-FIT_NPARAMS = 147  # n_emulsions, sens 8x16, d_max 8, gamma 8, hal_strength, hal_radius
+def _fit_sigs(L):
+    if not getattr(L, "_fit_sigs", False):
+        L.oracle_film_fit_nparams.restype = C.c_int
+        L.oracle_film_fit_nominal.restype = C.c_int
+        L.oracle_film_fit_nominal.argtypes = [C.c_int, C.c_int, C.c_uint64, _dp]
+        L.oracle_film_fit.restype = C.c_int
+        L.oracle_film_fit.argtypes = [_fp, _sz, _sz, C.c_int, C.c_int, C.c_uint64, C.c_float, _dp, C.c_int]
+        L.oracle_film_fit_gradient.restype = C.c_int
+        L.oracle_film_fit_gradient.argtypes = [_fp, _fp, _sz, _sz, C.c_int, C.c_int, C.c_uint64, C.c_float, _dp, _dp,
+                                               _fp]
+        L._fit_sigs = True
+    return L
 
 
-def film_fit_nominal(stock: str = "ColorNeg200") -> np.ndarray:
-    """Nominal parameters of the differentiable film model (flat f64 vector)."""
-    L = lib()
-    L.oracle_film_fit_nominal.restype = C.c_int
-    L.oracle_film_fit_nominal.argtypes = [C.c_int, _dp]
-    p = np.zeros(FIT_NPARAMS)
-    _rc(L.oracle_film_fit_nominal(STOCKS[stock], p.ctypes.data_as(_dp)), "film_fit_nominal")
-    return p
+# flat layout of the C API: n_emulsions, n_bins, n_knots, sens 8x81, knots 8x8, d_max 8, gamma 8,
+# hal_strength, hal_radius
+_FIT_SENS, _FIT_KNOTS = 3, 3 + 8 * 81
+_FIT_DMAX = _FIT_KNOTS + 64
+_FIT_GAMMA = _FIT_DMAX + 8
+_FIT_HAL = _FIT_GAMMA + 8
+FIT_NPARAMS = _FIT_HAL + 2
 
 
-def film_fit(rgb: np.ndarray, params: np.ndarray, stock: str = "ColorNeg200", pitch_um: float = 36000.0 / 6016,
-             fp64: bool = True) -> np.ndarray:
+def _fit_to_flat(params: dict, E: int, nb: int, K: int) -> np.ndarray:
+    v = np.zeros(FIT_NPARAMS)
+    v[0:3] = (E, nb, K)
+    s = np.zeros((8, 81))
+    s[:E, :nb] = np.asarray(params["sens"], np.float64)
+    v[_FIT_SENS:_FIT_KNOTS] = s.ravel()
+    if K:
+        k = np.zeros((8, 8))
+        k[:E, :K] = np.asarray(params["knots"], np.float64)
+        v[_FIT_KNOTS:_FIT_DMAX] = k.ravel()
+    v[_FIT_DMAX:_FIT_DMAX + E] = np.asarray(params["d_max"], np.float64)
+    v[_FIT_GAMMA:_FIT_GAMMA + 8] = 1.0
+    v[_FIT_GAMMA:_FIT_GAMMA + E] = np.asarray(params["gamma"], np.float64)
+    v[_FIT_HAL] = float(params["hal_strength"])
+    v[_FIT_HAL + 1] = float(params["hal_radius"])
+    return v
+
+
+def _fit_from_flat(v: np.ndarray) -> dict:
+    E, nb, K = int(v[0]), int(v[1]), int(v[2])
+    d = {"sens": v[_FIT_SENS:_FIT_KNOTS].reshape(8, 81)[:E, :nb].copy(),
+         "d_max": v[_FIT_DMAX:_FIT_DMAX + E].copy(), "gamma": v[_FIT_GAMMA:_FIT_GAMMA + E].copy(),
+         "hal_strength": float(v[_FIT_HAL]), "hal_radius": float(v[_FIT_HAL + 1])}
+    if K:
+        d["knots"] = v[_FIT_KNOTS:_FIT_DMAX].reshape(8, 8)[:E, :K].copy()
+    return d
+
+
+def film_fit_nominal(stock: str = "ColorNeg200", bins: int = 16, stock_seed: int = 0) -> dict:
+    """Nominal parameters of the differentiable film model: sens (E x bins), d_max, gamma,
+    hal_strength, hal_radius, and knots (E x n_knots) for a knot-curve stock."""
+    L = _fit_sigs(lib())
+    assert L.oracle_film_fit_nparams() == FIT_NPARAMS
+    v = np.zeros(FIT_NPARAMS)
+    _rc(L.oracle_film_fit_nominal(STOCKS[stock], bins, stock_seed, v.ctypes.data_as(_dp)), "film_fit_nominal")
+    return _fit_from_flat(v)
+
+
+def _fit_flat_for(params: dict, stock: str, bins: int, stock_seed: int) -> np.ndarray:
+    nom = film_fit_nominal(stock, bins, stock_seed)
+    E = len(nom["d_max"])
+    K = nom["knots"].shape[1] if "knots" in nom else 0
+    return _fit_to_flat(params, E, bins, K)
+
+
+def film_fit(rgb: np.ndarray, params: dict, stock: str = "ColorNeg200", pitch_um: float = 36000.0 / 6016,
+             bins: int = 16, stock_seed: int = 0, fp64: bool = True) -> np.ndarray:
     """The differentiable film model (film::process with grain, DIR and adjacency off,
     parameters overridden, scan normalised by the nominal Dmin) on an (H, W, 3) image."""
-    L = lib()
-    L.oracle_film_fit.restype = C.c_int
-    L.oracle_film_fit.argtypes = [_fp, _sz, _sz, C.c_int, C.c_float, _dp, C.c_int]
+    L = _fit_sigs(lib())
     out = np.array(rgb, np.float32, copy=True, order="C")
     h, w, _ = out.shape
-    p = np.ascontiguousarray(params, np.float64)
-    _rc(L.oracle_film_fit(_f(out), w, h, STOCKS[stock], pitch_um, p.ctypes.data_as(_dp), int(fp64)), "film_fit")
+    v = _fit_flat_for(params, stock, bins, stock_seed)
+    _rc(L.oracle_film_fit(_f(out), w, h, STOCKS[stock], bins, stock_seed, pitch_um, v.ctypes.data_as(_dp),
+                          int(fp64)), "film_fit")
     return out
+
+
+def film_fit_gradient(rgb: np.ndarray, gout: np.ndarray, params: dict, stock: str = "ColorNeg200",
+                      pitch_um: float = 36000.0 / 6016, bins: int = 16, stock_seed: int = 0):
+    """FP64 reverse pass of the model: (dL/dparams as a dict, the f64 forward's output)
+    for dL/dout = gout on one (H, W, 3) image."""
+    L = _fit_sigs(lib())
+    x = np.ascontiguousarray(rgb, np.float32)
+    gy = np.ascontiguousarray(gout, np.float32)
+    h, w, _ = x.shape
+    v = _fit_flat_for(params, stock, bins, stock_seed)
+    g = np.zeros(FIT_NPARAMS)
+    out = np.zeros_like(x)
+    _rc(L.oracle_film_fit_gradient(_f(x), _f(gy), w, h, STOCKS[stock], bins, stock_seed, pitch_um,
+                                   v.ctypes.data_as(_dp), g.ctypes.data_as(_dp), _f(out)), "film_fit_gradient")
+    return _fit_from_flat(g), out

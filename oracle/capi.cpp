@@ -229,39 +229,69 @@ void oracle_film_upsample(const float* rgb, double* out) { film_upsample(rgb, ou
 void oracle_film_spectrum_to_acescg(const double* s, double* out) { film_spectrum_to_acescg(s, out); }
 int oracle_film_normalized_dmin(int stock, float* out) { return film_normalized_dmin(stock, out); }
 
-// Differentiable film model: params = [n_emulsions, sens (8x16), d_max (8), gamma (8),
-// hal_strength, hal_radius] as doubles (147 values).
+// Differentiable film model: params = [n_emulsions, n_bins, n_knots, sens (8x81),
+// knot fractions (8x8), d_max (8), gamma (8), hal_strength, hal_radius] as doubles
+// (733 values); the model = (stock, bins, stock_seed, pitch).
+static constexpr int kFitSens = 3, kFitKnots = kFitSens + 8 * 81, kFitDmax = kFitKnots + 64, kFitGamma = kFitDmax + 8,
+                     kFitHal = kFitGamma + 8, kFitN = kFitHal + 2;
 static FilmFitParams unpack_fit(const double* p) {
   FilmFitParams f{};
   f.n_emulsions = (int)p[0];
-  for (int e = 0; e < 8; ++e)
-    for (int i = 0; i < 16; ++i) f.sens[e][i] = p[1 + e * 16 + i];
+  f.n_bins = (int)p[1];
+  f.n_knots = (int)p[2];
   for (int e = 0; e < 8; ++e) {
-    f.d_max[e] = p[129 + e];
-    f.gamma[e] = p[137 + e];
+    for (int i = 0; i < 81; ++i) f.sens[e][i] = p[kFitSens + e * 81 + i];
+    for (int k = 0; k < 8; ++k) f.knot_frac[e][k] = p[kFitKnots + e * 8 + k];
+    f.d_max[e] = p[kFitDmax + e];
+    f.gamma[e] = p[kFitGamma + e];
   }
-  f.hal_strength = p[145];
-  f.hal_radius = p[146];
+  f.hal_strength = p[kFitHal];
+  f.hal_radius = p[kFitHal + 1];
   return f;
 }
-int oracle_film_fit(float* rgb, size_t w, size_t h, int stock, float pitch_um, const double* params, int fp64) {
-  const FilmFitParams f = unpack_fit(params);
-  return fp64 ? film_fit_process<double>(rgb, w, h, stock, pitch_um, f)
-              : film_fit_process<float>(rgb, w, h, stock, pitch_um, f);
-}
-int oracle_film_fit_nominal(int stock, double* params) {
-  FilmFitParams f;
-  const int rc = film_fit_nominal(stock, &f);
-  if (rc) return rc;
-  params[0] = f.n_emulsions;
-  for (int e = 0; e < 8; ++e)
-    for (int i = 0; i < 16; ++i) params[1 + e * 16 + i] = f.sens[e][i];
+static void pack_fit(const FilmFitParams& f, double* p) {
+  for (int i = 0; i < kFitN; ++i) p[i] = 0.0;
+  p[0] = f.n_emulsions;
+  p[1] = f.n_bins;
+  p[2] = f.n_knots;
   for (int e = 0; e < 8; ++e) {
-    params[129 + e] = f.d_max[e];
-    params[137 + e] = f.gamma[e];
+    for (int i = 0; i < 81; ++i) p[kFitSens + e * 81 + i] = f.sens[e][i];
+    for (int k = 0; k < 8; ++k) p[kFitKnots + e * 8 + k] = f.knot_frac[e][k];
+    p[kFitDmax + e] = f.d_max[e];
+    p[kFitGamma + e] = f.gamma[e];
   }
-  params[145] = f.hal_strength;
-  params[146] = f.hal_radius;
+  p[kFitHal] = f.hal_strength;
+  p[kFitHal + 1] = f.hal_radius;
+}
+static FilmFitModel fit_model(int stock, int bins, uint64_t seed, float pitch) {
+  FilmFitModel m;
+  m.stock = stock;
+  m.bins = bins;
+  m.stock_seed = seed;
+  m.pitch_um = pitch;
+  return m;
+}
+int oracle_film_fit_nparams() { return kFitN; }
+int oracle_film_fit(float* rgb, size_t w, size_t h, int stock, int bins, uint64_t seed, float pitch_um,
+                    const double* params, int fp64) {
+  const FilmFitParams f = unpack_fit(params);
+  const FilmFitModel m = fit_model(stock, bins, seed, pitch_um);
+  return fp64 ? film_fit_process<double>(rgb, w, h, m, f) : film_fit_process<float>(rgb, w, h, m, f);
+}
+int oracle_film_fit_nominal(int stock, int bins, uint64_t seed, double* params) {
+  FilmFitParams f;
+  const int rc = film_fit_nominal(fit_model(stock, bins, seed, 1.0f), &f);
+  if (rc) return rc;
+  pack_fit(f, params);
+  return 0;
+}
+int oracle_film_fit_gradient(const float* rgb, const float* gout, size_t w, size_t h, int stock, int bins,
+                             uint64_t seed, float pitch_um, const double* params, double* grad, float* out_rgb) {
+  FilmFitParams g;
+  const int rc = film_fit_gradient(rgb, gout, w, h, fit_model(stock, bins, seed, pitch_um), unpack_fit(params), &g,
+                                   out_rgb);
+  if (rc) return rc;
+  pack_fit(g, grad);
   return 0;
 }
 

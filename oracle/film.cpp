@@ -417,14 +417,14 @@ struct Derived {
       e.reversal = L.reversal;
       e.d_max = fit ? (float)fit->d_max[ei] : L.d_max;
       e.gamma = fit ? (float)fit->gamma[ei] : L.gamma;
-      ++ei;
+      const int fe = ei++;
       e.recip_p = L.recip_p;
       const double mu = L.mu_ln, sg = L.sigma_ln;
       if (L.knots > 0) {
         e.lut.n = L.knots;
         for (int j = 0; j < L.knots; ++j) {
           e.lut.logs[j] = L.knot_x[j];
-          e.lut.fracs[j] = L.knot_f[j];
+          e.lut.fracs[j] = fit && fit->n_knots == L.knots ? fit->knot_frac[fe][j] : L.knot_f[j];
         }
       } else {
         e.lut.build(mu, sg, L.capture_k);
@@ -886,46 +886,465 @@ template int film_process<float>(float*, size_t, size_t, int, int, uint64_t, int
 template int film_process<double>(float*, size_t, size_t, int, int, uint64_t, int, float, FilmTrace*, const FilmExt&);
 
 // The differentiable film model: film::process (NegativeLinear, shutter 1/box
-// ISO, a given pixel pitch) with grain, DIR inhibition and adjacency off, the emulsion sensitivities,
-// D_max and gamma taken from `fit`, the halation gain and PSF width scaled by
-// fit->hal_strength / fit->hal_radius, and the scan normalised by the nominal
-// stock's Dmin (so the normalisation is not a function of the parameters).
+// ISO, a given pixel pitch) on the model's grid with grain, DIR inhibition and
+// adjacency off, the emulsion sensitivities, D_max, gamma and (knot-curve
+// stocks) the developable fraction at each knot taken from `fit`, the halation
+// gain and PSF width scaled by fit.hal_strength / fit.hal_radius, and the scan
+// normalised by the nominal stock's Dmin (so the normalisation is not a
+// function of the parameters).
+namespace {
+
+int fit_model_check(const FilmFitModel& m, const Src& src, const FilmFitParams* fit) {
+  if (!stock_ok(m.stock) || (m.bins != 16 && m.bins != 81) || !(m.pitch_um > 0.0f)) return -1;
+  if (!fit) return 0;
+  int E = 0, knots = 0;
+  for (const SrcLayer& L : src.layers)
+    if (L.kind == 0) {
+      ++E;
+      knots = L.knots;
+    }
+  if (fit->n_emulsions != E || fit->n_bins != m.bins || fit->n_knots != knots || knots > 8) return -1;
+  return 0;
+}
+
+double fit_scan_scale(const Src& src) {
+  const Derived nom(src);
+  double dmin[3];
+  dmin_reference(nom, dmin);
+  return 1.0 / std::max(std::max(std::max(dmin[0], dmin[1]), dmin[2]), 1e-12);
+}
+
+}  // namespace
+
 template <class P>
-int film_fit_process(float* rgb, size_t w, size_t h, int stock_idx, float pitch_um, const FilmFitParams& fit) {
-  if (!stock_ok(stock_idx) || w == 0 || h == 0) return -1;
-  const Src src = load_src(stock_idx, 16, 0);
-  double nominal_scale;
-  {
-    const Derived nom(src);
-    double dmin[3];
-    dmin_reference(nom, dmin);
-    nominal_scale = 1.0 / std::max(std::max(std::max(dmin[0], dmin[1]), dmin[2]), 1e-12);
-  }
+int film_fit_process(float* rgb, size_t w, size_t h, const FilmFitModel& m, const FilmFitParams& fit) {
+  if (w == 0 || h == 0 || !stock_ok(m.stock)) return -1;
+  const Src src = load_src(m.stock, m.bins, m.stock_seed);
+  if (fit_model_check(m, src, &fit)) return -1;
   Derived d(src, &fit);
   d.hal_strength = fit.hal_strength;
   d.hal_radius = fit.hal_radius;
-  d.scan_scale = nominal_scale;
+  d.scan_scale = fit_scan_scale(src);
   d.grain = d.dir = d.adjacency = false;
   std::vector<float> out(w * h * 3);
-  run_negative<P>(d, rgb, w, h, pitch_um, 1.0f / src.box_iso, 0, out.data(), nullptr);
+  run_negative<P>(d, rgb, w, h, m.pitch_um, 1.0f / src.box_iso, 0, out.data(), nullptr);
   std::copy(out.begin(), out.end(), rgb);
   return 0;
 }
-template int film_fit_process<float>(float*, size_t, size_t, int, float, const FilmFitParams&);
-template int film_fit_process<double>(float*, size_t, size_t, int, float, const FilmFitParams&);
+template int film_fit_process<float>(float*, size_t, size_t, const FilmFitModel&, const FilmFitParams&);
+template int film_fit_process<double>(float*, size_t, size_t, const FilmFitModel&, const FilmFitParams&);
 
-int film_fit_nominal(int stock_idx, FilmFitParams* out) {
-  if (!stock_ok(stock_idx)) return -1;
-  const Derived d(load_src(stock_idx, 16, 0));
+int film_fit_nominal(const FilmFitModel& m, FilmFitParams* out) {
+  if (!stock_ok(m.stock) || (m.bins != 16 && m.bins != 81)) return -1;
+  const Derived d(load_src(m.stock, m.bins, m.stock_seed));
   *out = FilmFitParams{};
   out->n_emulsions = (int)d.em.size();
+  out->n_bins = d.g.n;
   for (size_t e = 0; e < d.em.size(); ++e) {
-    for (int i = 0; i < NB; ++i) out->sens[e][i] = d.em[e].sens[i];
+    for (int i = 0; i < d.g.n; ++i) out->sens[e][i] = d.em[e].sens[i];
     out->d_max[e] = d.em[e].d_max;
     out->gamma[e] = d.em[e].gamma;
+    if (d.em[e].lut.n < 64) {
+      out->n_knots = d.em[e].lut.n;
+      for (int j = 0; j < d.em[e].lut.n; ++j) out->knot_frac[e][j] = d.em[e].lut.fracs[j];
+    }
   }
   out->hal_strength = 1.0;
   out->hal_radius = 1.0;
+  return 0;
+}
+
+// ---- the model's reverse pass in f64 -------------------------------------
+// The forward is film_fit_process<double> step by step (exposure walk, local
+// scatter, halation bounce, capture curve, H&D reduction with the coloured
+// mask, densitometric scan); the reverse applies the transpose of each step:
+//   scan:       dL/dt_i = w_i sum_j xyzbar_j cmf_j(i),  dL/ddens_i = -ln10 t_i dL/dt_i
+//   reduction:  D = d_max c^(1/g), M = 0.4 d_max (1 - clamp(D/d_max)) (mask stocks)
+//   capture:    f = (1 - t) f_lo + t f_hi at the bracketing knots, t linear in log10 P
+//   halation:   P_e = S_e + gain_e B, B = G_sigma(S_deepest), dB/dsigma = V'H + VH'
+//   scatter:    S = 0.82 A + 0.18 G_local(A)   (the blurs' transposes by scattering
+//               every output's taps back onto the sources it read, reflect-101)
+//   exposure:   A_e = (1/span) sum_i w_i phi_i (prod of T_l above e)(1 - T_e),
+//               phi_i = (M^-1 rgb)_+ . basis(i) / e_rel(i), T_e = exp(-2 rho t sens_e)
+// The f32 roundings of d_max, gamma, the halation gain and sigma in the forward
+// are taken as the identity; the bleed factor (the sensitivity's argmax, piecewise
+// constant) is held fixed.
+namespace {
+
+struct GradTaps {
+  int r = 0;
+  std::vector<double> k, dk;  // normalised taps of blur() and their derivative in sigma
+};
+
+GradTaps grad_taps(float sigma) {
+  GradTaps t;
+  t.r = (int)std::max(std::ceil(3.0f * sigma), 1.0f);
+  const double s = (double)sigma, inv_2s2 = 1.0 / (2.0 * s * s);
+  const size_t nt = (size_t)(2 * t.r + 1);
+  std::vector<double> g(nt), gd(nt);
+  double sg = 0.0, sgd = 0.0;
+  for (size_t i = 0; i < nt; ++i) {
+    const double x = (double)i - (double)t.r;
+    g[i] = std::exp(-x * x * inv_2s2);
+    gd[i] = g[i] * x * x / (s * s * s);
+    sg += g[i];
+    sgd += gd[i];
+  }
+  t.k.resize(nt);
+  t.dk.resize(nt);
+  for (size_t i = 0; i < nt; ++i) {
+    t.k[i] = g[i] / sg;
+    t.dk[i] = (gd[i] * sg - g[i] * sgd) / (sg * sg);
+  }
+  return t;
+}
+
+size_t refl101(long long x, long long n) {
+  for (;;) {
+    if (x < 0) x = -x;
+    else if (x >= n) x = 2 * n - 2 - x;
+    else return (size_t)x;
+    if (n == 1) return 0;
+  }
+}
+
+// out = V(H in): rows with taps kh, then columns with kv (reflect-101)
+void sep_blur(const std::vector<double>& in, std::vector<double>& out, size_t w, size_t h,
+              const std::vector<double>& kh, const std::vector<double>& kv, int r) {
+  std::vector<double> tmp(w * h);
+#pragma omp parallel for schedule(static)
+  for (long long y = 0; y < (long long)h; ++y)
+    for (size_t x = 0; x < w; ++x) {
+      double acc = 0.0;
+      for (int j = 0; j <= 2 * r; ++j) acc += in[y * w + refl101((long long)x + j - r, (long long)w)] * kh[j];
+      tmp[y * w + x] = acc;
+    }
+  out.assign(w * h, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long long y = 0; y < (long long)h; ++y)
+    for (size_t x = 0; x < w; ++x) {
+      double acc = 0.0;
+      for (int j = 0; j <= 2 * r; ++j) acc += tmp[refl101(y + j - r, (long long)h) * w + x] * kv[j];
+      out[y * w + x] = acc;
+    }
+}
+
+// out = (V(H .))^T in = H^T (V^T in), each forward output's taps scattered onto its sources
+void sep_blur_transpose(const std::vector<double>& in, std::vector<double>& out, size_t w, size_t h,
+                        const std::vector<double>& k, int r) {
+  std::vector<double> tmp(w * h, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long long x = 0; x < (long long)w; ++x)  // one column per thread: its scatter stays in the column
+    for (size_t y = 0; y < h; ++y)
+      for (int j = 0; j <= 2 * r; ++j) tmp[refl101((long long)y + j - r, (long long)h) * w + x] += k[j] * in[y * w + x];
+  out.assign(w * h, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long long y = 0; y < (long long)h; ++y)  // one row per thread
+    for (size_t x = 0; x < w; ++x)
+      for (int j = 0; j <= 2 * r; ++j) out[y * w + refl101((long long)x + j - r, (long long)w)] += k[j] * tmp[y * w + x];
+}
+
+}  // namespace
+
+int film_fit_gradient(const float* rgb, const float* gout, size_t w, size_t h, const FilmFitModel& m,
+                      const FilmFitParams& fit, FilmFitParams* grad, float* out_rgb) {
+  if (w == 0 || h == 0 || !grad || !stock_ok(m.stock)) return -1;
+  const Src src = load_src(m.stock, m.bins, m.stock_seed);
+  if (fit_model_check(m, src, &fit)) return -1;
+  Derived d(src, &fit);
+  if (d.any_reversal) return -1;
+  const double scale = fit_scan_scale(src);
+  const size_t n = w * h, E = d.em.size();
+  const Grid& g = d.g;
+  const int nb = g.n;
+  const Basis& Bs = basis(g);
+  const Cmf& Cm = cmf_for(g);
+  const double span = g.lam(nb - 1) - g.lam(0);
+  const float pitch = m.pitch_um;
+  double tw[NBMAX];
+  for (int i = 0; i < nb; ++i) tw[i] = (i == 0 || i == nb - 1) ? 0.5 * g.dl : g.dl;
+  // absorbing layers above each emulsion: transmittance product tab_e(i), emulsion layer index
+  std::vector<int> em_layer(E);
+  for (size_t e = 0; e < E; ++e) em_layer[e] = d.em[e].layer;
+  std::vector<std::vector<double>> tab(E, std::vector<double>(nb, 1.0));
+  for (size_t e = 0; e < E; ++e)
+    for (int l = 0; l < em_layer[e]; ++l)
+      if (d.absorbs[l])
+        for (int i = 0; i < nb; ++i) tab[e][i] *= d.trans[l][i];
+
+  // ---- forward: exposure planes A_e and the basis weights per pixel
+  std::vector<std::vector<double>> A(E, std::vector<double>(n));
+  std::vector<double> wgt(3 * n);
+#pragma omp parallel for schedule(static)
+  for (long long p = 0; p < (long long)n; ++p) {
+    const double px[3] = {(double)rgb[p * 3], (double)rgb[p * 3 + 1], (double)rgb[p * 3 + 2]};
+    for (int r = 0; r < 3; ++r) {
+      const double v = Bs.minv[r][0] * px[0] + Bs.minv[r][1] * px[1] + Bs.minv[r][2] * px[2];
+      wgt[p * 3 + r] = v < 0.0 ? 0.0 : v;
+    }
+    for (size_t e = 0; e < E; ++e) {
+      double acc = 0.0;
+      const size_t l = (size_t)em_layer[e];
+      for (int i = 0; i < nb; ++i) {
+        const double phi = (wgt[p * 3] * Bs.b[0][i] + wgt[p * 3 + 1] * Bs.b[1][i] + wgt[p * 3 + 2] * Bs.b[2][i]) /
+                           (550.0 / g.lam(i));
+        acc += tw[i] * phi * tab[e][i] * (1.0 - d.trans[l][i]);
+      }
+      A[e][p] = acc / span;
+    }
+  }
+  // local scatter
+  const float sigma_local = std::max(src.psf_local_um / std::max(pitch, 1e-6f), 0.0f);
+  const bool do_local = sigma_local >= 1e-3f;
+  const double f_loc = (double)0.18f, keep = 1.0 - f_loc;
+  GradTaps tl;
+  std::vector<std::vector<double>> S(E);
+  for (size_t e = 0; e < E; ++e) {
+    if (!do_local) {
+      S[e] = A[e];
+      continue;
+    }
+    if (tl.k.empty()) tl = grad_taps(sigma_local);
+    std::vector<double> bl;
+    sep_blur(A[e], bl, w, h, tl.k, tl.k, tl.r);
+    S[e].resize(n);
+    for (size_t i = 0; i < n; ++i) S[e][i] = keep * A[e][i] + f_loc * bl[i];
+  }
+  // halation
+  const double sigma_unit = (double)std::max(src.psf_halation_um / std::max(pitch, 1e-6f), 0.0f);
+  const float sigma_wide = (float)(sigma_unit * fit.hal_radius);
+  bool do_wide = sigma_wide >= 1e-3f && E > 0;
+  if (do_wide) {
+    float max_r = 0.0f;
+    const double bands[4] = {450.0, 550.0, 650.0, 700.0};
+    for (double bl : bands) max_r = std::max(max_r, (float)evaluate(d.reflect, g, bl));
+    do_wide = max_r > 0.0f;
+  }
+  std::vector<double> gain(E, 0.0), rb(E, 0.0), bleed(E, 0.0), rnum(E, 0.0), rden(E, 0.0);
+  std::vector<double> Bn;
+  GradTaps th;
+  if (do_wide) {
+    th = grad_taps(sigma_wide);
+    sep_blur(S[E - 1], Bn, w, h, th.k, th.k, th.r);
+    for (size_t e = 0; e < E; ++e) {
+      const Emulsion& em = d.em[e];
+      double peak = 550.0, mx = -INFINITY;
+      for (int i = 0; i < nb; ++i)
+        if (em.sens[i] > mx) { mx = em.sens[i]; peak = g.lam(i); }
+      const float wb = 0.20f, wg = 0.45f, wr = 1.0f;
+      float bl;
+      if (peak <= 450.0) bl = wb;
+      else if (peak <= 550.0) bl = wb + (wg - wb) * (float)((peak - 450.0) / 100.0);
+      else if (peak <= 650.0) bl = wg + (wr - wg) * (float)((peak - 550.0) / 100.0);
+      else bl = wr;
+      const double norm = integrate(em.sens, g);
+      double prod[NBMAX];
+      for (int i = 0; i < nb; ++i) prod[i] = d.reflect[i] * em.sens[i];
+      rnum[e] = integrate(prod, g);
+      rden[e] = norm;
+      const float r_e = norm <= 1e-9 ? (float)evaluate(d.reflect, g, 650.0) : (float)(rnum[e] / norm);
+      bleed[e] = bl;
+      rb[e] = (double)(r_e * bl);
+      const float gn = (float)(rb[e] * fit.hal_strength);
+      gain[e] = gn > 0.0f ? (double)gn : 0.0;
+    }
+  }
+
+  // ---- capture, reduction, scan forward and their reverse, per pixel
+  std::vector<std::vector<double>> Sbar(E, std::vector<double>(n, 0.0));
+  std::vector<double> Bbar(do_wide ? n : 0, 0.0);
+  // per-row sums (added in row order): d_max, gamma, gain, knot fractions
+  const size_t K = (size_t)fit.n_knots, nsum = 3 * E + E * K;
+  std::vector<double> rows(h * nsum, 0.0);
+  const double ln10 = 2.30258509299404568402;
+#pragma omp parallel for schedule(static)
+  for (long long y = 0; y < (long long)h; ++y) {
+    double* acc = rows.data() + (size_t)y * nsum;
+    for (size_t x = 0; x < w; ++x) {
+      const size_t p = (size_t)y * w + x;
+      double D[8], M[8], eff[8], fr[8], tt[8], dfdx[8], xv[8];
+      int lo[8];
+      bool clamp_lo[8], clamp_hi[8];
+      for (size_t e = 0; e < E; ++e) {
+        const Emulsion& em = d.em[e];
+        const Lut& L = em.lut;
+        const double P = S[e][p] + (do_wide && gain[e] > 0.0 ? gain[e] * Bn[p] : 0.0);
+        const double xx = P;  // reciprocity efficiency 1 at the shutter 1/box ISO
+        xv[e] = xx;
+        lo[e] = 0;
+        tt[e] = 0.0;
+        dfdx[e] = 0.0;
+        clamp_lo[e] = clamp_hi[e] = false;
+        double f;
+        if (!(xx > 0.0)) {
+          f = L.fracs[0];
+          clamp_lo[e] = true;
+        } else {
+          const double lp = std::log10(xx);
+          if (lp <= L.logs[0]) {
+            f = L.fracs[0];
+            clamp_lo[e] = true;
+          } else if (lp >= L.logs[L.n - 1]) {
+            f = L.fracs[L.n - 1];
+            clamp_hi[e] = true;
+          } else {
+            int a = 0, b = L.n - 1;
+            while (b - a > 1) {
+              const int mid = (a + b) / 2;
+              if (L.logs[mid] <= lp) a = mid;
+              else b = mid;
+            }
+            const double t = (lp - L.logs[a]) / (L.logs[b] - L.logs[a]);
+            f = L.fracs[a] * (1.0 - t) + L.fracs[b] * t;
+            lo[e] = a;
+            tt[e] = t;
+            dfdx[e] = (L.fracs[b] - L.fracs[a]) / (L.logs[b] - L.logs[a]) / (xx * ln10);
+          }
+        }
+        fr[e] = f;
+        const double c = rclamp(f, 0.0, 1.0);
+        eff[e] = c;
+        const double dm = (double)em.d_max, ig = 1.0 / (double)std::max(em.gamma, 1e-6f);
+        D[e] = dm * std::pow(c, ig);
+        M[e] = em.has_mask ? dm * (double)0.4f * (1.0 - rclamp(D[e] / dm, 0.0, 1.0)) : 0.0;
+      }
+      double dens[NBMAX], t[NBMAX];
+      for (int i = 0; i < nb; ++i) dens[i] = 0.0;
+      for (size_t e = 0; e < E; ++e)
+        for (int i = 0; i < nb; ++i) {
+          dens[i] += D[e] * d.em[e].eps[i];
+          if (d.em[e].has_mask) dens[i] += M[e] * d.em[e].mask[i];
+        }
+      for (int i = 0; i < nb; ++i) t[i] = std::pow(10.0, -dens[i]) * d.scanner[i];
+      if (out_rgb) {
+        double cc[3];
+        spectrum_to_acescg(t, g, cc);
+        for (int k = 0; k < 3; ++k) out_rgb[p * 3 + k] = (float)(cc[k] * scale);
+      }
+      // reverse
+      double xyzbar[3];
+      for (int j = 0; j < 3; ++j) {
+        xyzbar[j] = 0.0;
+        for (int k = 0; k < 3; ++k) xyzbar[j] += XYZ2AP1[k][j] * scale * (double)gout[p * 3 + k];
+      }
+      double Dbar[8] = {0}, Mbar[8] = {0};
+      for (int i = 0; i < nb; ++i) {
+        const double tbar = tw[i] * (xyzbar[0] * Cm.c[0][i] + xyzbar[1] * Cm.c[1][i] + xyzbar[2] * Cm.c[2][i]);
+        const double densbar = -ln10 * t[i] * tbar;
+        for (size_t e = 0; e < E; ++e) {
+          Dbar[e] += densbar * d.em[e].eps[i];
+          if (d.em[e].has_mask) Mbar[e] += densbar * d.em[e].mask[i];
+        }
+      }
+      double bsum = 0.0;
+      for (size_t e = 0; e < E; ++e) {
+        const Emulsion& em = d.em[e];
+        const double dm = (double)em.d_max, ig = 1.0 / (double)std::max(em.gamma, 1e-6f);
+        double Db = Dbar[e], dmb = 0.0;
+        if (em.has_mask) {
+          const double q = D[e] / dm;
+          if (q > 0.0 && q < 1.0) {
+            Db += -(double)0.4f * Mbar[e];
+            dmb += (double)0.4f * Mbar[e];
+          } else if (q <= 0.0) {
+            dmb += (double)0.4f * Mbar[e];
+          }
+        }
+        const double c = eff[e];
+        dmb += Db * std::pow(c, ig);
+        const double gb = c > 0.0 ? Db * D[e] * std::log(c) * (-ig * ig) : 0.0;
+        const bool inside = fr[e] > 0.0 && fr[e] < 1.0;
+        const double fbar = inside && c > 0.0 ? Db * dm * ig * std::pow(c, ig - 1.0) : 0.0;
+        acc[e] += dmb;
+        acc[E + e] += gb;
+        for (size_t k = 0; k < K; ++k) {
+          double wk = 0.0;
+          if (clamp_lo[e]) wk = k == 0 ? 1.0 : 0.0;
+          else if (clamp_hi[e]) wk = k == K - 1 ? 1.0 : 0.0;
+          else if ((int)k == lo[e]) wk = 1.0 - tt[e];
+          else if ((int)k == lo[e] + 1) wk = tt[e];
+          acc[3 * E + e * K + k] += fbar * wk;
+        }
+        const double Pb = fbar * dfdx[e];
+        Sbar[e][p] = Pb;
+        if (do_wide && gain[e] > 0.0) {
+          bsum += gain[e] * Pb;
+          acc[2 * E + e] += Pb * Bn[p];
+        }
+      }
+      if (do_wide) Bbar[p] = bsum;
+    }
+  }
+  std::vector<double> sums(nsum, 0.0);
+  for (size_t y = 0; y < h; ++y)
+    for (size_t k = 0; k < nsum; ++k) sums[k] += rows[y * nsum + k];
+
+  *grad = FilmFitParams{};
+  grad->n_emulsions = (int)E;
+  grad->n_bins = nb;
+  grad->n_knots = (int)K;
+  for (size_t e = 0; e < E; ++e) {
+    grad->d_max[e] = sums[e];
+    grad->gamma[e] = sums[E + e];
+    for (size_t k = 0; k < K; ++k) grad->knot_frac[e][k] = sums[3 * E + e * K + k];
+  }
+  // ---- halation reverse
+  if (do_wide) {
+    std::vector<double> back, d1, d2;
+    sep_blur_transpose(Bbar, back, w, h, th.k, th.r);
+    for (size_t i = 0; i < n; ++i) Sbar[E - 1][i] += back[i];
+    sep_blur(S[E - 1], d1, w, h, th.k, th.dk, th.r);
+    sep_blur(S[E - 1], d2, w, h, th.dk, th.k, th.r);
+    double sb = 0.0;
+    for (size_t i = 0; i < n; ++i) sb += Bbar[i] * (d1[i] + d2[i]);
+    grad->hal_radius = sb * sigma_unit;
+    double hs = 0.0;
+    for (size_t e = 0; e < E; ++e)
+      if (gain[e] > 0.0) hs += sums[2 * E + e] * rb[e];
+    grad->hal_strength = hs;
+  }
+  // ---- local scatter reverse: Abar = 0.82 Sbar + 0.18 G^T Sbar
+  std::vector<std::vector<double>> Abar(E);
+  for (size_t e = 0; e < E; ++e) {
+    if (!do_local) {
+      Abar[e] = Sbar[e];
+      continue;
+    }
+    std::vector<double> back;
+    sep_blur_transpose(Sbar[e], back, w, h, tl.k, tl.r);
+    Abar[e].resize(n);
+    for (size_t i = 0; i < n; ++i) Abar[e][i] = keep * Sbar[e][i] + f_loc * back[i];
+  }
+  // ---- exposure reverse: Q[e][k] = sum_p Abar_e(p) w_k(p), then the transmittance chain
+  std::vector<double> Q(E * 3, 0.0);
+  for (size_t e = 0; e < E; ++e)
+    for (int k = 0; k < 3; ++k) {
+      double acc = 0.0;
+      for (size_t p = 0; p < n; ++p) acc += Abar[e][p] * wgt[p * 3 + k];
+      Q[e * 3 + k] = acc;
+    }
+  for (size_t e2 = 0; e2 < E; ++e2) {
+    const SrcLayer& L2 = src.layers[(size_t)em_layer[e2]];
+    const double alpha2 = 2.0 * (double)L2.agx * (double)L2.thickness_um;
+    for (int i = 0; i < nb; ++i) {
+      double gsum = 0.0;
+      for (size_t e = 0; e < E; ++e) {
+        // Phi_e(i) = sum_p Abar_e(p) w_i phi_i(p) / span
+        const double phi_sum = tw[i] / span / (550.0 / g.lam(i)) *
+                               (Q[e * 3] * Bs.b[0][i] + Q[e * 3 + 1] * Bs.b[1][i] + Q[e * 3 + 2] * Bs.b[2][i]);
+        const double Te = d.trans[(size_t)em_layer[e]][i];
+        double dtau = 0.0;  // d(tab_e (1 - T_e)) / dsens_e2(i)
+        if (e == e2) dtau = alpha2 * tab[e][i] * Te;
+        else if (em_layer[e2] < em_layer[e]) dtau = -alpha2 * tab[e][i] * (1.0 - Te);
+        gsum += phi_sum * dtau;
+      }
+      if (do_wide && gain[e2] > 0.0 && rden[e2] > 1e-9) {
+        const double dr = tw[i] * (d.reflect[i] * rden[e2] - rnum[e2]) / (rden[e2] * rden[e2]);
+        gsum += sums[2 * E + e2] * fit.hal_strength * bleed[e2] * dr;
+      }
+      grad->sens[e2][i] = gsum;
+    }
+  }
   return 0;
 }
 

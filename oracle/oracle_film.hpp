@@ -22,18 +22,35 @@ struct FilmExt {
   float hal_strength = -1.0f;  // >= 0: halation gain strength * bleed_e
 };
 
+// The differentiable film model: stock (7: the seeded synthetic stock), spectral
+// grid (16 or 81 bins) and pixel pitch.
+struct FilmFitModel {
+  int stock = 1;
+  int bins = 16;
+  uint64_t stock_seed = 0;
+  float pitch_um = 36000.0f / 6016.0f;
+};
+
 // Parameters of the differentiable film model (emulsions top to bottom).
 struct FilmFitParams {
-  int n_emulsions;
-  double sens[8][16];          // sensitivity samples 400..700 nm
-  double d_max[8], gamma[8];   // characteristic curve D = d_max f^(1/gamma)
-  double hal_strength;         // halation gain multiplier
-  double hal_radius;           // halation PSF width multiplier
+  int n_emulsions = 0;
+  int n_bins = 16;
+  int n_knots = 0;             // knot-curve stock: the developable fraction at each knot
+  double sens[8][81] = {};     // sensitivity samples on the grid
+  double knot_frac[8][8] = {};
+  double d_max[8] = {}, gamma[8] = {};  // characteristic curve D = d_max f^(1/gamma)
+  double hal_strength = 1.0;   // halation gain multiplier
+  double hal_radius = 1.0;     // halation PSF width multiplier
 };
 
 template <class P>
-int film_fit_process(float* rgb, size_t w, size_t h, int stock_idx, float pitch_um, const FilmFitParams& fit);
-int film_fit_nominal(int stock_idx, FilmFitParams* out);
+int film_fit_process(float* rgb, size_t w, size_t h, const FilmFitModel& m, const FilmFitParams& fit);
+int film_fit_nominal(const FilmFitModel& m, FilmFitParams* out);
+// dL/dparams of film_fit_process<double> on one image for dL/dout = gout (an f64
+// reverse pass written from the forward's equations); out_rgb, when given,
+// receives the forward's output (w x h x 3) in f32.
+int film_fit_gradient(const float* rgb, const float* gout, size_t w, size_t h, const FilmFitModel& m,
+                      const FilmFitParams& fit, FilmFitParams* grad, float* out_rgb = nullptr);
 
 // film::process on an interleaved ACEScg buffer (absolute luminance), in place.
 // output: 0 NegativeLinear, 1 PositiveLinear.  shutter_seconds <= 0: 1/box ISO.
