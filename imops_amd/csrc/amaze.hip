@@ -81,13 +81,15 @@ __device__ __forceinline__ float clampnan(float x, float m, float mx) {
 struct Ctx {
   const AmazeArgs& a;
   Tile t;
+  uint32_t cfa4;     // the 2x2 pattern, one byte per site ((r & 1) * 2 + (c & 1)), in a register
   float* s;          // scratch slot
   float* cfa;        // LDS 128x128
   float* hvwt;       // LDS 128x64
   float* pmwt;       // LDS 128x64
   uint8_t* nyq;      // LDS 128x64
   uint8_t* nyq2;     // LDS 128x64
-  __device__ int color_at(int r, int c) const { return a.cfa[r & 1][c & 1]; }
+  // (the kernel-argument bytes indexed per lane would be a memory load per call)
+  __device__ int color_at(int r, int c) const { return (int)((cfa4 >> ((((r & 1) << 1) | (c & 1)) << 3)) & 0xffu); }
   __device__ int par(int rr) const { return color_at(t.top + rr, t.left + 2) & 1; }
   __device__ float get_in(int r, int c) const {
     r = r < 0 ? 0 : (r > a.height - 1 ? a.height - 1 : r);
@@ -227,40 +229,40 @@ __device__ void pass_colour_diff(Ctx& c, int tid) {
 // One step of the variance selection + clamp of demosaic.rs:700-786 for one
 // direction.  `prev` is the already-updated value two pixels back along the
 // chain, `cur`/`next` are the values written by pass_colour_diff.
+// Straight-line form of the reference's nested branches: at a G site
+// (gsite) the reference works with gint = -v + ci and the clamp target
+// -ulim(gint) + ci, elsewhere with gint = v + ci and ulim(gint) - ci; both
+// sides' expressions are formed on the selected operands exactly as the
+// reference writes them (one division, whichever side the lane is on), and
+// the reference's branch conditions pick the result.  A chain step is a
+// serial dependence, so the branch-free form (no divergent paths, no
+// per-branch issue gaps) is what sets the pass time.
 __device__ __forceinline__ float cd_update(float prev, float cur, float next, float aprev, float acur, float anext,
                                            bool gsite, float ci, float nm, float np) {
   float var = 3.0f * (sqr(prev) + sqr(cur) + sqr(next)) - sqr(prev + cur + next);
   float avar = 3.0f * (sqr(aprev) + sqr(acur) + sqr(anext)) - sqr(aprev + acur + anext);
-  float v = cur;
-  if (avar < var) v = acur;
-  if (gsite) {
-    const float gint = -v + ci;
-    if (v > 0.0f) {
-      if (3.0f * v > (gint + ci)) {
-        v = -ulim(gint, nm, np) + ci;
-      } else {
-        float w = 1.0f - 3.0f * v / (eps + gint + ci);
-        v = w * v + (1.0f - w) * (-ulim(gint, nm, np) + ci);
-      }
-    }
-    if (gint > clip_pt) v = -ulim(gint, nm, np) + ci;
-  } else {
-    const float gint = v + ci;
-    if (v < 0.0f) {
-      if (3.0f * v < -(gint + ci)) {
-        v = ulim(gint, nm, np) - ci;
-      } else {
-        float w = 1.0f + 3.0f * v / (eps + gint + ci);
-        v = w * v + (1.0f - w) * (ulim(gint, nm, np) - ci);
-      }
-    }
-    if (gint > clip_pt) v = ulim(gint, nm, np) - ci;
-  }
-  return v;
+  const float v = avar < var ? acur : cur;
+  const float gint = gsite ? -v + ci : v + ci;
+  const float u = ulim(gint, nm, np);
+  const float target = gsite ? -u + ci : u - ci;  // the clamp value
+  const float q = 3.0f * v / (eps + gint + ci);
+  const float w = gsite ? 1.0f - q : 1.0f + q;
+  const float blend = w * v + (1.0f - w) * target;
+  const bool side = gsite ? v > 0.0f : v < 0.0f;                       // the reference's first test
+  const bool hard = gsite ? 3.0f * v > (gint + ci) : 3.0f * v < -(gint + ci);  // its second
+  float r = side ? (hard ? target : blend) : v;
+  if (gint > clip_pt) r = target;
+  return r;
 }
 
 // hcd chains along rows (threads 0..255) and vcd chains along columns
-// (threads 256..511); each thread walks both parities of its line.
+// (threads 256..511); each thread walks both parities of its line.  A step
+// reads the not-yet-updated values two pixels ahead (`next`), which no step
+// of the chain has written yet (a step writes only its own pixel), so they
+// are loaded a block of kAhead steps early: the loads of block k + 1 are in
+// flight while block k runs, instead of one slot round trip per step.
+constexpr int kAhead = 4;
+
 __device__ void pass_cd_chains(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
@@ -274,20 +276,51 @@ __device__ void pass_cd_chains(Ctx& c, int tid) {
   if (lline >= hline) return;
   const int step = horiz ? 1 : V1;   // index distance of one pixel along the chain
   const int base = horiz ? lline * TS : lline;
-  int pos = lo + parity;
-  if (pos >= hi) return;
-  int i = base + pos * step;
-  float prev = cd[i - 2 * step];
-  float aprev = alt[i - 2 * step];
-  float cur = cd[i], acur = alt[i];
-  for (; pos < hi; pos += 2, i += 2 * step) {
-    const float next = cd[i + 2 * step], anext = alt[i + 2 * step];
-    const int rr = horiz ? lline : pos, cc = horiz ? pos : lline;
-    const bool gsite = (c.color_at(t.top + rr, t.left + cc) & 1) != 0;
-    const float v = cd_update(prev, cur, next, aprev, acur, anext, gsite, cfa[i], cfa[i - step], cfa[i + step]);
-    cd[i] = v;
-    prev = v; aprev = acur;
-    cur = next; acur = anext;
+  const int pos0 = lo + parity;
+  if (pos0 >= hi) return;
+  const int i0 = base + pos0 * step;
+  float prev = cd[i0 - 2 * step];
+  float aprev = alt[i0 - 2 * step];
+  float cur = cd[i0], acur = alt[i0];
+  // nx[k], anx[k]: `next` of step s0 + k (the values at pixel i0 + 2 step (s0 + k + 1))
+  float nx[kAhead], anx[kAhead];
+#pragma unroll
+  for (int k = 0; k < kAhead; ++k) {
+    nx[k] = anx[k] = 0.0f;
+    if (pos0 + 2 * k < hi) {
+      nx[k] = cd[i0 + 2 * step * (k + 1)];
+      anx[k] = alt[i0 + 2 * step * (k + 1)];
+    }
+  }
+  for (int s0 = 0; pos0 + 2 * s0 < hi; s0 += kAhead) {
+    float ny[kAhead], any[kAhead];
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      ny[k] = any[k] = 0.0f;
+      const int sk = s0 + kAhead + k;
+      if (pos0 + 2 * sk < hi) {
+        ny[k] = cd[i0 + 2 * step * (sk + 1)];
+        any[k] = alt[i0 + 2 * step * (sk + 1)];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      const int pos = pos0 + 2 * (s0 + k);
+      if (pos >= hi) break;
+      const int i = i0 + 2 * step * (s0 + k);
+      const float next = nx[k], anext = anx[k];
+      const int rr = horiz ? lline : pos, cc = horiz ? pos : lline;
+      const bool gsite = (c.color_at(t.top + rr, t.left + cc) & 1) != 0;
+      const float v = cd_update(prev, cur, next, aprev, acur, anext, gsite, cfa[i], cfa[i - step], cfa[i + step]);
+      cd[i] = v;
+      prev = v; aprev = acur;
+      cur = next; acur = anext;
+    }
+#pragma unroll
+    for (int k = 0; k < kAhead; ++k) {
+      nx[k] = ny[k];
+      anx[k] = any[k];
+    }
   }
 }
 
@@ -441,23 +474,42 @@ __device__ void pass_nyq_area(Ctx& c, int tid, const int* bb) {
 
 // Diagonal smoothing of a half-resolution weight plane, in place, row by row
 // in one wave (demosaic.rs:951-956 for hvwt, 1113-1117 for pmwt): a row reads
-// the already-smoothed row above and the untouched row below.
+// the already-smoothed row above and the untouched row below.  Lane l owns
+// half-plane column j = margin/2 + l of every row (site column 2j + parity);
+// on a Bayer pattern the row parity alternates, so the diagonal neighbours of
+// lane l are lanes l - 1 + p and l + p of the rows above and below (p = this
+// row's parity).  The smoothed row above stays in registers and the row below
+// is this wave's next read, so a row costs one LDS read and four lane
+// exchanges instead of a round trip through LDS per row; the one neighbour
+// outside the wave (lane -1, a column left of the margin that the smoothing
+// never writes) is read from LDS.
 __device__ void pass_diag_smooth(Ctx& c, int lane, float* w, int r0, int r1, int margin) {
+  if (r0 >= r1) return;
+  const int jl = margin / 2 + lane;  // this lane's half-plane column
+  const bool inrow = jl < 64;
+  auto own = [&](int rr) { return inrow ? w[rr * 64 + jl] : 0.0f; };
+  float above = own(r0 - 1);  // row r0 - 1 is outside the smoothed range: unchanged
+  float cur = own(r0);
   for (int rr = r0; rr < r1; ++rr) {
     const int p = c.par(rr);
-    const int cc = margin + p + 2 * lane;
-    float nv = 0.0f;
-    bool act = cc < c.t.cc1 - margin;
-    const int i = rr * TS + cc;
-    if (act) {
-      float alt = (w[(i - M1) >> 1] + w[(i - P1) >> 1] + w[(i + P1) >> 1] + w[(i + M1) >> 1]) / 4.0f;
-      float cur = w[i >> 1];
-      nv = fabsf(0.5f - cur) < fabsf(0.5f - alt) ? alt : cur;
+    const float below = own(rr + 1);  // rows below are not smoothed yet
+    const int sa = lane - 1 + p, sb = lane + p;
+    float a = __shfl(above, sa < 0 ? 0 : sa, 64), b = __shfl(above, sb > 63 ? 63 : sb, 64);
+    float cv = __shfl(below, sa < 0 ? 0 : sa, 64), d = __shfl(below, sb > 63 ? 63 : sb, 64);
+    if (sa < 0) {  // column margin/2 - 1: left of the smoothed range in both rows
+      a = w[(rr - 1) * 64 + margin / 2 - 1];
+      cv = w[(rr + 1) * 64 + margin / 2 - 1];
     }
-    __builtin_amdgcn_wave_barrier();
-    if (act) w[i >> 1] = nv;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    const int cc = margin + p + 2 * lane;
+    const bool act = cc < c.t.cc1 - margin;
+    float nv = cur;
+    if (act) {
+      const float alt = (a + b + cv + d) / 4.0f;
+      nv = fabsf(0.5f - cur) < fabsf(0.5f - alt) ? alt : cur;
+      w[rr * 64 + jl] = nv;
+    }
+    above = nv;
+    cur = below;
   }
 }
 
@@ -815,8 +867,10 @@ __global__ void __launch_bounds__(NT) amaze_kernel(AmazeArgs a) {
     if (a.cfa[0][0] == 0) { ex = 0; ey = 0; } else { ex = 1; ey = 1; }
   }
 
+  const uint32_t cfa4 = (uint32_t)a.cfa[0][0] | ((uint32_t)a.cfa[0][1] << 8) | ((uint32_t)a.cfa[1][0] << 16) |
+                        ((uint32_t)a.cfa[1][1] << 24);
   for (int tile = blockIdx.x; tile < a.tiles_x * a.tiles_y; tile += gridDim.x) {
-    Ctx c{a, {}, slot, s_cfa, s_hvwt, s_pmwt, s_nyq, s_nyq2};
+    Ctx c{a, {}, cfa4, slot, s_cfa, s_hvwt, s_pmwt, s_nyq, s_nyq2};
     const int ty = tile / a.tiles_x, tx = tile % a.tiles_x;
     Tile& t = c.t;
     t.top = -16 + 96 * ty;
