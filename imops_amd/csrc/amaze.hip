@@ -263,10 +263,43 @@ __device__ __forceinline__ float cd_update(float prev, float cur, float next, fl
 // flight while block k runs, instead of one slot round trip per step.
 constexpr int kAhead = 4;
 
+// The Nyquist test's gradient term (demosaic.rs:830-848, the second sum) at
+// site i; it reads only delhvsqsum, so it is formed while the chains run.
+__device__ __forceinline__ float nyq_grad_term(const float* delhvsqsum, int i) {
+  const float nyqthresh = 0.5f;
+  const float gg0 = nyqthresh * 0.07384411893421103f, gg1 = nyqthresh * 0.06207511968171489f,
+              gg2 = nyqthresh * 0.0521818194747806f, gg3 = nyqthresh * 0.03687419286733595f,
+              gg4 = nyqthresh * 0.03099732204057846f, gg5 = nyqthresh * 0.018413194161458882f;
+  return gg0 * delhvsqsum[i] +
+         gg1 * (delhvsqsum[i - V1] + delhvsqsum[i + 1] + delhvsqsum[i - 1] + delhvsqsum[i + V1]) +
+         gg2 * (delhvsqsum[i - M1] + delhvsqsum[i - P1] + delhvsqsum[i + P1] + delhvsqsum[i + M1]) +
+         gg3 * (delhvsqsum[i - V2] + delhvsqsum[i - 2] + delhvsqsum[i + 2] + delhvsqsum[i + V2]) +
+         gg4 * (delhvsqsum[i - V2 - 1] + delhvsqsum[i - V2 + 1] + delhvsqsum[i - TS - 2] + delhvsqsum[i - TS + 2] +
+                delhvsqsum[i + TS - 2] + delhvsqsum[i + TS + 2] + delhvsqsum[i + V2 - 1] + delhvsqsum[i + V2 + 1]) +
+         gg5 * (delhvsqsum[i - M2] + delhvsqsum[i - P2] + delhvsqsum[i + P2] + delhvsqsum[i + M2]);
+}
+
+// Threads 512..1023 while the chains run: nyq_grad_term of every site
+// pass_hvwt_nyq evaluates, into the pmwt LDS plane (unused until pass_diag).
+__device__ void pass_nyq_grad(Ctx& c, int tid, const float* __restrict__ delhvsqsum) {
+  const Tile& t = c.t;
+  for (int h = tid - 512; h < 8192; h += 512) {
+    const int rr = h >> 6, j = h & 63;
+    if (rr < 6 || rr >= t.rr1 - 6) continue;
+    const int p = c.par(rr);
+    const int cc = 2 * j + p;
+    if (cc < 6 + p || cc >= t.cc1 - 6) continue;
+    c.pmwt[h] = nyq_grad_term(delhvsqsum, rr * TS + cc);
+  }
+}
+
 __device__ void pass_cd_chains(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
-  if (tid >= 512) return;  // 128 lines x 2 parities per direction
+  if (tid >= 512) {  // 128 lines x 2 parities per direction; the rest form the Nyquist gradient term
+    pass_nyq_grad(c, tid, c.P(S_DELHVSQSUM));
+    return;
+  }
   const bool horiz = tid < 256;
   const int line = (tid & 255) >> 1, parity = tid & 1;
   float* cd = c.P(horiz ? S_HCD : S_VCD);
@@ -349,13 +382,8 @@ __device__ void pass_hvwt_nyq(Ctx& c, int tid, int* bbox) {
   const float* dgintv = c.P(S_DGINTV);
   const float* dginth = c.P(S_DGINTH);
   const float* cddiffsq = c.P(S_CDDIFFSQ);
-  const float* delhvsqsum = c.P(S_DELHVSQSUM);
   const float gaussodd0 = 0.14659727707323927f, gaussodd1 = 0.103592713382435f, gaussodd2 = 0.0732036125103057f,
               gaussodd3 = 0.0365543548389495f;
-  const float nyqthresh = 0.5f;
-  const float gg0 = nyqthresh * 0.07384411893421103f, gg1 = nyqthresh * 0.06207511968171489f,
-              gg2 = nyqthresh * 0.0521818194747806f, gg3 = nyqthresh * 0.03687419286733595f,
-              gg4 = nyqthresh * 0.03099732204057846f, gg5 = nyqthresh * 0.018413194161458882f;
   int minr = INT32_MAX, maxr = -1, minc = INT32_MAX, maxc = -1;
   for (int h = tid; h < 8192; h += NT) {
     const int rr = h >> 6, j = h & 63;
@@ -391,14 +419,7 @@ __device__ void pass_hvwt_nyq(Ctx& c, int tid, int* bbox) {
                     gaussodd1 * (cddiffsq[i - M1] + cddiffsq[i - P1] + cddiffsq[i + P1] + cddiffsq[i + M1]) +
                     gaussodd2 * (cddiffsq[i - V2] + cddiffsq[i - 2] + cddiffsq[i + 2] + cddiffsq[i + V2]) +
                     gaussodd3 * (cddiffsq[i - M2] + cddiffsq[i - P2] + cddiffsq[i + P2] + cddiffsq[i + M2])) -
-                   (gg0 * delhvsqsum[i] +
-                    gg1 * (delhvsqsum[i - V1] + delhvsqsum[i + 1] + delhvsqsum[i - 1] + delhvsqsum[i + V1]) +
-                    gg2 * (delhvsqsum[i - M1] + delhvsqsum[i - P1] + delhvsqsum[i + P1] + delhvsqsum[i + M1]) +
-                    gg3 * (delhvsqsum[i - V2] + delhvsqsum[i - 2] + delhvsqsum[i + 2] + delhvsqsum[i + V2]) +
-                    gg4 * (delhvsqsum[i - V2 - 1] + delhvsqsum[i - V2 + 1] + delhvsqsum[i - TS - 2] +
-                           delhvsqsum[i - TS + 2] + delhvsqsum[i + TS - 2] + delhvsqsum[i + TS + 2] +
-                           delhvsqsum[i + V2 - 1] + delhvsqsum[i + V2 + 1]) +
-                    gg5 * (delhvsqsum[i - M2] + delhvsqsum[i - P2] + delhvsqsum[i + P2] + delhvsqsum[i + M2]));
+                   c.pmwt[h];  // nyq_grad_term, formed while the chains ran
         if (nt > 0.0f) {
           ny = 1;
           minr = min(minr, rr); maxr = max(maxr, rr);
