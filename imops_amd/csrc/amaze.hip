@@ -42,16 +42,14 @@ constexpr float clip_pt = 1.0f, clip_pt8 = 1.0f * 0.8f;
 enum : int {
   S_RGBGREEN = 0,
   S_DELHVSQSUM = 1 * 16384,
-  S_DIRWTS0 = 2 * 16384,
-  S_DIRWTS1 = 3 * 16384,
-  S_VCD = 4 * 16384,
-  S_HCD = 5 * 16384,
-  S_VCDALT = 6 * 16384,
-  S_HCDALT = 7 * 16384,
-  S_CDDIFFSQ = 8 * 16384,
-  S_DGINTV = 9 * 16384,
-  S_DGINTH = 10 * 16384,
-  S_HALF = 11 * 16384,
+  S_VCD = 2 * 16384,
+  S_HCD = 3 * 16384,
+  S_VCDALT = 4 * 16384,
+  S_HCDALT = 5 * 16384,
+  S_CDDIFFSQ = 6 * 16384,
+  S_DGINTV = 7 * 16384,
+  S_DGINTH = 8 * 16384,
+  S_HALF = 9 * 16384,
   S_DGRB0 = S_HALF + 0 * 8192,
   S_DGRB1 = S_HALF + 1 * 8192,
   S_DELP = S_HALF + 2 * 8192,
@@ -138,25 +136,32 @@ __device__ void pass_fill(Ctx& c, int tid) {
   }
 }
 
-// demosaic.rs:612-622
+// dirwts[0] and dirwts[1] of demosaic.rs:612-622 at site i, formed from the
+// LDS CFA tile where a pass needs them instead of being kept as two planes of
+// the scratch slot (every read is at rows and columns 2 .. rr1-3 / cc1-3, the
+// reference's computed range).
+__device__ __forceinline__ float dirw0(const float* cfa, int i) {
+  const float delv = fabsf(cfa[i + V1] - cfa[i - V1]);
+  return eps + fabsf(cfa[i + V2] - cfa[i]) + fabsf(cfa[i] - cfa[i - V2]) + delv;
+}
+__device__ __forceinline__ float dirw1(const float* cfa, int i) {
+  const float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
+  return eps + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
+}
+
+// demosaic.rs:612-622 (delhvsqsum; the two dirwts planes are dirw0 / dirw1)
 __device__ void pass_gradients(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
-  float* dirwts0 = c.P(S_DIRWTS0);
-  float* dirwts1 = c.P(S_DIRWTS1);
   float* delhvsqsum = c.P(S_DELHVSQSUM);
   for (int i = tid; i < 16384; i += NT) {
     const int rr = i >> 7, cc = i & 127;
-    float d0 = 0.0f, d1 = 0.0f, dq = 0.0f;
+    float dq = 0.0f;
     if (rr >= 2 && rr < t.rr1 - 2 && cc >= 2 && cc < t.cc1 - 2) {
       float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
       float delv = fabsf(cfa[i + V1] - cfa[i - V1]);
-      d0 = eps + fabsf(cfa[i + V2] - cfa[i]) + fabsf(cfa[i] - cfa[i - V2]) + delv;
-      d1 = eps + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
       dq = sqr(delh) + sqr(delv);
     }
-    dirwts0[i] = d0;
-    dirwts1[i] = d1;
     delhvsqsum[i] = dq;
   }
 }
@@ -165,8 +170,6 @@ __device__ void pass_gradients(Ctx& c, int tid) {
 __device__ void pass_colour_diff(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
-  const float* dirwts0 = c.P(S_DIRWTS0);
-  const float* dirwts1 = c.P(S_DIRWTS1);
   float* vcd = c.P(S_VCD);
   float* hcd = c.P(S_HCD);
   float* vcdalt = c.P(S_VCDALT);
@@ -181,8 +184,8 @@ __device__ void pass_colour_diff(Ctx& c, int tid) {
     }
     const bool fcswitch = (((c.color_at(t.top + rr, t.left + 4) & 1) != 0) != (((cc - 4) & 1) != 0));
     const float ci = cfa[i];
-    const float dw0m2 = dirwts0[i - V2], dw0 = dirwts0[i], dw0p2 = dirwts0[i + V2];
-    const float dw1m2 = dirwts1[i - 2], dw1 = dirwts1[i], dw1p2 = dirwts1[i + 2];
+    const float dw0m2 = dirw0(cfa, i - V2), dw0 = dirw0(cfa, i), dw0p2 = dirw0(cfa, i + V2);
+    const float dw1m2 = dirw1(cfa, i - 2), dw1 = dirw1(cfa, i), dw1p2 = dirw1(cfa, i + 2);
     float cru = cfa[i - V1] * (dw0m2 + dw0) / fmaxf(dw0m2 * (eps + ci) + dw0 * (eps + cfa[i - V2]), 1e-10f);
     float crd = cfa[i + V1] * (dw0p2 + dw0) / fmaxf(dw0p2 * (eps + ci) + dw0 * (eps + cfa[i + V2]), 1e-10f);
     float crl = cfa[i - 1] * (dw1m2 + dw1) / fmaxf(dw1m2 * (eps + ci) + dw1 * (eps + cfa[i - 2]), 1e-10f);
@@ -195,8 +198,10 @@ __device__ void pass_colour_diff(Ctx& c, int tid) {
     float gdar = fabsf(1.0f - crd) < arthresh ? ci * crd : gdha;
     float glar = fabsf(1.0f - crl) < arthresh ? ci * crl : glha;
     float grar = fabsf(1.0f - crr) < arthresh ? ci * crr : grha;
-    float hwt = dirwts1[i - 1] / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
-    float vwt = dirwts0[i - V1] / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
+    const float dw1l = dirw1(cfa, i - 1), dw1r = dirw1(cfa, i + 1);
+    const float dw0u = dirw0(cfa, i - V1), dw0d = dirw0(cfa, i + V1);
+    float hwt = dw1l / fmaxf(dw1l + dw1r, 1e-10f);
+    float vwt = dw0u / fmaxf(dw0d + dw0u, 1e-10f);
     float gintvha = vwt * gdha + (1.0f - vwt) * guha;
     float ginthha = hwt * grha + (1.0f - hwt) * glha;
     float vcd_v, hcd_v, vcdalt_v, hcdalt_v;
@@ -377,8 +382,7 @@ __device__ void pass_hvwt_nyq(Ctx& c, int tid, int* bbox) {
   const Tile& t = c.t;
   const float* vcd = c.P(S_VCD);
   const float* hcd = c.P(S_HCD);
-  const float* dirwts0 = c.P(S_DIRWTS0);
-  const float* dirwts1 = c.P(S_DIRWTS1);
+  const float* cfa = c.cfa;
   const float* dgintv = c.P(S_DGINTV);
   const float* dginth = c.P(S_DGINTH);
   const float* cddiffsq = c.P(S_CDDIFFSQ);
@@ -402,8 +406,10 @@ __device__ void pass_hvwt_nyq(Ctx& c, int tid, int* bbox) {
         float dgrbvvard = sqr(vcd[i] - dave) + sqr(vcd[i + V1] - dave) + sqr(vcd[i + V2] - dave) + sqr(vcd[i + V3] - dave);
         float dgrbhvarl = sqr(hcd[i] - lave) + sqr(hcd[i - 1] - lave) + sqr(hcd[i - 2] - lave) + sqr(hcd[i - 3] - lave);
         float dgrbhvarr = sqr(hcd[i] - rave) + sqr(hcd[i + 1] - rave) + sqr(hcd[i + 2] - rave) + sqr(hcd[i + 3] - rave);
-        float hwt = dirwts1[i - 1] / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
-        float vwt = dirwts0[i - V1] / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
+        const float dw1l = dirw1(cfa, i - 1), dw1r = dirw1(cfa, i + 1);
+        const float dw0u = dirw0(cfa, i - V1), dw0d = dirw0(cfa, i + V1);
+        float hwt = dw1l / fmaxf(dw1l + dw1r, 1e-10f);
+        float vwt = dw0u / fmaxf(dw0d + dw0u, 1e-10f);
         float vcdvar = epssq + vwt * dgrbvvard + (1.0f - vwt) * dgrbvvaru;
         float hcdvar = epssq + hwt * dgrbhvarr + (1.0f - hwt) * dgrbhvarl;
         float dgrbvvaru1 = dgintv[i] + dgintv[i - V1] + dgintv[i - V2];
@@ -729,8 +735,6 @@ __device__ void pass_green_diag(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
   const float* rbint = c.P(S_RBINT);
-  const float* dirwts0 = c.P(S_DIRWTS0);
-  const float* dirwts1 = c.P(S_DIRWTS1);
   float* rgbgreen = c.P(S_RGBGREEN);
   float* dgrb0 = c.P(S_DGRB0);
   for (int h = tid; h < 8192; h += NT) {
@@ -750,8 +754,10 @@ __device__ void pass_green_diag(Ctx& c, int tid) {
     float gd = fabsf(1.0f - crd) < arthresh ? rb * crd : cfa[i + V1] + (rb - rbint[i1 + V1]) * 0.5f;
     float gl = fabsf(1.0f - crl) < arthresh ? rb * crl : cfa[i - 1] + (rb - rbint[i1 - 1]) * 0.5f;
     float gr = fabsf(1.0f - crr) < arthresh ? rb * crr : cfa[i + 1] + (rb - rbint[i1 + 1]) * 0.5f;
-    float gintv = (dirwts0[i - V1] * gd + dirwts0[i + V1] * gu) / fmaxf(dirwts0[i + V1] + dirwts0[i - V1], 1e-10f);
-    float ginth = (dirwts1[i - 1] * gr + dirwts1[i + 1] * gl) / fmaxf(dirwts1[i - 1] + dirwts1[i + 1], 1e-10f);
+    const float dw0u = dirw0(cfa, i - V1), dw0d = dirw0(cfa, i + V1);
+    const float dw1l = dirw1(cfa, i - 1), dw1r = dirw1(cfa, i + 1);
+    float gintv = (dw0u * gd + dw0d * gu) / fmaxf(dw0d + dw0u, 1e-10f);
+    float ginth = (dw1l * gr + dw1r * gl) / fmaxf(dw1l + dw1r, 1e-10f);
     if (gintv < rb) {
       if (2.0f * gintv < rb) {
         gintv = ulim(gintv, cfa[i - V1], cfa[i + V1]);
