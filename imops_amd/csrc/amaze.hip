@@ -41,15 +41,14 @@ constexpr float clip_pt = 1.0f, clip_pt8 = 1.0f * 0.8f;
 // Offsets (in floats) of the planes inside one scratch slot.
 enum : int {
   S_RGBGREEN = 0,
-  S_DELHVSQSUM = 1 * 16384,
-  S_VCD = 2 * 16384,
-  S_HCD = 3 * 16384,
-  S_VCDALT = 4 * 16384,
-  S_HCDALT = 5 * 16384,
-  S_CDDIFFSQ = 6 * 16384,
-  S_DGINTV = 7 * 16384,
-  S_DGINTH = 8 * 16384,
-  S_HALF = 9 * 16384,
+  S_VCD = 1 * 16384,
+  S_HCD = 2 * 16384,
+  S_VCDALT = 3 * 16384,
+  S_HCDALT = 4 * 16384,
+  S_CDDIFFSQ = 5 * 16384,
+  S_DGINTV = 6 * 16384,
+  S_DGINTH = 7 * 16384,
+  S_HALF = 8 * 16384,
   S_DGRB0 = S_HALF + 0 * 8192,
   S_DGRB1 = S_HALF + 1 * 8192,
   S_DELP = S_HALF + 2 * 8192,
@@ -147,23 +146,6 @@ __device__ __forceinline__ float dirw0(const float* cfa, int i) {
 __device__ __forceinline__ float dirw1(const float* cfa, int i) {
   const float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
   return eps + fabsf(cfa[i + 2] - cfa[i]) + fabsf(cfa[i] - cfa[i - 2]) + delh;
-}
-
-// demosaic.rs:612-622 (delhvsqsum; the two dirwts planes are dirw0 / dirw1)
-__device__ void pass_gradients(Ctx& c, int tid) {
-  const Tile& t = c.t;
-  const float* cfa = c.cfa;
-  float* delhvsqsum = c.P(S_DELHVSQSUM);
-  for (int i = tid; i < 16384; i += NT) {
-    const int rr = i >> 7, cc = i & 127;
-    float dq = 0.0f;
-    if (rr >= 2 && rr < t.rr1 - 2 && cc >= 2 && cc < t.cc1 - 2) {
-      float delh = fabsf(cfa[i + 1] - cfa[i - 1]);
-      float delv = fabsf(cfa[i + V1] - cfa[i - V1]);
-      dq = sqr(delh) + sqr(delv);
-    }
-    delhvsqsum[i] = dq;
-  }
 }
 
 // demosaic.rs:626-698
@@ -268,25 +250,35 @@ __device__ __forceinline__ float cd_update(float prev, float cur, float next, fl
 // flight while block k runs, instead of one slot round trip per step.
 constexpr int kAhead = 4;
 
+// delhvsqsum of demosaic.rs:612-622 at site p (inside rows and columns
+// 2 .. rr1-3 / cc1-3, where the reference computes it), from the LDS CFA tile.
+__device__ __forceinline__ float delhvsq(const float* cfa, int p) {
+  const float delh = fabsf(cfa[p + 1] - cfa[p - 1]);
+  const float delv = fabsf(cfa[p + V1] - cfa[p - V1]);
+  return sqr(delh) + sqr(delv);
+}
+
 // The Nyquist test's gradient term (demosaic.rs:830-848, the second sum) at
-// site i; it reads only delhvsqsum, so it is formed while the chains run.
-__device__ __forceinline__ float nyq_grad_term(const float* delhvsqsum, int i) {
+// site i (its 5x5 neighbourhood lies inside delhvsqsum's computed range at
+// every site pass_hvwt_nyq evaluates); it reads only the CFA tile, so it is
+// formed while the chains run and delhvsqsum is never stored.
+__device__ __forceinline__ float nyq_grad_term(const float* cfa, int i) {
+  auto dq = [cfa](int p) { return delhvsq(cfa, p); };
   const float nyqthresh = 0.5f;
   const float gg0 = nyqthresh * 0.07384411893421103f, gg1 = nyqthresh * 0.06207511968171489f,
               gg2 = nyqthresh * 0.0521818194747806f, gg3 = nyqthresh * 0.03687419286733595f,
               gg4 = nyqthresh * 0.03099732204057846f, gg5 = nyqthresh * 0.018413194161458882f;
-  return gg0 * delhvsqsum[i] +
-         gg1 * (delhvsqsum[i - V1] + delhvsqsum[i + 1] + delhvsqsum[i - 1] + delhvsqsum[i + V1]) +
-         gg2 * (delhvsqsum[i - M1] + delhvsqsum[i - P1] + delhvsqsum[i + P1] + delhvsqsum[i + M1]) +
-         gg3 * (delhvsqsum[i - V2] + delhvsqsum[i - 2] + delhvsqsum[i + 2] + delhvsqsum[i + V2]) +
-         gg4 * (delhvsqsum[i - V2 - 1] + delhvsqsum[i - V2 + 1] + delhvsqsum[i - TS - 2] + delhvsqsum[i - TS + 2] +
-                delhvsqsum[i + TS - 2] + delhvsqsum[i + TS + 2] + delhvsqsum[i + V2 - 1] + delhvsqsum[i + V2 + 1]) +
-         gg5 * (delhvsqsum[i - M2] + delhvsqsum[i - P2] + delhvsqsum[i + P2] + delhvsqsum[i + M2]);
+  return gg0 * dq(i) + gg1 * (dq(i - V1) + dq(i + 1) + dq(i - 1) + dq(i + V1)) +
+         gg2 * (dq(i - M1) + dq(i - P1) + dq(i + P1) + dq(i + M1)) +
+         gg3 * (dq(i - V2) + dq(i - 2) + dq(i + 2) + dq(i + V2)) +
+         gg4 * (dq(i - V2 - 1) + dq(i - V2 + 1) + dq(i - TS - 2) + dq(i - TS + 2) + dq(i + TS - 2) + dq(i + TS + 2) +
+                dq(i + V2 - 1) + dq(i + V2 + 1)) +
+         gg5 * (dq(i - M2) + dq(i - P2) + dq(i + P2) + dq(i + M2));
 }
 
 // Threads 512..1023 while the chains run: nyq_grad_term of every site
 // pass_hvwt_nyq evaluates, into the pmwt LDS plane (unused until pass_diag).
-__device__ void pass_nyq_grad(Ctx& c, int tid, const float* __restrict__ delhvsqsum) {
+__device__ void pass_nyq_grad(Ctx& c, int tid) {
   const Tile& t = c.t;
   for (int h = tid - 512; h < 8192; h += 512) {
     const int rr = h >> 6, j = h & 63;
@@ -294,7 +286,7 @@ __device__ void pass_nyq_grad(Ctx& c, int tid, const float* __restrict__ delhvsq
     const int p = c.par(rr);
     const int cc = 2 * j + p;
     if (cc < 6 + p || cc >= t.cc1 - 6) continue;
-    c.pmwt[h] = nyq_grad_term(delhvsqsum, rr * TS + cc);
+    c.pmwt[h] = nyq_grad_term(c.cfa, rr * TS + cc);
   }
 }
 
@@ -302,7 +294,7 @@ __device__ void pass_cd_chains(Ctx& c, int tid) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
   if (tid >= 512) {  // 128 lines x 2 parities per direction; the rest form the Nyquist gradient term
-    pass_nyq_grad(c, tid, c.P(S_DELHVSQSUM));
+    pass_nyq_grad(c, tid);
     return;
   }
   const bool horiz = tid < 256;
@@ -912,8 +904,6 @@ __global__ void __launch_bounds__(NT) amaze_kernel(AmazeArgs a) {
 
     if (tid == 0) { s_bbox[0] = INT32_MAX; s_bbox[1] = -1; s_bbox[2] = INT32_MAX; s_bbox[3] = -1; }
     pass_fill(c, tid);
-    __syncthreads();
-    pass_gradients(c, tid);
     __syncthreads();
     pass_colour_diff(c, tid);
     __syncthreads();

@@ -8,7 +8,7 @@ namespace imops {
 
 // Floats of global scratch one workgroup uses for one 128x128 tile:
 // 11 full-resolution and 11 half-resolution f32 planes.
-constexpr size_t kAmazeSlotFloats = 9 * 16384 + 11 * 8192;
+constexpr size_t kAmazeSlotFloats = 8 * 16384 + 11 * 8192;
 
 struct AmazeArgs {
   const float* in;   // normalised CFA plane, width*height (or null: read in16)
