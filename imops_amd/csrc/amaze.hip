@@ -309,6 +309,8 @@ __device__ void pass_cd_chains(Ctx& c, int tid) {
   const int pos0 = lo + parity;
   if (pos0 >= hi) return;
   const int i0 = base + pos0 * step;
+  // the colour is the same at every pixel of a chain (it steps by two pixels)
+  const bool gsite = (c.color_at(t.top + (horiz ? lline : pos0), t.left + (horiz ? pos0 : lline)) & 1) != 0;
   float prev = cd[i0 - 2 * step];
   float aprev = alt[i0 - 2 * step];
   float cur = cd[i0], acur = alt[i0];
@@ -333,15 +335,18 @@ __device__ void pass_cd_chains(Ctx& c, int tid) {
         any[k] = alt[i0 + 2 * step * (sk + 1)];
       }
     }
+    // the block's CFA values: cf[m] = cfa at pixel i0 + (2 s0 - 1 + m) step (read-only here);
+    // positions past the chain's end are clamped into the tile and never used
+    float cf[2 * kAhead + 1];
+#pragma unroll
+    for (int m = 0; m < 2 * kAhead + 1; ++m) cf[m] = cfa[min(i0 + (2 * s0 - 1 + m) * step, 16383)];
 #pragma unroll
     for (int k = 0; k < kAhead; ++k) {
       const int pos = pos0 + 2 * (s0 + k);
       if (pos >= hi) break;
       const int i = i0 + 2 * step * (s0 + k);
       const float next = nx[k], anext = anx[k];
-      const int rr = horiz ? lline : pos, cc = horiz ? pos : lline;
-      const bool gsite = (c.color_at(t.top + rr, t.left + cc) & 1) != 0;
-      const float v = cd_update(prev, cur, next, aprev, acur, anext, gsite, cfa[i], cfa[i - step], cfa[i + step]);
+      const float v = cd_update(prev, cur, next, aprev, acur, anext, gsite, cf[2 * k + 1], cf[2 * k], cf[2 * k + 2]);
       cd[i] = v;
       prev = v; aprev = acur;
       cur = next; acur = anext;
