@@ -538,7 +538,7 @@ __device__ void pass_diag_smooth(Ctx& c, int lane, float* w, int r0, int r1, int
 }
 
 // G at R/B sites after the hvwt smoothing (demosaic.rs:957-971).
-__device__ void pass_green(Ctx& c, int tid) {
+__device__ void pass_green(Ctx& c, int tid, int nthreads) {
   const Tile& t = c.t;
   const float* cfa = c.cfa;
   const float* vcd = c.P(S_VCD);
@@ -547,7 +547,7 @@ __device__ void pass_green(Ctx& c, int tid) {
   float* dgrb0 = c.P(S_DGRB0);
   float* dgrb2h = c.P(S_DGRB2H);
   float* dgrb2v = c.P(S_DGRB2V);
-  for (int h = tid; h < 8192; h += NT) {
+  for (int h = tid; h < 8192; h += nthreads) {
     const int rr = h >> 6, j = h & 63;
     float d0 = 0.0f, d2h = 0.0f, d2v = 0.0f;
     if (rr >= 8 && rr < t.rr1 - 8) {
@@ -938,18 +938,21 @@ __global__ void __launch_bounds__(NT) amaze_kernel(AmazeArgs a) {
     if (do_nyq) pass_nyq_area(c, tid, bb);
     __syncthreads();
 
-    // hvwt smoothing (wave 0) overlaps the diagonal setup (waves 1..7)
+    // hvwt smoothing (wave 0) overlaps the diagonal setup (waves 1..15)
     if (wave == 0) pass_diag_smooth(c, lane, s_hvwt, 8, t.rr1 - 8, 8);
     else pass_diag_setup(c, tid - 64, NT - 64);
     __syncthreads();
-    pass_green(c, tid);
+    pass_diag(c, tid);  // reads neither hvwt nor anything pass_green writes
     __syncthreads();
-    pass_diag(c, tid);
-    __syncthreads();
-    // pmwt smoothing (wave 0) overlaps the Nyquist G refinement (waves 1..7)
+    // pmwt smoothing (wave 0) overlaps G at R/B sites (waves 1..15; it reads the
+    // smoothed hvwt and no pmwt)
     if (wave == 0) pass_diag_smooth(c, lane, s_pmwt, 10, t.rr1 - 10, 10);
-    else if (do_nyq) pass_green_nyq(c, tid - 64, bb, NT - 64);
+    else pass_green(c, tid - 64, NT - 64);
     __syncthreads();
+    if (do_nyq) {  // the Nyquist G refinement reads pass_green's planes at neighbouring sites
+      pass_green_nyq(c, tid, bb, NT);
+      __syncthreads();
+    }
     pass_rbint(c, tid);
     __syncthreads();
     pass_green_diag(c, tid);
