@@ -50,6 +50,7 @@ constexpr float kEpsRel = 2e-5f;
 constexpr float kEpsAbs = 1e-3f;
 constexpr float kSumRel = 5e-5f;
 constexpr float kInf = __builtin_inff();
+constexpr float kNaN = __builtin_nanf("");
 
 struct RefMeta {
   float d15;  // K-th smallest tracked upper bound
@@ -168,6 +169,10 @@ __device__ __forceinline__ float wave_max(float v) {
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
   return v;
 }
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
 // Sum over the 16 lanes of this lane's group.
 __device__ __forceinline__ int group_sum(int v) {
   for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
@@ -182,6 +187,9 @@ __device__ __forceinline__ half8 load8a(const _Float16* p) {
   return __builtin_bit_cast(half8, v);
 }
 
+// kPD: the staged rows' pitch in dwords when known at compile time (the row
+// offsets inside a B fragment then become immediate LDS offsets), 0 otherwise.
+template <int kPD>
 __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // c0[ch][y][x] = fp16(v - shift); c1[ch][y][x] = c0[ch][y][x + 1], so every
@@ -398,13 +406,17 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     uy0 = max(wry0 * st - half, 0);
     uy1 = min((wry0 + wny - 1) * st + half, hmax);
     // A operand: lane (q, n) holds reference n (clamped into the valid part of
-    // the sub-block), K-group q.
+    // the sub-block), K-group q: see load_b for the K order.
     const int aix = min(n & 3, wnx - 1), aiy = min(n >> 2, wny - 1);
     const int arx = (wrx0 + aix) * st - X0, ary = (wry0 + aiy) * st - Y0;
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-      const int t = 4 * s + q, ch = t >> 3, dy = t & 7;
-      A[s] = load8(c0 + ch * plane + (ary + dy) * P + arx);
+      const _Float16* r = c0 + (s >> 1) * plane + (ary + 4 * (s & 1)) * P + arx + 2 * q;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        A[s][2 * i] = r[i * P];
+        A[s][2 * i + 1] = r[i * P + 1];
+      }
     }
   }
   const int ntc = (ux1 - ux0) / 16 + 1;
@@ -422,65 +434,80 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
   };
 
   // ---- tile walk.  Lane n evaluates candidate column cx = ux0 + 16 tc + n in
-  // every row: tile columns tc outermost, rows innermost.  A column's window
-  // tests (xin) are fixed for the whole column; a row's tests reduce to its
-  // window row index rr = cy - wy0 against row counts, except in "slow" rows
-  // (a reference's own row, or the row where its cap falls mid-row), which
-  // take the scan-index tests.  Columns past the union are clamped to one of
-  // the same parity for the loads and fail every window test.
+  // every row: tile columns tc outermost, rows innermost.  Window membership is
+  // folded into the distance: na[j] is replaced by NaN in columns outside
+  // reference j's window, the candidate norm by NaN in rows outside the
+  // window of the lane's reference row, and the reference's own pair by NaN in
+  // its row; every comparison with a NaN is false, so such a pair is neither
+  // accepted, rejected, tracked nor collected.  The cap is a scan-index test
+  // per pair.  Columns past the union are clamped to one of the same parity
+  // for the loads and lie outside every window.
   const int nrows = yvalid ? wy1 - wy0 + 1 : 0;
   const int sr = ry - wy0;  // window row of the references themselves
-  int sx[4];                // window column of reference j itself
-#pragma unroll
-  for (int j = 0; j < 4; ++j) sx[j] = rxa[j] - wx0[j];
   int cxc = 0;              // this lane's candidate column in the current tile column
   int oxc[4];               // its window column, per reference
-  bool xin[4];
+  float naC[4];             // na[j] inside reference j's window columns, NaN outside
   auto column = [&](int cx) {
     cxc = cx;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       oxc[j] = cx - wx0[j];
-      xin[j] = (unsigned)oxc[j] < (unsigned)ewv[j];
+      naC[j] = (unsigned)oxc[j] < (unsigned)ewv[j] ? na[j] : kNaN;
     }
   };
-  // Rows of the window wholly before the cap position `lim` (scan index of
-  // the last capped candidate); the row holding it mid-row is slow.  Slow rows
-  // of the wave as a 128-bit map over the union rows (wave-uniform).
-  auto cap_rows = [&](const int* lim, int* full, uint64_t& slow_lo, uint64_t& slow_hi) {
-    uint32_t m[4] = {0u, 0u, 0u, 0u};
-    auto mark = [&](int rr) {
-      const int u = wy0 + rr - uy0;
-      m[u >> 5] |= 1u << (u & 31);
-    };
-    if (yvalid) mark(sr);
+  // window rows wholly before the cap position `lim` (scan index of the last
+  // capped candidate)
+  auto rows_before = [&](const int* lim, int* full) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      full[j] = 0;
-      if (vj[j]) {
-        const bool all = lim[j] >= nrows * ew[j];
-        full[j] = all ? nrows : (lim[j] + 1) / ew[j];
-        if (!all && (lim[j] + 1) % ew[j] != 0) mark(full[j]);
-      }
+    for (int j = 0; j < 4; ++j) full[j] = !vj[j] ? 0 : (lim[j] >= nrows * ew[j] ? nrows : (lim[j] + 1) / ew[j]);
+  };
+  // union rows holding a reference of the wave (wave-uniform 128-bit map)
+  uint64_t self_lo = 0, self_hi = 0;
+  {
+    uint32_t m[4] = {0u, 0u, 0u, 0u};
+    if (yvalid) {
+      const int u = ry - uy0;
+      m[u >> 5] |= 1u << (u & 31);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k)
       for (int off = 1; off < 64; off <<= 1) m[k] |= (uint32_t)__shfl_xor((int)m[k], off, 64);
-    slow_lo = ((uint64_t)__builtin_amdgcn_readfirstlane(m[1]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[0]);
-    slow_hi = ((uint64_t)__builtin_amdgcn_readfirstlane(m[3]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[2]);
-  };
-  auto is_slow = [&](int cy, uint64_t lo, uint64_t hi) {
+    self_lo = ((uint64_t)__builtin_amdgcn_readfirstlane(m[1]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[0]);
+    self_hi = ((uint64_t)__builtin_amdgcn_readfirstlane(m[3]) << 32) | (uint32_t)__builtin_amdgcn_readfirstlane(m[2]);
+  }
+  auto self_row = [&](int cy) {
     const int u = cy - uy0;
-    return (((u < 64 ? lo >> (u & 63) : hi >> (u & 63))) & 1ull) != 0;
+    return (((u < 64 ? self_lo >> (u & 63) : self_hi >> (u & 63))) & 1ull) != 0;
+  };
+  // distances of the lane's four pairs in one row; out-of-window pairs are NaN
+  auto dists = [&](int cy, const floatx4& acc, float nbv, float* d) {
+    const int rr = cy - wy0;
+    const float nbq = (unsigned)rr < (unsigned)nrows ? nbv : kNaN;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d[j] = fmaf(-2.0f, acc[j], naC[j] + nbq);
+    if (self_row(cy)) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (rr == sr && cxc == rxa[j]) d[j] = kNaN;
+    }
   };
   // MFMA cross terms of the 16 references against the 16 candidates of one row
   // of a tile column.  The row's B fragments start at `row` (this lane's
-  // column, copy c0 or c1 by parity).
+  // column, copy c0 or c1 by parity).  K order: chunk s covers rows
+  // 4 (s & 1) .. 4 (s & 1) + 3 of channel s >> 1, and K-group q of it the patch
+  // columns 2q and 2q + 1 of those four rows (element 2i + j: row i, column
+  // 2q + j).  The four K-groups then read neighbouring dwords of one row in
+  // each LDS pass, shared between groups, instead of four rows whose dwords
+  // fall on the same banks.
+  const int PD = kPD ? kPD : P / 2;
   auto load_b = [&](const _Float16* row, half8* B) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(__builtin_assume_aligned(row + 2 * q, 4));
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-      const int t = 4 * s + q, ch = t >> 3, dy = t & 7;
-      B[s] = load8a(row + ch * plane + dy * P);
+      const uint32_t* r = p + (s >> 1) * (plane / 2) + 4 * (s & 1) * PD;
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 v = {r[0], r[PD], r[2 * PD], r[3 * PD]};
+      B[s] = __builtin_bit_cast(half8, v);
     }
   };
   auto mfma_row = [&](const half8* B) -> floatx4 {
@@ -489,11 +516,12 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     for (int s = 0; s < 6; ++s) acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[s], B[s], acc, 0, 0, 0);
     return acc;
   };
-  // Visits every tile of the union: on_column(cx) once per tile column, then
-  // on_tile(cy, acc, nbv) per row.  The products of row r + 1 are issued
+  // Visits the tiles of union rows [rb, re): on_column(cx) once per tile column,
+  // then on_tile(cy, acc, nbv) per row.  The products of row r + 1 are issued
   // before row r is evaluated, and the loads of row r + 2 after them.
-  auto sweep = [&](auto&& on_column, auto&& on_tile) {
-    const int nr = uy1 - uy0 + 1;
+  auto sweep = [&](int rb, int re, auto&& on_column, auto&& on_tile) {
+    const int nr = re - rb;
+    if (nr <= 0) return;
     for (int tc = 0; tc < ntc; ++tc) {
       const int cx = ux0 + 16 * tc + n;
       int cxl = cx;
@@ -502,8 +530,8 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         if (cxl < X0) cxl += 2;  // one-column union: read past it (inside the pitch)
       }
       const int odd = (cxl - X0) & 1;
-      const _Float16* bp = (odd ? c1 : c0) + (uy0 - Y0) * P + (cxl - X0 - odd);
-      const float* np = nb + (uy0 - Y0) * PN + (min(cxl, X1) - X0);
+      const _Float16* bp = (odd ? c1 : c0) + (uy0 + rb - Y0) * P + (cxl - X0 - odd);
+      const float* np = nb + (uy0 + rb - Y0) * PN + (min(cxl, X1) - X0);
       on_column(cx);
       half8 B[6];
       load_b(bp, B);
@@ -518,7 +546,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
           acc_next = mfma_row(B);
           if (r + 2 < nr) load_b(bp + (r + 2) * P, B);
         }
-        on_tile(uy0 + r, acc, nbv);
+        on_tile(uy0 + rb + r, acc, nbv);
       }
     }
   };
@@ -545,61 +573,58 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     b1[j] = min(b1[j], dv);
   };
   int full1[4] = {0, 0, 0, 0};  // window rows wholly before the geometric cap
+  const int nru = uy1 - uy0 + 1;   // union rows
+  int rows1 = nru;                 // union rows pass 1 swept
+  // rejections of reference (q, j) counted so far: its row counters, lane n
+  // reads words n and n + 16
+  auto rej_total = [&](int j) {
+    const uint32_t* rc = rejrow + (wave * 16 + 4 * q + j) * kRowWords;
+    const uint32_t w0 = rc[n], w1 = n + 16 < kRowWords ? rc[n + 16] : 0u;
+    return group_sum((int)((w0 & 0xffffu) + (w0 >> 16) + (w1 & 0xffffu) + (w1 >> 16)));
+  };
   if (wave_active) {
-    uint64_t slow_lo, slow_hi;
-    cap_rows(limgeo, full1, slow_lo, slow_hi);
-    // kSlow: the row holds a reference's own position or a cap mid-row
-    auto tile = [&](auto slow_tag, int cy, const floatx4& acc, float nbv) {
-      constexpr bool kSlow = decltype(slow_tag)::value;
-      const unsigned rr = (unsigned)(cy - wy0);
-      const bool yin = rr < (unsigned)nrows;
-      bool amb = false;
+    rows_before(limgeo, full1);
+    auto on_tile = [&](int cy, const floatx4& acc, float nbv) {
+      const int rr = cy - wy0;
       float d[4];
-      bool win[4], pre[4], sure[4], rej[4];
+      dists(cy, acc, nbv, d);
+      bool rj[4], am[4], trk[4], ok[4];
+      bool anyr = false, anya = false;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        d[j] = dist(j, acc[j], nbv);
-        sure[j] = d[j] <= asure[j];
-        rej[j] = d[j] > arej[j];
-        if (kSlow) {
-          win[j] = xin[j] & yin & !(((int)rr == sr) & (oxc[j] == sx[j]));
-          pre[j] = win[j] & ((int)rr * ew[j] + oxc[j] <= limgeo[j]);
-        } else {
-          win[j] = xin[j] & yin;
-          pre[j] = xin[j] & (rr < (unsigned)full1[j]);
-        }
-        track(j, pre[j] & sure[j], d[j]);
-        amb |= win[j] & !sure[j] & !rej[j];
+        const bool sure = d[j] <= asure[j];
+        rj[j] = d[j] > arej[j];
+        am[j] = (d[j] <= arej[j]) & !sure;
+        trk[j] = rr * ew[j] + oxc[j] <= limgeo[j];  // inside the geometric cap
+        ok[j] = sure & trk[j];
+        anyr |= rj[j];
+        anya |= am[j];
       }
       // surely rejected pairs: lane n < 4 of each quadrant adds the quadrant's
       // count for reference (q, n) to its row counter (two 16-bit row counters per word)
-      bool r[4];
+      if (__ballot(anyr)) {
+        uint32_t mine = 0u;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) r[j] = win[j] & rej[j];
-      if (__ballot(r[0] | r[1] | r[2] | r[3])) {
-        uint64_t mr[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) mr[j] = __ballot(r[j]);
-        const uint64_t msel = n == 0 ? mr[0] : (n == 1 ? mr[1] : (n == 2 ? mr[2] : mr[3]));
-        const uint32_t part = (uint32_t)(msel >> (16 * q)) & 0xffffu;
-        if (n < 4 && part)
-          atomicAdd(&rejrow[(wave * 16 + 4 * q + n) * kRowWords + (rr >> 1)], (uint32_t)__popc(part) << (16 * (rr & 1)));
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t c = (uint32_t)__popc((uint32_t)(__ballot(rj[j]) >> (16 * q)) & 0xffffu);
+          mine = n == j ? c : mine;
+        }
+        if (n < 4 && mine)
+          atomicAdd(&rejrow[(wave * 16 + 4 * q + n) * kRowWords + (rr >> 1)], mine << (16 * (rr & 1)));
       }
-      if (amb) {  // the threshold is within reach of some pair
+      if (__ballot(anya)) {  // the threshold is within reach of some pair
         const float rb = kU * hsqrt(nbv) * (1.0f + 1e-6f) + 1e-5f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          if (win[j] && !sure[j] && !rej[j]) {
-            const bool ok = classify(j, d[j], nbv, rb, cxc, cy);
-            track(j, ok && pre[j], d[j]);
-            if (!ok) atomicAdd(&rejrow[(wave * 16 + 4 * q + j) * kRowWords + (rr >> 1)], (rr & 1) ? 65536u : 1u);
+          if (am[j]) {
+            const bool acc_j = classify(j, d[j], nbv, rb, cxc, cy);
+            ok[j] = acc_j && trk[j];
+            if (!acc_j) atomicAdd(&rejrow[(wave * 16 + 4 * q + j) * kRowWords + (rr >> 1)], (rr & 1) ? 65536u : 1u);
           }
         }
       }
-    };
-    sweep(column, [&](int cy, const floatx4& acc, float nbv) {
-      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, cy, acc, nbv);
-      else tile(std::false_type{}, cy, acc, nbv);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) track(j, ok[j], d[j]);
       if ((cy - uy0) % 4 == 3) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -607,9 +632,36 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
           b2[j] = __builtin_amdgcn_update_dpp(0, b2[j], 0x127, 0xf, 0xf, false);
         }
       }
-    });
+    };
+    // Rows past every reference's cap cannot change a decision, so the sweep
+    // covers the union rows through the last geometric-cap row first (the
+    // tracked candidates all lie there) and then extends only while some
+    // reference has fewer than `cap` acceptances in the rows swept so far.
+    int rlast = -1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (vj[j]) rlast = max(rlast, wy0 - uy0 + (limgeo[j] == 0x7fffffff ? nrows - 1 : limgeo[j] / ew[j]));
+    int R = wave_max_i(rlast) + 1;
+    sweep(0, R, column, on_tile);
+    while (R < nru) {
+      int need = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rs = min(max(R - (wy0 - uy0), 0), nrows);
+        const int acc_so_far = rs * ew[j] - (sr < rs ? 1 : 0) - rej_total(j);
+        if (vj[j] && rs < nrows && acc_so_far < cap) need = max(need, (cap - acc_so_far + ew[j] - 1) / ew[j]);
+      }
+      need = wave_max_i(need);
+      if (need == 0) break;
+      const int R2 = min(nru, R + need);
+      sweep(R, R2, column, on_tile);
+      R = R2;
+    }
+    rows1 = R;
   }
-  // per reference: accepted count and group size
+  // per reference: accepted count and group size.  A reference whose window
+  // was not swept to its end has at least `cap` acceptances in the swept rows:
+  // any count above the cap takes the same decisions below.
   int nacc[4];
   bool anyrej[4];
 #pragma unroll
@@ -617,8 +669,8 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     // rejections of reference (q, j): its row counters, lane n reads words n and n + 16
     const uint32_t* rc = rejrow + (wave * 16 + 4 * q + j) * kRowWords;
     const uint32_t w0 = rc[n], w1 = n + 16 < kRowWords ? rc[n + 16] : 0u;
-    const int rj = (int)((w0 & 0xffffu) + (w0 >> 16) + (w1 & 0xffffu) + (w1 >> 16));
-    nacc[j] = npos[j] - group_sum(rj);
+    const int complete = rows1 - (wy0 - uy0) >= nrows;
+    nacc[j] = complete ? npos[j] - rej_total(j) : cap + 1;
     // rejections up to the row holding the geometric cap (all of that row: a
     // reference flagged without need gets the geometric cap from the walk)
     const int cr = min(full1[j], (yvalid ? wy1 - wy0 + 1 : 0) - 1);
@@ -785,33 +837,22 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
         cnt += __popcll(mq);
       }
     };
-    // capped candidates: the window rows wholly before the cap position, and
-    // slow rows (own row, cap mid-row) by scan index
-    int full[4];
-    uint64_t slow_lo, slow_hi;
-    cap_rows(limit, full, slow_lo, slow_hi);
     // The approximate top-K criterion (d <= dK) selects a subset of the exact
     // band (dK <= dA), and its list is only used where the band's list
     // overflowed: the sweep collects the band (kB false), and a second sweep
     // collects the top-K list of such references (kB true, dsel = dK there).
+    // Capped candidates: scan index <= limit.
     float dsel[4];
-    auto tile = [&](auto slow_tag, auto b_tag, int cy, const floatx4& acc, float nbv) {
-      constexpr bool kSlow = decltype(slow_tag)::value, kB = decltype(b_tag)::value;
-      const unsigned rr = (unsigned)(cy - wy0);
+    auto tile = [&](auto b_tag, int cy, const floatx4& acc, float nbv) {
+      constexpr bool kB = decltype(b_tag)::value;
+      const int rr = cy - wy0;
       float d[4];
+      dists(cy, acc, nbv, d);
       bool cand[4];
       bool any = false;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        d[j] = dist(j, acc[j], nbv);
-        bool pre;
-        if (kSlow) {
-          const bool win = xin[j] & (rr < (unsigned)nrows) & !(((int)rr == sr) & (oxc[j] == sx[j]));
-          pre = win & ((int)rr * ew[j] + oxc[j] <= limit[j]);
-        } else {
-          pre = xin[j] & (rr < (unsigned)full[j]);
-        }
-        cand[j] = pre & (d[j] <= dsel[j]);
+        cand[j] = (d[j] <= dsel[j]) & (rr * ew[j] + oxc[j] <= limit[j]);
         any |= cand[j];
       }
       // wave-uniform: every lane of a quadrant takes part in the appends, so the
@@ -831,10 +872,13 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     };
 #pragma unroll
     for (int j = 0; j < 4; ++j) dsel[j] = dA[j];
-    sweep(column, [&](int cy, const floatx4& acc, float nbv) {
-      if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, std::false_type{}, cy, acc, nbv);
-      else tile(std::false_type{}, std::false_type{}, cy, acc, nbv);
-    });
+    // union rows through the last row holding a capped candidate
+    int rlast = -1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (vj[j]) rlast = max(rlast, wy0 - uy0 + (limit[j] >= nrows * ew[j] ? nrows - 1 : limit[j] / ew[j]));
+    const int R2 = wave_max_i(rlast) + 1;
+    sweep(0, R2, column, [&](int cy, const floatx4& acc, float nbv) { tile(std::false_type{}, cy, acc, nbv); });
     bool needb = false;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -842,10 +886,7 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       needb |= cntA[j] > kListCap;
     }
     if (__ballot(needb)) {
-      sweep(column, [&](int cy, const floatx4& acc, float nbv) {
-        if (is_slow(cy, slow_lo, slow_hi)) tile(std::true_type{}, std::true_type{}, cy, acc, nbv);
-        else tile(std::false_type{}, std::true_type{}, cy, acc, nbv);
-      });
+      sweep(0, R2, column, [&](int cy, const floatx4& acc, float nbv) { tile(std::true_type{}, cy, acc, nbv); });
     }
     if (n == 0) {
 #pragma unroll
@@ -1288,7 +1329,10 @@ hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblo
 
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.overflow_count, (size_t)2, 0u);
-  hipLaunchKernelGGL(bm3d_match_mfma_kernel, dim3(a.n_blocks), dim3(XT), (size_t)a.smem, s, a);
+  if (a.pitch == 68)  // 39x39 windows at stride 3
+    hipLaunchKernelGGL(bm3d_match_mfma_kernel<34>, dim3(a.n_blocks), dim3(XT), (size_t)a.smem, s, a);
+  else
+    hipLaunchKernelGGL(bm3d_match_mfma_kernel<0>, dim3(a.n_blocks), dim3(XT), (size_t)a.smem, s, a);
   return hipGetLastError();
 }
 
