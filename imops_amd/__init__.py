@@ -281,6 +281,7 @@ class Image:
     def bm3d(self, intensity: float = 0.0, **explicit) -> "Image":
         """BM3D (bm3d.rs:34-61).  Keyword overrides: sigma (0..255 units), lambda_, max_dist,
         chroma_scale, detail, stride, search_window, max_group, max_candidates, and
+        group_transform ("hadamard", the reference's; "haar"), and
         precision ("fp32"; "mixed-match": FP16 matrix-core matching, same groups, FP32 filter;
         "mixed": that matcher plus the filter's transforms on FP16 matrix cores)."""
         if not explicit:
@@ -394,6 +395,8 @@ def bm3d_params(intensity: float = 0.0, **explicit) -> _lib.Bm3dParams:
         if k == "precision":
             v = {"fp32": _lib.BM3D_FP32, "mixed": _lib.BM3D_MIXED,
                      "mixed-match": _lib.BM3D_MIXED_MATCH}[v] if isinstance(v, str) else int(v)
+        elif k == "group_transform":
+            v = {"hadamard": _lib.BM3D_HADAMARD, "haar": _lib.BM3D_HAAR}[v] if isinstance(v, str) else int(v)
         setattr(p, "lambda_" if k == "lambda" else k, v)
     return p
 

@@ -32,6 +32,8 @@ DEMOSAIC_MARKESTEIJN3_1PASS = 6
 BM3D_FP32 = 0
 BM3D_MIXED = 1
 BM3D_MIXED_MATCH = 2
+BM3D_HADAMARD = 0
+BM3D_HAAR = 1
 
 _vp = C.c_void_p
 _sz = C.c_size_t
@@ -43,7 +45,7 @@ class Bm3dParams(C.Structure):
     """imops_bm3d_params (include/imops_gpu.h)."""
     _fields_ = [("sigma", C.c_float), ("lambda_", C.c_float), ("max_dist", C.c_float), ("chroma_scale", C.c_float),
                 ("detail", C.c_float), ("stride", C.c_int), ("search_window", C.c_int), ("max_group", C.c_int),
-                ("max_candidates", C.c_int), ("precision", C.c_int)]
+                ("max_candidates", C.c_int), ("precision", C.c_int), ("group_transform", C.c_int)]
 
 
 class FilmFitModel(C.Structure):

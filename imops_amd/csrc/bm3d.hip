@@ -602,6 +602,51 @@ __device__ __forceinline__ void wht_lanes(float* v, int k, int g, float scale) {
   for (int i = 0; i < 64; ++i) v[i] *= s;
 }
 
+// Haar group transform on the lanes (the in-place lifting layout: lane k of
+// the group ends as the coefficient whose lowest set bit is h = k & -k, the
+// block sum over [k - h, k) minus the one over [k, k + h), lane 0 as the sum of
+// all g; scales 1/sqrt(2h) and 1/sqrt(g)).  Each stage is the Walsh-Hadamard
+// stage's DPP fused multiply-add with a multiplier of 0 on the lanes that do
+// not take part (a lane that is not a multiple of h keeps its value).  The
+// upper lane of a pair computes y - x, the exact negation of x - y.  Forward:
+// every lane that takes part at stage h has only been a lower lane before, so
+// the negation shows only in the lanes' final values and the scale carries a
+// sign of -1 for k > 0.  Inverse (the transpose: scales first, then stages
+// from g/2 down to 1): before stage h the lower lane k of a pair holds its
+// value negated unless k = 0 (it was the upper lane at the stage of its
+// lowest bit) and the upper lane is untouched, so the multipliers are chosen
+// to keep each pair's relation, and every lane but 0 ends negated; the caller
+// folds that sign into the aggregation weight.  Groups of two (where both
+// transforms are the same matrix) take the Walsh-Hadamard path in the inverse,
+// whose order (butterfly, then scale) they then share bit for bit.
+__device__ __forceinline__ float haar_scale_lane(int k, int g) {
+  const int span = k == 0 ? g : 2 * (k & -k);
+  return span > 0 ? 1.0f / sqrtf((float)span) : 0.0f;  // g = 0: no group, lanes hold zeros
+}
+__device__ __forceinline__ float haar_mult(int k, int h, bool inverse) {
+  if (k & (h - 1)) return 0.0f;
+  if ((k & h) == 0) return (inverse && k != 0) ? -1.0f : 1.0f;  // lower lane
+  return (inverse && k != h) ? 1.0f : -1.0f;                    // upper lane
+}
+__device__ __forceinline__ void haar_lanes_fwd(float* v, int k, int g) {
+  if (g > 1) wht_stage<1>(v, haar_mult(k, 1, false));
+  if (g > 2) wht_stage<2>(v, haar_mult(k, 2, false));
+  if (g > 4) wht_stage<4>(v, haar_mult(k, 4, false));
+  if (g > 8) wht_stage<8>(v, haar_mult(k, 8, false));
+  const float s = k == 0 ? haar_scale_lane(0, g) : -haar_scale_lane(k, g);
+#pragma unroll
+  for (int i = 0; i < 64; ++i) v[i] *= s;
+}
+__device__ __forceinline__ void haar_lanes_inv(float* v, int k, int g) {
+  const float s = haar_scale_lane(k, g);
+#pragma unroll
+  for (int i = 0; i < 64; ++i) v[i] *= s;
+  if (g > 8) wht_stage<8>(v, haar_mult(k, 8, true));
+  if (g > 4) wht_stage<4>(v, haar_mult(k, 4, true));
+  if (g > 2) wht_stage<2>(v, haar_mult(k, 2, true));
+  if (g > 1) wht_stage<1>(v, haar_mult(k, 1, true));
+}
+
 // LDS: the footprint's i64 numerators, then its u32 denominators.  A
 // workgroup's denominator sum at one pixel is at most 64 references x 16
 // patches x 1e5 < 2^32, so the 32-bit partial sums are exact.
@@ -666,7 +711,8 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
       load_rows(grows, yr);
       const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
       dct2_cols(yr, gv, bm3d_table(0));
-      wht_lanes(gv, k, g, wscale);
+      if (a.haar) haar_lanes_fwd(gv, k, g);
+      else wht_lanes(gv, k, g, wscale);
       float weight;
       if (kStep1) {
         // hard_threshold (bm3d.rs:240-255): stack element 0 (patch 0, coefficient 0) is kept
@@ -708,14 +754,21 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
         weight = run > 0.0f ? 1.0f / run : 1.0f;
         load_rows(nrows, yr);
         dct2_cols(yr, nv, bm3d_table(0));
-        wht_lanes(nv, k, g, wscale);
+        if (a.haar) haar_lanes_fwd(nv, k, g);
+        else wht_lanes(nv, k, g, wscale);
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] *= gv[i];  // element 0 of patch 0: coefficient 1 (exact)
       }
       // inverse (bm3d.rs:508-524) and aggregation into the footprint
-      wht_lanes(nv, k, g, wscale);
+      // the inverse Haar leaves lanes k > 0 negated (see haar_lanes_inv); the
+      // inverse DCT is odd, so the numerator's weight carries the sign back
+      // (an exact product of two negated operands)
+      const bool haar_inv = a.haar && g > 2;
+      if (haar_inv) haar_lanes_inv(nv, k, g);
+      else wht_lanes(nv, k, g, wscale);
       idct2_regs(nv, bm3d_table(1));
       const ctab_ptr kaiser = bm3d_table(2);
+      const float nsign = (haar_inv && k != 0) ? -1.0f : 1.0f;
       if (act) {
 #pragma unroll
         for (int dy = 0; dy < 8; ++dy)
@@ -723,7 +776,7 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
           for (int dx = 0; dx < 8; ++dx) {
             const float kw = kaiser[dy * 8 + dx] * weight;
             const int fi = (py + dy - fy0) * fw + (px + dx - fx0);
-            atomicAdd(&acc[fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * kw));
+            atomicAdd(&acc[fi], (unsigned long long)to_fixed(nv[dy * 8 + dx] * (kw * nsign)));
             atomicAdd(&den[fi], (uint32_t)to_fixed_unit(kw));
           }
       }

@@ -32,6 +32,7 @@ struct Bm3dFilterArgs {
   int width, height;
   int refs_x, refs_y, stride, window, tile;
   float sigma, chroma_scale, lambda;
+  int haar;  // group transform: 0 Walsh-Hadamard (bm3d.rs:595-614), 1 Haar
   const uint32_t* groups;
   unsigned long long* num;  // 3 planes of i64 (two's complement in u64)
   unsigned long long* den;

@@ -298,6 +298,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       fa.sigma = p.sigma;
       fa.chroma_scale = p.chroma_scale;
       fa.lambda = p.lambda;
+      fa.haar = p.group_transform == IMOPS_BM3D_HAAR;
       fa.groups = groups;
       fa.num = num;
       fa.den = den;
@@ -358,6 +359,7 @@ int imops_bm3d_default_params(float intensity, imops_bm3d_params* out) {
     out->max_group = p.max_group;
     out->max_candidates = p.max_candidates;
     out->precision = p.precision;
+    out->group_transform = p.group_transform;
   });
 }
 
@@ -389,6 +391,9 @@ int imops_bm3d_ex(imops_ctx* ctx, imops_image* img, const imops_bm3d_params* q) 
     p.max_group = q->max_group;
     p.max_candidates = q->max_candidates;
     p.precision = q->precision;
+    p.group_transform = q->group_transform;
+    if (p.group_transform != IMOPS_BM3D_HADAMARD && p.group_transform != IMOPS_BM3D_HAAR)
+      throw Error{IMOPS_E_INVALID, "bm3d_ex: group_transform must be IMOPS_BM3D_HADAMARD or IMOPS_BM3D_HAAR"};
     if (p.precision != IMOPS_BM3D_FP32 && p.precision != IMOPS_BM3D_MIXED && p.precision != IMOPS_BM3D_MIXED_MATCH)
       throw Error{IMOPS_E_INVALID, "bm3d_ex: precision must be IMOPS_BM3D_FP32, _MIXED or _MIXED_MATCH"};
     if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.window > kBm3dMaxWindow || p.max_group < 1 ||

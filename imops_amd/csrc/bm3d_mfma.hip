@@ -1115,14 +1115,24 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
         const uint32_t pp = cact[j] ? grp[1 + 4 * q + j] : 0u;
         cbase[j] = ((int)(pp >> 16) - fy0) * fw + ((int)(pp & 0xffffu) - fx0);
       }
-      // Hadamard fragment (A of the 16x16x16 product): lane (row n, q) holds H[n][4q .. 4q+3]
-      const float hs = 1.0f / sqrtf((float)g);
-      half4 Hf;
+      // group-transform fragments (A of the 16x16x16 products): lane (row n, q)
+      // holds T[n][4q .. 4q+3] (forward) and T[4q .. 4q+3][n] (inverse, the
+      // transpose).  Walsh-Hadamard: the Sylvester matrix / sqrt(g), symmetric.
+      // Haar (in-place lifting order): row 0 = 1/sqrt(g) everywhere; row k > 0
+      // with lowest set bit h = +1/sqrt(2h) on [k - h, k), -1/sqrt(2h) on [k, k + h).
+      auto tcoef = [&](int r, int c) -> float {
+        if (r >= g || c >= g) return 0.0f;
+        if (!a.haar) return ((__popc(r & c) & 1) ? -1.0f : 1.0f) / sqrtf((float)g);
+        if (r == 0) return 1.0f / sqrtf((float)g);
+        const int h = r & -r;
+        const float v = 1.0f / sqrtf((float)(2 * h));
+        return (c >= r - h && c < r) ? v : ((c >= r && c < r + h) ? -v : 0.0f);
+      };
+      half4 Hf, Hi;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int kk = 4 * q + e;
-        const float v = (n < g && kk < g) ? ((__popc(n & kk) & 1) ? -hs : hs) : 0.0f;
-        Hf[e] = (_Float16)v;
+        Hf[e] = (_Float16)tcoef(n, 4 * q + e);
+        Hi[e] = (_Float16)tcoef(4 * q + e, n);
       }
       // forward: 2-D DCT (rows = patches) then Hadamard across the group
       const uint32_t p0 = grp[1];
@@ -1191,7 +1201,7 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-        const floatx4 y = __builtin_amdgcn_mfma_f32_16x16x16f16(Hf, to_h4(Z[t]), zero, 0, 0, 0);
+        const floatx4 y = __builtin_amdgcn_mfma_f32_16x16x16f16(Hi, to_h4(Z[t]), zero, 0, 0, 0);
         const int blk = ((2 * t + (n >> 3)) ^ (2 * q)) * 8 + (n & 7);
 #pragma unroll
         for (int j = 0; j < 4; ++j) tbuf[(4 * q + j) * 64 + blk] = (_Float16)y[j];

@@ -384,6 +384,11 @@ Stage make_stage(const Table& t) {
     else if (prec == "mixed") s.bm3d.precision = IMOPS_BM3D_MIXED;
     else if (prec == "mixed-match") s.bm3d.precision = IMOPS_BM3D_MIXED_MATCH;
     else throw Error{IMOPS_E_INVALID, "Invalid BM3D config"};
+    // extension key: the group transform across a group's patches
+    const std::string gt = str_param(t, "group_transform", "hadamard", "BM3D");
+    if (gt == "hadamard") s.bm3d.group_transform = IMOPS_BM3D_HADAMARD;
+    else if (gt == "haar") s.bm3d.group_transform = IMOPS_BM3D_HAAR;
+    else throw Error{IMOPS_E_INVALID, "Invalid BM3D config"};
   } else if (s.name == "ChromaDenoise") {  // modules.rs:688-722
     s.kind = K_CHROMA_BM3D;
     s.intensity = num_param(t, "intensity", 0.0, "ChromaDenoise");

@@ -23,6 +23,7 @@ struct Bm3dStageParams {
   int max_group = 16;           // group size cap (power of two)
   int max_candidates = 1024;    // accepted-candidate cap per reference patch
   int precision = IMOPS_BM3D_FP32;  // IMOPS_BM3D_FP32, _MIXED or _MIXED_MATCH
+  int group_transform = IMOPS_BM3D_HADAMARD;  // IMOPS_BM3D_HADAMARD (bm3d.rs:595-614) or IMOPS_BM3D_HAAR
   static Bm3dStageParams from_intensity(float i);
 };
 

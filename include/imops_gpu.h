@@ -43,6 +43,13 @@ extern "C" {
 #define IMOPS_CS_ACESCG 2
 #define IMOPS_CS_XYZD65 3
 
+/* BM3D group transform across the patches of a group
+ * (imops_bm3d_params.group_transform): the reference's Walsh-Hadamard
+ * (walsh_hadamard_1d, bm3d.rs:595-614) or the orthonormal multi-level Haar
+ * transform (an extension; identical for groups of two). */
+#define IMOPS_BM3D_HADAMARD 0
+#define IMOPS_BM3D_HAAR 1
+
 /* BM3D arithmetic (imops_bm3d_params.precision). */
 #define IMOPS_BM3D_FP32 0
 #define IMOPS_BM3D_MIXED 1
@@ -254,6 +261,7 @@ typedef struct imops_bm3d_params {
                            FP32 re-ranking (same groups as FP32), FP32 filter;
                            IMOPS_BM3D_MIXED (1): that matcher plus the collaborative filter's
                            DCT / Hadamard transforms on FP16 matrix cores (FP32 accumulation) */
+  int group_transform;  /* IMOPS_BM3D_HADAMARD (0, the reference's) or IMOPS_BM3D_HAAR (1) */
 } imops_bm3d_params;
 
 /* Fill `out` with the parameters intensity maps to (from_intensity). */

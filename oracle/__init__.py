@@ -47,6 +47,7 @@ _SIGS = {
     "oracle_cfa_coeffs": (C.c_int, [_fp, _sz, _fp]),
     "oracle_srgb_oetf": (C.c_float, [C.c_float]),
     "oracle_bm3d": (C.c_int, [_fp, _sz, _sz, _fp, C.c_int, C.POINTER(C.c_longlong)]),
+    "oracle_group_transform_1d": (C.c_int, [C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int]),
     "oracle_bm3d_intensity_params": (C.c_int, [C.c_float, _fp]),
     "oracle_chroma_bm3d": (C.c_int, [_fp, _sz, _sz, C.c_float]),
     "oracle_bm3d_tables": (C.c_int, [_fp, _fp, _fp]),
@@ -202,22 +203,34 @@ def srgb_oetf(x: float) -> float:
 
 
 BM3D_FIELDS = ("intensity", "sigma", "lambda", "max_dist", "chroma_scale", "detail", "stride", "window",
-               "max_group", "max_candidates")
+               "max_group", "max_candidates", "group_transform")
+GROUP_TRANSFORMS = {"hadamard": 0, "haar": 1}
 
 
 def bm3d_params(intensity: float, **overrides) -> dict:
-    """Bm3dParams::from_intensity (bm3d.rs:22-32) plus detail strength, as a dict."""
+    """Bm3dParams::from_intensity (bm3d.rs:22-32) plus detail strength, as a dict;
+    group_transform "hadamard" (the reference's) or "haar" (extension)."""
     v = np.empty(10, np.float32)
     lib().oracle_bm3d_intensity_params(intensity, _f(v))
-    p = dict(zip(BM3D_FIELDS, [float(x) for x in v]))
+    p = dict(zip(BM3D_FIELDS, [float(x) for x in v] + [0.0]))
     p.update(overrides)
     return p
+
+
+def group_transform_1d(x, kind: str = "hadamard", inverse: bool = False, fp64: bool = True) -> np.ndarray:
+    """One column of the group transform (bm3d.rs:595-614 for "hadamard"; the Haar option)."""
+    d = np.array(x, np.float64, copy=True)
+    _rc(lib().oracle_group_transform_1d(d.ctypes.data_as(C.POINTER(C.c_double)), d.size, GROUP_TRANSFORMS[kind],
+                                        int(inverse), int(fp64)), "group_transform_1d")
+    return d
 
 
 def bm3d(rgb: np.ndarray, params: dict, fp64: bool = False):
     """BM3D of an (H, W, 3) f32 image in [0,1]; returns (output, (n_refs, n_grouped_patches))."""
     out = np.array(rgb, np.float32, copy=True, order="C")
     h, w, _ = out.shape
+    gt = params.get("group_transform", 0)
+    params = dict(params, group_transform=GROUP_TRANSFORMS[gt] if isinstance(gt, str) else gt)
     v = np.array([params[k] for k in BM3D_FIELDS], np.float32)
     st = (C.c_longlong * 2)()
     _rc(lib().oracle_bm3d(_f(out), w, h, _f(v), int(fp64), st), "bm3d")

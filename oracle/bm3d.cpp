@@ -130,23 +130,74 @@ void wht(T* d, int n) {
   for (int i = 0; i < n; ++i) d[i] *= scale;
 }
 
+// Haar group transform (an option beside the reference's Walsh-Hadamard,
+// which bm3d.rs:595-614 fixes; no reference counterpart).  Orthonormal
+// multi-level Haar in the in-place lifting layout: stage h = 1, 2, 4, .. < n
+// takes the butterfly (x, y) -> (x + y, x - y) of positions k and k + h for
+// every k that is a multiple of 2h, the same butterfly and operand order as
+// the Walsh-Hadamard stages but on those pairs only.  Position 0 ends as the
+// sum of all n values and position k > 0, whose lowest set bit is h, as the
+// sum over [k - h, k) minus the sum over [k, k + h); the scale 1/sqrt(n) at
+// position 0 and 1/sqrt(2h) elsewhere makes the rows orthonormal.  The
+// inverse is the transpose: the same scales first, then the stages from
+// h = n/2 down to 1.  For n = 2 the two transforms are the same matrix; groups
+// of two are evaluated in the Walsh-Hadamard order (butterfly, then scale) in
+// both directions, so they give the Walsh-Hadamard bits.
 template <class T>
-void transform3d(T* s, int g, const Tables<T>& tb) {
+T haar_scale(int k, int n) {
+  const int span = k == 0 ? n : 2 * (k & -k);
+  return (T)1 / std::sqrt((T)span);
+}
+
+template <class T>
+void haar_stage(T* d, int n, int h) {
+  for (int k = 0; k + h < n; k += 2 * h) {
+    T x = d[k], y = d[k + h];
+    d[k] = x + y;
+    d[k + h] = x - y;
+  }
+}
+
+template <class T>
+void haar_fwd(T* d, int n) {
+  for (int h = 1; h < n; h *= 2) haar_stage(d, n, h);
+  for (int k = 0; k < n; ++k) d[k] *= haar_scale<T>(k, n);
+}
+
+template <class T>
+void haar_inv(T* d, int n) {
+  for (int k = 0; k < n; ++k) d[k] *= haar_scale<T>(k, n);
+  for (int h = n / 2; h >= 1; h /= 2) haar_stage(d, n, h);
+}
+
+template <class T>
+void group_fwd(T* col, int g, int kind) {
+  if (kind == 1) haar_fwd(col, g);
+  else wht(col, g);
+}
+template <class T>
+void group_inv(T* col, int g, int kind) {
+  if (kind == 1 && g > 2) haar_inv(col, g);
+  else wht(col, g);  // the Walsh-Hadamard transform with 1/sqrt(n) is its own inverse
+}
+
+template <class T>
+void transform3d(T* s, int g, const Tables<T>& tb, int kind) {
   for (int i = 0; i < g; ++i) dct2(s + i * 64, tb.dct);
   for (int i = 0; i < 64; ++i) {
     T col[32];
     for (int k = 0; k < g; ++k) col[k] = s[k * 64 + i];
-    wht(col, g);
+    group_fwd(col, g, kind);
     for (int k = 0; k < g; ++k) s[k * 64 + i] = col[k];
   }
 }
 
 template <class T>
-void inverse3d(T* s, int g, const Tables<T>& tb) {
+void inverse3d(T* s, int g, const Tables<T>& tb, int kind) {
   for (int i = 0; i < 64; ++i) {
     T col[32];
     for (int k = 0; k < g; ++k) col[k] = s[k * 64 + i];
-    wht(col, g);
+    group_inv(col, g, kind);
     for (int k = 0; k < g; ++k) s[k * 64 + i] = col[k];
   }
   for (int i = 0; i < g; ++i) idct2(s + i * 64, tb.idct);
@@ -265,8 +316,8 @@ std::vector<std::vector<T>> run_step(const std::vector<std::vector<T>>& noisy, c
       for (int k = 0; k < g; ++k) extract(guide[ch], w, gx[k], gy[k], &gs[k * 64]);
       if (step1) ns = gs;
       else for (int k = 0; k < g; ++k) extract(noisy[ch], w, gx[k], gy[k], &ns[k * 64]);
-      transform3d(gs.data(), g, tb);
-      if (!step1) transform3d(ns.data(), g, tb);
+      transform3d(gs.data(), g, tb, p.group_transform);
+      if (!step1) transform3d(ns.data(), g, tb, p.group_transform);
       T weight;
       if (step1) {
         const T th = (T)p.lambda * chs;
@@ -276,7 +327,7 @@ std::vector<std::vector<T>> run_step(const std::vector<std::vector<T>>& noisy, c
       } else {
         weight = wiener(ns.data(), gs.data(), g * 64, chs);
       }
-      inverse3d(ns.data(), g, tb);
+      inverse3d(ns.data(), g, tb, p.group_transform);
       std::atomic<int64_t>* nacc = &num[ch * count];
       std::atomic<int64_t>* dacc = &den[ch * count];
       for (int k = 0; k < g; ++k)
@@ -445,4 +496,21 @@ void bm3d_tables(float* dct, float* idct, float* kaiser) {
   for (int i = 0; i < 64; ++i) { dct[i] = t.dct[i]; idct[i] = t.idct[i]; kaiser[i] = t.kaiser[i]; }
 }
 
+}  // namespace oracle
+
+namespace oracle {
+// The 1-D group transforms on one column of n values (kind 0: Walsh-Hadamard,
+// 1: Haar), forward or inverse, in f32 or f64.
+void group_transform_1d(double* d, int n, int kind, bool inverse, bool fp64) {
+  if (fp64) {
+    if (inverse) group_inv(d, n, kind);
+    else group_fwd(d, n, kind);
+  } else {
+    float f[32];
+    for (int i = 0; i < n; ++i) f[i] = (float)d[i];
+    if (inverse) group_inv(f, n, kind);
+    else group_fwd(f, n, kind);
+    for (int i = 0; i < n; ++i) d[i] = f[i];
+  }
+}
 }  // namespace oracle

@@ -108,8 +108,9 @@ int oracle_gamma(float* rgb, size_t n, float gamma) { gamma_encode(rgb, n, gamma
 int oracle_cfa_coeffs(float* rgb, size_t n, const float* wb) { cfa_coeffs(rgb, n, wb); return 0; }
 float oracle_srgb_oetf(float x) { return srgb_oetf(x); }
 
-// BM3D with explicit parameters; params[10] = {intensity, sigma, lambda,
-// max_dist, chroma_scale, detail, stride, window, max_group, max_candidates}.
+// BM3D with explicit parameters; params[11] = {intensity, sigma, lambda,
+// max_dist, chroma_scale, detail, stride, window, max_group, max_candidates,
+// group_transform (0 Walsh-Hadamard, 1 Haar)}.
 int oracle_bm3d(float* rgb, size_t w, size_t h, const float* params, int fp64, long long* stats2) {
   Bm3dParams p;
   p.intensity = params[0];
@@ -122,6 +123,8 @@ int oracle_bm3d(float* rgb, size_t w, size_t h, const float* params, int fp64, l
   p.window = (int)params[7];
   p.max_group = (int)params[8];
   p.max_candidates = (int)params[9];
+  p.group_transform = (int)params[10];
+  if (p.group_transform != 0 && p.group_transform != 1) return -1;
   if (p.stride < 1 || p.window < 1 || (p.window & 1) == 0 || p.max_group < 1 || p.max_group > 32 ||
       p.max_candidates < 1)
     return -1;
@@ -129,6 +132,12 @@ int oracle_bm3d(float* rgb, size_t w, size_t h, const float* params, int fp64, l
   if (fp64) bm3d<double>(rgb, w, h, p, &st);
   else bm3d<float>(rgb, w, h, p, &st);
   if (stats2) { stats2[0] = st.refs; stats2[1] = st.groups; }
+  return 0;
+}
+
+int oracle_group_transform_1d(double* d, int n, int kind, int inverse, int fp64) {
+  if (n < 1 || n > 32 || (n & (n - 1)) || (kind != 0 && kind != 1)) return -1;
+  group_transform_1d(d, n, kind, inverse != 0, fp64 != 0);
   return 0;
 }
 

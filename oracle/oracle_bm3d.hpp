@@ -15,6 +15,7 @@ struct Bm3dParams {
   int window = 19;
   int max_group = 16;
   int max_candidates = 1024;
+  int group_transform = 0;  // 0: Walsh-Hadamard (bm3d.rs:595-614), 1: Haar (extension)
   // Bm3dParams::from_intensity (bm3d.rs:22-32) + detail strength (bm3d.rs:53)
   static Bm3dParams from_intensity(float i);
 };
@@ -27,5 +28,6 @@ template <class T>
 void bm3d(float* rgb, size_t w, size_t h, const Bm3dParams& p, Bm3dStats* stats);
 void chroma_bm3d(float* rgb, size_t w, size_t h, float intensity);
 void bm3d_tables(float* dct, float* idct, float* kaiser);
+void group_transform_1d(double* d, int n, int kind, bool inverse, bool fp64);
 
 }  // namespace oracle
