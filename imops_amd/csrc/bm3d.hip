@@ -63,6 +63,7 @@ struct MatchShared {
   float dist[kMaxWin * kMaxWin];      // +inf when rejected
   int chunk_count[(kMaxWin * kMaxWin + 63) / 64];  // accepted candidates per 64 in scan order
   unsigned long long top[MT / 64][16];  // each wave's smallest (dist bits, scan index) keys
+  int band_count;                       // accepted candidates in the first band of rows
 };
 constexpr int kMatchChunks = (kMaxWin * kMaxWin + 63) / 64;
 constexpr int kChunksPerWave = (kMatchChunks + MT / 64 - 1) / (MT / 64);
@@ -129,12 +130,13 @@ __device__ __forceinline__ void ssd_channel(const float* win, const float* ref, 
   for (int q = 0; q < Q; ++q) out[q] = mind[q] < 0x80000000u ? __uint_as_float(mind[q] + stop1[q]) : acc[q];
 }
 
-// Widest candidate block per thread that keeps the window in one or two passes
-// of the workgroup (cost ~ passes x per-row work).
+// Candidate block width per thread for ny rows of nx candidates: the width
+// with the least per-thread work over the workgroup's passes (cost ~ passes x
+// per-row work).
 __device__ __forceinline__ int ssd_block_width(int nx, int ny) {
   int best = 2, best_cost = 1 << 30;
-  const int widths[3] = {2, 4, 7};
-  for (int i = 0; i < 3; ++i) {
+  const int widths[5] = {2, 3, 4, 5, 7};
+  for (int i = 0; i < 5; ++i) {
     const int q = widths[i];
     const int passes = ((nx + q - 1) / q * ny + MT - 1) / MT;
     const int cost = passes * (26 * q + (q + 7) / 2 + 4);
@@ -146,23 +148,25 @@ __device__ __forceinline__ int ssd_block_width(int nx, int ny) {
   return best;
 }
 
-// Joint 3-channel distances of every candidate of the staged window, in scan
+// Joint 3-channel distances of the candidates in window rows [y0, y1), in scan
 // order, into S.dist (+inf when rejected) -- block_matching_joint's per-candidate
 // logic (bm3d.rs:318-350): channel 1 and 2 stop at thr minus the distance so
 // far, a candidate whose partial sum exceeds thr is dropped, the total must be
 // below thr.  The reference position itself is excluded (prepended later).
+// Returns this thread's number of accepted candidates.
 template <int Q>
-__device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, int span_x, int col0, int ref_cx,
-                                               int ref_cy, float thr) {
+__device__ __forceinline__ int ssd_candidates(MatchShared& S, int nx, int y0, int y1, int span_x, int col0, int ref_cx,
+                                              int ref_cy, float thr) {
   const int tid = threadIdx.x;
   const int nbx = (nx + Q - 1) / Q;
-  const int nblk = nbx * ny;
+  const int nblk = nbx * (y1 - y0);
   const uint32_t thr1 = __float_as_uint(thr) + 1u;
+  int accepted = 0;
   for (int base = 0; base < nblk; base += MT) {
     const int blk = base + tid;
     const bool vblk = blk < nblk;
-    const int cy = vblk ? blk / nbx : 0;
-    const int cx0 = vblk ? (blk - cy * nbx) * Q : 0;
+    const int cy = vblk ? y0 + blk / nbx : y0;
+    const int cx0 = vblk ? (blk - (cy - y0) * nbx) * Q : 0;
     bool live[Q];
     bool any = false;
 #pragma unroll
@@ -205,9 +209,23 @@ __device__ __forceinline__ void ssd_candidates(MatchShared& S, int nx, int ny, i
           const float total = d0[q] + d1[q] + d2[q];
           if (total < thr) dist = total / 64.0f;
         }
+        accepted += dist != __builtin_inff() ? 1 : 0;
         S.dist[cy * nx + cx] = dist;
       }
     }
+  }
+  return accepted;
+}
+
+// ssd_candidates at block width q (one of the instantiated widths)
+__device__ __forceinline__ int ssd_rows(MatchShared& S, int q, int nx, int y0, int y1, int span_x, int col0, int ref_cx,
+                                        int ref_cy, float thr) {
+  switch (q) {
+    case 7: return ssd_candidates<7>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
+    case 5: return ssd_candidates<5>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
+    case 4: return ssd_candidates<4>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
+    case 3: return ssd_candidates<3>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
+    default: return ssd_candidates<2>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
   }
 }
 
@@ -258,14 +276,30 @@ __device__ __forceinline__ void match_staged_ref(const Bm3dMatchArgs& a, MatchSh
   const float thr = a.threshold * 64.0f;
   const int ncand = nx * ny;
   const int ref_cx = rx - sx0, ref_cy = ry - sy0;
+  const int cap = a.max_candidates - 1;
   if (!(thr >= 0.0f)) {
     // every candidate's first row already exceeds a negative threshold (or thr is NaN)
     for (int c = tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
   } else {
-    switch (a.block_width ? a.block_width : ssd_block_width(nx, ny)) {
-      case 7: ssd_candidates<7>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
-      case 4: ssd_candidates<4>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
-      default: ssd_candidates<2>(S, nx, ny, span_x, col0, ref_cx, ref_cy, thr); break;
+    // Only the first `cap` accepted candidates in scan order join the group, so
+    // rows after the one holding the cap-th acceptance cannot change it: the
+    // rows through the geometric cap (the row of scan position cap, where the
+    // cap falls when every candidate before it is accepted) plus three are
+    // evaluated first, and the rest only if they hold fewer than `cap`
+    // acceptances; otherwise the rest count as rejected.
+    const int r1 = min(ny, cap / nx + 4);
+    const int q1 = a.block_width ? a.block_width : ssd_block_width(nx, r1);
+    const int mine = ssd_rows(S, q1, nx, 0, r1, span_x, col0, ref_cx, ref_cy, thr);
+    if (tid == 0) S.band_count = 0;
+    __syncthreads();
+    if (mine) atomicAdd(&S.band_count, mine);
+    __syncthreads();
+    if (r1 < ny) {
+      if (S.band_count >= cap) {  // workgroup-uniform
+        for (int c = r1 * nx + tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
+      } else {
+        ssd_rows(S, ssd_block_width(nx, ny - r1), nx, r1, ny, span_x, col0, ref_cx, ref_cy, thr);
+      }
     }
   }
   __syncthreads();
@@ -287,7 +321,6 @@ __device__ __forceinline__ void match_staged_ref(const Bm3dMatchArgs& a, MatchSh
     if (lane >= off) incl += o;
   }
   const int total_acc = __shfl(incl, 63, 64);
-  const int cap = a.max_candidates - 1;
   const int n_list = 1 + min(total_acc, cap);
   const int limit = min(a.max_group, n_list);
   int g = 1;
@@ -650,7 +683,9 @@ __device__ __forceinline__ void haar_lanes_inv(float* v, int k, int g) {
 // LDS: the footprint's i64 numerators, then its u32 denominators.  A
 // workgroup's denominator sum at one pixel is at most 64 references x 16
 // patches x 1e5 < 2^32, so the 32-bit partial sums are exact.
-template <bool kStep1, bool kTrial>
+// kHaar: the Haar group transform instead of the reference's Walsh-Hadamard
+// (a separate instantiation, so the reference's path compiles unchanged).
+template <bool kStep1, bool kTrial, bool kHaar>
 __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
   extern __shared__ unsigned long long acc[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -711,7 +746,7 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
       load_rows(grows, yr);
       const float wscale = g > 0 ? 1.0f / sqrtf((float)g) : 0.0f;
       dct2_cols(yr, gv, bm3d_table(0));
-      if (a.haar) haar_lanes_fwd(gv, k, g);
+      if (kHaar) haar_lanes_fwd(gv, k, g);
       else wht_lanes(gv, k, g, wscale);
       float weight;
       if (kStep1) {
@@ -754,7 +789,7 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
         weight = run > 0.0f ? 1.0f / run : 1.0f;
         load_rows(nrows, yr);
         dct2_cols(yr, nv, bm3d_table(0));
-        if (a.haar) haar_lanes_fwd(nv, k, g);
+        if (kHaar) haar_lanes_fwd(nv, k, g);
         else wht_lanes(nv, k, g, wscale);
 #pragma unroll
         for (int i = 0; i < 64; ++i) nv[i] *= gv[i];  // element 0 of patch 0: coefficient 1 (exact)
@@ -763,7 +798,7 @@ __global__ void __launch_bounds__(FT, 3) bm3d_filter_kernel(Bm3dFilterArgs a) {
       // the inverse Haar leaves lanes k > 0 negated (see haar_lanes_inv); the
       // inverse DCT is odd, so the numerator's weight carries the sign back
       // (an exact product of two negated operands)
-      const bool haar_inv = a.haar && g > 2;
+      const bool haar_inv = kHaar && g > 2;
       if (haar_inv) haar_lanes_inv(nv, k, g);
       else wht_lanes(nv, k, g, wscale);
       idct2_regs(nv, bm3d_table(1));
@@ -1024,12 +1059,15 @@ hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t 
 }
 
 hipError_t bm3d_filter_launch(const Bm3dFilterArgs& a, bool step1, int nblocks, size_t smem, hipStream_t s, bool trial) {
-  if (trial) {
-    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, true>), dim3(nblocks), dim3(FT), smem, s, a);
-    else hipLaunchKernelGGL((bm3d_filter_kernel<false, true>), dim3(nblocks), dim3(FT), smem, s, a);
+  if (a.haar) {  // (trial launches share the real kernels)
+    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, false, true>), dim3(nblocks), dim3(FT), smem, s, a);
+    else hipLaunchKernelGGL((bm3d_filter_kernel<false, false, true>), dim3(nblocks), dim3(FT), smem, s, a);
+  } else if (trial) {
+    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, true, false>), dim3(nblocks), dim3(FT), smem, s, a);
+    else hipLaunchKernelGGL((bm3d_filter_kernel<false, true, false>), dim3(nblocks), dim3(FT), smem, s, a);
   } else {
-    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, false>), dim3(nblocks), dim3(FT), smem, s, a);
-    else hipLaunchKernelGGL((bm3d_filter_kernel<false, false>), dim3(nblocks), dim3(FT), smem, s, a);
+    if (step1) hipLaunchKernelGGL((bm3d_filter_kernel<true, false, false>), dim3(nblocks), dim3(FT), smem, s, a);
+    else hipLaunchKernelGGL((bm3d_filter_kernel<false, false, false>), dim3(nblocks), dim3(FT), smem, s, a);
   }
   return hipGetLastError();
 }

@@ -18,7 +18,7 @@ struct Bm3dMatchArgs {
   uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
   const uint32_t* ref_list = nullptr;   // list launch: the patches to match
   const uint32_t* ref_count = nullptr;  // device count of ref_list entries
-  int block_width = 0;                  // candidates per thread (2, 4, 7); 0: chosen per window
+  int block_width = 0;                  // candidates per thread in the first band of rows (2, 3, 4, 5, 7); 0: chosen per window
   bool pairs = false;                   // one workgroup per two adjacent patches (bm3d_match_pairs_fit)
 };
 

@@ -266,7 +266,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
         } else {
           // candidate block width per thread: every choice selects the same groups
           m.block_width = tuned_choice(
-              ctx, "bm3d_match/" + shape_key + (step1 ? "/1" : "/2"), {0, 2, 4, 7}, 0,
+              ctx, "bm3d_match/" + shape_key + (step1 ? "/1" : "/2"), {0, 2, 4, 5, 7}, 0,
               [&](int q) {
                 Bm3dMatchArgs t = m;
                 t.block_width = q;

@@ -1059,7 +1059,8 @@ constexpr int filter_threads(int tile) { return tile >= 8 ? 1024 : 512; }
 constexpr int filter_fixed_lds(int threads) { return 16384 + (threads / 64) * 2048; }
 
 // kTrial only separates autotuner trial launches from real ones in kernel traces.
-template <bool kStep1, bool kTrial, int FM>
+// kHaar: the Haar group transform instead of the reference's Walsh-Hadamard.
+template <bool kStep1, bool kTrial, int FM, bool kHaar>
 __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, const half8* __restrict__ frags) {
   constexpr int kFilterFixedLds = filter_fixed_lds(FM);
   extern __shared__ __align__(16) unsigned char fsh[];
@@ -1120,19 +1121,29 @@ __global__ void __launch_bounds__(FM) bm3d_filter_mfma_kernel(Bm3dFilterArgs a, 
       // transpose).  Walsh-Hadamard: the Sylvester matrix / sqrt(g), symmetric.
       // Haar (in-place lifting order): row 0 = 1/sqrt(g) everywhere; row k > 0
       // with lowest set bit h = +1/sqrt(2h) on [k - h, k), -1/sqrt(2h) on [k, k + h).
-      auto tcoef = [&](int r, int c) -> float {
-        if (r >= g || c >= g) return 0.0f;
-        if (!a.haar) return ((__popc(r & c) & 1) ? -1.0f : 1.0f) / sqrtf((float)g);
-        if (r == 0) return 1.0f / sqrtf((float)g);
-        const int h = r & -r;
-        const float v = 1.0f / sqrtf((float)(2 * h));
-        return (c >= r - h && c < r) ? v : ((c >= r && c < r + h) ? -v : 0.0f);
-      };
       half4 Hf, Hi;
+      if (!kHaar) {
+        const float hs = 1.0f / sqrtf((float)g);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        Hf[e] = (_Float16)tcoef(n, 4 * q + e);
-        Hi[e] = (_Float16)tcoef(4 * q + e, n);
+        for (int e = 0; e < 4; ++e) {
+          const int kk = 4 * q + e;
+          const float v = (n < g && kk < g) ? ((__popc(n & kk) & 1) ? -hs : hs) : 0.0f;
+          Hf[e] = (_Float16)v;
+        }
+        Hi = Hf;
+      } else {
+        auto tcoef = [&](int r, int c) -> float {
+          if (r >= g || c >= g) return 0.0f;
+          if (r == 0) return 1.0f / sqrtf((float)g);
+          const int h = r & -r;
+          const float v = 1.0f / sqrtf((float)(2 * h));
+          return (c >= r - h && c < r) ? v : ((c >= r && c < r + h) ? -v : 0.0f);
+        };
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          Hf[e] = (_Float16)tcoef(n, 4 * q + e);
+          Hi[e] = (_Float16)tcoef(4 * q + e, n);
+        }
       }
       // forward: 2-D DCT (rows = patches) then Hadamard across the group
       const uint32_t p0 = grp[1];
@@ -1315,8 +1326,18 @@ hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblo
   const half8* f = reinterpret_cast<const half8*>(frags);
   const int threads = filter_threads(a.tile);
 #define LAUNCH_FILTER(S1, TR, TH) \
-  hipLaunchKernelGGL((bm3d_filter_mfma_kernel<S1, TR, TH>), dim3(nblocks), dim3(TH), smem, s, a, f)
-  if (threads == 1024) {
+  hipLaunchKernelGGL((bm3d_filter_mfma_kernel<S1, TR, TH, false>), dim3(nblocks), dim3(TH), smem, s, a, f)
+#define LAUNCH_HAAR(S1, TH) \
+  hipLaunchKernelGGL((bm3d_filter_mfma_kernel<S1, false, TH, true>), dim3(nblocks), dim3(TH), smem, s, a, f)
+  if (a.haar) {  // (trial launches share the real kernels)
+    if (threads == 1024) {
+      if (step1) LAUNCH_HAAR(true, 1024);
+      else LAUNCH_HAAR(false, 1024);
+    } else {
+      if (step1) LAUNCH_HAAR(true, 512);
+      else LAUNCH_HAAR(false, 512);
+    }
+  } else if (threads == 1024) {
     if (trial) {
       if (step1) LAUNCH_FILTER(true, true, 1024);
       else LAUNCH_FILTER(false, true, 1024);
@@ -1334,6 +1355,7 @@ hipError_t bm3d_filter_mfma_launch(const Bm3dFilterArgs& a, bool step1, int nblo
     }
   }
 #undef LAUNCH_FILTER
+#undef LAUNCH_HAAR
   return hipGetLastError();
 }
 
