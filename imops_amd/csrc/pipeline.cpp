@@ -459,6 +459,8 @@ void run_stages(imops_ctx* ctx, const std::vector<Stage>& stages, imops_image* i
   size_t k = 0;
   while (k < stages.size()) {
     const Stage& s = stages[k];
+    // a roctx range per stage (one for a fused run of elementwise stages)
+    const MarkerRange range(is_pointwise(s) ? "elementwise stages" : s.name.c_str());
     if (is_pointwise(s)) {
       // Fuse the run of elementwise stages, tracking the colour space as the
       // stages would update it.

@@ -13,6 +13,8 @@
 #include "peaks.hpp"
 #include "runtime.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 namespace imops {
 
 namespace {
@@ -130,7 +132,10 @@ void KernelTimer::release() {
   pinned.clear();
 }
 
-TimedRegion::TimedRegion(imops_ctx* ctx, const char* name) : ctx_(ctx) {
+MarkerRange::MarkerRange(const char* name) { roctxRangePushA(name); }
+MarkerRange::~MarkerRange() { roctxRangePop(); }
+
+TimedRegion::TimedRegion(imops_ctx* ctx, const char* name) : range_(name), ctx_(ctx) {
   if (!ctx_ || !ctx_->timer.enabled) return;
   hipEvent_t start = nullptr;
   if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop_) != hipSuccess) {

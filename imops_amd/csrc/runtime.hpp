@@ -113,6 +113,17 @@ struct imops_ctx {
 
 namespace imops {
 // Records start/stop events on the context stream around its scope when timing is on.
+// A roctx range on the calling host thread for the scope (what a profiler's
+// marker trace shows: `rocprofv3 --marker-trace`); the library's stages and
+// timed regions open one each.
+class MarkerRange {
+ public:
+  explicit MarkerRange(const char* name);
+  ~MarkerRange();
+  MarkerRange(const MarkerRange&) = delete;
+  MarkerRange& operator=(const MarkerRange&) = delete;
+};
+
 class TimedRegion {
  public:
   TimedRegion(imops_ctx* ctx, const char* name);
@@ -121,6 +132,7 @@ class TimedRegion {
   TimedRegion& operator=(const TimedRegion&) = delete;
 
  private:
+  MarkerRange range_;
   imops_ctx* ctx_;
   hipEvent_t stop_ = nullptr;
 };
