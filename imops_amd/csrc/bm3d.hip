@@ -50,23 +50,15 @@ __global__ void rgb_to_ycbcr_kernel(const float* __restrict__ rgb, size_t n, flo
 }
 
 // ---------------------------------------------------------------- matching
-// One workgroup per reference patch; the clamped search window of the three
-// guide planes is staged in LDS.
-constexpr int MT = 256;               // matching threads
-constexpr int kMaxWin = kBm3dMaxWindow; // largest supported search window
-constexpr int kMaxSpan = kMaxWin + 7;  // staged rows/cols
-constexpr int kWinCap = kMaxSpan * (kMaxSpan + 4);  // staged floats per channel (one window, or a pair's union)
-
-struct MatchShared {
-  float win[3][kWinCap];
-  float ref[3][64];
-  float dist[kMaxWin * kMaxWin];      // +inf when rejected
-  int chunk_count[(kMaxWin * kMaxWin + 63) / 64];  // accepted candidates per 64 in scan order
-  unsigned long long top[MT / 64][16];  // each wave's smallest (dist bits, scan index) keys
-  int band_count;                       // accepted candidates in the first band of rows
-};
-constexpr int kMatchChunks = (kMaxWin * kMaxWin + 63) / 64;
-constexpr int kChunksPerWave = (kMatchChunks + MT / 64 - 1) / (MT / 64);
+// One wave per reference patch.  A workgroup of up to four waves takes that
+// many horizontally adjacent reference patches of one patch row: their clamped
+// search windows share rows and overlap in columns, so the union is staged once
+// in LDS per guide plane (row pitch a.match_pitch <= 64, one lane per column),
+// and each wave then matches its own patch without further barriers.
+constexpr int kMatchWaves = 4;   // reference patches (waves) per workgroup at most
+constexpr int kMatchPad = 16;    // floats after each staged plane (reads past a row end)
+constexpr int kTopK = 8;         // smallest keys a lane keeps; refilled when a lane runs out
+constexpr uint32_t kNoKey = 0xffffffffu;
 
 // Wave-wide minimum of a u32: butterflies inside each 16-lane DPP row (xor 1
 // and 2 as quad permutes, xor 4 as a half-row mirror then a quad reversal,
@@ -92,9 +84,9 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 // tracked with one unsigned difference and min per row:
 // bits(acc) - (bits(stop) + 1) < 2^31 exactly when acc > stop.  (A NaN sum
 // never meets the threshold tests, as in the reference, whichever branch it
-// takes here.)
+// takes here.)  `ref` is the reference patch, 8 floats per row.
 template <int Q>
-__device__ __forceinline__ void ssd_channel(const float* win, const float* ref, int span_x, const uint32_t* stop1,
+__device__ __forceinline__ void ssd_channel(const float* win, const float* ref, int pitch, const uint32_t* stop1,
                                             float* out) {
   float acc[Q];
   uint32_t mind[Q];
@@ -105,10 +97,10 @@ __device__ __forceinline__ void ssd_channel(const float* win, const float* ref, 
   }
 #pragma unroll 1
   for (int dy = 0; dy < 8; ++dy) {
-    const float* row = win + dy * span_x;
+    const float* row = win + dy * pitch;
     float wv[Q + 7], rv[8];
-    // values past the row end (at most Q - 2, read from the next row or the
-    // following LDS array) only reach candidates with cx >= nx, which are dead
+    // values past the row end (at most Q - 1) only reach candidates with
+    // cx >= nx, which are dead
 #pragma unroll
     for (int t = 0; t < Q + 7; ++t) wv[t] = row[t];
 #pragma unroll
@@ -130,15 +122,59 @@ __device__ __forceinline__ void ssd_channel(const float* win, const float* ref, 
   for (int q = 0; q < Q; ++q) out[q] = mind[q] < 0x80000000u ? __uint_as_float(mind[q] + stop1[q]) : acc[q];
 }
 
-// Candidate block width per thread for ny rows of nx candidates: the width
-// with the least per-thread work over the workgroup's passes (cost ~ passes x
-// per-row work).
+// Joint 3-channel distances of one lane's block of Q candidates (window offset
+// `off` of its first candidate) -- block_matching_joint's per-candidate logic
+// (bm3d.rs:318-350): channel 1 and 2 stop at thr minus the distance so far, a
+// candidate whose partial sum exceeds thr is dropped, the total must be below
+// thr.  dist = total / 64 for accepted candidates, +inf otherwise.
+template <int Q>
+__device__ __forceinline__ void block_dists(const float* win, int plane, int pitch, const float* ref, int off,
+                                            bool (&live)[Q], float thr, float (&dist)[Q]) {
+  const uint32_t thr1 = __float_as_uint(thr) + 1u;
+  float d0[Q], d1[Q], d2[Q];
+  uint32_t stop1[Q];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    stop1[q] = thr1;
+    d0[q] = d1[q] = d2[q] = 0.0f;
+    any |= live[q];
+  }
+  if (__any(any)) ssd_channel<Q>(win + off, ref, pitch, stop1, d0);
+  any = false;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (d0[q] > thr) live[q] = false;
+    stop1[q] = __float_as_uint(thr - d0[q]) + 1u;
+    any |= live[q];
+  }
+  if (__any(any)) ssd_channel<Q>(win + plane + off, ref + 64, pitch, stop1, d1);
+  any = false;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    if (d0[q] + d1[q] > thr) live[q] = false;
+    stop1[q] = __float_as_uint(thr - (d0[q] + d1[q])) + 1u;
+    any |= live[q];
+  }
+  if (__any(any)) ssd_channel<Q>(win + 2 * plane + off, ref + 128, pitch, stop1, d2);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    dist[q] = __builtin_inff();
+    if (live[q]) {
+      const float total = d0[q] + d1[q] + d2[q];
+      if (total < thr) dist[q] = total / 64.0f;
+    }
+  }
+}
+
+// Candidate block width for a wave over ny rows of nx candidates: the width
+// with the least per-lane work (cost ~ passes x per-row work).
 __device__ __forceinline__ int ssd_block_width(int nx, int ny) {
   int best = 2, best_cost = 1 << 30;
   const int widths[5] = {2, 3, 4, 5, 7};
   for (int i = 0; i < 5; ++i) {
     const int q = widths[i];
-    const int passes = ((nx + q - 1) / q * ny + MT - 1) / MT;
+    const int passes = ((nx + q - 1) / q * ny + 63) / 64;
     const int cost = passes * (26 * q + (q + 7) / 2 + 4);
     if (cost < best_cost) {
       best_cost = cost;
@@ -146,87 +182,6 @@ __device__ __forceinline__ int ssd_block_width(int nx, int ny) {
     }
   }
   return best;
-}
-
-// Joint 3-channel distances of the candidates in window rows [y0, y1), in scan
-// order, into S.dist (+inf when rejected) -- block_matching_joint's per-candidate
-// logic (bm3d.rs:318-350): channel 1 and 2 stop at thr minus the distance so
-// far, a candidate whose partial sum exceeds thr is dropped, the total must be
-// below thr.  The reference position itself is excluded (prepended later).
-// Returns this thread's number of accepted candidates.
-template <int Q>
-__device__ __forceinline__ int ssd_candidates(MatchShared& S, int nx, int y0, int y1, int span_x, int col0, int ref_cx,
-                                              int ref_cy, float thr) {
-  const int tid = threadIdx.x;
-  const int nbx = (nx + Q - 1) / Q;
-  const int nblk = nbx * (y1 - y0);
-  const uint32_t thr1 = __float_as_uint(thr) + 1u;
-  int accepted = 0;
-  for (int base = 0; base < nblk; base += MT) {
-    const int blk = base + tid;
-    const bool vblk = blk < nblk;
-    const int cy = vblk ? y0 + blk / nbx : y0;
-    const int cx0 = vblk ? (blk - (cy - y0) * nbx) * Q : 0;
-    bool live[Q];
-    bool any = false;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int cx = cx0 + q;
-      live[q] = vblk && cx < nx && !(cx == ref_cx && cy == ref_cy);
-      any |= live[q];
-    }
-    float d0[Q], d1[Q], d2[Q];
-    uint32_t stop1[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      stop1[q] = thr1;
-      d0[q] = d1[q] = d2[q] = 0.0f;
-    }
-    const int off = cy * span_x + col0 + cx0;
-    if (__any(any)) ssd_channel<Q>(S.win[0] + off, S.ref[0], span_x, stop1, d0);
-    any = false;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      if (d0[q] > thr) live[q] = false;
-      stop1[q] = __float_as_uint(thr - d0[q]) + 1u;
-      any |= live[q];
-    }
-    if (__any(any)) ssd_channel<Q>(S.win[1] + off, S.ref[1], span_x, stop1, d1);
-    any = false;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      if (d0[q] + d1[q] > thr) live[q] = false;
-      stop1[q] = __float_as_uint(thr - (d0[q] + d1[q])) + 1u;
-      any |= live[q];
-    }
-    if (__any(any)) ssd_channel<Q>(S.win[2] + off, S.ref[2], span_x, stop1, d2);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int cx = cx0 + q;
-      if (vblk && cx < nx) {
-        float dist = __builtin_inff();
-        if (live[q]) {
-          const float total = d0[q] + d1[q] + d2[q];
-          if (total < thr) dist = total / 64.0f;
-        }
-        accepted += dist != __builtin_inff() ? 1 : 0;
-        S.dist[cy * nx + cx] = dist;
-      }
-    }
-  }
-  return accepted;
-}
-
-// ssd_candidates at block width q (one of the instantiated widths)
-__device__ __forceinline__ int ssd_rows(MatchShared& S, int q, int nx, int y0, int y1, int span_x, int col0, int ref_cx,
-                                        int ref_cy, float thr) {
-  switch (q) {
-    case 7: return ssd_candidates<7>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
-    case 5: return ssd_candidates<5>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
-    case 4: return ssd_candidates<4>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
-    case 3: return ssd_candidates<3>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
-    default: return ssd_candidates<2>(S, nx, y0, y1, span_x, col0, ref_cx, ref_cy, thr);
-  }
 }
 
 // Search window of one reference patch (clamped to the image, bm3d.rs:296-305).
@@ -246,204 +201,218 @@ __device__ __forceinline__ RefGeom ref_geom(const Bm3dMatchArgs& a, int ref_id) 
   return G;
 }
 
-// Stage rows sy0 .. sy0 + rows - 1 and columns sx0 .. sx0 + cols - 1 (cols <= 64)
-// of the three guide planes, row pitch cols: one row per wave and step, one lane
-// per column.
-__device__ __forceinline__ void stage_window(const Bm3dMatchArgs& a, MatchShared& S, int sx0, int sy0, int cols,
-                                             int rows) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int ch = 0; ch < 3; ++ch) {
-    const float* plane = a.guide[ch];
-    for (int yy = wave; yy < rows; yy += MT / 64)
-      if (lane < cols) S.win[ch][yy * cols + lane] = plane[(size_t)(sy0 + yy) * a.width + sx0 + lane];
-  }
-}
-
-// Match one reference patch against the staged window, whose columns for this
-// patch start at col0 (row pitch `pitch`); writes the patch's group.
-__device__ __forceinline__ void match_staged_ref(const Bm3dMatchArgs& a, MatchShared& S, const int ref_id,
-                                                 const RefGeom& G, int col0, int pitch) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rx = G.rx, ry = G.ry, sx0 = G.sx0, sy0 = G.sy0, nx = G.nx, ny = G.ny;
-  const int span_x = pitch;
-  // the reference patch lies inside its own window
-  if (tid < 192) {
-    const int ch = tid >> 6, e = tid & 63;
-    S.ref[ch][e] = S.win[ch][(ry - sy0 + (e >> 3)) * pitch + col0 + (rx - sx0) + (e & 7)];
-  }
-  __syncthreads();
-
+// Match one reference patch with one wave against the staged window (columns
+// for this patch from col0) and write its group (bm3d.rs:281-365).
+//
+// The lanes walk the window in scan order, 64 blocks of Q candidates per pass;
+// after each pass a ballot prefix over the lanes' accepted counts gives every
+// candidate its rank among the accepted ones, so only the first max_candidates
+// - 1 acceptances are eligible, and the walk stops once that many are found
+// (later candidates cannot join).  Each lane keeps its kTopK smallest eligible
+// keys (distance bits, packed position: the position orders like the scan
+// index) sorted in registers; the group is the reference patch followed by the
+// g - 1 smallest keys, taken one per round as the wave minimum of the lanes'
+// heads.  A lane that had more than kTopK eligible keys and has given up all
+// it kept triggers a refill: the walk runs again and every lane keeps the
+// kTopK smallest keys above the last key it gave up (the sums are
+// deterministic, so the ranks and keys come out the same).
+template <int Q>
+__device__ __forceinline__ void match_wave(const Bm3dMatchArgs& a, const float* win, const float* ref, int col0,
+                                        const RefGeom& G, int ref_id) {
+  const int lane = threadIdx.x & 63;
+  const int nx = G.nx, ny = G.ny, pitch = a.match_pitch, plane = a.match_plane;
+  const int ref_cx = G.rx - G.sx0, ref_cy = G.ry - G.sy0;
   const float thr = a.threshold * 64.0f;
-  const int ncand = nx * ny;
-  const int ref_cx = rx - sx0, ref_cy = ry - sy0;
   const int cap = a.max_candidates - 1;
-  if (!(thr >= 0.0f)) {
-    // every candidate's first row already exceeds a negative threshold (or thr is NaN)
-    for (int c = tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
-  } else {
-    // Only the first `cap` accepted candidates in scan order join the group, so
-    // rows after the one holding the cap-th acceptance cannot change it: the
-    // rows through the geometric cap (the row of scan position cap, where the
-    // cap falls when every candidate before it is accepted) plus three are
-    // evaluated first, and the rest only if they hold fewer than `cap`
-    // acceptances; otherwise the rest count as rejected.
-    const int r1 = min(ny, cap / nx + 4);
-    const int q1 = a.block_width ? a.block_width : ssd_block_width(nx, r1);
-    const int mine = ssd_rows(S, q1, nx, 0, r1, span_x, col0, ref_cx, ref_cy, thr);
-    if (tid == 0) S.band_count = 0;
-    __syncthreads();
-    if (mine) atomicAdd(&S.band_count, mine);
-    __syncthreads();
-    if (r1 < ny) {
-      if (S.band_count >= cap) {  // workgroup-uniform
-        for (int c = r1 * nx + tid; c < ncand; c += MT) S.dist[c] = __builtin_inff();
-      } else {
-        ssd_rows(S, ssd_block_width(nx, ny - r1), nx, r1, ny, span_x, col0, ref_cx, ref_cy, thr);
-      }
-    }
-  }
-  __syncthreads();
+  const int nbx = (nx + Q - 1) / Q, nblk = nbx * ny;
+  const uint32_t pos0 = (uint32_t)G.sx0 | ((uint32_t)G.sy0 << 16);
+  uint32_t kd[kTopK], kc[kTopK];
+  bool over = false;
+  int total = 0;  // accepted candidates so far in scan order (wave-uniform)
+  uint32_t fd = 0, fc = 0;  // the lane's last selected key
+  bool fset = false;
 
-  // Cap: only the first max_candidates-1 accepted candidates (scan order) join
-  // the reference patch.  Accepted counts per 64-candidate chunk, then each
-  // candidate's rank among the accepted ones from a prefix over the chunks.
-  const int nchunks = (ncand + 63) >> 6;
-  for (int k = wave; k < nchunks; k += MT / 64) {
-    const int c = (k << 6) + lane;
-    const unsigned long long m = __ballot(c < ncand && S.dist[c] != __builtin_inff());
-    if (lane == 0) S.chunk_count[k] = __popcll(m);
-  }
-  __syncthreads();
-  const int cnt_l = lane < nchunks ? S.chunk_count[lane] : 0;
-  int incl = cnt_l;  // inclusive prefix over chunks, chunk k at lane k
-  for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += o;
-  }
-  const int total_acc = __shfl(incl, 63, 64);
-  const int n_list = 1 + min(total_acc, cap);
+  auto walk = [&]() {
+#pragma unroll
+    for (int i = 0; i < kTopK; ++i) kd[i] = kc[i] = kNoKey;
+    over = false;
+    total = 0;
+    if (!(thr >= 0.0f)) return;  // every first row already exceeds a negative (or NaN) threshold
+    for (int base = 0; base < nblk && total < cap; base += 64) {
+      const int blk = base + lane;
+      const bool vblk = blk < nblk;
+      const int cy = vblk ? blk / nbx : 0;
+      const int cx0 = vblk ? (blk - cy * nbx) * Q : 0;
+      bool live[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int cx = cx0 + q;
+        live[q] = vblk && cx < nx && !(cx == ref_cx && cy == ref_cy);
+      }
+      float dist[Q];
+      block_dists<Q>(win, plane, pitch, ref, cy * pitch + col0 + cx0, live, thr, dist);
+      int cnt = 0;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) cnt += dist[q] != __builtin_inff() ? 1 : 0;
+      // exclusive prefix of the lanes' counts (< 8) and their sum, bit by bit
+      int rank = total, wsum = 0;
+#pragma unroll
+      for (int bit = 0; bit < 3; ++bit) {
+        const unsigned long long m = __ballot((cnt >> bit) & 1);
+        rank += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+        wsum += __popcll(m) << bit;
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const uint32_t x = __float_as_uint(dist[q]);
+        const bool acc = dist[q] != __builtin_inff();
+        const uint32_t c = pos0 + ((uint32_t)cy << 16) + (uint32_t)(cx0 + q);
+        const bool above = !fset || x > fd || (x == fd && c > fc);
+        const uint32_t key = (acc && rank < cap && above) ? x : kNoKey;
+        rank += acc ? 1 : 0;
+        // a key falls off (or is not kept) when the list is full
+        over |= key != kNoKey && kd[kTopK - 1] != kNoKey;
+        if (__any(key < kd[kTopK - 1])) {
+          // sorted insert; an equal key stays behind the ones already kept
+          // (their positions are smaller)
+          bool lt[kTopK];
+#pragma unroll
+          for (int i = 0; i < kTopK; ++i) lt[i] = key < kd[i];
+#pragma unroll
+          for (int i = kTopK - 1; i >= 1; --i) {
+            kc[i] = lt[i - 1] ? kc[i - 1] : (lt[i] ? c : kc[i]);
+            kd[i] = lt[i - 1] ? kd[i - 1] : (lt[i] ? key : kd[i]);
+          }
+          kc[0] = lt[0] ? c : kc[0];
+          kd[0] = lt[0] ? key : kd[0];
+        }
+      }
+      total += wsum;
+    }
+  };
+
+  walk();
+  const int n_list = 1 + min(total, cap);
   const int limit = min(a.max_group, n_list);
   int g = 1;
   while (g * 2 <= limit) g *= 2;
   const int need = g - 1;
-
-  // Selection: the reference patch first, then the need smallest keys
-  // (dist bits, scan index) among the capped accepted candidates; dist >= 0 so
-  // its bit pattern orders like the value.  Each wave finds its own smallest
-  // need keys (wave w holds chunks w, w + 4, ...; lane l candidate 64k + l) by
-  // repeated wave minima over the lanes' current heads, then one wave ranks
-  // the merged lists.
-  // Keys of one lane: the distance bits of its candidates in chunk order (the
-  // scan index grows with the chunk), +inf bits when not eligible.
-  constexpr uint32_t kInfBits = 0x7f800000u;
-  uint32_t kb[kChunksPerWave];
+  uint32_t* out = a.groups + (size_t)ref_id * 17;
+  for (int r = 0; r < need; ++r) {
+    if (__any(kd[0] == kNoKey && over)) walk();  // rare: refill above each lane's last selected key
+    const uint32_t wb = wave_min_u32(kd[0]);
+    const unsigned long long m = __ballot(kd[0] == wb);
+    // equal distances: the smallest position (scan index) wins
+    const uint32_t wc = __popcll(m) == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)kc[0], __ffsll((long long)m) - 1)
+                                         : wave_min_u32(kd[0] == wb ? kc[0] : kNoKey);
+    if (kd[0] == wb && kc[0] == wc) {
+      out[2 + r] = wc;
+      fd = wb;
+      fc = wc;
+      fset = true;
 #pragma unroll
-  for (int i = 0; i < kChunksPerWave; ++i) {
-    const int k = wave + i * (MT / 64);
-    const int c = (k << 6) + lane;
-    const bool accepted = k < nchunks && c < ncand && S.dist[c] != __builtin_inff();
-    const unsigned long long m = __ballot(accepted);
-    const int excl = __shfl(incl - cnt_l, k < nchunks ? k : 0, 64);
-    const int rank = excl + __popcll(m & ((1ull << lane) - 1ull));
-    kb[i] = (accepted && rank < cap) ? __float_as_uint(S.dist[c]) : kInfBits;
-  }
-  // lane head: its first smallest key
-  uint32_t hb = kb[0];
-  int hi = 0;
-#pragma unroll
-  for (int i = 1; i < kChunksPerWave; ++i)
-    if (kb[i] < hb) {
-      hb = kb[i];
-      hi = i;
+      for (int i = 0; i < kTopK - 1; ++i) {
+        kd[i] = kd[i + 1];
+        kc[i] = kc[i + 1];
+      }
+      kd[kTopK - 1] = kc[kTopK - 1] = kNoKey;
     }
-  for (int r = 0; r < 16; ++r) {
-    unsigned long long key = ~0ull;
-    if (r < need) {  // workgroup-uniform
-      const uint32_t wb = wave_min_u32(hb);
-      if (wb != kInfBits) {
-        const uint32_t hc = (uint32_t)(((wave + hi * (MT / 64)) << 6) + lane);
-        const unsigned long long m = __ballot(hb == wb);
-        // equal distances: the smallest scan index wins (one lane holds it)
-        const uint32_t wc = __popcll(m) == 1 ? (uint32_t)__builtin_amdgcn_readlane((int)hc, __ffsll((long long)m) - 1)
-                                             : wave_min_u32(hb == wb ? hc : 0xffffffffu);
-        key = ((unsigned long long)wb << 32) | wc;
-        if (hb == wb && hc == wc) {  // the winning lane drops the key and finds its next head
+  }
+  if (lane == 0) {
+    out[0] = (uint32_t)g;
+    out[1] = (uint32_t)G.rx | ((uint32_t)G.ry << 16);
+  }
+}
+
+// match_wave at the block width the launch asks for (or the one for the rows
+// the cap usually needs)
+__device__ __forceinline__ void match_wave_q(const Bm3dMatchArgs& a, const float* win, const float* ref, int col0,
+                                             const RefGeom& G, int ref_id) {
+  const int q = a.block_width ? a.block_width
+                              : ssd_block_width(G.nx, min(G.ny, (a.max_candidates - 1) / G.nx + 4));
+  switch (q) {
+    case 7: match_wave<7>(a, win, ref, col0, G, ref_id); break;
+    case 5: match_wave<5>(a, win, ref, col0, G, ref_id); break;
+    case 4: match_wave<4>(a, win, ref, col0, G, ref_id); break;
+    case 3: match_wave<3>(a, win, ref, col0, G, ref_id); break;
+    default: match_wave<2>(a, win, ref, col0, G, ref_id); break;
+  }
+}
+
+// Stage rows sy0 .. sy0 + rows - 1 and columns sx0 .. sx0 + cols - 1 (cols <= 64)
+// of the three guide planes at row pitch a.match_pitch, one lane per column,
+// rows spread over the workgroup's nw waves; four rows of the three planes are
+// loaded before they are stored.
+__device__ __forceinline__ void stage_window(const Bm3dMatchArgs& a, float* win, int sx0, int sy0, int cols, int rows,
+                                             int wave, int nw) {
+  const int lane = threadIdx.x & 63;
+  const int pitch = a.match_pitch, plane = a.match_plane;
+  const bool col_ok = lane < cols;
+  for (int y = wave; y < rows; y += 4 * nw) {
+    float v[4][3];
 #pragma unroll
-          for (int i = 0; i < kChunksPerWave; ++i)
-            if (i == hi) kb[i] = kInfBits;
-          hb = kb[0];
-          hi = 0;
+    for (int u = 0; u < 4; ++u) {
+      const int yy = y + u * nw;
+      const bool ok = col_ok && yy < rows;
+      const size_t g = (size_t)(sy0 + (ok ? yy : 0)) * a.width + sx0 + (ok ? lane : 0);
 #pragma unroll
-          for (int i = 1; i < kChunksPerWave; ++i)
-            if (kb[i] < hb) {
-              hb = kb[i];
-              hi = i;
-            }
-        }
+      for (int ch = 0; ch < 3; ++ch) v[u][ch] = ok ? a.guide[ch][g] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int yy = y + u * nw;
+      if (col_ok && yy < rows) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) win[ch * plane + yy * pitch + lane] = v[u][ch];
       }
     }
-    if (lane == 0) S.top[wave][r] = key;
-  }
-  __syncthreads();
-  uint32_t* out = a.groups + (size_t)ref_id * 17;
-  if (wave == 0) {
-    const unsigned long long mine = S.top[lane >> 4][lane & 15];
-    int rank = 0;
-    for (int j = 0; j < MT / 64 * 16; ++j) rank += S.top[j >> 4][j & 15] < mine ? 1 : 0;
-    if (mine != ~0ull && rank < need) {
-      const int c = (int)(mine & 0xffffffffu);
-      const int cy = c / nx, cx = c - cy * nx;
-      out[2 + rank] = (uint32_t)(sx0 + cx) | ((uint32_t)(sy0 + cy) << 16);
-    }
-    if (lane == 0) {
-      out[0] = (uint32_t)g;
-      out[1] = (uint32_t)rx | ((uint32_t)ry << 16);
-    }
   }
 }
 
-// One workgroup per reference patch, or (a.pairs) per two horizontally adjacent
-// patches of a row matched against one staged union of their windows.  kTrial
-// only separates autotuner trial launches from real ones in kernel traces.
+// The reference patch (3 x 8 x 8 floats) of one wave, from the guide planes.
+__device__ __forceinline__ void stage_ref(const Bm3dMatchArgs& a, float* ref, const RefGeom& G) {
+  const int lane = threadIdx.x & 63;
+  const size_t g = (size_t)(G.ry + (lane >> 3)) * a.width + G.rx + (lane & 7);
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) ref[ch * 64 + lane] = a.guide[ch][g];
+}
+
+// a.match_group reference patches of one patch row per workgroup, one per
+// wave.  kTrial only separates autotuner trial launches from real ones in
+// kernel traces.
 template <bool kTrial>
-__global__ void __launch_bounds__(MT) bm3d_match_kernel(Bm3dMatchArgs a) {
-  extern __shared__ unsigned char smem_raw[];
-  MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
-  if (!a.pairs) {
-    const RefGeom G = ref_geom(a, blockIdx.x);
-    stage_window(a, S, G.sx0, G.sy0, G.nx + 7, G.ny + 7);
-    __syncthreads();
-    match_staged_ref(a, S, blockIdx.x, G, 0, G.nx + 7);
-    return;
-  }
-  const int pairs_x = (a.refs_x + 1) / 2;
-  const int row = blockIdx.x / pairs_x, px = 2 * (blockIdx.x % pairs_x);
+__global__ void __launch_bounds__(kMatchWaves * 64) bm3d_match_kernel(Bm3dMatchArgs a) {
+  extern __shared__ float match_smem[];
+  const int R = a.match_group;
+  const int groups_x = (a.refs_x + R - 1) / R;
+  const int row = blockIdx.x / groups_x, px = R * (blockIdx.x - row * groups_x);
   const int r0 = row * a.refs_x + px;
-  const bool has1 = px + 1 < a.refs_x;
+  const int nref = min(R, a.refs_x - px);
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const RefGeom G0 = ref_geom(a, r0);
-  const RefGeom G1 = has1 ? ref_geom(a, r0 + 1) : G0;
-  const int pitch = G1.sx0 + G1.nx - G0.sx0 + 7;  // union columns (same rows)
-  stage_window(a, S, G0.sx0, G0.sy0, pitch, G0.ny + 7);
+  const RefGeom GL = ref_geom(a, r0 + nref - 1);
+  const RefGeom G = wave < nref ? ref_geom(a, r0 + wave) : G0;
+  float* ref = match_smem + 3 * a.match_plane + wave * 192;
+  stage_window(a, match_smem, G0.sx0, G0.sy0, GL.sx0 + GL.nx - G0.sx0 + 7, G0.ny + 7, wave, nw);
+  if (wave < nref) stage_ref(a, ref, G);
   __syncthreads();
-  match_staged_ref(a, S, r0, G0, 0, pitch);
-  if (has1) {
-    __syncthreads();
-    match_staged_ref(a, S, r0 + 1, G1, G1.sx0 - G0.sx0, pitch);
-  }
+  if (wave >= nref) return;  // wave-uniform; no barrier follows
+  match_wave_q(a, match_smem, ref, G.sx0 - G0.sx0, G, r0 + wave);
 }
 
-// The patches named in ref_list[0 .. *ref_count), grid-stride (the patches the
-// matrix-core matcher hands back).
-__global__ void __launch_bounds__(MT) bm3d_match_list_kernel(Bm3dMatchArgs a) {
-  extern __shared__ unsigned char smem_raw[];
-  MatchShared& S = *reinterpret_cast<MatchShared*>(smem_raw);
+// The patches named in ref_list[0 .. *ref_count) (the patches the matrix-core
+// matcher hands back), one wave per workgroup, grid-stride.
+__global__ void __launch_bounds__(64) bm3d_match_list_kernel(Bm3dMatchArgs a) {
+  extern __shared__ float match_smem[];
+  float* ref = match_smem + 3 * a.match_plane;
   const uint32_t n = *a.ref_count;
   for (uint32_t it = blockIdx.x; it < n; it += gridDim.x) {
     const int ref_id = (int)a.ref_list[it];
     const RefGeom G = ref_geom(a, ref_id);
-    stage_window(a, S, G.sx0, G.sy0, G.nx + 7, G.ny + 7);
+    stage_window(a, match_smem, G.sx0, G.sy0, G.nx + 7, G.ny + 7, 0, 1);
+    stage_ref(a, ref, G);
     __syncthreads();
-    match_staged_ref(a, S, ref_id, G, 0, G.nx + 7);
+    match_wave_q(a, match_smem, ref, 0, G, ref_id);
     __syncthreads();
   }
 }
@@ -1013,7 +982,6 @@ int grid1d(size_t n) {
 
 }  // namespace
 
-size_t bm3d_match_smem() { return sizeof(MatchShared); }
 
 hipError_t bm3d_upload_tables(const float* dct, const float* idct, const float* kaiser, hipStream_t s) {
   hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(g_bm3d_tables), dct, 64 * sizeof(float), 0, hipMemcpyHostToDevice, s);
@@ -1041,20 +1009,36 @@ hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, fl
   return hipGetLastError();
 }
 
-bool bm3d_match_pairs_fit(int window, int stride) {
-  const int span = window + 7, pitch = window + stride + 7;  // interior references have the widest union
-  return pitch <= 64 && span * pitch <= kWinCap;
+void bm3d_match_layout(Bm3dMatchArgs& a) {
+  // the widest union of R adjacent windows is that of interior patches,
+  // (R - 1) * stride + window + 7 columns; it must fit one lane per column
+  int R = kMatchWaves;
+  while (R > 1 && (R - 1) * a.stride + a.window + 7 > 64) --R;
+  a.match_group = R;
+  a.match_pitch = (R - 1) * a.stride + a.window + 7;
+  a.match_plane = (a.window + 7) * a.match_pitch + kMatchPad;
+}
+
+static size_t match_smem(const Bm3dMatchArgs& a, int waves) {
+  return (size_t)(3 * a.match_plane + waves * 192) * sizeof(float);
 }
 
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial) {
-  const int grid = a.pairs ? (nrefs / a.refs_x) * ((a.refs_x + 1) / 2) : nrefs;
-  if (trial) hipLaunchKernelGGL(bm3d_match_kernel<true>, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
-  else hipLaunchKernelGGL(bm3d_match_kernel<false>, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
+  if (a.match_group < 1 || a.match_group > kMatchWaves || a.match_pitch < a.window + 7 || a.match_pitch > 64 ||
+      a.match_plane < (a.window + 7) * a.match_pitch + kMatchPad)
+    return hipErrorInvalidValue;
+  const int R = a.match_group;
+  const int grid = (nrefs / a.refs_x) * ((a.refs_x + R - 1) / R);
+  const size_t smem = match_smem(a, R);
+  if (trial) hipLaunchKernelGGL(bm3d_match_kernel<true>, dim3(grid), dim3(64 * R), smem, s, a);
+  else hipLaunchKernelGGL(bm3d_match_kernel<false>, dim3(grid), dim3(64 * R), smem, s, a);
   return hipGetLastError();
 }
 
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(bm3d_match_list_kernel, dim3(grid), dim3(MT), bm3d_match_smem(), s, a);
+  if (a.match_pitch < a.window + 7 || a.match_pitch > 64 || a.match_plane < (a.window + 7) * a.match_pitch + kMatchPad)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(bm3d_match_list_kernel, dim3(grid), dim3(64), match_smem(a, 1), s, a);
   return hipGetLastError();
 }
 

@@ -18,8 +18,10 @@ struct Bm3dMatchArgs {
   uint32_t* groups;       // per reference patch: [g, (x | y<<16) x 16]
   const uint32_t* ref_list = nullptr;   // list launch: the patches to match
   const uint32_t* ref_count = nullptr;  // device count of ref_list entries
-  int block_width = 0;                  // candidates per thread in the first band of rows (2, 3, 4, 5, 7); 0: chosen per window
-  bool pairs = false;                   // one workgroup per two adjacent patches (bm3d_match_pairs_fit)
+  int block_width = 0;                  // candidates per lane and pass (2, 3, 4, 5, 7); 0: chosen per window
+  // staging layout (bm3d_match_layout): adjacent patches per workgroup, LDS row
+  // pitch and floats per staged plane
+  int match_group = 1, match_pitch = 0, match_plane = 0;
 };
 
 struct Bm3dFilterArgs {
@@ -63,12 +65,12 @@ struct Bm3dBlurTaps {
   float k[33];
 };
 
-size_t bm3d_match_smem();
 size_t bm3d_filter_smem(int footprint_pixels);
 hipError_t bm3d_rgb_to_ycbcr(const float* rgb, size_t n, float* y, float* cb, float* cr, hipStream_t s);
 hipError_t bm3d_match_launch(const Bm3dMatchArgs& a, int nrefs, hipStream_t s, bool trial = false);
-// whether two adjacent patches' union window fits the matcher's staging area
-bool bm3d_match_pairs_fit(int window, int stride);
+// the matcher's staging layout for this window and stride (match_group,
+// match_pitch, match_plane)
+void bm3d_match_layout(Bm3dMatchArgs& a);
 hipError_t bm3d_match_list_launch(const Bm3dMatchArgs& a, int grid, hipStream_t s);
 Bm3dMfmaArgs bm3d_mfma_layout(int width, int height, int stride, int window);
 hipError_t bm3d_match_mfma_launch(const Bm3dMfmaArgs& a, hipStream_t s);

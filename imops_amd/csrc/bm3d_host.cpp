@@ -234,7 +234,7 @@ void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
       m.max_candidates = p.max_candidates;
       m.max_group = p.max_group;
       m.groups = groups;
-      m.pairs = bm3d_match_pairs_fit(p.window, p.stride);
+      bm3d_match_layout(m);
       {
         TimedRegion tr(ctx, "bm3d_match");
         if (use_mfma) {

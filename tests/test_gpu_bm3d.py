@@ -53,6 +53,32 @@ def test_bm3d_config_geometry(gpu_ctx):
     _check(got, want)
 
 
+def _stripes(w, h, seed):
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(0.1, 0.9, (8, 3))
+    c = rng.uniform(-0.1, 0.1, ((w + 11) // 12, 3))
+    y, x = np.arange(h)[:, None], np.arange(w)[None, :]
+    return np.clip(g[y % 8] + c[x // 12], 0, 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("window", [39, 19])
+def test_bm3d_exact_copies_eight_rows_apart(gpu_ctx, window):
+    """Rows repeat every 8 lines and columns are flat over 12-pixel segments, so a
+    patch's exact copies (distance 0, ties broken by scan order) lie 8 rows apart
+    in the same few columns.  At 39x39 the matcher's lanes walk 8 blocks of 5
+    candidates per window row, so those copies fall to one lane: more of the
+    group than the 8 keys a lane keeps, which makes the wave walk the window
+    again for that lane's next keys (a CPU model of the selection counts 262 of
+    the first 300 interior patches needing it)."""
+    rgb = _stripes(120, 100, 7)
+    i = 0.02 * 255 / 80
+    p = oracle.bm3d_params(i, sigma=0.02 * 255, stride=3, window=window)
+    p["lambda"] = 2.7
+    want, _ = oracle.bm3d(rgb, p)
+    got = _gpu(gpu_ctx, rgb, i, sigma=np.float32(0.02 * 255), lambda_=2.7, stride=3, search_window=window)
+    _check(got, want)
+
+
 def test_bm3d_candidate_cap_and_group_limits(gpu_ctx):
     rgb = _noisy_scene(100, 90, seed=5)
     for cap, grp in [(40, 16), (1024, 4), (7, 8), (1024, 1)]:
