@@ -3,10 +3,11 @@
 //
 // Behaviour follows pichromatic/src/bm3d.rs bit for bit:
 //  * matching (bm3d.rs:281-391): every candidate's 3-channel SSD is summed in
-//    the reference's order (row-major, one running f32 sum per channel) with
-//    the same per-row early exits and per-channel stop thresholds, accepted
-//    candidates are capped at max_candidates in scan order, and the group is
-//    the first prev_pow2(min(16, n)) entries of the (dist, scan order) sort;
+//    the reference's order (row-major, one running f32 sum per channel); the
+//    reference's per-row early exits change no decision and no accepted
+//    distance (ssd_channel), so the full sums are taken; accepted candidates
+//    are capped at max_candidates in scan order, and the group is the first
+//    prev_pow2(min(16, n)) entries of the (dist, scan order) sort;
 //  * filtering (bm3d.rs:128-238, 490-614): 8x8 DCT as the reference's
 //    row-dot-products, WHT across the group with 1/sqrt(n), hard threshold /
 //    Wiener, inverse; every output sample is one thread's sequential sum in
@@ -76,25 +77,26 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 }
 
 // compute_ssd_flat (bm3d.rs:367-391) for Q horizontally adjacent candidates of
-// one channel: the running sum over rows in the reference's order, returning
-// the sum at the first row end where it exceeds stop, else the full sum.  Rows
-// are summed unconditionally; the sum never decreases (non-negative terms), so
-// the early-exit value is the smallest row-end sum above stop.  For sums and
-// stop >= 0 the float bit patterns order like the values, so that value is
-// tracked with one unsigned difference and min per row:
-// bits(acc) - (bits(stop) + 1) < 2^31 exactly when acc > stop.  (A NaN sum
-// never meets the threshold tests, as in the reference, whichever branch it
-// takes here.)  `ref` is the reference patch, 8 floats per row.
+// one channel: the running sum over the rows in the reference's order, every
+// row.  The reference stops at the first row end whose sum exceeds a stop
+// value (thr for channel 0, thr minus the distance so far for channels 1 and
+// 2); the full sum decides the same:
+//  * the sums never decrease (non-negative terms, monotone rounding), so a
+//    stop is hit exactly when the full sum exceeds it;
+//  * a candidate that hits a stop is rejected by the reference: in channel 0
+//    by d0 > thr; in channel k > 0 the partial sum d > fl(thr - s) (s the
+//    rounded distance so far, s <= thr) forces fl(s + d) >= thr, since
+//    fl(s + d) < thr would need s + d < thr, i.e. d < thr - s, and then
+//    d <= fl(thr - s) for the float d; the total can then not be below thr,
+//    and a larger (full) sum keeps every test failing the same way;
+//  * a candidate that hits no stop has the full sums as its distances.
+// (NaN sums fail every threshold test on both sides.)  `ref` is the
+// reference patch, 8 floats per row.
 template <int Q>
-__device__ __forceinline__ void ssd_channel(const float* win, const float* ref, int pitch, const uint32_t* stop1,
-                                            float* out) {
+__device__ __forceinline__ void ssd_channel(const float* win, const float* ref, int pitch, float* out) {
   float acc[Q];
-  uint32_t mind[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    acc[q] = 0.0f;
-    mind[q] = 0xffffffffu;
-  }
+  for (int q = 0; q < Q; ++q) acc[q] = 0.0f;
 #pragma unroll 1
   for (int dy = 0; dy < 8; ++dy) {
     const float* row = win + dy * pitch;
@@ -114,49 +116,42 @@ __device__ __forceinline__ void ssd_channel(const float* win, const float* ref, 
         a_ += diff * diff;
       }
       acc[q] = a_;
-      const uint32_t d = __float_as_uint(a_) - stop1[q];
-      mind[q] = d < mind[q] ? d : mind[q];
     }
   }
 #pragma unroll
-  for (int q = 0; q < Q; ++q) out[q] = mind[q] < 0x80000000u ? __uint_as_float(mind[q] + stop1[q]) : acc[q];
+  for (int q = 0; q < Q; ++q) out[q] = acc[q];
 }
 
 // Joint 3-channel distances of one lane's block of Q candidates (window offset
 // `off` of its first candidate) -- block_matching_joint's per-candidate logic
-// (bm3d.rs:318-350): channel 1 and 2 stop at thr minus the distance so far, a
-// candidate whose partial sum exceeds thr is dropped, the total must be below
-// thr.  dist = total / 64 for accepted candidates, +inf otherwise.
+// (bm3d.rs:318-350) on the full channel sums: a candidate whose distance so far
+// exceeds thr is dropped, the total must be below thr.  dist = total / 64 for
+// accepted candidates, +inf otherwise.
 template <int Q>
 __device__ __forceinline__ void block_dists(const float* win, int plane, int pitch, const float* ref, int off,
                                             bool (&live)[Q], float thr, float (&dist)[Q]) {
-  const uint32_t thr1 = __float_as_uint(thr) + 1u;
   float d0[Q], d1[Q], d2[Q];
-  uint32_t stop1[Q];
   bool any = false;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    stop1[q] = thr1;
     d0[q] = d1[q] = d2[q] = 0.0f;
     any |= live[q];
   }
-  if (__any(any)) ssd_channel<Q>(win + off, ref, pitch, stop1, d0);
+  if (__any(any)) ssd_channel<Q>(win + off, ref, pitch, d0);
   any = false;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     if (d0[q] > thr) live[q] = false;
-    stop1[q] = __float_as_uint(thr - d0[q]) + 1u;
     any |= live[q];
   }
-  if (__any(any)) ssd_channel<Q>(win + plane + off, ref + 64, pitch, stop1, d1);
+  if (__any(any)) ssd_channel<Q>(win + plane + off, ref + 64, pitch, d1);
   any = false;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     if (d0[q] + d1[q] > thr) live[q] = false;
-    stop1[q] = __float_as_uint(thr - (d0[q] + d1[q])) + 1u;
     any |= live[q];
   }
-  if (__any(any)) ssd_channel<Q>(win + 2 * plane + off, ref + 128, pitch, stop1, d2);
+  if (__any(any)) ssd_channel<Q>(win + 2 * plane + off, ref + 128, pitch, d2);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     dist[q] = __builtin_inff();
