@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "bm3d.hpp"
 
 namespace imops {
@@ -123,42 +125,35 @@ __device__ __forceinline__ void ssd_channel(const float* win, const float* ref, 
 }
 
 // Joint 3-channel distances of one lane's block of Q candidates (window offset
-// `off` of its first candidate) -- block_matching_joint's per-candidate logic
-// (bm3d.rs:318-350) on the full channel sums: a candidate whose distance so far
-// exceeds thr is dropped, the total must be below thr.  dist = total / 64 for
+// `off` of its first candidate; bit q of `valid` set for real candidates) --
+// block_matching_joint's per-candidate logic (bm3d.rs:318-350) on the full
+// channel sums.  Its tests d0 > thr and d0 + d1 > thr are implied by the last
+// one, total = (d0 + d1) + d2 < thr (the sums are non-negative and rounding
+// is monotone), so they only serve here to skip a channel for the whole wave
+// when no candidate of any lane can still pass.  dist = total / 64 for
 // accepted candidates, +inf otherwise.
 template <int Q>
 __device__ __forceinline__ void block_dists(const float* win, int plane, int pitch, const float* ref, int off,
-                                            bool (&live)[Q], float thr, float (&dist)[Q]) {
+                                            uint32_t valid, float thr, float (&dist)[Q]) {
   float d0[Q], d1[Q], d2[Q];
-  bool any = false;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) d0[q] = d1[q] = d2[q] = 0.0f;
+  if (__any(valid != 0)) ssd_channel<Q>(win + off, ref, pitch, d0);
+  float m = __builtin_inff();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) m = (valid >> q) & 1 ? fminf(m, d0[q]) : m;
+  if (__any(m <= thr)) ssd_channel<Q>(win + plane + off, ref + 64, pitch, d1);
+  m = __builtin_inff();
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    d0[q] = d1[q] = d2[q] = 0.0f;
-    any |= live[q];
+    d0[q] = d0[q] + d1[q];
+    m = (valid >> q) & 1 ? fminf(m, d0[q]) : m;
   }
-  if (__any(any)) ssd_channel<Q>(win + off, ref, pitch, d0);
-  any = false;
+  if (__any(m <= thr)) ssd_channel<Q>(win + 2 * plane + off, ref + 128, pitch, d2);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    if (d0[q] > thr) live[q] = false;
-    any |= live[q];
-  }
-  if (__any(any)) ssd_channel<Q>(win + plane + off, ref + 64, pitch, d1);
-  any = false;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    if (d0[q] + d1[q] > thr) live[q] = false;
-    any |= live[q];
-  }
-  if (__any(any)) ssd_channel<Q>(win + 2 * plane + off, ref + 128, pitch, d2);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    dist[q] = __builtin_inff();
-    if (live[q]) {
-      const float total = d0[q] + d1[q] + d2[q];
-      if (total < thr) dist[q] = total / 64.0f;
-    }
+    const float total = d0[q] + d2[q];
+    dist[q] = ((valid >> q) & 1) && total < thr ? total / 64.0f : __builtin_inff();
   }
 }
 
@@ -224,69 +219,97 @@ __device__ __forceinline__ void match_wave(const Bm3dMatchArgs& a, const float* 
   uint32_t kd[kTopK], kc[kTopK];
   bool over = false;
   int total = 0;  // accepted candidates so far in scan order (wave-uniform)
-  uint32_t fd = 0, fc = 0;  // the lane's last selected key
-  bool fset = false;
+  long long fk = -1;  // the lane's last selected key (distance bits << 32 | position), -1: none yet
 
-  auto walk = [&]() {
+  // One lane-block of QB candidates starting at (cx0, cy), all lanes in scan
+  // order: distances, ranks among the accepted, kept keys.  kFloor: keep only
+  // keys above the lane's last selected one (the refill walk).
+  auto step = [&](auto qb, auto floor_tag, bool vblk, int cy, int cx0) {
+    constexpr int QB = decltype(qb)::value;
+    constexpr bool kFloor = decltype(floor_tag)::value;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) valid |= (uint32_t)(vblk && cx0 + q < nx) << q;
+    if (cy == ref_cy && (unsigned)(ref_cx - cx0) < (unsigned)QB) valid &= ~(1u << (ref_cx - cx0));
+    float dist[QB];
+    block_dists<QB>(win, plane, pitch, ref, cy * pitch + col0 + cx0, valid, thr, dist);
+    int cnt = 0;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) cnt += dist[q] != __builtin_inff() ? 1 : 0;
+    // exclusive prefix of the lanes' counts (< 8) and their sum, bit by bit
+    int rank = total, wsum = 0;
+#pragma unroll
+    for (int bit = 0; bit < 3; ++bit) {
+      if (bit > 0 && QB < (1 << bit)) break;
+      const unsigned long long m = __ballot((cnt >> bit) & 1);
+      rank += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+      wsum += __popcll(m) << bit;
+    }
+    const uint32_t crow = pos0 + ((uint32_t)cy << 16) + (uint32_t)cx0;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+      const uint32_t x = __float_as_uint(dist[q]);
+      const bool acc = dist[q] != __builtin_inff();
+      const uint32_t c = crow + q;
+      bool keep = acc && rank < cap;
+      if (kFloor) keep = keep && (((long long)x << 32) | c) > fk;
+      const uint32_t key = keep ? x : kNoKey;
+      rank += acc ? 1 : 0;
+      // a key falls off (or is not kept) when the list is full
+      over |= keep && kd[kTopK - 1] != kNoKey;
+      if (__any(key < kd[kTopK - 1])) {
+        // sorted insert; an equal key stays behind the ones already kept
+        // (their positions are smaller)
+        bool lt[kTopK];
+#pragma unroll
+        for (int i = 0; i < kTopK; ++i) lt[i] = key < kd[i];
+#pragma unroll
+        for (int i = kTopK - 1; i >= 1; --i) {
+          kc[i] = lt[i - 1] ? kc[i - 1] : (lt[i] ? c : kc[i]);
+          kd[i] = lt[i - 1] ? kd[i - 1] : (lt[i] ? key : kd[i]);
+        }
+        kc[0] = lt[0] ? c : kc[0];
+        kd[0] = lt[0] ? key : kd[0];
+      }
+    }
+    total += wsum;
+  };
+
+  // The window in scan order: passes of Q-wide blocks (window rows split into
+  // nbx blocks; a lane's block moves on by 64 blocks a pass) while a whole
+  // pass cannot reach the cap, then passes of single candidates from the first
+  // position not walked, so that at most 63 candidates past the cap are
+  // evaluated.
+  auto walk = [&](auto floor_tag) {
 #pragma unroll
     for (int i = 0; i < kTopK; ++i) kd[i] = kc[i] = kNoKey;
     over = false;
     total = 0;
     if (!(thr >= 0.0f)) return;  // every first row already exceeds a negative (or NaN) threshold
-    for (int base = 0; base < nblk && total < cap; base += 64) {
-      const int blk = base + lane;
-      const bool vblk = blk < nblk;
-      const int cy = vblk ? blk / nbx : 0;
-      const int cx0 = vblk ? (blk - cy * nbx) * Q : 0;
-      bool live[Q];
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int cx = cx0 + q;
-        live[q] = vblk && cx < nx && !(cx == ref_cx && cy == ref_cy);
+    const int drow = 64 / nbx, dbx = 64 - drow * nbx;
+    int cy = lane / nbx, bx = lane - cy * nbx;  // this lane's block in the pass
+    int base = 0;
+    for (; base < nblk && total + 64 * Q <= cap; base += 64) {
+      step(std::integral_constant<int, Q>{}, floor_tag, cy < ny, cy < ny ? cy : 0, cy < ny ? bx * Q : 0);
+      cy += drow;
+      bx += dbx;
+      if (bx >= nbx) {
+        bx -= nbx;
+        ++cy;
       }
-      float dist[Q];
-      block_dists<Q>(win, plane, pitch, ref, cy * pitch + col0 + cx0, live, thr, dist);
-      int cnt = 0;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) cnt += dist[q] != __builtin_inff() ? 1 : 0;
-      // exclusive prefix of the lanes' counts (< 8) and their sum, bit by bit
-      int rank = total, wsum = 0;
-#pragma unroll
-      for (int bit = 0; bit < 3; ++bit) {
-        const unsigned long long m = __ballot((cnt >> bit) & 1);
-        rank += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
-        wsum += __popcll(m) << bit;
-      }
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const uint32_t x = __float_as_uint(dist[q]);
-        const bool acc = dist[q] != __builtin_inff();
-        const uint32_t c = pos0 + ((uint32_t)cy << 16) + (uint32_t)(cx0 + q);
-        const bool above = !fset || x > fd || (x == fd && c > fc);
-        const uint32_t key = (acc && rank < cap && above) ? x : kNoKey;
-        rank += acc ? 1 : 0;
-        // a key falls off (or is not kept) when the list is full
-        over |= key != kNoKey && kd[kTopK - 1] != kNoKey;
-        if (__any(key < kd[kTopK - 1])) {
-          // sorted insert; an equal key stays behind the ones already kept
-          // (their positions are smaller)
-          bool lt[kTopK];
-#pragma unroll
-          for (int i = 0; i < kTopK; ++i) lt[i] = key < kd[i];
-#pragma unroll
-          for (int i = kTopK - 1; i >= 1; --i) {
-            kc[i] = lt[i - 1] ? kc[i - 1] : (lt[i] ? c : kc[i]);
-            kd[i] = lt[i - 1] ? kd[i - 1] : (lt[i] ? key : kd[i]);
-          }
-          kc[0] = lt[0] ? c : kc[0];
-          kd[0] = lt[0] ? key : kd[0];
-        }
-      }
-      total += wsum;
+    }
+    if (base >= nblk) return;
+    const int brow = base / nbx;
+    const int ncand = nx * ny;
+    for (int p = brow * nx + (base - brow * nbx) * Q; p < ncand && total < cap; p += 64) {
+      const int c = p + lane;
+      const bool v = c < ncand;
+      const int y = v ? c / nx : 0;
+      step(std::integral_constant<int, 1>{}, floor_tag, v, y, v ? c - y * nx : 0);
     }
   };
 
-  walk();
+  walk(std::false_type{});
   const int n_list = 1 + min(total, cap);
   const int limit = min(a.max_group, n_list);
   int g = 1;
@@ -294,7 +317,7 @@ __device__ __forceinline__ void match_wave(const Bm3dMatchArgs& a, const float* 
   const int need = g - 1;
   uint32_t* out = a.groups + (size_t)ref_id * 17;
   for (int r = 0; r < need; ++r) {
-    if (__any(kd[0] == kNoKey && over)) walk();  // rare: refill above each lane's last selected key
+    if (__any(kd[0] == kNoKey && over)) walk(std::true_type{});  // rare: refill above each lane's last selected key
     const uint32_t wb = wave_min_u32(kd[0]);
     const unsigned long long m = __ballot(kd[0] == wb);
     // equal distances: the smallest position (scan index) wins
@@ -302,9 +325,7 @@ __device__ __forceinline__ void match_wave(const Bm3dMatchArgs& a, const float* 
                                          : wave_min_u32(kd[0] == wb ? kc[0] : kNoKey);
     if (kd[0] == wb && kc[0] == wc) {
       out[2 + r] = wc;
-      fd = wb;
-      fc = wc;
-      fset = true;
+      fk = ((long long)wb << 32) | wc;
 #pragma unroll
       for (int i = 0; i < kTopK - 1; ++i) {
         kd[i] = kd[i + 1];
@@ -336,17 +357,19 @@ __device__ __forceinline__ void match_wave_q(const Bm3dMatchArgs& a, const float
 
 // Stage rows sy0 .. sy0 + rows - 1 and columns sx0 .. sx0 + cols - 1 (cols <= 64)
 // of the three guide planes at row pitch a.match_pitch, one lane per column,
-// rows spread over the workgroup's nw waves; four rows of the three planes are
-// loaded before they are stored.
+// rows spread over the workgroup's nw waves; twelve rows of the three planes
+// are loaded before they are stored (one batch covers a 4-wave workgroup's
+// window).
 __device__ __forceinline__ void stage_window(const Bm3dMatchArgs& a, float* win, int sx0, int sy0, int cols, int rows,
                                              int wave, int nw) {
+  constexpr int kRows = 12;
   const int lane = threadIdx.x & 63;
   const int pitch = a.match_pitch, plane = a.match_plane;
   const bool col_ok = lane < cols;
-  for (int y = wave; y < rows; y += 4 * nw) {
-    float v[4][3];
+  for (int y = wave; y < rows; y += kRows * nw) {
+    float v[kRows][3];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kRows; ++u) {
       const int yy = y + u * nw;
       const bool ok = col_ok && yy < rows;
       const size_t g = (size_t)(sy0 + (ok ? yy : 0)) * a.width + sx0 + (ok ? lane : 0);
@@ -354,7 +377,7 @@ __device__ __forceinline__ void stage_window(const Bm3dMatchArgs& a, float* win,
       for (int ch = 0; ch < 3; ++ch) v[u][ch] = ok ? a.guide[ch][g] : 0.0f;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kRows; ++u) {
       const int yy = y + u * nw;
       if (col_ok && yy < rows) {
 #pragma unroll
