@@ -43,7 +43,7 @@ CROP_W, CROP_H = 768, 512
 # HBM bytes per bm3d_match_kernel launch at this workload, from the FETCH_SIZE / WRITE_SIZE
 # counter runs summarised in profiles/full_chain_6016x4016_kernel_stats.md (FETCH scaled by
 # 2.000 and WRITE by 1.000, the factors a 1 GiB copy kernel reads in the same runs).
-MATCH_HBM_BYTES = (2 * 1135072.7 + 219455.1) * 1024
+MATCH_HBM_BYTES = (2 * 1137609.1 + 196130.8) * 1024
 
 STAGES_TMPL = """
 [[pipeline_modules]]
