@@ -149,6 +149,9 @@ void stage_chroma_bm3d(imops_ctx* ctx, imops_image* img, float intensity) {
 
 void stage_bm3d(imops_ctx* ctx, imops_image* img, const Bm3dStageParams& p) {
   require_rgb(img, "BM3D");
+  // group tables pack patch positions as x | y << 16
+  if (img->width > 65535 || img->height > 65535)
+    throw Error{IMOPS_E_INVALID, "BM3D: width and height must be at most 65535"};
   TimedRegion stage_timer(ctx, "bm3d");
   const int w = (int)img->width, h = (int)img->height;
   const size_t n = (size_t)w * h;

@@ -79,3 +79,16 @@ def test_failed_film_preparation_is_not_cached():
     finally:
         _lib.check(_lib.lib().imops_debug_fail_allocations(0))
         ctx.close()
+
+
+def test_bm3d_rejects_frames_wider_than_its_position_format(gpu_ctx):
+    """Group tables pack patch positions as x | y << 16: a 65536-pixel-wide frame
+    is refused before any kernel runs, and the context stays usable."""
+    print("runtime: bm3d width limit", flush=True)
+    img = im.Image(gpu_ctx).set_rgb(np.zeros((9, 65536, 3), np.float32), im.CS_NONE)
+    with pytest.raises(im.ImopsError) as err:
+        img.bm3d(0.1)
+    assert err.value.code == _lib.E_INVALID and "65535" in err.value.message
+    ok = im.Image(gpu_ctx).set_rgb(im.synth_scene_rgb(64, 48, 3), im.CS_NONE)
+    ok.bm3d(0.1)
+    assert np.isfinite(ok.rgb()).all()
