@@ -208,7 +208,7 @@ __device__ __forceinline__ RefGeom ref_geom(const Bm3dMatchArgs& a, int ref_id) 
 // deterministic, so the ranks and keys come out the same).
 template <int Q>
 __device__ __forceinline__ void match_wave(const Bm3dMatchArgs& a, const float* win, const float* ref, int col0,
-                                        const RefGeom& G, int ref_id) {
+                                           const RefGeom& G, int ref_id) {
   const int lane = threadIdx.x & 63;
   const int nx = G.nx, ny = G.ny, pitch = a.match_pitch, plane = a.match_plane;
   const int ref_cx = G.rx - G.sx0, ref_cy = G.ry - G.sy0;
