@@ -117,3 +117,15 @@ def test_bm3d_through_stage_list(gpu_ctx):
     im.run_pixel_pipeline(img, cfg)
     want, _ = oracle.bm3d(rgb, oracle.bm3d_params(0.15))
     _check(img.rgb(), want)
+
+
+@pytest.mark.parametrize("stride,window", [(6, 41), (10, 41), (17, 41), (1, 41), (2, 1), (4, 7)])
+def test_bm3d_staging_layouts(gpu_ctx, stride, window):
+    """Strides and windows that give every staging layout of the matcher: four,
+    three, two and one reference patches per workgroup sharing a staged window
+    ((R - 1) * stride + window + 7 <= 64 columns), a 1x1 window and a 7x7 one."""
+    print(f"bm3d staging stride {stride} window {window}", flush=True)
+    rgb = _noisy_scene(90, 70, seed=stride * 100 + window)
+    p = oracle.bm3d_params(0.2, stride=stride, window=window)
+    want, _ = oracle.bm3d(rgb, p)
+    _check(_gpu(gpu_ctx, rgb, 0.2, stride=stride, search_window=window), want)
