@@ -129,3 +129,22 @@ def test_bm3d_staging_layouts(gpu_ctx, stride, window):
     p = oracle.bm3d_params(0.2, stride=stride, window=window)
     want, _ = oracle.bm3d(rgb, p)
     _check(_gpu(gpu_ctx, rgb, 0.2, stride=stride, search_window=window), want)
+
+
+@pytest.mark.parametrize("window", [39, 19])
+def test_bm3d_nan_and_inf_pixels(gpu_ctx, window):
+    """NaN and +-Inf pixels: the reference's row sums stop at the first row end
+    above the threshold (an Inf row stops it at once) while the GPU sums every
+    row (an Inf minus an Inf later in the patch makes its full sum NaN); both
+    reject such candidates, so the groups, and the fixed-point aggregation that
+    turns NaN contributions into zero, agree bit for bit."""
+    rgb = _noisy_scene(96, 80, seed=11)
+    rng = np.random.default_rng(12)
+    for v in (np.nan, np.inf, -np.inf):
+        ys, xs = rng.integers(0, 80, 6), rng.integers(0, 96, 6)
+        rgb[ys, xs, rng.integers(0, 3, 6)] = v
+    p = oracle.bm3d_params(0.1, stride=3, window=window)
+    want, _ = oracle.bm3d(rgb, p)
+    got = _gpu(gpu_ctx, rgb, 0.1, stride=3, search_window=window)
+    bad = np.count_nonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad == 0, f"{bad} mismatching values"
