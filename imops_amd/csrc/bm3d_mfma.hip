@@ -251,7 +251,11 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
     shift[2] = wave_sum(g2) * (1.0f / 64.0f);
   }
   // the FP16 copy c0, straight from the guide planes: the workgroup's threads
-  // take consecutive elements of the 3 x SY x SX region, 16 loads in flight each
+  // take consecutive elements of the 3 x SY x SX region, 16 loads in flight each.
+  // A shifted value that is not finite or does not fit FP16 (|v| > 65504) would
+  // make distances NaN, which the sweeps read as "outside the window": such a
+  // block is matched by the plain matcher instead.
+  bool unfit = false;
   {
     const int tot_el = 3 * SY * SX;
     const float* const g0 = a.guide[0];
@@ -281,10 +285,20 @@ __global__ void __launch_bounds__(XT) bm3d_match_mfma_kernel(Bm3dMfmaArgs a) {
       }
 #pragma unroll
       for (int u = 0; u < 16; ++u)
-        if (dst[u] >= 0) c0[dst[u]] = (_Float16)(v[u] - (chs[u] == 0 ? shift[0] : (chs[u] == 1 ? shift[1] : shift[2])));
+        if (dst[u] >= 0) {
+          const float hv = v[u] - (chs[u] == 0 ? shift[0] : (chs[u] == 1 ? shift[1] : shift[2]));
+          unfit |= !(fabsf(hv) <= 65504.0f);
+          c0[dst[u]] = (_Float16)hv;
+        }
     }
   }
-  __syncthreads();
+  if (__syncthreads_or(unfit)) {  // workgroup-uniform: every reference of the block to the plain matcher
+    if (tid < 64 && (tid & 7) < nvx && (tid >> 3) < nvy) {
+      const uint32_t at = atomicAdd(a.overflow_count, 1u);
+      a.overflow_list[at] = (uint32_t)((r0y + (tid >> 3)) * a.refs_x + r0x + (tid & 7));
+    }
+    return;
+  }
   // norms: per pixel row, the sum over channels, then over 8 columns, of the squared
   // FP16 values (read back from the copy) into the row-sum scratch hs (c1's area);
   // a lane takes four adjacent columns from twelve loaded values per channel

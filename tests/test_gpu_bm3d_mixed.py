@@ -127,3 +127,24 @@ def test_bm3d_mixed_through_stage_list(gpu_ctx, precision):
     approx, _ = gpu_ctx.timing("bm3d_approx_refs")
     gpu_ctx.set_timing(False)
     _check(img.rgb(), int(approx), rgb, oracle.bm3d_params(0.15), exact=precision == "mixed-match")
+
+
+def test_bm3d_mixed_match_non_finite_pixels(gpu_ctx):
+    """NaN and +-Inf pixels would turn FP16 distances into NaN, which the
+    matrix-core sweeps read as "outside the window" (such a pair would count as
+    accepted); the blocks whose staged region holds a value that is not finite
+    or past FP16's range go to the plain matcher, so "mixed-match" keeps the FP32
+    form's bits.  (Finite values that fit FP16 but dwarf the rest, such as RGB
+    300, widen the block's error bounds until its selections are approximate,
+    which the library reports; that case is not exact by design.)"""
+    rgb = _noisy_scene(120, 96, seed=21)
+    rng = np.random.default_rng(22)
+    for v in (np.nan, np.inf, -np.inf):
+        ys, xs = rng.integers(0, 96, 4), rng.integers(0, 120, 4)
+        rgb[ys, xs, rng.integers(0, 3, 4)] = v
+    p = oracle.bm3d_params(0.1, stride=3, window=39)
+    got, approx = _mixed(gpu_ctx, rgb, 0.1, stride=3, search_window=39)
+    want, _ = oracle.bm3d(rgb, p)
+    assert approx == 0
+    bad = np.count_nonzero(got.view(np.uint32) != want.view(np.uint32))
+    assert bad == 0, f"{bad} mismatching values"
